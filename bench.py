@@ -1,0 +1,179 @@
+#!/usr/bin/env python3
+"""GI frames/s of the radiance-cascade pass on one MI355X.
+
+Workload (default): 1920x1080 G-buffer rasterised on the GPU from the seeded
+box scene (seed 0, 32 boxes, 4 emissive), 6 cascade levels, fp32 storage; a
+step is one full GI frame (occupancy + jump flood + SDF + 6 levels of
+march/merge + gather) with the G-buffer already resident in HBM.
+
+Prints ONE JSON line.  `roofline` is the dominant kernel's algorithmic bytes
+per launch over its mean launch time, measured here with HIP events recorded
+on the stream the kernels run on; `cpu_baseline` times the FP32 CPU
+restatement of the same maths (the oracle's FP32 flavour, OpenMP) on the same
+G-buffer on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+
+
+def texel_bytes_contract(storage: str) -> int:
+    return 16 if storage == "f32" else 8
+
+
+def kernel_alg_bytes(W, H, L, storage, rays):
+    """Algorithmic bytes per launch of each kernel, from the compulsory-traffic
+    model B_alg = W H (28 + 8 + 12) + 2 b sum_{i>=1} R_i split by the kernel
+    that moves each term: seed kernel reads the 28 B/px G-buffer; the JFA
+    tail writes the 4 B/px SDF; level i >= 1 writes merged level i (b R_i)
+    and reads merged level i+1 (b R_{i+1}); level 0 reads the SDF (4 B/px),
+    merged level 1 (b R_1) and writes 12 B/px irradiance."""
+    b = texel_bytes_contract(storage)
+    n = W * H
+    out = {"seed": 28 * n, "jfa": 0, "jfa_tail": 4 * n}
+    for i in range(1, L):
+        out[f"level{i}"] = b * rays[i] + (b * rays[i + 1] if i + 1 < L else 0)
+    out["level0"] = 4 * n + 12 * n + (b * rays[1] if L > 1 else 0)
+    return out
+
+
+def b_alg(W, H, L, storage, rays):
+    return W * H * 48 + 2 * texel_bytes_contract(storage) * sum(rays[1:L])
+
+
+def cpu_baseline(depth, emissive, L, seconds):
+    """FP32 CPU restatement (OpenMP) on this host's cores: frames/s."""
+    from oracle import rco
+    threads = max(1, min(16, os.cpu_count() or 1))
+    rco.set_threads(threads)
+    rco.run(depth, emissive, L, np.float32)  # warm-up (page-in, thread pool)
+    frames, t0 = 0, time.perf_counter()
+    while True:
+        rco.run(depth, emissive, L, np.float32)
+        frames += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    return {"value": frames / el, "unit": "GI frames/s", "cores": threads, "kind": "port",
+            "sample": f"{frames} full frames of the same workload in {el:.1f} s "
+                      f"(oracle FP32 flavour, OpenMP, {threads} threads)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--levels", type=int, default=6)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--boxes", type=int, default=32)
+    ap.add_argument("--emissive", type=int, default=4)
+    ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"])
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py runs on exactly one GPU: the GI pass is single-GPU by design "
+              "(multi-GPU sharding is described in the design notes, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import radiancecascade_amd as rc
+
+    W, H, L = args.width, args.height, args.levels
+    dev = torch.device("cuda:0")
+    boxes = rc.scene_generate(args.seed, args.boxes, args.emissive)
+    depth = torch.empty((H, W), dtype=torch.float32, device=dev)
+    albedo = torch.empty((3, H, W), dtype=torch.float32, device=dev)
+    emis = torch.empty((3, H, W), dtype=torch.float32, device=dev)
+    irr = torch.empty((3, H, W), dtype=torch.float32, device=dev)
+    rc.gbuffer_raster(boxes, W, H, depth, albedo, emis)
+    stream = torch.cuda.current_stream()
+
+    stage = rc.RadianceCascadeStage(W, H, levels=L, storage=args.storage)
+    for _ in range(args.warmup):
+        stage.run_device(depth, emis, irr, stream)
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(args.steps):
+        stage.run_device(depth, emis, irr, stream)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms_total = ev0.elapsed_time(ev1)
+    ms_per_step = ms_total / args.steps
+    fps = 1000.0 / ms_per_step
+
+    # Per-kernel launch times: a second context with HIP events around every
+    # stage on the same stream (synchronises once per frame, so it is not the
+    # throughput figure above).
+    prof = rc.RadianceCascadeStage(W, H, levels=L, storage=args.storage, profile=True)
+    for _ in range(5):
+        prof.run_device(depth, emis, irr, stream)
+    prof.reset_stats()
+    n_prof = max(20, min(args.steps, 100))
+    for _ in range(n_prof):
+        prof.run_device(depth, emis, irr, stream)
+    torch.cuda.synchronize()
+    st = prof.stats()
+    rays = [rc.level_rays(W, H, i) for i in range(L)]
+    alg = kernel_alg_bytes(W, H, L, args.storage, rays)
+    per_launch_ms = {k: st.stage_ms[k] / st.stage_launches[k] for k in st.stage_ms}
+    per_frame_ms = {k: st.stage_ms[k] / st.runs for k in st.stage_ms}
+    dominant = max(per_frame_ms, key=per_frame_ms.get)
+    dom_ms = per_launch_ms[dominant]
+    achieved = alg[dominant] / (dom_ms * 1e-3) / 1e9
+    frame_alg = b_alg(W, H, L, args.storage, rays)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(depth.cpu().numpy(), emis.cpu().numpy(), L, args.cpu_seconds)
+
+    line = {
+        "metric": "GI frames/s @1080p 6 cascades: surrogate vs plain HIP at <=2% rel-L2 error",
+        "value": fps,
+        "unit": "GI frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32" if args.storage == "f32" else f"fp32 ({args.storage} storage)",
+        "data": "synthetic",
+        "config": {"workload": f"plain HIP path, {W}x{H} G-buffer, {L} cascades, {args.storage} storage, "
+                               f"seeded box scene (seed {args.seed}, {args.boxes} boxes, {args.emissive} emissive)",
+                   "path": "plain", "width": W, "height": H, "levels": L, "storage": args.storage,
+                   "rays_per_frame": int(sum(rays))},
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "alg_bytes_per_launch": alg[dominant], "mean_launch_ms": dom_ms},
+        "frame_roofline": {"alg_bytes": frame_alg, "achieved": frame_alg / (ms_per_step * 1e-3) / 1e9,
+                           "frac": frame_alg / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "stage_ms_per_frame": {k: round(v, 5) for k, v in per_frame_ms.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    stage.close()
+    prof.close()
+
+
+if __name__ == "__main__":
+    main()

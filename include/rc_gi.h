@@ -1,0 +1,161 @@
+/*
+ * rc_gi.h -- C ABI of the MI355X radiance-cascade GI pass (librcgi.so).
+ *
+ * The reference renderer (jw910731/RadianceCascade) exposes one pass
+ * interface, the Rust trait
+ *
+ *     pub trait RenderStage<T> {                                src/app.rs:3-7
+ *         fn render(&self, state: &mut T, view: &TextureView, encoder: &mut CommandEncoder);
+ *         fn resize(&mut self, device: &Device, config: &SurfaceConfiguration);
+ *         fn update(&mut self, state: &T, queue: &Queue);
+ *     }
+ *
+ * implemented by DefaultRenderer (src/renderer.rs:558-632) and driven once per
+ * frame by AppInternal::handle_redraw (src/window/app.rs:221-267) in the order
+ * update -> render -> submit.  The entry points below are what a Rust FFI
+ * crate implementing RenderStage<AppState> for a GI stage binds:
+ *
+ *   rc_create      <- DefaultRenderer::new             src/renderer.rs:168-555
+ *   rc_resize      <- RenderStage::resize              src/app.rs:5, src/renderer.rs:615-618
+ *   rc_update_host <- RenderStage::update (uploads)    src/app.rs:6, src/renderer.rs:620-631
+ *   rc_render_host <- RenderStage::render              src/app.rs:4, src/renderer.rs:559-613
+ *   rc_run_device  <- RenderStage::render, device-resident G-buffer, stream ordered
+ *   rc_destroy     <- Drop of the stage (wgpu resources, src/renderer.rs:156-165)
+ *
+ * The reference has no GI pass at all (its only pass is the Blinn-Phong
+ * forward raster); the maths is the published radiance-cascade method.
+ *
+ * Conventions: plain-old-data structs carry a version field; every call
+ * returns an rc_status and never throws; the context owns all device memory it
+ * allocates, the caller owns the buffers it passes; one context is used by one
+ * thread at a time and its device calls are ordered on the stream given (calls
+ * on one context must not overlap on different streams).  Device pointers are
+ * HIP device pointers; `stream` is a hipStream_t passed as void* (NULL = the
+ * null stream).
+ */
+#ifndef RC_GI_H
+#define RC_GI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RC_ABI_VERSION 1u
+#define RC_MAX_LEVELS 8
+#define RC_MAX_DIM 16384
+#define RC_MAX_BOXES 256
+
+typedef enum {
+    RC_OK = 0,
+    RC_ERR_INVALID = 1,     /* bad dimensions, level count, version, storage */
+    RC_ERR_NULL = 2,        /* a required pointer was NULL */
+    RC_ERR_NO_DEVICE = 3,   /* no HIP device visible */
+    RC_ERR_HIP = 4,         /* a HIP runtime call failed (see rc_last_error) */
+    RC_ERR_OOM = 5,         /* device allocation failed */
+    RC_ERR_STATE = 6        /* call out of order (e.g. render before update) */
+} rc_status;
+
+/* Storage precision of the materialised merged cascade levels.  Marching,
+ * merging and the gather always accumulate in fp32. */
+typedef enum { RC_STORAGE_F32 = 0, RC_STORAGE_F16 = 1, RC_STORAGE_BF16 = 2 } rc_storage;
+
+/* rc_config.flags */
+#define RC_FLAG_PROFILE    0x1u  /* time every stage with HIP events (rc_get_stats) */
+#define RC_FLAG_KEEP_DEBUG 0x2u  /* also write the jump-flood field and level 0 for rc_debug_* */
+
+typedef struct {
+    uint32_t version;   /* RC_ABI_VERSION */
+    uint32_t width;     /* 1 .. RC_MAX_DIM */
+    uint32_t height;    /* 1 .. RC_MAX_DIM */
+    uint32_t levels;    /* 1 .. RC_MAX_LEVELS; 0 = enough to cover the diagonal */
+    uint32_t storage;   /* rc_storage */
+    uint32_t flags;     /* RC_FLAG_* */
+    int32_t device;     /* HIP device ordinal */
+    uint32_t reserved;
+} rc_config;
+
+/* Planar fp32 G-buffer: depth [H][W] (linear view depth), emissive [3][H][W]
+ * (radiance), albedo [3][H][W].  Albedo is not read by the GI pass (GI is
+ * emission transport; compositing multiplies by albedo outside the pass) and
+ * may be NULL. */
+typedef struct {
+    const float *depth;
+    const float *albedo;
+    const float *emissive;
+} rc_gbuffer;
+
+/* Per-stage device times in milliseconds, summed over the runs since the last
+ * rc_reset_stats (only with RC_FLAG_PROFILE), and the run count. */
+#define RC_STAGE_SEED 0
+#define RC_STAGE_JFA 1          /* all global jump-flood passes */
+#define RC_STAGE_JFA_TAIL 2     /* fused short-step passes + distance field */
+#define RC_STAGE_LEVEL0 3       /* RC_STAGE_LEVEL0 + i = level i march/merge; level 0 includes the gather */
+#define RC_NUM_STAGES (RC_STAGE_LEVEL0 + RC_MAX_LEVELS)
+typedef struct {
+    uint32_t version;
+    uint32_t runs;
+    double stage_ms[RC_NUM_STAGES];
+    uint32_t stage_launches[RC_NUM_STAGES];
+    uint64_t level_rays[RC_MAX_LEVELS];
+    uint64_t level_bytes_stored[RC_MAX_LEVELS]; /* bytes of merged level i written per run */
+} rc_stats;
+
+typedef struct rc_ctx rc_ctx;
+
+/* ---- the pass (RenderStage mirror) -------------------------------------- */
+rc_status rc_create(const rc_config *cfg, rc_ctx **out);
+rc_status rc_resize(rc_ctx *ctx, uint32_t width, uint32_t height, uint32_t levels);
+rc_status rc_update_host(rc_ctx *ctx, const rc_gbuffer *host);
+rc_status rc_render_host(rc_ctx *ctx, float *irradiance_host);
+rc_status rc_run_device(rc_ctx *ctx, const rc_gbuffer *dev, float *irradiance_dev, void *stream);
+void rc_destroy(rc_ctx *ctx);
+
+/* ---- introspection -------------------------------------------------------- */
+rc_status rc_get_config(const rc_ctx *ctx, rc_config *out);
+rc_status rc_get_stats(const rc_ctx *ctx, rc_stats *out);
+rc_status rc_reset_stats(rc_ctx *ctx);
+const char *rc_last_error(const rc_ctx *ctx);   /* ctx may be NULL: last global error */
+const char *rc_status_string(rc_status s);
+uint32_t rc_abi_version(void);
+rc_status rc_device_count(int *count);
+
+/* Parity readbacks of the last run (device destination, stream ordered).
+ * rc_debug_seeds / rc_debug_level(.., 0, ..) need RC_FLAG_KEEP_DEBUG.
+ *   seeds : uint32 [H][W]  final jump-flood field, (y<<16)|x or 0xFFFFFFFF
+ *   sdf   : float  [H][W]  distance field in px (+inf with no seed)
+ *   level : float  [Py_i][Px_i][max(N_i/4,1)][4], merged level i averaged over
+ *           the 4 children of each level-(i-1) direction (fp32 whatever the
+ *           storage; level 0 is the per-probe fluence) */
+rc_status rc_debug_seeds(rc_ctx *ctx, uint32_t *dev_out, void *stream);
+rc_status rc_debug_sdf(rc_ctx *ctx, float *dev_out, void *stream);
+rc_status rc_debug_level(rc_ctx *ctx, uint32_t level, float *dev_out, void *stream);
+/* March-only probe of one level of the last run's distance field: per ray in
+ * canonical order ((py*Px+px)*N+k) the hit pixel y*W+x, -1 (left the image)
+ * or -2 (reached the interval end); and the ray's canonical index. */
+rc_status rc_debug_march(rc_ctx *ctx, uint32_t level, int32_t *hit_dev, int64_t *index_dev, void *stream);
+
+/* ---- cascade geometry (integer, identical to the kernels' indexing) ------- */
+rc_status rc_probe_grid(uint32_t width, uint32_t height, uint32_t level, uint32_t *px, uint32_t *py);
+uint64_t rc_level_rays(uint32_t width, uint32_t height, uint32_t level);
+uint32_t rc_levels_for_coverage(uint32_t width, uint32_t height);
+
+/* ---- seeded synthetic scenes (the benchmark's G-buffer producer) ---------- */
+typedef struct {
+    double lo[3], hi[3];
+    double albedo[3];
+    double emit[3];
+} rc_box;
+/* SplitMix64-seeded box scene in the unit-cube room (see the design notes'
+ * scene specification).  boxes must hold n_boxes entries. */
+rc_status rc_scene_generate(uint64_t seed, uint32_t n_boxes, uint32_t n_emissive, rc_box *boxes);
+/* Analytic FP64 raster of the scene to a planar fp32 G-buffer on the device. */
+rc_status rc_gbuffer_raster(const rc_box *boxes_host, uint32_t n_boxes, uint32_t width, uint32_t height,
+                            float *depth_dev, float *albedo_dev, float *emissive_dev, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,14 @@
+"""MI355X-native radiance-cascade GI pass (HIP kernels for gfx950 behind a C ABI).
+
+The pass replaces the reference renderer's ``RenderStage`` slot
+(src/app.rs:3-7) with jump flood -> per-level interval march and top-down
+merge -> level-0 gather to per-pixel irradiance.  The kernels live in
+``librcgi.so`` (built from ``csrc/`` by ``radiancecascade_amd.build``); there
+is no CPU fallback and importing the stage fails if the library is absent.
+"""
+from ._abi import RcError, load  # noqa: F401
+from .stage import (RadianceCascadeStage, device_count, gbuffer_raster, level_rays,  # noqa: F401
+                    levels_for_coverage, probe_grid, scene_generate)
+
+__all__ = ["RadianceCascadeStage", "RcError", "device_count", "gbuffer_raster", "level_rays",
+           "levels_for_coverage", "load", "probe_grid", "scene_generate"]

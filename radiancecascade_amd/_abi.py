@@ -1,0 +1,121 @@
+"""ctypes binding of include/rc_gi.h (librcgi.so).
+
+The library is the HIP build in this directory; there is no CPU fallback.
+Loading fails loudly when the shared object is missing.  When torch is
+importable it is imported first, so that the process holds one HIP runtime
+(torch's wheel ships its own libamdhip64.so.7, which then also serves
+librcgi.so) and torch tensors can be handed to the library as device memory.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librcgi.so")
+
+ABI_VERSION = 1
+MAX_LEVELS = 8
+MAX_DIM = 16384
+MAX_BOXES = 256
+
+OK, ERR_INVALID, ERR_NULL, ERR_NO_DEVICE, ERR_HIP, ERR_OOM, ERR_STATE = range(7)
+STATUS_NAMES = {0: "RC_OK", 1: "RC_ERR_INVALID", 2: "RC_ERR_NULL", 3: "RC_ERR_NO_DEVICE",
+                4: "RC_ERR_HIP", 5: "RC_ERR_OOM", 6: "RC_ERR_STATE"}
+
+STORAGE = {"f32": 0, "fp32": 0, "f16": 1, "fp16": 1, "bf16": 2}
+FLAG_PROFILE = 0x1
+FLAG_KEEP_DEBUG = 0x2
+
+STAGE_SEED, STAGE_JFA, STAGE_JFA_TAIL, STAGE_LEVEL0 = 0, 1, 2, 3
+NUM_STAGES = STAGE_LEVEL0 + MAX_LEVELS
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("version", ctypes.c_uint32), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("levels", ctypes.c_uint32), ("storage", ctypes.c_uint32), ("flags", ctypes.c_uint32),
+                ("device", ctypes.c_int32), ("reserved", ctypes.c_uint32)]
+
+
+class GBuffer(ctypes.Structure):
+    _fields_ = [("depth", ctypes.c_void_p), ("albedo", ctypes.c_void_p), ("emissive", ctypes.c_void_p)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("version", ctypes.c_uint32), ("runs", ctypes.c_uint32),
+                ("stage_ms", ctypes.c_double * NUM_STAGES),
+                ("stage_launches", ctypes.c_uint32 * NUM_STAGES),
+                ("level_rays", ctypes.c_uint64 * MAX_LEVELS),
+                ("level_bytes_stored", ctypes.c_uint64 * MAX_LEVELS)]
+
+
+class Box(ctypes.Structure):
+    _fields_ = [("lo", ctypes.c_double * 3), ("hi", ctypes.c_double * 3),
+                ("albedo", ctypes.c_double * 3), ("emit", ctypes.c_double * 3)]
+
+
+# name -> (restype, argtypes); every symbol include/rc_gi.h declares.
+_P = ctypes.c_void_p
+_u32, _i32, _u64 = ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint64
+SIGNATURES = {
+    "rc_create": (_i32, [ctypes.POINTER(Config), ctypes.POINTER(_P)]),
+    "rc_resize": (_i32, [_P, _u32, _u32, _u32]),
+    "rc_update_host": (_i32, [_P, ctypes.POINTER(GBuffer)]),
+    "rc_render_host": (_i32, [_P, _P]),
+    "rc_run_device": (_i32, [_P, ctypes.POINTER(GBuffer), _P, _P]),
+    "rc_destroy": (None, [_P]),
+    "rc_get_config": (_i32, [_P, ctypes.POINTER(Config)]),
+    "rc_get_stats": (_i32, [_P, ctypes.POINTER(Stats)]),
+    "rc_reset_stats": (_i32, [_P]),
+    "rc_last_error": (ctypes.c_char_p, [_P]),
+    "rc_status_string": (ctypes.c_char_p, [_i32]),
+    "rc_abi_version": (_u32, []),
+    "rc_device_count": (_i32, [ctypes.POINTER(ctypes.c_int)]),
+    "rc_debug_seeds": (_i32, [_P, _P, _P]),
+    "rc_debug_sdf": (_i32, [_P, _P, _P]),
+    "rc_debug_level": (_i32, [_P, _u32, _P, _P]),
+    "rc_debug_march": (_i32, [_P, _u32, _P, _P, _P]),
+    "rc_probe_grid": (_i32, [_u32, _u32, _u32, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
+    "rc_level_rays": (_u64, [_u32, _u32, _u32]),
+    "rc_levels_for_coverage": (_u32, [_u32, _u32]),
+    "rc_scene_generate": (_i32, [_u64, _u32, _u32, _P]),
+    "rc_gbuffer_raster": (_i32, [_P, _u32, _u32, _u32, _P, _P, _P, _P]),
+}
+
+_lib = None
+
+
+class RcError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        self.status = status
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+
+
+def load() -> ctypes.CDLL:
+    """Load librcgi.so once; raise ImportError if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m radiancecascade_amd.build` "
+                          "(the GI pass has no CPU fallback)")
+    try:
+        import torch  # noqa: F401  (one HIP runtime per process; see module docstring)
+    except ImportError:
+        pass
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.rc_abi_version() != ABI_VERSION:
+        raise ImportError("librcgi.so ABI version mismatch")
+    _lib = lib
+    return lib
+
+
+def check(status: int, ctx=None) -> None:
+    if status != OK:
+        lib = load()
+        msg = lib.rc_last_error(ctx)
+        raise RcError(status, msg.decode() if msg else "")

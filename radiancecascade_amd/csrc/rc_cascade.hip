@@ -1,0 +1,337 @@
+// rc_cascade.hip -- per-level interval march, top-down bilinear merge,
+// 4-child direction reduction, and the level-0 gather to irradiance.
+//
+// None of this exists in the reference (its only pass is the forward raster,
+// src/renderer.rs:559-613); the maths is Sannikov's radiance cascades (2023):
+//
+//   march  : from c = ((px+1/2)s, (py+1/2)s) along theta_k, t from t0: sample
+//            q = floor(c + t dir) (fused multiply-add per axis); off-image ->
+//            (0, alpha 1); SDF(q) < 1/2 -> (E(q), alpha 0); else
+//            t += max(SDF(q), 1); t >= t1 -> (0, alpha 1).
+//   merge  : R_i(p,k) = march + alpha * bilinear_b A_{i+1}(b, k) over the 4
+//            upper probes around p (upper coordinate p/2 - 1/4, indices
+//            clamped, weights 3/4 and 1/4: x first, then y).
+//   reduce : A_i(p,k') = 1/4 ((R(4k') + R(4k'+1)) + (R(4k'+2) + R(4k'+3))),
+//            the only form in which level i-1 reads level i.
+//   gather : irradiance(pixel) = bilinear over the level-0 fluence A_0.
+//
+// Work mapping: a workgroup (4 wave64) owns TX x TY probes x KD consecutive
+// directions, lanes direction-fastest.  The 4 children of a level-(i-1)
+// direction sit in one lane quad, so the reduction is two DPP quad
+// permutations and no LDS.  Before marching, the workgroup stages the upper
+// level's probe neighbourhood for its direction slice in LDS (one global read
+// per texel instead of 16 L2 reads per ray).  Rays that hit skip the merge.
+#include "rc_internal.hpp"
+
+namespace rc {
+
+__device__ inline int xcd_remap_c(int b, int nb)
+{
+    const int q = nb >> 3, r = nb & 7;
+    const int xcd = b & 7, idx = b >> 3;
+    return xcd * q + min(xcd, r) + idx;
+}
+
+// Sum over a lane quad in the order (v0+v1)+(v2+v3).  Every lane of the quad
+// ends with the same bits (fp addition is commutative).
+__device__ inline float quad_sum(float v)
+{
+    v = v + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));  // lane ^ 1
+    v = v + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));  // lane ^ 2
+    return v;
+}
+
+struct MarchResult {
+    float r, g, b;
+    bool miss;
+    int hit;  // pixel index, -1 off-image, -2 interval end
+};
+
+// LOAD_E = false: hit code only, the emissive planes are never touched.
+template <bool LOAD_E = true>
+__device__ inline MarchResult march_ray(const float *__restrict__ sdf, const float *__restrict__ emissive, int W,
+                                        int H, float cx, float cy, float dx, float dy, float t0, float t1)
+{
+    const float fW = (float)W, fH = (float)H;
+    MarchResult m{0.f, 0.f, 0.f, true, -2};
+    float t = t0;
+    while (t < t1) {
+        const float x = floorf(fmaf(t, dx, cx));
+        const float y = floorf(fmaf(t, dy, cy));
+        if (!(x >= 0.f && y >= 0.f && x < fW && y < fH)) {
+            m.hit = -1;
+            return m;
+        }
+        const int q = (int)y * W + (int)x;
+        const float d = sdf[q];
+        if (d < 0.5f) {
+            if (LOAD_E) {
+                const size_t plane = (size_t)W * H;
+                m.r = emissive[q];
+                m.g = emissive[plane + q];
+                m.b = emissive[2 * plane + q];
+            }
+            m.miss = false;
+            m.hit = q;
+            return m;
+        }
+        t = t + fmaxf(d, 1.0f);
+    }
+    return m;
+}
+
+// Bilinear of four staged texels with the lower probe's parity weights.
+__device__ inline float3 bilerp4(float4 u00, float4 u10, float4 u01, float4 u11, int x, int y)
+{
+    const float wx1 = (x & 1) ? 0.25f : 0.75f, wx0 = 1.0f - wx1;
+    const float wy1 = (y & 1) ? 0.25f : 0.75f, wy0 = 1.0f - wy1;
+    float3 r;
+    r.x = fmaf(wy1, fmaf(wx1, u11.x, wx0 * u01.x), wy0 * fmaf(wx1, u10.x, wx0 * u00.x));
+    r.y = fmaf(wy1, fmaf(wx1, u11.y, wx0 * u01.y), wy0 * fmaf(wx1, u10.y, wx0 * u00.y));
+    r.z = fmaf(wy1, fmaf(wx1, u11.z, wx0 * u01.z), wy0 * fmaf(wx1, u10.z, wx0 * u00.z));
+    return r;
+}
+
+template <int TX, int TY, int KD, Storage S, bool UPPER>
+__global__ __launch_bounds__(kBlock) void k_level(LevelArgs a)
+{
+    static_assert(TX * TY * KD == kBlock, "tile must hold 256 rays");
+    using TT = Texel<S>;
+    constexpr int UCX = TX / 2 + 2, UCY = TY / 2 + 2;
+    __shared__ float4 su[UPPER ? UCX * UCY * KD : 1];
+
+    const int tiles_x = (a.PX + TX - 1) / TX;
+    const int slices = a.N / KD;
+    const int lb = xcd_remap_c(blockIdx.x, gridDim.x);
+    const int slice = lb % slices, tile = lb / slices;
+    const int px0 = (tile % tiles_x) * TX, py0 = (tile / tiles_x) * TY, k0 = slice * KD;
+
+    const int kk = threadIdx.x % KD, pl = threadIdx.x / KD;
+    const int px = px0 + pl % TX, py = py0 + pl / TX, k = k0 + kk;
+    const bool active = px < a.PX && py < a.PY;
+
+    const int ulx = (px0 - 1) >> 1, uly = (py0 - 1) >> 1;
+    if (UPPER) {
+        const typename TT::T *up = (const typename TT::T *)a.upper;
+        for (int i = threadIdx.x; i < UCX * UCY * KD; i += kBlock) {
+            const int j = i % KD, sx = (i / KD) % UCX, sy = i / (KD * UCX);
+            const int ux = min(max(ulx + sx, 0), a.UX - 1), uy = min(max(uly + sy, 0), a.UY - 1);
+            su[i] = TT::unpack(up[((size_t)uy * a.UX + ux) * a.N + k0 + j]);
+        }
+        __syncthreads();
+    }
+
+    float r = 0.f, g = 0.f, b = 0.f, al = 0.f;
+    if (active) {
+        const float2 dir = a.dirs[k];
+        const float cx = ((float)px + 0.5f) * a.spacing, cy = ((float)py + 0.5f) * a.spacing;
+        MarchResult m = march_ray(a.sdf, a.emissive, a.W, a.H, cx, cy, dir.x, dir.y, a.t0, a.t1);
+        r = m.r;
+        g = m.g;
+        b = m.b;
+        al = m.miss ? 1.0f : 0.0f;
+        if (UPPER && m.miss) {
+            const int sx = ((px - 1) >> 1) - ulx, sy = ((py - 1) >> 1) - uly;
+            const float4 u00 = su[(sy * UCX + sx) * KD + kk];
+            const float4 u10 = su[(sy * UCX + sx + 1) * KD + kk];
+            const float4 u01 = su[((sy + 1) * UCX + sx) * KD + kk];
+            const float4 u11 = su[((sy + 1) * UCX + sx + 1) * KD + kk];
+            const float3 v = bilerp4(u00, u10, u01, u11, px, py);
+            r = v.x;
+            g = v.y;
+            b = v.z;
+        }
+    }
+    r = quad_sum(r) * 0.25f;
+    g = quad_sum(g) * 0.25f;
+    b = quad_sum(b) * 0.25f;
+    al = quad_sum(al) * 0.25f;
+    if (active && (kk & 3) == 0) {
+        typename TT::T *out = (typename TT::T *)a.out;
+        out[((size_t)py * a.PX + px) * (a.N >> 2) + (k >> 2)] = TT::pack(make_float4(r, g, b, al));
+    }
+}
+
+// Level 0 (4 directions, interval [0,2)) fused with the gather.  A tile owns
+// 16x16 level-0 probes = 32x32 pixels; it marches the tile's probes plus a
+// 1-probe halo (the pixels' bilinear footprint), reduces each probe's 4
+// directions to its fluence in LDS, and writes planar fp32 irradiance.
+template <Storage S, bool UPPER>
+__global__ __launch_bounds__(kBlock) void k_level0(LevelArgs a, float *__restrict__ irr, float *__restrict__ dbg)
+{
+    using TT = Texel<S>;
+    constexpr int TP = 16, NS = TP + 2, UC = TP / 2 + 2;
+    constexpr int RAYS = NS * NS * 4, ROUNDS = (RAYS + kBlock - 1) / kBlock;
+    __shared__ float4 su[UPPER ? UC * UC * 4 : 1];
+    __shared__ float4 fl[NS * NS];
+
+    const int tiles_x = (a.PX + TP - 1) / TP;
+    const int lb = xcd_remap_c(blockIdx.x, gridDim.x);
+    const int px0 = (lb % tiles_x) * TP, py0 = (lb / tiles_x) * TP;
+    const int ulx = (px0 - 2) >> 1, uly = (py0 - 2) >> 1;
+    if (UPPER) {
+        const typename TT::T *up = (const typename TT::T *)a.upper;
+        for (int i = threadIdx.x; i < UC * UC * 4; i += kBlock) {
+            const int j = i & 3, sx = (i >> 2) % UC, sy = (i >> 2) / UC;
+            const int ux = min(max(ulx + sx, 0), a.UX - 1), uy = min(max(uly + sy, 0), a.UY - 1);
+            su[i] = TT::unpack(up[((size_t)uy * a.UX + ux) * 4 + j]);
+        }
+        __syncthreads();
+    }
+
+#pragma unroll 1
+    for (int round = 0; round < ROUNDS; ++round) {
+        const int ray = round * kBlock + threadIdx.x;
+        const int slot = ray >> 2, k = ray & 3;
+        const int px = px0 - 1 + slot % NS, py = py0 - 1 + slot / NS;
+        const bool active = slot < NS * NS && px >= 0 && py >= 0 && px < a.PX && py < a.PY;
+        float r = 0.f, g = 0.f, b = 0.f, al = 0.f;
+        if (active) {
+            const float2 dir = a.dirs[k];
+            const float cx = ((float)px + 0.5f) * 2.0f, cy = ((float)py + 0.5f) * 2.0f;
+            MarchResult m = march_ray(a.sdf, a.emissive, a.W, a.H, cx, cy, dir.x, dir.y, 0.0f, 2.0f);
+            r = m.r;
+            g = m.g;
+            b = m.b;
+            al = m.miss ? 1.0f : 0.0f;
+            if (UPPER && m.miss) {
+                const int sx = ((px - 1) >> 1) - ulx, sy = ((py - 1) >> 1) - uly;
+                const float4 u00 = su[(sy * UC + sx) * 4 + k];
+                const float4 u10 = su[(sy * UC + sx + 1) * 4 + k];
+                const float4 u01 = su[((sy + 1) * UC + sx) * 4 + k];
+                const float4 u11 = su[((sy + 1) * UC + sx + 1) * 4 + k];
+                const float3 v = bilerp4(u00, u10, u01, u11, px, py);
+                r = v.x;
+                g = v.y;
+                b = v.z;
+            }
+        }
+        r = quad_sum(r) * 0.25f;
+        g = quad_sum(g) * 0.25f;
+        b = quad_sum(b) * 0.25f;
+        al = quad_sum(al) * 0.25f;
+        if (active && k == 0) {
+            fl[slot] = make_float4(r, g, b, al);
+            const int sx = slot % NS, sy = slot / NS;
+            if (dbg && sx >= 1 && sx <= TP && sy >= 1 && sy <= TP)
+                reinterpret_cast<float4 *>(dbg)[(size_t)py * a.PX + px] = make_float4(r, g, b, al);
+        }
+    }
+    __syncthreads();
+
+    const size_t plane = (size_t)a.W * a.H;
+    for (int i = threadIdx.x; i < 4 * TP * TP; i += kBlock) {
+        const int x = 2 * px0 + (i & (2 * TP - 1)), y = 2 * py0 + i / (2 * TP);
+        if (x >= a.W || y >= a.H) continue;
+        const int bx0 = (x - 1) >> 1, by0 = (y - 1) >> 1;
+        const int sx0 = min(max(bx0, 0), a.PX - 1) - (px0 - 1), sx1 = min(max(bx0 + 1, 0), a.PX - 1) - (px0 - 1);
+        const int sy0 = min(max(by0, 0), a.PY - 1) - (py0 - 1), sy1 = min(max(by0 + 1, 0), a.PY - 1) - (py0 - 1);
+        const float3 v = bilerp4(fl[sy0 * NS + sx0], fl[sy0 * NS + sx1], fl[sy1 * NS + sx0], fl[sy1 * NS + sx1], x, y);
+        const size_t p = (size_t)y * a.W + x;
+        irr[p] = v.x;
+        irr[plane + p] = v.y;
+        irr[2 * plane + p] = v.z;
+    }
+}
+
+// March-only probe for parity tests: hit code and canonical index per ray.
+__global__ __launch_bounds__(kBlock) void k_march_probe(LevelArgs a, int32_t *__restrict__ hit,
+                                                        int64_t *__restrict__ index)
+{
+    const int64_t ray = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t total = (int64_t)a.PX * a.PY * a.N;
+    if (ray >= total) return;
+    const int k = (int)(ray % a.N);
+    const int64_t p = ray / a.N;
+    const int px = (int)(p % a.PX), py = (int)(p / a.PX);
+    const float2 dir = a.dirs[k];
+    const float cx = ((float)px + 0.5f) * a.spacing, cy = ((float)py + 0.5f) * a.spacing;
+    const MarchResult m = march_ray<false>(a.sdf, nullptr, a.W, a.H, cx, cy, dir.x, dir.y, a.t0, a.t1);
+    hit[ray] = m.hit;
+    index[ray] = ((int64_t)py * a.PX + px) * a.N + k;
+}
+
+template <Storage S>
+__global__ __launch_bounds__(kBlock) void k_to_f32(const void *__restrict__ in, size_t texels, float4 *__restrict__ out)
+{
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < texels) out[i] = Texel<S>::unpack(((const typename Texel<S>::T *)in)[i]);
+}
+
+int tile_kd(int shape) { return shape == TILE_8x8x4 ? 4 : shape == TILE_4x4x16 ? 16 : shape == TILE_2x2x64 ? 64 : 256; }
+int tile_tx(int shape) { return shape == TILE_8x8x4 ? 8 : shape == TILE_4x4x16 ? 4 : shape == TILE_2x2x64 ? 2 : 1; }
+int tile_ty(int shape) { return tile_tx(shape); }
+
+template <int TX, int TY, int KD, Storage S>
+static void launch_level_shape(const LevelArgs &a, hipStream_t s)
+{
+    const int blocks = ((a.PX + TX - 1) / TX) * ((a.PY + TY - 1) / TY) * (a.N / KD);
+    if (a.upper) hipLaunchKernelGGL((k_level<TX, TY, KD, S, true>), dim3(blocks), dim3(kBlock), 0, s, a);
+    else hipLaunchKernelGGL((k_level<TX, TY, KD, S, false>), dim3(blocks), dim3(kBlock), 0, s, a);
+}
+
+template <Storage S>
+static hipError_t launch_level_st(const LevelArgs &a, int shape, hipStream_t s)
+{
+    if (tile_kd(shape) > a.N) return hipErrorInvalidValue;
+    switch (shape) {
+    case TILE_8x8x4: launch_level_shape<8, 8, 4, S>(a, s); break;
+    case TILE_4x4x16: launch_level_shape<4, 4, 16, S>(a, s); break;
+    case TILE_2x2x64: launch_level_shape<2, 2, 64, S>(a, s); break;
+    case TILE_1x1x256: launch_level_shape<1, 1, 256, S>(a, s); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_level(const LevelArgs &a, Storage st, int shape, hipStream_t s)
+{
+    switch (st) {
+    case Storage::F32: return launch_level_st<Storage::F32>(a, shape, s);
+    case Storage::F16: return launch_level_st<Storage::F16>(a, shape, s);
+    case Storage::BF16: return launch_level_st<Storage::BF16>(a, shape, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+template <Storage S>
+static hipError_t launch_level0_st(const LevelArgs &a, float *irr, float *dbg, hipStream_t s)
+{
+    const int blocks = ((a.PX + 15) / 16) * ((a.PY + 15) / 16);
+    if (a.upper) hipLaunchKernelGGL((k_level0<S, true>), dim3(blocks), dim3(kBlock), 0, s, a, irr, dbg);
+    else hipLaunchKernelGGL((k_level0<S, false>), dim3(blocks), dim3(kBlock), 0, s, a, irr, dbg);
+    return hipGetLastError();
+}
+
+hipError_t launch_level0_gather(const LevelArgs &a, Storage st, float *irr, float *dbg, hipStream_t s)
+{
+    switch (st) {
+    case Storage::F32: return launch_level0_st<Storage::F32>(a, irr, dbg, s);
+    case Storage::F16: return launch_level0_st<Storage::F16>(a, irr, dbg, s);
+    case Storage::BF16: return launch_level0_st<Storage::BF16>(a, irr, dbg, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_level_to_f32(const void *level, Storage st, size_t texels, float *out, hipStream_t s)
+{
+    const unsigned blocks = (unsigned)((texels + kBlock - 1) / kBlock);
+    if (blocks == 0) return hipSuccess;
+    switch (st) {
+    case Storage::F32: hipLaunchKernelGGL(k_to_f32<Storage::F32>, dim3(blocks), dim3(kBlock), 0, s, level, texels, (float4 *)out); break;
+    case Storage::F16: hipLaunchKernelGGL(k_to_f32<Storage::F16>, dim3(blocks), dim3(kBlock), 0, s, level, texels, (float4 *)out); break;
+    case Storage::BF16: hipLaunchKernelGGL(k_to_f32<Storage::BF16>, dim3(blocks), dim3(kBlock), 0, s, level, texels, (float4 *)out); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_march_probe(const LevelArgs &a, int32_t *hit, int64_t *index, hipStream_t s)
+{
+    const int64_t total = (int64_t)a.PX * a.PY * a.N;
+    const unsigned blocks = (unsigned)((total + kBlock - 1) / kBlock);
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_march_probe, dim3(blocks), dim3(kBlock), 0, s, a, hit, index);
+    return hipGetLastError();
+}
+
+}  // namespace rc

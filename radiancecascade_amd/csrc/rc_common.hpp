@@ -1,0 +1,93 @@
+// rc_common.hpp -- shared constants, cascade geometry and storage types for
+// the gfx950 radiance-cascade kernels.
+//
+// Geometry (all integer): level i has probe spacing s_i = 2*2^i px,
+// N_i = 4*4^i directions, a ceil-sized probe grid Px_i x Py_i and the ray
+// interval [2(4^i-1)/3, 2(4^(i+1)-1)/3) px.  Probe (px,py) is centred at
+// ((px+1/2) s_i, (py+1/2) s_i); direction k is theta = 2 pi (k+1/2)/N_i and its
+// children at level i+1 are 4k..4k+3.
+//
+// Materialised merged level i (i >= 1) is stored probe-major with the
+// direction innermost: A_i[py][px][k'][texel] for k' < N_{i-1} = N_i/4, where
+// A_i(p,k') = 1/4 sum_j R_i(p, 4k'+j) is the merged radiance already averaged
+// over the 4 children the level below reads.  A probe's texels are one
+// contiguous run, so a wave's reads and writes over consecutive directions
+// coalesce.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rc {
+
+constexpr uint32_t kSeedNone = 0xFFFFFFFFu;
+constexpr int kBlock = 256;  // threads per workgroup for every kernel (4 wave64)
+
+__host__ __device__ inline int probe_spacing(int level) { return 2 << level; }
+__host__ __device__ inline int num_dirs(int level) { return 4 << (2 * level); }
+__host__ __device__ inline int grid_dim(int pixels, int level)
+{
+    int s = probe_spacing(level);
+    return (pixels + s - 1) / s;
+}
+__host__ __device__ inline int64_t interval_t0(int level) { return (2 * ((1ll << (2 * level)) - 1)) / 3; }
+__host__ __device__ inline int64_t interval_t1(int level) { return (2 * ((1ll << (2 * level + 2)) - 1)) / 3; }
+
+// ---- storage texels ---------------------------------------------------------
+// fp32: float4.  fp16 / bf16: four 16-bit lanes packed in a uint2 (8 bytes).
+enum class Storage : int { F32 = 0, F16 = 1, BF16 = 2 };
+
+template <Storage S> struct Texel;
+
+template <> struct Texel<Storage::F32> {
+    using T = float4;
+    __device__ static inline T pack(float4 v) { return v; }
+    __device__ static inline float4 unpack(T v) { return v; }
+};
+
+__device__ inline uint32_t f32_to_bf16_rne(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (u >> 16) | 0x40u;  // quiet NaN
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return u >> 16;
+}
+__device__ inline float bf16_to_f32(uint32_t h) { return __uint_as_float(h << 16); }
+
+__device__ inline uint32_t f32_to_f16_rne(float f)
+{
+    _Float16 h = (_Float16)f;  // round to nearest even
+    return (uint32_t)__builtin_bit_cast(uint16_t, h);
+}
+__device__ inline float f16_to_f32(uint32_t h) { return (float)__builtin_bit_cast(_Float16, (uint16_t)(h & 0xFFFFu)); }
+
+template <> struct Texel<Storage::F16> {
+    using T = uint2;
+    __device__ static inline T pack(float4 v)
+    {
+        return make_uint2(f32_to_f16_rne(v.x) | (f32_to_f16_rne(v.y) << 16),
+                          f32_to_f16_rne(v.z) | (f32_to_f16_rne(v.w) << 16));
+    }
+    __device__ static inline float4 unpack(T v)
+    {
+        return make_float4(f16_to_f32(v.x), f16_to_f32(v.x >> 16), f16_to_f32(v.y), f16_to_f32(v.y >> 16));
+    }
+};
+
+template <> struct Texel<Storage::BF16> {
+    using T = uint2;
+    __device__ static inline T pack(float4 v)
+    {
+        return make_uint2(f32_to_bf16_rne(v.x) | (f32_to_bf16_rne(v.y) << 16),
+                          f32_to_bf16_rne(v.z) | (f32_to_bf16_rne(v.w) << 16));
+    }
+    __device__ static inline float4 unpack(T v)
+    {
+        return make_float4(bf16_to_f32(v.x & 0xFFFFu), bf16_to_f32(v.x >> 16), bf16_to_f32(v.y & 0xFFFFu),
+                           bf16_to_f32(v.y >> 16));
+    }
+};
+
+inline size_t texel_bytes(Storage s) { return s == Storage::F32 ? 16 : 8; }
+
+}  // namespace rc

@@ -1,0 +1,48 @@
+// rc_internal.hpp -- declarations shared by the library's translation units.
+#pragma once
+
+#include <string>
+
+#include "../../include/rc_gi.h"
+#include "rc_common.hpp"
+
+namespace rc {
+
+// ---- scene (rc_scene.hip) ---------------------------------------------------
+void scene_generate(uint64_t seed, int n_boxes, int n_emissive, rc_box *boxes);
+hipError_t launch_raster(const double *boxes_dev, int n, int W, int H, float *depth, float *albedo, float *emissive,
+                         hipStream_t stream);
+
+// ---- occupancy + jump flood (rc_jfa.hip) --------------------------------------
+int jfa_num_passes(int W, int H);
+int jfa_tail_step(int W, int H);  // largest step fused into the LDS tail (0 = none)
+hipError_t launch_seed_init(const float *depth, const float *emissive, uint32_t *seed, int W, int H, hipStream_t s);
+hipError_t launch_jfa_pass(const uint32_t *in, uint32_t *out, int W, int H, int step, hipStream_t s);
+hipError_t launch_jfa_tail(const uint32_t *in, uint32_t *seeds_out, float *sdf, int W, int H, hipStream_t s);
+
+// ---- cascades (rc_cascade.hip) ----------------------------------------------
+// Workgroup tile of a level kernel: TX x TY probes x KD directions, TX*TY*KD = 256.
+enum TileShape : int { TILE_8x8x4 = 0, TILE_4x4x16 = 1, TILE_2x2x64 = 2, TILE_1x1x256 = 3, TILE_COUNT = 4 };
+int tile_kd(int shape);
+int tile_tx(int shape);
+int tile_ty(int shape);
+
+struct LevelArgs {
+    const float *sdf;        // [H][W]
+    const float *emissive;   // [3][H][W]
+    const float2 *dirs;      // [N] unit directions of this level
+    const void *upper;       // merged level i+1 (storage texels), nullptr at the top level
+    void *out;               // merged level i (storage texels)
+    int W, H;
+    int level, PX, PY, N;    // this level
+    int UX, UY;              // probe grid of level i+1
+    float t0, t1, spacing;
+};
+
+hipError_t launch_level(const LevelArgs &a, Storage st, int shape, hipStream_t s);
+hipError_t launch_level0_gather(const LevelArgs &a, Storage st, float *irradiance, float *debug_fluence,
+                                hipStream_t s);
+hipError_t launch_level_to_f32(const void *level, Storage st, size_t texels, float *out, hipStream_t s);
+hipError_t launch_march_probe(const LevelArgs &a, int32_t *hit, int64_t *index, hipStream_t s);
+
+}  // namespace rc

@@ -1,0 +1,214 @@
+"""Host-side mirror of the reference's pass interface for the GI pass.
+
+The reference renderer drives every pass through the Rust trait
+``RenderStage<T>`` (src/app.rs:3-7): ``update(&mut self, state, queue)`` runs
+first each frame and is the only method that receives the queue, so uploads
+happen there (src/renderer.rs:620-631); ``render(&self, state, view,
+encoder)`` records the pass (src/renderer.rs:559-613); ``resize(&mut self,
+device, config)`` recreates size-dependent resources (src/renderer.rs:615-618).
+:class:`RadianceCascadeStage` keeps those three names and that order over the
+C ABI of librcgi.so, plus ``run_device`` for callers whose G-buffer is already
+in device memory.  Where the reference ``unwrap``s and panics
+(src/renderer.rs:176, src/window/app.rs:43,57,64), this raises
+:class:`RcError`.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _abi
+from ._abi import RcError, check
+
+
+def _dev_ptr(x, nbytes: int | None = None, name: str = "buffer") -> int:
+    """Device address of a torch tensor on the GPU, or an int address."""
+    if x is None:
+        return 0
+    if isinstance(x, int):
+        return x
+    try:
+        import torch
+    except ImportError as e:  # pragma: no cover
+        raise TypeError(f"{name}: pass a device address (int)") from e
+    if not isinstance(x, torch.Tensor):
+        raise TypeError(f"{name}: expected a torch tensor or int address")
+    if not x.is_cuda:
+        raise ValueError(f"{name}: tensor must be on the GPU")
+    if not x.is_contiguous():
+        raise ValueError(f"{name}: tensor must be contiguous")
+    if nbytes is not None and x.numel() * x.element_size() < nbytes:
+        raise ValueError(f"{name}: tensor holds {x.numel() * x.element_size()} bytes, needs {nbytes}")
+    return x.data_ptr()
+
+
+def _stream_handle(stream) -> int:
+    if stream is None:
+        try:
+            import torch
+            return torch.cuda.current_stream().cuda_stream
+        except Exception:
+            return 0
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream
+
+
+@dataclass
+class StageStats:
+    runs: int
+    stage_ms: dict
+    stage_launches: dict
+    level_rays: list
+    level_bytes_stored: list
+
+
+class RadianceCascadeStage:
+    """The GI pass: jump flood -> per-level march/merge -> gather."""
+
+    def __init__(self, width: int, height: int, levels: int = 0, storage: str = "f32",
+                 device: int = 0, profile: bool = False, debug: bool = False):
+        lib = _abi.load()
+        if storage not in _abi.STORAGE:
+            raise ValueError(f"storage must be one of {sorted(_abi.STORAGE)}")
+        cfg = _abi.Config(version=_abi.ABI_VERSION, width=width, height=height, levels=levels,
+                          storage=_abi.STORAGE[storage],
+                          flags=(_abi.FLAG_PROFILE if profile else 0) | (_abi.FLAG_KEEP_DEBUG if debug else 0),
+                          device=device, reserved=0)
+        ctx = ctypes.c_void_p()
+        check(lib.rc_create(ctypes.byref(cfg), ctypes.byref(ctx)))
+        self._lib = lib
+        self._ctx = ctx
+        self._sync_config()
+
+    # -- RenderStage::resize (src/app.rs:5) ----------------------------------
+    def resize(self, width: int, height: int, levels: int = 0) -> None:
+        check(self._lib.rc_resize(self._ctx, width, height, levels), self._ctx)
+        self._sync_config()
+
+    # -- RenderStage::update (src/app.rs:6): host G-buffer upload ------------
+    def update(self, depth: np.ndarray, emissive: np.ndarray, albedo: np.ndarray | None = None) -> None:
+        H, W = self.height, self.width
+        depth = np.ascontiguousarray(depth, np.float32)
+        emissive = np.ascontiguousarray(emissive, np.float32)
+        if depth.shape != (H, W) or emissive.shape != (3, H, W):
+            raise ValueError(f"G-buffer must be depth [{H},{W}] and emissive [3,{H},{W}]")
+        g = _abi.GBuffer(depth=depth.ctypes.data, albedo=None, emissive=emissive.ctypes.data)
+        check(self._lib.rc_update_host(self._ctx, ctypes.byref(g)), self._ctx)
+
+    # -- RenderStage::render (src/app.rs:4): run the pass --------------------
+    def render(self) -> np.ndarray:
+        out = np.empty((3, self.height, self.width), np.float32)
+        check(self._lib.rc_render_host(self._ctx, out.ctypes.data), self._ctx)
+        return out
+
+    # -- device-resident path, ordered on `stream` ---------------------------
+    def run_device(self, depth, emissive, irradiance, stream=None) -> None:
+        n = self.width * self.height * 4
+        g = _abi.GBuffer(depth=_dev_ptr(depth, n, "depth"), albedo=None,
+                         emissive=_dev_ptr(emissive, 3 * n, "emissive"))
+        check(self._lib.rc_run_device(self._ctx, ctypes.byref(g), _dev_ptr(irradiance, 3 * n, "irradiance"),
+                                      _stream_handle(stream)), self._ctx)
+
+    # -- introspection -------------------------------------------------------
+    def _sync_config(self) -> None:
+        cfg = _abi.Config()
+        check(self._lib.rc_get_config(self._ctx, ctypes.byref(cfg)), self._ctx)
+        self.width, self.height, self.levels = cfg.width, cfg.height, cfg.levels
+        self.storage = {0: "f32", 1: "f16", 2: "bf16"}[cfg.storage]
+
+    def stats(self) -> StageStats:
+        s = _abi.Stats()
+        check(self._lib.rc_get_stats(self._ctx, ctypes.byref(s)), self._ctx)
+        names = {_abi.STAGE_SEED: "seed", _abi.STAGE_JFA: "jfa", _abi.STAGE_JFA_TAIL: "jfa_tail"}
+        for i in range(_abi.MAX_LEVELS):
+            names[_abi.STAGE_LEVEL0 + i] = f"level{i}"
+        ms = {names[i]: s.stage_ms[i] for i in range(_abi.NUM_STAGES) if s.stage_launches[i]}
+        la = {names[i]: s.stage_launches[i] for i in range(_abi.NUM_STAGES) if s.stage_launches[i]}
+        return StageStats(runs=s.runs, stage_ms=ms, stage_launches=la,
+                          level_rays=list(s.level_rays)[: self.levels],
+                          level_bytes_stored=list(s.level_bytes_stored)[: self.levels])
+
+    def reset_stats(self) -> None:
+        check(self._lib.rc_reset_stats(self._ctx), self._ctx)
+
+    def level_shape(self, level: int) -> tuple[int, int, int, int]:
+        px, py = probe_grid(self.width, self.height, level)
+        return (py, px, max((4 << (2 * level)) // 4, 1), 4)
+
+    def debug_seeds(self, out, stream=None) -> None:
+        check(self._lib.rc_debug_seeds(self._ctx, _dev_ptr(out, self.width * self.height * 4, "out"),
+                                       _stream_handle(stream)), self._ctx)
+
+    def debug_sdf(self, out, stream=None) -> None:
+        check(self._lib.rc_debug_sdf(self._ctx, _dev_ptr(out, self.width * self.height * 4, "out"),
+                                     _stream_handle(stream)), self._ctx)
+
+    def debug_level(self, level: int, out, stream=None) -> None:
+        n = int(np.prod(self.level_shape(level))) * 4
+        check(self._lib.rc_debug_level(self._ctx, level, _dev_ptr(out, n, "out"), _stream_handle(stream)),
+              self._ctx)
+
+    def debug_march(self, level: int, hit, index, stream=None) -> None:
+        rays = level_rays(self.width, self.height, level)
+        check(self._lib.rc_debug_march(self._ctx, level, _dev_ptr(hit, rays * 4, "hit"),
+                                       _dev_ptr(index, rays * 8, "index"), _stream_handle(stream)), self._ctx)
+
+    def close(self) -> None:
+        if getattr(self, "_ctx", None) is not None and self._ctx.value:
+            self._lib.rc_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---- geometry and scenes ------------------------------------------------------
+def probe_grid(width: int, height: int, level: int) -> tuple[int, int]:
+    lib = _abi.load()
+    px, py = ctypes.c_uint32(), ctypes.c_uint32()
+    check(lib.rc_probe_grid(width, height, level, ctypes.byref(px), ctypes.byref(py)))
+    return px.value, py.value
+
+
+def level_rays(width: int, height: int, level: int) -> int:
+    return _abi.load().rc_level_rays(width, height, level)
+
+
+def levels_for_coverage(width: int, height: int) -> int:
+    return _abi.load().rc_levels_for_coverage(width, height)
+
+
+def device_count() -> int:
+    n = ctypes.c_int()
+    st = _abi.load().rc_device_count(ctypes.byref(n))
+    return n.value if st == _abi.OK else 0
+
+
+def scene_generate(seed: int, n_boxes: int, n_emissive: int) -> np.ndarray:
+    """Box list as an (n, 12) float64 array: lo xyz, hi xyz, albedo rgb, emit rgb."""
+    boxes = (_abi.Box * max(n_boxes, 1))()
+    check(_abi.load().rc_scene_generate(seed, n_boxes, n_emissive, ctypes.cast(boxes, ctypes.c_void_p)))
+    arr = np.ctypeslib.as_array(ctypes.cast(boxes, ctypes.POINTER(ctypes.c_double)), shape=(max(n_boxes, 1) * 12,))
+    return arr.reshape(-1, 12)[:n_boxes].copy()
+
+
+def gbuffer_raster(boxes: np.ndarray, width: int, height: int, depth, albedo, emissive, stream=None) -> None:
+    """Rasterise a box scene into device G-buffer tensors (planar fp32)."""
+    b = np.ascontiguousarray(boxes, np.float64).reshape(-1, 12)
+    buf = b if b.size else np.zeros((1, 12))
+    n = width * height * 4
+    check(_abi.load().rc_gbuffer_raster(buf.ctypes.data, b.shape[0], width, height,
+                                        _dev_ptr(depth, n, "depth"), _dev_ptr(albedo, 3 * n, "albedo"),
+                                        _dev_ptr(emissive, 3 * n, "emissive"), _stream_handle(stream)))
