@@ -41,7 +41,9 @@ def kernel_alg_bytes(W, H, L, storage, rays):
     merged level 1 (b R_1) and writes 12 B/px irradiance."""
     b = texel_bytes_contract(storage)
     n = W * H
-    out = {"seed": 28 * n, "jfa": 0, "jfa_tail": 4 * n}
+    # a global jump-flood pass is charged the informative 8 B/px (seed field
+    # read once, written once); the contract itself has no JFA term.
+    out = {"seed": 28 * n, "jfa": 8 * n, "jfa_tail": 4 * n}
     for i in range(1, L):
         out[f"level{i}"] = b * rays[i] + (b * rays[i + 1] if i + 1 < L else 0)
     out["level0"] = 4 * n + 12 * n + (b * rays[1] if L > 1 else 0)
@@ -84,6 +86,7 @@ def main():
     ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"])
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -106,6 +109,7 @@ def main():
     stream = torch.cuda.current_stream()
 
     stage = rc.RadianceCascadeStage(W, H, levels=L, storage=args.storage)
+    tiling = stage.autotune(depth, emis, irr, iters=3, stream=stream) if not args.no_autotune else stage.tiling()
     for _ in range(args.warmup):
         stage.run_device(depth, emis, irr, stream)
     torch.cuda.synchronize()
@@ -124,6 +128,9 @@ def main():
     # stage on the same stream (synchronises once per frame, so it is not the
     # throughput figure above).
     prof = rc.RadianceCascadeStage(W, H, levels=L, storage=args.storage, profile=True)
+    for lvl, ((tx, ty, kd), rpt) in tiling.items():
+        shape = [k for k, v in rc._abi.TILE_SHAPES.items() if v == (tx, ty, kd)][0]
+        prof.set_tiling(lvl, shape, rpt)
     for _ in range(5):
         prof.run_device(depth, emis, irr, stream)
     prof.reset_stats()
@@ -161,7 +168,8 @@ def main():
         "config": {"workload": f"plain HIP path, {W}x{H} G-buffer, {L} cascades, {args.storage} storage, "
                                f"seeded box scene (seed {args.seed}, {args.boxes} boxes, {args.emissive} emissive)",
                    "path": "plain", "width": W, "height": H, "levels": L, "storage": args.storage,
-                   "rays_per_frame": int(sum(rays))},
+                   "rays_per_frame": int(sum(rays)),
+                   "tiling": {str(k): f"{v[0][0]}x{v[0][1]}x{v[0][2]} r{v[1]}" for k, v in tiling.items()}},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "alg_bytes_per_launch": alg[dominant], "mean_launch_ms": dom_ms},

@@ -65,6 +65,8 @@ typedef enum { RC_STORAGE_F32 = 0, RC_STORAGE_F16 = 1, RC_STORAGE_BF16 = 2 } rc_
 /* rc_config.flags */
 #define RC_FLAG_PROFILE    0x1u  /* time every stage with HIP events (rc_get_stats) */
 #define RC_FLAG_KEEP_DEBUG 0x2u  /* also write the jump-flood field and level 0 for rc_debug_* */
+#define RC_FLAG_NO_GRAPH   0x4u  /* rc_run_device launches kernels one by one instead of
+                                    replaying the frame as a captured hipGraph */
 
 typedef struct {
     uint32_t version;   /* RC_ABI_VERSION */
@@ -112,6 +114,19 @@ rc_status rc_update_host(rc_ctx *ctx, const rc_gbuffer *host);
 rc_status rc_render_host(rc_ctx *ctx, float *irradiance_host);
 rc_status rc_run_device(rc_ctx *ctx, const rc_gbuffer *dev, float *irradiance_dev, void *stream);
 void rc_destroy(rc_ctx *ctx);
+
+/* ---- level-kernel tiling (autotuner) ----------------------------------------
+ * A level-i kernel (i >= 1) runs workgroups of TX x TY probes x KD directions
+ * (TX*TY*KD = 256 lanes) with R rays per lane (probe rows TY apart).
+ * shape: 0 = 8x8x4, 1 = 4x4x16, 2 = 2x2x64, 3 = 1x1x256 (KD <= N_i required);
+ * rays_per_lane: 1, 2 or 4.  Every tiling computes identical results.
+ * rc_autotune times every legal (shape, rays_per_lane) of every level on the
+ * given device G-buffer (iters launches each, HIP events) and keeps the
+ * fastest; rc_get_level_tiling reports the current choice. */
+#define RC_NUM_TILE_SHAPES 4
+rc_status rc_set_level_tiling(rc_ctx *ctx, uint32_t level, uint32_t shape, uint32_t rays_per_lane);
+rc_status rc_get_level_tiling(const rc_ctx *ctx, uint32_t level, uint32_t *shape, uint32_t *rays_per_lane);
+rc_status rc_autotune(rc_ctx *ctx, const rc_gbuffer *dev, float *irradiance_dev, uint32_t iters, void *stream);
 
 /* ---- introspection -------------------------------------------------------- */
 rc_status rc_get_config(const rc_ctx *ctx, rc_config *out);

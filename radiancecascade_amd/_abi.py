@@ -26,6 +26,9 @@ STATUS_NAMES = {0: "RC_OK", 1: "RC_ERR_INVALID", 2: "RC_ERR_NULL", 3: "RC_ERR_NO
 STORAGE = {"f32": 0, "fp32": 0, "f16": 1, "fp16": 1, "bf16": 2}
 FLAG_PROFILE = 0x1
 FLAG_KEEP_DEBUG = 0x2
+FLAG_NO_GRAPH = 0x4
+
+TILE_SHAPES = {0: (8, 8, 4), 1: (4, 4, 16), 2: (2, 2, 64), 3: (1, 1, 256)}
 
 STAGE_SEED, STAGE_JFA, STAGE_JFA_TAIL, STAGE_LEVEL0 = 0, 1, 2, 3
 NUM_STAGES = STAGE_LEVEL0 + MAX_LEVELS
@@ -64,6 +67,9 @@ SIGNATURES = {
     "rc_render_host": (_i32, [_P, _P]),
     "rc_run_device": (_i32, [_P, ctypes.POINTER(GBuffer), _P, _P]),
     "rc_destroy": (None, [_P]),
+    "rc_set_level_tiling": (_i32, [_P, _u32, _u32, _u32]),
+    "rc_get_level_tiling": (_i32, [_P, _u32, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
+    "rc_autotune": (_i32, [_P, ctypes.POINTER(GBuffer), _P, _u32, _P]),
     "rc_get_config": (_i32, [_P, ctypes.POINTER(Config)]),
     "rc_get_stats": (_i32, [_P, ctypes.POINTER(Stats)]),
     "rc_reset_stats": (_i32, [_P]),

@@ -30,12 +30,16 @@ struct rc_ctx {
     float2 *dirs = nullptr;
     size_t dir_off[RC_MAX_LEVELS] = {};
     int tile[RC_MAX_LEVELS] = {};
+    int rpt[RC_MAX_LEVELS] = {};
     int dbg_seed_buf = -1;
     bool ran = false;
     // host-staging path (rc_update_host / rc_render_host)
     float *g_depth = nullptr, *g_emissive = nullptr, *irr = nullptr;
     bool have_gbuffer = false;
-    hipStream_t stream = nullptr;
+    // whole-frame hipGraph for rc_run_device, keyed on the caller's buffers
+    hipStream_t cap_stream = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    const void *g_key[3] = {nullptr, nullptr, nullptr};
     // profiling
     hipEvent_t ev[2 * RC_NUM_STAGES] = {};
     bool ev_used[RC_NUM_STAGES] = {};
@@ -84,8 +88,16 @@ static int default_tile(int level, int L)
     return TILE_8x8x4;
 }
 
+static void drop_graph(rc_ctx *c)
+{
+    if (c->gexec) hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+    c->g_key[0] = c->g_key[1] = c->g_key[2] = nullptr;
+}
+
 static void release(rc_ctx *c)
 {
+    drop_graph(c);
     for (auto &p : c->seed) { if (p) hipFree(p); p = nullptr; }
     if (c->sdf) hipFree(c->sdf);
     c->sdf = nullptr;
@@ -130,6 +142,7 @@ static rc_status allocate(rc_ctx *c, uint32_t W, uint32_t H, uint32_t L)
             dirs.push_back(make_float2((float)cos(th), (float)sin(th)));
         }
         c->tile[i] = default_tile(i, (int)L);
+        c->rpt[i] = 2;
     }
     HIP_TRY(c, hipMalloc(&c->dirs, dirs.size() * sizeof(float2)));
     HIP_TRY(c, hipMemcpy(c->dirs, dirs.data(), dirs.size() * sizeof(float2), hipMemcpyHostToDevice));
@@ -176,9 +189,10 @@ struct StageTimer {
     }
 };
 
-static rc_status run_frame(rc_ctx *c, const float *depth, const float *emissive, float *irr, hipStream_t s)
+static rc_status run_frame(rc_ctx *c, const float *depth, const float *emissive, float *irr, hipStream_t s,
+                           bool capturing = false)
 {
-    const bool prof = (c->cfg.flags & RC_FLAG_PROFILE) != 0;
+    const bool prof = !capturing && (c->cfg.flags & RC_FLAG_PROFILE) != 0;
     const bool dbg = (c->cfg.flags & RC_FLAG_KEEP_DEBUG) != 0;
     StageTimer tm{c, s, prof};
     for (auto &u : c->ev_used) u = false;
@@ -204,7 +218,7 @@ static rc_status run_frame(rc_ctx *c, const float *depth, const float *emissive,
 
     for (int i = c->L - 1; i >= 1; --i) {
         tm.begin(RC_STAGE_LEVEL0 + i);
-        HIP_TRY(c, launch_level(level_args(c, i, emissive), c->storage, c->tile[i], s));
+        HIP_TRY(c, launch_level(level_args(c, i, emissive), c->storage, c->tile[i], c->rpt[i], s));
         tm.end(RC_STAGE_LEVEL0 + i);
     }
     tm.begin(RC_STAGE_LEVEL0);
@@ -221,6 +235,45 @@ static rc_status run_frame(rc_ctx *c, const float *depth, const float *emissive,
             c->stats.stage_launches[st] += st == RC_STAGE_JFA ? (uint32_t)global_passes : 1u;
         }
     }
+    if (!capturing) {
+        c->stats.runs += 1;
+        c->ran = true;
+    }
+    return RC_OK;
+}
+
+// The frame's 15 launches (at 1080p) replayed as one graph: profiling and
+// debug runs, which record events or extra outputs, launch directly.
+static bool graph_enabled(const rc_ctx *c)
+{
+    return (c->cfg.flags & (RC_FLAG_PROFILE | RC_FLAG_KEEP_DEBUG | RC_FLAG_NO_GRAPH)) == 0;
+}
+
+static rc_status run_frame_graph(rc_ctx *c, const float *depth, const float *emissive, float *irr, hipStream_t s)
+{
+    if (!c->gexec || c->g_key[0] != depth || c->g_key[1] != emissive || c->g_key[2] != irr) {
+        drop_graph(c);
+        if (!c->cap_stream) HIP_TRY(c, hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking));
+        HIP_TRY(c, hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal));
+        rc_status st = run_frame(c, depth, emissive, irr, c->cap_stream, true);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(c->cap_stream, &g);
+        if (st != RC_OK) {
+            if (g) hipGraphDestroy(g);
+            return st;
+        }
+        if (e != hipSuccess) return fail(c, RC_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            c->gexec = nullptr;
+            return fail(c, RC_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+        c->g_key[0] = depth;
+        c->g_key[1] = emissive;
+        c->g_key[2] = irr;
+    }
+    HIP_TRY(c, hipGraphLaunch(c->gexec, s));
     c->stats.runs += 1;
     c->ran = true;
     return RC_OK;
@@ -268,7 +321,7 @@ rc_status rc_create(const rc_config *cfg, rc_ctx **out)
     if (!dims_ok(cfg->width, cfg->height)) return fail(nullptr, RC_ERR_INVALID, "rc_create: width/height out of range");
     if (cfg->levels > RC_MAX_LEVELS) return fail(nullptr, RC_ERR_INVALID, "rc_create: levels out of range");
     if (cfg->storage > RC_STORAGE_BF16) return fail(nullptr, RC_ERR_INVALID, "rc_create: unknown storage");
-    if (cfg->flags & ~(RC_FLAG_PROFILE | RC_FLAG_KEEP_DEBUG)) return fail(nullptr, RC_ERR_INVALID, "rc_create: unknown flags");
+    if (cfg->flags & ~(RC_FLAG_PROFILE | RC_FLAG_KEEP_DEBUG | RC_FLAG_NO_GRAPH)) return fail(nullptr, RC_ERR_INVALID, "rc_create: unknown flags");
     int ndev = 0;
     if (rc_device_count(&ndev) != RC_OK) return RC_ERR_NO_DEVICE;
     if (cfg->device < 0 || cfg->device >= ndev) return fail(nullptr, RC_ERR_INVALID, "rc_create: device ordinal out of range");
@@ -313,6 +366,7 @@ rc_status rc_run_device(rc_ctx *c, const rc_gbuffer *g, float *irr, void *stream
     if (!c) return fail(nullptr, RC_ERR_NULL, "rc_run_device: ctx is NULL");
     if (!g || !g->depth || !g->emissive || !irr) return fail(c, RC_ERR_NULL, "rc_run_device: G-buffer or output is NULL");
     HIP_TRY(c, hipSetDevice(c->cfg.device));
+    if (graph_enabled(c)) return run_frame_graph(c, g->depth, g->emissive, irr, (hipStream_t)stream);
     return run_frame(c, g->depth, g->emissive, irr, (hipStream_t)stream);
 }
 
@@ -353,6 +407,7 @@ void rc_destroy(rc_ctx *c)
     release(c);
     for (auto &ev : c->ev)
         if (ev) hipEventDestroy(ev);
+    if (c->cap_stream) hipStreamDestroy(c->cap_stream);
     delete c;
 }
 
@@ -420,6 +475,73 @@ rc_status rc_debug_march(rc_ctx *c, uint32_t level, int32_t *hit_dev, int64_t *i
     LevelArgs a = level_args(c, (int)level, nullptr);
     HIP_TRY(c, launch_march_probe(a, hit_dev, index_dev, (hipStream_t)stream));
     return RC_OK;
+}
+
+static bool tiling_legal(int level, int shape, int rpt)
+{
+    return level >= 1 && shape >= 0 && shape < TILE_COUNT && tile_kd(shape) <= num_dirs(level) &&
+           (rpt == 1 || rpt == 2 || rpt == 4);
+}
+
+rc_status rc_set_level_tiling(rc_ctx *c, uint32_t level, uint32_t shape, uint32_t rpt)
+{
+    if (!c) return fail(nullptr, RC_ERR_NULL, "rc_set_level_tiling: ctx is NULL");
+    if ((int)level >= c->L || !tiling_legal((int)level, (int)shape, (int)rpt))
+        return fail(c, RC_ERR_INVALID, "rc_set_level_tiling: illegal level/shape/rays_per_lane");
+    c->tile[level] = (int)shape;
+    c->rpt[level] = (int)rpt;
+    drop_graph(c);
+    return RC_OK;
+}
+
+rc_status rc_get_level_tiling(const rc_ctx *c, uint32_t level, uint32_t *shape, uint32_t *rpt)
+{
+    if (!c || !shape || !rpt) return fail(nullptr, RC_ERR_NULL, "rc_get_level_tiling: NULL argument");
+    if (level == 0 || (int)level >= c->L) return fail(nullptr, RC_ERR_INVALID, "rc_get_level_tiling: level out of range");
+    *shape = (uint32_t)c->tile[level];
+    *rpt = (uint32_t)c->rpt[level];
+    return RC_OK;
+}
+
+// Time every legal tiling of every level i >= 1 on this G-buffer and keep the
+// fastest.  A level kernel's cost does not depend on the upper level's
+// values, so after one full frame each candidate is launched in isolation.
+rc_status rc_autotune(rc_ctx *c, const rc_gbuffer *g, float *irr, uint32_t iters, void *stream)
+{
+    if (!c) return fail(nullptr, RC_ERR_NULL, "rc_autotune: ctx is NULL");
+    if (!g || !g->depth || !g->emissive || !irr) return fail(c, RC_ERR_NULL, "rc_autotune: G-buffer or output is NULL");
+    if (iters == 0) iters = 1;
+    HIP_TRY(c, hipSetDevice(c->cfg.device));
+    hipStream_t s = (hipStream_t)stream;
+    const uint32_t saved_flags = c->cfg.flags;
+    c->cfg.flags &= ~RC_FLAG_PROFILE;
+    rc_status st = run_frame(c, g->depth, g->emissive, irr, s);
+    c->cfg.flags = saved_flags;
+    if (st != RC_OK) return st;
+    hipEvent_t e0 = c->ev[0], e1 = c->ev[1];
+    for (int i = 1; i < c->L; ++i) {
+        float best = 1e30f;
+        int best_shape = c->tile[i], best_rpt = c->rpt[i];
+        const LevelArgs a = level_args(c, i, g->emissive);
+        for (int shape = 0; shape < TILE_COUNT; ++shape) {
+            for (int rpt : {1, 2, 4}) {
+                if (!tiling_legal(i, shape, rpt)) continue;
+                HIP_TRY(c, launch_level(a, c->storage, shape, rpt, s));  // warm
+                HIP_TRY(c, hipEventRecord(e0, s));
+                for (uint32_t it = 0; it < iters; ++it) HIP_TRY(c, launch_level(a, c->storage, shape, rpt, s));
+                HIP_TRY(c, hipEventRecord(e1, s));
+                HIP_TRY(c, hipEventSynchronize(e1));
+                float ms = 0.f;
+                HIP_TRY(c, hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) { best = ms; best_shape = shape; best_rpt = rpt; }
+            }
+        }
+        c->tile[i] = best_shape;
+        c->rpt[i] = best_rpt;
+    }
+    drop_graph(c);
+    // leave the level buffers consistent with one full frame
+    return run_frame(c, g->depth, g->emissive, irr, s);
 }
 
 rc_status rc_probe_grid(uint32_t W, uint32_t H, uint32_t level, uint32_t *px, uint32_t *py)

@@ -92,23 +92,74 @@ __device__ inline float3 bilerp4(float4 u00, float4 u10, float4 u01, float4 u11,
     return r;
 }
 
-template <int TX, int TY, int KD, Storage S, bool UPPER>
+// RPT rays per lane marched in lockstep: each iteration issues the RPT
+// distance-field loads back to back, so a lane keeps RPT independent load
+// chains in flight (the march is a chain of dependent gathers and is bound by
+// their latency, not by bandwidth).  A finished ray's slot keeps loading the
+// (cached) first texel until every ray of the lane is done.
+template <int RPT>
+__device__ inline void march_rays(const float *__restrict__ sdf, int W, int H, const float *cx, const float *cy,
+                                  float dx, float dy, float t0, float t1, const bool *active, int *hitq)
+{
+    const float fW = (float)W, fH = (float)H;
+    float t[RPT];
+    bool alive[RPT];
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+        t[j] = t0;
+        alive[j] = active[j] && t0 < t1;
+        hitq[j] = -1;
+    }
+    for (;;) {
+        int q[RPT];
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            const float x = floorf(fmaf(t[j], dx, cx[j]));
+            const float y = floorf(fmaf(t[j], dy, cy[j]));
+            const bool inside = x >= 0.f && y >= 0.f && x < fW && y < fH;
+            alive[j] = alive[j] && inside;
+            q[j] = alive[j] ? (int)y * W + (int)x : 0;
+        }
+        float d[RPT];
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) d[j] = sdf[q[j]];
+        bool any = false;
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            if (alive[j]) {
+                if (d[j] < 0.5f) {
+                    hitq[j] = q[j];
+                    alive[j] = false;
+                } else {
+                    t[j] = t[j] + fmaxf(d[j], 1.0f);
+                    alive[j] = t[j] < t1;
+                }
+            }
+            any = any || alive[j];
+        }
+        if (!any) break;
+    }
+}
+
+// Level i >= 1.  Workgroup tile: TX x (TY*RPT) probes x KD directions; lane
+// (probe column, direction) owns the RPT probes stacked TY rows apart.
+template <int TX, int TY, int KD, int RPT, Storage S, bool UPPER>
 __global__ __launch_bounds__(kBlock) void k_level(LevelArgs a)
 {
-    static_assert(TX * TY * KD == kBlock, "tile must hold 256 rays");
+    static_assert(TX * TY * KD == kBlock, "a tile row block holds 256 rays");
     using TT = Texel<S>;
-    constexpr int UCX = TX / 2 + 2, UCY = TY / 2 + 2;
+    constexpr int TYR = TY * RPT;
+    constexpr int UCX = TX / 2 + 2, UCY = TYR / 2 + 2;
     __shared__ float4 su[UPPER ? UCX * UCY * KD : 1];
 
     const int tiles_x = (a.PX + TX - 1) / TX;
     const int slices = a.N / KD;
     const int lb = xcd_remap_c(blockIdx.x, gridDim.x);
     const int slice = lb % slices, tile = lb / slices;
-    const int px0 = (tile % tiles_x) * TX, py0 = (tile / tiles_x) * TY, k0 = slice * KD;
+    const int px0 = (tile % tiles_x) * TX, py0 = (tile / tiles_x) * TYR, k0 = slice * KD;
 
     const int kk = threadIdx.x % KD, pl = threadIdx.x / KD;
-    const int px = px0 + pl % TX, py = py0 + pl / TX, k = k0 + kk;
-    const bool active = px < a.PX && py < a.PY;
+    const int px = px0 + pl % TX, pyb = py0 + pl / TX, k = k0 + kk;
 
     const int ulx = (px0 - 1) >> 1, uly = (py0 - 1) >> 1;
     if (UPPER) {
@@ -121,34 +172,52 @@ __global__ __launch_bounds__(kBlock) void k_level(LevelArgs a)
         __syncthreads();
     }
 
-    float r = 0.f, g = 0.f, b = 0.f, al = 0.f;
-    if (active) {
-        const float2 dir = a.dirs[k];
-        const float cx = ((float)px + 0.5f) * a.spacing, cy = ((float)py + 0.5f) * a.spacing;
-        MarchResult m = march_ray(a.sdf, a.emissive, a.W, a.H, cx, cy, dir.x, dir.y, a.t0, a.t1);
-        r = m.r;
-        g = m.g;
-        b = m.b;
-        al = m.miss ? 1.0f : 0.0f;
-        if (UPPER && m.miss) {
-            const int sx = ((px - 1) >> 1) - ulx, sy = ((py - 1) >> 1) - uly;
-            const float4 u00 = su[(sy * UCX + sx) * KD + kk];
-            const float4 u10 = su[(sy * UCX + sx + 1) * KD + kk];
-            const float4 u01 = su[((sy + 1) * UCX + sx) * KD + kk];
-            const float4 u11 = su[((sy + 1) * UCX + sx + 1) * KD + kk];
-            const float3 v = bilerp4(u00, u10, u01, u11, px, py);
-            r = v.x;
-            g = v.y;
-            b = v.z;
-        }
+    const float2 dir = a.dirs[k];
+    float cx[RPT], cy[RPT];
+    bool active[RPT];
+    int hitq[RPT];
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+        const int py = pyb + j * TY;
+        cx[j] = ((float)px + 0.5f) * a.spacing;
+        cy[j] = ((float)py + 0.5f) * a.spacing;
+        active[j] = px < a.PX && py < a.PY;
     }
-    r = quad_sum(r) * 0.25f;
-    g = quad_sum(g) * 0.25f;
-    b = quad_sum(b) * 0.25f;
-    al = quad_sum(al) * 0.25f;
-    if (active && (kk & 3) == 0) {
-        typename TT::T *out = (typename TT::T *)a.out;
-        out[((size_t)py * a.PX + px) * (a.N >> 2) + (k >> 2)] = TT::pack(make_float4(r, g, b, al));
+    march_rays<RPT>(a.sdf, a.W, a.H, cx, cy, dir.x, dir.y, a.t0, a.t1, active, hitq);
+
+    const size_t plane = (size_t)a.W * a.H;
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+        const int py = pyb + j * TY;
+        float r = 0.f, g = 0.f, b = 0.f, al = 0.f;
+        if (active[j]) {
+            if (hitq[j] >= 0) {
+                r = a.emissive[hitq[j]];
+                g = a.emissive[plane + hitq[j]];
+                b = a.emissive[2 * plane + hitq[j]];
+            } else {
+                al = 1.0f;
+                if (UPPER) {
+                    const int sx = ((px - 1) >> 1) - ulx, sy = ((py - 1) >> 1) - uly;
+                    const float4 u00 = su[(sy * UCX + sx) * KD + kk];
+                    const float4 u10 = su[(sy * UCX + sx + 1) * KD + kk];
+                    const float4 u01 = su[((sy + 1) * UCX + sx) * KD + kk];
+                    const float4 u11 = su[((sy + 1) * UCX + sx + 1) * KD + kk];
+                    const float3 v = bilerp4(u00, u10, u01, u11, px, py);
+                    r = v.x;
+                    g = v.y;
+                    b = v.z;
+                }
+            }
+        }
+        r = quad_sum(r) * 0.25f;
+        g = quad_sum(g) * 0.25f;
+        b = quad_sum(b) * 0.25f;
+        al = quad_sum(al) * 0.25f;
+        if (active[j] && (kk & 3) == 0) {
+            typename TT::T *out = (typename TT::T *)a.out;
+            out[((size_t)py * a.PX + px) * (a.N >> 2) + (k >> 2)] = TT::pack(make_float4(r, g, b, al));
+        }
     }
 }
 
@@ -262,34 +331,45 @@ int tile_kd(int shape) { return shape == TILE_8x8x4 ? 4 : shape == TILE_4x4x16 ?
 int tile_tx(int shape) { return shape == TILE_8x8x4 ? 8 : shape == TILE_4x4x16 ? 4 : shape == TILE_2x2x64 ? 2 : 1; }
 int tile_ty(int shape) { return tile_tx(shape); }
 
-template <int TX, int TY, int KD, Storage S>
+template <int TX, int TY, int KD, int RPT, Storage S>
 static void launch_level_shape(const LevelArgs &a, hipStream_t s)
 {
-    const int blocks = ((a.PX + TX - 1) / TX) * ((a.PY + TY - 1) / TY) * (a.N / KD);
-    if (a.upper) hipLaunchKernelGGL((k_level<TX, TY, KD, S, true>), dim3(blocks), dim3(kBlock), 0, s, a);
-    else hipLaunchKernelGGL((k_level<TX, TY, KD, S, false>), dim3(blocks), dim3(kBlock), 0, s, a);
+    const int blocks = ((a.PX + TX - 1) / TX) * ((a.PY + TY * RPT - 1) / (TY * RPT)) * (a.N / KD);
+    if (a.upper) hipLaunchKernelGGL((k_level<TX, TY, KD, RPT, S, true>), dim3(blocks), dim3(kBlock), 0, s, a);
+    else hipLaunchKernelGGL((k_level<TX, TY, KD, RPT, S, false>), dim3(blocks), dim3(kBlock), 0, s, a);
 }
 
-template <Storage S>
-static hipError_t launch_level_st(const LevelArgs &a, int shape, hipStream_t s)
+template <int RPT, Storage S>
+static hipError_t launch_level_rpt(const LevelArgs &a, int shape, hipStream_t s)
 {
-    if (tile_kd(shape) > a.N) return hipErrorInvalidValue;
     switch (shape) {
-    case TILE_8x8x4: launch_level_shape<8, 8, 4, S>(a, s); break;
-    case TILE_4x4x16: launch_level_shape<4, 4, 16, S>(a, s); break;
-    case TILE_2x2x64: launch_level_shape<2, 2, 64, S>(a, s); break;
-    case TILE_1x1x256: launch_level_shape<1, 1, 256, S>(a, s); break;
+    case TILE_8x8x4: launch_level_shape<8, 8, 4, RPT, S>(a, s); break;
+    case TILE_4x4x16: launch_level_shape<4, 4, 16, RPT, S>(a, s); break;
+    case TILE_2x2x64: launch_level_shape<2, 2, 64, RPT, S>(a, s); break;
+    case TILE_1x1x256: launch_level_shape<1, 1, 256, RPT, S>(a, s); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_level(const LevelArgs &a, Storage st, int shape, hipStream_t s)
+template <Storage S>
+static hipError_t launch_level_st(const LevelArgs &a, int shape, int rpt, hipStream_t s)
+{
+    if (tile_kd(shape) > a.N) return hipErrorInvalidValue;
+    switch (rpt) {
+    case 1: return launch_level_rpt<1, S>(a, shape, s);
+    case 2: return launch_level_rpt<2, S>(a, shape, s);
+    case 4: return launch_level_rpt<4, S>(a, shape, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_level(const LevelArgs &a, Storage st, int shape, int rpt, hipStream_t s)
 {
     switch (st) {
-    case Storage::F32: return launch_level_st<Storage::F32>(a, shape, s);
-    case Storage::F16: return launch_level_st<Storage::F16>(a, shape, s);
-    case Storage::BF16: return launch_level_st<Storage::BF16>(a, shape, s);
+    case Storage::F32: return launch_level_st<Storage::F32>(a, shape, rpt, s);
+    case Storage::F16: return launch_level_st<Storage::F16>(a, shape, rpt, s);
+    case Storage::BF16: return launch_level_st<Storage::BF16>(a, shape, rpt, s);
     }
     return hipErrorInvalidValue;
 }

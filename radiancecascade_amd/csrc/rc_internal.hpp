@@ -39,7 +39,7 @@ struct LevelArgs {
     float t0, t1, spacing;
 };
 
-hipError_t launch_level(const LevelArgs &a, Storage st, int shape, hipStream_t s);
+hipError_t launch_level(const LevelArgs &a, Storage st, int shape, int rays_per_lane, hipStream_t s);
 hipError_t launch_level0_gather(const LevelArgs &a, Storage st, float *irradiance, float *debug_fluence,
                                 hipStream_t s);
 hipError_t launch_level_to_f32(const void *level, Storage st, size_t texels, float *out, hipStream_t s);

@@ -28,11 +28,19 @@ __device__ inline int xcd_remap(int b, int nb)
     return xcd * q + min(xcd, r) + idx;
 }
 
+// Inside the kernels "no seed" is a fake seed at (40000, 40000): for any
+// pixel of an image up to 16384 px wide its squared distance exceeds 2^30,
+// above every real candidate's (< 2 * 16383^2 < 2^30), and stays below 2^32.
+// So the (d^2, seed) minimum needs no special case; the external value
+// kSeedNone is restored on output.
+constexpr uint32_t kFarSeed = (40000u << 16) | 40000u;
+
+// Order key (d^2, seed_y, seed_x): the packed seed already orders (y, x).
 __device__ inline uint64_t jfa_key(uint32_t s, int x, int y)
 {
-    if (s == kSeedNone) return ~0ull;
     const int dx = (int)(s & 0xFFFFu) - x, dy = (int)(s >> 16) - y;
-    return ((uint64_t)(uint32_t)(dx * dx + dy * dy) << 32) | s;
+    const uint32_t d2 = (uint32_t)__mul24(dx, dx) + (uint32_t)__mul24(dy, dy);
+    return ((uint64_t)d2 << 32) | s;
 }
 
 int jfa_num_passes(int W, int H)
@@ -60,14 +68,16 @@ __global__ __launch_bounds__(kBlock) void k_seed_init(const float *__restrict__ 
     const int y = (b / bx_count) * 4 + (threadIdx.x >> 6);
     if (x >= W || y >= H) return;
     const size_t plane = (size_t)W * H, p = (size_t)y * W + x;
-    bool occ = emissive[p] > 0.0f || emissive[plane + p] > 0.0f || emissive[2 * plane + p] > 0.0f;
+    // all loads unconditional (clamped addresses) so they are in flight together
+    const float e0 = emissive[p], e1 = emissive[plane + p], e2 = emissive[2 * plane + p];
     const float dp = depth[p];
+    const float dl = depth[p - (x > 0)], dr = depth[p + (x + 1 < W)];
+    const float du = depth[p - (y > 0 ? (size_t)W : 0)], dd = depth[p + (y + 1 < H ? (size_t)W : 0)];
     const float thr = 0.05f * dp;
-    if (!occ && x > 0) occ = depth[p - 1] - dp > thr;
-    if (!occ && x + 1 < W) occ = depth[p + 1] - dp > thr;
-    if (!occ && y > 0) occ = depth[p - W] - dp > thr;
-    if (!occ && y + 1 < H) occ = depth[p + W] - dp > thr;
-    seed[p] = occ ? (((uint32_t)y << 16) | (uint32_t)x) : kSeedNone;
+    // an out-of-image neighbour reads p itself and is masked out
+    const bool occ = e0 > 0.0f || e1 > 0.0f || e2 > 0.0f || (x > 0 && dl - dp > thr) ||
+                     (x + 1 < W && dr - dp > thr) || (y > 0 && du - dp > thr) || (y + 1 < H && dd - dp > thr);
+    seed[p] = occ ? (((uint32_t)y << 16) | (uint32_t)x) : kFarSeed;
 }
 
 __global__ __launch_bounds__(kBlock) void k_jfa_pass(const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
@@ -77,26 +87,32 @@ __global__ __launch_bounds__(kBlock) void k_jfa_pass(const uint32_t *__restrict_
     const int x = (b % bx_count) * 64 + (threadIdx.x & 63);
     const int y = (b / bx_count) * 4 + (threadIdx.x >> 6);
     if (x >= W || y >= H) return;
-    uint64_t best = jfa_key(in[(size_t)y * W + x], x, y);
+    // 9 unconditional loads at clamped coordinates (in flight together); a
+    // candidate whose true coordinate is off-image is replaced by the far seed
+    const int xs[3] = {x - step, x, x + step}, ys[3] = {y - step, y, y + step};
+    uint32_t cand[9];
 #pragma unroll
-    for (int dy = -1; dy <= 1; ++dy) {
-        const int qy = y + dy * step;
-        if (qy < 0 || qy >= H) continue;
-#pragma unroll
-        for (int dx = -1; dx <= 1; ++dx) {
-            const int qx = x + dx * step;
-            if ((dx == 0 && dy == 0) || qx < 0 || qx >= W) continue;
-            const uint64_t k = jfa_key(in[(size_t)qy * W + qx], x, y);
-            best = k < best ? k : best;
-        }
+    for (int j = 0; j < 9; ++j) {
+        const int qx = min(max(xs[j % 3], 0), W - 1), qy = min(max(ys[j / 3], 0), H - 1);
+        cand[j] = in[(size_t)qy * W + qx];
     }
-    out[(size_t)y * W + x] = best == ~0ull ? kSeedNone : (uint32_t)best;
+    uint64_t best = jfa_key(cand[4], x, y);
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        if (j == 4) continue;
+        const bool inb = xs[j % 3] >= 0 && xs[j % 3] < W && ys[j / 3] >= 0 && ys[j / 3] < H;
+        const uint64_t k = jfa_key(inb ? cand[j] : kFarSeed, x, y);
+        best = k < best ? k : best;
+    }
+    out[(size_t)y * W + x] = (uint32_t)best;
 }
 
 // Fused passes S, S/2, ..., 1 over a 32x32 output tile.  The tile is loaded
-// with a halo of 2S-1 pixels; each pass shrinks the valid region by its step.
-// Cells outside the image stay kSeedNone in every pass, which is exactly the
-// global kernel's "out-of-bounds neighbours are skipped".
+// with a halo of 2S-1 pixels (R <= 62 cells a side); each pass shrinks the
+// valid region by its step.  Threads are laid out 64 x 4: lane = column, so
+// every LDS access of a wave is one row of consecutive words.  Cells outside
+// the image stay the far seed in every pass, which is exactly the global
+// kernel's "out-of-bounds neighbours are skipped".
 template <int S>
 __global__ __launch_bounds__(kBlock) void k_jfa_tail(const uint32_t *__restrict__ in,
                                                      uint32_t *__restrict__ seeds_out, float *__restrict__ sdf,
@@ -105,14 +121,30 @@ __global__ __launch_bounds__(kBlock) void k_jfa_tail(const uint32_t *__restrict_
     constexpr int T = 32;
     constexpr int HALO = S > 0 ? 2 * S - 1 : 0;
     constexpr int R = T + 2 * HALO;
+    static_assert(R <= 64, "one column per lane");
     __shared__ uint32_t buf[2][R * R];
 
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int x0 = (b % bx_count) * T - HALO;
     const int y0 = (b / bx_count) * T - HALO;
-    for (int i = threadIdx.x; i < R * R; i += kBlock) {
-        const int gx = x0 + i % R, gy = y0 + i / R;
-        buf[0][i] = (gx >= 0 && gy >= 0 && gx < W && gy < H) ? in[(size_t)gy * W + gx] : kSeedNone;
+    const int cx = threadIdx.x & 63, cy = threadIdx.x >> 6;
+    {
+        // all rows of this lane's column loaded (clamped, unconditional) before
+        // any LDS write, so the global loads overlap
+        constexpr int ROWS = (R + 3) / 4;
+        const int gx = x0 + cx, qx = min(max(gx, 0), W - 1);
+        uint32_t v[ROWS];
+#pragma unroll
+        for (int j = 0; j < ROWS; ++j) {
+            const int gy = y0 + cy + 4 * j, qy = min(max(gy, 0), H - 1);
+            v[j] = in[(size_t)qy * W + qx];
+        }
+#pragma unroll
+        for (int j = 0; j < ROWS; ++j) {
+            const int ry = cy + 4 * j, gy = y0 + ry;
+            if (cx < R && ry < R)
+                buf[0][ry * R + cx] = (gx >= 0 && gy >= 0 && gx < W && gy < H) ? v[j] : kFarSeed;
+        }
     }
     __syncthreads();
 
@@ -121,40 +153,44 @@ __global__ __launch_bounds__(kBlock) void k_jfa_tail(const uint32_t *__restrict_
 #pragma unroll
     for (int s = S; s >= 1; s >>= 1) {
         off += s;
-        const int n = R - 2 * off;
-        for (int i = threadIdx.x; i < n * n; i += kBlock) {
-            const int rx = off + i % n, ry = off + i / n;
-            const int gx = x0 + rx, gy = y0 + ry;
-            uint32_t res = kSeedNone;
-            if (gx >= 0 && gy >= 0 && gx < W && gy < H) {
-                uint64_t best = jfa_key(buf[cur][ry * R + rx], gx, gy);
+        const int rx = off + cx;
+        if (rx < R - off) {
+            const int gx = x0 + rx;
+            const bool xin = gx >= 0 && gx < W;
+            for (int ry = off + cy; ry < R - off; ry += 4) {
+                const int gy = y0 + ry;
+                const uint32_t *src = buf[cur];
+                uint32_t res = kFarSeed;
+                if (xin && gy >= 0 && gy < H) {
+                    uint64_t best = jfa_key(src[ry * R + rx], gx, gy);
 #pragma unroll
-                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dy = -1; dy <= 1; ++dy)
 #pragma unroll
-                    for (int dx = -1; dx <= 1; ++dx) {
-                        if (dx == 0 && dy == 0) continue;
-                        const uint64_t k = jfa_key(buf[cur][(ry + dy * s) * R + rx + dx * s], gx, gy);
-                        best = k < best ? k : best;
-                    }
-                res = best == ~0ull ? kSeedNone : (uint32_t)best;
+                        for (int dx = -1; dx <= 1; ++dx) {
+                            if (dx == 0 && dy == 0) continue;
+                            const uint64_t k = jfa_key(src[(ry + dy * s) * R + rx + dx * s], gx, gy);
+                            best = k < best ? k : best;
+                        }
+                    res = (uint32_t)best;
+                }
+                buf[cur ^ 1][ry * R + rx] = res;
             }
-            buf[cur ^ 1][ry * R + rx] = res;
         }
         __syncthreads();
         cur ^= 1;
     }
 
-    for (int i = threadIdx.x; i < T * T; i += kBlock) {
-        const int gx = x0 + HALO + i % T, gy = y0 + HALO + i / T;
-        if (gx >= W || gy >= H) continue;
-        const uint32_t sd = buf[cur][(HALO + i / T) * R + HALO + i % T];
-        float d = __builtin_huge_valf();
-        if (sd != kSeedNone) {
+    if (cx < T) {
+        const int gx = x0 + HALO + cx;
+        for (int ty = cy; ty < T; ty += 4) {
+            const int gy = y0 + HALO + ty;
+            if (gx >= W || gy >= H) continue;
+            const uint32_t sd = buf[cur][(HALO + ty) * R + HALO + cx];
             const int dx = (int)(sd & 0xFFFFu) - gx, dy = (int)(sd >> 16) - gy;
-            d = (float)sqrt((double)(uint32_t)(dx * dx + dy * dy));
+            const float d = sd == kFarSeed ? __builtin_huge_valf() : (float)sqrt((double)(uint32_t)(dx * dx + dy * dy));
+            sdf[(size_t)gy * W + gx] = d;
+            if (seeds_out) seeds_out[(size_t)gy * W + gx] = sd == kFarSeed ? kSeedNone : sd;
         }
-        sdf[(size_t)gy * W + gx] = d;
-        if (seeds_out) seeds_out[(size_t)gy * W + gx] = sd;
     }
 }
 

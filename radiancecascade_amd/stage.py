@@ -69,13 +69,14 @@ class RadianceCascadeStage:
     """The GI pass: jump flood -> per-level march/merge -> gather."""
 
     def __init__(self, width: int, height: int, levels: int = 0, storage: str = "f32",
-                 device: int = 0, profile: bool = False, debug: bool = False):
+                 device: int = 0, profile: bool = False, debug: bool = False, graph: bool = True):
         lib = _abi.load()
         if storage not in _abi.STORAGE:
             raise ValueError(f"storage must be one of {sorted(_abi.STORAGE)}")
         cfg = _abi.Config(version=_abi.ABI_VERSION, width=width, height=height, levels=levels,
                           storage=_abi.STORAGE[storage],
-                          flags=(_abi.FLAG_PROFILE if profile else 0) | (_abi.FLAG_KEEP_DEBUG if debug else 0),
+                          flags=(_abi.FLAG_PROFILE if profile else 0) | (_abi.FLAG_KEEP_DEBUG if debug else 0)
+                          | (0 if graph else _abi.FLAG_NO_GRAPH),
                           device=device, reserved=0)
         ctx = ctypes.c_void_p()
         check(lib.rc_create(ctypes.byref(cfg), ctypes.byref(ctx)))
@@ -111,6 +112,27 @@ class RadianceCascadeStage:
                          emissive=_dev_ptr(emissive, 3 * n, "emissive"))
         check(self._lib.rc_run_device(self._ctx, ctypes.byref(g), _dev_ptr(irradiance, 3 * n, "irradiance"),
                                       _stream_handle(stream)), self._ctx)
+
+    # -- level-kernel tiling ---------------------------------------------------
+    def set_tiling(self, level: int, shape: int, rays_per_lane: int) -> None:
+        check(self._lib.rc_set_level_tiling(self._ctx, level, shape, rays_per_lane), self._ctx)
+
+    def tiling(self) -> dict:
+        """{level: ((TX, TY, KD), rays_per_lane)} for levels >= 1."""
+        out = {}
+        for i in range(1, self.levels):
+            sh, r = ctypes.c_uint32(), ctypes.c_uint32()
+            check(self._lib.rc_get_level_tiling(self._ctx, i, ctypes.byref(sh), ctypes.byref(r)), self._ctx)
+            out[i] = (_abi.TILE_SHAPES[sh.value], r.value)
+        return out
+
+    def autotune(self, depth, emissive, irradiance, iters: int = 5, stream=None) -> dict:
+        n = self.width * self.height * 4
+        g = _abi.GBuffer(depth=_dev_ptr(depth, n, "depth"), albedo=None,
+                         emissive=_dev_ptr(emissive, 3 * n, "emissive"))
+        check(self._lib.rc_autotune(self._ctx, ctypes.byref(g), _dev_ptr(irradiance, 3 * n, "irradiance"),
+                                    iters, _stream_handle(stream)), self._ctx)
+        return self.tiling()
 
     # -- introspection -------------------------------------------------------
     def _sync_config(self) -> None:

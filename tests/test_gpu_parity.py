@@ -15,10 +15,12 @@ torch = pytest.importorskip("torch")
 
 pytestmark = pytest.mark.gpu
 
-# rel-L2 of the fp32 HIP irradiance / merged levels against the FP64 oracle.
-# The only source of difference is fp32 ray positions flipping a hit at a
-# pixel edge (measured ~6e-5 at 1080p on the CPU FP32 flavour).
-REL_L2_TOL_F32 = 5e-4
+# rel-L2 of the fp32 HIP output against the FP64 oracle.  The only source of
+# difference is fp32 ray positions flipping a hit at a pixel edge: measured
+# on the CPU FP32 flavour (which the kernels match bit for bit) at up to
+# 3.7e-4 for irradiance and 1.7e-3 for a single merged level at 1080p.
+REL_L2_TOL_F32 = 1e-3
+REL_L2_TOL_F32_LEVEL = 5e-3
 # fraction of march hit codes that may differ from the FP64 oracle's
 HIT_MISMATCH_TOL_F64 = 1e-3
 
@@ -138,7 +140,7 @@ def test_pass_parity(rc, oracle, W, H, L, seed, nb, ne):
         e64 = rel_l2(g["levels"][i][..., :3], lv64[i][..., :3])
         _say(f"  level {i}: bit-exact vs fp32 oracle {exact}, rel-L2 vs fp64 {e64:.3e}")
         assert exact, f"level {i} differs from the fp32 oracle (max {np.abs(g['levels'][i] - lv32[i]).max()})"
-        assert e64 <= REL_L2_TOL_F32
+        assert e64 <= REL_L2_TOL_F32_LEVEL
     assert np.array_equal(g["irr"].view(np.uint32), irr32.view(np.uint32)), \
         f"irradiance differs from fp32 oracle (max {np.abs(g['irr'] - irr32).max()})"
     e64 = rel_l2(g["irr"], irr64)
@@ -179,3 +181,64 @@ def test_resize(rc, oracle):
             st.update(d, e)
             irr = st.render()
             assert np.array_equal(irr.view(np.uint32), oracle.run(d, e, L, np.float32).view(np.uint32))
+
+
+def _device_gbuffer(rc, W, H, seed, nb, ne):
+    dev = torch.device("cuda:0")
+    depth = torch.empty((H, W), dtype=torch.float32, device=dev)
+    emis = torch.empty((3, H, W), dtype=torch.float32, device=dev)
+    rc.gbuffer_raster(rc.scene_generate(seed, nb, ne), W, H, depth, None, emis)
+    return depth, emis
+
+
+def test_every_tiling_is_bit_exact(rc, oracle):
+    """Every (tile shape, rays per lane) of every level computes the same
+    frame: the tiling only changes which lane marches which ray."""
+    W, H, L, seed = 640, 360, 6, 1
+    _say(f"tilings W={W} H={H} L={L}")
+    depth, emis = _device_gbuffer(rc, W, H, seed, 32, 4)
+    d, _, e = oracle.gbuffer(seed, W, H, 32, 4)
+    ref = oracle.run(d, e, L, np.float32)
+    irr = torch.empty((3, H, W), dtype=torch.float32, device="cuda:0")
+    with rc.RadianceCascadeStage(W, H, levels=L) as st:
+        default = st.tiling()
+        n = 0
+        for i in range(1, L):
+            for shape, (tx, ty, kd) in rc._abi.TILE_SHAPES.items():
+                if kd > 4 * 4 ** i:
+                    with pytest.raises(rc.RcError):
+                        st.set_tiling(i, shape, 1)
+                    continue
+                for rpt in (1, 2, 4):
+                    st.set_tiling(i, shape, rpt)
+                    st.run_device(depth, emis, irr)
+                    torch.cuda.synchronize()
+                    got = irr.cpu().numpy()
+                    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), (i, shape, rpt)
+                    n += 1
+            shape0 = [k for k, v in rc._abi.TILE_SHAPES.items() if v == default[i][0]][0]
+            st.set_tiling(i, shape0, default[i][1])
+        assert n >= 40
+        chosen = st.autotune(depth, emis, irr, iters=2)
+        assert set(chosen) == set(range(1, L))
+        st.run_device(depth, emis, irr)
+        torch.cuda.synchronize()
+        assert np.array_equal(irr.cpu().numpy().view(np.uint32), ref.view(np.uint32))
+
+
+def test_graph_replay_matches_direct(rc, oracle):
+    """The captured-graph frame equals the launch-by-launch frame, replay after
+    replay, with eager work and synchronisation in between."""
+    W, H, L = 480, 270, 6
+    depth, emis = _device_gbuffer(rc, W, H, 3, 32, 4)
+    a = torch.empty((3, H, W), dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    with rc.RadianceCascadeStage(W, H, levels=L, graph=False) as direct, \
+            rc.RadianceCascadeStage(W, H, levels=L, graph=True) as graphed:
+        direct.run_device(depth, emis, a)
+        torch.cuda.synchronize()
+        for rep in range(4):
+            b.fill_(-1.0)
+            graphed.run_device(depth, emis, b)
+            torch.cuda.synchronize()
+            assert torch.equal(a, b), f"replay {rep}"

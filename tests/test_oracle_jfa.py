@@ -1,0 +1,128 @@
+"""Pins for the oracle's occupancy rule, jump flood and distance field.
+
+The brute-force nearest seed is checked against scipy's exact Euclidean
+distance transform; the jump flood is exact for one seed and never nearer
+than the exact answer otherwise (Rong & Tan 2006 error behaviour).
+"""
+import numpy as np
+import pytest
+
+from scipy import ndimage
+
+
+def _seed_field(W, H, pts):
+    s = np.full((H, W), 0xFFFFFFFF, np.uint32)
+    for x, y in pts:
+        s[y, x] = (y << 16) | x
+    return s
+
+
+def _d2(seeds):
+    H, W = seeds.shape
+    yy, xx = np.mgrid[0:H, 0:W]
+    sx = (seeds & 0xFFFF).astype(np.int64)
+    sy = (seeds >> 16).astype(np.int64)
+    d2 = (sx - xx) ** 2 + (sy - yy) ** 2
+    return np.where(seeds == 0xFFFFFFFF, -1, d2)
+
+
+def test_pass_count(oracle):
+    assert [oracle.jfa_num_passes(w, w) for w in (1, 2, 3, 4, 5, 256, 257, 1920)] == [0, 1, 2, 2, 3, 8, 9, 11]
+    assert oracle.jfa_num_passes(1920, 1080) == 11
+
+
+def test_occupancy_rule(oracle):
+    W, H = 8, 6
+    depth = np.ones((H, W), np.float32)
+    depth[:, 4:] = 1.06  # right half 6 % farther: column 3 is the near side
+    depth[0, :] = np.float32(1.0)
+    depth[0, 4:] = 1.04  # row 0: only 4 % farther -> not a silhouette
+    emis = np.zeros((3, H, W), np.float32)
+    emis[2, 5, 0] = 0.5  # blue-only emitter still occupies
+    s = oracle.seed_init(depth, emis)
+    occ = s != oracle.SEED_NONE
+    expect = np.zeros((H, W), bool)
+    expect[1:, 3] = True
+    expect[5, 0] = True
+    assert np.array_equal(occ, expect)
+    y, x = 5, 0
+    assert s[y, x] == (y << 16) | x
+
+
+def test_single_seed_exact(oracle):
+    for (W, H, x, y) in [(64, 64, 10, 50), (37, 5, 36, 0), (1, 1, 0, 0), (100, 3, 0, 2)]:
+        s = oracle.jfa(_seed_field(W, H, [(x, y)]))
+        assert np.all(s == ((y << 16) | x))
+        sdf = oracle.sdf_f32(s)
+        yy, xx = np.mgrid[0:H, 0:W]
+        assert np.allclose(sdf, np.sqrt((xx - x) ** 2 + (yy - y) ** 2), rtol=1e-7)
+
+
+def test_empty_field(oracle):
+    s = oracle.jfa(_seed_field(33, 17, []))
+    assert np.all(s == oracle.SEED_NONE)
+    assert np.all(np.isinf(oracle.sdf_f32(s)))
+
+
+def test_bruteforce_matches_scipy_edt(oracle):
+    rng = np.random.default_rng(1)
+    for (W, H, n) in [(48, 40, 5), (64, 64, 60), (31, 57, 300)]:
+        pts = {(int(rng.integers(W)), int(rng.integers(H))) for _ in range(n)}
+        init = _seed_field(W, H, pts)
+        near = oracle.nearest_seed_bruteforce(init)
+        edt = ndimage.distance_transform_edt(init == 0xFFFFFFFF)
+        assert np.allclose(np.sqrt(_d2(near)), edt, rtol=0, atol=1e-12)
+
+
+def test_tie_break(oracle):
+    # pixel (5,5) is equidistant from (3,5) and (7,5) and from (5,3),(5,7):
+    # (d2, y, x) lexicographic picks (5,3) (smallest y)
+    init = _seed_field(11, 11, [(3, 5), (7, 5), (5, 3), (5, 7)])
+    near = oracle.nearest_seed_bruteforce(init)
+    assert near[5, 5] == (3 << 16) | 5
+    assert oracle.jfa(init)[5, 5] == (3 << 16) | 5
+
+
+def test_jfa_vs_exact(oracle):
+    rng = np.random.default_rng(7)
+    for (W, H, n) in [(64, 64, 3), (64, 64, 40), (50, 30, 200)]:
+        pts = {(int(rng.integers(W)), int(rng.integers(H))) for _ in range(n)}
+        init = _seed_field(W, H, pts)
+        jfa = oracle.jfa(init)
+        exact = oracle.nearest_seed_bruteforce(init)
+        dj, de = _d2(jfa), _d2(exact)
+        assert np.all(dj >= de)                      # never nearer than exact
+        assert np.all(np.isin(jfa, list(init[init != 0xFFFFFFFF])))  # only real seeds
+        assert (dj != de).mean() < 0.01              # JFA's rare misses
+        for x, y in pts:                             # seeds keep themselves
+            assert jfa[y, x] == (y << 16) | x
+
+
+def test_pass_is_order_independent(oracle):
+    """The min-key rule makes a pass independent of neighbour order: the
+    result equals the brute-force minimum over the 9 candidates."""
+    rng = np.random.default_rng(3)
+    W, H = 23, 19
+    pts = {(int(rng.integers(W)), int(rng.integers(H))) for _ in range(30)}
+    s = oracle.jfa_pass(_seed_field(W, H, pts), 4)
+    s = oracle.jfa_pass(s, 2)
+    out = oracle.jfa_pass(s, 1)
+    for y in range(H):
+        for x in range(W):
+            cands = []
+            for dy in (-1, 0, 1):
+                for dx in (-1, 0, 1):
+                    qx, qy = x + dx, y + dy
+                    if 0 <= qx < W and 0 <= qy < H and s[qy, qx] != 0xFFFFFFFF:
+                        v = int(s[qy, qx])
+                        cands.append(((v & 0xFFFF) - x) ** 2 + ((v >> 16) - y) ** 2 << 32 | v)
+            assert out[y, x] == (min(cands) & 0xFFFFFFFF if cands else 0xFFFFFFFF)
+
+
+def test_sdf_values(oracle):
+    init = _seed_field(9, 9, [(0, 0)])
+    s = oracle.jfa(init)
+    f32 = oracle.sdf_f32(s)
+    f64 = oracle.sdf_f64(s)
+    assert f32[0, 0] == 0 and f32[3, 4] == np.float32(5.0) and f64[8, 8] == pytest.approx(8 * np.sqrt(2), rel=1e-15)
+    assert f32.dtype == np.float32 and f64.dtype == np.float64
