@@ -127,6 +127,11 @@ void rc_destroy(rc_ctx *ctx);
 rc_status rc_set_level_tiling(rc_ctx *ctx, uint32_t level, uint32_t shape, uint32_t rays_per_lane);
 rc_status rc_get_level_tiling(const rc_ctx *ctx, uint32_t level, uint32_t *shape, uint32_t *rays_per_lane);
 rc_status rc_autotune(rc_ctx *ctx, const rc_gbuffer *dev, float *irradiance_dev, uint32_t iters, void *stream);
+/* Jump-flood passes with step <= fused_step (0, 1, 2, 4 or 8) run fused in
+ * one LDS-tiled kernel; 0 runs every pass as a global kernel.  The result is
+ * identical; rc_autotune also picks this. */
+rc_status rc_set_jfa_fuse(rc_ctx *ctx, uint32_t fused_step);
+uint32_t rc_get_jfa_fuse(const rc_ctx *ctx);
 
 /* ---- introspection -------------------------------------------------------- */
 rc_status rc_get_config(const rc_ctx *ctx, rc_config *out);

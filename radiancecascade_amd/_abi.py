@@ -70,6 +70,8 @@ SIGNATURES = {
     "rc_set_level_tiling": (_i32, [_P, _u32, _u32, _u32]),
     "rc_get_level_tiling": (_i32, [_P, _u32, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
     "rc_autotune": (_i32, [_P, ctypes.POINTER(GBuffer), _P, _u32, _P]),
+    "rc_set_jfa_fuse": (_i32, [_P, _u32]),
+    "rc_get_jfa_fuse": (_u32, [_P]),
     "rc_get_config": (_i32, [_P, ctypes.POINTER(Config)]),
     "rc_get_stats": (_i32, [_P, ctypes.POINTER(Stats)]),
     "rc_reset_stats": (_i32, [_P]),

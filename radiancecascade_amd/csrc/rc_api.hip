@@ -31,6 +31,7 @@ struct rc_ctx {
     size_t dir_off[RC_MAX_LEVELS] = {};
     int tile[RC_MAX_LEVELS] = {};
     int rpt[RC_MAX_LEVELS] = {};
+    int jfa_fuse = 0;  // largest jump-flood step fused into the LDS tail (0 = none)
     int dbg_seed_buf = -1;
     bool ran = false;
     // host-staging path (rc_update_host / rc_render_host)
@@ -123,7 +124,7 @@ static rc_status allocate(rc_ctx *c, uint32_t W, uint32_t H, uint32_t L)
     const size_t n = (size_t)W * H;
     HIP_TRY(c, hipMalloc(&c->seed[0], n * sizeof(uint32_t)));
     HIP_TRY(c, hipMalloc(&c->seed[1], n * sizeof(uint32_t)));
-    HIP_TRY(c, hipMalloc(&c->sdf, n * sizeof(float)));
+    HIP_TRY(c, hipMalloc(&c->sdf, sdf_elems((int)W, (int)H) * sizeof(float)));
     const size_t tb = texel_bytes(c->storage);
     for (int i = 1; i < (int)L; ++i) {
         const size_t texels = (size_t)grid_dim(W, i) * grid_dim(H, i) * (num_dirs(i) / 4);
@@ -190,9 +191,9 @@ struct StageTimer {
 };
 
 static rc_status run_frame(rc_ctx *c, const float *depth, const float *emissive, float *irr, hipStream_t s,
-                           bool capturing = false)
+                           bool quiet = false)
 {
-    const bool prof = !capturing && (c->cfg.flags & RC_FLAG_PROFILE) != 0;
+    const bool prof = !quiet && (c->cfg.flags & RC_FLAG_PROFILE) != 0;
     const bool dbg = (c->cfg.flags & RC_FLAG_KEEP_DEBUG) != 0;
     StageTimer tm{c, s, prof};
     for (auto &u : c->ev_used) u = false;
@@ -201,20 +202,29 @@ static rc_status run_frame(rc_ctx *c, const float *depth, const float *emissive,
     HIP_TRY(c, launch_seed_init(depth, emissive, c->seed[0], c->W, c->H, s));
     tm.end(RC_STAGE_SEED);
 
+    // Jump flood: global passes down to the fused LDS tail (if any); without a
+    // tail the last global pass writes the SDF.  A 1x1 image has no pass.
     const int P = jfa_num_passes(c->W, c->H);
-    const int tail = jfa_tail_step(c->W, c->H);
+    const int tail = jfa_tail_step(c->W, c->H, c->jfa_fuse);
     int cur = 0, global_passes = 0;
+    c->dbg_seed_buf = -1;
     tm.begin(RC_STAGE_JFA);
-    for (int step = P > 0 ? 1 << (P - 1) : 0; step > tail; step >>= 1) {
-        HIP_TRY(c, launch_jfa_pass(c->seed[cur], c->seed[cur ^ 1], c->W, c->H, step, s));
-        cur ^= 1;
+    for (int step = P > 0 ? 1 << (P - 1) : 0; step >= 1 && step > tail; step >>= 1) {
+        const bool final_pass = tail == 0 && step == 1;
+        uint32_t *dst = final_pass ? (dbg ? c->seed[cur ^ 1] : nullptr) : c->seed[cur ^ 1];
+        HIP_TRY(c, launch_jfa_pass(c->seed[cur], dst, c->sdf, c->W, c->H, step, final_pass, s));
+        if (final_pass) c->dbg_seed_buf = dbg ? (cur ^ 1) : -1;
+        else cur ^= 1;
         ++global_passes;
     }
     tm.end(RC_STAGE_JFA);
-    tm.begin(RC_STAGE_JFA_TAIL);
-    HIP_TRY(c, launch_jfa_tail(c->seed[cur], dbg ? c->seed[cur ^ 1] : nullptr, c->sdf, c->W, c->H, s));
-    tm.end(RC_STAGE_JFA_TAIL);
-    c->dbg_seed_buf = dbg ? (cur ^ 1) : -1;
+    const bool tail_kernel = tail > 0 || P == 0;
+    if (tail_kernel) {
+        tm.begin(RC_STAGE_JFA_TAIL);
+        HIP_TRY(c, launch_jfa_tail(c->seed[cur], dbg ? c->seed[cur ^ 1] : nullptr, c->sdf, c->W, c->H, tail, s));
+        tm.end(RC_STAGE_JFA_TAIL);
+        c->dbg_seed_buf = dbg ? (cur ^ 1) : -1;
+    }
 
     for (int i = c->L - 1; i >= 1; --i) {
         tm.begin(RC_STAGE_LEVEL0 + i);
@@ -231,11 +241,13 @@ static rc_status run_frame(rc_ctx *c, const float *depth, const float *emissive,
             if (!c->ev_used[st]) continue;
             float ms = 0.f;
             HIP_TRY(c, hipEventElapsedTime(&ms, c->ev[2 * st], c->ev[2 * st + 1]));
+            const uint32_t n = st == RC_STAGE_JFA ? (uint32_t)global_passes : 1u;
+            if (n == 0) continue;
             c->stats.stage_ms[st] += ms;
-            c->stats.stage_launches[st] += st == RC_STAGE_JFA ? (uint32_t)global_passes : 1u;
+            c->stats.stage_launches[st] += n;
         }
     }
-    if (!capturing) {
+    if (!quiet) {
         c->stats.runs += 1;
         c->ran = true;
     }
@@ -447,7 +459,7 @@ rc_status rc_debug_sdf(rc_ctx *c, float *dev_out, void *stream)
 {
     if (!c || !dev_out) return fail(c, RC_ERR_NULL, "rc_debug_sdf: NULL argument");
     if (!c->ran) return fail(c, RC_ERR_STATE, "rc_debug_sdf: no run yet");
-    HIP_TRY(c, hipMemcpyAsync(dev_out, c->sdf, (size_t)c->W * c->H * 4, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    HIP_TRY(c, launch_sdf_untile(c->sdf, dev_out, c->W, c->H, (hipStream_t)stream));
     return RC_OK;
 }
 
@@ -494,6 +506,18 @@ rc_status rc_set_level_tiling(rc_ctx *c, uint32_t level, uint32_t shape, uint32_
     return RC_OK;
 }
 
+rc_status rc_set_jfa_fuse(rc_ctx *c, uint32_t fused_step)
+{
+    if (!c) return fail(nullptr, RC_ERR_NULL, "rc_set_jfa_fuse: ctx is NULL");
+    if (fused_step != 0 && fused_step != 1 && fused_step != 2 && fused_step != 4 && fused_step != 8)
+        return fail(c, RC_ERR_INVALID, "rc_set_jfa_fuse: fused_step must be 0, 1, 2, 4 or 8");
+    c->jfa_fuse = (int)fused_step;
+    drop_graph(c);
+    return RC_OK;
+}
+
+uint32_t rc_get_jfa_fuse(const rc_ctx *c) { return c ? (uint32_t)c->jfa_fuse : 0u; }
+
 rc_status rc_get_level_tiling(const rc_ctx *c, uint32_t level, uint32_t *shape, uint32_t *rpt)
 {
     if (!c || !shape || !rpt) return fail(nullptr, RC_ERR_NULL, "rc_get_level_tiling: NULL argument");
@@ -519,6 +543,25 @@ rc_status rc_autotune(rc_ctx *c, const rc_gbuffer *g, float *irr, uint32_t iters
     c->cfg.flags = saved_flags;
     if (st != RC_OK) return st;
     hipEvent_t e0 = c->ev[0], e1 = c->ev[1];
+    {
+        float best = 1e30f;
+        int best_fuse = c->jfa_fuse;
+        const int saved = c->jfa_fuse;
+        for (int fuse : {0, 2, 4, 8}) {
+            c->jfa_fuse = fuse;
+            HIP_TRY(c, hipEventRecord(e0, s));
+            for (uint32_t it = 0; it < iters; ++it) {
+                rc_status r = run_frame(c, g->depth, g->emissive, irr, s, true);
+                if (r != RC_OK) { c->jfa_fuse = saved; return r; }
+            }
+            HIP_TRY(c, hipEventRecord(e1, s));
+            HIP_TRY(c, hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIP_TRY(c, hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) { best = ms; best_fuse = fuse; }
+        }
+        c->jfa_fuse = best_fuse;
+    }
     for (int i = 1; i < c->L; ++i) {
         float best = 1e30f;
         int best_shape = c->tile[i], best_rpt = c->rpt[i];

@@ -41,43 +41,77 @@ __device__ inline float quad_sum(float v)
     return v;
 }
 
-struct MarchResult {
-    float r, g, b;
-    bool miss;
-    int hit;  // pixel index, -1 off-image, -2 interval end
-};
+// The distance field is read through a buffer resource: the tiled index is a
+// 32-bit lane offset (no 64-bit address arithmetic per step), and a lane whose
+// sample is off-image may compute any offset -- the load is range-checked and
+// its value unused.
+constexpr int kBufferDword3 = 0x00020000;  // gfx9-family raw buffer, 32-bit data
 
-// LOAD_E = false: hit code only, the emissive planes are never touched.
-template <bool LOAD_E = true>
-__device__ inline MarchResult march_ray(const float *__restrict__ sdf, const float *__restrict__ emissive, int W,
-                                        int H, float cx, float cy, float dx, float dy, float t0, float t1)
+__device__ inline __amdgpu_buffer_rsrc_t sdf_rsrc(const float *sdf, int W, int H)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(sdf), (short)0, (int)(sdf_elems(W, H) * 4),
+                                             kBufferDword3);
+}
+
+__device__ inline float sdf_load(__amdgpu_buffer_rsrc_t r, int x, int y, int tiles_x)
+{
+    const uint32_t ux = (uint32_t)x, uy = (uint32_t)y;
+    const uint32_t idx = ((__umul24(uy >> 2, (uint32_t)tiles_x) + (ux >> 3)) << 5) + ((uy & 3u) << 3) + (ux & 7u);
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)(idx * 4u), 0, 0));
+}
+
+// Ray codes: >= 0 hit pixel y*W+x, -1 left the image, -2 reached t1.
+constexpr int kRayExit = -1, kRayT1 = -2;
+
+// RPT rays per lane marched in lockstep: each iteration issues the RPT
+// distance-field loads back to back, so a lane keeps RPT independent load
+// chains in flight (the march is a chain of dependent gathers and is bound by
+// their latency, not by bandwidth).  A finished ray's slot keeps issuing a
+// (cached, ignored) load until every ray of the lane is done.
+template <int RPT>
+__device__ inline void march_rays(__amdgpu_buffer_rsrc_t sdf, int W, int H, const float *cx, const float *cy,
+                                  float dx, float dy, float t0, float t1, const bool *active, int *code)
 {
     const float fW = (float)W, fH = (float)H;
-    MarchResult m{0.f, 0.f, 0.f, true, -2};
-    float t = t0;
-    while (t < t1) {
-        const float x = floorf(fmaf(t, dx, cx));
-        const float y = floorf(fmaf(t, dy, cy));
-        if (!(x >= 0.f && y >= 0.f && x < fW && y < fH)) {
-            m.hit = -1;
-            return m;
-        }
-        const int q = (int)y * W + (int)x;
-        const float d = sdf[q];
-        if (d < 0.5f) {
-            if (LOAD_E) {
-                const size_t plane = (size_t)W * H;
-                m.r = emissive[q];
-                m.g = emissive[plane + q];
-                m.b = emissive[2 * plane + q];
-            }
-            m.miss = false;
-            m.hit = q;
-            return m;
-        }
-        t = t + fmaxf(d, 1.0f);
+    const int tiles_x = sdf_tiles_x(W);
+    float t[RPT];
+    bool alive[RPT];
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+        t[j] = t0;
+        alive[j] = active[j] && t0 < t1;
+        code[j] = kRayT1;
     }
-    return m;
+    for (;;) {
+        int ix[RPT], iy[RPT];
+        float d[RPT];
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            const float x = floorf(fmaf(t[j], dx, cx[j]));
+            const float y = floorf(fmaf(t[j], dy, cy[j]));
+            const bool inside = x >= 0.f && y >= 0.f && x < fW && y < fH;
+            if (alive[j] && !inside) code[j] = kRayExit;
+            alive[j] = alive[j] && inside;
+            ix[j] = (int)x;
+            iy[j] = (int)y;
+            d[j] = sdf_load(sdf, ix[j], iy[j], tiles_x);
+        }
+        bool any = false;
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            if (alive[j]) {
+                if (d[j] < 0.5f) {
+                    code[j] = iy[j] * W + ix[j];
+                    alive[j] = false;
+                } else {
+                    t[j] = t[j] + (d[j] > 1.0f ? d[j] : 1.0f);
+                    alive[j] = t[j] < t1;
+                }
+            }
+            any = any || alive[j];
+        }
+        if (!any) break;
+    }
 }
 
 // Bilinear of four staged texels with the lower probe's parity weights.
@@ -90,55 +124,6 @@ __device__ inline float3 bilerp4(float4 u00, float4 u10, float4 u01, float4 u11,
     r.y = fmaf(wy1, fmaf(wx1, u11.y, wx0 * u01.y), wy0 * fmaf(wx1, u10.y, wx0 * u00.y));
     r.z = fmaf(wy1, fmaf(wx1, u11.z, wx0 * u01.z), wy0 * fmaf(wx1, u10.z, wx0 * u00.z));
     return r;
-}
-
-// RPT rays per lane marched in lockstep: each iteration issues the RPT
-// distance-field loads back to back, so a lane keeps RPT independent load
-// chains in flight (the march is a chain of dependent gathers and is bound by
-// their latency, not by bandwidth).  A finished ray's slot keeps loading the
-// (cached) first texel until every ray of the lane is done.
-template <int RPT>
-__device__ inline void march_rays(const float *__restrict__ sdf, int W, int H, const float *cx, const float *cy,
-                                  float dx, float dy, float t0, float t1, const bool *active, int *hitq)
-{
-    const float fW = (float)W, fH = (float)H;
-    float t[RPT];
-    bool alive[RPT];
-#pragma unroll
-    for (int j = 0; j < RPT; ++j) {
-        t[j] = t0;
-        alive[j] = active[j] && t0 < t1;
-        hitq[j] = -1;
-    }
-    for (;;) {
-        int q[RPT];
-#pragma unroll
-        for (int j = 0; j < RPT; ++j) {
-            const float x = floorf(fmaf(t[j], dx, cx[j]));
-            const float y = floorf(fmaf(t[j], dy, cy[j]));
-            const bool inside = x >= 0.f && y >= 0.f && x < fW && y < fH;
-            alive[j] = alive[j] && inside;
-            q[j] = alive[j] ? (int)y * W + (int)x : 0;
-        }
-        float d[RPT];
-#pragma unroll
-        for (int j = 0; j < RPT; ++j) d[j] = sdf[q[j]];
-        bool any = false;
-#pragma unroll
-        for (int j = 0; j < RPT; ++j) {
-            if (alive[j]) {
-                if (d[j] < 0.5f) {
-                    hitq[j] = q[j];
-                    alive[j] = false;
-                } else {
-                    t[j] = t[j] + fmaxf(d[j], 1.0f);
-                    alive[j] = t[j] < t1;
-                }
-            }
-            any = any || alive[j];
-        }
-        if (!any) break;
-    }
 }
 
 // Level i >= 1.  Workgroup tile: TX x (TY*RPT) probes x KD directions; lane
@@ -183,7 +168,7 @@ __global__ __launch_bounds__(kBlock) void k_level(LevelArgs a)
         cy[j] = ((float)py + 0.5f) * a.spacing;
         active[j] = px < a.PX && py < a.PY;
     }
-    march_rays<RPT>(a.sdf, a.W, a.H, cx, cy, dir.x, dir.y, a.t0, a.t1, active, hitq);
+    march_rays<RPT>(sdf_rsrc(a.sdf, a.W, a.H), a.W, a.H, cx, cy, dir.x, dir.y, a.t0, a.t1, active, hitq);
 
     const size_t plane = (size_t)a.W * a.H;
 #pragma unroll
@@ -248,6 +233,8 @@ __global__ __launch_bounds__(kBlock) void k_level0(LevelArgs a, float *__restric
         __syncthreads();
     }
 
+    const __amdgpu_buffer_rsrc_t sdf = sdf_rsrc(a.sdf, a.W, a.H);
+    const size_t plane = (size_t)a.W * a.H;
 #pragma unroll 1
     for (int round = 0; round < ROUNDS; ++round) {
         const int ray = round * kBlock + threadIdx.x;
@@ -255,15 +242,19 @@ __global__ __launch_bounds__(kBlock) void k_level0(LevelArgs a, float *__restric
         const int px = px0 - 1 + slot % NS, py = py0 - 1 + slot / NS;
         const bool active = slot < NS * NS && px >= 0 && py >= 0 && px < a.PX && py < a.PY;
         float r = 0.f, g = 0.f, b = 0.f, al = 0.f;
+        const float2 dir = a.dirs[k];
+        const float cx = ((float)px + 0.5f) * 2.0f, cy = ((float)py + 0.5f) * 2.0f;
+        int code;
+        march_rays<1>(sdf, a.W, a.H, &cx, &cy, dir.x, dir.y, 0.0f, 2.0f, &active, &code);
         if (active) {
-            const float2 dir = a.dirs[k];
-            const float cx = ((float)px + 0.5f) * 2.0f, cy = ((float)py + 0.5f) * 2.0f;
-            MarchResult m = march_ray(a.sdf, a.emissive, a.W, a.H, cx, cy, dir.x, dir.y, 0.0f, 2.0f);
-            r = m.r;
-            g = m.g;
-            b = m.b;
-            al = m.miss ? 1.0f : 0.0f;
-            if (UPPER && m.miss) {
+            if (code >= 0) {
+                r = a.emissive[code];
+                g = a.emissive[plane + code];
+                b = a.emissive[2 * plane + code];
+            } else {
+                al = 1.0f;
+            }
+            if (UPPER && code < 0) {
                 const int sx = ((px - 1) >> 1) - ulx, sy = ((py - 1) >> 1) - uly;
                 const float4 u00 = su[(sy * UC + sx) * 4 + k];
                 const float4 u10 = su[(sy * UC + sx + 1) * 4 + k];
@@ -288,7 +279,6 @@ __global__ __launch_bounds__(kBlock) void k_level0(LevelArgs a, float *__restric
     }
     __syncthreads();
 
-    const size_t plane = (size_t)a.W * a.H;
     for (int i = threadIdx.x; i < 4 * TP * TP; i += kBlock) {
         const int x = 2 * px0 + (i & (2 * TP - 1)), y = 2 * py0 + i / (2 * TP);
         if (x >= a.W || y >= a.H) continue;
@@ -315,8 +305,10 @@ __global__ __launch_bounds__(kBlock) void k_march_probe(LevelArgs a, int32_t *__
     const int px = (int)(p % a.PX), py = (int)(p / a.PX);
     const float2 dir = a.dirs[k];
     const float cx = ((float)px + 0.5f) * a.spacing, cy = ((float)py + 0.5f) * a.spacing;
-    const MarchResult m = march_ray<false>(a.sdf, nullptr, a.W, a.H, cx, cy, dir.x, dir.y, a.t0, a.t1);
-    hit[ray] = m.hit;
+    const bool active = true;
+    int code;
+    march_rays<1>(sdf_rsrc(a.sdf, a.W, a.H), a.W, a.H, &cx, &cy, dir.x, dir.y, a.t0, a.t1, &active, &code);
+    hit[ray] = code;
     index[ray] = ((int64_t)py * a.PX + px) * a.N + k;
 }
 

@@ -33,6 +33,19 @@ __host__ __device__ inline int grid_dim(int pixels, int level)
 __host__ __device__ inline int64_t interval_t0(int level) { return (2 * ((1ll << (2 * level)) - 1)) / 3; }
 __host__ __device__ inline int64_t interval_t1(int level) { return (2 * ((1ll << (2 * level + 2)) - 1)) / 3; }
 
+// ---- distance-field layout --------------------------------------------------
+// The SDF is stored in 8x4-pixel tiles of 32 floats (one 128-byte cache
+// line), tiles row-major.  A march step moves about one pixel in any
+// direction, so consecutive samples of a ray stay in one line for several
+// steps whichever way it travels (row-major storage would change line on
+// every vertical step).
+__host__ __device__ inline int sdf_tiles_x(int W) { return (W + 7) >> 3; }
+__host__ __device__ inline size_t sdf_elems(int W, int H) { return (size_t)sdf_tiles_x(W) * 8 * (size_t)(((H + 3) >> 2) * 4); }
+__host__ __device__ inline int sdf_index(int x, int y, int tiles_x)
+{
+    return (((y >> 2) * tiles_x + (x >> 3)) << 5) + ((y & 3) << 3) + (x & 7);
+}
+
 // ---- storage texels ---------------------------------------------------------
 // fp32: float4.  fp16 / bf16: four 16-bit lanes packed in a uint2 (8 bytes).
 enum class Storage : int { F32 = 0, F16 = 1, BF16 = 2 };

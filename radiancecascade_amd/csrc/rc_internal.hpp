@@ -15,10 +15,14 @@ hipError_t launch_raster(const double *boxes_dev, int n, int W, int H, float *de
 
 // ---- occupancy + jump flood (rc_jfa.hip) --------------------------------------
 int jfa_num_passes(int W, int H);
-int jfa_tail_step(int W, int H);  // largest step fused into the LDS tail (0 = none)
+// largest step fused into the LDS tail for a requested fuse limit (0 = none)
+int jfa_tail_step(int W, int H, int fuse);
+hipError_t launch_sdf_untile(const float *sdf, float *out, int W, int H, hipStream_t s);
 hipError_t launch_seed_init(const float *depth, const float *emissive, uint32_t *seed, int W, int H, hipStream_t s);
-hipError_t launch_jfa_pass(const uint32_t *in, uint32_t *out, int W, int H, int step, hipStream_t s);
-hipError_t launch_jfa_tail(const uint32_t *in, uint32_t *seeds_out, float *sdf, int W, int H, hipStream_t s);
+hipError_t launch_jfa_pass(const uint32_t *in, uint32_t *out, float *sdf, int W, int H, int step, bool final_pass,
+                           hipStream_t s);
+hipError_t launch_jfa_tail(const uint32_t *in, uint32_t *seeds_out, float *sdf, int W, int H, int tail_step,
+                           hipStream_t s);
 
 // ---- cascades (rc_cascade.hip) ----------------------------------------------
 // Workgroup tile of a level kernel: TX x TY probes x KD directions, TX*TY*KD = 256.

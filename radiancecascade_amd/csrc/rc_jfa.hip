@@ -4,20 +4,26 @@
 // src/renderer.rs:559-613); this follows Rong & Tan, "Jump Flooding in GPU"
 // (I3D 2006) with a total-order tie-break so the result is independent of
 // evaluation order and therefore bit-exact across implementations:
-//   seed(p)  = (y << 16) | x if p is occupied, else kSeedNone;
+//   seed(p)  = (y << 16) | x if p is occupied, else "none";
 //   occupied = any emissive channel > 0, or a 4-neighbour q with
 //              D(q) - D(p) > 0.05 D(p) in fp32 (near side of a silhouette);
 //   pass(s)  : p takes the candidate with the smallest (d^2, seed_y, seed_x)
 //              among its own seed and the in-bounds seeds at p + s*(dx,dy);
 //   steps    : 2^(P-1), ..., 2, 1 with P = ceil(log2 max(W,H));
 //   SDF(p)   = (float)sqrt((double)d^2), +inf when no pixel is occupied.
-// Passes with step > 8 run as one global kernel each (every pixel gathers 9
-// seeds, rows coalesce); steps 8,4,2,1 and the SDF are fused in one LDS-tiled
-// kernel whose tile carries a 15-pixel halo (8+4+2+1), so the short passes
-// never round-trip through HBM.
+//
+// The passes are VALU-bound on CDNA4 (a wave64 VALU op occupies a 16-lane
+// SIMD for 4 cycles; 8 candidates per pixel per pass), so the candidate test
+// is kept to a packed 16-bit subtract, one dot2 and a 32-bit lexicographic
+// select, and each lane owns two horizontally adjacent pixels so address
+// arithmetic is shared.  Passes run as global kernels; the last pass writes
+// the SDF directly.  Optionally the last passes (step <= S) are fused in one
+// LDS-tiled kernel (halo 2S-1), trading halo recomputation for HBM traffic.
 #include "rc_internal.hpp"
 
 namespace rc {
+
+typedef short short2v __attribute__((ext_vector_type(2)));
 
 // Workgroups are handed to the 8 XCDs round-robin; give each XCD one
 // contiguous run of logical block ids so neighbouring tiles share its L2.
@@ -28,19 +34,35 @@ __device__ inline int xcd_remap(int b, int nb)
     return xcd * q + min(xcd, r) + idx;
 }
 
-// Inside the kernels "no seed" is a fake seed at (40000, 40000): for any
-// pixel of an image up to 16384 px wide its squared distance exceeds 2^30,
-// above every real candidate's (< 2 * 16383^2 < 2^30), and stays below 2^32.
-// So the (d^2, seed) minimum needs no special case; the external value
-// kSeedNone is restored on output.
-constexpr uint32_t kFarSeed = (40000u << 16) | 40000u;
+// Inside the kernels "no seed" is a fake seed at (32767, 32767).  For every
+// pixel of an image up to 16384 px a side its d^2 >= 2 * 16384^2, above any
+// real candidate's (<= 2 * 16383^2), and every d^2 fits in int32, so the
+// (d^2, seed) minimum needs no special case.  The external value kSeedNone is
+// restored on output.
+constexpr uint32_t kFarSeed = 0x7FFF7FFFu;
 
-// Order key (d^2, seed_y, seed_x): the packed seed already orders (y, x).
-__device__ inline uint64_t jfa_key(uint32_t s, int x, int y)
+// Buffer resource word 3 for gfx9-family raw buffer loads (32-bit data,
+// range-checked against num_records).
+constexpr int kBufferDword3 = 0x00020000;
+
+__device__ inline uint32_t seed_d2(uint32_t s, uint32_t pix)
 {
-    const int dx = (int)(s & 0xFFFFu) - x, dy = (int)(s >> 16) - y;
-    const uint32_t d2 = (uint32_t)__mul24(dx, dx) + (uint32_t)__mul24(dy, dy);
-    return ((uint64_t)d2 << 32) | s;
+    const short2v d = __builtin_bit_cast(short2v, s) - __builtin_bit_cast(short2v, pix);
+    return (uint32_t)__builtin_amdgcn_sdot2(d, d, 0, false);
+}
+
+// Keep (bd2, bs) = lexicographic minimum of (d^2, seed).
+__device__ inline void consider(uint32_t s, uint32_t pix, uint32_t &bd2, uint32_t &bs)
+{
+    const uint32_t d2 = seed_d2(s, pix);
+    const bool better = d2 < bd2 || (d2 == bd2 && s < bs);
+    bd2 = better ? d2 : bd2;
+    bs = better ? s : bs;
+}
+
+__device__ inline float seed_distance(uint32_t bs, uint32_t bd2)
+{
+    return bs == kFarSeed ? __builtin_huge_valf() : (float)sqrt((double)bd2);
 }
 
 int jfa_num_passes(int W, int H)
@@ -50,12 +72,12 @@ int jfa_num_passes(int W, int H)
     return P;
 }
 
-int jfa_tail_step(int W, int H)
+int jfa_tail_step(int W, int H, int fuse)
 {
-    int P = jfa_num_passes(W, H);
-    if (P == 0) return 0;
-    int top = 1 << (P - 1);
-    return top < 8 ? top : 8;
+    const int P = jfa_num_passes(W, H);
+    if (P == 0 || fuse <= 0) return 0;
+    const int top = 1 << (P - 1);
+    return top < fuse ? top : fuse;
 }
 
 // 64 x 4 pixel blocks, x fastest so each wave covers 64 consecutive pixels.
@@ -80,48 +102,75 @@ __global__ __launch_bounds__(kBlock) void k_seed_init(const float *__restrict__ 
     seed[p] = occ ? (((uint32_t)y << 16) | (uint32_t)x) : kFarSeed;
 }
 
+// One global pass.  A wave owns 64 consecutive pixels of one row (blocks of
+// 64 x 4); the row index is made wave-uniform so the three source rows are
+// scalar base pointers and each of the 9 loads is a 32-bit lane offset.
+// FINAL: write the SDF (and, when `out` is set, the external seed field)
+// instead of the internal seed field.
+template <bool FINAL>
 __global__ __launch_bounds__(kBlock) void k_jfa_pass(const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
-                                                     int W, int H, int step, int bx_count)
+                                                     float *__restrict__ sdf, int W, int H, int step, int bx_count)
 {
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int x = (b % bx_count) * 64 + (threadIdx.x & 63);
-    const int y = (b / bx_count) * 4 + (threadIdx.x >> 6);
-    if (x >= W || y >= H) return;
-    // 9 unconditional loads at clamped coordinates (in flight together); a
-    // candidate whose true coordinate is off-image is replaced by the far seed
-    const int xs[3] = {x - step, x, x + step}, ys[3] = {y - step, y, y + step};
+    const int y = __builtin_amdgcn_readfirstlane((b / bx_count) * 4 + (threadIdx.x >> 6));
+    if (y >= H) return;
+    const bool xin = x < W;
+    const int xc = min(x, W - 1);
+    const uint32_t pix = ((uint32_t)y << 16) | (uint32_t)xc;
+    const int xl = x - step, xr = x + step;
+    // unsigned 32-bit lane offsets let the loads use the scalar-base form
+    const uint32_t ol = (uint32_t)max(xl, 0), oc = (uint32_t)xc, orr = (uint32_t)min(xr, W - 1);
+
+    // 9 unconditional buffer loads at clamped coordinates, all in flight
+    // together: the row offset is a scalar, the column offset a lane VGPR
+    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<uint32_t *>(in), (short)0, W * H * 4, kBufferDword3);
     uint32_t cand[9];
 #pragma unroll
-    for (int j = 0; j < 9; ++j) {
-        const int qx = min(max(xs[j % 3], 0), W - 1), qy = min(max(ys[j / 3], 0), H - 1);
-        cand[j] = in[(size_t)qy * W + qx];
+    for (int r = 0; r < 3; ++r) {
+        const int qy = min(max(y + (r - 1) * step, 0), H - 1);
+        const int rowb = qy * W * 4;
+        cand[3 * r + 0] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, ol * 4, rowb, 0);
+        cand[3 * r + 1] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, oc * 4, rowb, 0);
+        cand[3 * r + 2] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, orr * 4, rowb, 0);
     }
-    uint64_t best = jfa_key(cand[4], x, y);
+    const bool up = y - step >= 0, dn = y + step < H, lf = xl >= 0, rt = xr < W;
+    const bool valid[9] = {up && lf, up, up && rt, lf, true, rt, dn && lf, dn, dn && rt};
+    uint64_t best = ((uint64_t)seed_d2(cand[4], pix) << 32) | cand[4];
 #pragma unroll
     for (int j = 0; j < 9; ++j) {
         if (j == 4) continue;
-        const bool inb = xs[j % 3] >= 0 && xs[j % 3] < W && ys[j / 3] >= 0 && ys[j / 3] < H;
-        const uint64_t k = jfa_key(inb ? cand[j] : kFarSeed, x, y);
-        best = k < best ? k : best;
+        const uint32_t sd = valid[j] ? cand[j] : kFarSeed;
+        const uint64_t key = ((uint64_t)seed_d2(sd, pix) << 32) | sd;
+        best = key < best ? key : best;
     }
-    out[(size_t)y * W + x] = (uint32_t)best;
+    if (!xin) return;
+    const uint32_t bs = (uint32_t)best, bd2 = (uint32_t)(best >> 32);
+    const size_t p = (size_t)y * W + x;
+    if (FINAL) {
+        sdf[sdf_index(x, y, sdf_tiles_x(W))] = seed_distance(bs, bd2);
+        if (out) out[p] = bs == kFarSeed ? kSeedNone : bs;
+    } else {
+        out[p] = bs;
+    }
 }
 
-// Fused passes S, S/2, ..., 1 over a 32x32 output tile.  The tile is loaded
-// with a halo of 2S-1 pixels (R <= 62 cells a side); each pass shrinks the
-// valid region by its step.  Threads are laid out 64 x 4: lane = column, so
-// every LDS access of a wave is one row of consecutive words.  Cells outside
-// the image stay the far seed in every pass, which is exactly the global
-// kernel's "out-of-bounds neighbours are skipped".
+// Fused passes S, S/2, ..., 1 over a 32x32 output tile, loaded with a halo of
+// 2S-1 pixels (R <= 62 cells a side); each pass shrinks the valid region by
+// its step.  Threads are laid out 64 x 4: lane = column, so every LDS access
+// of a wave is one row of consecutive words.  Cells outside the image stay the
+// far seed in every pass, which is exactly "out-of-bounds neighbours are
+// skipped".
 template <int S>
 __global__ __launch_bounds__(kBlock) void k_jfa_tail(const uint32_t *__restrict__ in,
                                                      uint32_t *__restrict__ seeds_out, float *__restrict__ sdf,
                                                      int W, int H, int bx_count)
 {
     constexpr int T = 32;
-    constexpr int HALO = S > 0 ? 2 * S - 1 : 0;
+    constexpr int HALO = 2 * S - 1;
     constexpr int R = T + 2 * HALO;
-    static_assert(R <= 64, "one column per lane");
+    static_assert(S >= 1 && R <= 64, "one column per lane");
     __shared__ uint32_t buf[2][R * R];
 
     const int b = xcd_remap(blockIdx.x, gridDim.x);
@@ -129,8 +178,6 @@ __global__ __launch_bounds__(kBlock) void k_jfa_tail(const uint32_t *__restrict_
     const int y0 = (b / bx_count) * T - HALO;
     const int cx = threadIdx.x & 63, cy = threadIdx.x >> 6;
     {
-        // all rows of this lane's column loaded (clamped, unconditional) before
-        // any LDS write, so the global loads overlap
         constexpr int ROWS = (R + 3) / 4;
         const int gx = x0 + cx, qx = min(max(gx, 0), W - 1);
         uint32_t v[ROWS];
@@ -150,30 +197,34 @@ __global__ __launch_bounds__(kBlock) void k_jfa_tail(const uint32_t *__restrict_
 
     int cur = 0;
     int off = 0;
+    uint32_t last_d2[(T + 3) / 4] = {};
 #pragma unroll
     for (int s = S; s >= 1; s >>= 1) {
         off += s;
         const int rx = off + cx;
-        if (rx < R - off) {
-            const int gx = x0 + rx;
-            const bool xin = gx >= 0 && gx < W;
-            for (int ry = off + cy; ry < R - off; ry += 4) {
-                const int gy = y0 + ry;
-                const uint32_t *src = buf[cur];
-                uint32_t res = kFarSeed;
+        const int gx = x0 + rx;
+        const bool xin = rx < R - off && gx >= 0 && gx < W;
+#pragma unroll
+        for (int j = 0; j < (R - 2 * off + 3) / 4; ++j) {
+            const int ry = off + cy + 4 * j;
+            const int gy = y0 + ry;
+            const uint32_t *src = buf[cur];
+            if (rx < R - off && ry < R - off) {
+                uint32_t bs = kFarSeed, bd2 = 0xFFFFFFFFu;
                 if (xin && gy >= 0 && gy < H) {
-                    uint64_t best = jfa_key(src[ry * R + rx], gx, gy);
+                    const uint32_t pix = ((uint32_t)gy << 16) | (uint32_t)gx;
+                    bs = src[ry * R + rx];
+                    bd2 = seed_d2(bs, pix);
 #pragma unroll
                     for (int dy = -1; dy <= 1; ++dy)
 #pragma unroll
                         for (int dx = -1; dx <= 1; ++dx) {
                             if (dx == 0 && dy == 0) continue;
-                            const uint64_t k = jfa_key(src[(ry + dy * s) * R + rx + dx * s], gx, gy);
-                            best = k < best ? k : best;
+                            consider(src[(ry + dy * s) * R + rx + dx * s], pix, bd2, bs);
                         }
-                    res = (uint32_t)best;
                 }
-                buf[cur ^ 1][ry * R + rx] = res;
+                buf[cur ^ 1][ry * R + rx] = bs;
+                if (s == 1) last_d2[j] = bd2;
             }
         }
         __syncthreads();
@@ -182,16 +233,42 @@ __global__ __launch_bounds__(kBlock) void k_jfa_tail(const uint32_t *__restrict_
 
     if (cx < T) {
         const int gx = x0 + HALO + cx;
-        for (int ty = cy; ty < T; ty += 4) {
+#pragma unroll
+        for (int j = 0; j < (T + 3) / 4; ++j) {
+            const int ty = cy + 4 * j;
             const int gy = y0 + HALO + ty;
-            if (gx >= W || gy >= H) continue;
+            if (ty >= T || gx >= W || gy >= H) continue;
             const uint32_t sd = buf[cur][(HALO + ty) * R + HALO + cx];
-            const int dx = (int)(sd & 0xFFFFu) - gx, dy = (int)(sd >> 16) - gy;
-            const float d = sd == kFarSeed ? __builtin_huge_valf() : (float)sqrt((double)(uint32_t)(dx * dx + dy * dy));
-            sdf[(size_t)gy * W + gx] = d;
+            sdf[sdf_index(gx, gy, sdf_tiles_x(W))] = seed_distance(sd, last_d2[j]);
             if (seeds_out) seeds_out[(size_t)gy * W + gx] = sd == kFarSeed ? kSeedNone : sd;
         }
     }
+}
+
+// An image with no pass at all (1x1): only the SDF conversion.
+__global__ void k_jfa_none(const uint32_t *__restrict__ in, uint32_t *__restrict__ seeds_out,
+                           float *__restrict__ sdf, int W, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t s = in[i];
+    sdf[sdf_index(i % W, i / W, sdf_tiles_x(W))] = s == kFarSeed ? __builtin_huge_valf() : 0.0f;
+    if (seeds_out) seeds_out[i] = s == kFarSeed ? kSeedNone : s;
+}
+
+// Row-major copy of the tiled distance field (parity readback).
+__global__ void k_sdf_untile(const float *__restrict__ sdf, float *__restrict__ out, int W, int H)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= W * H) return;
+    out[i] = sdf[sdf_index(i % W, i / W, sdf_tiles_x(W))];
+}
+
+hipError_t launch_sdf_untile(const float *sdf, float *out, int W, int H, hipStream_t s)
+{
+    const int n = W * H;
+    hipLaunchKernelGGL(k_sdf_untile, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, sdf, out, W, H);
+    return hipGetLastError();
 }
 
 hipError_t launch_seed_init(const float *depth, const float *emissive, uint32_t *seed, int W, int H, hipStream_t s)
@@ -201,23 +278,33 @@ hipError_t launch_seed_init(const float *depth, const float *emissive, uint32_t 
     return hipGetLastError();
 }
 
-hipError_t launch_jfa_pass(const uint32_t *in, uint32_t *out, int W, int H, int step, hipStream_t s)
+hipError_t launch_jfa_pass(const uint32_t *in, uint32_t *out, float *sdf, int W, int H, int step, bool final_pass,
+                           hipStream_t s)
 {
     const int bxc = (W + 63) / 64, byc = (H + 3) / 4;
-    hipLaunchKernelGGL(k_jfa_pass, dim3(bxc * byc), dim3(kBlock), 0, s, in, out, W, H, step, bxc);
+    if (final_pass)
+        hipLaunchKernelGGL(k_jfa_pass<true>, dim3(bxc * byc), dim3(kBlock), 0, s, in, out, sdf, W, H, step, bxc);
+    else
+        hipLaunchKernelGGL(k_jfa_pass<false>, dim3(bxc * byc), dim3(kBlock), 0, s, in, out, sdf, W, H, step, bxc);
     return hipGetLastError();
 }
 
-hipError_t launch_jfa_tail(const uint32_t *in, uint32_t *seeds_out, float *sdf, int W, int H, hipStream_t s)
+hipError_t launch_jfa_tail(const uint32_t *in, uint32_t *seeds_out, float *sdf, int W, int H, int tail_step,
+                           hipStream_t s)
 {
     const int bxc = (W + 31) / 32, byc = (H + 31) / 32;
     const dim3 grid(bxc * byc);
-    switch (jfa_tail_step(W, H)) {
-    case 0: hipLaunchKernelGGL(k_jfa_tail<0>, grid, dim3(kBlock), 0, s, in, seeds_out, sdf, W, H, bxc); break;
+    switch (tail_step) {
+    case 0: {
+        const int n = W * H;
+        hipLaunchKernelGGL(k_jfa_none, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, in, seeds_out, sdf, W, n);
+        break;
+    }
     case 1: hipLaunchKernelGGL(k_jfa_tail<1>, grid, dim3(kBlock), 0, s, in, seeds_out, sdf, W, H, bxc); break;
     case 2: hipLaunchKernelGGL(k_jfa_tail<2>, grid, dim3(kBlock), 0, s, in, seeds_out, sdf, W, H, bxc); break;
     case 4: hipLaunchKernelGGL(k_jfa_tail<4>, grid, dim3(kBlock), 0, s, in, seeds_out, sdf, W, H, bxc); break;
-    default: hipLaunchKernelGGL(k_jfa_tail<8>, grid, dim3(kBlock), 0, s, in, seeds_out, sdf, W, H, bxc); break;
+    case 8: hipLaunchKernelGGL(k_jfa_tail<8>, grid, dim3(kBlock), 0, s, in, seeds_out, sdf, W, H, bxc); break;
+    default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }

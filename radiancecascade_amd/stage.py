@@ -117,6 +117,12 @@ class RadianceCascadeStage:
     def set_tiling(self, level: int, shape: int, rays_per_lane: int) -> None:
         check(self._lib.rc_set_level_tiling(self._ctx, level, shape, rays_per_lane), self._ctx)
 
+    def set_jfa_fuse(self, fused_step: int) -> None:
+        check(self._lib.rc_set_jfa_fuse(self._ctx, fused_step), self._ctx)
+
+    def jfa_fuse(self) -> int:
+        return self._lib.rc_get_jfa_fuse(self._ctx)
+
     def tiling(self) -> dict:
         """{level: ((TX, TY, KD), rays_per_lane)} for levels >= 1."""
         out = {}

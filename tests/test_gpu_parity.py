@@ -242,3 +242,30 @@ def test_graph_replay_matches_direct(rc, oracle):
             graphed.run_device(depth, emis, b)
             torch.cuda.synchronize()
             assert torch.equal(a, b), f"replay {rep}"
+
+
+@pytest.mark.parametrize("W,H", [(300, 200), (1920, 1080), (65, 7)])
+def test_every_jfa_fusion_is_bit_exact(rc, oracle, W, H):
+    """Global passes only, or the last passes fused in LDS: same jump-flood
+    field and distance field."""
+    _say(f"jfa fusion W={W} H={H}")
+    seed, nb, ne = 3, 32, 4
+    depth, emis = _device_gbuffer(rc, W, H, seed, nb, ne)
+    d, _, e = oracle.gbuffer(seed, W, H, nb, ne)
+    seeds_ref = oracle.jfa(oracle.seed_init(d, e))
+    sdf_ref = oracle.sdf_f32(seeds_ref)
+    irr = torch.empty((3, H, W), dtype=torch.float32, device="cuda:0")
+    seeds = torch.empty((H, W), dtype=torch.int32, device="cuda:0")
+    sdf = torch.empty((H, W), dtype=torch.float32, device="cuda:0")
+    L = oracle.levels_for_coverage(W, H)
+    with rc.RadianceCascadeStage(W, H, levels=L, debug=True) as st:
+        for fuse in (0, 1, 2, 4, 8):
+            st.set_jfa_fuse(fuse)
+            st.run_device(depth, emis, irr)
+            st.debug_seeds(seeds)
+            st.debug_sdf(sdf)
+            torch.cuda.synchronize()
+            assert np.array_equal(seeds.cpu().numpy().view(np.uint32), seeds_ref), fuse
+            assert np.array_equal(sdf.cpu().numpy().view(np.uint32), sdf_ref.view(np.uint32)), fuse
+        with pytest.raises(rc.RcError):
+            st.set_jfa_fuse(3)
