@@ -47,6 +47,10 @@ def kernel_alg_bytes(W, H, L, storage, rays):
     for i in range(1, L):
         out[f"level{i}"] = b * rays[i] + (b * rays[i + 1] if i + 1 < L else 0)
     out["level0"] = 4 * n + 12 * n + (b * rays[1] if L > 1 else 0)
+    # split march of all levels >= 1: reads the SDF once (4 B/px) and produces
+    # one b-byte ray-interval result per ray (the march row of the model);
+    # the merge kernels are then charged the same contract terms as above.
+    out["march"] = 4 * n + b * sum(rays[1:L])
     return out
 
 
@@ -131,6 +135,9 @@ def main():
     for lvl, ((tx, ty, kd), rpt) in tiling.items():
         shape = [k for k, v in rc._abi.TILE_SHAPES.items() if v == (tx, ty, kd)][0]
         prof.set_tiling(lvl, shape, rpt)
+    split, spec = stage.march()
+    prof.set_march(split, spec)
+    prof.set_jfa_fuse(stage.jfa_fuse())
     for _ in range(5):
         prof.run_device(depth, emis, irr, stream)
     prof.reset_stats()
@@ -169,7 +176,8 @@ def main():
                                f"seeded box scene (seed {args.seed}, {args.boxes} boxes, {args.emissive} emissive)",
                    "path": "plain", "width": W, "height": H, "levels": L, "storage": args.storage,
                    "rays_per_frame": int(sum(rays)),
-                   "tiling": {str(k): f"{v[0][0]}x{v[0][1]}x{v[0][2]} r{v[1]}" for k, v in tiling.items()}},
+                   "tiling": {str(k): f"{v[0][0]}x{v[0][1]}x{v[0][2]} r{v[1]}" for k, v in tiling.items()},
+                   "march": {"split": split, "speculation": spec}, "jfa_fuse": stage.jfa_fuse()},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "alg_bytes_per_launch": alg[dominant], "mean_launch_ms": dom_ms},

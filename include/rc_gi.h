@@ -95,7 +95,8 @@ typedef struct {
 #define RC_STAGE_JFA 1          /* all global jump-flood passes */
 #define RC_STAGE_JFA_TAIL 2     /* fused short-step passes + distance field */
 #define RC_STAGE_LEVEL0 3       /* RC_STAGE_LEVEL0 + i = level i march/merge; level 0 includes the gather */
-#define RC_NUM_STAGES (RC_STAGE_LEVEL0 + RC_MAX_LEVELS)
+#define RC_STAGE_MARCH (RC_STAGE_LEVEL0 + RC_MAX_LEVELS) /* split march of all levels >= 1 */
+#define RC_NUM_STAGES (RC_STAGE_MARCH + 1)
 typedef struct {
     uint32_t version;
     uint32_t runs;
@@ -132,6 +133,14 @@ rc_status rc_autotune(rc_ctx *ctx, const rc_gbuffer *dev, float *irradiance_dev,
  * identical; rc_autotune also picks this. */
 rc_status rc_set_jfa_fuse(rc_ctx *ctx, uint32_t fused_step);
 uint32_t rc_get_jfa_fuse(const rc_ctx *ctx);
+/* split = 1: every level >= 1 is marched by one launch (top level first, so
+ * the longest rays start earliest and the levels' dependent chains overlap),
+ * then the levels are merged top-down from the stored ray codes; split = 0:
+ * each level kernel marches and merges its own rays.  speculation (1, 2, 4)
+ * is the split march's look-ahead depth.  The result is identical; rc_autotune
+ * also picks these. */
+rc_status rc_set_march(rc_ctx *ctx, uint32_t split, uint32_t speculation);
+rc_status rc_get_march(const rc_ctx *ctx, uint32_t *split, uint32_t *speculation);
 
 /* ---- introspection -------------------------------------------------------- */
 rc_status rc_get_config(const rc_ctx *ctx, rc_config *out);

@@ -31,7 +31,8 @@ FLAG_NO_GRAPH = 0x4
 TILE_SHAPES = {0: (8, 8, 4), 1: (4, 4, 16), 2: (2, 2, 64), 3: (1, 1, 256)}
 
 STAGE_SEED, STAGE_JFA, STAGE_JFA_TAIL, STAGE_LEVEL0 = 0, 1, 2, 3
-NUM_STAGES = STAGE_LEVEL0 + MAX_LEVELS
+STAGE_MARCH = STAGE_LEVEL0 + MAX_LEVELS
+NUM_STAGES = STAGE_MARCH + 1
 
 
 class Config(ctypes.Structure):
@@ -72,6 +73,8 @@ SIGNATURES = {
     "rc_autotune": (_i32, [_P, ctypes.POINTER(GBuffer), _P, _u32, _P]),
     "rc_set_jfa_fuse": (_i32, [_P, _u32]),
     "rc_get_jfa_fuse": (_u32, [_P]),
+    "rc_set_march": (_i32, [_P, _u32, _u32]),
+    "rc_get_march": (_i32, [_P, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
     "rc_get_config": (_i32, [_P, ctypes.POINTER(Config)]),
     "rc_get_stats": (_i32, [_P, ctypes.POINTER(Stats)]),
     "rc_reset_stats": (_i32, [_P]),

@@ -32,6 +32,12 @@ struct rc_ctx {
     int tile[RC_MAX_LEVELS] = {};
     int rpt[RC_MAX_LEVELS] = {};
     int jfa_fuse = 0;  // largest jump-flood step fused into the LDS tail (0 = none)
+    // split march: levels >= 1 marched by one k_march_all launch into `codes`,
+    // then merged top-down; otherwise each level kernel marches its own rays
+    int march_split = 1;
+    int spec = 1;      // speculation depth of the split march (1, 2, 4)
+    int *codes = nullptr;
+    size_t code_off[RC_MAX_LEVELS] = {};
     int dbg_seed_buf = -1;
     bool ran = false;
     // host-staging path (rc_update_host / rc_render_host)
@@ -107,6 +113,8 @@ static void release(rc_ctx *c)
     c->dbg_level0 = nullptr;
     if (c->dirs) hipFree(c->dirs);
     c->dirs = nullptr;
+    if (c->codes) hipFree(c->codes);
+    c->codes = nullptr;
     if (c->g_depth) hipFree(c->g_depth);
     if (c->g_emissive) hipFree(c->g_emissive);
     if (c->irr) hipFree(c->irr);
@@ -145,6 +153,12 @@ static rc_status allocate(rc_ctx *c, uint32_t W, uint32_t H, uint32_t L)
         c->tile[i] = default_tile(i, (int)L);
         c->rpt[i] = 2;
     }
+    size_t ncodes = 0;
+    for (int i = 1; i < (int)L; ++i) {
+        c->code_off[i] = ncodes;
+        ncodes += (size_t)grid_dim(W, i) * grid_dim(H, i) * num_dirs(i);
+    }
+    if (ncodes) HIP_TRY(c, hipMalloc(&c->codes, ncodes * sizeof(int)));
     HIP_TRY(c, hipMalloc(&c->dirs, dirs.size() * sizeof(float2)));
     HIP_TRY(c, hipMemcpy(c->dirs, dirs.data(), dirs.size() * sizeof(float2), hipMemcpyHostToDevice));
     for (int i = 0; i < RC_MAX_LEVELS; ++i) {
@@ -173,7 +187,32 @@ static LevelArgs level_args(const rc_ctx *c, int i, const float *emissive)
     a.t0 = (float)interval_t0(i);
     a.t1 = (float)interval_t1(i);
     a.spacing = (float)probe_spacing(i);
+    a.codes = nullptr;
     return a;
+}
+
+static MarchAllArgs march_all_args(const rc_ctx *c)
+{
+    MarchAllArgs m{};
+    m.sdf = c->sdf;
+    m.W = c->W;
+    m.H = c->H;
+    int blocks = 0;
+    for (int i = c->L - 1; i >= 1; --i) {
+        MarchLevel &l = m.lev[m.nlev++];
+        l.dirs = c->dirs + c->dir_off[i];
+        l.codes = c->codes + c->code_off[i];
+        l.PX = grid_dim(c->W, i);
+        l.PY = grid_dim(c->H, i);
+        l.N = num_dirs(i);
+        l.t0 = (float)interval_t0(i);
+        l.t1 = (float)interval_t1(i);
+        l.spacing = (float)probe_spacing(i);
+        l.block_begin = blocks;
+        l.blocks = march_all_blocks(l.PX, l.PY, l.N);
+        blocks += l.blocks;
+    }
+    return m;
 }
 
 struct StageTimer {
@@ -226,9 +265,17 @@ static rc_status run_frame(rc_ctx *c, const float *depth, const float *emissive,
         c->dbg_seed_buf = dbg ? (cur ^ 1) : -1;
     }
 
+    const bool split = c->march_split && c->L > 1;
+    if (split) {
+        tm.begin(RC_STAGE_MARCH);
+        HIP_TRY(c, launch_march_all(march_all_args(c), c->spec, s));
+        tm.end(RC_STAGE_MARCH);
+    }
     for (int i = c->L - 1; i >= 1; --i) {
         tm.begin(RC_STAGE_LEVEL0 + i);
-        HIP_TRY(c, launch_level(level_args(c, i, emissive), c->storage, c->tile[i], c->rpt[i], s));
+        LevelArgs a = level_args(c, i, emissive);
+        if (split) a.codes = c->codes + c->code_off[i];
+        HIP_TRY(c, launch_level(a, c->storage, c->tile[i], c->rpt[i], s));
         tm.end(RC_STAGE_LEVEL0 + i);
     }
     tm.begin(RC_STAGE_LEVEL0);
@@ -485,7 +532,7 @@ rc_status rc_debug_march(rc_ctx *c, uint32_t level, int32_t *hit_dev, int64_t *i
     if (!c->ran) return fail(c, RC_ERR_STATE, "rc_debug_march: no run yet");
     // march-only: the probe kernel reads the distance field, never emissive
     LevelArgs a = level_args(c, (int)level, nullptr);
-    HIP_TRY(c, launch_march_probe(a, hit_dev, index_dev, (hipStream_t)stream));
+    HIP_TRY(c, launch_march_probe(a, c->spec, hit_dev, index_dev, (hipStream_t)stream));
     return RC_OK;
 }
 
@@ -517,6 +564,25 @@ rc_status rc_set_jfa_fuse(rc_ctx *c, uint32_t fused_step)
 }
 
 uint32_t rc_get_jfa_fuse(const rc_ctx *c) { return c ? (uint32_t)c->jfa_fuse : 0u; }
+
+rc_status rc_set_march(rc_ctx *c, uint32_t split, uint32_t speculation)
+{
+    if (!c) return fail(nullptr, RC_ERR_NULL, "rc_set_march: ctx is NULL");
+    if (split > 1 || (speculation != 1 && speculation != 2 && speculation != 4))
+        return fail(c, RC_ERR_INVALID, "rc_set_march: split must be 0/1, speculation 1, 2 or 4");
+    c->march_split = (int)split;
+    c->spec = (int)speculation;
+    drop_graph(c);
+    return RC_OK;
+}
+
+rc_status rc_get_march(const rc_ctx *c, uint32_t *split, uint32_t *speculation)
+{
+    if (!c || !split || !speculation) return fail(nullptr, RC_ERR_NULL, "rc_get_march: NULL argument");
+    *split = (uint32_t)c->march_split;
+    *speculation = (uint32_t)c->spec;
+    return RC_OK;
+}
 
 rc_status rc_get_level_tiling(const rc_ctx *c, uint32_t level, uint32_t *shape, uint32_t *rpt)
 {
@@ -562,10 +628,35 @@ rc_status rc_autotune(rc_ctx *c, const rc_gbuffer *g, float *irr, uint32_t iters
         }
         c->jfa_fuse = best_fuse;
     }
+    {
+        float best = 1e30f;
+        int best_split = c->march_split, best_spec = c->spec;
+        for (int split : {0, 1}) {
+            for (int spec : {1, 2, 4}) {
+                if (!split && spec != 1) continue;
+                c->march_split = split;
+                c->spec = spec;
+                HIP_TRY(c, hipEventRecord(e0, s));
+                for (uint32_t it = 0; it < iters; ++it) {
+                    rc_status r = run_frame(c, g->depth, g->emissive, irr, s, true);
+                    if (r != RC_OK) return r;
+                }
+                HIP_TRY(c, hipEventRecord(e1, s));
+                HIP_TRY(c, hipEventSynchronize(e1));
+                float ms = 0.f;
+                HIP_TRY(c, hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) { best = ms; best_split = split; best_spec = spec; }
+            }
+        }
+        c->march_split = best_split;
+        c->spec = best_spec;
+        if (best_split) HIP_TRY(c, launch_march_all(march_all_args(c), c->spec, s));
+    }
     for (int i = 1; i < c->L; ++i) {
         float best = 1e30f;
         int best_shape = c->tile[i], best_rpt = c->rpt[i];
-        const LevelArgs a = level_args(c, i, g->emissive);
+        LevelArgs a = level_args(c, i, g->emissive);
+        if (c->march_split) a.codes = c->codes + c->code_off[i];
         for (int shape = 0; shape < TILE_COUNT; ++shape) {
             for (int rpt : {1, 2, 4}) {
                 if (!tiling_legal(i, shape, rpt)) continue;

@@ -114,6 +114,71 @@ __device__ inline void march_rays(__amdgpu_buffer_rsrc_t sdf, int W, int H, cons
     }
 }
 
+// One ray per lane with SPEC-deep speculation.  After a sample with distance
+// d the next sample is exactly t + max(d, 1); a ray grazing a surface keeps
+// taking the same step, so each round also loads the SPEC-1 samples that
+// follow if the step stays unchanged (positions built with the same fp32
+// additions the sequential march would make).  The samples are then checked
+// in order; the run of accepted ones ends at the first sample whose step
+// differs, which starts the next round.  The visited sample sequence, and so
+// every result bit, is that of the one-sample-per-step march.
+template <int SPEC>
+__device__ inline int march_ray_spec(__amdgpu_buffer_rsrc_t sdf, int W, int H, float cx, float cy, float dx,
+                                     float dy, float t0, float t1, bool active)
+{
+    const float fW = (float)W, fH = (float)H;
+    const int tiles_x = sdf_tiles_x(W);
+    int code = kRayT1;
+    bool alive = active && t0 < t1;
+    float t = t0, guess = 1.0f;
+    while (alive) {
+        float p[SPEC], d[SPEC];
+        int ix[SPEC], iy[SPEC];
+        bool ins[SPEC];
+        p[0] = t;
+#pragma unroll
+        for (int j = 1; j < SPEC; ++j) p[j] = p[j - 1] + guess;
+#pragma unroll
+        for (int j = 0; j < SPEC; ++j) {
+            const float x = floorf(fmaf(p[j], dx, cx));
+            const float y = floorf(fmaf(p[j], dy, cy));
+            ins[j] = x >= 0.f && y >= 0.f && x < fW && y < fH;
+            ix[j] = (int)x;
+            iy[j] = (int)y;
+            d[j] = sdf_load(sdf, ix[j], iy[j], tiles_x);
+        }
+        bool go = true;
+#pragma unroll
+        for (int j = 0; j < SPEC; ++j) {
+            if (!go) continue;
+            if (j > 0 && !(p[j] < t1)) {  // the previous step reached the interval end
+                alive = false;
+                go = false;
+                continue;
+            }
+            if (!ins[j]) {
+                code = kRayExit;
+                alive = false;
+                go = false;
+                continue;
+            }
+            if (d[j] < 0.5f) {
+                code = iy[j] * W + ix[j];
+                alive = false;
+                go = false;
+                continue;
+            }
+            const float step = d[j] > 1.0f ? d[j] : 1.0f;
+            if (j + 1 < SPEC && step == guess) continue;  // p[j+1] == p[j] + step
+            t = p[j] + step;
+            guess = step;
+            alive = t < t1;
+            go = false;
+        }
+    }
+    return code;
+}
+
 // Bilinear of four staged texels with the lower probe's parity weights.
 __device__ inline float3 bilerp4(float4 u00, float4 u10, float4 u01, float4 u11, int x, int y)
 {
@@ -128,7 +193,7 @@ __device__ inline float3 bilerp4(float4 u00, float4 u10, float4 u01, float4 u11,
 
 // Level i >= 1.  Workgroup tile: TX x (TY*RPT) probes x KD directions; lane
 // (probe column, direction) owns the RPT probes stacked TY rows apart.
-template <int TX, int TY, int KD, int RPT, Storage S, bool UPPER>
+template <int TX, int TY, int KD, int RPT, Storage S, bool UPPER, bool FROM_CODES>
 __global__ __launch_bounds__(kBlock) void k_level(LevelArgs a)
 {
     static_assert(TX * TY * KD == kBlock, "a tile row block holds 256 rays");
@@ -168,7 +233,16 @@ __global__ __launch_bounds__(kBlock) void k_level(LevelArgs a)
         cy[j] = ((float)py + 0.5f) * a.spacing;
         active[j] = px < a.PX && py < a.PY;
     }
-    march_rays<RPT>(sdf_rsrc(a.sdf, a.W, a.H), a.W, a.H, cx, cy, dir.x, dir.y, a.t0, a.t1, active, hitq);
+    if (FROM_CODES) {
+        // the rays were marched by k_march_all; read their codes (4 per quad)
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            const int py = min(pyb + j * TY, a.PY - 1);
+            hitq[j] = a.codes[((size_t)py * a.PX + min(px, a.PX - 1)) * a.N + k];
+        }
+    } else {
+        march_rays<RPT>(sdf_rsrc(a.sdf, a.W, a.H), a.W, a.H, cx, cy, dir.x, dir.y, a.t0, a.t1, active, hitq);
+    }
 
     const size_t plane = (size_t)a.W * a.H;
 #pragma unroll
@@ -293,7 +367,40 @@ __global__ __launch_bounds__(kBlock) void k_level0(LevelArgs a, float *__restric
     }
 }
 
+// All levels >= 1 marched by one launch; each ray's code (hit pixel, -1 left
+// the image, -2 reached t1) is written in canonical order for the merge
+// kernels.  The block list is level-major with the top level first, so the
+// levels whose rays are longest start earliest, and the long dependent chains
+// of different levels run concurrently instead of one kernel after another.
+// Blocks are 256 rays: KD = min(N, 256) consecutive directions of
+// 256/KD probes (a square-ish probe tile); blocks go round-robin to the XCDs.
+template <int SPEC>
+__global__ __launch_bounds__(kBlock) void k_march_all(MarchAllArgs a)
+{
+    int lv = 0;
+#pragma unroll 1
+    while (lv + 1 < a.nlev && (int)blockIdx.x >= a.lev[lv + 1].block_begin) ++lv;
+    const MarchLevel &L = a.lev[lv];
+    const int b = blockIdx.x - L.block_begin;
+    const int KD = L.N < kBlock ? L.N : kBlock;
+    const int probes = kBlock / KD;                 // 1, 4 or 16 per block
+    const int tx = probes == 16 ? 4 : probes == 4 ? 2 : 1;
+    const int slices = L.N / KD;
+    const int slice = b % slices, tile = b / slices;
+    const int tiles_x = (L.PX + tx - 1) / tx;
+    const int kk = threadIdx.x % KD, pl = threadIdx.x / KD;
+    const int px = (tile % tiles_x) * tx + pl % tx, py = (tile / tiles_x) * tx + pl / tx;
+    const int k = slice * KD + kk;
+    const bool active = px < L.PX && py < L.PY;
+    const float2 dir = L.dirs[k];
+    const float cx = ((float)px + 0.5f) * L.spacing, cy = ((float)py + 0.5f) * L.spacing;
+    const int code = march_ray_spec<SPEC>(sdf_rsrc(a.sdf, a.W, a.H), a.W, a.H, cx, cy, dir.x, dir.y, L.t0, L.t1,
+                                          active);
+    if (active) L.codes[((size_t)py * L.PX + px) * L.N + k] = code;
+}
+
 // March-only probe for parity tests: hit code and canonical index per ray.
+template <int SPEC>
 __global__ __launch_bounds__(kBlock) void k_march_probe(LevelArgs a, int32_t *__restrict__ hit,
                                                         int64_t *__restrict__ index)
 {
@@ -305,10 +412,7 @@ __global__ __launch_bounds__(kBlock) void k_march_probe(LevelArgs a, int32_t *__
     const int px = (int)(p % a.PX), py = (int)(p / a.PX);
     const float2 dir = a.dirs[k];
     const float cx = ((float)px + 0.5f) * a.spacing, cy = ((float)py + 0.5f) * a.spacing;
-    const bool active = true;
-    int code;
-    march_rays<1>(sdf_rsrc(a.sdf, a.W, a.H), a.W, a.H, &cx, &cy, dir.x, dir.y, a.t0, a.t1, &active, &code);
-    hit[ray] = code;
+    hit[ray] = march_ray_spec<SPEC>(sdf_rsrc(a.sdf, a.W, a.H), a.W, a.H, cx, cy, dir.x, dir.y, a.t0, a.t1, true);
     index[ray] = ((int64_t)py * a.PX + px) * a.N + k;
 }
 
@@ -327,8 +431,13 @@ template <int TX, int TY, int KD, int RPT, Storage S>
 static void launch_level_shape(const LevelArgs &a, hipStream_t s)
 {
     const int blocks = ((a.PX + TX - 1) / TX) * ((a.PY + TY * RPT - 1) / (TY * RPT)) * (a.N / KD);
-    if (a.upper) hipLaunchKernelGGL((k_level<TX, TY, KD, RPT, S, true>), dim3(blocks), dim3(kBlock), 0, s, a);
-    else hipLaunchKernelGGL((k_level<TX, TY, KD, RPT, S, false>), dim3(blocks), dim3(kBlock), 0, s, a);
+    if (a.codes) {
+        if (a.upper) hipLaunchKernelGGL((k_level<TX, TY, KD, RPT, S, true, true>), dim3(blocks), dim3(kBlock), 0, s, a);
+        else hipLaunchKernelGGL((k_level<TX, TY, KD, RPT, S, false, true>), dim3(blocks), dim3(kBlock), 0, s, a);
+    } else {
+        if (a.upper) hipLaunchKernelGGL((k_level<TX, TY, KD, RPT, S, true, false>), dim3(blocks), dim3(kBlock), 0, s, a);
+        else hipLaunchKernelGGL((k_level<TX, TY, KD, RPT, S, false, false>), dim3(blocks), dim3(kBlock), 0, s, a);
+    }
 }
 
 template <int RPT, Storage S>
@@ -385,6 +494,26 @@ hipError_t launch_level0_gather(const LevelArgs &a, Storage st, float *irr, floa
     return hipErrorInvalidValue;
 }
 
+hipError_t launch_march_all(const MarchAllArgs &a, int spec, hipStream_t s)
+{
+    const int blocks = a.lev[a.nlev - 1].block_begin + a.lev[a.nlev - 1].blocks;
+    if (blocks <= 0) return hipSuccess;
+    switch (spec) {
+    case 1: hipLaunchKernelGGL(k_march_all<1>, dim3(blocks), dim3(kBlock), 0, s, a); break;
+    case 2: hipLaunchKernelGGL(k_march_all<2>, dim3(blocks), dim3(kBlock), 0, s, a); break;
+    case 4: hipLaunchKernelGGL(k_march_all<4>, dim3(blocks), dim3(kBlock), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+int march_all_blocks(int PX, int PY, int N)
+{
+    const int KD = N < kBlock ? N : kBlock, probes = kBlock / KD;
+    const int tx = probes == 16 ? 4 : probes == 4 ? 2 : 1;
+    return ((PX + tx - 1) / tx) * ((PY + tx - 1) / tx) * (N / KD);
+}
+
 hipError_t launch_level_to_f32(const void *level, Storage st, size_t texels, float *out, hipStream_t s)
 {
     const unsigned blocks = (unsigned)((texels + kBlock - 1) / kBlock);
@@ -397,12 +526,17 @@ hipError_t launch_level_to_f32(const void *level, Storage st, size_t texels, flo
     return hipGetLastError();
 }
 
-hipError_t launch_march_probe(const LevelArgs &a, int32_t *hit, int64_t *index, hipStream_t s)
+hipError_t launch_march_probe(const LevelArgs &a, int spec, int32_t *hit, int64_t *index, hipStream_t s)
 {
     const int64_t total = (int64_t)a.PX * a.PY * a.N;
     const unsigned blocks = (unsigned)((total + kBlock - 1) / kBlock);
     if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_march_probe, dim3(blocks), dim3(kBlock), 0, s, a, hit, index);
+    switch (spec) {
+    case 1: hipLaunchKernelGGL(k_march_probe<1>, dim3(blocks), dim3(kBlock), 0, s, a, hit, index); break;
+    case 2: hipLaunchKernelGGL(k_march_probe<2>, dim3(blocks), dim3(kBlock), 0, s, a, hit, index); break;
+    case 4: hipLaunchKernelGGL(k_march_probe<4>, dim3(blocks), dim3(kBlock), 0, s, a, hit, index); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

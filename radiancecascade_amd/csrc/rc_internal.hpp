@@ -41,12 +41,29 @@ struct LevelArgs {
     int level, PX, PY, N;    // this level
     int UX, UY;              // probe grid of level i+1
     float t0, t1, spacing;
+    const int *codes;        // pre-marched ray codes of this level, or nullptr (march in place)
 };
+
+// k_march_all: every level >= 1 in one launch (level-major, top level first).
+struct MarchLevel {
+    const float2 *dirs;
+    int *codes;              // [PY][PX][N] ray codes
+    int PX, PY, N;
+    float t0, t1, spacing;
+    int block_begin, blocks;
+};
+struct MarchAllArgs {
+    const float *sdf;
+    int W, H, nlev;
+    MarchLevel lev[RC_MAX_LEVELS];
+};
+int march_all_blocks(int PX, int PY, int N);
+hipError_t launch_march_all(const MarchAllArgs &a, int spec, hipStream_t s);
 
 hipError_t launch_level(const LevelArgs &a, Storage st, int shape, int rays_per_lane, hipStream_t s);
 hipError_t launch_level0_gather(const LevelArgs &a, Storage st, float *irradiance, float *debug_fluence,
                                 hipStream_t s);
 hipError_t launch_level_to_f32(const void *level, Storage st, size_t texels, float *out, hipStream_t s);
-hipError_t launch_march_probe(const LevelArgs &a, int32_t *hit, int64_t *index, hipStream_t s);
+hipError_t launch_march_probe(const LevelArgs &a, int spec, int32_t *hit, int64_t *index, hipStream_t s);
 
 }  // namespace rc

@@ -117,6 +117,14 @@ class RadianceCascadeStage:
     def set_tiling(self, level: int, shape: int, rays_per_lane: int) -> None:
         check(self._lib.rc_set_level_tiling(self._ctx, level, shape, rays_per_lane), self._ctx)
 
+    def set_march(self, split: bool, speculation: int = 1) -> None:
+        check(self._lib.rc_set_march(self._ctx, int(split), speculation), self._ctx)
+
+    def march(self) -> tuple[bool, int]:
+        sp, k = ctypes.c_uint32(), ctypes.c_uint32()
+        check(self._lib.rc_get_march(self._ctx, ctypes.byref(sp), ctypes.byref(k)), self._ctx)
+        return bool(sp.value), k.value
+
     def set_jfa_fuse(self, fused_step: int) -> None:
         check(self._lib.rc_set_jfa_fuse(self._ctx, fused_step), self._ctx)
 
@@ -150,7 +158,8 @@ class RadianceCascadeStage:
     def stats(self) -> StageStats:
         s = _abi.Stats()
         check(self._lib.rc_get_stats(self._ctx, ctypes.byref(s)), self._ctx)
-        names = {_abi.STAGE_SEED: "seed", _abi.STAGE_JFA: "jfa", _abi.STAGE_JFA_TAIL: "jfa_tail"}
+        names = {_abi.STAGE_SEED: "seed", _abi.STAGE_JFA: "jfa", _abi.STAGE_JFA_TAIL: "jfa_tail",
+                 _abi.STAGE_MARCH: "march"}
         for i in range(_abi.MAX_LEVELS):
             names[_abi.STAGE_LEVEL0 + i] = f"level{i}"
         ms = {names[i]: s.stage_ms[i] for i in range(_abi.NUM_STAGES) if s.stage_launches[i]}

@@ -191,16 +191,18 @@ def _device_gbuffer(rc, W, H, seed, nb, ne):
     return depth, emis
 
 
-def test_every_tiling_is_bit_exact(rc, oracle):
+@pytest.mark.parametrize("split", [0, 1])
+def test_every_tiling_is_bit_exact(rc, oracle, split):
     """Every (tile shape, rays per lane) of every level computes the same
     frame: the tiling only changes which lane marches which ray."""
     W, H, L, seed = 640, 360, 6, 1
-    _say(f"tilings W={W} H={H} L={L}")
+    _say(f"tilings W={W} H={H} L={L} split={split}")
     depth, emis = _device_gbuffer(rc, W, H, seed, 32, 4)
     d, _, e = oracle.gbuffer(seed, W, H, 32, 4)
     ref = oracle.run(d, e, L, np.float32)
     irr = torch.empty((3, H, W), dtype=torch.float32, device="cuda:0")
     with rc.RadianceCascadeStage(W, H, levels=L) as st:
+        st.set_march(split, 1)
         default = st.tiling()
         n = 0
         for i in range(1, L):
@@ -269,3 +271,36 @@ def test_every_jfa_fusion_is_bit_exact(rc, oracle, W, H):
             assert np.array_equal(sdf.cpu().numpy().view(np.uint32), sdf_ref.view(np.uint32)), fuse
         with pytest.raises(rc.RcError):
             st.set_jfa_fuse(3)
+
+
+@pytest.mark.parametrize("W,H,L,seed", [(1920, 1080, 6, 0), (512, 384, 5, 1), (97, 61, 6, 3)])
+def test_every_march_mode_is_bit_exact(rc, oracle, W, H, L, seed):
+    """Per-level march+merge, or one split march of all levels (speculation
+    depth 1, 2, 4) then top-down merges: identical hit codes, levels and
+    irradiance."""
+    _say(f"march modes W={W} H={H} L={L} seed={seed}")
+    depth, emis = _device_gbuffer(rc, W, H, seed, 32, 4)
+    d, _, e = oracle.gbuffer(seed, W, H, 32, 4)
+    irr32, lv32, seeds = oracle.run(d, e, L, np.float32, want_levels=True, want_seeds=True)
+    sdf = oracle.sdf_f32(seeds)
+    hits_ref = [oracle.march_level(sdf, e, i)[1].ravel() for i in range(L)]
+    irr = torch.empty((3, H, W), dtype=torch.float32, device="cuda:0")
+    with rc.RadianceCascadeStage(W, H, levels=L, debug=True) as st:
+        for split, spec in [(0, 1), (1, 1), (1, 2), (1, 4)]:
+            st.set_march(split, spec)
+            assert st.march() == (bool(split), spec)
+            st.run_device(depth, emis, irr)
+            torch.cuda.synchronize()
+            assert np.array_equal(irr.cpu().numpy().view(np.uint32), irr32.view(np.uint32)), (split, spec)
+            for i in range(L):
+                lv = torch.empty(st.level_shape(i), dtype=torch.float32, device="cuda:0")
+                st.debug_level(i, lv)
+                n = rc.level_rays(W, H, i)
+                h = torch.empty(n, dtype=torch.int32, device="cuda:0")
+                ix = torch.empty(n, dtype=torch.int64, device="cuda:0")
+                st.debug_march(i, h, ix)
+                torch.cuda.synchronize()
+                assert np.array_equal(lv.cpu().numpy().view(np.uint32), lv32[i].view(np.uint32)), (split, spec, i)
+                assert np.array_equal(h.cpu().numpy(), hits_ref[i]), (split, spec, i)
+        with pytest.raises(rc.RcError):
+            st.set_march(1, 3)
