@@ -135,8 +135,8 @@ def main():
     for lvl, ((tx, ty, kd), rpt) in tiling.items():
         shape = [k for k, v in rc._abi.TILE_SHAPES.items() if v == (tx, ty, kd)][0]
         prof.set_tiling(lvl, shape, rpt)
-    split, spec = stage.march()
-    prof.set_march(split, spec)
+    split, spec, rpl = stage.march()
+    prof.set_march(split, spec, rpl)
     prof.set_jfa_fuse(stage.jfa_fuse())
     for _ in range(5):
         prof.run_device(depth, emis, irr, stream)
@@ -177,7 +177,7 @@ def main():
                    "path": "plain", "width": W, "height": H, "levels": L, "storage": args.storage,
                    "rays_per_frame": int(sum(rays)),
                    "tiling": {str(k): f"{v[0][0]}x{v[0][1]}x{v[0][2]} r{v[1]}" for k, v in tiling.items()},
-                   "march": {"split": split, "speculation": spec}, "jfa_fuse": stage.jfa_fuse()},
+                   "march": {"split": split, "speculation": spec, "rays_per_lane": rpl}, "jfa_fuse": stage.jfa_fuse()},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "alg_bytes_per_launch": alg[dominant], "mean_launch_ms": dom_ms},

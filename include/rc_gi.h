@@ -137,10 +137,11 @@ uint32_t rc_get_jfa_fuse(const rc_ctx *ctx);
  * the longest rays start earliest and the levels' dependent chains overlap),
  * then the levels are merged top-down from the stored ray codes; split = 0:
  * each level kernel marches and merges its own rays.  speculation (1, 2, 4)
- * is the split march's look-ahead depth.  The result is identical; rc_autotune
- * also picks these. */
-rc_status rc_set_march(rc_ctx *ctx, uint32_t split, uint32_t speculation);
-rc_status rc_get_march(const rc_ctx *ctx, uint32_t *split, uint32_t *speculation);
+ * is the split march's look-ahead depth; rays_per_lane (1, 2, 4, 8) how many
+ * rays a lane of the split march takes one after another.  The result is
+ * identical; rc_autotune also picks these. */
+rc_status rc_set_march(rc_ctx *ctx, uint32_t split, uint32_t speculation, uint32_t rays_per_lane);
+rc_status rc_get_march(const rc_ctx *ctx, uint32_t *split, uint32_t *speculation, uint32_t *rays_per_lane);
 
 /* ---- introspection -------------------------------------------------------- */
 rc_status rc_get_config(const rc_ctx *ctx, rc_config *out);
@@ -165,6 +166,14 @@ rc_status rc_debug_level(rc_ctx *ctx, uint32_t level, float *dev_out, void *stre
  * canonical order ((py*Px+px)*N+k) the hit pixel y*W+x, -1 (left the image)
  * or -2 (reached the interval end); and the ray's canonical index. */
 rc_status rc_debug_march(rc_ctx *ctx, uint32_t level, int32_t *hit_dev, int64_t *index_dev, void *stream);
+
+/* ---- error norms (FP64 reductions on the GPU) ------------------------------
+ * x: n fp32 values; ref: n values, fp64 (ref_is_f64 = 1) or fp32.  out_dev
+ * (3 doubles, device): sum (x-ref)^2, sum ref^2, max |x-ref|; rel-L2 =
+ * sqrt(out[0]/out[1]).  Deterministic (fixed two-stage order).  Uses the
+ * context's workspace, so calls on one context are stream ordered. */
+rc_status rc_error_norms(rc_ctx *ctx, const float *x, const void *ref, uint32_t ref_is_f64, uint64_t n,
+                         double *out_dev, void *stream);
 
 /* ---- cascade geometry (integer, identical to the kernels' indexing) ------- */
 rc_status rc_probe_grid(uint32_t width, uint32_t height, uint32_t level, uint32_t *px, uint32_t *py);

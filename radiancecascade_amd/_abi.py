@@ -73,8 +73,8 @@ SIGNATURES = {
     "rc_autotune": (_i32, [_P, ctypes.POINTER(GBuffer), _P, _u32, _P]),
     "rc_set_jfa_fuse": (_i32, [_P, _u32]),
     "rc_get_jfa_fuse": (_u32, [_P]),
-    "rc_set_march": (_i32, [_P, _u32, _u32]),
-    "rc_get_march": (_i32, [_P, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
+    "rc_set_march": (_i32, [_P, _u32, _u32, _u32]),
+    "rc_get_march": (_i32, [_P, ctypes.POINTER(_u32), ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
     "rc_get_config": (_i32, [_P, ctypes.POINTER(Config)]),
     "rc_get_stats": (_i32, [_P, ctypes.POINTER(Stats)]),
     "rc_reset_stats": (_i32, [_P]),
@@ -86,6 +86,7 @@ SIGNATURES = {
     "rc_debug_sdf": (_i32, [_P, _P, _P]),
     "rc_debug_level": (_i32, [_P, _u32, _P, _P]),
     "rc_debug_march": (_i32, [_P, _u32, _P, _P, _P]),
+    "rc_error_norms": (_i32, [_P, _P, _P, _u32, _u64, _P, _P]),
     "rc_probe_grid": (_i32, [_u32, _u32, _u32, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
     "rc_level_rays": (_u64, [_u32, _u32, _u32]),
     "rc_levels_for_coverage": (_u32, [_u32, _u32]),

@@ -36,8 +36,10 @@ struct rc_ctx {
     // then merged top-down; otherwise each level kernel marches its own rays
     int march_split = 1;
     int spec = 1;      // speculation depth of the split march (1, 2, 4)
+    int march_m = 1;   // rays per lane, marched back to back (1, 2, 4, 8)
     int *codes = nullptr;
     size_t code_off[RC_MAX_LEVELS] = {};
+    double *norm_ws = nullptr;  // error-norm partials
     int dbg_seed_buf = -1;
     bool ran = false;
     // host-staging path (rc_update_host / rc_render_host)
@@ -115,6 +117,8 @@ static void release(rc_ctx *c)
     c->dirs = nullptr;
     if (c->codes) hipFree(c->codes);
     c->codes = nullptr;
+    if (c->norm_ws) hipFree(c->norm_ws);
+    c->norm_ws = nullptr;
     if (c->g_depth) hipFree(c->g_depth);
     if (c->g_emissive) hipFree(c->g_emissive);
     if (c->irr) hipFree(c->irr);
@@ -159,6 +163,7 @@ static rc_status allocate(rc_ctx *c, uint32_t W, uint32_t H, uint32_t L)
         ncodes += (size_t)grid_dim(W, i) * grid_dim(H, i) * num_dirs(i);
     }
     if (ncodes) HIP_TRY(c, hipMalloc(&c->codes, ncodes * sizeof(int)));
+    HIP_TRY(c, hipMalloc(&c->norm_ws, norm_workspace_bytes()));
     HIP_TRY(c, hipMalloc(&c->dirs, dirs.size() * sizeof(float2)));
     HIP_TRY(c, hipMemcpy(c->dirs, dirs.data(), dirs.size() * sizeof(float2), hipMemcpyHostToDevice));
     for (int i = 0; i < RC_MAX_LEVELS; ++i) {
@@ -208,8 +213,11 @@ static MarchAllArgs march_all_args(const rc_ctx *c)
         l.t0 = (float)interval_t0(i);
         l.t1 = (float)interval_t1(i);
         l.spacing = (float)probe_spacing(i);
+        l.log2n = 2 + 2 * i;
+        l.inv_px = 1.0f / (float)l.PX;
         l.block_begin = blocks;
-        l.blocks = march_all_blocks(l.PX, l.PY, l.N);
+        const int64_t rays = (int64_t)l.PX * l.PY * l.N, per = (int64_t)kBlock * c->march_m;
+        l.blocks = (int)((rays + per - 1) / per);
         blocks += l.blocks;
     }
     return m;
@@ -268,7 +276,7 @@ static rc_status run_frame(rc_ctx *c, const float *depth, const float *emissive,
     const bool split = c->march_split && c->L > 1;
     if (split) {
         tm.begin(RC_STAGE_MARCH);
-        HIP_TRY(c, launch_march_all(march_all_args(c), c->spec, s));
+        HIP_TRY(c, launch_march_all(march_all_args(c), c->spec, c->march_m, s));
         tm.end(RC_STAGE_MARCH);
     }
     for (int i = c->L - 1; i >= 1; --i) {
@@ -565,22 +573,25 @@ rc_status rc_set_jfa_fuse(rc_ctx *c, uint32_t fused_step)
 
 uint32_t rc_get_jfa_fuse(const rc_ctx *c) { return c ? (uint32_t)c->jfa_fuse : 0u; }
 
-rc_status rc_set_march(rc_ctx *c, uint32_t split, uint32_t speculation)
+rc_status rc_set_march(rc_ctx *c, uint32_t split, uint32_t speculation, uint32_t rays_per_lane)
 {
     if (!c) return fail(nullptr, RC_ERR_NULL, "rc_set_march: ctx is NULL");
-    if (split > 1 || (speculation != 1 && speculation != 2 && speculation != 4))
-        return fail(c, RC_ERR_INVALID, "rc_set_march: split must be 0/1, speculation 1, 2 or 4");
+    if (split > 1 || (speculation != 1 && speculation != 2 && speculation != 4) ||
+        (rays_per_lane != 1 && rays_per_lane != 2 && rays_per_lane != 4 && rays_per_lane != 8))
+        return fail(c, RC_ERR_INVALID, "rc_set_march: split 0/1, speculation 1/2/4, rays_per_lane 1/2/4/8");
     c->march_split = (int)split;
     c->spec = (int)speculation;
+    c->march_m = (int)rays_per_lane;
     drop_graph(c);
     return RC_OK;
 }
 
-rc_status rc_get_march(const rc_ctx *c, uint32_t *split, uint32_t *speculation)
+rc_status rc_get_march(const rc_ctx *c, uint32_t *split, uint32_t *speculation, uint32_t *rays_per_lane)
 {
-    if (!c || !split || !speculation) return fail(nullptr, RC_ERR_NULL, "rc_get_march: NULL argument");
+    if (!c || !split || !speculation || !rays_per_lane) return fail(nullptr, RC_ERR_NULL, "rc_get_march: NULL argument");
     *split = (uint32_t)c->march_split;
     *speculation = (uint32_t)c->spec;
+    *rays_per_lane = (uint32_t)c->march_m;
     return RC_OK;
 }
 
@@ -630,12 +641,14 @@ rc_status rc_autotune(rc_ctx *c, const rc_gbuffer *g, float *irr, uint32_t iters
     }
     {
         float best = 1e30f;
-        int best_split = c->march_split, best_spec = c->spec;
-        for (int split : {0, 1}) {
-            for (int spec : {1, 2, 4}) {
-                if (!split && spec != 1) continue;
+        int best_split = c->march_split, best_spec = c->spec, best_m = c->march_m;
+        for (int split : {0, 1})
+        for (int spec : {1, 2, 4})
+            for (int m : {1, 2, 4, 8}) {
+                if (!split && (spec != 1 || m != 1)) continue;
                 c->march_split = split;
                 c->spec = spec;
+                c->march_m = m;
                 HIP_TRY(c, hipEventRecord(e0, s));
                 for (uint32_t it = 0; it < iters; ++it) {
                     rc_status r = run_frame(c, g->depth, g->emissive, irr, s, true);
@@ -645,12 +658,12 @@ rc_status rc_autotune(rc_ctx *c, const rc_gbuffer *g, float *irr, uint32_t iters
                 HIP_TRY(c, hipEventSynchronize(e1));
                 float ms = 0.f;
                 HIP_TRY(c, hipEventElapsedTime(&ms, e0, e1));
-                if (ms < best) { best = ms; best_split = split; best_spec = spec; }
+                if (ms < best) { best = ms; best_split = split; best_spec = spec; best_m = m; }
             }
-        }
         c->march_split = best_split;
         c->spec = best_spec;
-        if (best_split) HIP_TRY(c, launch_march_all(march_all_args(c), c->spec, s));
+        c->march_m = best_m;
+        if (best_split) HIP_TRY(c, launch_march_all(march_all_args(c), c->spec, c->march_m, s));
     }
     for (int i = 1; i < c->L; ++i) {
         float best = 1e30f;
@@ -676,6 +689,16 @@ rc_status rc_autotune(rc_ctx *c, const rc_gbuffer *g, float *irr, uint32_t iters
     drop_graph(c);
     // leave the level buffers consistent with one full frame
     return run_frame(c, g->depth, g->emissive, irr, s);
+}
+
+rc_status rc_error_norms(rc_ctx *c, const float *x, const void *ref, uint32_t ref_is_f64, uint64_t n,
+                         double *out_dev, void *stream)
+{
+    if (!c || !x || !ref || !out_dev) return fail(c, RC_ERR_NULL, "rc_error_norms: NULL argument");
+    if (n == 0) return fail(c, RC_ERR_INVALID, "rc_error_norms: empty input");
+    HIP_TRY(c, hipSetDevice(c->cfg.device));
+    HIP_TRY(c, launch_error_norms(x, ref, ref_is_f64 != 0, (size_t)n, c->norm_ws, out_dev, (hipStream_t)stream));
+    return RC_OK;
 }
 
 rc_status rc_probe_grid(uint32_t W, uint32_t H, uint32_t level, uint32_t *px, uint32_t *py)

@@ -53,11 +53,17 @@ __device__ inline __amdgpu_buffer_rsrc_t sdf_rsrc(const float *sdf, int W, int H
                                              kBufferDword3);
 }
 
+// Row-major pixel index q = y*W + x (24-bit multiply: W, y < 2^14) -- the
+// hit code and the emissive index.
+__device__ inline int pixel_index(int x, int y, int W) { return (int)__umul24((uint32_t)y, (uint32_t)W) + x; }
+
+// Tiled SDF load.  For an off-image (x, y) the offset may be anything; the
+// range-checked load is then unused.
 __device__ inline float sdf_load(__amdgpu_buffer_rsrc_t r, int x, int y, int tiles_x)
 {
     const uint32_t ux = (uint32_t)x, uy = (uint32_t)y;
-    const uint32_t idx = ((__umul24(uy >> 2, (uint32_t)tiles_x) + (ux >> 3)) << 5) + ((uy & 3u) << 3) + (ux & 7u);
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)(idx * 4u), 0, 0));
+    const uint32_t off = (__umul24(uy >> 2, (uint32_t)tiles_x << 5) + ((ux & ~7u) << 2) + ((uy & 3u) << 3) + (ux & 7u)) << 2;
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
 }
 
 // Ray codes: >= 0 hit pixel y*W+x, -1 left the image, -2 reached t1.
@@ -72,8 +78,6 @@ template <int RPT>
 __device__ inline void march_rays(__amdgpu_buffer_rsrc_t sdf, int W, int H, const float *cx, const float *cy,
                                   float dx, float dy, float t0, float t1, const bool *active, int *code)
 {
-    const float fW = (float)W, fH = (float)H;
-    const int tiles_x = sdf_tiles_x(W);
     float t[RPT];
     bool alive[RPT];
 #pragma unroll
@@ -83,25 +87,25 @@ __device__ inline void march_rays(__amdgpu_buffer_rsrc_t sdf, int W, int H, cons
         code[j] = kRayT1;
     }
     for (;;) {
-        int ix[RPT], iy[RPT];
+        int q[RPT];
         float d[RPT];
 #pragma unroll
         for (int j = 0; j < RPT; ++j) {
-            const float x = floorf(fmaf(t[j], dx, cx[j]));
-            const float y = floorf(fmaf(t[j], dy, cy[j]));
-            const bool inside = x >= 0.f && y >= 0.f && x < fW && y < fH;
+            // positions are finite, so the integer test equals 0 <= x < W, 0 <= y < H
+            const int ix = (int)floorf(fmaf(t[j], dx, cx[j]));
+            const int iy = (int)floorf(fmaf(t[j], dy, cy[j]));
+            const bool inside = (uint32_t)ix < (uint32_t)W && (uint32_t)iy < (uint32_t)H;
             if (alive[j] && !inside) code[j] = kRayExit;
             alive[j] = alive[j] && inside;
-            ix[j] = (int)x;
-            iy[j] = (int)y;
-            d[j] = sdf_load(sdf, ix[j], iy[j], tiles_x);
+            q[j] = pixel_index(ix, iy, W);
+            d[j] = sdf_load(sdf, ix, iy, sdf_tiles_x(W));
         }
         bool any = false;
 #pragma unroll
         for (int j = 0; j < RPT; ++j) {
             if (alive[j]) {
                 if (d[j] < 0.5f) {
-                    code[j] = iy[j] * W + ix[j];
+                    code[j] = q[j];
                     alive[j] = false;
                 } else {
                     t[j] = t[j] + (d[j] > 1.0f ? d[j] : 1.0f);
@@ -114,69 +118,79 @@ __device__ inline void march_rays(__amdgpu_buffer_rsrc_t sdf, int W, int H, cons
     }
 }
 
-// One ray per lane with SPEC-deep speculation.  After a sample with distance
-// d the next sample is exactly t + max(d, 1); a ray grazing a surface keeps
-// taking the same step, so each round also loads the SPEC-1 samples that
-// follow if the step stays unchanged (positions built with the same fp32
-// additions the sequential march would make).  The samples are then checked
-// in order; the run of accepted ones ends at the first sample whose step
-// differs, which starts the next round.  The visited sample sequence, and so
-// every result bit, is that of the one-sample-per-step march.
+// One ray with SPEC-deep speculation.  After a sample with distance d the
+// next sample is exactly t + max(d, 1); a ray grazing a surface keeps taking
+// the same step, so each round also loads the SPEC-1 samples that follow if
+// the step stays unchanged (positions built with the same fp32 additions the
+// sequential march would make).  The samples are then checked in order; the
+// run of accepted ones ends at the first sample whose step differs, which
+// starts the next round.  The visited sample sequence, and so every result
+// bit, is that of the one-sample-per-step march.
+struct Ray {
+    float cx, cy, dx, dy, t, guess;
+    int code;
+    bool alive;
+};
+
+__device__ inline Ray ray_begin(float cx, float cy, float dx, float dy, float t0, float t1, bool active)
+{
+    return Ray{cx, cy, dx, dy, t0, 1.0f, kRayT1, active && t0 < t1};
+}
+
+template <int SPEC>
+__device__ inline void ray_round(Ray &r, __amdgpu_buffer_rsrc_t sdf, int W, int H, float t1)
+{
+    float p[SPEC], d[SPEC];
+    int q[SPEC];
+    bool ins[SPEC];
+    p[0] = r.t;
+#pragma unroll
+    for (int j = 1; j < SPEC; ++j) p[j] = p[j - 1] + r.guess;
+#pragma unroll
+    for (int j = 0; j < SPEC; ++j) {
+        const int ix = (int)floorf(fmaf(p[j], r.dx, r.cx));
+        const int iy = (int)floorf(fmaf(p[j], r.dy, r.cy));
+        ins[j] = (uint32_t)ix < (uint32_t)W && (uint32_t)iy < (uint32_t)H;
+        q[j] = pixel_index(ix, iy, W);
+        d[j] = sdf_load(sdf, ix, iy, sdf_tiles_x(W));
+    }
+    bool go = true;
+#pragma unroll
+    for (int j = 0; j < SPEC; ++j) {
+        if (!go) continue;
+        if (j > 0 && !(p[j] < t1)) {  // the previous step reached the interval end
+            r.alive = false;
+            go = false;
+            continue;
+        }
+        if (!ins[j]) {
+            r.code = kRayExit;
+            r.alive = false;
+            go = false;
+            continue;
+        }
+        if (d[j] < 0.5f) {
+            r.code = q[j];
+            r.alive = false;
+            go = false;
+            continue;
+        }
+        const float step = d[j] > 1.0f ? d[j] : 1.0f;
+        if (j + 1 < SPEC && step == r.guess) continue;  // p[j+1] == p[j] + step
+        r.t = p[j] + step;
+        r.guess = step;
+        r.alive = r.t < t1;
+        go = false;
+    }
+}
+
 template <int SPEC>
 __device__ inline int march_ray_spec(__amdgpu_buffer_rsrc_t sdf, int W, int H, float cx, float cy, float dx,
                                      float dy, float t0, float t1, bool active)
 {
-    const float fW = (float)W, fH = (float)H;
-    const int tiles_x = sdf_tiles_x(W);
-    int code = kRayT1;
-    bool alive = active && t0 < t1;
-    float t = t0, guess = 1.0f;
-    while (alive) {
-        float p[SPEC], d[SPEC];
-        int ix[SPEC], iy[SPEC];
-        bool ins[SPEC];
-        p[0] = t;
-#pragma unroll
-        for (int j = 1; j < SPEC; ++j) p[j] = p[j - 1] + guess;
-#pragma unroll
-        for (int j = 0; j < SPEC; ++j) {
-            const float x = floorf(fmaf(p[j], dx, cx));
-            const float y = floorf(fmaf(p[j], dy, cy));
-            ins[j] = x >= 0.f && y >= 0.f && x < fW && y < fH;
-            ix[j] = (int)x;
-            iy[j] = (int)y;
-            d[j] = sdf_load(sdf, ix[j], iy[j], tiles_x);
-        }
-        bool go = true;
-#pragma unroll
-        for (int j = 0; j < SPEC; ++j) {
-            if (!go) continue;
-            if (j > 0 && !(p[j] < t1)) {  // the previous step reached the interval end
-                alive = false;
-                go = false;
-                continue;
-            }
-            if (!ins[j]) {
-                code = kRayExit;
-                alive = false;
-                go = false;
-                continue;
-            }
-            if (d[j] < 0.5f) {
-                code = iy[j] * W + ix[j];
-                alive = false;
-                go = false;
-                continue;
-            }
-            const float step = d[j] > 1.0f ? d[j] : 1.0f;
-            if (j + 1 < SPEC && step == guess) continue;  // p[j+1] == p[j] + step
-            t = p[j] + step;
-            guess = step;
-            alive = t < t1;
-            go = false;
-        }
-    }
-    return code;
+    Ray r = ray_begin(cx, cy, dx, dy, t0, t1, active);
+    while (r.alive) ray_round<SPEC>(r, sdf, W, H, t1);
+    return r.code;
 }
 
 // Bilinear of four staged texels with the lower probe's parity weights.
@@ -372,31 +386,50 @@ __global__ __launch_bounds__(kBlock) void k_level0(LevelArgs a, float *__restric
 // kernels.  The block list is level-major with the top level first, so the
 // levels whose rays are longest start earliest, and the long dependent chains
 // of different levels run concurrently instead of one kernel after another.
-// Blocks are 256 rays: KD = min(N, 256) consecutive directions of
-// 256/KD probes (a square-ish probe tile); blocks go round-robin to the XCDs.
-template <int SPEC>
+//
+// A workgroup owns 256*M consecutive canonical rays ((py*Px+px)*N+k, so a
+// wave is 64 consecutive directions of one probe, or 64/N whole probes) and
+// lane l marches rays base+l, base+l+256, ... one after another without
+// waiting for the other lanes: when its ray ends it starts the next one in the
+// same loop iteration (march lengths differ by two orders of magnitude, so
+// one-ray-per-lane waves idle most lanes while their longest ray runs).
+// Blocks go round-robin to the XCDs, which interleaves heavy and light tiles.
+template <int SPEC, int M>
 __global__ __launch_bounds__(kBlock) void k_march_all(MarchAllArgs a)
 {
     int lv = 0;
 #pragma unroll 1
     while (lv + 1 < a.nlev && (int)blockIdx.x >= a.lev[lv + 1].block_begin) ++lv;
     const MarchLevel &L = a.lev[lv];
-    const int b = blockIdx.x - L.block_begin;
-    const int KD = L.N < kBlock ? L.N : kBlock;
-    const int probes = kBlock / KD;                 // 1, 4 or 16 per block
-    const int tx = probes == 16 ? 4 : probes == 4 ? 2 : 1;
-    const int slices = L.N / KD;
-    const int slice = b % slices, tile = b / slices;
-    const int tiles_x = (L.PX + tx - 1) / tx;
-    const int kk = threadIdx.x % KD, pl = threadIdx.x / KD;
-    const int px = (tile % tiles_x) * tx + pl % tx, py = (tile / tiles_x) * tx + pl / tx;
-    const int k = slice * KD + kk;
-    const bool active = px < L.PX && py < L.PY;
-    const float2 dir = L.dirs[k];
-    const float cx = ((float)px + 0.5f) * L.spacing, cy = ((float)py + 0.5f) * L.spacing;
-    const int code = march_ray_spec<SPEC>(sdf_rsrc(a.sdf, a.W, a.H), a.W, a.H, cx, cy, dir.x, dir.y, L.t0, L.t1,
-                                          active);
-    if (active) L.codes[((size_t)py * L.PX + px) * L.N + k] = code;
+    const __amdgpu_buffer_rsrc_t sdf = sdf_rsrc(a.sdf, a.W, a.H);
+    const int base = ((int)blockIdx.x - L.block_begin) * kBlock * M + (int)threadIdx.x;
+    const int total = L.PX * L.PY * L.N;
+
+    int j = 0, ray = base;
+    Ray r;
+    auto start = [&]() {
+        const bool active = ray < total;
+        const int rr = active ? ray : 0;
+        const int k = rr & (L.N - 1), probe = rr >> L.log2n;
+        // probe / PX by an fp32 reciprocal, corrected to the exact quotient
+        int py = (int)((float)probe * L.inv_px);
+        int px = probe - py * L.PX;
+        if (px < 0) { --py; px += L.PX; }
+        else if (px >= L.PX) { ++py; px -= L.PX; }
+        const float2 dir = L.dirs[k];
+        r = ray_begin(((float)px + 0.5f) * L.spacing, ((float)py + 0.5f) * L.spacing, dir.x, dir.y, L.t0, L.t1,
+                      active);
+    };
+    start();
+    for (;;) {
+        if (r.alive) ray_round<SPEC>(r, sdf, a.W, a.H, L.t1);
+        if (j < M && !r.alive) {
+            if (ray < total) L.codes[ray] = r.code;
+            ray += kBlock;
+            if (++j < M) start();
+        }
+        if (!__any(j < M)) break;
+    }
 }
 
 // March-only probe for parity tests: hit code and canonical index per ray.
@@ -494,25 +527,32 @@ hipError_t launch_level0_gather(const LevelArgs &a, Storage st, float *irr, floa
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_march_all(const MarchAllArgs &a, int spec, hipStream_t s)
+template <int SPEC>
+static hipError_t launch_march_all_spec(const MarchAllArgs &a, int m, int blocks, hipStream_t s)
 {
-    const int blocks = a.lev[a.nlev - 1].block_begin + a.lev[a.nlev - 1].blocks;
-    if (blocks <= 0) return hipSuccess;
-    switch (spec) {
-    case 1: hipLaunchKernelGGL(k_march_all<1>, dim3(blocks), dim3(kBlock), 0, s, a); break;
-    case 2: hipLaunchKernelGGL(k_march_all<2>, dim3(blocks), dim3(kBlock), 0, s, a); break;
-    case 4: hipLaunchKernelGGL(k_march_all<4>, dim3(blocks), dim3(kBlock), 0, s, a); break;
+    switch (m) {
+    case 1: hipLaunchKernelGGL((k_march_all<SPEC, 1>), dim3(blocks), dim3(kBlock), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_march_all<SPEC, 2>), dim3(blocks), dim3(kBlock), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((k_march_all<SPEC, 4>), dim3(blocks), dim3(kBlock), 0, s, a); break;
+    case 8: hipLaunchKernelGGL((k_march_all<SPEC, 8>), dim3(blocks), dim3(kBlock), 0, s, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-int march_all_blocks(int PX, int PY, int N)
+hipError_t launch_march_all(const MarchAllArgs &a, int spec, int m, hipStream_t s)
 {
-    const int KD = N < kBlock ? N : kBlock, probes = kBlock / KD;
-    const int tx = probes == 16 ? 4 : probes == 4 ? 2 : 1;
-    return ((PX + tx - 1) / tx) * ((PY + tx - 1) / tx) * (N / KD);
+    const int blocks = a.lev[a.nlev - 1].block_begin + a.lev[a.nlev - 1].blocks;
+    if (blocks <= 0) return hipSuccess;
+    switch (spec) {
+    case 1: return launch_march_all_spec<1>(a, m, blocks, s);
+    case 2: return launch_march_all_spec<2>(a, m, blocks, s);
+    case 4: return launch_march_all_spec<4>(a, m, blocks, s);
+    }
+    return hipErrorInvalidValue;
 }
+
+
 
 hipError_t launch_level_to_f32(const void *level, Storage st, size_t texels, float *out, hipStream_t s)
 {

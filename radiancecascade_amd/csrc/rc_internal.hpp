@@ -48,22 +48,27 @@ struct LevelArgs {
 struct MarchLevel {
     const float2 *dirs;
     int *codes;              // [PY][PX][N] ray codes
-    int PX, PY, N;
+    int PX, PY, N, log2n;
+    float inv_px;            // 1/PX in fp32 (probe -> row, corrected)
     float t0, t1, spacing;
-    int block_begin, blocks;
+    int block_begin, blocks; // workgroups of 256*M canonical rays
 };
 struct MarchAllArgs {
     const float *sdf;
     int W, H, nlev;
     MarchLevel lev[RC_MAX_LEVELS];
 };
-int march_all_blocks(int PX, int PY, int N);
-hipError_t launch_march_all(const MarchAllArgs &a, int spec, hipStream_t s);
+hipError_t launch_march_all(const MarchAllArgs &a, int speculation, int rays_per_lane, hipStream_t s);
 
 hipError_t launch_level(const LevelArgs &a, Storage st, int shape, int rays_per_lane, hipStream_t s);
 hipError_t launch_level0_gather(const LevelArgs &a, Storage st, float *irradiance, float *debug_fluence,
                                 hipStream_t s);
 hipError_t launch_level_to_f32(const void *level, Storage st, size_t texels, float *out, hipStream_t s);
 hipError_t launch_march_probe(const LevelArgs &a, int spec, int32_t *hit, int64_t *index, hipStream_t s);
+
+// ---- error norms (rc_norms.hip) ---------------------------------------------
+size_t norm_workspace_bytes();
+hipError_t launch_error_norms(const float *x, const void *ref, bool ref_f64, size_t n, double *workspace,
+                              double *out, hipStream_t s);
 
 }  // namespace rc

@@ -117,13 +117,13 @@ class RadianceCascadeStage:
     def set_tiling(self, level: int, shape: int, rays_per_lane: int) -> None:
         check(self._lib.rc_set_level_tiling(self._ctx, level, shape, rays_per_lane), self._ctx)
 
-    def set_march(self, split: bool, speculation: int = 1) -> None:
-        check(self._lib.rc_set_march(self._ctx, int(split), speculation), self._ctx)
+    def set_march(self, split: bool, speculation: int = 1, rays_per_lane: int = 1) -> None:
+        check(self._lib.rc_set_march(self._ctx, int(split), speculation, rays_per_lane), self._ctx)
 
-    def march(self) -> tuple[bool, int]:
-        sp, k = ctypes.c_uint32(), ctypes.c_uint32()
-        check(self._lib.rc_get_march(self._ctx, ctypes.byref(sp), ctypes.byref(k)), self._ctx)
-        return bool(sp.value), k.value
+    def march(self) -> tuple[bool, int, int]:
+        sp, k, m = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        check(self._lib.rc_get_march(self._ctx, ctypes.byref(sp), ctypes.byref(k), ctypes.byref(m)), self._ctx)
+        return bool(sp.value), k.value, m.value
 
     def set_jfa_fuse(self, fused_step: int) -> None:
         check(self._lib.rc_set_jfa_fuse(self._ctx, fused_step), self._ctx)
@@ -192,6 +192,21 @@ class RadianceCascadeStage:
         rays = level_rays(self.width, self.height, level)
         check(self._lib.rc_debug_march(self._ctx, level, _dev_ptr(hit, rays * 4, "hit"),
                                        _dev_ptr(index, rays * 8, "index"), _stream_handle(stream)), self._ctx)
+
+    def error_norms(self, x, ref, stream=None) -> dict:
+        """rel-L2 and max-abs of device fp32 `x` against device `ref` (fp32 or
+        fp64 tensor of the same length), reduced in FP64 on the GPU."""
+        import torch
+        n = x.numel()
+        if ref.numel() != n:
+            raise ValueError("x and ref differ in length")
+        out = torch.empty(3, dtype=torch.float64, device=x.device)
+        check(self._lib.rc_error_norms(self._ctx, _dev_ptr(x, n * 4, "x"),
+                                       _dev_ptr(ref, n * ref.element_size(), "ref"),
+                                       int(ref.dtype == torch.float64), n, _dev_ptr(out, 24, "out"),
+                                       _stream_handle(stream)), self._ctx)
+        sd, sr, mx = out.cpu().tolist()
+        return {"rel_l2": (sd / sr) ** 0.5 if sr > 0 else sd ** 0.5, "max_abs": mx, "sum_sq_diff": sd, "sum_sq_ref": sr}
 
     def close(self) -> None:
         if getattr(self, "_ctx", None) is not None and self._ctx.value:
