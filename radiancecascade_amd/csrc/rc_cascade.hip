@@ -66,6 +66,10 @@ __device__ inline float sdf_load(__amdgpu_buffer_rsrc_t r, int x, int y, int til
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
 }
 
+// max(d, 1) as one v_med3 (no NaN canonicalisation: d is never NaN).  For
+// d = +inf it gives FLT_MAX instead of inf; either ends the interval.
+__device__ inline float march_step(float d) { return __builtin_amdgcn_fmed3f(d, 1.0f, 3.4028235e38f); }
+
 // Ray codes: >= 0 hit pixel y*W+x, -1 left the image, -2 reached t1.
 constexpr int kRayExit = -1, kRayT1 = -2;
 
@@ -92,8 +96,8 @@ __device__ inline void march_rays(__amdgpu_buffer_rsrc_t sdf, int W, int H, cons
 #pragma unroll
         for (int j = 0; j < RPT; ++j) {
             // positions are finite, so the integer test equals 0 <= x < W, 0 <= y < H
-            const int ix = (int)floorf(fmaf(t[j], dx, cx[j]));
-            const int iy = (int)floorf(fmaf(t[j], dy, cy[j]));
+            const int ix = __float2int_rd(fmaf(t[j], dx, cx[j]));  // floor, one instruction
+            const int iy = __float2int_rd(fmaf(t[j], dy, cy[j]));
             const bool inside = (uint32_t)ix < (uint32_t)W && (uint32_t)iy < (uint32_t)H;
             if (alive[j] && !inside) code[j] = kRayExit;
             alive[j] = alive[j] && inside;
@@ -108,7 +112,7 @@ __device__ inline void march_rays(__amdgpu_buffer_rsrc_t sdf, int W, int H, cons
                     code[j] = q[j];
                     alive[j] = false;
                 } else {
-                    t[j] = t[j] + (d[j] > 1.0f ? d[j] : 1.0f);
+                    t[j] = t[j] + march_step(d[j]);
                     alive[j] = t[j] < t1;
                 }
             }
@@ -148,8 +152,8 @@ __device__ inline void ray_round(Ray &r, __amdgpu_buffer_rsrc_t sdf, int W, int 
     for (int j = 1; j < SPEC; ++j) p[j] = p[j - 1] + r.guess;
 #pragma unroll
     for (int j = 0; j < SPEC; ++j) {
-        const int ix = (int)floorf(fmaf(p[j], r.dx, r.cx));
-        const int iy = (int)floorf(fmaf(p[j], r.dy, r.cy));
+        const int ix = __float2int_rd(fmaf(p[j], r.dx, r.cx));  // floor, one instruction
+        const int iy = __float2int_rd(fmaf(p[j], r.dy, r.cy));
         ins[j] = (uint32_t)ix < (uint32_t)W && (uint32_t)iy < (uint32_t)H;
         q[j] = pixel_index(ix, iy, W);
         d[j] = sdf_load(sdf, ix, iy, sdf_tiles_x(W));
@@ -175,7 +179,7 @@ __device__ inline void ray_round(Ray &r, __amdgpu_buffer_rsrc_t sdf, int W, int 
             go = false;
             continue;
         }
-        const float step = d[j] > 1.0f ? d[j] : 1.0f;
+        const float step = march_step(d[j]);
         if (j + 1 < SPEC && step == r.guess) continue;  // p[j+1] == p[j] + step
         r.t = p[j] + step;
         r.guess = step;
