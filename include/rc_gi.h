@@ -96,7 +96,13 @@ typedef struct {
 #define RC_STAGE_JFA_TAIL 2     /* fused short-step passes + distance field */
 #define RC_STAGE_LEVEL0 3       /* RC_STAGE_LEVEL0 + i = level i march/merge; level 0 includes the gather */
 #define RC_STAGE_MARCH (RC_STAGE_LEVEL0 + RC_MAX_LEVELS) /* split march of all levels >= 1 */
-#define RC_NUM_STAGES (RC_STAGE_MARCH + 1)
+/* learned far field (rc_set_surrogate mode 1) */
+#define RC_STAGE_NEAR_SDF (RC_STAGE_MARCH + 1)  /* exact distance field up to 10 px */
+#define RC_STAGE_MIP (RC_STAGE_MARCH + 2)       /* emitter / occluder mip pyramid */
+#define RC_STAGE_CONE (RC_STAGE_MARCH + 3)      /* cone trace on the level-2 probe grid */
+#define RC_STAGE_SURROGATE (RC_STAGE_MARCH + 4) /* level-1 march + MLP + merge */
+#define RC_STAGE_FALLBACK (RC_STAGE_MARCH + 5)  /* plain far field + patch of flagged tiles */
+#define RC_NUM_STAGES (RC_STAGE_MARCH + 6)
 typedef struct {
     uint32_t version;
     uint32_t runs;
@@ -104,6 +110,12 @@ typedef struct {
     uint32_t stage_launches[RC_NUM_STAGES];
     uint64_t level_rays[RC_MAX_LEVELS];
     uint64_t level_bytes_stored[RC_MAX_LEVELS]; /* bytes of merged level i written per run */
+    /* learned far field: frames run, frames that fell back, flagged tiles
+     * (summed over frames) and tiles per frame */
+    uint64_t surrogate_frames;
+    uint64_t fallback_frames;
+    uint64_t flagged_tiles;
+    uint64_t tiles_per_frame;
 } rc_stats;
 
 typedef struct rc_ctx rc_ctx;
@@ -174,6 +186,85 @@ rc_status rc_debug_march(rc_ctx *ctx, uint32_t level, int32_t *hit_dev, int64_t 
  * context's workspace, so calls on one context are stream ordered. */
 rc_status rc_error_norms(rc_ctx *ctx, const float *x, const void *ref, uint32_t ref_is_f64, uint64_t n,
                          double *out_dev, void *stream);
+
+/* ---- learned far field ------------------------------------------------------
+ * No counterpart in the reference (it has no GI).  Mode 1 replaces levels >= 2
+ * of the frame: a cone trace through an emitter / occluder mip pyramid on the
+ * level-2 probe grid gives, per level-1 probe and direction k, a far-field
+ * estimate c_k; an MLP (48 -> 64 -> 64 -> 64 -> 64, ReLU, BF16 weights on MFMA,
+ * FP32 accumulation) maps x = standardised log1p(c) to z and the far field is
+ * B_k = max(c_k + out_scale * z_k, 0); output 48 predicts log of the squared
+ * error.  Tiles of 8x8 level-1 probes whose mean predicted error exceeds
+ * `threshold` take the plain cascades' level 1 (fallback: one full plain frame
+ * plus a patch, decided on the host after the frame).  Needs levels >= 3. */
+#define RC_MLP_INPUTS 48
+#define RC_MLP_OUTPUTS 64
+#define RC_MLP_PARAMS 15616  /* for l = 0..3: W_l [K_l][64] (K = 48, 64, 64, 64) then b_l [64] */
+#define RC_SURROGATE_BLOB_MAGIC "RCSURR01"
+typedef struct {
+    uint32_t version;   /* RC_ABI_VERSION */
+    uint32_t mode;      /* 0 plain cascades, 1 learned far field */
+    float threshold;    /* fallback threshold (normalised squared error); +inf: never */
+    uint32_t reserved;
+} rc_surrogate_config;
+rc_status rc_set_surrogate(rc_ctx *ctx, const rc_surrogate_config *cfg);
+rc_status rc_get_surrogate(const rc_ctx *ctx, rc_surrogate_config *out);
+/* Network parameters (host arrays): params RC_MLP_PARAMS fp32; in_mean,
+ * in_istd RC_MLP_INPUTS each (x = (log1p(c) - mean) * istd). */
+rc_status rc_mlp_set(rc_ctx *ctx, const float *params, const float *in_mean, const float *in_istd, float out_scale);
+rc_status rc_mlp_get(const rc_ctx *ctx, float *params, float *in_mean, float *in_istd, float *out_scale);
+/* Serialised weights: "RCSURR01", uint32 version, inputs, outputs, params,
+ * float out_scale, uint32 train_scenes, then in_mean, in_istd, params (fp32
+ * little endian).  size = bytes needed (call with buf NULL to query). */
+rc_status rc_surrogate_save(const rc_ctx *ctx, void *buf, uint64_t capacity, uint64_t *size);
+rc_status rc_surrogate_load(rc_ctx *ctx, const void *buf, uint64_t size);
+/* The network's kernels on device buffers (parity hooks for the training and
+ * inference kernels).  feats: [n][48] log1p features (standardised inside);
+ * out: [n][64] raw outputs z.  rc_mlp_gradient: targets [n][48] normalised
+ * residuals; grad: RC_MLP_PARAMS fp32 of the batch-mean loss
+ * L = sum (z - t)^2 / (48 n) + 0.05 sum (z_48 - log(mse_s + 1e-6))^2 / n;
+ * loss: 1 float.  rc_mlp_adam: one Adam step (0.9, 0.999, 1e-8) with grad. */
+rc_status rc_mlp_forward(rc_ctx *ctx, const float *feats_dev, uint32_t n, float *out_dev, void *stream);
+rc_status rc_mlp_gradient(rc_ctx *ctx, const float *feats_dev, const float *targets_dev, uint32_t n, float *grad_dev,
+                          float *loss_dev, void *stream);
+rc_status rc_mlp_adam(rc_ctx *ctx, const float *grad_dev, float lr, void *stream);
+/* Training rows of one G-buffer at the context's size: runs the plain frame,
+ * then per level-1 probe writes feats [P][48] = log1p(c) and targets [P][48]
+ * = B_plain - c (unnormalised), P = Px_1 * Py_1. */
+rc_status rc_surrogate_samples(rc_ctx *ctx, const rc_gbuffer *dev, float *feats_dev, float *targets_dev,
+                               void *stream);
+/* Train the context's network on the GPU on plain-path outputs of seeded
+ * scenes (rc_scene_generate(scene_seed0 + s, n_boxes, n_emissive)). */
+typedef struct {
+    uint32_t version;
+    uint32_t width, height, levels;  /* training G-buffer size and cascade count */
+    uint64_t scene_seed0;
+    uint32_t scenes, n_boxes, n_emissive;
+    uint32_t epochs;
+    uint32_t batch;                  /* samples per step, a multiple of 32 */
+    float lr, lr_decay;              /* step size lr * lr_decay^epoch */
+    uint32_t reserved;
+    uint64_t seed;                   /* initial weights and shuffling */
+} rc_train_config;
+#define RC_TRAIN_MAX_EPOCHS 64
+typedef struct {
+    uint32_t version;
+    uint32_t epochs;
+    uint64_t samples;                /* training rows */
+    uint64_t steps;
+    float epoch_loss[RC_TRAIN_MAX_EPOCHS];
+    double gen_ms, train_ms;         /* data generation and training wall time */
+    float out_scale;
+    uint32_t reserved;
+} rc_train_report;
+rc_status rc_surrogate_train(rc_ctx *ctx, const rc_train_config *cfg, rc_train_report *report);
+/* Readbacks of the last learned-far-field frame (device destinations, any may
+ * be NULL): cone [Py_2][Px_2][16][4] (rgb, transmittance); level1 [Py_1][Px_1][4][4]
+ * merged level 1 (fp32); tile_err [ceil(Py_1/8)][ceil(Px_1/8)]. */
+rc_status rc_debug_surrogate(rc_ctx *ctx, float *cone_dev, float *level1_dev, float *tile_err_dev, void *stream);
+/* Mip level m >= 1 of the last learned-far-field frame: emc [h][w][4]
+ * (mean E*[emitter] rgb, mean [emitter]) and mo [h][w] (mean [occluder]). */
+rc_status rc_debug_mip(rc_ctx *ctx, uint32_t level, float *emc_dev, float *mo_dev, void *stream);
 
 /* ---- cascade geometry (integer, identical to the kernels' indexing) ------- */
 rc_status rc_probe_grid(uint32_t width, uint32_t height, uint32_t level, uint32_t *px, uint32_t *py);

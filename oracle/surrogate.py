@@ -1,0 +1,236 @@
+"""FP64 reference of the learned far field (near distance field, emitter /
+occluder mip pyramid, cone trace, MLP features and the level-1 composition).
+
+TEST INFRASTRUCTURE ONLY: the checker for the GPU kernels of
+radiancecascade_amd/csrc/rc_surrogate.hip; the product never imports it.
+
+The reference repository has no global illumination and no learned part
+(its only pass is the forward raster, src/renderer.rs:559-613); the rules
+restated here are the project's own and are pinned in
+tests/test_oracle_surrogate.py by a library EDT (scipy), block means computed
+directly from pixels, closed forms of the cone trace (empty scene, fully
+emissive scene, linearity, mirror symmetry) and the reduction of the
+composition to the plain merge (oracle/rco.c) when the far field is exact.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import mlp, rco
+
+NEAR_R = 10                      # exact distances up to the level-1 interval end
+N_DIRS = 16                      # level-1 directions = cones per level-2 probe
+CONE_T0 = 6.0
+CONE_GROW = 0.75
+T_MIN = 1e-4
+
+
+# ---- near distance field (rc_surrogate.hip k_near_sdf) -------------------------
+def near_sdf(seed_init_field: np.ndarray) -> np.ndarray:
+    """float32 [H][W]: (float)sqrt(d^2) of the nearest occupied pixel where
+    d^2 <= 100, +inf elsewhere (brute force over the 21x21 window)."""
+    occ = seed_init_field != rco.SEED_NONE
+    H, W = occ.shape
+    best = np.full((H, W), NEAR_R * NEAR_R + 1, np.int64)
+    pad = np.zeros((H + 2 * NEAR_R, W + 2 * NEAR_R), bool)
+    pad[NEAR_R:NEAR_R + H, NEAR_R:NEAR_R + W] = occ
+    for dy in range(-NEAR_R, NEAR_R + 1):
+        for dx in range(-NEAR_R, NEAR_R + 1):
+            d2 = dx * dx + dy * dy
+            if d2 > NEAR_R * NEAR_R:
+                continue
+            hit = pad[NEAR_R + dy:NEAR_R + dy + H, NEAR_R + dx:NEAR_R + dx + W]
+            best = np.where(hit & (d2 < best), d2, best)
+    out = np.full((H, W), np.inf, np.float32)
+    ok = best <= NEAR_R * NEAR_R
+    out[ok] = np.sqrt(best[ok].astype(np.float64)).astype(np.float32)
+    return out
+
+
+# ---- mip pyramid (k_mip, k_mip_top) -------------------------------------------
+def top_level(W: int, H: int) -> int:
+    top = 1
+    while (1 << top) < W or (1 << top) < H:
+        top += 1
+    return min(top, 14)
+
+
+def pixel_fields(seed_init_field: np.ndarray, emissive: np.ndarray):
+    """(E*[emitter] [3,H,W], [emitter] [H,W], [occupied and not emitter] [H,W])."""
+    e = np.asarray(emissive, np.float64)
+    em = (e > 0).any(axis=0)
+    occ = seed_init_field != rco.SEED_NONE
+    return e * em[None], em.astype(np.float64), (occ & ~em).astype(np.float64)
+
+
+def _pool(a: np.ndarray) -> np.ndarray:
+    h, w = a.shape[-2:]
+    h2, w2 = (h + 1) // 2, (w + 1) // 2
+    pad = [(0, 0)] * (a.ndim - 2) + [(0, 2 * h2 - h), (0, 2 * w2 - w)]
+    a = np.pad(a, pad)
+    return 0.25 * ((a[..., 0::2, 0::2] + a[..., 0::2, 1::2]) + (a[..., 1::2, 0::2] + a[..., 1::2, 1::2]))
+
+
+def mip(seed_init_field: np.ndarray, emissive: np.ndarray):
+    """levels[m] for m = 1..top: (emc [h][w][4] = mean E*[em] rgb, mean [em]; mo [h][w])."""
+    H, W = seed_init_field.shape
+    EM, ME, MO = pixel_fields(seed_init_field, emissive)
+    cur = np.concatenate([EM, ME[None]], 0), MO
+    levels = [None]
+    for _ in range(1, top_level(W, H) + 1):
+        cur = (_pool(cur[0]), _pool(cur[1]))
+        levels.append((np.moveaxis(cur[0], 0, -1), cur[1]))
+    return levels
+
+
+# ---- cone trace (k_cone) ------------------------------------------------------
+def cone_schedule(W: int, H: int):
+    """[(t, m, len)] as the host builds it (double, then rounded to fp32)."""
+    top = top_level(W, H)
+    tmax = math.sqrt(float(W) * W + float(H) * H) + 2.0
+    tn = math.tan(math.pi / 16.0)
+    t, out = CONE_T0, []
+    while t < tmax and len(out) < 64:
+        dia = max(2.0 * t * tn, 1.0)
+        m = min(max(int(math.floor(math.log2(dia))), 1), top)
+        step = CONE_GROW * dia
+        out.append((float(np.float32(t)), m, float(np.float32(min(step, float(1 << m))))))
+        t += step
+    return out
+
+
+def _sample(level, x, y, m):
+    emc, mo = level
+    h, w = mo.shape
+    s = float(1 << m)
+    fx = x / s - 0.5
+    fy = y / s - 0.5
+    x0 = np.floor(fx).astype(np.int64)
+    y0 = np.floor(fy).astype(np.int64)
+    ax, ay = fx - x0, fy - y0
+    e = np.zeros(x.shape + (4,))
+    o = np.zeros(x.shape)
+    for dy, wy in ((0, 1 - ay), (1, ay)):
+        for dx, wx in ((0, 1 - ax), (1, ax)):
+            xi, yi = x0 + dx, y0 + dy
+            ok = (xi >= 0) & (xi < w) & (yi >= 0) & (yi < h)
+            xc, yc = np.clip(xi, 0, w - 1), np.clip(yi, 0, h - 1)
+            wt = wx * wy * ok
+            e += emc[yc, xc] * wt[..., None]
+            o += mo[yc, xc] * wt
+    return e, o
+
+
+def cone_trace(levels, W: int, H: int, schedule=None) -> np.ndarray:
+    """[QY][QX][16][4]: far-field radiance rgb and transmittance per level-2
+    probe and level-1 direction (the fp32 direction table of rco)."""
+    schedule = schedule if schedule is not None else cone_schedule(W, H)
+    QX, QY = rco.probe_grid(W, H, 2)
+    dirs = rco.dir_table_f32(1).astype(np.float64)
+    qy, qx = np.mgrid[0:QY, 0:QX]
+    cx = (qx + 0.5) * 8.0
+    cy = (qy + 0.5) * 8.0
+    out = np.zeros((QY, QX, N_DIRS, 4))
+    for k in range(N_DIRS):
+        dx, dy = dirs[k]
+        T = np.ones(cx.shape)
+        L = np.zeros(cx.shape + (3,))
+        live = np.ones(cx.shape, bool)
+        for t, m, ln in schedule:
+            e, o = _sample(levels[m], t * dx + cx, t * dy + cy, m)
+            ao = 1.0 - (1.0 - np.minimum(o, 0.999)) ** ln
+            a = np.minimum(e[..., 3] + ao, 1.0)
+            L = np.where(live[..., None], L + T[..., None] * e[..., :3], L)
+            T = np.where(live, T * (1.0 - a), T)
+            live = live & (T >= T_MIN)
+        out[:, :, k, :3] = L
+        out[:, :, k, 3] = T
+    return out
+
+
+# ---- features and composition (k_surr_level1, k_surr_sample) -------------------
+def _axis(n: int, un: int):
+    p = np.arange(n)
+    base = (p - 1) >> 1
+    w1 = np.where(p & 1, 0.25, 0.75)
+    return np.clip(base, 0, un - 1), np.clip(base + 1, 0, un - 1), 1.0 - w1, w1
+
+
+def upper_bilinear(U: np.ndarray, PX: int, PY: int) -> np.ndarray:
+    """[PY][PX][16][3]: entries of the level-2 grid array U [QY][QX][16][>=3]
+    interpolated at the level-1 probes with the merge's weights."""
+    QY, QX = U.shape[:2]
+    x0, x1, wx0, wx1 = _axis(PX, QX)
+    y0, y1, wy0, wy1 = _axis(PY, QY)
+    A = U[..., :3]
+    r0, r1 = A[y0], A[y1]
+    top = r0[:, x0] * wx0[None, :, None, None] + r0[:, x1] * wx1[None, :, None, None]
+    bot = r1[:, x0] * wx0[None, :, None, None] + r1[:, x1] * wx1[None, :, None, None]
+    return top * wy0[:, None, None, None] + bot * wy1[:, None, None, None]
+
+
+def features(cone: np.ndarray, W: int, H: int):
+    """(c [P][48] cone radiance at the level-1 probes, x [P][48] = log1p(c))."""
+    PX, PY = rco.probe_grid(W, H, 1)
+    c = upper_bilinear(cone, PX, PY).reshape(PX * PY, 48)
+    return c, np.log1p(c)
+
+
+def samples(cone: np.ndarray, A2: np.ndarray, W: int, H: int):
+    """Training rows: (log1p(c), B - c) with B the plain level-2 bilinear."""
+    PX, PY = rco.probe_grid(W, H, 1)
+    c, x = features(cone, W, H)
+    B = upper_bilinear(A2, PX, PY).reshape(PX * PY, 48)
+    return x, B - c
+
+
+def compose_level1(march1: np.ndarray, c: np.ndarray, z: np.ndarray, out_scale: float) -> np.ndarray:
+    """Merged level 1 [PY][PX][4][4] from the level-1 march [PY][PX][16][4],
+    cone radiance c [P][48] and network outputs z [P][>=48]."""
+    PY, PX = march1.shape[:2]
+    B = np.maximum(c + out_scale * z[:, :48], 0.0).reshape(PY, PX, 16, 3)
+    al = march1[..., 3:4]
+    R = np.concatenate([march1[..., :3] + al * B, al], -1)
+    return 0.25 * ((R[:, :, 0::4] + R[:, :, 1::4]) + (R[:, :, 2::4] + R[:, :, 3::4]))
+
+
+def params_to_layers(params: np.ndarray):
+    """Flat RC_MLP_PARAMS vector -> [[W_l [K][64], b_l [64]]] (oracle/mlp.py layout)."""
+    p = np.asarray(params, np.float64).reshape(-1)
+    out, off = [], 0
+    for k, n in mlp.SIZES:
+        W = p[off:off + k * n].reshape(k, n)
+        off += k * n
+        b = p[off:off + n]
+        off += n
+        out.append([W, b])
+    assert off == p.size
+    return out
+
+
+def layers_to_params(layers) -> np.ndarray:
+    return np.concatenate([np.concatenate([W.reshape(-1), b.reshape(-1)]) for W, b in layers])
+
+
+def run(depth, emissive, L, params, in_mean, in_istd, out_scale, want=False):
+    """Whole learned-far-field frame in FP64 (levels 0 and 1 marched on the
+    near distance field; the network evaluated with the GPU's bf16 rounding
+    points).  Returns irradiance [3,H,W] (and intermediates if want)."""
+    H, W = depth.shape
+    seed0 = rco.seed_init(depth, emissive)
+    sdf = near_sdf(seed0).astype(np.float64)
+    m1, _ = rco.march_level(sdf, emissive, 1)
+    m0, _ = rco.march_level(sdf, emissive, 0)
+    levels = mip(seed0, emissive)
+    cone = cone_trace(levels, W, H)
+    c, x = features(cone, W, H)
+    xn = (x - np.asarray(in_mean, np.float64)) * np.asarray(in_istd, np.float64)
+    z, _ = mlp.forward(params_to_layers(params), xn, emulate_bf16=True)
+    A1 = compose_level1(m1, c, z, out_scale)
+    A0 = rco.merge_level(W, H, 0, m0, A1)
+    irr = rco.gather(W, H, A0)
+    if want:
+        return irr, dict(sdf=sdf, m1=m1, m0=m0, levels=levels, cone=cone, c=c, x=x, z=z, A1=A1)
+    return irr

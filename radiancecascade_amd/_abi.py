@@ -32,7 +32,12 @@ TILE_SHAPES = {0: (8, 8, 4), 1: (4, 4, 16), 2: (2, 2, 64), 3: (1, 1, 256)}
 
 STAGE_SEED, STAGE_JFA, STAGE_JFA_TAIL, STAGE_LEVEL0 = 0, 1, 2, 3
 STAGE_MARCH = STAGE_LEVEL0 + MAX_LEVELS
-NUM_STAGES = STAGE_MARCH + 1
+STAGE_NEAR_SDF, STAGE_MIP, STAGE_CONE, STAGE_SURROGATE, STAGE_FALLBACK = range(STAGE_MARCH + 1, STAGE_MARCH + 6)
+NUM_STAGES = STAGE_MARCH + 6
+
+MLP_INPUTS, MLP_OUTPUTS, MLP_PARAMS = 48, 64, 15616
+SURROGATE_BLOB_MAGIC = b"RCSURR01"
+TRAIN_MAX_EPOCHS = 64
 
 
 class Config(ctypes.Structure):
@@ -50,7 +55,29 @@ class Stats(ctypes.Structure):
                 ("stage_ms", ctypes.c_double * NUM_STAGES),
                 ("stage_launches", ctypes.c_uint32 * NUM_STAGES),
                 ("level_rays", ctypes.c_uint64 * MAX_LEVELS),
-                ("level_bytes_stored", ctypes.c_uint64 * MAX_LEVELS)]
+                ("level_bytes_stored", ctypes.c_uint64 * MAX_LEVELS),
+                ("surrogate_frames", ctypes.c_uint64), ("fallback_frames", ctypes.c_uint64),
+                ("flagged_tiles", ctypes.c_uint64), ("tiles_per_frame", ctypes.c_uint64)]
+
+
+class SurrogateConfig(ctypes.Structure):
+    _fields_ = [("version", ctypes.c_uint32), ("mode", ctypes.c_uint32), ("threshold", ctypes.c_float),
+                ("reserved", ctypes.c_uint32)]
+
+
+class TrainConfig(ctypes.Structure):
+    _fields_ = [("version", ctypes.c_uint32), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("levels", ctypes.c_uint32), ("scene_seed0", ctypes.c_uint64), ("scenes", ctypes.c_uint32),
+                ("n_boxes", ctypes.c_uint32), ("n_emissive", ctypes.c_uint32), ("epochs", ctypes.c_uint32),
+                ("batch", ctypes.c_uint32), ("lr", ctypes.c_float), ("lr_decay", ctypes.c_float),
+                ("reserved", ctypes.c_uint32), ("seed", ctypes.c_uint64)]
+
+
+class TrainReport(ctypes.Structure):
+    _fields_ = [("version", ctypes.c_uint32), ("epochs", ctypes.c_uint32), ("samples", ctypes.c_uint64),
+                ("steps", ctypes.c_uint64), ("epoch_loss", ctypes.c_float * TRAIN_MAX_EPOCHS),
+                ("gen_ms", ctypes.c_double), ("train_ms", ctypes.c_double), ("out_scale", ctypes.c_float),
+                ("reserved", ctypes.c_uint32)]
 
 
 class Box(ctypes.Structure):
@@ -90,6 +117,19 @@ SIGNATURES = {
     "rc_probe_grid": (_i32, [_u32, _u32, _u32, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
     "rc_level_rays": (_u64, [_u32, _u32, _u32]),
     "rc_levels_for_coverage": (_u32, [_u32, _u32]),
+    "rc_set_surrogate": (_i32, [_P, ctypes.POINTER(SurrogateConfig)]),
+    "rc_get_surrogate": (_i32, [_P, ctypes.POINTER(SurrogateConfig)]),
+    "rc_mlp_set": (_i32, [_P, _P, _P, _P, ctypes.c_float]),
+    "rc_mlp_get": (_i32, [_P, _P, _P, _P, ctypes.POINTER(ctypes.c_float)]),
+    "rc_surrogate_save": (_i32, [_P, _P, _u64, ctypes.POINTER(_u64)]),
+    "rc_surrogate_load": (_i32, [_P, _P, _u64]),
+    "rc_mlp_forward": (_i32, [_P, _P, _u32, _P, _P]),
+    "rc_mlp_gradient": (_i32, [_P, _P, _P, _u32, _P, _P, _P]),
+    "rc_mlp_adam": (_i32, [_P, _P, ctypes.c_float, _P]),
+    "rc_surrogate_samples": (_i32, [_P, ctypes.POINTER(GBuffer), _P, _P, _P]),
+    "rc_surrogate_train": (_i32, [_P, ctypes.POINTER(TrainConfig), ctypes.POINTER(TrainReport)]),
+    "rc_debug_surrogate": (_i32, [_P, _P, _P, _P, _P]),
+    "rc_debug_mip": (_i32, [_P, _u32, _P, _P, _P]),
     "rc_scene_generate": (_i32, [_u64, _u32, _u32, _P]),
     "rc_gbuffer_raster": (_i32, [_P, _u32, _u32, _u32, _P, _P, _P, _P]),
 }

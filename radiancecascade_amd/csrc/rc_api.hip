@@ -10,6 +10,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <chrono>
 #include <new>
 #include <string>
 #include <vector>
@@ -49,6 +50,28 @@ struct rc_ctx {
     hipStream_t cap_stream = nullptr;
     hipGraphExec_t gexec = nullptr;
     const void *g_key[3] = {nullptr, nullptr, nullptr};
+    // learned far field
+    int surr_mode = 0;
+    float surr_thr = __builtin_huge_valf();
+    MlpDev mlp{};
+    bool mlp_ready = false;
+    float out_scale = 1.0f;
+    uint32_t train_scenes = 0;
+    int adam_t = 0;
+    float *mlp_partial = nullptr, *mlp_loss_part = nullptr;
+    int *mlp_idx = nullptr;
+    int mlp_cap_rows = 0;
+    bool surr_alloc = false;
+    MipArgs mip{};
+    ConeSchedule sch{};
+    float4 *cone = nullptr;
+    void *surr_a1 = nullptr;
+    float *tile_err = nullptr;
+    int *flag_count = nullptr, *flag_list = nullptr;
+    int *flag_host = nullptr;  // pinned
+    int surr_tiles = 0, surr_tiles_x = 0;
+    bool surr_ran = false;
+    float *scratch_irr = nullptr;  // irradiance of frames run only for their levels
     // profiling
     hipEvent_t ev[2 * RC_NUM_STAGES] = {};
     bool ev_used[RC_NUM_STAGES] = {};
@@ -119,6 +142,23 @@ static void release(rc_ctx *c)
     c->codes = nullptr;
     if (c->norm_ws) hipFree(c->norm_ws);
     c->norm_ws = nullptr;
+    if (c->mip.emc) hipFree(c->mip.emc);
+    if (c->mip.mo) hipFree(c->mip.mo);
+    c->mip.emc = nullptr;
+    c->mip.mo = nullptr;
+    if (c->cone) hipFree(c->cone);
+    if (c->surr_a1) hipFree(c->surr_a1);
+    if (c->tile_err) hipFree(c->tile_err);
+    if (c->flag_count) hipFree(c->flag_count);
+    if (c->flag_list) hipFree(c->flag_list);
+    c->cone = nullptr;
+    c->surr_a1 = nullptr;
+    c->tile_err = nullptr;
+    c->flag_count = c->flag_list = nullptr;
+    c->surr_alloc = false;
+    c->surr_ran = false;
+    if (c->scratch_irr) hipFree(c->scratch_irr);
+    c->scratch_irr = nullptr;
     if (c->g_depth) hipFree(c->g_depth);
     if (c->g_emissive) hipFree(c->g_emissive);
     if (c->irr) hipFree(c->irr);
@@ -237,6 +277,21 @@ struct StageTimer {
     }
 };
 
+static rc_status collect_profile(rc_ctx *c, hipStream_t s, int jfa_launches)
+{
+    HIP_TRY(c, hipStreamSynchronize(s));
+    for (int st = 0; st < RC_NUM_STAGES; ++st) {
+        if (!c->ev_used[st]) continue;
+        float ms = 0.f;
+        HIP_TRY(c, hipEventElapsedTime(&ms, c->ev[2 * st], c->ev[2 * st + 1]));
+        const uint32_t n = st == RC_STAGE_JFA ? (uint32_t)jfa_launches : 1u;
+        if (n == 0) continue;
+        c->stats.stage_ms[st] += ms;
+        c->stats.stage_launches[st] += n;
+    }
+    return RC_OK;
+}
+
 static rc_status run_frame(rc_ctx *c, const float *depth, const float *emissive, float *irr, hipStream_t s,
                            bool quiet = false)
 {
@@ -291,22 +346,152 @@ static rc_status run_frame(rc_ctx *c, const float *depth, const float *emissive,
     tm.end(RC_STAGE_LEVEL0);
 
     if (prof) {
-        HIP_TRY(c, hipStreamSynchronize(s));
-        for (int st = 0; st < RC_NUM_STAGES; ++st) {
-            if (!c->ev_used[st]) continue;
-            float ms = 0.f;
-            HIP_TRY(c, hipEventElapsedTime(&ms, c->ev[2 * st], c->ev[2 * st + 1]));
-            const uint32_t n = st == RC_STAGE_JFA ? (uint32_t)global_passes : 1u;
-            if (n == 0) continue;
-            c->stats.stage_ms[st] += ms;
-            c->stats.stage_launches[st] += n;
-        }
+        rc_status st = collect_profile(c, s, global_passes);
+        if (st != RC_OK) return st;
     }
     if (!quiet) {
         c->stats.runs += 1;
         c->ran = true;
     }
     return RC_OK;
+}
+
+// ---- learned far field ---------------------------------------------------------
+static void free_mlp(rc_ctx *c)
+{
+    void *p[] = {c->mlp.master, c->mlp.adam_m, c->mlp.adam_v, c->mlp.wt, c->mlp.wk, c->mlp.bias, c->mlp.in_mean,
+                 c->mlp.in_istd, c->mlp_partial, c->mlp_loss_part, c->mlp_idx};
+    for (void *q : p)
+        if (q) hipFree(q);
+    c->mlp = MlpDev{};
+    c->mlp_partial = c->mlp_loss_part = nullptr;
+    c->mlp_idx = nullptr;
+    c->mlp_cap_rows = 0;
+    if (c->flag_host) hipHostFree(c->flag_host);
+    c->flag_host = nullptr;
+}
+
+static rc_status alloc_mlp(rc_ctx *c)
+{
+    HIP_TRY(c, hipMalloc(&c->mlp.master, kMlpParams * sizeof(float)));
+    HIP_TRY(c, hipMalloc(&c->mlp.adam_m, kMlpParams * sizeof(float)));
+    HIP_TRY(c, hipMalloc(&c->mlp.adam_v, kMlpParams * sizeof(float)));
+    HIP_TRY(c, hipMalloc(&c->mlp.wt, kMlpWt * sizeof(uint16_t)));
+    HIP_TRY(c, hipMalloc(&c->mlp.wk, kMlpWt * sizeof(uint16_t)));
+    HIP_TRY(c, hipMalloc(&c->mlp.bias, 4 * 64 * sizeof(float)));
+    HIP_TRY(c, hipMalloc(&c->mlp.in_mean, kMlpIn * sizeof(float)));
+    HIP_TRY(c, hipMalloc(&c->mlp.in_istd, kMlpIn * sizeof(float)));
+    HIP_TRY(c, hipMemset(c->mlp.master, 0, kMlpParams * sizeof(float)));
+    HIP_TRY(c, hipMemset(c->mlp.adam_m, 0, kMlpParams * sizeof(float)));
+    HIP_TRY(c, hipMemset(c->mlp.adam_v, 0, kMlpParams * sizeof(float)));
+    HIP_TRY(c, hipHostMalloc(&c->flag_host, sizeof(int), hipHostMallocDefault));
+    return RC_OK;
+}
+
+// Frame buffers of the learned far field (allocated on first use, freed with
+// the frame buffers on resize).
+static rc_status ensure_surrogate(rc_ctx *c)
+{
+    if (c->surr_alloc) return RC_OK;
+    if (c->L < 3) return fail(c, RC_ERR_INVALID, "learned far field needs levels >= 3");
+    const int top = cone_top_level(c->W, c->H);
+    mip_layout(c->W, c->H, top, c->mip);
+    cone_schedule(c->W, c->H, top, c->sch);
+    const size_t cells = mip_cells(c->mip);
+    HIP_TRY(c, hipMalloc(&c->mip.emc, cells * sizeof(float4)));
+    HIP_TRY(c, hipMalloc(&c->mip.mo, cells * sizeof(float)));
+    const int QX = grid_dim(c->W, 2), QY = grid_dim(c->H, 2);
+    HIP_TRY(c, hipMalloc(&c->cone, (size_t)QX * QY * 16 * sizeof(float4)));
+    const int PX = grid_dim(c->W, 1), PY = grid_dim(c->H, 1);
+    HIP_TRY(c, hipMalloc(&c->surr_a1, (size_t)PX * PY * 4 * texel_bytes(c->storage)));
+    c->surr_tiles_x = (PX + 7) / 8;
+    c->surr_tiles = c->surr_tiles_x * ((PY + 7) / 8);
+    HIP_TRY(c, hipMalloc(&c->tile_err, (size_t)c->surr_tiles * sizeof(float)));
+    HIP_TRY(c, hipMalloc(&c->flag_list, (size_t)c->surr_tiles * sizeof(int)));
+    HIP_TRY(c, hipMalloc(&c->flag_count, sizeof(int)));
+    HIP_TRY(c, hipMemset(c->flag_count, 0, sizeof(int)));
+    c->surr_alloc = true;
+    return RC_OK;
+}
+
+static ConeArgs cone_args(const rc_ctx *c)
+{
+    ConeArgs a{};
+    a.mip = c->mip;
+    a.sch = c->sch;
+    a.dirs = c->dirs + c->dir_off[1];
+    a.cone = c->cone;
+    a.QX = grid_dim(c->W, 2);
+    a.QY = grid_dim(c->H, 2);
+    a.spacing = (float)probe_spacing(2);
+    return a;
+}
+
+// seed init -> near distance field -> mip pyramid -> cones -> level 1 (march +
+// MLP + merge) -> level 0 + gather from the learned level 1.
+static rc_status run_frame_surrogate(rc_ctx *c, const float *depth, const float *emissive, float *irr, hipStream_t s,
+                                     bool quiet)
+{
+    const bool prof = !quiet && (c->cfg.flags & RC_FLAG_PROFILE) != 0;
+    const bool dbg = (c->cfg.flags & RC_FLAG_KEEP_DEBUG) != 0;
+    StageTimer tm{c, s, prof};
+    for (auto &u : c->ev_used) u = false;
+    tm.begin(RC_STAGE_SEED);
+    HIP_TRY(c, launch_seed_init(depth, emissive, c->seed[0], c->W, c->H, s));
+    tm.end(RC_STAGE_SEED);
+    tm.begin(RC_STAGE_NEAR_SDF);
+    HIP_TRY(c, launch_near_sdf(c->seed[0], c->sdf, c->W, c->H, s));
+    tm.end(RC_STAGE_NEAR_SDF);
+    tm.begin(RC_STAGE_MIP);
+    HIP_TRY(c, launch_mip(c->seed[0], emissive, c->mip, c->W, c->H, c->flag_count, s));
+    tm.end(RC_STAGE_MIP);
+    tm.begin(RC_STAGE_CONE);
+    HIP_TRY(c, launch_cone(cone_args(c), s));
+    tm.end(RC_STAGE_CONE);
+    SurrLevel1Args a{};
+    a.sdf = c->sdf;
+    a.emissive = emissive;
+    a.dirs = c->dirs + c->dir_off[1];
+    a.cone = c->cone;
+    a.out = c->surr_a1;
+    a.tile_err = c->tile_err;
+    a.flag_count = c->flag_count;
+    a.flag_list = c->flag_list;
+    a.W = c->W;
+    a.H = c->H;
+    a.PX = grid_dim(c->W, 1);
+    a.PY = grid_dim(c->H, 1);
+    a.QX = grid_dim(c->W, 2);
+    a.QY = grid_dim(c->H, 2);
+    a.tiles_x = c->surr_tiles_x;
+    a.tiles = c->surr_tiles;
+    a.out_scale = c->out_scale;
+    a.threshold = c->surr_thr;
+    tm.begin(RC_STAGE_SURROGATE);
+    HIP_TRY(c, launch_surr_level1(a, c->mlp, c->storage, s));
+    tm.end(RC_STAGE_SURROGATE);
+    tm.begin(RC_STAGE_LEVEL0);
+    LevelArgs a0 = level_args(c, 0, emissive);
+    a0.upper = c->surr_a1;
+    HIP_TRY(c, launch_level0_gather(a0, c->storage, irr, dbg ? c->dbg_level0 : nullptr, s));
+    tm.end(RC_STAGE_LEVEL0);
+    if (prof) {
+        rc_status st = collect_profile(c, s, 0);
+        if (st != RC_OK) return st;
+    }
+    if (!quiet) {
+        c->stats.runs += 1;
+        c->ran = true;
+    }
+    c->surr_ran = true;
+    return RC_OK;
+}
+
+static rc_status run_frame_any(rc_ctx *c, const float *depth, const float *emissive, float *irr, hipStream_t s,
+                               bool quiet)
+{
+    if (c->surr_mode) return run_frame_surrogate(c, depth, emissive, irr, s, quiet);
+    return run_frame(c, depth, emissive, irr, s, quiet);
 }
 
 // The frame's 15 launches (at 1080p) replayed as one graph: profiling and
@@ -322,7 +507,7 @@ static rc_status run_frame_graph(rc_ctx *c, const float *depth, const float *emi
         drop_graph(c);
         if (!c->cap_stream) HIP_TRY(c, hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking));
         HIP_TRY(c, hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal));
-        rc_status st = run_frame(c, depth, emissive, irr, c->cap_stream, true);
+        rc_status st = run_frame_any(c, depth, emissive, irr, c->cap_stream, true);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(c->cap_stream, &g);
         if (st != RC_OK) {
@@ -343,6 +528,116 @@ static rc_status run_frame_graph(rc_ctx *c, const float *depth, const float *emi
     HIP_TRY(c, hipGraphLaunch(c->gexec, s));
     c->stats.runs += 1;
     c->ran = true;
+    return RC_OK;
+}
+
+// One frame on stream s.  In learned mode with a finite threshold the host
+// reads the flagged-tile count after the frame (one 4-byte copy and a stream
+// sync); if any tile is flagged the plain frame runs, the flagged tiles of the
+// learned level 1 are replaced by the plain ones and level 0 is gathered
+// again.
+static rc_status frame(rc_ctx *c, const float *depth, const float *emissive, float *irr, hipStream_t s)
+{
+    if (c->surr_mode) {
+        if (!c->mlp_ready)
+            return fail(c, RC_ERR_STATE, "learned far field: no network (rc_mlp_set, rc_surrogate_load or rc_surrogate_train)");
+        rc_status st = ensure_surrogate(c);
+        if (st != RC_OK) return st;
+    }
+    rc_status st = graph_enabled(c) ? run_frame_graph(c, depth, emissive, irr, s)
+                                    : run_frame_any(c, depth, emissive, irr, s, false);
+    if (st != RC_OK || !c->surr_mode) return st;
+    c->stats.surrogate_frames += 1;
+    c->stats.tiles_per_frame = (uint64_t)c->surr_tiles;
+    if (!(c->surr_thr < __builtin_huge_valf())) return RC_OK;
+    HIP_TRY(c, hipMemcpyAsync(c->flag_host, c->flag_count, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    const int n = *c->flag_host;
+    if (n <= 0) return RC_OK;
+    const bool prof = (c->cfg.flags & RC_FLAG_PROFILE) != 0;
+    if (prof) HIP_TRY(c, hipEventRecord(c->ev[2 * RC_STAGE_FALLBACK], s));
+    st = run_frame(c, depth, emissive, irr, s, true);
+    if (st != RC_OK) return st;
+    HIP_TRY(c, launch_tile_patch(c->level[1], c->surr_a1, c->tile_err, c->surr_thr, grid_dim(c->W, 1),
+                                 grid_dim(c->H, 1), c->storage, s));
+    LevelArgs a0 = level_args(c, 0, emissive);
+    a0.upper = c->surr_a1;
+    HIP_TRY(c, launch_level0_gather(a0, c->storage, irr, nullptr, s));
+    if (prof) {
+        HIP_TRY(c, hipEventRecord(c->ev[2 * RC_STAGE_FALLBACK + 1], s));
+        HIP_TRY(c, hipStreamSynchronize(s));
+        float ms = 0.f;
+        HIP_TRY(c, hipEventElapsedTime(&ms, c->ev[2 * RC_STAGE_FALLBACK], c->ev[2 * RC_STAGE_FALLBACK + 1]));
+        c->stats.stage_ms[RC_STAGE_FALLBACK] += ms;
+        c->stats.stage_launches[RC_STAGE_FALLBACK] += 1;
+    }
+    c->stats.fallback_frames += 1;
+    c->stats.flagged_tiles += (uint64_t)n;
+    return RC_OK;
+}
+
+// Training rows of the G-buffer at the context's size (plain frame first; the
+// seed field is rebuilt because the jump flood reuses its buffer).
+static rc_status gen_samples(rc_ctx *c, const float *depth, const float *emissive, float *feats, float *targets,
+                             hipStream_t s)
+{
+    rc_status st = ensure_surrogate(c);
+    if (st != RC_OK) return st;
+    if (!c->scratch_irr) HIP_TRY(c, hipMalloc(&c->scratch_irr, 3 * (size_t)c->W * c->H * sizeof(float)));
+    st = run_frame(c, depth, emissive, c->scratch_irr, s, true);
+    if (st != RC_OK) return st;
+    HIP_TRY(c, launch_seed_init(depth, emissive, c->seed[0], c->W, c->H, s));
+    HIP_TRY(c, launch_mip(c->seed[0], emissive, c->mip, c->W, c->H, c->flag_count, s));
+    HIP_TRY(c, launch_cone(cone_args(c), s));
+    SurrSampleArgs a{};
+    a.cone = c->cone;
+    a.upper = c->level[2];
+    a.feats = feats;
+    a.targets = targets;
+    a.PX = grid_dim(c->W, 1);
+    a.PY = grid_dim(c->H, 1);
+    a.QX = grid_dim(c->W, 2);
+    a.QY = grid_dim(c->H, 2);
+    HIP_TRY(c, launch_surr_sample(a, c->storage, s));
+    return RC_OK;
+}
+
+static uint64_t mix64(uint64_t &state)
+{
+    uint64_t z = (state += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Scratch for the gradient hook: per-workgroup partial gradients and losses.
+static rc_status ensure_mlp_scratch(rc_ctx *c, int parts, int rows)
+{
+    if (parts > c->mlp_cap_rows) {
+        if (c->mlp_partial) hipFree(c->mlp_partial);
+        if (c->mlp_loss_part) hipFree(c->mlp_loss_part);
+        c->mlp_partial = c->mlp_loss_part = nullptr;
+        HIP_TRY(c, hipMalloc(&c->mlp_partial, (size_t)parts * kMlpParams * sizeof(float)));
+        HIP_TRY(c, hipMalloc(&c->mlp_loss_part, (size_t)parts * sizeof(float)));
+        c->mlp_cap_rows = parts;
+    }
+    (void)rows;
+    return RC_OK;
+}
+
+static rc_status mlp_upload(rc_ctx *c, const float *params, const float *mean, const float *istd, float out_scale)
+{
+    HIP_TRY(c, hipMemcpy(c->mlp.master, params, kMlpParams * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->mlp.in_mean, mean, kMlpIn * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->mlp.in_istd, istd, kMlpIn * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(c, launch_mlp_refresh(c->mlp, nullptr));
+    HIP_TRY(c, hipMemset(c->mlp.adam_m, 0, kMlpParams * sizeof(float)));
+    HIP_TRY(c, hipMemset(c->mlp.adam_v, 0, kMlpParams * sizeof(float)));
+    HIP_TRY(c, hipDeviceSynchronize());
+    c->adam_t = 0;
+    c->out_scale = out_scale;
+    c->mlp_ready = true;
+    drop_graph(c);
     return RC_OK;
 }
 
@@ -404,7 +699,8 @@ rc_status rc_create(const rc_config *cfg, rc_ctx **out)
     for (auto &ev : c->ev) {
         if (hipEventCreate(&ev) != hipSuccess) { rc_destroy(c); return fail(nullptr, RC_ERR_HIP, "hipEventCreate failed"); }
     }
-    rc_status st = allocate(c, cfg->width, cfg->height, L);
+    rc_status st = alloc_mlp(c);
+    if (st == RC_OK) st = allocate(c, cfg->width, cfg->height, L);
     if (st != RC_OK) {
         std::string msg = c->err;
         rc_destroy(c);
@@ -425,7 +721,9 @@ rc_status rc_resize(rc_ctx *c, uint32_t W, uint32_t H, uint32_t L)
     c->cfg.width = W;
     c->cfg.height = H;
     c->cfg.levels = L;
-    return allocate(c, W, H, L);
+    rc_status st = allocate(c, W, H, L);
+    if (st == RC_OK && c->surr_mode) st = ensure_surrogate(c);
+    return st;
 }
 
 rc_status rc_run_device(rc_ctx *c, const rc_gbuffer *g, float *irr, void *stream)
@@ -433,8 +731,7 @@ rc_status rc_run_device(rc_ctx *c, const rc_gbuffer *g, float *irr, void *stream
     if (!c) return fail(nullptr, RC_ERR_NULL, "rc_run_device: ctx is NULL");
     if (!g || !g->depth || !g->emissive || !irr) return fail(c, RC_ERR_NULL, "rc_run_device: G-buffer or output is NULL");
     HIP_TRY(c, hipSetDevice(c->cfg.device));
-    if (graph_enabled(c)) return run_frame_graph(c, g->depth, g->emissive, irr, (hipStream_t)stream);
-    return run_frame(c, g->depth, g->emissive, irr, (hipStream_t)stream);
+    return frame(c, g->depth, g->emissive, irr, (hipStream_t)stream);
 }
 
 rc_status rc_update_host(rc_ctx *c, const rc_gbuffer *host)
@@ -460,7 +757,7 @@ rc_status rc_render_host(rc_ctx *c, float *irr_host)
     if (!irr_host) return fail(c, RC_ERR_NULL, "rc_render_host: output is NULL");
     if (!c->have_gbuffer) return fail(c, RC_ERR_STATE, "rc_render_host: no G-buffer uploaded (call rc_update_host)");
     HIP_TRY(c, hipSetDevice(c->cfg.device));
-    rc_status st = run_frame(c, c->g_depth, c->g_emissive, c->irr, nullptr);
+    rc_status st = frame(c, c->g_depth, c->g_emissive, c->irr, nullptr);
     if (st != RC_OK) return st;
     HIP_TRY(c, hipMemcpy(irr_host, c->irr, 3 * (size_t)c->W * c->H * sizeof(float), hipMemcpyDeviceToHost));
     return RC_OK;
@@ -472,6 +769,7 @@ void rc_destroy(rc_ctx *c)
     hipSetDevice(c->cfg.device);
     hipDeviceSynchronize();
     release(c);
+    free_mlp(c);
     for (auto &ev : c->ev)
         if (ev) hipEventDestroy(ev);
     if (c->cap_stream) hipStreamDestroy(c->cap_stream);
@@ -743,6 +1041,370 @@ rc_status rc_gbuffer_raster(const rc_box *boxes, uint32_t n, uint32_t W, uint32_
     if (e == hipSuccess) e = e2;
     if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
     if (e != hipSuccess) return fail(nullptr, RC_ERR_HIP, std::string("rc_gbuffer_raster: ") + hipGetErrorString(e));
+    return RC_OK;
+}
+
+// ---- learned far field --------------------------------------------------------
+rc_status rc_set_surrogate(rc_ctx *c, const rc_surrogate_config *cfg)
+{
+    if (!c || !cfg) return fail(c, RC_ERR_NULL, "rc_set_surrogate: NULL argument");
+    if (cfg->version != RC_ABI_VERSION || cfg->mode > 1 || cfg->threshold != cfg->threshold)
+        return fail(c, RC_ERR_INVALID, "rc_set_surrogate: bad version, mode or threshold");
+    HIP_TRY(c, hipSetDevice(c->cfg.device));
+    if (cfg->mode == 1) {
+        rc_status st = ensure_surrogate(c);
+        if (st != RC_OK) return st;
+    }
+    c->surr_mode = (int)cfg->mode;
+    c->surr_thr = cfg->threshold;
+    drop_graph(c);
+    return RC_OK;
+}
+
+rc_status rc_get_surrogate(const rc_ctx *c, rc_surrogate_config *out)
+{
+    if (!c || !out) return fail(nullptr, RC_ERR_NULL, "rc_get_surrogate: NULL argument");
+    out->version = RC_ABI_VERSION;
+    out->mode = (uint32_t)c->surr_mode;
+    out->threshold = c->surr_thr;
+    out->reserved = 0;
+    return RC_OK;
+}
+
+rc_status rc_mlp_set(rc_ctx *c, const float *params, const float *mean, const float *istd, float out_scale)
+{
+    if (!c || !params || !mean || !istd) return fail(c, RC_ERR_NULL, "rc_mlp_set: NULL argument");
+    if (!(out_scale > 0.0f) || !(out_scale < __builtin_huge_valf()))
+        return fail(c, RC_ERR_INVALID, "rc_mlp_set: out_scale must be positive and finite");
+    HIP_TRY(c, hipSetDevice(c->cfg.device));
+    return mlp_upload(c, params, mean, istd, out_scale);
+}
+
+rc_status rc_mlp_get(const rc_ctx *c, float *params, float *mean, float *istd, float *out_scale)
+{
+    if (!c) return fail(nullptr, RC_ERR_NULL, "rc_mlp_get: ctx is NULL");
+    rc_ctx *m = const_cast<rc_ctx *>(c);
+    HIP_TRY(m, hipSetDevice(c->cfg.device));
+    HIP_TRY(m, hipDeviceSynchronize());
+    if (params) HIP_TRY(m, hipMemcpy(params, c->mlp.master, kMlpParams * sizeof(float), hipMemcpyDeviceToHost));
+    if (mean) HIP_TRY(m, hipMemcpy(mean, c->mlp.in_mean, kMlpIn * sizeof(float), hipMemcpyDeviceToHost));
+    if (istd) HIP_TRY(m, hipMemcpy(istd, c->mlp.in_istd, kMlpIn * sizeof(float), hipMemcpyDeviceToHost));
+    if (out_scale) *out_scale = c->out_scale;
+    return RC_OK;
+}
+
+// blob: magic[8], u32 version, inputs, outputs, params, f32 out_scale, u32
+// train_scenes, f32 in_mean[48], in_istd[48], params[15616]
+static const uint32_t kBlobVersion = 1;
+static size_t blob_bytes() { return 8 + 6 * 4 + (2 * kMlpIn + kMlpParams) * sizeof(float); }
+
+rc_status rc_surrogate_save(const rc_ctx *c, void *buf, uint64_t cap, uint64_t *size)
+{
+    if (!c || !size) return fail(nullptr, RC_ERR_NULL, "rc_surrogate_save: NULL argument");
+    *size = blob_bytes();
+    if (!buf) return RC_OK;
+    rc_ctx *m = const_cast<rc_ctx *>(c);
+    if (cap < blob_bytes()) return fail(m, RC_ERR_INVALID, "rc_surrogate_save: buffer too small");
+    if (!c->mlp_ready) return fail(m, RC_ERR_STATE, "rc_surrogate_save: no network");
+    std::vector<float> params(kMlpParams), mean(kMlpIn), istd(kMlpIn);
+    float os = 0.f;
+    rc_status st = rc_mlp_get(c, params.data(), mean.data(), istd.data(), &os);
+    if (st != RC_OK) return st;
+    uint8_t *p = (uint8_t *)buf;
+    memcpy(p, RC_SURROGATE_BLOB_MAGIC, 8);
+    const uint32_t hdr[4] = {kBlobVersion, (uint32_t)kMlpIn, (uint32_t)kMlpOut, (uint32_t)kMlpParams};
+    memcpy(p + 8, hdr, 16);
+    memcpy(p + 24, &os, 4);
+    memcpy(p + 28, &c->train_scenes, 4);
+    memcpy(p + 32, mean.data(), kMlpIn * 4);
+    memcpy(p + 32 + kMlpIn * 4, istd.data(), kMlpIn * 4);
+    memcpy(p + 32 + 2 * kMlpIn * 4, params.data(), kMlpParams * 4);
+    return RC_OK;
+}
+
+rc_status rc_surrogate_load(rc_ctx *c, const void *buf, uint64_t size)
+{
+    if (!c || !buf) return fail(c, RC_ERR_NULL, "rc_surrogate_load: NULL argument");
+    const uint8_t *p = (const uint8_t *)buf;
+    if (size != blob_bytes() || memcmp(p, RC_SURROGATE_BLOB_MAGIC, 8) != 0)
+        return fail(c, RC_ERR_INVALID, "rc_surrogate_load: not a weights blob of this format");
+    uint32_t hdr[4];
+    memcpy(hdr, p + 8, 16);
+    if (hdr[0] != kBlobVersion || hdr[1] != (uint32_t)kMlpIn || hdr[2] != (uint32_t)kMlpOut ||
+        hdr[3] != (uint32_t)kMlpParams)
+        return fail(c, RC_ERR_INVALID, "rc_surrogate_load: version or network shape mismatch");
+    float os;
+    memcpy(&os, p + 24, 4);
+    std::vector<float> params(kMlpParams), mean(kMlpIn), istd(kMlpIn);
+    memcpy(mean.data(), p + 32, kMlpIn * 4);
+    memcpy(istd.data(), p + 32 + kMlpIn * 4, kMlpIn * 4);
+    memcpy(params.data(), p + 32 + 2 * kMlpIn * 4, kMlpParams * 4);
+    rc_status st = rc_mlp_set(c, params.data(), mean.data(), istd.data(), os);
+    if (st == RC_OK) memcpy(&c->train_scenes, p + 28, 4);
+    return st;
+}
+
+rc_status rc_mlp_forward(rc_ctx *c, const float *feats, uint32_t n, float *out, void *stream)
+{
+    if (!c || !feats || !out) return fail(c, RC_ERR_NULL, "rc_mlp_forward: NULL argument");
+    if (!c->mlp_ready) return fail(c, RC_ERR_STATE, "rc_mlp_forward: no network");
+    if (n > (1u << 30)) return fail(c, RC_ERR_INVALID, "rc_mlp_forward: n too large");
+    HIP_TRY(c, hipSetDevice(c->cfg.device));
+    HIP_TRY(c, launch_mlp_infer(c->mlp, feats, (int)n, out, (hipStream_t)stream));
+    return RC_OK;
+}
+
+rc_status rc_mlp_gradient(rc_ctx *c, const float *feats, const float *targets, uint32_t n, float *grad, float *loss,
+                          void *stream)
+{
+    if (!c || !feats || !targets || !grad || !loss) return fail(c, RC_ERR_NULL, "rc_mlp_gradient: NULL argument");
+    if (!c->mlp_ready) return fail(c, RC_ERR_STATE, "rc_mlp_gradient: no network");
+    if (n < 1 || n > (1u << 22)) return fail(c, RC_ERR_INVALID, "rc_mlp_gradient: n out of range (1 .. 2^22)");
+    HIP_TRY(c, hipSetDevice(c->cfg.device));
+    hipStream_t s = (hipStream_t)stream;
+    const int parts = (int)((n + 31) / 32);
+    rc_status st = ensure_mlp_scratch(c, parts, (int)n);
+    if (st != RC_OK) return st;
+    if ((int)n > 0) {
+        if (c->mlp_idx) hipFree(c->mlp_idx);
+        c->mlp_idx = nullptr;
+        std::vector<int> idx(n);
+        for (uint32_t i = 0; i < n; ++i) idx[i] = (int)i;
+        HIP_TRY(c, hipMalloc(&c->mlp_idx, n * sizeof(int)));
+        HIP_TRY(c, hipMemcpy(c->mlp_idx, idx.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    }
+    HIP_TRY(c, launch_mlp_train(c->mlp, feats, targets, c->mlp_idx, (int)n, 1, c->mlp_partial, c->mlp_loss_part, parts,
+                                s));
+    HIP_TRY(c, launch_mlp_reduce(c->mlp_partial, parts, grad, s));
+    HIP_TRY(c, launch_sum_f32(c->mlp_loss_part, parts, loss, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    return RC_OK;
+}
+
+static rc_status adam_step(rc_ctx *c, const float *grad, float lr, hipStream_t s)
+{
+    const double b1 = 0.9, b2 = 0.999;
+    c->adam_t += 1;
+    const float c1 = (float)(1.0 - pow(b1, c->adam_t)), c2 = (float)(1.0 - pow(b2, c->adam_t));
+    HIP_TRY(c, launch_mlp_adam(c->mlp, grad, lr, (float)b1, (float)b2, 1e-8f, c1, c2, s));
+    return RC_OK;
+}
+
+rc_status rc_mlp_adam(rc_ctx *c, const float *grad, float lr, void *stream)
+{
+    if (!c || !grad) return fail(c, RC_ERR_NULL, "rc_mlp_adam: NULL argument");
+    if (!c->mlp_ready) return fail(c, RC_ERR_STATE, "rc_mlp_adam: no network");
+    HIP_TRY(c, hipSetDevice(c->cfg.device));
+    rc_status st = adam_step(c, grad, lr, (hipStream_t)stream);
+    if (st == RC_OK) drop_graph(c);
+    return st;
+}
+
+rc_status rc_surrogate_samples(rc_ctx *c, const rc_gbuffer *g, float *feats, float *targets, void *stream)
+{
+    if (!c || !g || !g->depth || !g->emissive || !feats || !targets)
+        return fail(c, RC_ERR_NULL, "rc_surrogate_samples: NULL argument");
+    HIP_TRY(c, hipSetDevice(c->cfg.device));
+    return gen_samples(c, g->depth, g->emissive, feats, targets, (hipStream_t)stream);
+}
+
+// He-uniform initial weights (limit sqrt(6 / fan_in)) from SplitMix64; the
+// output layer's radiance columns are halved and its padding columns zero;
+// biases zero.
+static void init_params(uint64_t seed, std::vector<float> &p)
+{
+    uint64_t st = seed;
+    p.assign(kMlpParams, 0.0f);
+    size_t off = 0;
+    for (int l = 0; l < 4; ++l) {
+        const int K = l == 0 ? kMlpIn : kMlpHid;
+        const double lim = sqrt(6.0 / K);
+        for (int i = 0; i < K; ++i)
+            for (int o = 0; o < 64; ++o) {
+                const double u = (double)(mix64(st) >> 11) * 0x1.0p-53;
+                double w = (2.0 * u - 1.0) * lim;
+                if (l == 3 && o > kMlpErr) w = 0.0;
+                if (l == 3 && o < kMlpRad) w *= 0.5;
+                p[off + (size_t)i * 64 + o] = (float)w;
+            }
+        off += (size_t)K * 64 + 64;
+    }
+}
+
+rc_status rc_surrogate_train(rc_ctx *c, const rc_train_config *cfg, rc_train_report *rep)
+{
+    if (!c || !cfg) return fail(c, RC_ERR_NULL, "rc_surrogate_train: NULL argument");
+    if (cfg->version != RC_ABI_VERSION || cfg->scenes < 1 || cfg->epochs < 1 || cfg->epochs > RC_TRAIN_MAX_EPOCHS ||
+        cfg->batch < 32 || cfg->batch % 32 != 0 || cfg->batch > (1u << 22) || cfg->levels < 3 ||
+        cfg->levels > RC_MAX_LEVELS || !dims_ok(cfg->width, cfg->height) || cfg->n_boxes > RC_MAX_BOXES ||
+        cfg->n_emissive > cfg->n_boxes || !(cfg->lr > 0.0f))
+        return fail(c, RC_ERR_INVALID, "rc_surrogate_train: bad configuration");
+    HIP_TRY(c, hipSetDevice(c->cfg.device));
+    auto now = []() { return std::chrono::steady_clock::now(); };
+    const auto t_start = now();
+
+    // data: a plain context at the training size generates the rows
+    rc_config tcfg{RC_ABI_VERSION, cfg->width, cfg->height, cfg->levels, RC_STORAGE_F32, RC_FLAG_NO_GRAPH,
+                   c->cfg.device, 0};
+    rc_ctx *tc = nullptr;
+    rc_status st = rc_create(&tcfg, &tc);
+    if (st != RC_OK) return fail(c, st, std::string("rc_surrogate_train: ") + rc_last_error(nullptr));
+    struct Guard {
+        rc_ctx *tc;
+        std::vector<void *> dev;
+        ~Guard()
+        {
+            for (void *p : dev)
+                if (p) hipFree(p);
+            rc_destroy(tc);
+        }
+    } guard{tc, {}};
+    const size_t px = (size_t)cfg->width * cfg->height;
+    const size_t P = (size_t)grid_dim(cfg->width, 1) * grid_dim(cfg->height, 1);
+    const size_t N = P * cfg->scenes;
+    float *depth = nullptr, *emis = nullptr, *feats = nullptr, *targets = nullptr;
+    double *boxes = nullptr, *stats = nullptr;
+    int *idx = nullptr;
+    float *partial = nullptr, *loss_part = nullptr, *grad = nullptr, *step_loss = nullptr;
+    HIP_TRY(c, hipMalloc(&depth, px * sizeof(float)));
+    guard.dev.push_back(depth);
+    HIP_TRY(c, hipMalloc(&emis, 3 * px * sizeof(float)));
+    guard.dev.push_back(emis);
+    HIP_TRY(c, hipMalloc(&feats, N * kMlpIn * sizeof(float)));
+    guard.dev.push_back(feats);
+    HIP_TRY(c, hipMalloc(&targets, N * kMlpRad * sizeof(float)));
+    guard.dev.push_back(targets);
+    const size_t nb_scene = cfg->n_boxes ? cfg->n_boxes : 1;
+    std::vector<rc_box> hb((size_t)cfg->scenes * nb_scene);
+    for (uint32_t sc = 0; sc < cfg->scenes; ++sc)
+        scene_generate(cfg->scene_seed0 + sc, (int)cfg->n_boxes, (int)cfg->n_emissive, hb.data() + sc * nb_scene);
+    HIP_TRY(c, hipMalloc(&boxes, hb.size() * sizeof(rc_box)));
+    guard.dev.push_back(boxes);
+    HIP_TRY(c, hipMemcpy(boxes, hb.data(), hb.size() * sizeof(rc_box), hipMemcpyHostToDevice));
+    hipStream_t s = nullptr;
+    for (uint32_t sc = 0; sc < cfg->scenes; ++sc) {
+        HIP_TRY(c, launch_raster(boxes + sc * nb_scene * (sizeof(rc_box) / sizeof(double)), (int)cfg->n_boxes,
+                                 (int)cfg->width, (int)cfg->height, depth, nullptr, emis, s));
+        st = gen_samples(tc, depth, emis, feats + sc * P * kMlpIn, targets + sc * P * kMlpRad, s);
+        if (st != RC_OK) return fail(c, st, std::string("rc_surrogate_train: ") + tc->err);
+    }
+    // normalisation: per-input mean and std, one RMS scale for the targets
+    const int parts = 1024;
+    HIP_TRY(c, hipMalloc(&stats, (size_t)parts * 2 * kMlpIn * sizeof(double)));
+    guard.dev.push_back(stats);
+    std::vector<double> hs((size_t)parts * 2 * kMlpIn);
+    std::vector<float> mean(kMlpIn), istd(kMlpIn);
+    HIP_TRY(c, launch_col_stats(feats, N, kMlpIn, stats, parts, s));
+    HIP_TRY(c, hipMemcpy(hs.data(), stats, hs.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int j = 0; j < kMlpIn; ++j) {
+        double a = 0.0, b = 0.0;
+        for (int p = 0; p < parts; ++p) {
+            a += hs[(size_t)p * 2 * kMlpIn + j];
+            b += hs[(size_t)p * 2 * kMlpIn + kMlpIn + j];
+        }
+        const double mu = a / (double)N, var = fmax(b / (double)N - mu * mu, 0.0);
+        mean[j] = (float)mu;
+        istd[j] = (float)(1.0 / fmax(sqrt(var), 1e-3));
+    }
+    HIP_TRY(c, launch_col_stats(targets, N, kMlpRad, stats, parts, s));
+    HIP_TRY(c, hipMemcpy(hs.data(), stats, hs.size() * sizeof(double), hipMemcpyDeviceToHost));
+    double sq = 0.0;
+    for (int p = 0; p < parts; ++p)
+        for (int j = 0; j < kMlpRad; ++j) sq += hs[(size_t)p * 2 * kMlpRad + kMlpRad + j];
+    const float out_scale = (float)fmax(sqrt(sq / ((double)N * kMlpRad)), 1e-6);
+    HIP_TRY(c, launch_scale(targets, N * kMlpRad, 1.0f / out_scale, s));
+    HIP_TRY(c, hipDeviceSynchronize());
+    const auto t_gen = now();
+
+    // network
+    std::vector<float> params;
+    init_params(cfg->seed, params);
+    st = mlp_upload(c, params.data(), mean.data(), istd.data(), out_scale);
+    if (st != RC_OK) return st;
+    const int batch = (int)cfg->batch;
+    const int blocks = batch / 32;
+    const int nb = blocks > 1024 ? (blocks + 1023) / 1024 : 1;
+    const int wgs = (blocks + nb - 1) / nb;
+    const size_t steps = N / (size_t)batch;
+    if (steps < 1) return fail(c, RC_ERR_INVALID, "rc_surrogate_train: fewer samples than one batch");
+    HIP_TRY(c, hipMalloc(&idx, N * sizeof(int)));
+    guard.dev.push_back(idx);
+    HIP_TRY(c, hipMalloc(&partial, (size_t)wgs * kMlpParams * sizeof(float)));
+    guard.dev.push_back(partial);
+    HIP_TRY(c, hipMalloc(&loss_part, (size_t)wgs * sizeof(float)));
+    guard.dev.push_back(loss_part);
+    HIP_TRY(c, hipMalloc(&grad, kMlpParams * sizeof(float)));
+    guard.dev.push_back(grad);
+    HIP_TRY(c, hipMalloc(&step_loss, steps * sizeof(float)));
+    guard.dev.push_back(step_loss);
+    std::vector<int> perm(N);
+    std::vector<float> hl(steps);
+    uint64_t rs = cfg->seed ^ 0x5DEECE66Dull;
+    if (rep) {
+        memset(rep, 0, sizeof(*rep));
+        rep->version = RC_ABI_VERSION;
+    }
+    for (uint32_t e = 0; e < cfg->epochs; ++e) {
+        for (size_t i = 0; i < N; ++i) perm[i] = (int)i;
+        for (size_t i = N - 1; i > 0; --i) {
+            const size_t j = (size_t)(mix64(rs) % (uint64_t)(i + 1));
+            const int t = perm[i];
+            perm[i] = perm[j];
+            perm[j] = t;
+        }
+        HIP_TRY(c, hipMemcpy(idx, perm.data(), N * sizeof(int), hipMemcpyHostToDevice));
+        const float lr = cfg->lr * powf(cfg->lr_decay > 0.0f ? cfg->lr_decay : 1.0f, (float)e);
+        for (size_t k = 0; k < steps; ++k) {
+            HIP_TRY(c, launch_mlp_train(c->mlp, feats, targets, idx + k * batch, batch, nb, partial, loss_part, wgs, s));
+            HIP_TRY(c, launch_mlp_reduce(partial, wgs, grad, s));
+            HIP_TRY(c, launch_sum_f32(loss_part, wgs, step_loss + k, s));
+            st = adam_step(c, grad, lr, s);
+            if (st != RC_OK) return st;
+        }
+        HIP_TRY(c, hipMemcpy(hl.data(), step_loss, steps * sizeof(float), hipMemcpyDeviceToHost));
+        double tot = 0.0;
+        for (float v : hl) tot += v;
+        if (rep) rep->epoch_loss[e] = (float)(tot / (double)steps);
+    }
+    HIP_TRY(c, hipDeviceSynchronize());
+    c->train_scenes = cfg->scenes;
+    c->mlp_ready = true;
+    drop_graph(c);
+    if (rep) {
+        rep->epochs = cfg->epochs;
+        rep->samples = N;
+        rep->steps = steps * cfg->epochs;
+        rep->gen_ms = std::chrono::duration<double, std::milli>(t_gen - t_start).count();
+        rep->train_ms = std::chrono::duration<double, std::milli>(now() - t_gen).count();
+        rep->out_scale = out_scale;
+    }
+    return RC_OK;
+}
+
+rc_status rc_debug_surrogate(rc_ctx *c, float *cone, float *level1, float *tile_err, void *stream)
+{
+    if (!c) return fail(nullptr, RC_ERR_NULL, "rc_debug_surrogate: ctx is NULL");
+    if (!c->surr_ran) return fail(c, RC_ERR_STATE, "rc_debug_surrogate: no learned-far-field frame yet");
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(c, hipSetDevice(c->cfg.device));
+    const size_t Q = (size_t)grid_dim(c->W, 2) * grid_dim(c->H, 2), P = (size_t)grid_dim(c->W, 1) * grid_dim(c->H, 1);
+    if (cone) HIP_TRY(c, hipMemcpyAsync(cone, c->cone, Q * 16 * sizeof(float4), hipMemcpyDeviceToDevice, s));
+    if (level1) HIP_TRY(c, launch_level_to_f32(c->surr_a1, c->storage, P * 4, level1, s));
+    if (tile_err)
+        HIP_TRY(c, hipMemcpyAsync(tile_err, c->tile_err, (size_t)c->surr_tiles * sizeof(float), hipMemcpyDeviceToDevice, s));
+    return RC_OK;
+}
+
+rc_status rc_debug_mip(rc_ctx *c, uint32_t level, float *emc, float *mo, void *stream)
+{
+    if (!c) return fail(nullptr, RC_ERR_NULL, "rc_debug_mip: ctx is NULL");
+    if (!c->surr_alloc || !c->surr_ran) return fail(c, RC_ERR_STATE, "rc_debug_mip: no learned-far-field frame yet");
+    if (level < 1 || (int)level > c->mip.levels) return fail(c, RC_ERR_INVALID, "rc_debug_mip: level out of range");
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(c, hipSetDevice(c->cfg.device));
+    const size_t n = (size_t)c->mip.w[level] * c->mip.h[level];
+    if (emc) HIP_TRY(c, hipMemcpyAsync(emc, c->mip.emc + c->mip.off[level], n * sizeof(float4), hipMemcpyDeviceToDevice, s));
+    if (mo) HIP_TRY(c, hipMemcpyAsync(mo, c->mip.mo + c->mip.off[level], n * sizeof(float), hipMemcpyDeviceToDevice, s));
     return RC_OK;
 }
 

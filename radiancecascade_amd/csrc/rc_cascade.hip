@@ -22,6 +22,7 @@
 // level's probe neighbourhood for its direction slice in LDS (one global read
 // per texel instead of 16 L2 reads per ray).  Rays that hit skip the merge.
 #include "rc_internal.hpp"
+#include "rc_march.hpp"
 
 namespace rc {
 
@@ -30,96 +31,6 @@ __device__ inline int xcd_remap_c(int b, int nb)
     const int q = nb >> 3, r = nb & 7;
     const int xcd = b & 7, idx = b >> 3;
     return xcd * q + min(xcd, r) + idx;
-}
-
-// Sum over a lane quad in the order (v0+v1)+(v2+v3).  Every lane of the quad
-// ends with the same bits (fp addition is commutative).
-__device__ inline float quad_sum(float v)
-{
-    v = v + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));  // lane ^ 1
-    v = v + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));  // lane ^ 2
-    return v;
-}
-
-// The distance field is read through a buffer resource: the tiled index is a
-// 32-bit lane offset (no 64-bit address arithmetic per step), and a lane whose
-// sample is off-image may compute any offset -- the load is range-checked and
-// its value unused.
-constexpr int kBufferDword3 = 0x00020000;  // gfx9-family raw buffer, 32-bit data
-
-__device__ inline __amdgpu_buffer_rsrc_t sdf_rsrc(const float *sdf, int W, int H)
-{
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(sdf), (short)0, (int)(sdf_elems(W, H) * 4),
-                                             kBufferDword3);
-}
-
-// Row-major pixel index q = y*W + x (24-bit multiply: W, y < 2^14) -- the
-// hit code and the emissive index.
-__device__ inline int pixel_index(int x, int y, int W) { return (int)__umul24((uint32_t)y, (uint32_t)W) + x; }
-
-// Tiled SDF load.  For an off-image (x, y) the offset may be anything; the
-// range-checked load is then unused.
-__device__ inline float sdf_load(__amdgpu_buffer_rsrc_t r, int x, int y, int tiles_x)
-{
-    const uint32_t ux = (uint32_t)x, uy = (uint32_t)y;
-    const uint32_t off = (__umul24(uy >> 2, (uint32_t)tiles_x << 5) + ((ux & ~7u) << 2) + ((uy & 3u) << 3) + (ux & 7u)) << 2;
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
-}
-
-// max(d, 1) as one v_med3 (no NaN canonicalisation: d is never NaN).  For
-// d = +inf it gives FLT_MAX instead of inf; either ends the interval.
-__device__ inline float march_step(float d) { return __builtin_amdgcn_fmed3f(d, 1.0f, 3.4028235e38f); }
-
-// Ray codes: >= 0 hit pixel y*W+x, -1 left the image, -2 reached t1.
-constexpr int kRayExit = -1, kRayT1 = -2;
-
-// RPT rays per lane marched in lockstep: each iteration issues the RPT
-// distance-field loads back to back, so a lane keeps RPT independent load
-// chains in flight (the march is a chain of dependent gathers and is bound by
-// their latency, not by bandwidth).  A finished ray's slot keeps issuing a
-// (cached, ignored) load until every ray of the lane is done.
-template <int RPT>
-__device__ inline void march_rays(__amdgpu_buffer_rsrc_t sdf, int W, int H, const float *cx, const float *cy,
-                                  float dx, float dy, float t0, float t1, const bool *active, int *code)
-{
-    float t[RPT];
-    bool alive[RPT];
-#pragma unroll
-    for (int j = 0; j < RPT; ++j) {
-        t[j] = t0;
-        alive[j] = active[j] && t0 < t1;
-        code[j] = kRayT1;
-    }
-    for (;;) {
-        int q[RPT];
-        float d[RPT];
-#pragma unroll
-        for (int j = 0; j < RPT; ++j) {
-            // positions are finite, so the integer test equals 0 <= x < W, 0 <= y < H
-            const int ix = __float2int_rd(fmaf(t[j], dx, cx[j]));  // floor, one instruction
-            const int iy = __float2int_rd(fmaf(t[j], dy, cy[j]));
-            const bool inside = (uint32_t)ix < (uint32_t)W && (uint32_t)iy < (uint32_t)H;
-            if (alive[j] && !inside) code[j] = kRayExit;
-            alive[j] = alive[j] && inside;
-            q[j] = pixel_index(ix, iy, W);
-            d[j] = sdf_load(sdf, ix, iy, sdf_tiles_x(W));
-        }
-        bool any = false;
-#pragma unroll
-        for (int j = 0; j < RPT; ++j) {
-            if (alive[j]) {
-                if (d[j] < 0.5f) {
-                    code[j] = q[j];
-                    alive[j] = false;
-                } else {
-                    t[j] = t[j] + march_step(d[j]);
-                    alive[j] = t[j] < t1;
-                }
-            }
-            any = any || alive[j];
-        }
-        if (!any) break;
-    }
 }
 
 // One ray with SPEC-deep speculation.  After a sample with distance d the
@@ -195,18 +106,6 @@ __device__ inline int march_ray_spec(__amdgpu_buffer_rsrc_t sdf, int W, int H, f
     Ray r = ray_begin(cx, cy, dx, dy, t0, t1, active);
     while (r.alive) ray_round<SPEC>(r, sdf, W, H, t1);
     return r.code;
-}
-
-// Bilinear of four staged texels with the lower probe's parity weights.
-__device__ inline float3 bilerp4(float4 u00, float4 u10, float4 u01, float4 u11, int x, int y)
-{
-    const float wx1 = (x & 1) ? 0.25f : 0.75f, wx0 = 1.0f - wx1;
-    const float wy1 = (y & 1) ? 0.25f : 0.75f, wy0 = 1.0f - wy1;
-    float3 r;
-    r.x = fmaf(wy1, fmaf(wx1, u11.x, wx0 * u01.x), wy0 * fmaf(wx1, u10.x, wx0 * u00.x));
-    r.y = fmaf(wy1, fmaf(wx1, u11.y, wx0 * u01.y), wy0 * fmaf(wx1, u10.y, wx0 * u00.y));
-    r.z = fmaf(wy1, fmaf(wx1, u11.z, wx0 * u01.z), wy0 * fmaf(wx1, u10.z, wx0 * u00.z));
-    return r;
 }
 
 // Level i >= 1.  Workgroup tile: TX x (TY*RPT) probes x KD directions; lane

@@ -71,4 +71,96 @@ size_t norm_workspace_bytes();
 hipError_t launch_error_norms(const float *x, const void *ref, bool ref_f64, size_t n, double *workspace,
                               double *out, hipStream_t s);
 
+// ---- surrogate MLP (rc_mlp.hip) ---------------------------------------------
+// 48 -> 64 -> 64 -> 64 -> 64, ReLU on the hidden layers; outputs 0..47 the
+// far-field residual (16 directions x rgb, normalised), 48 the log of the
+// predicted squared error, 49..63 padding.
+constexpr int kMlpIn = 48, kMlpHid = 64, kMlpOut = 64, kMlpRad = 48, kMlpErr = 48;
+constexpr int kMlpParams = kMlpIn * 64 + 64 + 3 * (64 * 64 + 64);  // 15616 fp32 master parameters
+constexpr int kMlpWt = 64 * kMlpIn + 3 * 64 * kMlpHid;            // bf16 weights per copy
+constexpr float kMlpErrEps = 1e-6f, kMlpErrWeight = 0.05f;
+
+struct MlpDev {
+    float *master;          // [kMlpParams]: for l = 0..3, W_l [K_l][64] (input-major) then b_l [64]
+    float *adam_m, *adam_v; // Adam moments, same layout
+    uint16_t *wt;           // bf16 W_l^T [64][K_l] (output-major), layers back to back
+    uint16_t *wk;           // bf16 W_l [K_l][64] (input-major)
+    float *bias;            // [4][64]
+    float *in_mean, *in_istd;  // [48] input standardisation
+};
+
+hipError_t launch_mlp_infer(const MlpDev &m, const float *feats, int n, float *out, hipStream_t s);
+hipError_t launch_mlp_train(const MlpDev &m, const float *feats, const float *targets, const int *idx, int batch,
+                            int nb, float *partial, float *loss_part, int parts, hipStream_t s);
+hipError_t launch_mlp_reduce(const float *partial, int parts, float *grad, hipStream_t s);
+hipError_t launch_mlp_adam(const MlpDev &m, const float *grad, float lr, float b1, float b2, float eps, float c1,
+                           float c2, hipStream_t s);
+hipError_t launch_mlp_refresh(const MlpDev &m, hipStream_t s);
+hipError_t launch_sum_f32(const float *x, int n, float *out, hipStream_t s);
+
+// ---- surrogate far field (rc_surrogate.hip) ---------------------------------
+// Emitter / occluder mip pyramid: level m (1..levels) has cells of 2^m x 2^m
+// pixels, ceil(W/2^m) x ceil(H/2^m) of them; cell = mean over its pixels
+// (pixels outside the image count as empty) of (E*[emitter], [emitter]) in
+// emc and of [occupied and not emitter] in mo.
+constexpr int kMipMax = 14;
+struct MipArgs {
+    float4 *emc;            // all levels back to back
+    float *mo;
+    int levels;             // top level index (>= 1)
+    int w[kMipMax + 1], h[kMipMax + 1];
+    size_t off[kMipMax + 1];
+};
+// Cone trace schedule: every cone takes the same distances t_n, so the mip
+// level m_n and the occluder path length len_n are uniform over a launch.
+constexpr int kConeMaxSteps = 64;
+struct ConeSchedule {
+    int n;
+    float t[kConeMaxSteps];
+    float len[kConeMaxSteps];
+    int m[kConeMaxSteps];
+};
+struct ConeArgs {
+    MipArgs mip;
+    ConeSchedule sch;
+    const float2 *dirs;     // the 16 level-1 directions
+    float4 *cone;           // [QY][QX][16]: far-field radiance rgb, transmittance
+    int QX, QY;             // level-2 probe grid
+    float spacing;          // level-2 probe spacing (8)
+};
+struct SurrLevel1Args {
+    const float *sdf;
+    const float *emissive;
+    const float2 *dirs;     // level-1 directions
+    const float4 *cone;
+    void *out;              // merged level 1 (storage texels)
+    float *tile_err;        // [tiles] mean predicted squared error (normalised)
+    int *flag_count;        // tiles above threshold (counter)
+    int *flag_list;         // their indices
+    int W, H, PX, PY, QX, QY, tiles_x, tiles;
+    float out_scale, threshold;
+};
+struct SurrSampleArgs {
+    const float4 *cone;
+    const void *upper;      // merged level 2 of the plain path (storage texels)
+    float *feats;           // [rows][48] log1p(cone radiance)
+    float *targets;         // [rows][48] upper bilinear minus cone radiance
+    int PX, PY, QX, QY;
+};
+
+void mip_layout(int W, int H, int top_level, MipArgs &m);   // fills w, h, off, levels
+size_t mip_cells(const MipArgs &m);
+void cone_schedule(int W, int H, int top_level, ConeSchedule &s);
+int cone_top_level(int W, int H);
+hipError_t launch_near_sdf(const uint32_t *seed, float *sdf, int W, int H, hipStream_t s);
+hipError_t launch_mip(const uint32_t *seed, const float *emissive, const MipArgs &m, int W, int H, int *zero_me,
+                      hipStream_t s);
+hipError_t launch_cone(const ConeArgs &a, hipStream_t s);
+hipError_t launch_surr_level1(const SurrLevel1Args &a, const MlpDev &m, Storage st, hipStream_t s);
+hipError_t launch_surr_sample(const SurrSampleArgs &a, Storage st, hipStream_t s);
+hipError_t launch_tile_patch(const void *plain, void *surr, const float *tile_err, float threshold, int PX, int PY,
+                             Storage st, hipStream_t s);
+hipError_t launch_col_stats(const float *x, size_t rows, int cols, double *partial, int parts, hipStream_t s);
+hipError_t launch_scale(float *x, size_t n, float k, hipStream_t s);
+
 }  // namespace rc

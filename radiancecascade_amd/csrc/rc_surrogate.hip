@@ -1,0 +1,681 @@
+// rc_surrogate.hip -- the learned far field: the levels >= 2 of a frame are
+// replaced by a cone-traced estimate of the radiance arriving from beyond the
+// level-1 interval, corrected per level-1 probe by the MLP (rc_mlp.hip).
+//
+// None of this exists in the reference (it has no global illumination at
+// all; its only pass is the forward raster at src/renderer.rs:559-613).  The
+// frame is
+//   seed init (rc_jfa.hip)                      occupancy
+//   k_near_sdf                                  exact distance field up to 10 px
+//   k_mip / k_mip_top                           emitter / occluder mip pyramid
+//   k_cone                                      16 cones per level-2 probe
+//   k_surr_level1                               level-1 march + MLP + merge
+//   k_level0 (rc_cascade.hip)                   level 0 + gather
+// Cone tracing through a prefiltered pyramid is Crassin et al.'s voxel cone
+// tracing (2011) in screen space; here it is the MLP's input feature, not the
+// answer.
+//
+// Cone model (per level-2 probe c and level-1 direction k, 16 cones of
+// half-angle pi/16): samples at the schedule's distances t_n, bilinear in the
+// mip level m_n whose cells match the cone width; occluder opacity
+// a_o = 1 - (1 - min(mo, 0.999))^len_n (a thin silhouette blocks the whole
+// cone cross-section it crosses), total opacity a = min(me + a_o, 1);
+// L += T * emc.rgb, T *= 1 - a; a cone stops once T < 1e-4.
+#include <math.h>
+
+#include "rc_internal.hpp"
+#include "rc_march.hpp"
+
+namespace rc {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+constexpr uint32_t kFarSeedS = 0x7FFF7FFFu;  // the jump flood's internal "no seed" value
+
+// ---- near-field distance field ---------------------------------------------
+// Exact Euclidean distance to the nearest occupied pixel wherever it is at most
+// kNearR pixels, +inf elsewhere.  Levels 0 and 1 end their intervals at 2 and
+// 10 px, so every step they take from such a field (a hit needs distance 0; a
+// distance above 10 ends the interval in one step either way) equals the step
+// they take from the full field.  Per 32x32 tile: occupancy row masks of the
+// tile plus a 10-px halo (one ballot per row), the distance to the nearest
+// occupied pixel in the row within +-10 (count of zero bits), then
+// d^2 = min over |dy| <= 10 of g(y+dy)^2 + dy^2.
+constexpr int kNearR = 10, kNearT = 32, kNearS = kNearT + 2 * kNearR;
+
+__global__ __launch_bounds__(kBlock) void k_near_sdf(const uint32_t *__restrict__ seed, float *__restrict__ sdf, int W,
+                                                     int H, int tiles_x)
+{
+    __shared__ uint64_t mask[kNearS];
+    __shared__ uint8_t g[kNearS][kNearT];
+    const int x0 = (blockIdx.x % tiles_x) * kNearT, y0 = (blockIdx.x / tiles_x) * kNearT;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int r = wave; r < kNearS; r += kBlock / 64) {
+        const int y = y0 - kNearR + r, x = x0 - kNearR + lane;
+        bool occ = false;
+        if (lane < kNearS && x >= 0 && x < W && y >= 0 && y < H) occ = seed[(size_t)y * W + x] != kFarSeedS;
+        const uint64_t m = __ballot(occ);
+        if (lane == 0) mask[r] = m;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kNearS * kNearT; i += kBlock) {
+        const int r = i / kNearT, j = i % kNearT, b = j + kNearR;
+        const uint64_t m = mask[r];
+        const uint64_t right = m >> b;         // bit d: column b + d
+        const uint64_t left = m << (63 - b);   // bit 63 - d: column b - d
+        int d = kNearR + 1;
+        if (right) d = min(d, (int)__builtin_ctzll(right));
+        if (left) d = min(d, (int)__builtin_clzll(left));
+        g[r][j] = (uint8_t)d;
+    }
+    __syncthreads();
+    const int tx = sdf_tiles_x(W);
+    for (int i = threadIdx.x; i < kNearT * kNearT; i += kBlock) {
+        const int j = i % kNearT, r = i / kNearT, x = x0 + j, y = y0 + r;
+        if (x >= W || y >= H) continue;
+        int best = kNearR * kNearR + 1;
+#pragma unroll
+        for (int dy = -kNearR; dy <= kNearR; ++dy) {
+            const int gv = g[r + kNearR + dy][j];
+            best = min(best, gv * gv + dy * dy);
+        }
+        sdf[sdf_index(x, y, tx)] = best <= kNearR * kNearR ? (float)sqrt((double)best) : __builtin_huge_valf();
+    }
+}
+
+hipError_t launch_near_sdf(const uint32_t *seed, float *sdf, int W, int H, hipStream_t s)
+{
+    const int tx = (W + kNearT - 1) / kNearT, ty = (H + kNearT - 1) / kNearT;
+    hipLaunchKernelGGL(k_near_sdf, dim3(tx * ty), dim3(kBlock), 0, s, seed, sdf, W, H, tx);
+    return hipGetLastError();
+}
+
+// ---- mip pyramid -------------------------------------------------------------
+int cone_top_level(int W, int H)
+{
+    int top = 1;
+    while ((1 << top) < W || (1 << top) < H) ++top;
+    return top < kMipMax ? top : kMipMax;
+}
+
+void mip_layout(int W, int H, int top, MipArgs &m)
+{
+    m.levels = top;
+    size_t off = 0;
+    for (int l = 0; l <= kMipMax; ++l) {
+        if (l == 0 || l > top) {
+            m.w[l] = m.h[l] = 0;
+            m.off[l] = off;
+            continue;
+        }
+        m.w[l] = (W + (1 << l) - 1) >> l;
+        m.h[l] = (H + (1 << l) - 1) >> l;
+        m.off[l] = off;
+        off += (size_t)m.w[l] * m.h[l];
+    }
+}
+
+size_t mip_cells(const MipArgs &m) { return m.off[m.levels] + (size_t)m.w[m.levels] * m.h[m.levels]; }
+
+// Levels 1..6 of a 64x64-pixel tile in LDS (cells are aligned to the tile).
+constexpr int kMipTile = 64, kMipInTile = 6;
+
+__global__ __launch_bounds__(kBlock) void k_mip(const uint32_t *__restrict__ seed, const float *__restrict__ emissive,
+                                                MipArgs m, int W, int H, int tiles_x)
+{
+    __shared__ float4 se[32 * 32];
+    __shared__ float so[32 * 32];
+    const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
+    const size_t plane = (size_t)W * H;
+    for (int c = threadIdx.x; c < 32 * 32; c += kBlock) {
+        const int cx = c & 31, cy = c >> 5;
+        float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
+        float o = 0.f;
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+            const int x = tx * kMipTile + 2 * cx + (d & 1), y = ty * kMipTile + 2 * cy + (d >> 1);
+            if (x < W && y < H) {
+                const size_t p = (size_t)y * W + x;
+                const float r = emissive[p], g = emissive[plane + p], b = emissive[2 * plane + p];
+                if (r > 0.f || g > 0.f || b > 0.f) {
+                    e.x += r;
+                    e.y += g;
+                    e.z += b;
+                    e.w += 1.f;
+                } else if (seed[p] != kFarSeedS) {
+                    o += 1.f;
+                }
+            }
+        }
+        e = make_float4(e.x * 0.25f, e.y * 0.25f, e.z * 0.25f, e.w * 0.25f);
+        o *= 0.25f;
+        se[c] = e;
+        so[c] = o;
+        const int gx = tx * 32 + cx, gy = ty * 32 + cy;
+        if (gx < m.w[1] && gy < m.h[1]) {
+            m.emc[m.off[1] + (size_t)gy * m.w[1] + gx] = e;
+            m.mo[m.off[1] + (size_t)gy * m.w[1] + gx] = o;
+        }
+    }
+    __syncthreads();
+    int n = 32;
+    const int last = m.levels < kMipInTile ? m.levels : kMipInTile;
+    for (int l = 2; l <= last; ++l) {
+        const int nn = n >> 1;
+        const bool have = (int)threadIdx.x < nn * nn;
+        float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
+        float o = 0.f;
+        if (have) {
+            const int cx = threadIdx.x % nn, cy = threadIdx.x / nn;
+            const int i00 = (2 * cy) * n + 2 * cx;
+            const float4 a = se[i00], b = se[i00 + 1], c = se[i00 + n], d = se[i00 + n + 1];
+            e = make_float4(0.25f * ((a.x + b.x) + (c.x + d.x)), 0.25f * ((a.y + b.y) + (c.y + d.y)),
+                            0.25f * ((a.z + b.z) + (c.z + d.z)), 0.25f * ((a.w + b.w) + (c.w + d.w)));
+            o = 0.25f * ((so[i00] + so[i00 + 1]) + (so[i00 + n] + so[i00 + n + 1]));
+        }
+        __syncthreads();
+        if (have) {
+            se[threadIdx.x] = e;
+            so[threadIdx.x] = o;
+            const int cx = threadIdx.x % nn, cy = threadIdx.x / nn;
+            const int gx = tx * nn + cx, gy = ty * nn + cy;
+            if (gx < m.w[l] && gy < m.h[l]) {
+                m.emc[m.off[l] + (size_t)gy * m.w[l] + gx] = e;
+                m.mo[m.off[l] + (size_t)gy * m.w[l] + gx] = o;
+            }
+        }
+        __syncthreads();
+        n = nn;
+    }
+}
+
+// Levels 7..top from level 6, one workgroup; also clears the fallback counter
+// of the frame (a kernel, not a memset node, so graph replays stay exact).
+__global__ __launch_bounds__(kBlock) void k_mip_top(MipArgs m, int *zero_me)
+{
+    if (zero_me && threadIdx.x == 0) *zero_me = 0;
+    for (int l = kMipInTile + 1; l <= m.levels; ++l) {
+        const int w = m.w[l], h = m.h[l], cw = m.w[l - 1], ch = m.h[l - 1];
+        const float4 *ce = m.emc + m.off[l - 1];
+        const float *co = m.mo + m.off[l - 1];
+        for (int i = threadIdx.x; i < w * h; i += kBlock) {
+            const int x = i % w, y = i / w;
+            float4 v[4];
+            float o[4];
+#pragma unroll
+            for (int d = 0; d < 4; ++d) {
+                const int cx = 2 * x + (d & 1), cy = 2 * y + (d >> 1);
+                const bool in = cx < cw && cy < ch;
+                v[d] = in ? ce[(size_t)cy * cw + cx] : make_float4(0.f, 0.f, 0.f, 0.f);
+                o[d] = in ? co[(size_t)cy * cw + cx] : 0.f;
+            }
+            m.emc[m.off[l] + i] = make_float4(0.25f * ((v[0].x + v[1].x) + (v[2].x + v[3].x)),
+                                              0.25f * ((v[0].y + v[1].y) + (v[2].y + v[3].y)),
+                                              0.25f * ((v[0].z + v[1].z) + (v[2].z + v[3].z)),
+                                              0.25f * ((v[0].w + v[1].w) + (v[2].w + v[3].w)));
+            m.mo[m.off[l] + i] = 0.25f * ((o[0] + o[1]) + (o[2] + o[3]));
+        }
+        __threadfence_block();
+        __syncthreads();
+    }
+}
+
+hipError_t launch_mip(const uint32_t *seed, const float *emissive, const MipArgs &m, int W, int H, int *zero_me,
+                      hipStream_t s)
+{
+    const int tx = (W + kMipTile - 1) / kMipTile, ty = (H + kMipTile - 1) / kMipTile;
+    hipLaunchKernelGGL(k_mip, dim3(tx * ty), dim3(kBlock), 0, s, seed, emissive, m, W, H, tx);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_mip_top, dim3(1), dim3(kBlock), 0, s, m, zero_me);
+    return hipGetLastError();
+}
+
+// ---- cone trace --------------------------------------------------------------
+// Schedule: t_0 = 6 px from the level-2 probe centre (a level-1 probe sits 2.8
+// px from it and its far field starts at 10 px), cone diameter
+// dia = max(2 t tan(pi/16), 1), mip level floor(log2 dia) clamped to
+// [1, top], next t = t + 0.75 dia, up to the image diagonal + 2.
+void cone_schedule(int W, int H, int top, ConeSchedule &s)
+{
+    const double tmax = sqrt((double)W * W + (double)H * H) + 2.0, tn = tan(M_PI / 16.0);
+    double t = 6.0;
+    s.n = 0;
+    while (t < tmax && s.n < kConeMaxSteps) {
+        const double dia = fmax(2.0 * t * tn, 1.0);
+        int m = (int)floor(log2(dia));
+        m = m < 1 ? 1 : (m > top ? top : m);
+        const double step = 0.75 * dia;
+        s.t[s.n] = (float)t;
+        s.m[s.n] = m;
+        s.len[s.n] = (float)fmin(step, (double)(1 << m));
+        ++s.n;
+        t += step;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_cone(ConeArgs a)
+{
+    const int gi = blockIdx.x * kBlock + threadIdx.x;
+    const int k = gi & 15, q = gi >> 4;
+    if (q >= a.QX * a.QY) return;
+    const int qx = q % a.QX, qy = q / a.QX;
+    const float cx = ((float)qx + 0.5f) * a.spacing, cy = ((float)qy + 0.5f) * a.spacing;
+    const float2 d = a.dirs[k];
+    float T = 1.0f, r = 0.f, g = 0.f, b = 0.f;
+    for (int n = 0; n < a.sch.n; ++n) {
+        const int m = a.sch.m[n];
+        const float t = a.sch.t[n];
+        const float inv = 1.0f / (float)(1 << m);
+        const float fx = fmaf(t, d.x, cx) * inv - 0.5f, fy = fmaf(t, d.y, cy) * inv - 0.5f;
+        const float flx = floorf(fx), fly = floorf(fy);
+        const int ix = (int)flx, iy = (int)fly;
+        const float ax = fx - flx, ay = fy - fly;
+        const int w = a.mip.w[m], h = a.mip.h[m];
+        const float4 *E = a.mip.emc + a.mip.off[m];
+        const float *O = a.mip.mo + a.mip.off[m];
+        float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
+        float o = 0.f;
+#pragma unroll
+        for (int tap = 0; tap < 4; ++tap) {
+            const int x = ix + (tap & 1), y = iy + (tap >> 1);
+            const float wt = ((tap & 1) ? ax : 1.0f - ax) * ((tap >> 1) ? ay : 1.0f - ay);
+            if (x >= 0 && y >= 0 && x < w && y < h) {
+                const float4 v = E[(size_t)y * w + x];
+                e.x = fmaf(wt, v.x, e.x);
+                e.y = fmaf(wt, v.y, e.y);
+                e.z = fmaf(wt, v.z, e.z);
+                e.w = fmaf(wt, v.w, e.w);
+                o = fmaf(wt, O[(size_t)y * w + x], o);
+            }
+        }
+        const float ao = 1.0f - exp2f(a.sch.len[n] * log2f(1.0f - fminf(o, 0.999f)));
+        const float al = fminf(e.w + ao, 1.0f);
+        r = fmaf(T, e.x, r);
+        g = fmaf(T, e.y, g);
+        b = fmaf(T, e.z, b);
+        T = T * (1.0f - al);
+        if (T < 1e-4f) break;
+    }
+    a.cone[gi] = make_float4(r, g, b, T);
+}
+
+hipError_t launch_cone(const ConeArgs &a, hipStream_t s)
+{
+    const long long threads = (long long)a.QX * a.QY * 16;
+    hipLaunchKernelGGL(k_cone, dim3((unsigned)((threads + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+// ---- level 1 with the learned far field ----------------------------------------
+// A workgroup owns 8x8 level-1 probes per iteration (persistent over tiles, so
+// the bf16 weights are staged in LDS once).  Lane (probe p, quad ko) marches
+// the 4 directions 4ko..4ko+3 of p over [2, 10) on the distance field, and
+// builds their 12 inputs: log1p of the cone radiance bilinearly interpolated
+// from the level-2 grid (the merge's weights).  Wave w then computes the
+// 32x32 output block (rows 32(w&1), columns 32(w>>1)) of each MLP layer on
+// MFMA.  The far field is B = max(c + s * z, 0); a ray that reaches t1 takes
+// B, a hit takes E(hit); the quad is reduced to the merged level-1 texel
+// exactly as the plain merge does.  The error head exp(z_48), averaged over
+// the tile, is the tile's predicted mean squared error (normalised units).
+constexpr int kSStrideIn = kMlpIn + 8, kSStrideHid = kMlpHid + 8;
+
+__device__ inline bf16x8 ld8s(const uint16_t *p) { return *reinterpret_cast<const bf16x8 *>(p); }
+__device__ inline int acc_row_s(int v, int lane) { return ((v >> 2) << 3) + ((lane >> 5) << 2) + (v & 3); }
+
+template <int K>
+__device__ inline floatx16 mfma_block(const uint16_t *in, int sin, int rb, const uint16_t *w, int c0)
+{
+    const int lane = threadIdx.x & 63, r = lane & 31, kh = (lane >> 5) * 8;
+    floatx16 c = {};
+#pragma unroll
+    for (int k0 = 0; k0 < K; k0 += 16)
+        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ld8s(in + (rb + r) * sin + k0 + kh), ld8s(w + (c0 + r) * K + k0 + kh),
+                                                    c, 0, 0, 0);
+    return c;
+}
+
+template <int RPT>
+__device__ inline void march_rays_dirs(__amdgpu_buffer_rsrc_t sdf, int W, int H, float cx, float cy, const float *dx,
+                                       const float *dy, float t0, float t1, bool active, int *code)
+{
+    float t[RPT];
+    bool alive[RPT];
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+        t[j] = t0;
+        alive[j] = active && t0 < t1;
+        code[j] = kRayT1;
+    }
+    for (;;) {
+        int q[RPT];
+        float d[RPT];
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            const int ix = __float2int_rd(fmaf(t[j], dx[j], cx));
+            const int iy = __float2int_rd(fmaf(t[j], dy[j], cy));
+            const bool inside = (uint32_t)ix < (uint32_t)W && (uint32_t)iy < (uint32_t)H;
+            if (alive[j] && !inside) code[j] = kRayExit;
+            alive[j] = alive[j] && inside;
+            q[j] = pixel_index(ix, iy, W);
+            d[j] = sdf_load(sdf, ix, iy, sdf_tiles_x(W));
+        }
+        bool any = false;
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            if (alive[j]) {
+                if (d[j] < 0.5f) {
+                    code[j] = q[j];
+                    alive[j] = false;
+                } else {
+                    t[j] = t[j] + march_step(d[j]);
+                    alive[j] = t[j] < t1;
+                }
+            }
+            any = any || alive[j];
+        }
+        if (!any) break;
+    }
+}
+
+template <Storage S>
+__global__ __launch_bounds__(kBlock) void k_surr_level1(SurrLevel1Args a, MlpDev m)
+{
+    using TT = Texel<S>;
+    __shared__ __attribute__((aligned(16))) uint16_t wt_s[kMlpWt];
+    __shared__ float bias_s[4 * 64], mean_s[kMlpIn], istd_s[kMlpIn];
+    __shared__ __attribute__((aligned(16))) uint16_t xs[64 * kSStrideIn];
+    __shared__ __attribute__((aligned(16))) uint16_t ha[64 * kSStrideHid], hb[64 * kSStrideHid];
+    __shared__ float zs[64][kMlpErr + 2];
+    __shared__ float red[kBlock / 64][2];
+
+    for (int i = threadIdx.x; i < kMlpWt / 8; i += kBlock)
+        reinterpret_cast<uint4 *>(wt_s)[i] = reinterpret_cast<const uint4 *>(m.wt)[i];
+    for (int i = threadIdx.x; i < 4 * 64; i += kBlock) bias_s[i] = m.bias[i];
+    for (int i = threadIdx.x; i < kMlpIn; i += kBlock) {
+        mean_s[i] = m.in_mean[i];
+        istd_s[i] = m.in_istd[i];
+    }
+    __syncthreads();
+    const uint16_t *wl[4] = {wt_s, wt_s + 64 * kMlpIn, wt_s + 64 * kMlpIn + 64 * kMlpHid,
+                             wt_s + 64 * kMlpIn + 2 * 64 * kMlpHid};
+
+    const __amdgpu_buffer_rsrc_t sdf = sdf_rsrc(a.sdf, a.W, a.H);
+    const int p = threadIdx.x >> 2, ko = threadIdx.x & 3;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int rb = (wave & 1) * 32, c0 = (wave >> 1) * 32;
+    const size_t plane = (size_t)a.W * a.H;
+    float dx[4], dy[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float2 d = a.dirs[4 * ko + j];
+        dx[j] = d.x;
+        dy[j] = d.y;
+    }
+
+#pragma unroll 1
+    for (int tile = blockIdx.x; tile < a.tiles; tile += gridDim.x) {
+        const int px = (tile % a.tiles_x) * 8 + (p & 7), py = (tile / a.tiles_x) * 8 + (p >> 3);
+        const bool valid = px < a.PX && py < a.PY;
+        const float pcx = ((float)px + 0.5f) * 4.0f, pcy = ((float)py + 0.5f) * 4.0f;
+        int code[4];
+        march_rays_dirs<4>(sdf, a.W, a.H, pcx, pcy, dx, dy, 2.0f, 10.0f, valid, code);
+
+        // cone features of the 4 directions (bilinear over the level-2 grid)
+        const int bx = (px - 1) >> 1, by = (py - 1) >> 1;
+        const int qx0 = min(max(bx, 0), a.QX - 1), qx1 = min(max(bx + 1, 0), a.QX - 1);
+        const int qy0 = min(max(by, 0), a.QY - 1), qy1 = min(max(by + 1, 0), a.QY - 1);
+        float3 cf[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int k = 4 * ko + j;
+            if (valid) {
+                const float4 u00 = a.cone[((size_t)qy0 * a.QX + qx0) * 16 + k];
+                const float4 u10 = a.cone[((size_t)qy0 * a.QX + qx1) * 16 + k];
+                const float4 u01 = a.cone[((size_t)qy1 * a.QX + qx0) * 16 + k];
+                const float4 u11 = a.cone[((size_t)qy1 * a.QX + qx1) * 16 + k];
+                cf[j] = bilerp4(u00, u10, u01, u11, px, py);
+            } else {
+                cf[j] = make_float3(0.f, 0.f, 0.f);
+            }
+            const int col = 3 * k;
+            xs[p * kSStrideIn + col + 0] =
+                (uint16_t)f32_to_bf16_rne(valid ? (log1pf(cf[j].x) - mean_s[col + 0]) * istd_s[col + 0] : 0.f);
+            xs[p * kSStrideIn + col + 1] =
+                (uint16_t)f32_to_bf16_rne(valid ? (log1pf(cf[j].y) - mean_s[col + 1]) * istd_s[col + 1] : 0.f);
+            xs[p * kSStrideIn + col + 2] =
+                (uint16_t)f32_to_bf16_rne(valid ? (log1pf(cf[j].z) - mean_s[col + 2]) * istd_s[col + 2] : 0.f);
+        }
+        __syncthreads();
+
+        // MLP, 4 layers; wave (row block, column block)
+        {
+            floatx16 acc = mfma_block<kMlpIn>(xs, kSStrideIn, rb, wl[0], c0);
+            const float bb = bias_s[0 * 64 + c0 + (lane & 31)];
+#pragma unroll
+            for (int v = 0; v < 16; ++v)
+                ha[(rb + acc_row_s(v, lane)) * kSStrideHid + c0 + (lane & 31)] =
+                    (uint16_t)f32_to_bf16_rne(fmaxf(acc[v] + bb, 0.0f));
+        }
+        __syncthreads();
+        {
+            floatx16 acc = mfma_block<kMlpHid>(ha, kSStrideHid, rb, wl[1], c0);
+            const float bb = bias_s[1 * 64 + c0 + (lane & 31)];
+#pragma unroll
+            for (int v = 0; v < 16; ++v)
+                hb[(rb + acc_row_s(v, lane)) * kSStrideHid + c0 + (lane & 31)] =
+                    (uint16_t)f32_to_bf16_rne(fmaxf(acc[v] + bb, 0.0f));
+        }
+        __syncthreads();
+        {
+            floatx16 acc = mfma_block<kMlpHid>(hb, kSStrideHid, rb, wl[2], c0);
+            const float bb = bias_s[2 * 64 + c0 + (lane & 31)];
+#pragma unroll
+            for (int v = 0; v < 16; ++v)
+                ha[(rb + acc_row_s(v, lane)) * kSStrideHid + c0 + (lane & 31)] =
+                    (uint16_t)f32_to_bf16_rne(fmaxf(acc[v] + bb, 0.0f));
+        }
+        __syncthreads();
+        {
+            floatx16 acc = mfma_block<kMlpHid>(ha, kSStrideHid, rb, wl[3], c0);
+            const int col = c0 + (lane & 31);
+            const float bb = bias_s[3 * 64 + col];
+            if (col <= kMlpErr) {
+#pragma unroll
+                for (int v = 0; v < 16; ++v) zs[rb + acc_row_s(v, lane)][col] = acc[v] + bb;
+            }
+        }
+        __syncthreads();
+
+        // compose the merged level-1 texel of directions 4ko..4ko+3
+        float rr[4], gg[4], bbv[4], al[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int k = 4 * ko + j;
+            if (code[j] >= 0) {
+                rr[j] = a.emissive[code[j]];
+                gg[j] = a.emissive[plane + code[j]];
+                bbv[j] = a.emissive[2 * plane + code[j]];
+                al[j] = 0.0f;
+            } else {
+                rr[j] = fmaxf(fmaf(a.out_scale, zs[p][3 * k + 0], cf[j].x), 0.0f);
+                gg[j] = fmaxf(fmaf(a.out_scale, zs[p][3 * k + 1], cf[j].y), 0.0f);
+                bbv[j] = fmaxf(fmaf(a.out_scale, zs[p][3 * k + 2], cf[j].z), 0.0f);
+                al[j] = 1.0f;
+            }
+        }
+        if (valid) {
+            const float4 o = make_float4(((rr[0] + rr[1]) + (rr[2] + rr[3])) * 0.25f,
+                                         ((gg[0] + gg[1]) + (gg[2] + gg[3])) * 0.25f,
+                                         ((bbv[0] + bbv[1]) + (bbv[2] + bbv[3])) * 0.25f,
+                                         ((al[0] + al[1]) + (al[2] + al[3])) * 0.25f);
+            typename TT::T *out = (typename TT::T *)a.out;
+            out[((size_t)py * a.PX + px) * 4 + ko] = TT::pack(o);
+        }
+
+        // tile error estimate
+        float e = (valid && ko == 0) ? expf(zs[p][kMlpErr]) : 0.0f;
+        float cnt = (valid && ko == 0) ? 1.0f : 0.0f;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            e += __shfl_xor(e, off, 64);
+            cnt += __shfl_xor(cnt, off, 64);
+        }
+        if (lane == 0) {
+            red[wave][0] = e;
+            red[wave][1] = cnt;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const float se = (red[0][0] + red[1][0]) + (red[2][0] + red[3][0]);
+            const float sc = (red[0][1] + red[1][1]) + (red[2][1] + red[3][1]);
+            const float mean = sc > 0.f ? se / sc : 0.f;
+            a.tile_err[tile] = mean;
+            if (mean > a.threshold) {
+                const int slot = atomicAdd(a.flag_count, 1);
+                a.flag_list[slot] = tile;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+hipError_t launch_surr_level1(const SurrLevel1Args &a, const MlpDev &m, Storage st, hipStream_t s)
+{
+    int grid = a.tiles < 512 ? a.tiles : 512;
+    if (grid < 1) return hipSuccess;
+    switch (st) {
+    case Storage::F32: hipLaunchKernelGGL(k_surr_level1<Storage::F32>, dim3(grid), dim3(kBlock), 0, s, a, m); break;
+    case Storage::F16: hipLaunchKernelGGL(k_surr_level1<Storage::F16>, dim3(grid), dim3(kBlock), 0, s, a, m); break;
+    case Storage::BF16: hipLaunchKernelGGL(k_surr_level1<Storage::BF16>, dim3(grid), dim3(kBlock), 0, s, a, m); break;
+    }
+    return hipGetLastError();
+}
+
+// ---- training samples ------------------------------------------------------------
+// One row per level-1 probe: inputs log1p(c) and target residual B - c, where c
+// is the cone radiance and B the plain path's far field (the level-2 merged
+// texels interpolated exactly as the level-1 merge reads them).
+template <Storage S>
+__global__ __launch_bounds__(kBlock) void k_surr_sample(SurrSampleArgs a)
+{
+    using TT = Texel<S>;
+    const int probe = blockIdx.x * kBlock + threadIdx.x;
+    if (probe >= a.PX * a.PY) return;
+    const int px = probe % a.PX, py = probe / a.PX;
+    const int bx = (px - 1) >> 1, by = (py - 1) >> 1;
+    const int qx0 = min(max(bx, 0), a.QX - 1), qx1 = min(max(bx + 1, 0), a.QX - 1);
+    const int qy0 = min(max(by, 0), a.QY - 1), qy1 = min(max(by + 1, 0), a.QY - 1);
+    const typename TT::T *up = (const typename TT::T *)a.upper;
+    for (int k = 0; k < 16; ++k) {
+        const float3 c = bilerp4(a.cone[((size_t)qy0 * a.QX + qx0) * 16 + k], a.cone[((size_t)qy0 * a.QX + qx1) * 16 + k],
+                                 a.cone[((size_t)qy1 * a.QX + qx0) * 16 + k], a.cone[((size_t)qy1 * a.QX + qx1) * 16 + k],
+                                 px, py);
+        const float3 u = bilerp4(TT::unpack(up[((size_t)qy0 * a.QX + qx0) * 16 + k]),
+                                 TT::unpack(up[((size_t)qy0 * a.QX + qx1) * 16 + k]),
+                                 TT::unpack(up[((size_t)qy1 * a.QX + qx0) * 16 + k]),
+                                 TT::unpack(up[((size_t)qy1 * a.QX + qx1) * 16 + k]), px, py);
+        float *f = a.feats + (size_t)probe * kMlpIn + 3 * k;
+        float *t = a.targets + (size_t)probe * kMlpRad + 3 * k;
+        f[0] = log1pf(c.x);
+        f[1] = log1pf(c.y);
+        f[2] = log1pf(c.z);
+        t[0] = u.x - c.x;
+        t[1] = u.y - c.y;
+        t[2] = u.z - c.z;
+    }
+}
+
+hipError_t launch_surr_sample(const SurrSampleArgs &a, Storage st, hipStream_t s)
+{
+    const int n = a.PX * a.PY;
+    const dim3 g((n + kBlock - 1) / kBlock);
+    switch (st) {
+    case Storage::F32: hipLaunchKernelGGL(k_surr_sample<Storage::F32>, g, dim3(kBlock), 0, s, a); break;
+    case Storage::F16: hipLaunchKernelGGL(k_surr_sample<Storage::F16>, g, dim3(kBlock), 0, s, a); break;
+    case Storage::BF16: hipLaunchKernelGGL(k_surr_sample<Storage::BF16>, g, dim3(kBlock), 0, s, a); break;
+    }
+    return hipGetLastError();
+}
+
+// ---- fallback patch: flagged tiles take the plain path's merged level 1 ----------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_tile_patch(const T *__restrict__ plain, T *__restrict__ surr,
+                                                       const float *__restrict__ tile_err, float threshold, int PX,
+                                                       int PY, int tiles_x)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= PX * PY * 4) return;
+    const int probe = i >> 2, px = probe % PX, py = probe / PX;
+    if (tile_err[(py >> 3) * tiles_x + (px >> 3)] > threshold) surr[i] = plain[i];
+}
+
+hipError_t launch_tile_patch(const void *plain, void *surr, const float *tile_err, float threshold, int PX, int PY,
+                             Storage st, hipStream_t s)
+{
+    const int n = PX * PY * 4, tx = (PX + 7) / 8;
+    const dim3 g((n + kBlock - 1) / kBlock);
+    if (st == Storage::F32)
+        hipLaunchKernelGGL(k_tile_patch<float4>, g, dim3(kBlock), 0, s, (const float4 *)plain, (float4 *)surr, tile_err,
+                           threshold, PX, PY, tx);
+    else
+        hipLaunchKernelGGL(k_tile_patch<uint2>, g, dim3(kBlock), 0, s, (const uint2 *)plain, (uint2 *)surr, tile_err,
+                           threshold, PX, PY, tx);
+    return hipGetLastError();
+}
+
+// ---- column statistics (FP64, fixed order) and scaling --------------------------
+// partial[b][0..cols) = sum, partial[b][cols..2 cols) = sum of squares over
+// the rows of block b's contiguous chunk.
+__global__ __launch_bounds__(kBlock) void k_col_stats(const float *__restrict__ x, size_t rows, int cols,
+                                                      double *__restrict__ partial)
+{
+    __shared__ double s1[kBlock], s2[kBlock];
+    const int groups = kBlock / cols;
+    const int col = threadIdx.x % cols, grp = threadIdx.x / cols;
+    const size_t chunk = (rows + gridDim.x - 1) / gridDim.x;
+    const size_t r0 = (size_t)blockIdx.x * chunk, r1 = r0 + chunk < rows ? r0 + chunk : rows;
+    double a = 0.0, b = 0.0;
+    if (grp < groups)
+        for (size_t r = r0 + grp; r < r1; r += groups) {
+            const double v = x[r * cols + col];
+            a += v;
+            b += v * v;
+        }
+    s1[threadIdx.x] = a;
+    s2[threadIdx.x] = b;
+    __syncthreads();
+    if ((int)threadIdx.x < cols) {
+        double ta = 0.0, tb = 0.0;
+        for (int g = 0; g < groups; ++g) {
+            ta += s1[g * cols + threadIdx.x];
+            tb += s2[g * cols + threadIdx.x];
+        }
+        partial[(size_t)blockIdx.x * 2 * cols + threadIdx.x] = ta;
+        partial[(size_t)blockIdx.x * 2 * cols + cols + threadIdx.x] = tb;
+    }
+}
+
+hipError_t launch_col_stats(const float *x, size_t rows, int cols, double *partial, int parts, hipStream_t s)
+{
+    if (cols < 1 || cols > kBlock) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_col_stats, dim3(parts), dim3(kBlock), 0, s, x, rows, cols, partial);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(kBlock) void k_scale(float *__restrict__ x, size_t n, float k)
+{
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) x[i] *= k;
+}
+
+hipError_t launch_scale(float *x, size_t n, float k, hipStream_t s)
+{
+    size_t blocks = (n + kBlock - 1) / kBlock;
+    if (blocks > 65536) blocks = 65536;
+    if (blocks < 1) return hipSuccess;
+    hipLaunchKernelGGL(k_scale, dim3((unsigned)blocks), dim3(kBlock), 0, s, x, n, k);
+    return hipGetLastError();
+}
+
+}  // namespace rc

@@ -15,6 +15,7 @@ in device memory.  Where the reference ``unwrap``s and panics
 from __future__ import annotations
 
 import ctypes
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -63,6 +64,10 @@ class StageStats:
     stage_launches: dict
     level_rays: list
     level_bytes_stored: list
+    surrogate_frames: int = 0
+    fallback_frames: int = 0
+    flagged_tiles: int = 0
+    tiles_per_frame: int = 0
 
 
 class RadianceCascadeStage:
@@ -159,14 +164,17 @@ class RadianceCascadeStage:
         s = _abi.Stats()
         check(self._lib.rc_get_stats(self._ctx, ctypes.byref(s)), self._ctx)
         names = {_abi.STAGE_SEED: "seed", _abi.STAGE_JFA: "jfa", _abi.STAGE_JFA_TAIL: "jfa_tail",
-                 _abi.STAGE_MARCH: "march"}
+                 _abi.STAGE_MARCH: "march", _abi.STAGE_NEAR_SDF: "near_sdf", _abi.STAGE_MIP: "mip",
+                 _abi.STAGE_CONE: "cone", _abi.STAGE_SURROGATE: "surrogate", _abi.STAGE_FALLBACK: "fallback"}
         for i in range(_abi.MAX_LEVELS):
             names[_abi.STAGE_LEVEL0 + i] = f"level{i}"
         ms = {names[i]: s.stage_ms[i] for i in range(_abi.NUM_STAGES) if s.stage_launches[i]}
         la = {names[i]: s.stage_launches[i] for i in range(_abi.NUM_STAGES) if s.stage_launches[i]}
         return StageStats(runs=s.runs, stage_ms=ms, stage_launches=la,
                           level_rays=list(s.level_rays)[: self.levels],
-                          level_bytes_stored=list(s.level_bytes_stored)[: self.levels])
+                          level_bytes_stored=list(s.level_bytes_stored)[: self.levels],
+                          surrogate_frames=s.surrogate_frames, fallback_frames=s.fallback_frames,
+                          flagged_tiles=s.flagged_tiles, tiles_per_frame=s.tiles_per_frame)
 
     def reset_stats(self) -> None:
         check(self._lib.rc_reset_stats(self._ctx), self._ctx)
@@ -208,6 +216,136 @@ class RadianceCascadeStage:
         sd, sr, mx = out.cpu().tolist()
         return {"rel_l2": (sd / sr) ** 0.5 if sr > 0 else sd ** 0.5, "max_abs": mx, "sum_sq_diff": sd, "sum_sq_ref": sr}
 
+    # ---- learned far field ----------------------------------------------------
+    def set_surrogate(self, enabled: bool, threshold: float = float("inf")) -> None:
+        """Replace levels >= 2 by the cone features + MLP (enabled) or run the
+        plain cascades.  Tiles whose mean predicted squared error (normalised
+        units) exceeds `threshold` fall back to the plain cascades."""
+        c = _abi.SurrogateConfig(_abi.ABI_VERSION, int(bool(enabled)), float(threshold), 0)
+        check(self._lib.rc_set_surrogate(self._ctx, ctypes.byref(c)), self._ctx)
+
+    def surrogate(self) -> tuple[bool, float]:
+        c = _abi.SurrogateConfig()
+        check(self._lib.rc_get_surrogate(self._ctx, ctypes.byref(c)), self._ctx)
+        return bool(c.mode), float(c.threshold)
+
+    def mlp_set(self, params: np.ndarray, in_mean: np.ndarray, in_istd: np.ndarray, out_scale: float) -> None:
+        p = np.ascontiguousarray(params, np.float32).reshape(-1)
+        m = np.ascontiguousarray(in_mean, np.float32).reshape(-1)
+        d = np.ascontiguousarray(in_istd, np.float32).reshape(-1)
+        if p.size != _abi.MLP_PARAMS or m.size != _abi.MLP_INPUTS or d.size != _abi.MLP_INPUTS:
+            raise ValueError("parameter arrays have the wrong size")
+        check(self._lib.rc_mlp_set(self._ctx, p.ctypes.data, m.ctypes.data, d.ctypes.data, float(out_scale)), self._ctx)
+
+    def mlp_get(self) -> dict:
+        p = np.empty(_abi.MLP_PARAMS, np.float32)
+        m = np.empty(_abi.MLP_INPUTS, np.float32)
+        d = np.empty(_abi.MLP_INPUTS, np.float32)
+        o = ctypes.c_float()
+        check(self._lib.rc_mlp_get(self._ctx, p.ctypes.data, m.ctypes.data, d.ctypes.data, ctypes.byref(o)), self._ctx)
+        return {"params": p, "in_mean": m, "in_istd": d, "out_scale": float(o.value)}
+
+    def weights_blob(self) -> bytes:
+        n = ctypes.c_uint64()
+        check(self._lib.rc_surrogate_save(self._ctx, None, 0, ctypes.byref(n)), self._ctx)
+        buf = ctypes.create_string_buffer(n.value)
+        check(self._lib.rc_surrogate_save(self._ctx, buf, n.value, ctypes.byref(n)), self._ctx)
+        return buf.raw[: n.value]
+
+    def load_weights_blob(self, blob: bytes) -> None:
+        buf = ctypes.create_string_buffer(blob, len(blob))
+        check(self._lib.rc_surrogate_load(self._ctx, buf, len(blob)), self._ctx)
+
+    def save_weights(self, path: str, meta: dict | None = None) -> str:
+        """Write the weights blob and a `<path>.json` sidecar with its SHA-256."""
+        import hashlib
+        import json
+        blob = self.weights_blob()
+        with open(path, "wb") as f:
+            f.write(blob)
+        side = {"file": os.path.basename(path), "bytes": len(blob), "sha256": hashlib.sha256(blob).hexdigest(),
+                "format": _abi.SURROGATE_BLOB_MAGIC.decode()}
+        if meta:
+            side.update(meta)
+        with open(path + ".json", "w") as f:
+            json.dump(side, f, indent=1)
+        return side["sha256"]
+
+    def load_weights(self, path: str | None = None, verify: bool = True) -> None:
+        """Load a weights blob (default: the packaged network), checking the
+        SHA-256 recorded in its `.json` sidecar."""
+        import hashlib
+        import json
+        path = path or packaged_weights()
+        with open(path, "rb") as f:
+            blob = f.read()
+        if verify:
+            with open(path + ".json") as f:
+                side = json.load(f)
+            if hashlib.sha256(blob).hexdigest() != side["sha256"] or len(blob) != side["bytes"]:
+                raise RcError(_abi.ERR_INVALID, f"{path}: SHA-256 does not match its sidecar")
+        self.load_weights_blob(blob)
+
+    def mlp_forward(self, feats, out, stream=None) -> None:
+        """out [n][64] = raw network outputs for log1p features [n][48] (device fp32)."""
+        n = feats.shape[0]
+        check(self._lib.rc_mlp_forward(self._ctx, _dev_ptr(feats, n * 48 * 4, "feats"), n,
+                                       _dev_ptr(out, n * 64 * 4, "out"), _stream_handle(stream)), self._ctx)
+
+    def mlp_gradient(self, feats, targets, grad, loss, stream=None) -> None:
+        n = feats.shape[0]
+        check(self._lib.rc_mlp_gradient(self._ctx, _dev_ptr(feats, n * 48 * 4, "feats"),
+                                        _dev_ptr(targets, n * 48 * 4, "targets"), n,
+                                        _dev_ptr(grad, _abi.MLP_PARAMS * 4, "grad"), _dev_ptr(loss, 4, "loss"),
+                                        _stream_handle(stream)), self._ctx)
+
+    def mlp_adam(self, grad, lr: float, stream=None) -> None:
+        check(self._lib.rc_mlp_adam(self._ctx, _dev_ptr(grad, _abi.MLP_PARAMS * 4, "grad"), float(lr),
+                                    _stream_handle(stream)), self._ctx)
+
+    def surrogate_samples(self, depth, emissive, feats, targets, stream=None) -> None:
+        """Training rows of one device G-buffer: feats/targets [Px_1*Py_1][48]."""
+        px, py = probe_grid(self.width, self.height, 1)
+        n = px * py
+        g = _abi.GBuffer(_dev_ptr(depth, self.width * self.height * 4, "depth"), None,
+                         _dev_ptr(emissive, 3 * self.width * self.height * 4, "emissive"))
+        check(self._lib.rc_surrogate_samples(self._ctx, ctypes.byref(g), _dev_ptr(feats, n * 48 * 4, "feats"),
+                                             _dev_ptr(targets, n * 48 * 4, "targets"), _stream_handle(stream)),
+              self._ctx)
+
+    def train_surrogate(self, width: int = 512, height: int = 512, levels: int = 6, scene_seed0: int = 10000,
+                        scenes: int = 10000, n_boxes: int = 32, n_emissive: int = 4, epochs: int = 8,
+                        batch: int = 16384, lr: float = 2e-3, lr_decay: float = 0.7, seed: int = 1) -> dict:
+        """Train this context's network on the GPU (rc_surrogate_train)."""
+        cfg = _abi.TrainConfig(_abi.ABI_VERSION, width, height, levels, scene_seed0, scenes, n_boxes, n_emissive,
+                               epochs, batch, lr, lr_decay, 0, seed)
+        rep = _abi.TrainReport()
+        check(self._lib.rc_surrogate_train(self._ctx, ctypes.byref(cfg), ctypes.byref(rep)), self._ctx)
+        return {"epochs": rep.epochs, "samples": rep.samples, "steps": rep.steps,
+                "epoch_loss": list(rep.epoch_loss)[: rep.epochs], "gen_ms": rep.gen_ms, "train_ms": rep.train_ms,
+                "out_scale": rep.out_scale}
+
+    def surrogate_shapes(self) -> dict:
+        px, py = probe_grid(self.width, self.height, 1)
+        qx, qy = probe_grid(self.width, self.height, 2)
+        return {"level1": (py, px), "level2": (qy, qx), "tiles": ((py + 7) // 8, (px + 7) // 8)}
+
+    def debug_surrogate(self, cone=None, level1=None, tile_err=None, stream=None) -> None:
+        sh = self.surrogate_shapes()
+        (qy, qx), (py, px), (ty, tx) = sh["level2"], sh["level1"], sh["tiles"]
+        check(self._lib.rc_debug_surrogate(
+            self._ctx, None if cone is None else _dev_ptr(cone, qx * qy * 64 * 4, "cone"),
+            None if level1 is None else _dev_ptr(level1, px * py * 16 * 4, "level1"),
+            None if tile_err is None else _dev_ptr(tile_err, tx * ty * 4, "tile_err"), _stream_handle(stream)),
+            self._ctx)
+
+    def debug_mip(self, level: int, emc=None, mo=None, stream=None) -> None:
+        w = -(-self.width // (1 << level))
+        h = -(-self.height // (1 << level))
+        check(self._lib.rc_debug_mip(self._ctx, level, None if emc is None else _dev_ptr(emc, w * h * 16, "emc"),
+                                     None if mo is None else _dev_ptr(mo, w * h * 4, "mo"), _stream_handle(stream)),
+              self._ctx)
+
     def close(self) -> None:
         if getattr(self, "_ctx", None) is not None and self._ctx.value:
             self._lib.rc_destroy(self._ctx)
@@ -224,6 +362,11 @@ class RadianceCascadeStage:
             self.close()
         except Exception:
             pass
+
+
+def packaged_weights() -> str:
+    """Path of the network trained on the GPU and shipped with the package."""
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "surrogate.bin")
 
 
 # ---- geometry and scenes ------------------------------------------------------

@@ -1,0 +1,247 @@
+"""GPU parity of the learned far field (HIP kernels through the C ABI) against
+the FP64 oracle (oracle/surrogate.py, oracle/mlp.py).
+
+Bit-exact: the near distance field (integer d^2, the same sqrt rounding) and
+the fallback path with every tile flagged (it is the plain path).  Within
+tolerance: the mip pyramid (fp32 sums of at most 4^m terms), the cone trace
+(fp32 bilinear, exp2/log2), the network (bf16 operands; the GPU accumulates
+in fp32 in another order, which can move a hidden activation across a bf16
+rounding boundary), its gradient and one Adam step, and the whole frame.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+MIP_RTOL = 1e-5
+CONE_REL_L2 = 1e-4
+MLP_REL_L2 = 2e-3          # forward outputs vs the bf16-emulating FP64 network
+GRAD_REL_L2 = 1e-2         # per layer weight gradient
+FRAME_REL_L2 = 2e-3        # merged level 1 and irradiance vs the FP64 frame
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def rel_l2(x, ref):
+    x = np.asarray(x, np.float64)
+    ref = np.asarray(ref, np.float64)
+    den = np.linalg.norm(ref)
+    return np.linalg.norm(x - ref) / den if den > 0 else np.linalg.norm(x - ref)
+
+
+@pytest.fixture(scope="module")
+def rc():
+    import radiancecascade_amd as rc
+    if rc.device_count() < 1:
+        pytest.skip("no HIP device")
+    return rc
+
+
+def _net(seed=3, scale=0.5):
+    from oracle import mlp
+    from oracle import surrogate as so
+    rng = np.random.default_rng(seed)
+    params = so.layers_to_params(mlp.init_params(seed)).astype(np.float32)
+    mean = rng.normal(0.2, 0.1, 48).astype(np.float32)
+    istd = rng.uniform(0.5, 2.0, 48).astype(np.float32)
+    return params, mean, istd, np.float32(scale)
+
+
+def _gbuffer(rc, W, H, seed, nb, ne):
+    dev = torch.device("cuda:0")
+    boxes = rc.scene_generate(seed, nb, ne)
+    depth = torch.empty((H, W), dtype=torch.float32, device=dev)
+    emis = torch.empty((3, H, W), dtype=torch.float32, device=dev)
+    rc.gbuffer_raster(boxes, W, H, depth, None, emis)
+    torch.cuda.synchronize()
+    return depth, emis
+
+
+CASES = [(256, 256, 5, 9, 8, 2), (200, 136, 4, 9, 8, 2)]
+
+
+@pytest.mark.parametrize("W,H,L,seed,nb,ne", CASES)
+def test_surrogate_frame_parity(rc, W, H, L, seed, nb, ne):
+    from oracle import surrogate as so
+    _say(f"[surrogate frame] {W}x{H} L={L} seed={seed}")
+    depth, emis = _gbuffer(rc, W, H, seed, nb, ne)
+    params, mean, istd, scale = _net()
+    irr = torch.empty((3, H, W), dtype=torch.float32, device="cuda:0")
+    with rc.RadianceCascadeStage(W, H, levels=L, debug=True) as st:
+        st.mlp_set(params, mean, istd, scale)
+        st.set_surrogate(True)
+        st.run_device(depth, emis, irr)
+        sh = st.surrogate_shapes()
+        (qy, qx), (py, px), (ty, tx) = sh["level2"], sh["level1"], sh["tiles"]
+        cone = torch.empty((qy, qx, 16, 4), dtype=torch.float32, device="cuda:0")
+        a1 = torch.empty((py, px, 4, 4), dtype=torch.float32, device="cuda:0")
+        terr = torch.empty((ty, tx), dtype=torch.float32, device="cuda:0")
+        sdf = torch.empty((H, W), dtype=torch.float32, device="cuda:0")
+        st.debug_surrogate(cone, a1, terr)
+        st.debug_sdf(sdf)
+        mips = []
+        top = so.top_level(W, H)
+        for m in range(1, top + 1):
+            w, h = -(-W // (1 << m)), -(-H // (1 << m))
+            e = torch.empty((h, w, 4), dtype=torch.float32, device="cuda:0")
+            o = torch.empty((h, w), dtype=torch.float32, device="cuda:0")
+            st.debug_mip(m, e, o)
+            mips.append((e, o))
+        torch.cuda.synchronize()
+        s = st.stats()
+    d = depth.cpu().numpy()
+    e = emis.cpu().numpy()
+    ref_irr, ref = so.run(d, e, L, params, mean, istd, float(scale), want=True)
+    # near distance field: bit-exact
+    assert np.array_equal(sdf.cpu().numpy(), ref["sdf"].astype(np.float32))
+    for m, (eg, og) in enumerate(mips, start=1):
+        re, ro = ref["levels"][m]
+        assert np.allclose(eg.cpu().numpy(), re, rtol=MIP_RTOL, atol=1e-6), m
+        assert np.allclose(og.cpu().numpy(), ro, rtol=MIP_RTOL, atol=1e-6), m
+    cg = cone.cpu().numpy()
+    r_cone = rel_l2(cg[..., :3], ref["cone"][..., :3])
+    t_cone = np.abs(cg[..., 3] - ref["cone"][..., 3]).max()
+    r_a1 = rel_l2(a1.cpu().numpy(), ref["A1"])
+    r_irr = rel_l2(irr.cpu().numpy(), ref_irr)
+    _say(f"  cone rel-L2 {r_cone:.2e}  T max {t_cone:.2e}  level1 rel-L2 {r_a1:.2e}  irradiance rel-L2 {r_irr:.2e}")
+    assert r_cone <= CONE_REL_L2 and t_cone <= 1e-4
+    assert r_a1 <= FRAME_REL_L2 and r_irr <= FRAME_REL_L2
+    assert np.isfinite(terr.cpu().numpy()).all()
+    assert s.surrogate_frames == 1 and s.fallback_frames == 0 and s.tiles_per_frame == ty * tx
+
+
+def _rand_io(n, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.gamma(1.0, 0.5, (n, 48)).astype(np.float32)      # log1p features are >= 0
+    t = rng.normal(0.0, 1.0, (n, 48)).astype(np.float32)
+    return x, t
+
+
+def test_mlp_forward_parity(rc):
+    from oracle import mlp
+    from oracle import surrogate as so
+    params, mean, istd, scale = _net(5)
+    x, _ = _rand_io(1000, 1)
+    with rc.RadianceCascadeStage(64, 64, levels=3) as st:
+        st.mlp_set(params, mean, istd, scale)
+        xd = torch.from_numpy(x).cuda()
+        out = torch.empty((1000, 64), dtype=torch.float32, device="cuda:0")
+        st.mlp_forward(xd, out)
+        torch.cuda.synchronize()
+    z, _ = mlp.forward(so.params_to_layers(params), (x.astype(np.float64) - mean) * istd, emulate_bf16=True)
+    r = rel_l2(out.cpu().numpy()[:, :49], z[:, :49])
+    _say(f"  forward rel-L2 {r:.2e}")
+    assert r <= MLP_REL_L2
+    assert np.all(out.cpu().numpy()[:, 49:] == 0)
+
+
+def test_mlp_gradient_and_adam_parity(rc):
+    from oracle import mlp
+    from oracle import surrogate as so
+    params, mean, istd, scale = _net(7)
+    n = 777                                                  # ragged: 24 full blocks + 9 rows
+    x, t = _rand_io(n, 2)
+    layers = so.params_to_layers(params)
+    xn = (x.astype(np.float64) - mean) * istd
+    z, cache = mlp.forward(layers, xn, emulate_bf16=True)
+    loss, g4 = mlp.loss_and_grad_out(z, t.astype(np.float64))
+    grads = mlp.backward(layers, cache, g4, emulate_bf16=True)
+    with rc.RadianceCascadeStage(64, 64, levels=3) as st:
+        st.mlp_set(params, mean, istd, scale)
+        xd, td = torch.from_numpy(x).cuda(), torch.from_numpy(t).cuda()
+        grad = torch.empty(15616, dtype=torch.float32, device="cuda:0")
+        lo = torch.empty(1, dtype=torch.float32, device="cuda:0")
+        st.mlp_gradient(xd, td, grad, lo)
+        g = grad.cpu().numpy()
+        st.mlp_adam(grad, 1e-3)
+        after = st.mlp_get()["params"]
+    gl = so.params_to_layers(g)
+    r_loss = abs(float(lo.item()) - loss) / abs(loss)
+    _say(f"  loss {lo.item():.6f} vs {loss:.6f} (rel {r_loss:.2e})")
+    assert r_loss <= 1e-4
+    for l in range(4):
+        rw = rel_l2(gl[l][0], grads[l][0])
+        rb = rel_l2(gl[l][1], grads[l][1])
+        _say(f"  layer {l}: dW rel-L2 {rw:.2e}  db rel-L2 {rb:.2e}")
+        assert rw <= GRAD_REL_L2 and rb <= GRAD_REL_L2
+    # one Adam step from the GPU's own gradient (first step: p - lr * g / (|g| + eps))
+    opt = mlp.Adam(so.params_to_layers(params), lr=1e-3)
+    ref = so.layers_to_params(opt.step(so.params_to_layers(params), gl))
+    assert np.allclose(after, ref, rtol=1e-6, atol=1e-7)
+
+
+def test_training_rows_parity(rc):
+    from oracle import rco
+    from oracle import surrogate as so
+    W, H, L = 160, 128, 5
+    depth, emis = _gbuffer(rc, W, H, 9, 8, 2)
+    px, py = rc.probe_grid(W, H, 1)
+    feats = torch.empty((px * py, 48), dtype=torch.float32, device="cuda:0")
+    targ = torch.empty((px * py, 48), dtype=torch.float32, device="cuda:0")
+    with rc.RadianceCascadeStage(W, H, levels=L) as st:
+        st.surrogate_samples(depth, emis, feats, targ)
+        torch.cuda.synchronize()
+    d, e = depth.cpu().numpy(), emis.cpu().numpy()
+    _, lv = rco.run(d, e, L, want_levels=True)
+    s0 = rco.seed_init(d, e)
+    cone = so.cone_trace(so.mip(s0, e), W, H)
+    x, t = so.samples(cone, lv[2], W, H)
+    rx, rt = rel_l2(feats.cpu().numpy(), x), rel_l2(targ.cpu().numpy(), t)
+    _say(f"  rows: features rel-L2 {rx:.2e}  targets rel-L2 {rt:.2e}")
+    assert rx <= 1e-4 and rt <= 5e-3
+
+
+def test_fallback_everywhere_is_the_plain_path(rc):
+    W, H, L = 256, 192, 5
+    depth, emis = _gbuffer(rc, W, H, 9, 8, 2)
+    params, mean, istd, scale = _net()
+    plain = torch.empty((3, H, W), dtype=torch.float32, device="cuda:0")
+    surr = torch.empty((3, H, W), dtype=torch.float32, device="cuda:0")
+    with rc.RadianceCascadeStage(W, H, levels=L) as st:
+        st.run_device(depth, emis, plain)
+        st.mlp_set(params, mean, istd, scale)
+        st.set_surrogate(True, threshold=-1.0)        # every tile is above the threshold
+        st.run_device(depth, emis, surr)
+        torch.cuda.synchronize()
+        s = st.stats()
+    assert s.fallback_frames == 1 and s.flagged_tiles == s.tiles_per_frame > 0
+    assert torch.equal(plain, surr)
+
+
+def test_surrogate_graph_replay_matches_direct_launches(rc):
+    W, H, L = 240, 160, 5
+    depth, emis = _gbuffer(rc, W, H, 9, 8, 2)
+    params, mean, istd, scale = _net()
+    outs = []
+    for graph in (True, False):
+        with rc.RadianceCascadeStage(W, H, levels=L, graph=graph) as st:
+            st.mlp_set(params, mean, istd, scale)
+            st.set_surrogate(True)
+            for _ in range(3):
+                irr = torch.empty((3, H, W), dtype=torch.float32, device="cuda:0")
+                st.run_device(depth, emis, irr)
+                torch.cuda.synchronize()
+                outs.append(irr.clone())
+    assert all(torch.equal(outs[0], o) for o in outs[1:])
+
+
+def test_training_reduces_loss(rc):
+    with rc.RadianceCascadeStage(64, 64, levels=3) as st:
+        rep = st.train_surrogate(width=128, height=128, levels=5, scene_seed0=10000, scenes=6, epochs=4,
+                                 batch=512, lr=2e-3, lr_decay=0.8, seed=1)
+        _say(f"  losses {rep['epoch_loss']}  rows {rep['samples']}  scale {rep['out_scale']:.4f}")
+        assert rep["samples"] == 6 * 32 * 32 and rep["steps"] == 4 * (6 * 1024 // 512)
+        assert all(math.isfinite(v) for v in rep["epoch_loss"])
+        assert rep["epoch_loss"][-1] < rep["epoch_loss"][0]
+        blob = st.weights_blob()
+    with rc.RadianceCascadeStage(64, 64, levels=3) as st2:
+        st2.load_weights_blob(blob)
+        assert st2.weights_blob() == blob

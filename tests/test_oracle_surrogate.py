@@ -1,0 +1,145 @@
+"""Pins for the learned-far-field oracle (oracle/surrogate.py): a library
+EDT, block means taken directly from pixels, closed forms and symmetries of
+the cone trace, and the reduction of the level-1 composition to the plain
+merge of oracle/rco.c when the far field is the exact one."""
+import numpy as np
+import pytest
+
+from oracle import rco
+from oracle import surrogate as so
+
+
+def _occ_field(H, W, p, seed):
+    rng = np.random.default_rng(seed)
+    occ = rng.random((H, W)) < p
+    yy, xx = np.mgrid[0:H, 0:W]
+    return np.where(occ, (yy.astype(np.uint32) << 16) | xx.astype(np.uint32), np.uint32(rco.SEED_NONE)).astype(np.uint32)
+
+
+@pytest.mark.parametrize("H,W,p,seed", [(48, 64, 0.01, 1), (37, 53, 0.002, 2), (64, 64, 0.0005, 3)])
+def test_near_sdf_matches_library_edt(H, W, p, seed):
+    ndimage = pytest.importorskip("scipy.ndimage")
+    field = _occ_field(H, W, p, seed)
+    occ = field != rco.SEED_NONE
+    if not occ.any():
+        occ[H // 2, W // 3] = True
+        field[H // 2, W // 3] = ((H // 2) << 16) | (W // 3)
+    edt = ndimage.distance_transform_edt(~occ)
+    got = so.near_sdf(field)
+    near = edt <= so.NEAR_R
+    assert np.array_equal(got[near], edt[near].astype(np.float32))
+    assert np.isinf(got[~near]).all()
+
+
+def test_near_sdf_empty_and_jfa_agreement():
+    assert np.isinf(so.near_sdf(np.full((20, 30), rco.SEED_NONE, np.uint32))).all()
+    d, _, e = rco.gbuffer(9, 128, 96, 8, 2)
+    s0 = rco.seed_init(d, e)
+    jfa = rco.sdf_f32(rco.jfa(s0))
+    got = so.near_sdf(s0)
+    m = jfa <= so.NEAR_R
+    # the jump flood is exact almost everywhere; the near field is exact
+    assert (got[m] == jfa[m]).mean() > 0.999
+    assert np.all(got[m] <= jfa[m])
+
+
+def test_mip_equals_direct_block_means():
+    d, _, e = rco.gbuffer(9, 200, 136, 8, 2)
+    s0 = rco.seed_init(d, e)
+    EM, ME, MO = so.pixel_fields(s0, e)
+    levels = so.mip(s0, e)
+    H, W = s0.shape
+    assert len(levels) - 1 == so.top_level(W, H) == 8
+    for m in range(1, len(levels)):
+        b = 1 << m
+        h, w = -(-H // b), -(-W // b)
+        def block(a):
+            pad = np.zeros(a.shape[:-2] + (h * b, w * b))
+            pad[..., :H, :W] = a
+            return pad.reshape(a.shape[:-2] + (h, b, w, b)).sum(axis=(-3, -1)) / (b * b)
+        emc, mo = levels[m]
+        assert emc.shape == (h, w, 4) and mo.shape == (h, w)
+        assert np.allclose(emc[..., :3], np.moveaxis(block(EM), 0, -1), rtol=1e-12, atol=1e-12)
+        assert np.allclose(emc[..., 3], block(ME), rtol=1e-12, atol=1e-12)
+        assert np.allclose(mo, block(MO), rtol=1e-12, atol=1e-12)
+    # mass: sum over the top cell times its area is the image total
+    top_emc, _ = levels[-1]
+    assert np.isclose(top_emc[..., 0].sum() * 4 ** (len(levels) - 1), EM[0].sum())
+
+
+def test_cone_schedule_shape():
+    s = so.cone_schedule(1920, 1080)
+    ts = [t for t, _, _ in s]
+    assert ts[0] == 6.0 and all(b > a for a, b in zip(ts, ts[1:]))
+    assert ts[-1] < np.hypot(1920, 1080) + 2 <= ts[-1] + 0.75 * max(2 * ts[-1] * np.tan(np.pi / 16), 1)
+    assert all(1 <= m <= so.top_level(1920, 1080) for _, m, _ in s)
+    assert all(ln <= 2 ** m for _, m, ln in s)
+    assert len(s) < 64
+
+
+def test_cone_empty_scene():
+    H = W = 64
+    field = np.full((H, W), rco.SEED_NONE, np.uint32)
+    cone = so.cone_trace(so.mip(field, np.zeros((3, H, W), np.float32)), W, H)
+    assert np.all(cone[..., :3] == 0) and np.all(cone[..., 3] == 1)
+
+
+def test_cone_fully_emissive_scene():
+    H = W = 128
+    e = np.full((3, H, W), 2.5, np.float32)
+    e[1] = 4.0
+    field = rco.seed_init(np.ones((H, W), np.float32), e)
+    cone = so.cone_trace(so.mip(field, e), W, H)
+    # interior probes: the first sample lies in fully emissive cells -> L = E, T = 0
+    inner = cone[3:-3, 3:-3]
+    assert np.allclose(inner[..., 0], 2.5) and np.allclose(inner[..., 1], 4.0)
+    assert np.all(inner[..., 3] == 0)
+
+
+def test_cone_linear_in_emission_and_mirror_symmetric():
+    W = H = 256
+    d, _, e = rco.gbuffer(9, W, H, 8, 2)
+    e = e.copy()
+    d = d.copy()
+    # mirror the left half onto the right: a scene symmetric under x -> W - x
+    e[:, :, W // 2:] = e[:, :, :W // 2][:, :, ::-1]
+    d[:, W // 2:] = d[:, :W // 2][:, ::-1]
+    s0 = rco.seed_init(d, e)
+    c1 = so.cone_trace(so.mip(s0, e), W, H)
+    c2 = so.cone_trace(so.mip(s0, 2 * e), W, H)
+    assert np.allclose(c2[..., :3], 2 * c1[..., :3], rtol=1e-12, atol=1e-12)
+    assert np.array_equal(c2[..., 3], c1[..., 3])
+    occ = s0 != rco.SEED_NONE
+    assert np.array_equal(occ, occ[:, ::-1])
+    mk = [(7 - k) % 16 for k in range(16)]
+    mir = c1[:, ::-1][:, :, mk]
+    assert np.allclose(mir, c1, rtol=1e-9, atol=1e-9)
+    assert c1[..., :3].sum() > 0
+
+
+def test_compose_with_exact_far_field_is_the_plain_merge():
+    W, H, L = 96, 80, 5
+    d, _, e = rco.gbuffer(9, W, H, 8, 2)
+    irr, lv = rco.run(d, e, L, want_levels=True)
+    sdf = rco.sdf_f64(rco.jfa(rco.seed_init(d, e)))
+    m1, _ = rco.march_level(sdf, e, 1)
+    PX, PY = rco.probe_grid(W, H, 1)
+    B = so.upper_bilinear(lv[2], PX, PY).reshape(-1, 48)
+    A1 = so.compose_level1(m1, B, np.zeros((B.shape[0], 64)), 1.0)
+    assert np.allclose(A1, rco.merge_level(W, H, 1, m1, lv[2]), rtol=1e-12, atol=1e-12)
+    assert np.allclose(A1, lv[1], rtol=1e-12, atol=1e-12)
+
+
+def test_samples_and_params_roundtrip():
+    rng = np.random.default_rng(0)
+    cone = rng.random((5, 7, 16, 4))
+    A2 = rng.random((5, 7, 16, 4))
+    x, t = so.samples(cone, A2, 7 * 8 - 3, 5 * 8 - 5)
+    c, _ = so.features(cone, 7 * 8 - 3, 5 * 8 - 5)
+    assert np.allclose(np.expm1(x) + t, so.upper_bilinear(A2, *rco.probe_grid(53, 35, 1)).reshape(-1, 48))
+    from oracle import mlp
+    layers = mlp.init_params(3)
+    p = so.layers_to_params(layers)
+    assert p.size == 15616
+    back = so.params_to_layers(p)
+    assert all(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) for a, b in zip(layers, back))
