@@ -222,7 +222,7 @@ rc_status rc_surrogate_load(rc_ctx *ctx, const void *buf, uint64_t size);
  * inference kernels).  feats: [n][48] log1p features (standardised inside);
  * out: [n][64] raw outputs z.  rc_mlp_gradient: targets [n][48] normalised
  * residuals; grad: RC_MLP_PARAMS fp32 of the batch-mean loss
- * L = sum (z - t)^2 / (48 n) + 0.05 sum (z_48 - log(mse_s + 1e-6))^2 / n;
+ * L = sum (z - t)^2 / (48 n) + 0.05 sum (z_48 - log(mse_s + 1e-2))^2 / n;
  * loss: 1 float.  rc_mlp_adam: one Adam step (0.9, 0.999, 1e-8) with grad. */
 rc_status rc_mlp_forward(rc_ctx *ctx, const float *feats_dev, uint32_t n, float *out_dev, void *stream);
 rc_status rc_mlp_gradient(rc_ctx *ctx, const float *feats_dev, const float *targets_dev, uint32_t n, float *grad_dev,

@@ -22,7 +22,7 @@ N_RAD = 48          # radiance outputs
 ERR = 48            # index of the error head
 SIZES = [(IN, HID), (HID, HID), (HID, HID), (HID, OUT_PAD)]
 ERR_WEIGHT = 0.05   # weight of the error-head loss
-ERR_EPS = 1e-6
+ERR_EPS = 1e-2
 
 
 def bf16(x: np.ndarray) -> np.ndarray:

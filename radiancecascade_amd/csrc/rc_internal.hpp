@@ -78,7 +78,7 @@ hipError_t launch_error_norms(const float *x, const void *ref, bool ref_f64, siz
 constexpr int kMlpIn = 48, kMlpHid = 64, kMlpOut = 64, kMlpRad = 48, kMlpErr = 48;
 constexpr int kMlpParams = kMlpIn * 64 + 64 + 3 * (64 * 64 + 64);  // 15616 fp32 master parameters
 constexpr int kMlpWt = 64 * kMlpIn + 3 * 64 * kMlpHid;            // bf16 weights per copy
-constexpr float kMlpErrEps = 1e-6f, kMlpErrWeight = 0.05f;
+constexpr float kMlpErrEps = 1e-2f, kMlpErrWeight = 0.05f;
 
 struct MlpDev {
     float *master;          // [kMlpParams]: for l = 0..3, W_l [K_l][64] (input-major) then b_l [64]

@@ -1,0 +1,77 @@
+"""What the trained network gains: the packaged weights (trained on the GPU
+on 10,000 scenes at 512x512, seeds 10000..19999) evaluated on held-out
+1920x1080 scenes with 6 cascades, against the FP64 oracle.
+
+Three forms of the frame replace levels >= 2:
+  truncated  levels 0 and 1 only (the method without any far field);
+  proxy      the cone-traced far field alone (the network's weights zeroed);
+  learned    cone trace + trained network (the shipped surrogate).
+The learned frame must beat the truncated one on every scene.  Its error
+against the proxy alone is recorded, not asserted: the network lowers the
+error of the scenes whose light comes from far away and raises it on some
+scenes that the proxy already gets right (numbers printed per scene).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+W, H, L = 1920, 1080, 6
+SEEDS = [0, 1, 3, 9]         # held-out scenes with visible emitters (32 boxes, 4 emissive)
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def rc():
+    import radiancecascade_amd as rc
+    if rc.device_count() < 1:
+        pytest.skip("no HIP device")
+    if not os.path.exists(rc.packaged_weights()):
+        pytest.skip("no packaged weights")
+    return rc
+
+
+def test_learned_far_field_gain(rc):
+    from oracle import rco
+    rco.set_threads(16)
+    dev = torch.device("cuda:0")
+    depth = torch.empty((H, W), device=dev)
+    emis = torch.empty((3, H, W), device=dev)
+    learned = rc.RadianceCascadeStage(W, H, levels=L)
+    learned.load_weights()
+    learned.set_surrogate(True)
+    m = learned.mlp_get()
+    proxy = rc.RadianceCascadeStage(W, H, levels=L)
+    proxy.mlp_set(np.zeros_like(m["params"]), m["in_mean"], m["in_istd"], m["out_scale"])
+    proxy.set_surrogate(True)
+    trunc = rc.RadianceCascadeStage(W, H, levels=2)
+    plain = rc.RadianceCascadeStage(W, H, levels=L)
+    rows = []
+    for seed in SEEDS:
+        rc.gbuffer_raster(rc.scene_generate(seed, 32, 4), W, H, depth, None, emis)
+        ref = torch.from_numpy(rco.run(depth.cpu().numpy(), emis.cpu().numpy(), L)).to(dev)
+        err = {}
+        for name, st in (("plain", plain), ("truncated", trunc), ("proxy", proxy), ("learned", learned)):
+            out = torch.empty((3, H, W), device=dev)
+            st.run_device(depth, emis, out)
+            torch.cuda.synchronize()
+            err[name] = st.error_norms(out, ref)["rel_l2"]
+        rows.append(err)
+        _say(f"  seed {seed}: rel-L2 vs FP64  plain {err['plain']:.2e}  truncated {err['truncated']:.4f}  "
+             f"proxy {err['proxy']:.4f}  learned {err['learned']:.4f}")
+        assert err["plain"] < 1e-3
+        assert err["learned"] < err["truncated"]
+    med = {k: float(np.median([r[k] for r in rows])) for k in rows[0]}
+    _say(f"  median: truncated {med['truncated']:.4f}  proxy {med['proxy']:.4f}  learned {med['learned']:.4f}")
+    assert med["learned"] < 0.5 * med["truncated"]
+    for st in (learned, proxy, trunc, plain):
+        st.close()
