@@ -49,7 +49,7 @@ def near_sdf(seed_init_field: np.ndarray) -> np.ndarray:
     return out
 
 
-# ---- mip pyramid (k_mip, k_mip_top) -------------------------------------------
+# ---- mip pyramid (k_seed_surr, k_mip_mid, k_mip_top) -------------------------------------------
 def top_level(W: int, H: int) -> int:
     top = 1
     while (1 << top) < W or (1 << top) < H:
@@ -66,22 +66,35 @@ def pixel_fields(seed_init_field: np.ndarray, emissive: np.ndarray):
 
 
 def _pool(a: np.ndarray) -> np.ndarray:
+    """2x2 means in the array's own precision, zero outside."""
     h, w = a.shape[-2:]
     h2, w2 = (h + 1) // 2, (w + 1) // 2
     pad = [(0, 0)] * (a.ndim - 2) + [(0, 2 * h2 - h), (0, 2 * w2 - w)]
     a = np.pad(a, pad)
-    return 0.25 * ((a[..., 0::2, 0::2] + a[..., 0::2, 1::2]) + (a[..., 1::2, 0::2] + a[..., 1::2, 1::2]))
+    q = a.dtype.type(0.25)
+    return q * ((a[..., 0::2, 0::2] + a[..., 0::2, 1::2]) + (a[..., 1::2, 0::2] + a[..., 1::2, 1::2]))
+
+
+def _f16(a: np.ndarray) -> np.ndarray:
+    return np.asarray(a, np.float32).astype(np.float16).astype(np.float32)
 
 
 def mip(seed_init_field: np.ndarray, emissive: np.ndarray):
-    """levels[m] for m = 1..top: (emc [h][w][4] = mean E*[em] rgb, mean [em]; mo [h][w])."""
+    """levels[m] for m = 1..top: (emc [h][w][4] = mean E*[em] rgb, [em] coverage;
+    mo [h][w] = [occluder] coverage).  Radiance means are exact (FP64);
+    coverages are stored as fp16: each level's is the fp32 mean of its 4
+    stored children, ((x0y0 + x1y0) + (x0y1 + x1y1)) / 4, rounded to fp16
+    (exact up to level 5, where the means are multiples of 4^-5)."""
     H, W = seed_init_field.shape
     EM, ME, MO = pixel_fields(seed_init_field, emissive)
-    cur = np.concatenate([EM, ME[None]], 0), MO
+    rgb = EM
+    cov = np.stack([ME, MO]).astype(np.float32)
     levels = [None]
     for _ in range(1, top_level(W, H) + 1):
-        cur = (_pool(cur[0]), _pool(cur[1]))
-        levels.append((np.moveaxis(cur[0], 0, -1), cur[1]))
+        rgb = _pool(rgb)
+        cov = _f16(_pool(cov))
+        levels.append((np.concatenate([np.moveaxis(rgb, 0, -1), cov[0][..., None].astype(np.float64)], -1),
+                       cov[1].astype(np.float64)))
     return levels
 
 
@@ -102,30 +115,38 @@ def cone_schedule(W: int, H: int):
 
 
 def _sample(level, x, y, m):
+    """Bilinear sample of level m at pixel positions (x, y); cells outside the
+    level are zero (the stored one-cell border).  Returns (emc [..,4], mo,
+    inside) where `inside` is false once the footprint has left the image
+    (lower cell index outside [-1, w-1] x [-1, h-1]; the cone then stops)."""
     emc, mo = level
     h, w = mo.shape
+    pe = np.zeros((h + 2, w + 2, 4))
+    pe[1:-1, 1:-1] = emc
+    po = np.zeros((h + 2, w + 2))
+    po[1:-1, 1:-1] = mo
     s = float(1 << m)
     fx = x / s - 0.5
     fy = y / s - 0.5
     x0 = np.floor(fx).astype(np.int64)
     y0 = np.floor(fy).astype(np.int64)
     ax, ay = fx - x0, fy - y0
+    inside = (x0 >= -1) & (x0 < w) & (y0 >= -1) & (y0 < h)
     e = np.zeros(x.shape + (4,))
     o = np.zeros(x.shape)
     for dy, wy in ((0, 1 - ay), (1, ay)):
         for dx, wx in ((0, 1 - ax), (1, ax)):
-            xi, yi = x0 + dx, y0 + dy
-            ok = (xi >= 0) & (xi < w) & (yi >= 0) & (yi < h)
-            xc, yc = np.clip(xi, 0, w - 1), np.clip(yi, 0, h - 1)
-            wt = wx * wy * ok
-            e += emc[yc, xc] * wt[..., None]
-            o += mo[yc, xc] * wt
-    return e, o
+            xc, yc = np.clip(x0 + 1 + dx, 0, w + 1), np.clip(y0 + 1 + dy, 0, h + 1)
+            wt = wx * wy
+            e += pe[yc, xc] * wt[..., None]
+            o += po[yc, xc] * wt
+    return e, o, inside
 
 
 def cone_trace(levels, W: int, H: int, schedule=None) -> np.ndarray:
     """[QY][QX][16][4]: far-field radiance rgb and transmittance per level-2
-    probe and level-1 direction (the fp32 direction table of rco)."""
+    probe and level-1 direction (the fp32 direction table of rco).  A cone
+    stops once T < 1e-4 or once its footprint has left the image."""
     schedule = schedule if schedule is not None else cone_schedule(W, H)
     QX, QY = rco.probe_grid(W, H, 2)
     dirs = rco.dir_table_f32(1).astype(np.float64)
@@ -139,7 +160,8 @@ def cone_trace(levels, W: int, H: int, schedule=None) -> np.ndarray:
         L = np.zeros(cx.shape + (3,))
         live = np.ones(cx.shape, bool)
         for t, m, ln in schedule:
-            e, o = _sample(levels[m], t * dx + cx, t * dy + cy, m)
+            e, o, inside = _sample(levels[m], t * dx + cx, t * dy + cy, m)
+            live = live & inside
             ao = 1.0 - (1.0 - np.minimum(o, 0.999)) ** ln
             a = np.minimum(e[..., 3] + ao, 1.0)
             L = np.where(live[..., None], L + T[..., None] * e[..., :3], L)

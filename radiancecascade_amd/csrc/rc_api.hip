@@ -65,6 +65,7 @@ struct rc_ctx {
     MipArgs mip{};
     ConeSchedule sch{};
     float4 *cone = nullptr;
+    uint64_t *occ_bits = nullptr;  // occupancy, one bit per pixel, rows of ceil(W/64) words
     void *surr_a1 = nullptr;
     float *tile_err = nullptr;
     int *flag_count = nullptr, *flag_list = nullptr;
@@ -72,6 +73,9 @@ struct rc_ctx {
     int surr_tiles = 0, surr_tiles_x = 0;
     bool surr_ran = false;
     float *scratch_irr = nullptr;  // irradiance of frames run only for their levels
+    // the near distance field runs on a side stream beside the mip / cone chain
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // profiling
     hipEvent_t ev[2 * RC_NUM_STAGES] = {};
     bool ev_used[RC_NUM_STAGES] = {};
@@ -142,10 +146,10 @@ static void release(rc_ctx *c)
     c->codes = nullptr;
     if (c->norm_ws) hipFree(c->norm_ws);
     c->norm_ws = nullptr;
-    if (c->mip.emc) hipFree(c->mip.emc);
-    if (c->mip.mo) hipFree(c->mip.mo);
-    c->mip.emc = nullptr;
-    c->mip.mo = nullptr;
+    if (c->mip.tex) hipFree(c->mip.tex);
+    c->mip.tex = nullptr;
+    if (c->occ_bits) hipFree(c->occ_bits);
+    c->occ_bits = nullptr;
     if (c->cone) hipFree(c->cone);
     if (c->surr_a1) hipFree(c->surr_a1);
     if (c->tile_err) hipFree(c->tile_err);
@@ -396,10 +400,12 @@ static rc_status ensure_surrogate(rc_ctx *c)
     if (c->L < 3) return fail(c, RC_ERR_INVALID, "learned far field needs levels >= 3");
     const int top = cone_top_level(c->W, c->H);
     mip_layout(c->W, c->H, top, c->mip);
-    cone_schedule(c->W, c->H, top, c->sch);
+    cone_schedule(c->W, c->H, c->mip, c->sch);
     const size_t cells = mip_cells(c->mip);
-    HIP_TRY(c, hipMalloc(&c->mip.emc, cells * sizeof(float4)));
-    HIP_TRY(c, hipMalloc(&c->mip.mo, cells * sizeof(float)));
+    if (cells * sizeof(float4) > 0x7FFFFFFFull) return fail(c, RC_ERR_INVALID, "learned far field: image too large");
+    HIP_TRY(c, hipMalloc(&c->mip.tex, cells * sizeof(float4)));
+    HIP_TRY(c, hipMemset(c->mip.tex, 0, cells * sizeof(float4)));  // the zero borders
+    HIP_TRY(c, hipMalloc(&c->occ_bits, (size_t)c->H * ((c->W + 63) / 64) * sizeof(uint64_t)));
     const int QX = grid_dim(c->W, 2), QY = grid_dim(c->H, 2);
     HIP_TRY(c, hipMalloc(&c->cone, (size_t)QX * QY * 16 * sizeof(float4)));
     const int PX = grid_dim(c->W, 1), PY = grid_dim(c->H, 1);
@@ -410,6 +416,9 @@ static rc_status ensure_surrogate(rc_ctx *c)
     HIP_TRY(c, hipMalloc(&c->flag_list, (size_t)c->surr_tiles * sizeof(int)));
     HIP_TRY(c, hipMalloc(&c->flag_count, sizeof(int)));
     HIP_TRY(c, hipMemset(c->flag_count, 0, sizeof(int)));
+    if (!c->side) HIP_TRY(c, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    if (!c->ev_fork) HIP_TRY(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    if (!c->ev_join) HIP_TRY(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     c->surr_alloc = true;
     return RC_OK;
 }
@@ -418,6 +427,7 @@ static ConeArgs cone_args(const rc_ctx *c)
 {
     ConeArgs a{};
     a.mip = c->mip;
+    a.mip_bytes = (uint32_t)(mip_cells(c->mip) * sizeof(float4));
     a.sch = c->sch;
     a.dirs = c->dirs + c->dir_off[1];
     a.cone = c->cone;
@@ -427,8 +437,8 @@ static ConeArgs cone_args(const rc_ctx *c)
     return a;
 }
 
-// seed init -> near distance field -> mip pyramid -> cones -> level 1 (march +
-// MLP + merge) -> level 0 + gather from the learned level 1.
+// occupancy bits + mip 1..2 -> near distance field -> mip 3..top -> cones ->
+// level 1 (march + MLP + merge) -> level 0 + gather from the learned level 1.
 static rc_status run_frame_surrogate(rc_ctx *c, const float *depth, const float *emissive, float *irr, hipStream_t s,
                                      bool quiet)
 {
@@ -437,17 +447,29 @@ static rc_status run_frame_surrogate(rc_ctx *c, const float *depth, const float 
     StageTimer tm{c, s, prof};
     for (auto &u : c->ev_used) u = false;
     tm.begin(RC_STAGE_SEED);
-    HIP_TRY(c, launch_seed_init(depth, emissive, c->seed[0], c->W, c->H, s));
+    HIP_TRY(c, launch_seed_surr(depth, emissive, c->occ_bits, c->mip, c->W, c->H, s));
     tm.end(RC_STAGE_SEED);
-    tm.begin(RC_STAGE_NEAR_SDF);
-    HIP_TRY(c, launch_near_sdf(c->seed[0], c->sdf, c->W, c->H, s));
-    tm.end(RC_STAGE_NEAR_SDF);
+    // The near distance field needs only the occupancy bits: it runs on the
+    // side stream while the mip pyramid and the cones run here (a profiled
+    // frame runs it in line so each stage has its own time).
+    const bool fork = !prof;
+    if (fork) {
+        HIP_TRY(c, hipEventRecord(c->ev_fork, s));
+        HIP_TRY(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
+        HIP_TRY(c, launch_near_sdf(c->occ_bits, c->sdf, c->W, c->H, c->side));
+        HIP_TRY(c, hipEventRecord(c->ev_join, c->side));
+    } else {
+        tm.begin(RC_STAGE_NEAR_SDF);
+        HIP_TRY(c, launch_near_sdf(c->occ_bits, c->sdf, c->W, c->H, s));
+        tm.end(RC_STAGE_NEAR_SDF);
+    }
     tm.begin(RC_STAGE_MIP);
-    HIP_TRY(c, launch_mip(c->seed[0], emissive, c->mip, c->W, c->H, c->flag_count, s));
+    HIP_TRY(c, launch_mip_rest(c->mip, c->flag_count, s));
     tm.end(RC_STAGE_MIP);
     tm.begin(RC_STAGE_CONE);
     HIP_TRY(c, launch_cone(cone_args(c), s));
     tm.end(RC_STAGE_CONE);
+    if (fork) HIP_TRY(c, hipStreamWaitEvent(s, c->ev_join, 0));
     SurrLevel1Args a{};
     a.sdf = c->sdf;
     a.emissive = emissive;
@@ -576,8 +598,8 @@ static rc_status frame(rc_ctx *c, const float *depth, const float *emissive, flo
     return RC_OK;
 }
 
-// Training rows of the G-buffer at the context's size (plain frame first; the
-// seed field is rebuilt because the jump flood reuses its buffer).
+// Training rows of the G-buffer at the context's size: the plain frame, then
+// the mip pyramid and cones of the same G-buffer.
 static rc_status gen_samples(rc_ctx *c, const float *depth, const float *emissive, float *feats, float *targets,
                              hipStream_t s)
 {
@@ -586,8 +608,8 @@ static rc_status gen_samples(rc_ctx *c, const float *depth, const float *emissiv
     if (!c->scratch_irr) HIP_TRY(c, hipMalloc(&c->scratch_irr, 3 * (size_t)c->W * c->H * sizeof(float)));
     st = run_frame(c, depth, emissive, c->scratch_irr, s, true);
     if (st != RC_OK) return st;
-    HIP_TRY(c, launch_seed_init(depth, emissive, c->seed[0], c->W, c->H, s));
-    HIP_TRY(c, launch_mip(c->seed[0], emissive, c->mip, c->W, c->H, c->flag_count, s));
+    HIP_TRY(c, launch_seed_surr(depth, emissive, c->occ_bits, c->mip, c->W, c->H, s));
+    HIP_TRY(c, launch_mip_rest(c->mip, c->flag_count, s));
     HIP_TRY(c, launch_cone(cone_args(c), s));
     SurrSampleArgs a{};
     a.cone = c->cone;
@@ -773,6 +795,9 @@ void rc_destroy(rc_ctx *c)
     for (auto &ev : c->ev)
         if (ev) hipEventDestroy(ev);
     if (c->cap_stream) hipStreamDestroy(c->cap_stream);
+    if (c->side) hipStreamDestroy(c->side);
+    if (c->ev_fork) hipEventDestroy(c->ev_fork);
+    if (c->ev_join) hipEventDestroy(c->ev_join);
     delete c;
 }
 
@@ -1403,8 +1428,7 @@ rc_status rc_debug_mip(rc_ctx *c, uint32_t level, float *emc, float *mo, void *s
     hipStream_t s = (hipStream_t)stream;
     HIP_TRY(c, hipSetDevice(c->cfg.device));
     const size_t n = (size_t)c->mip.w[level] * c->mip.h[level];
-    if (emc) HIP_TRY(c, hipMemcpyAsync(emc, c->mip.emc + c->mip.off[level], n * sizeof(float4), hipMemcpyDeviceToDevice, s));
-    if (mo) HIP_TRY(c, hipMemcpyAsync(mo, c->mip.mo + c->mip.off[level], n * sizeof(float), hipMemcpyDeviceToDevice, s));
+    if (n && (emc || mo)) HIP_TRY(c, launch_mip_unpack(c->mip, (int)level, emc, mo, s));
     return RC_OK;
 }
 

@@ -226,17 +226,30 @@ __global__ __launch_bounds__(kBlock) void k_level0(LevelArgs a, float *__restric
 
     const __amdgpu_buffer_rsrc_t sdf = sdf_rsrc(a.sdf, a.W, a.H);
     const size_t plane = (size_t)a.W * a.H;
-#pragma unroll 1
+    // the lane's ROUNDS rays (same direction k: 256 rays per round) marched in
+    // lockstep, so their dependent distance-field loads overlap
+    const int k = threadIdx.x & 3;
+    const float2 dir = a.dirs[k];
+    float cxs[ROUNDS], cys[ROUNDS];
+    bool acts[ROUNDS];
+    int codes[ROUNDS];
+#pragma unroll
+    for (int round = 0; round < ROUNDS; ++round) {
+        const int slot = (round * kBlock + (int)threadIdx.x) >> 2;
+        const int px = px0 - 1 + slot % NS, py = py0 - 1 + slot / NS;
+        acts[round] = slot < NS * NS && px >= 0 && py >= 0 && px < a.PX && py < a.PY;
+        cxs[round] = ((float)px + 0.5f) * 2.0f;
+        cys[round] = ((float)py + 0.5f) * 2.0f;
+    }
+    march_rays<ROUNDS>(sdf, a.W, a.H, cxs, cys, dir.x, dir.y, 0.0f, 2.0f, acts, codes);
+#pragma unroll
     for (int round = 0; round < ROUNDS; ++round) {
         const int ray = round * kBlock + threadIdx.x;
-        const int slot = ray >> 2, k = ray & 3;
+        const int slot = ray >> 2;
         const int px = px0 - 1 + slot % NS, py = py0 - 1 + slot / NS;
-        const bool active = slot < NS * NS && px >= 0 && py >= 0 && px < a.PX && py < a.PY;
+        const bool active = acts[round];
+        const int code = codes[round];
         float r = 0.f, g = 0.f, b = 0.f, al = 0.f;
-        const float2 dir = a.dirs[k];
-        const float cx = ((float)px + 0.5f) * 2.0f, cy = ((float)py + 0.5f) * 2.0f;
-        int code;
-        march_rays<1>(sdf, a.W, a.H, &cx, &cy, dir.x, dir.y, 0.0f, 2.0f, &active, &code);
         if (active) {
             if (code >= 0) {
                 r = a.emissive[code];

@@ -59,13 +59,8 @@ template <> struct Texel<Storage::F32> {
     __device__ static inline float4 unpack(T v) { return v; }
 };
 
-__device__ inline uint32_t f32_to_bf16_rne(float f)
-{
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (u >> 16) | 0x40u;  // quiet NaN
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return u >> 16;
-}
+// fp32 -> bf16, round to nearest even: one v_cvt_pk_bf16_f32 on gfx950.
+__device__ inline uint32_t f32_to_bf16_rne(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 __device__ inline float bf16_to_f32(uint32_t h) { return __uint_as_float(h << 16); }
 
 __device__ inline uint32_t f32_to_f16_rne(float f)

@@ -100,28 +100,41 @@ hipError_t launch_sum_f32(const float *x, int n, float *out, hipStream_t s);
 
 // ---- surrogate far field (rc_surrogate.hip) ---------------------------------
 // Emitter / occluder mip pyramid: level m (1..levels) has cells of 2^m x 2^m
-// pixels, ceil(W/2^m) x ceil(H/2^m) of them; cell = mean over its pixels
-// (pixels outside the image count as empty) of (E*[emitter], [emitter]) in
-// emc and of [occupied and not emitter] in mo.
+// pixels, ceil(W/2^m) x ceil(H/2^m) of them; texel = (mean E*[emitter] rgb,
+// coverage) over the cell's pixels (pixels outside the image are empty), the
+// coverage word holding fp16 means of [emitter] (low) and [occupied, not
+// emitter] (high).  Each level is stored with a one-cell border of zero
+// texels ((w+2) x (h+2), cell (x, y) at (y+1)(w+2) + x+1), written once at
+// allocation, so a bilinear footprint that overlaps the image edge needs no
+// per-tap test.
 constexpr int kMipMax = 14;
 struct MipArgs {
-    float4 *emc;            // all levels back to back
-    float *mo;
+    float4 *tex;            // all levels back to back
     int levels;             // top level index (>= 1)
     int w[kMipMax + 1], h[kMipMax + 1];
     size_t off[kMipMax + 1];
 };
+__host__ __device__ inline size_t mip_at(const MipArgs &m, int l, int x, int y)
+{
+    return m.off[l] + (size_t)(y + 1) * (size_t)(m.w[l] + 2) + (size_t)(x + 1);
+}
 // Cone trace schedule: every cone takes the same distances t_n, so the mip
-// level m_n and the occluder path length len_n are uniform over a launch.
+// level m_n and the occluder path length len_n are uniform over a launch; a
+// step carries what its samples need (one scalar load of 32 bytes).
 constexpr int kConeMaxSteps = 64;
+struct ConeStep {
+    float t, len, inv_cell;  // distance, occluder path length, 2^-m
+    int m, w, h;             // mip level and its size in cells
+    uint32_t off, wp;        // first (border) texel of the level, padded row length w+2
+};
 struct ConeSchedule {
     int n;
-    float t[kConeMaxSteps];
-    float len[kConeMaxSteps];
-    int m[kConeMaxSteps];
+    int pad[7];
+    ConeStep s[kConeMaxSteps];
 };
 struct ConeArgs {
     MipArgs mip;
+    uint32_t mip_bytes;     // size of the whole pyramid (range of the buffer loads)
     ConeSchedule sch;
     const float2 *dirs;     // the 16 level-1 directions
     float4 *cone;           // [QY][QX][16]: far-field radiance rgb, transmittance
@@ -150,11 +163,13 @@ struct SurrSampleArgs {
 
 void mip_layout(int W, int H, int top_level, MipArgs &m);   // fills w, h, off, levels
 size_t mip_cells(const MipArgs &m);
-void cone_schedule(int W, int H, int top_level, ConeSchedule &s);
+void cone_schedule(int W, int H, const MipArgs &m, ConeSchedule &s);
 int cone_top_level(int W, int H);
-hipError_t launch_near_sdf(const uint32_t *seed, float *sdf, int W, int H, hipStream_t s);
-hipError_t launch_mip(const uint32_t *seed, const float *emissive, const MipArgs &m, int W, int H, int *zero_me,
-                      hipStream_t s);
+hipError_t launch_seed_surr(const float *depth, const float *emissive, uint64_t *bits, const MipArgs &m, int W, int H,
+                            hipStream_t s);
+hipError_t launch_near_sdf(const uint64_t *bits, float *sdf, int W, int H, hipStream_t s);
+hipError_t launch_mip_rest(const MipArgs &m, int *zero_me, hipStream_t s);
+hipError_t launch_mip_unpack(const MipArgs &m, int level, float *emc, float *mo, hipStream_t s);
 hipError_t launch_cone(const ConeArgs &a, hipStream_t s);
 hipError_t launch_surr_level1(const SurrLevel1Args &a, const MlpDev &m, Storage st, hipStream_t s);
 hipError_t launch_surr_sample(const SurrSampleArgs &a, Storage st, hipStream_t s);

@@ -5,12 +5,12 @@
 // None of this exists in the reference (it has no global illumination at
 // all; its only pass is the forward raster at src/renderer.rs:559-613).  The
 // frame is
-//   seed init (rc_jfa.hip)                      occupancy
-//   k_near_sdf                                  exact distance field up to 10 px
-//   k_mip / k_mip_top                           emitter / occluder mip pyramid
-//   k_cone                                      16 cones per level-2 probe
-//   k_surr_level1                               level-1 march + MLP + merge
-//   k_level0 (rc_cascade.hip)                   level 0 + gather
+//   k_seed_surr                 occupancy bits + mip levels 1, 2
+//   k_near_sdf                  exact distance field up to 10 px
+//   k_mip_mid / k_mip_top       mip levels 3..top
+//   k_cone                      16 cones per level-2 probe
+//   k_surr_level1               level-1 march + MLP + merge
+//   k_level0 (rc_cascade.hip)   level 0 + gather
 // Cone tracing through a prefiltered pyramid is Crassin et al.'s voxel cone
 // tracing (2011) in screen space; here it is the MLP's input feature, not the
 // answer.
@@ -20,7 +20,8 @@
 // mip level m_n whose cells match the cone width; occluder opacity
 // a_o = 1 - (1 - min(mo, 0.999))^len_n (a thin silhouette blocks the whole
 // cone cross-section it crosses), total opacity a = min(me + a_o, 1);
-// L += T * emc.rgb, T *= 1 - a; a cone stops once T < 1e-4.
+// L += T * emc.rgb, T *= 1 - a; a cone stops once T < 1e-4 or once its
+// bilinear footprint lies wholly outside the image.
 #include <math.h>
 
 #include "rc_internal.hpp"
@@ -31,39 +32,132 @@ namespace rc {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
-constexpr uint32_t kFarSeedS = 0x7FFF7FFFu;  // the jump flood's internal "no seed" value
+// ---- packed mip texels ----------------------------------------------------------
+// texel = (mean E*[emitter] r, g, b, coverage) with coverage = fp16 [emitter]
+// mean (low half) | fp16 [occluder] mean (high half), both rounded to nearest
+// even.  A parent cell is the mean of its 4 stored children in the order
+// ((x0y0 + x1y0) + (x0y1 + x1y1)) * 1/4, fp32, then packed.
+__device__ inline float pack_cov(float me, float mo)
+{
+    return __uint_as_float(f32_to_f16_rne(me) | (f32_to_f16_rne(mo) << 16));
+}
+__device__ inline void unpack_cov(float w, float &me, float &mo)
+{
+    const uint32_t u = __float_as_uint(w);
+    me = f16_to_f32(u & 0xFFFFu);
+    mo = f16_to_f32(u >> 16);
+}
+__device__ inline float4 mean4(float4 a, float4 b, float4 c, float4 d)
+{
+    float ma, oa, mb, ob, mc, oc, md, od;
+    unpack_cov(a.w, ma, oa);
+    unpack_cov(b.w, mb, ob);
+    unpack_cov(c.w, mc, oc);
+    unpack_cov(d.w, md, od);
+    return make_float4(0.25f * ((a.x + b.x) + (c.x + d.x)), 0.25f * ((a.y + b.y) + (c.y + d.y)),
+                       0.25f * ((a.z + b.z) + (c.z + d.z)),
+                       pack_cov(0.25f * ((ma + mb) + (mc + md)), 0.25f * ((oa + ob) + (oc + od))));
+}
+
+// ---- occupancy bits + mip levels 1 and 2 -------------------------------------------
+// One 64x4-pixel block (a wave per row): the occupancy rule of the jump
+// flood's seed pass (emissive > 0 in any channel, or the near side of a depth
+// edge), one 64-bit ballot per row into bits[y][x/64], and the block's mip
+// cells: 32x2 of level 1 and 16x1 of level 2.  Pixels outside the image are
+// empty.
+__global__ __launch_bounds__(kBlock) void k_seed_surr(const float *__restrict__ depth, const float *__restrict__ emissive,
+                                                      uint64_t *__restrict__ bits, int words, MipArgs m, int W, int H,
+                                                      int bx_count)
+{
+    __shared__ float4 pe[4][64];
+    __shared__ float4 c1[2][32];
+    const int bx = blockIdx.x % bx_count, by = blockIdx.x / bx_count;
+    const int lane = threadIdx.x & 63, row = threadIdx.x >> 6;
+    const int x = bx * 64 + lane, y = by * 4 + row;
+    const bool in = x < W && y < H;
+    bool occ = false, em = false;
+    float e0 = 0.f, e1 = 0.f, e2 = 0.f;
+    if (in) {
+        const size_t plane = (size_t)W * H, p = (size_t)y * W + x;
+        e0 = emissive[p];
+        e1 = emissive[plane + p];
+        e2 = emissive[2 * plane + p];
+        const float dp = depth[p];
+        const float dl = depth[p - (x > 0)], dr = depth[p + (x + 1 < W)];
+        const float du = depth[p - (y > 0 ? (size_t)W : 0)], dd = depth[p + (y + 1 < H ? (size_t)W : 0)];
+        const float thr = 0.05f * dp;
+        em = e0 > 0.0f || e1 > 0.0f || e2 > 0.0f;
+        occ = em || (x > 0 && dl - dp > thr) || (x + 1 < W && dr - dp > thr) || (y > 0 && du - dp > thr) ||
+              (y + 1 < H && dd - dp > thr);
+    }
+    const uint64_t w = __ballot(occ);
+    if (lane == 0 && y < H) bits[(size_t)y * words + bx] = w;
+    pe[row][lane] = em ? make_float4(e0, e1, e2, pack_cov(1.0f, 0.0f))
+                       : make_float4(0.f, 0.f, 0.f, pack_cov(0.0f, occ ? 1.0f : 0.0f));
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int cx = threadIdx.x & 31, cy = threadIdx.x >> 5;
+        const float4 t = mean4(pe[2 * cy][2 * cx], pe[2 * cy][2 * cx + 1], pe[2 * cy + 1][2 * cx], pe[2 * cy + 1][2 * cx + 1]);
+        c1[cy][cx] = t;
+        const int gx = bx * 32 + cx, gy = by * 2 + cy;
+        if (gx < m.w[1] && gy < m.h[1]) m.tex[mip_at(m, 1, gx, gy)] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < 16 && m.levels >= 2) {
+        const int cx = threadIdx.x;
+        const float4 t = mean4(c1[0][2 * cx], c1[0][2 * cx + 1], c1[1][2 * cx], c1[1][2 * cx + 1]);
+        const int gx = bx * 16 + cx, gy = by;
+        if (gx < m.w[2] && gy < m.h[2]) m.tex[mip_at(m, 2, gx, gy)] = t;
+    }
+}
+
+hipError_t launch_seed_surr(const float *depth, const float *emissive, uint64_t *bits, const MipArgs &m, int W, int H,
+                            hipStream_t s)
+{
+    const int bxc = (W + 63) / 64, byc = (H + 3) / 4;
+    hipLaunchKernelGGL(k_seed_surr, dim3(bxc * byc), dim3(kBlock), 0, s, depth, emissive, bits, bxc, m, W, H, bxc);
+    return hipGetLastError();
+}
 
 // ---- near-field distance field ---------------------------------------------
 // Exact Euclidean distance to the nearest occupied pixel wherever it is at most
 // kNearR pixels, +inf elsewhere.  Levels 0 and 1 end their intervals at 2 and
 // 10 px, so every step they take from such a field (a hit needs distance 0; a
 // distance above 10 ends the interval in one step either way) equals the step
-// they take from the full field.  Per 32x32 tile: occupancy row masks of the
-// tile plus a 10-px halo (one ballot per row), the distance to the nearest
-// occupied pixel in the row within +-10 (count of zero bits), then
-// d^2 = min over |dy| <= 10 of g(y+dy)^2 + dy^2.
+// they take from the full field.  Per 32x32 tile: the occupancy row masks of
+// the tile plus a 10-px halo (two words of the bit field per row), the
+// distance to the nearest occupied pixel of the row within +-10 (count of
+// zero bits), then d^2 = min over |dy| <= 10 of g(y+dy)^2 + dy^2.
 constexpr int kNearR = 10, kNearT = 32, kNearS = kNearT + 2 * kNearR;
 
-__global__ __launch_bounds__(kBlock) void k_near_sdf(const uint32_t *__restrict__ seed, float *__restrict__ sdf, int W,
-                                                     int H, int tiles_x)
+__global__ __launch_bounds__(kBlock) void k_near_sdf(const uint64_t *__restrict__ bits, int words,
+                                                     float *__restrict__ sdf, int W, int H, int tiles_x)
 {
     __shared__ uint64_t mask[kNearS];
     __shared__ uint8_t g[kNearS][kNearT];
     const int x0 = (blockIdx.x % tiles_x) * kNearT, y0 = (blockIdx.x / tiles_x) * kNearT;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int r = wave; r < kNearS; r += kBlock / 64) {
-        const int y = y0 - kNearR + r, x = x0 - kNearR + lane;
-        bool occ = false;
-        if (lane < kNearS && x >= 0 && x < W && y >= 0 && y < H) occ = seed[(size_t)y * W + x] != kFarSeedS;
-        const uint64_t m = __ballot(occ);
-        if (lane == 0) mask[r] = m;
+    if (threadIdx.x < kNearS) {
+        const int r = threadIdx.x, y = y0 - kNearR + r;
+        uint64_t mk = 0;
+        if (y >= 0 && y < H) {
+            const uint64_t *rowp = bits + (size_t)y * words;
+            const int xs = x0 - kNearR;  // bit j of mk <-> column xs + j
+            if (xs < 0) {
+                mk = rowp[0] << (-xs);
+            } else {
+                const int wi = xs >> 6, sh = xs & 63;
+                const uint64_t lo = rowp[wi], hi = wi + 1 < words ? rowp[wi + 1] : 0ull;
+                mk = sh ? (lo >> sh) | (hi << (64 - sh)) : lo;
+            }
+        }
+        mask[r] = mk;
     }
     __syncthreads();
     for (int i = threadIdx.x; i < kNearS * kNearT; i += kBlock) {
         const int r = i / kNearT, j = i % kNearT, b = j + kNearR;
-        const uint64_t m = mask[r];
-        const uint64_t right = m >> b;         // bit d: column b + d
-        const uint64_t left = m << (63 - b);   // bit 63 - d: column b - d
+        const uint64_t mk = mask[r];
+        const uint64_t right = mk >> b;         // bit d: column b + d
+        const uint64_t left = mk << (63 - b);   // bit 63 - d: column b - d
         int d = kNearR + 1;
         if (right) d = min(d, (int)__builtin_ctzll(right));
         if (left) d = min(d, (int)__builtin_clzll(left));
@@ -84,14 +178,14 @@ __global__ __launch_bounds__(kBlock) void k_near_sdf(const uint32_t *__restrict_
     }
 }
 
-hipError_t launch_near_sdf(const uint32_t *seed, float *sdf, int W, int H, hipStream_t s)
+hipError_t launch_near_sdf(const uint64_t *bits, float *sdf, int W, int H, hipStream_t s)
 {
     const int tx = (W + kNearT - 1) / kNearT, ty = (H + kNearT - 1) / kNearT;
-    hipLaunchKernelGGL(k_near_sdf, dim3(tx * ty), dim3(kBlock), 0, s, seed, sdf, W, H, tx);
+    hipLaunchKernelGGL(k_near_sdf, dim3(tx * ty), dim3(kBlock), 0, s, bits, (W + 63) / 64, sdf, W, H, tx);
     return hipGetLastError();
 }
 
-// ---- mip pyramid -------------------------------------------------------------
+// ---- mip pyramid, levels 3.. -----------------------------------------------------
 int cone_top_level(int W, int H)
 {
     int top = 1;
@@ -112,78 +206,42 @@ void mip_layout(int W, int H, int top, MipArgs &m)
         m.w[l] = (W + (1 << l) - 1) >> l;
         m.h[l] = (H + (1 << l) - 1) >> l;
         m.off[l] = off;
-        off += (size_t)m.w[l] * m.h[l];
+        off += (size_t)(m.w[l] + 2) * (m.h[l] + 2);
     }
 }
 
-size_t mip_cells(const MipArgs &m) { return m.off[m.levels] + (size_t)m.w[m.levels] * m.h[m.levels]; }
+size_t mip_cells(const MipArgs &m) { return m.off[m.levels] + (size_t)(m.w[m.levels] + 2) * (m.h[m.levels] + 2); }
 
-// Levels 1..6 of a 64x64-pixel tile in LDS (cells are aligned to the tile).
-constexpr int kMipTile = 64, kMipInTile = 6;
+// Levels 3..6 of a 16x16 block of level-2 cells (64x64 pixels) in LDS.
+constexpr int kMidTile = 16, kMidLast = 6;
 
-__global__ __launch_bounds__(kBlock) void k_mip(const uint32_t *__restrict__ seed, const float *__restrict__ emissive,
-                                                MipArgs m, int W, int H, int tiles_x)
+__global__ __launch_bounds__(kBlock) void k_mip_mid(MipArgs m, int tiles_x)
 {
-    __shared__ float4 se[32 * 32];
-    __shared__ float so[32 * 32];
+    __shared__ float4 se[kMidTile * kMidTile];
     const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
-    const size_t plane = (size_t)W * H;
-    for (int c = threadIdx.x; c < 32 * 32; c += kBlock) {
-        const int cx = c & 31, cy = c >> 5;
-        float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
-        float o = 0.f;
-#pragma unroll
-        for (int d = 0; d < 4; ++d) {
-            const int x = tx * kMipTile + 2 * cx + (d & 1), y = ty * kMipTile + 2 * cy + (d >> 1);
-            if (x < W && y < H) {
-                const size_t p = (size_t)y * W + x;
-                const float r = emissive[p], g = emissive[plane + p], b = emissive[2 * plane + p];
-                if (r > 0.f || g > 0.f || b > 0.f) {
-                    e.x += r;
-                    e.y += g;
-                    e.z += b;
-                    e.w += 1.f;
-                } else if (seed[p] != kFarSeedS) {
-                    o += 1.f;
-                }
-            }
-        }
-        e = make_float4(e.x * 0.25f, e.y * 0.25f, e.z * 0.25f, e.w * 0.25f);
-        o *= 0.25f;
-        se[c] = e;
-        so[c] = o;
-        const int gx = tx * 32 + cx, gy = ty * 32 + cy;
-        if (gx < m.w[1] && gy < m.h[1]) {
-            m.emc[m.off[1] + (size_t)gy * m.w[1] + gx] = e;
-            m.mo[m.off[1] + (size_t)gy * m.w[1] + gx] = o;
-        }
+    {
+        const int cx = threadIdx.x % kMidTile, cy = threadIdx.x / kMidTile;
+        const int gx = tx * kMidTile + cx, gy = ty * kMidTile + cy;
+        se[threadIdx.x] = (gx < m.w[2] && gy < m.h[2]) ? m.tex[mip_at(m, 2, gx, gy)]
+                                                       : make_float4(0.f, 0.f, 0.f, pack_cov(0.f, 0.f));
     }
     __syncthreads();
-    int n = 32;
-    const int last = m.levels < kMipInTile ? m.levels : kMipInTile;
-    for (int l = 2; l <= last; ++l) {
+    int n = kMidTile;
+    const int lastl = m.levels < kMidLast ? m.levels : kMidLast;
+    for (int l = 3; l <= lastl; ++l) {
         const int nn = n >> 1;
         const bool have = (int)threadIdx.x < nn * nn;
-        float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
-        float o = 0.f;
+        float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
         if (have) {
-            const int cx = threadIdx.x % nn, cy = threadIdx.x / nn;
-            const int i00 = (2 * cy) * n + 2 * cx;
-            const float4 a = se[i00], b = se[i00 + 1], c = se[i00 + n], d = se[i00 + n + 1];
-            e = make_float4(0.25f * ((a.x + b.x) + (c.x + d.x)), 0.25f * ((a.y + b.y) + (c.y + d.y)),
-                            0.25f * ((a.z + b.z) + (c.z + d.z)), 0.25f * ((a.w + b.w) + (c.w + d.w)));
-            o = 0.25f * ((so[i00] + so[i00 + 1]) + (so[i00 + n] + so[i00 + n + 1]));
+            const int cx = threadIdx.x % nn, cy = threadIdx.x / nn, i00 = (2 * cy) * n + 2 * cx;
+            t = mean4(se[i00], se[i00 + 1], se[i00 + n], se[i00 + n + 1]);
         }
         __syncthreads();
         if (have) {
-            se[threadIdx.x] = e;
-            so[threadIdx.x] = o;
+            se[threadIdx.x] = t;
             const int cx = threadIdx.x % nn, cy = threadIdx.x / nn;
             const int gx = tx * nn + cx, gy = ty * nn + cy;
-            if (gx < m.w[l] && gy < m.h[l]) {
-                m.emc[m.off[l] + (size_t)gy * m.w[l] + gx] = e;
-                m.mo[m.off[l] + (size_t)gy * m.w[l] + gx] = o;
-            }
+            if (gx < m.w[l] && gy < m.h[l]) m.tex[mip_at(m, l, gx, gy)] = t;
         }
         __syncthreads();
         n = nn;
@@ -191,44 +249,58 @@ __global__ __launch_bounds__(kBlock) void k_mip(const uint32_t *__restrict__ see
 }
 
 // Levels 7..top from level 6, one workgroup; also clears the fallback counter
-// of the frame (a kernel, not a memset node, so graph replays stay exact).
+// of the frame (a kernel store, not a memset node, so graph replays stay exact).
 __global__ __launch_bounds__(kBlock) void k_mip_top(MipArgs m, int *zero_me)
 {
     if (zero_me && threadIdx.x == 0) *zero_me = 0;
-    for (int l = kMipInTile + 1; l <= m.levels; ++l) {
+    for (int l = kMidLast + 1; l <= m.levels; ++l) {
         const int w = m.w[l], h = m.h[l], cw = m.w[l - 1], ch = m.h[l - 1];
-        const float4 *ce = m.emc + m.off[l - 1];
-        const float *co = m.mo + m.off[l - 1];
+        const float4 zero = make_float4(0.f, 0.f, 0.f, pack_cov(0.f, 0.f));
         for (int i = threadIdx.x; i < w * h; i += kBlock) {
             const int x = i % w, y = i / w;
             float4 v[4];
-            float o[4];
 #pragma unroll
             for (int d = 0; d < 4; ++d) {
                 const int cx = 2 * x + (d & 1), cy = 2 * y + (d >> 1);
-                const bool in = cx < cw && cy < ch;
-                v[d] = in ? ce[(size_t)cy * cw + cx] : make_float4(0.f, 0.f, 0.f, 0.f);
-                o[d] = in ? co[(size_t)cy * cw + cx] : 0.f;
+                v[d] = (cx < cw && cy < ch) ? m.tex[mip_at(m, l - 1, cx, cy)] : zero;
             }
-            m.emc[m.off[l] + i] = make_float4(0.25f * ((v[0].x + v[1].x) + (v[2].x + v[3].x)),
-                                              0.25f * ((v[0].y + v[1].y) + (v[2].y + v[3].y)),
-                                              0.25f * ((v[0].z + v[1].z) + (v[2].z + v[3].z)),
-                                              0.25f * ((v[0].w + v[1].w) + (v[2].w + v[3].w)));
-            m.mo[m.off[l] + i] = 0.25f * ((o[0] + o[1]) + (o[2] + o[3]));
+            m.tex[mip_at(m, l, x, y)] = mean4(v[0], v[1], v[2], v[3]);
         }
         __threadfence_block();
         __syncthreads();
     }
 }
 
-hipError_t launch_mip(const uint32_t *seed, const float *emissive, const MipArgs &m, int W, int H, int *zero_me,
-                      hipStream_t s)
+hipError_t launch_mip_rest(const MipArgs &m, int *zero_me, hipStream_t s)
 {
-    const int tx = (W + kMipTile - 1) / kMipTile, ty = (H + kMipTile - 1) / kMipTile;
-    hipLaunchKernelGGL(k_mip, dim3(tx * ty), dim3(kBlock), 0, s, seed, emissive, m, W, H, tx);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    if (m.levels >= 3) {
+        const int tx = (m.w[2] + kMidTile - 1) / kMidTile, ty = (m.h[2] + kMidTile - 1) / kMidTile;
+        hipLaunchKernelGGL(k_mip_mid, dim3(tx * ty), dim3(kBlock), 0, s, m, tx);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(k_mip_top, dim3(1), dim3(kBlock), 0, s, m, zero_me);
+    return hipGetLastError();
+}
+
+// Unpacked copy of mip level l (parity readback): emc [h][w][4], mo [h][w].
+__global__ __launch_bounds__(kBlock) void k_mip_unpack(MipArgs m, int l, float4 *__restrict__ emc, float *__restrict__ mo)
+{
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    const size_t n = (size_t)m.w[l] * m.h[l];
+    if (i >= n) return;
+    const float4 t = m.tex[mip_at(m, l, (int)(i % m.w[l]), (int)(i / m.w[l]))];
+    float me, o;
+    unpack_cov(t.w, me, o);
+    if (emc) emc[i] = make_float4(t.x, t.y, t.z, me);
+    if (mo) mo[i] = o;
+}
+
+hipError_t launch_mip_unpack(const MipArgs &m, int level, float *emc, float *mo, hipStream_t s)
+{
+    const size_t n = (size_t)m.w[level] * m.h[level];
+    hipLaunchKernelGGL(k_mip_unpack, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, m, level,
+                       (float4 *)emc, mo);
     return hipGetLastError();
 }
 
@@ -237,8 +309,9 @@ hipError_t launch_mip(const uint32_t *seed, const float *emissive, const MipArgs
 // px from it and its far field starts at 10 px), cone diameter
 // dia = max(2 t tan(pi/16), 1), mip level floor(log2 dia) clamped to
 // [1, top], next t = t + 0.75 dia, up to the image diagonal + 2.
-void cone_schedule(int W, int H, int top, ConeSchedule &s)
+void cone_schedule(int W, int H, const MipArgs &mip, ConeSchedule &s)
 {
+    const int top = mip.levels;
     const double tmax = sqrt((double)W * W + (double)H * H) + 2.0, tn = tan(M_PI / 16.0);
     double t = 6.0;
     s.n = 0;
@@ -247,70 +320,97 @@ void cone_schedule(int W, int H, int top, ConeSchedule &s)
         int m = (int)floor(log2(dia));
         m = m < 1 ? 1 : (m > top ? top : m);
         const double step = 0.75 * dia;
-        s.t[s.n] = (float)t;
-        s.m[s.n] = m;
-        s.len[s.n] = (float)fmin(step, (double)(1 << m));
-        ++s.n;
+        ConeStep &c = s.s[s.n++];
+        c.t = (float)t;
+        c.len = (float)fmin(step, (double)(1 << m));
+        c.inv_cell = 1.0f / (float)(1 << m);
+        c.m = m;
+        c.w = mip.w[m];
+        c.h = mip.h[m];
+        c.off = (uint32_t)mip.off[m];
+        c.wp = (uint32_t)(mip.w[m] + 2);
         t += step;
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_cone(ConeArgs a)
+// Workgroups are dealt to the 8 XCDs round-robin; give each XCD a contiguous
+// run of logical blocks (a band of probe rows), so the fine mip levels it
+// reads stay in its own L2.
+__device__ inline int xcd_chunk(int b, int nb)
 {
-    const int gi = blockIdx.x * kBlock + threadIdx.x;
-    const int k = gi & 15, q = gi >> 4;
-    if (q >= a.QX * a.QY) return;
-    const int qx = q % a.QX, qy = q / a.QX;
+    const int q = nb >> 3, r = nb & 7;
+    const int xcd = b & 7, idx = b >> 3;
+    return xcd * q + min(xcd, r) + idx;
+}
+
+// A workgroup is 64 consecutive level-2 probes of one row x 4 directions, a
+// wave per direction: the lanes of a wave sample points 8 px apart at the same
+// distance, so from mip level 3 up they share cells (and cache lines).  The 4
+// taps of a step are one 32-bit offset and 4 range-checked buffer loads (the
+// zero border stands in for taps beyond the image edge).
+__global__ __launch_bounds__(kBlock) void k_cone(ConeArgs a, int seg_x)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lb = xcd_chunk(blockIdx.x, gridDim.x);
+    const int kgrp = lb / (seg_x * a.QY), rest = lb % (seg_x * a.QY);
+    const int qy = rest / seg_x, seg = rest % seg_x;
+    const int k = kgrp * 4 + wave;
+    const int qx = seg * 64 + lane;
+    if (qx >= a.QX) return;
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(a.mip.tex, (short)0, (int)a.mip_bytes, kBufferDword3);
     const float cx = ((float)qx + 0.5f) * a.spacing, cy = ((float)qy + 0.5f) * a.spacing;
     const float2 d = a.dirs[k];
     float T = 1.0f, r = 0.f, g = 0.f, b = 0.f;
     for (int n = 0; n < a.sch.n; ++n) {
-        const int m = a.sch.m[n];
-        const float t = a.sch.t[n];
-        const float inv = 1.0f / (float)(1 << m);
-        const float fx = fmaf(t, d.x, cx) * inv - 0.5f, fy = fmaf(t, d.y, cy) * inv - 0.5f;
+        const ConeStep st = a.sch.s[n];
+        const float fx = fmaf(st.t, d.x, cx) * st.inv_cell - 0.5f, fy = fmaf(st.t, d.y, cy) * st.inv_cell - 0.5f;
         const float flx = floorf(fx), fly = floorf(fy);
         const int ix = (int)flx, iy = (int)fly;
+        // the footprint has left the image: a straight ray does not come back
+        if (!(ix >= -1 && ix < st.w && iy >= -1 && iy < st.h)) break;
         const float ax = fx - flx, ay = fy - fly;
-        const int w = a.mip.w[m], h = a.mip.h[m];
-        const float4 *E = a.mip.emc + a.mip.off[m];
-        const float *O = a.mip.mo + a.mip.off[m];
-        float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
-        float o = 0.f;
+        const uint32_t o00 = (st.off + __umul24((uint32_t)(iy + 1), st.wp) + (uint32_t)(ix + 1)) << 4;
+        const uint32_t orow = st.wp << 4;
+        const float4 v00 = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, (int)o00, 0, 0));
+        const float4 v10 = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, (int)(o00 + 16u), 0, 0));
+        const float4 v01 = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, (int)(o00 + orow), 0, 0));
+        const float4 v11 =
+            __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, (int)(o00 + orow + 16u), 0, 0));
+        const float4 v[4] = {v00, v10, v01, v11};
+        const float wt[4] = {(1.0f - ax) * (1.0f - ay), ax * (1.0f - ay), (1.0f - ax) * ay, ax * ay};
+        float er = 0.f, eg = 0.f, eb = 0.f, me = 0.f, mo = 0.f;
 #pragma unroll
         for (int tap = 0; tap < 4; ++tap) {
-            const int x = ix + (tap & 1), y = iy + (tap >> 1);
-            const float wt = ((tap & 1) ? ax : 1.0f - ax) * ((tap >> 1) ? ay : 1.0f - ay);
-            if (x >= 0 && y >= 0 && x < w && y < h) {
-                const float4 v = E[(size_t)y * w + x];
-                e.x = fmaf(wt, v.x, e.x);
-                e.y = fmaf(wt, v.y, e.y);
-                e.z = fmaf(wt, v.z, e.z);
-                e.w = fmaf(wt, v.w, e.w);
-                o = fmaf(wt, O[(size_t)y * w + x], o);
-            }
+            float cme, cmo;
+            unpack_cov(v[tap].w, cme, cmo);
+            er = fmaf(wt[tap], v[tap].x, er);
+            eg = fmaf(wt[tap], v[tap].y, eg);
+            eb = fmaf(wt[tap], v[tap].z, eb);
+            me = fmaf(wt[tap], cme, me);
+            mo = fmaf(wt[tap], cmo, mo);
         }
-        const float ao = 1.0f - exp2f(a.sch.len[n] * log2f(1.0f - fminf(o, 0.999f)));
-        const float al = fminf(e.w + ao, 1.0f);
-        r = fmaf(T, e.x, r);
-        g = fmaf(T, e.y, g);
-        b = fmaf(T, e.z, b);
+        // 1 - (1 - mo)^len with the hardware log2 / exp2 (argument in [1e-3, 1])
+        const float ao = 1.0f - __builtin_amdgcn_exp2f(st.len * __builtin_amdgcn_logf(1.0f - fminf(mo, 0.999f)));
+        const float al = fminf(me + ao, 1.0f);
+        r = fmaf(T, er, r);
+        g = fmaf(T, eg, g);
+        b = fmaf(T, eb, b);
         T = T * (1.0f - al);
         if (T < 1e-4f) break;
     }
-    a.cone[gi] = make_float4(r, g, b, T);
+    a.cone[((size_t)qy * a.QX + qx) * 16 + k] = make_float4(r, g, b, T);
 }
 
 hipError_t launch_cone(const ConeArgs &a, hipStream_t s)
 {
-    const long long threads = (long long)a.QX * a.QY * 16;
-    hipLaunchKernelGGL(k_cone, dim3((unsigned)((threads + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, a);
+    const int seg_x = (a.QX + 63) / 64;
+    hipLaunchKernelGGL(k_cone, dim3(seg_x * a.QY * 4), dim3(kBlock), 0, s, a, seg_x);
     return hipGetLastError();
 }
 
 // ---- level 1 with the learned far field ----------------------------------------
-// A workgroup owns 8x8 level-1 probes per iteration (persistent over tiles, so
-// the bf16 weights are staged in LDS once).  Lane (probe p, quad ko) marches
+// A workgroup owns 8x8 level-1 probes per iteration (persistent over tiles).  Lane (probe p, quad ko) marches
 // the 4 directions 4ko..4ko+3 of p over [2, 10) on the distance field, and
 // builds their 12 inputs: log1p of the cone radiance bilinearly interpolated
 // from the level-2 grid (the merge's weights).  Wave w then computes the
@@ -320,6 +420,10 @@ hipError_t launch_cone(const ConeArgs &a, hipStream_t s)
 // exactly as the plain merge does.  The error head exp(z_48), averaged over
 // the tile, is the tile's predicted mean squared error (normalised units).
 constexpr int kSStrideIn = kMlpIn + 8, kSStrideHid = kMlpHid + 8;
+
+// Input feature of a cone radiance value: log(1 + c) through the hardware
+// log2 (rounded to bf16 before the network, far below fp32 log1p error).
+__device__ inline float feat_log(float c) { return __logf(1.0f + c); }
 
 __device__ inline bf16x8 ld8s(const uint16_t *p) { return *reinterpret_cast<const bf16x8 *>(p); }
 __device__ inline int acc_row_s(int v, int lane) { return ((v >> 2) << 3) + ((lane >> 5) << 2) + (v & 3); }
@@ -383,23 +487,22 @@ template <Storage S>
 __global__ __launch_bounds__(kBlock) void k_surr_level1(SurrLevel1Args a, MlpDev m)
 {
     using TT = Texel<S>;
-    __shared__ __attribute__((aligned(16))) uint16_t wt_s[kMlpWt];
     __shared__ float bias_s[4 * 64], mean_s[kMlpIn], istd_s[kMlpIn];
     __shared__ __attribute__((aligned(16))) uint16_t xs[64 * kSStrideIn];
     __shared__ __attribute__((aligned(16))) uint16_t ha[64 * kSStrideHid], hb[64 * kSStrideHid];
     __shared__ float zs[64][kMlpErr + 2];
     __shared__ float red[kBlock / 64][2];
 
-    for (int i = threadIdx.x; i < kMlpWt / 8; i += kBlock)
-        reinterpret_cast<uint4 *>(wt_s)[i] = reinterpret_cast<const uint4 *>(m.wt)[i];
     for (int i = threadIdx.x; i < 4 * 64; i += kBlock) bias_s[i] = m.bias[i];
     for (int i = threadIdx.x; i < kMlpIn; i += kBlock) {
         mean_s[i] = m.in_mean[i];
         istd_s[i] = m.in_istd[i];
     }
     __syncthreads();
-    const uint16_t *wl[4] = {wt_s, wt_s + 64 * kMlpIn, wt_s + 64 * kMlpIn + 64 * kMlpHid,
-                             wt_s + 64 * kMlpIn + 2 * 64 * kMlpHid};
+    // B operands straight from the bf16 weights in global memory (30 KB, held
+    // in every CU's L1 and the L2): the LDS then fits 4 workgroups per CU.
+    const uint16_t *wl[4] = {m.wt, m.wt + 64 * kMlpIn, m.wt + 64 * kMlpIn + 64 * kMlpHid,
+                             m.wt + 64 * kMlpIn + 2 * 64 * kMlpHid};
 
     const __amdgpu_buffer_rsrc_t sdf = sdf_rsrc(a.sdf, a.W, a.H);
     const int p = threadIdx.x >> 2, ko = threadIdx.x & 3;
@@ -414,8 +517,12 @@ __global__ __launch_bounds__(kBlock) void k_surr_level1(SurrLevel1Args a, MlpDev
         dy[j] = d.y;
     }
 
+    // tiles are dealt XCD by XCD in contiguous ranges (the cone and distance
+    // field rows they read stay in that XCD's L2)
+    const int per_xcd = (int)gridDim.x >> 3, xcd = blockIdx.x & 7, idx = blockIdx.x >> 3;
+    const int t_begin = (int)((long long)a.tiles * xcd / 8), t_end = (int)((long long)a.tiles * (xcd + 1) / 8);
 #pragma unroll 1
-    for (int tile = blockIdx.x; tile < a.tiles; tile += gridDim.x) {
+    for (int tile = t_begin + idx; tile < t_end; tile += per_xcd) {
         const int px = (tile % a.tiles_x) * 8 + (p & 7), py = (tile / a.tiles_x) * 8 + (p >> 3);
         const bool valid = px < a.PX && py < a.PY;
         const float pcx = ((float)px + 0.5f) * 4.0f, pcy = ((float)py + 0.5f) * 4.0f;
@@ -441,11 +548,11 @@ __global__ __launch_bounds__(kBlock) void k_surr_level1(SurrLevel1Args a, MlpDev
             }
             const int col = 3 * k;
             xs[p * kSStrideIn + col + 0] =
-                (uint16_t)f32_to_bf16_rne(valid ? (log1pf(cf[j].x) - mean_s[col + 0]) * istd_s[col + 0] : 0.f);
+                (uint16_t)f32_to_bf16_rne(valid ? (feat_log(cf[j].x) - mean_s[col + 0]) * istd_s[col + 0] : 0.f);
             xs[p * kSStrideIn + col + 1] =
-                (uint16_t)f32_to_bf16_rne(valid ? (log1pf(cf[j].y) - mean_s[col + 1]) * istd_s[col + 1] : 0.f);
+                (uint16_t)f32_to_bf16_rne(valid ? (feat_log(cf[j].y) - mean_s[col + 1]) * istd_s[col + 1] : 0.f);
             xs[p * kSStrideIn + col + 2] =
-                (uint16_t)f32_to_bf16_rne(valid ? (log1pf(cf[j].z) - mean_s[col + 2]) * istd_s[col + 2] : 0.f);
+                (uint16_t)f32_to_bf16_rne(valid ? (feat_log(cf[j].z) - mean_s[col + 2]) * istd_s[col + 2] : 0.f);
         }
         __syncthreads();
 
@@ -543,8 +650,9 @@ __global__ __launch_bounds__(kBlock) void k_surr_level1(SurrLevel1Args a, MlpDev
 
 hipError_t launch_surr_level1(const SurrLevel1Args &a, const MlpDev &m, Storage st, hipStream_t s)
 {
-    int grid = a.tiles < 512 ? a.tiles : 512;
-    if (grid < 1) return hipSuccess;
+    // a multiple of 8 workgroups (one share per XCD), at most 4 per CU
+    int grid = a.tiles < 1024 ? ((a.tiles + 7) / 8) * 8 : 1024;
+    if (a.tiles < 1) return hipSuccess;
     switch (st) {
     case Storage::F32: hipLaunchKernelGGL(k_surr_level1<Storage::F32>, dim3(grid), dim3(kBlock), 0, s, a, m); break;
     case Storage::F16: hipLaunchKernelGGL(k_surr_level1<Storage::F16>, dim3(grid), dim3(kBlock), 0, s, a, m); break;
@@ -578,9 +686,9 @@ __global__ __launch_bounds__(kBlock) void k_surr_sample(SurrSampleArgs a)
                                  TT::unpack(up[((size_t)qy1 * a.QX + qx1) * 16 + k]), px, py);
         float *f = a.feats + (size_t)probe * kMlpIn + 3 * k;
         float *t = a.targets + (size_t)probe * kMlpRad + 3 * k;
-        f[0] = log1pf(c.x);
-        f[1] = log1pf(c.y);
-        f[2] = log1pf(c.z);
+        f[0] = feat_log(c.x);
+        f[1] = feat_log(c.y);
+        f[2] = feat_log(c.z);
         t[0] = u.x - c.x;
         t[1] = u.y - c.y;
         t[2] = u.z - c.z;
