@@ -6,11 +6,16 @@ box scene (seed 0, 32 boxes, 4 emissive), 6 cascade levels, fp32 storage; a
 step is one full GI frame (occupancy + jump flood + SDF + 6 levels of
 march/merge + gather) with the G-buffer already resident in HBM.
 
-Prints ONE JSON line.  `roofline` is the dominant kernel's algorithmic bytes
-per launch over its mean launch time, measured here with HIP events recorded
-on the stream the kernels run on; `cpu_baseline` times the FP32 CPU
-restatement of the same maths (the oracle's FP32 flavour, OpenMP) on the same
-G-buffer on this host.
+Prints ONE JSON line.  `value` is the plain path (the form that meets the
+2% rel-L2 bound: its error against the FP64 oracle is ~1e-4, pinned by the
+parity tests).  `surrogate` reports the learned far field (levels >= 2
+replaced by the cone-traced features + MLP, packaged weights) on the same
+G-buffer: its frames/s, the ratio to the plain path, and its rel-L2 judged by
+the plain path on the GPU (FP64 reduction).  `roofline` is the dominant
+kernel's algorithmic bytes per launch over its mean launch time, measured
+here with HIP events recorded on the stream the kernels run on;
+`cpu_baseline` times the FP32 CPU restatement of the same maths (the oracle's
+FP32 flavour, OpenMP) on the same G-buffer on this host.
 """
 from __future__ import annotations
 
@@ -58,6 +63,26 @@ def b_alg(W, H, L, storage, rays):
     return W * H * 48 + 2 * texel_bytes_contract(storage) * sum(rays[1:L])
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "frame_1080p_hbm_traffic.md")
+KERNEL_NAMES = {"march": "k_march_all", "jfa": "k_jfa_pass<false>", "seed": "k_seed_init", "level0": "k_level0<"}
+
+
+def profiled_traffic(stage, W, H, L, storage):
+    """HBM bytes per launch of the stage's kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-kernel factor there),
+    for the workload that summary was taken on; None otherwise."""
+    if (W, H, L, storage) != (1920, 1080, 6, "f32") or stage not in KERNEL_NAMES or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    for line in open(TRAFFIC_PROFILE):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 5 and cells[0].strip("`").startswith("rc::" + KERNEL_NAMES[stage]):
+            try:
+                return (float(cells[3]) + float(cells[4])) * 1e6
+            except ValueError:
+                return None
+    return None
+
+
 def cpu_baseline(depth, emissive, L, seconds):
     """FP32 CPU restatement (OpenMP) on this host's cores: frames/s."""
     from oracle import rco
@@ -76,6 +101,58 @@ def cpu_baseline(depth, emissive, L, seconds):
                       f"(oracle FP32 flavour, OpenMP, {threads} threads)"}
 
 
+# MLP FLOP per level-1 probe (48 -> 64 -> 64 -> 64 -> 64, multiply-adds x 2)
+MLP_FLOP_PER_PROBE = 2 * (48 * 64 + 3 * 64 * 64)
+BF16_MFMA_PEAK_TFLOPS = 2500.0  # MI355X dense bf16
+
+
+def surrogate_leg(rc, torch, W, H, L, args, depth, emis, irr_plain, stream, fps_plain):
+    """The learned far field on the same G-buffer: frames/s (graph replay),
+    rel-L2 against the plain path's irradiance (GPU FP64 reduction), per-stage
+    times and the MFMA roofline of the level-1 kernel that runs the MLP."""
+    sur = rc.RadianceCascadeStage(W, H, levels=L, storage=args.storage)
+    sur.load_weights(args.weights)
+    sur.set_surrogate(True)
+    out = torch.empty_like(irr_plain)
+    for _ in range(args.warmup):
+        sur.run_device(depth, emis, out, stream)
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(args.steps):
+        sur.run_device(depth, emis, out, stream)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / args.steps
+    err = sur.error_norms(out, irr_plain, stream)
+    prof = rc.RadianceCascadeStage(W, H, levels=L, storage=args.storage, profile=True)
+    prof.load_weights(args.weights)
+    prof.set_surrogate(True)
+    for _ in range(5):
+        prof.run_device(depth, emis, out, stream)
+    prof.reset_stats()
+    for _ in range(50):
+        prof.run_device(depth, emis, out, stream)
+    torch.cuda.synchronize()
+    st = prof.stats()
+    per_frame = {k: st.stage_ms[k] / st.runs for k in st.stage_ms}
+    px, py = rc.probe_grid(W, H, 1)
+    flop = MLP_FLOP_PER_PROBE * px * py
+    mlp_ms = per_frame.get("surrogate", float("nan"))
+    tflops = flop / (mlp_ms * 1e-3) / 1e12
+    sur.close()
+    prof.close()
+    return {"fps": 1000.0 / ms, "ms_per_step": ms, "ratio_vs_plain": (1000.0 / ms) / fps_plain,
+            "rel_l2_vs_plain": err["rel_l2"], "max_abs_vs_plain": err["max_abs"],
+            "meets_2pct": bool(err["rel_l2"] <= 0.02),
+            "fallback": "off (threshold +inf)",
+            "stage_ms_per_frame": {k: round(v, 5) for k, v in per_frame.items()},
+            "mlp_roofline": {"bound": "mfma", "kernel": "surrogate (level-1 march + MLP + merge)",
+                             "flop_per_launch": flop, "achieved": tflops, "peak": BF16_MFMA_PEAK_TFLOPS,
+                             "unit": "TFLOP/s", "frac": tflops / BF16_MFMA_PEAK_TFLOPS}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,7 +168,22 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--weights", default=None, help="surrogate weights (default: the packaged network)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy a 256 MiB device buffer 10 times (the reference kernel for scaling the "
+                         "FETCH_SIZE / WRITE_SIZE counters of rocprofv3 to bytes)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = 64 << 20
+        x = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(10):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 10}), flush=True)
+        return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -155,6 +247,10 @@ def main():
     achieved = alg[dominant] / (dom_ms * 1e-3) / 1e9
     frame_alg = b_alg(W, H, L, args.storage, rays)
 
+    surrogate = None
+    if not args.no_surrogate and L >= 3:
+        surrogate = surrogate_leg(rc, torch, W, H, L, args, depth, emis, irr, stream, fps)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(depth.cpu().numpy(), emis.cpu().numpy(), L, args.cpu_seconds)
@@ -179,11 +275,15 @@ def main():
                    "tiling": {str(k): f"{v[0][0]}x{v[0][1]}x{v[0][2]} r{v[1]}" for k, v in tiling.items()},
                    "march": {"split": split, "speculation": spec, "rays_per_lane": rpl}, "jfa_fuse": stage.jfa_fuse()},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": profiled_traffic(dominant, W, H, L, args.storage),
+                     "traffic_source": "profiles/frame_1080p_hbm_traffic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                                       "copy-calibrated)",
                      "alg_bytes_per_launch": alg[dominant], "mean_launch_ms": dom_ms},
         "frame_roofline": {"alg_bytes": frame_alg, "achieved": frame_alg / (ms_per_step * 1e-3) / 1e9,
                            "frac": frame_alg / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "stage_ms_per_frame": {k: round(v, 5) for k, v in per_frame_ms.items()},
+        "surrogate": surrogate,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
