@@ -11,7 +11,9 @@ Prints ONE JSON line.  `value` is the plain path (the form that meets the
 parity tests).  `surrogate` reports the learned far field (levels >= 2
 replaced by the cone-traced features + MLP, packaged weights) on the same
 G-buffer: its frames/s, the ratio to the plain path, and its rel-L2 judged by
-the plain path on the GPU (FP64 reduction).  `roofline` is the dominant
+the plain path on the GPU (FP64 reduction).  `batch` reports the
+multi-view workload (64 seeded views of uneven sizes, one graph launch) in
+aggregate megapixels/s.  `roofline` is the dominant
 kernel's algorithmic bytes per launch over its mean launch time, measured
 here with HIP events recorded on the stream the kernels run on;
 `cpu_baseline` times the FP32 CPU restatement of the same maths (the oracle's
@@ -153,6 +155,41 @@ def surrogate_leg(rc, torch, W, H, L, args, depth, emis, irr_plain, stream, fps_
                              "unit": "TFLOP/s", "frac": tflops / BF16_MFMA_PEAK_TFLOPS}}
 
 
+def batch_leg(rc, torch, args, stream):
+    """The multi-view workload: 64 seeded views (seeds 1000..1063) of uneven
+    sizes (rc.batch_views_spec), plain cascades with the coverage rule's level
+    count, run as one graph launch (8 captured streams), tiling autotuned per
+    distinct view size.  Aggregate megapixels of irradiance per second."""
+    dev = torch.device("cuda:0")
+    spec = rc.batch_views_spec(1000, 64)
+    views = []
+    for seed, W, H, nb, ne in spec:
+        d = torch.empty((H, W), device=dev)
+        e = torch.empty((3, H, W), device=dev)
+        rc.gbuffer_raster(rc.scene_generate(seed, nb, ne), W, H, d, None, e)
+        views.append((d, e, torch.empty((3, H, W), device=dev)))
+    b = rc.RadianceCascadeBatch(views, storage=args.storage, streams=8)
+    b.autotune(iters=1, stream=stream)
+    for _ in range(3):
+        b.run(stream)
+    torch.cuda.synchronize()
+    n = 10
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(n):
+        b.run(stream)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / n
+    mp = sum(W * H for _, W, H, _, _ in spec) / 1e6
+    levels = [b.levels(i) for i in range(len(spec))]
+    b.close()
+    return {"views": len(spec), "megapixels": mp, "ms_per_batch": ms, "mpix_per_s": mp / (ms * 1e-3),
+            "views_per_s": len(spec) / (ms * 1e-3), "levels": {str(L): levels.count(L) for L in sorted(set(levels))},
+            "streams": 8, "launches": "one graph launch per batch"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -169,6 +206,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--weights", default=None, help="surrogate weights (default: the packaged network)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 256 MiB device buffer 10 times (the reference kernel for scaling the "
@@ -251,6 +289,8 @@ def main():
     if not args.no_surrogate and L >= 3:
         surrogate = surrogate_leg(rc, torch, W, H, L, args, depth, emis, irr, stream, fps)
 
+    batch = None if args.no_batch else batch_leg(rc, torch, args, stream)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(depth.cpu().numpy(), emis.cpu().numpy(), L, args.cpu_seconds)
@@ -284,6 +324,7 @@ def main():
                            "frac": frame_alg / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "stage_ms_per_frame": {k: round(v, 5) for k, v in per_frame_ms.items()},
         "surrogate": surrogate,
+        "batch": batch,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -266,6 +266,32 @@ rc_status rc_debug_surrogate(rc_ctx *ctx, float *cone_dev, float *level1_dev, fl
  * (mean E*[emitter] rgb, mean [emitter]) and mo [h][w] (mean [occluder]). */
 rc_status rc_debug_mip(rc_ctx *ctx, uint32_t level, float *emc_dev, float *mo_dev, void *stream);
 
+/* ---- batches of views (one graph launch) ------------------------------------
+ * Independent views of uneven sizes (e.g. the sub-views of a renderer's
+ * frame, or many cameras) run as ONE hipGraph launch.  Each view's frame is
+ * captured once; the views are dealt round-robin to `streams` captured streams
+ * so that small views run concurrently.  rc_batch_autotune tunes each distinct
+ * (width, height, levels) once, on its first view, and gives the choice to the
+ * views of that size.  The view buffers are the caller's and are bound at
+ * create time (the graph keeps their addresses).  Plain cascades only. */
+typedef struct {
+    uint32_t width, height;
+    uint32_t levels;                 /* 0 = coverage rule */
+    uint32_t reserved;
+    const float *depth;              /* device [H][W] */
+    const float *emissive;           /* device [3][H][W] */
+    float *irradiance;               /* device [3][H][W] */
+} rc_view;
+typedef struct rc_batch rc_batch;
+rc_status rc_batch_create(const rc_view *views, uint32_t n_views, uint32_t storage, uint32_t streams, int device,
+                          rc_batch **out);
+rc_status rc_batch_autotune(rc_batch *batch, uint32_t iters, void *stream);
+rc_status rc_batch_run(rc_batch *batch, void *stream);
+/* levels of view i after the coverage rule */
+rc_status rc_batch_view_levels(const rc_batch *batch, uint32_t i, uint32_t *levels);
+const char *rc_batch_last_error(const rc_batch *batch);
+void rc_batch_destroy(rc_batch *batch);
+
 /* ---- cascade geometry (integer, identical to the kernels' indexing) ------- */
 rc_status rc_probe_grid(uint32_t width, uint32_t height, uint32_t level, uint32_t *px, uint32_t *py);
 uint64_t rc_level_rays(uint32_t width, uint32_t height, uint32_t level);

@@ -80,6 +80,12 @@ class TrainReport(ctypes.Structure):
                 ("reserved", ctypes.c_uint32)]
 
 
+class View(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("levels", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32), ("depth", ctypes.c_void_p), ("emissive", ctypes.c_void_p),
+                ("irradiance", ctypes.c_void_p)]
+
+
 class Box(ctypes.Structure):
     _fields_ = [("lo", ctypes.c_double * 3), ("hi", ctypes.c_double * 3),
                 ("albedo", ctypes.c_double * 3), ("emit", ctypes.c_double * 3)]
@@ -130,6 +136,12 @@ SIGNATURES = {
     "rc_surrogate_train": (_i32, [_P, ctypes.POINTER(TrainConfig), ctypes.POINTER(TrainReport)]),
     "rc_debug_surrogate": (_i32, [_P, _P, _P, _P, _P]),
     "rc_debug_mip": (_i32, [_P, _u32, _P, _P, _P]),
+    "rc_batch_create": (_i32, [ctypes.POINTER(View), _u32, _u32, _u32, _i32, ctypes.POINTER(_P)]),
+    "rc_batch_autotune": (_i32, [_P, _u32, _P]),
+    "rc_batch_run": (_i32, [_P, _P]),
+    "rc_batch_view_levels": (_i32, [_P, _u32, ctypes.POINTER(_u32)]),
+    "rc_batch_last_error": (ctypes.c_char_p, [_P]),
+    "rc_batch_destroy": (None, [_P]),
     "rc_scene_generate": (_i32, [_u64, _u32, _u32, _P]),
     "rc_gbuffer_raster": (_i32, [_P, _u32, _u32, _u32, _P, _P, _P, _P]),
 }

@@ -1432,4 +1432,181 @@ rc_status rc_debug_mip(rc_ctx *c, uint32_t level, float *emc, float *mo, void *s
     return RC_OK;
 }
 
+// ---- batches of views --------------------------------------------------------
+struct rc_batch {
+    std::vector<rc_ctx *> ctx;
+    std::vector<rc_view> views;
+    int device = 0;
+    std::vector<hipStream_t> streams;
+    std::vector<hipEvent_t> joins;
+    hipEvent_t fork = nullptr;
+    hipStream_t cap = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::string err;
+};
+
+static rc_status bfail(rc_batch *b, rc_status s, const std::string &msg)
+{
+    if (b) b->err = msg;
+    g_err = msg;
+    return s;
+}
+
+#define BHIP_TRY(b, expr)                                                                            \
+    do {                                                                                            \
+        hipError_t e_ = (expr);                                                                     \
+        if (e_ != hipSuccess) return bfail((b), RC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+void rc_batch_destroy(rc_batch *b)
+{
+    if (!b) return;
+    hipSetDevice(b->device);
+    hipDeviceSynchronize();
+    if (b->gexec) hipGraphExecDestroy(b->gexec);
+    for (auto *c : b->ctx) rc_destroy(c);
+    for (auto st : b->streams) hipStreamDestroy(st);
+    for (auto e : b->joins) hipEventDestroy(e);
+    if (b->fork) hipEventDestroy(b->fork);
+    if (b->cap) hipStreamDestroy(b->cap);
+    delete b;
+}
+
+rc_status rc_batch_create(const rc_view *views, uint32_t n, uint32_t storage, uint32_t nstreams, int device,
+                          rc_batch **out)
+{
+    if (!views || !out) return bfail(nullptr, RC_ERR_NULL, "rc_batch_create: views or out is NULL");
+    *out = nullptr;
+    if (n == 0 || n > 4096) return bfail(nullptr, RC_ERR_INVALID, "rc_batch_create: 1 .. 4096 views");
+    if (nstreams == 0 || nstreams > 16) return bfail(nullptr, RC_ERR_INVALID, "rc_batch_create: 1 .. 16 streams");
+    for (uint32_t i = 0; i < n; ++i)
+        if (!views[i].depth || !views[i].emissive || !views[i].irradiance)
+            return bfail(nullptr, RC_ERR_NULL, "rc_batch_create: a view has a NULL buffer");
+    rc_batch *b = new (std::nothrow) rc_batch();
+    if (!b) return bfail(nullptr, RC_ERR_OOM, "rc_batch_create: host allocation failed");
+    b->device = device;
+    b->views.assign(views, views + n);
+    for (uint32_t i = 0; i < n; ++i) {
+        rc_config cfg{RC_ABI_VERSION, views[i].width, views[i].height, views[i].levels, storage, RC_FLAG_NO_GRAPH,
+                      device, 0};
+        rc_ctx *c = nullptr;
+        rc_status st = rc_create(&cfg, &c);
+        if (st != RC_OK) {
+            std::string msg = std::string("rc_batch_create: view ") + std::to_string(i) + ": " + rc_last_error(nullptr);
+            rc_batch_destroy(b);
+            return bfail(nullptr, st, msg);
+        }
+        b->ctx.push_back(c);
+    }
+    hipError_t e = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+    for (uint32_t k = 0; k < nstreams && e == hipSuccess; ++k) {
+        hipStream_t st = nullptr;
+        hipEvent_t ev = nullptr;
+        e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+        if (e == hipSuccess) b->streams.push_back(st);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e == hipSuccess) b->joins.push_back(ev);
+    }
+    if (e != hipSuccess) {
+        rc_batch_destroy(b);
+        return bfail(nullptr, RC_ERR_HIP, std::string("rc_batch_create: ") + hipGetErrorString(e));
+    }
+    *out = b;
+    return RC_OK;
+}
+
+rc_status rc_batch_view_levels(const rc_batch *b, uint32_t i, uint32_t *levels)
+{
+    if (!b || !levels) return bfail(nullptr, RC_ERR_NULL, "rc_batch_view_levels: NULL argument");
+    if (i >= b->ctx.size()) return bfail(const_cast<rc_batch *>(b), RC_ERR_INVALID, "rc_batch_view_levels: no such view");
+    *levels = (uint32_t)b->ctx[i]->L;
+    return RC_OK;
+}
+
+rc_status rc_batch_autotune(rc_batch *b, uint32_t iters, void *stream)
+{
+    if (!b) return bfail(nullptr, RC_ERR_NULL, "rc_batch_autotune: batch is NULL");
+    BHIP_TRY(b, hipSetDevice(b->device));
+    std::vector<int> leader(b->ctx.size(), -1);
+    for (size_t i = 0; i < b->ctx.size(); ++i) {
+        for (size_t j = 0; j < i; ++j)
+            if (leader[j] == (int)j && b->ctx[j]->W == b->ctx[i]->W && b->ctx[j]->H == b->ctx[i]->H &&
+                b->ctx[j]->L == b->ctx[i]->L) {
+                leader[i] = (int)j;
+                break;
+            }
+        if (leader[i] >= 0) continue;
+        leader[i] = (int)i;
+        rc_gbuffer g{b->views[i].depth, nullptr, b->views[i].emissive};
+        rc_status st = rc_autotune(b->ctx[i], &g, b->views[i].irradiance, iters, stream);
+        if (st != RC_OK) return bfail(b, st, std::string("rc_batch_autotune: ") + b->ctx[i]->err);
+    }
+    for (size_t i = 0; i < b->ctx.size(); ++i) {
+        const rc_ctx *l = b->ctx[leader[i]];
+        rc_ctx *c = b->ctx[i];
+        for (int k = 0; k < RC_MAX_LEVELS; ++k) {
+            c->tile[k] = l->tile[k];
+            c->rpt[k] = l->rpt[k];
+        }
+        c->jfa_fuse = l->jfa_fuse;
+        c->march_split = l->march_split;
+        c->spec = l->spec;
+        c->march_m = l->march_m;
+    }
+    if (b->gexec) hipGraphExecDestroy(b->gexec);
+    b->gexec = nullptr;
+    return RC_OK;
+}
+
+// Capture: fork the capture stream into the worker streams, each view's frame
+// on stream (view mod streams), join them back.
+static rc_status batch_capture(rc_batch *b)
+{
+    BHIP_TRY(b, hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+    rc_status st = RC_OK;
+    hipError_t e = hipEventRecord(b->fork, b->cap);
+    for (size_t k = 0; k < b->streams.size() && e == hipSuccess; ++k) e = hipStreamWaitEvent(b->streams[k], b->fork, 0);
+    for (size_t i = 0; i < b->ctx.size() && e == hipSuccess && st == RC_OK; ++i) {
+        const rc_view &v = b->views[i];
+        st = run_frame(b->ctx[i], v.depth, v.emissive, v.irradiance, b->streams[i % b->streams.size()], true);
+    }
+    for (size_t k = 0; k < b->streams.size() && e == hipSuccess; ++k) {
+        e = hipEventRecord(b->joins[k], b->streams[k]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(b->cap, b->joins[k], 0);
+    }
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(b->cap, &g);
+    if (st != RC_OK || e != hipSuccess || e2 != hipSuccess) {
+        if (g) hipGraphDestroy(g);
+        if (st != RC_OK) return bfail(b, st, "rc_batch_run: capture of a view failed");
+        return bfail(b, RC_ERR_HIP, std::string("rc_batch_run: capture: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        b->gexec = nullptr;
+        return bfail(b, RC_ERR_HIP, std::string("rc_batch_run: hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    return RC_OK;
+}
+
+rc_status rc_batch_run(rc_batch *b, void *stream)
+{
+    if (!b) return bfail(nullptr, RC_ERR_NULL, "rc_batch_run: batch is NULL");
+    BHIP_TRY(b, hipSetDevice(b->device));
+    if (!b->gexec) {
+        rc_status st = batch_capture(b);
+        if (st != RC_OK) return st;
+    }
+    BHIP_TRY(b, hipGraphLaunch(b->gexec, (hipStream_t)stream));
+    for (auto *c : b->ctx) {
+        c->stats.runs += 1;
+        c->ran = true;
+    }
+    return RC_OK;
+}
+
+const char *rc_batch_last_error(const rc_batch *b) { return b ? b->err.c_str() : g_err.c_str(); }
+
 }  // extern "C"

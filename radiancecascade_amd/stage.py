@@ -364,6 +364,89 @@ class RadianceCascadeStage:
             pass
 
 
+class RadianceCascadeBatch:
+    """Many views of uneven sizes as one graph launch (rc_batch_*).
+
+    views: sequence of (depth [H,W], emissive [3,H,W], irradiance [3,H,W])
+    device tensors; levels: per-view cascade counts (0 = coverage rule)."""
+
+    def __init__(self, views, levels=None, storage: str = "f32", streams: int = 8, device: int = 0):
+        lib = _abi.load()
+        n = len(views)
+        arr = (_abi.View * n)()
+        if n == 0:
+            raise ValueError("a batch needs at least one view")
+        for i, (d, e, o) in enumerate(views):
+            if d is None or e is None or o is None:
+                raise ValueError(f"view {i}: depth, emissive and irradiance buffers are required")
+            H, W = d.shape[-2], d.shape[-1]
+            arr[i] = _abi.View(W, H, 0 if levels is None else int(levels[i]), 0, _dev_ptr(d, W * H * 4, "depth"),
+                               _dev_ptr(e, 3 * W * H * 4, "emissive"), _dev_ptr(o, 3 * W * H * 4, "irradiance"))
+        h = ctypes.c_void_p()
+        st = lib.rc_batch_create(arr, n, _abi.STORAGE[storage], streams, device, ctypes.byref(h))
+        if st != _abi.OK:
+            raise RcError(st, (lib.rc_batch_last_error(None) or b"").decode())
+        self._lib, self._h, self._views = lib, h, list(views)
+        self.sizes = [(int(a.width), int(a.height)) for a in arr]
+
+    def _check(self, st):
+        if st != _abi.OK:
+            raise RcError(st, (self._lib.rc_batch_last_error(self._h) or b"").decode())
+
+    def autotune(self, iters: int = 2, stream=None) -> None:
+        self._check(self._lib.rc_batch_autotune(self._h, iters, _stream_handle(stream)))
+
+    def run(self, stream=None) -> None:
+        self._check(self._lib.rc_batch_run(self._h, _stream_handle(stream)))
+
+    def levels(self, i: int) -> int:
+        v = ctypes.c_uint32()
+        self._check(self._lib.rc_batch_view_levels(self._h, i, ctypes.byref(v)))
+        return int(v.value)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.rc_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+BATCH_SIZES = (256, 512, 768, 1024, 1536, 2048)
+_M64 = (1 << 64) - 1
+
+
+def batch_views_spec(seed0: int = 1000, n: int = 64, n_emissive: int = 4):
+    """The multi-view workload: view j uses scene seed seed0 + j; its width,
+    height (each from BATCH_SIZES) and box count (16..64) are the first three
+    draws u of SplitMix64(seed ^ 0x5EED000000000000), index floor(6u),
+    floor(6u), 16 + floor(49u).  Returns [(seed, W, H, n_boxes, n_emissive)]."""
+    out = []
+    for j in range(n):
+        seed = seed0 + j
+        state = (seed ^ 0x5EED000000000000) & _M64
+        u = []
+        for _ in range(3):
+            state = (state + 0x9E3779B97F4A7C15) & _M64
+            z = state
+            z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+            z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+            z ^= z >> 31
+            u.append((z >> 11) * 2.0 ** -53)
+        out.append((seed, BATCH_SIZES[int(6 * u[0])], BATCH_SIZES[int(6 * u[1])], 16 + int(49 * u[2]), n_emissive))
+    return out
+
+
 def packaged_weights() -> str:
     """Path of the network trained on the GPU and shipped with the package."""
     return os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "surrogate.bin")
