@@ -155,6 +155,24 @@ def surrogate_leg(rc, torch, W, H, L, args, depth, emis, irr_plain, stream, fps_
                              "unit": "TFLOP/s", "frac": tflops / BF16_MFMA_PEAK_TFLOPS}}
 
 
+def host_staging_leg(rc, W, H, L, storage, depth, emis, frames=20):
+    """rc_update_host + rc_render_host per frame (host G-buffer in, host
+    irradiance out: the PCIe-inclusive rate of the staging boundary)."""
+    d = depth.cpu().numpy()
+    e = emis.cpu().numpy()
+    st = rc.RadianceCascadeStage(W, H, levels=L, storage=storage)
+    st.update(d, e)
+    st.render()
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        st.update(d, e)
+        st.render()
+    el = time.perf_counter() - t0
+    st.close()
+    return {"fps": frames / el, "ms_per_frame": 1e3 * el / frames,
+            "bytes_per_frame": 4 * W * H * (1 + 3 + 3), "note": "host G-buffer upload + frame + irradiance download"}
+
+
 def batch_leg(rc, torch, args, stream):
     """The multi-view workload: 64 seeded views (seeds 1000..1063) of uneven
     sizes (rc.batch_views_spec), plain cascades with the coverage rule's level
@@ -290,6 +308,7 @@ def main():
         surrogate = surrogate_leg(rc, torch, W, H, L, args, depth, emis, irr, stream, fps)
 
     batch = None if args.no_batch else batch_leg(rc, torch, args, stream)
+    staging = host_staging_leg(rc, W, H, L, args.storage, depth, emis)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -325,6 +344,7 @@ def main():
         "stage_ms_per_frame": {k: round(v, 5) for k, v in per_frame_ms.items()},
         "surrogate": surrogate,
         "batch": batch,
+        "host_staging": staging,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
