@@ -24,6 +24,7 @@ NEAR_R = 10                      # exact distances up to the level-1 interval en
 N_DIRS = 16                      # level-1 directions = cones per level-2 probe
 CONE_T0 = 6.0
 CONE_GROW = 0.75
+CONE_SPLIT = 60.0                # near segment from level-2 probes, far from level-3
 T_MIN = 1e-4
 
 
@@ -143,16 +144,16 @@ def _sample(level, x, y, m):
     return e, o, inside
 
 
-def cone_trace(levels, W: int, H: int, schedule=None) -> np.ndarray:
-    """[QY][QX][16][4]: far-field radiance rgb and transmittance per level-2
-    probe and level-1 direction (the fp32 direction table of rco).  A cone
-    stops once T < 1e-4 or once its footprint has left the image."""
-    schedule = schedule if schedule is not None else cone_schedule(W, H)
-    QX, QY = rco.probe_grid(W, H, 2)
+def cone_trace_grid(levels, W: int, H: int, level: int, schedule) -> np.ndarray:
+    """[QY][QX][16][4] cones from the probes of `level` (spacing 2^(level+1))
+    along the given schedule: radiance rgb and transmittance.  A cone stops
+    once T < 1e-4 or once its footprint has left the image."""
+    QX, QY = rco.probe_grid(W, H, level)
+    sp = float(2 << level)
     dirs = rco.dir_table_f32(1).astype(np.float64)
     qy, qx = np.mgrid[0:QY, 0:QX]
-    cx = (qx + 0.5) * 8.0
-    cy = (qy + 0.5) * 8.0
+    cx = (qx + 0.5) * sp
+    cy = (qy + 0.5) * sp
     out = np.zeros((QY, QX, N_DIRS, 4))
     for k in range(N_DIRS):
         dx, dy = dirs[k]
@@ -169,6 +170,24 @@ def cone_trace(levels, W: int, H: int, schedule=None) -> np.ndarray:
             live = live & (T >= T_MIN)
         out[:, :, k, :3] = L
         out[:, :, k, 3] = T
+    return out
+
+
+def cone_trace(levels, W: int, H: int, schedule=None) -> np.ndarray:
+    """[QY][QX][16][4] at the level-2 probes: the near segment (t < 60) traced
+    from them, then the far segment traced from the level-3 probes and
+    appended with the merge's bilinear weights: L += T * F.rgb, T *= F.T."""
+    schedule = schedule if schedule is not None else cone_schedule(W, H)
+    near = [s for s in schedule if s[0] < CONE_SPLIT]
+    far = [s for s in schedule if s[0] >= CONE_SPLIT]
+    c2 = cone_trace_grid(levels, W, H, 2, near)
+    c3 = cone_trace_grid(levels, W, H, 3, far)
+    QX, QY = rco.probe_grid(W, H, 2)
+    F = upper_bilinear(c3, QX, QY)
+    FT = upper_bilinear(np.repeat(c3[..., 3:4], 3, -1), QX, QY)[..., 0]
+    out = c2.copy()
+    out[..., :3] = c2[..., :3] + c2[..., 3:4] * F
+    out[..., 3] = c2[..., 3] * FT
     return out
 
 

@@ -63,8 +63,8 @@ struct rc_ctx {
     int mlp_cap_rows = 0;
     bool surr_alloc = false;
     MipArgs mip{};
-    ConeSchedule sch{};
-    float4 *cone = nullptr;
+    ConeSchedule sch{}, sch_far{};
+    float4 *cone = nullptr, *cone_far = nullptr;
     uint64_t *occ_bits = nullptr;  // occupancy, one bit per pixel, rows of ceil(W/64) words
     void *surr_a1 = nullptr;
     float *tile_err = nullptr;
@@ -151,6 +151,8 @@ static void release(rc_ctx *c)
     if (c->occ_bits) hipFree(c->occ_bits);
     c->occ_bits = nullptr;
     if (c->cone) hipFree(c->cone);
+    if (c->cone_far) hipFree(c->cone_far);
+    c->cone_far = nullptr;
     if (c->surr_a1) hipFree(c->surr_a1);
     if (c->tile_err) hipFree(c->tile_err);
     if (c->flag_count) hipFree(c->flag_count);
@@ -400,7 +402,7 @@ static rc_status ensure_surrogate(rc_ctx *c)
     if (c->L < 3) return fail(c, RC_ERR_INVALID, "learned far field needs levels >= 3");
     const int top = cone_top_level(c->W, c->H);
     mip_layout(c->W, c->H, top, c->mip);
-    cone_schedule(c->W, c->H, c->mip, c->sch);
+    cone_schedule(c->W, c->H, c->mip, c->sch, c->sch_far);
     const size_t cells = mip_cells(c->mip);
     if (cells * sizeof(float4) > 0x7FFFFFFFull) return fail(c, RC_ERR_INVALID, "learned far field: image too large");
     HIP_TRY(c, hipMalloc(&c->mip.tex, cells * sizeof(float4)));
@@ -408,6 +410,7 @@ static rc_status ensure_surrogate(rc_ctx *c)
     HIP_TRY(c, hipMalloc(&c->occ_bits, (size_t)c->H * ((c->W + 63) / 64) * sizeof(uint64_t)));
     const int QX = grid_dim(c->W, 2), QY = grid_dim(c->H, 2);
     HIP_TRY(c, hipMalloc(&c->cone, (size_t)QX * QY * 16 * sizeof(float4)));
+    HIP_TRY(c, hipMalloc(&c->cone_far, (size_t)grid_dim(c->W, 3) * grid_dim(c->H, 3) * 16 * sizeof(float4)));
     const int PX = grid_dim(c->W, 1), PY = grid_dim(c->H, 1);
     HIP_TRY(c, hipMalloc(&c->surr_a1, (size_t)PX * PY * 4 * texel_bytes(c->storage)));
     c->surr_tiles_x = (PX + 7) / 8;
@@ -423,18 +426,31 @@ static rc_status ensure_surrogate(rc_ctx *c)
     return RC_OK;
 }
 
-static ConeArgs cone_args(const rc_ctx *c)
+// The two cone launches: the far segment on the level-3 grid, then the near
+// segment on the level-2 grid, which appends the far one.
+static rc_status launch_cones(rc_ctx *c, hipStream_t s)
 {
     ConeArgs a{};
     a.mip = c->mip;
     a.mip_bytes = (uint32_t)(mip_cells(c->mip) * sizeof(float4));
-    a.sch = c->sch;
     a.dirs = c->dirs + c->dir_off[1];
+    a.sch = c->sch_far;
+    a.cone = c->cone_far;
+    a.QX = grid_dim(c->W, 3);
+    a.QY = grid_dim(c->H, 3);
+    a.spacing = (float)probe_spacing(3);
+    a.far = nullptr;
+    HIP_TRY(c, launch_cone(a, s));
+    a.sch = c->sch;
     a.cone = c->cone;
     a.QX = grid_dim(c->W, 2);
     a.QY = grid_dim(c->H, 2);
     a.spacing = (float)probe_spacing(2);
-    return a;
+    a.far = c->cone_far;
+    a.FX = grid_dim(c->W, 3);
+    a.FY = grid_dim(c->H, 3);
+    HIP_TRY(c, launch_cone(a, s));
+    return RC_OK;
 }
 
 // occupancy bits + mip 1..2 -> near distance field -> mip 3..top -> cones ->
@@ -450,8 +466,8 @@ static rc_status run_frame_surrogate(rc_ctx *c, const float *depth, const float 
     HIP_TRY(c, launch_seed_surr(depth, emissive, c->occ_bits, c->mip, c->W, c->H, s));
     tm.end(RC_STAGE_SEED);
     // The near distance field needs only the occupancy bits: it runs on the
-    // side stream while the mip pyramid and the cones run here (a profiled
-    // frame runs it in line so each stage has its own time).
+    // side stream beside the mip and cone chain, joined before level 1 (a
+    // profiled frame runs it in line so each stage has its own time).
     const bool fork = !prof;
     if (fork) {
         HIP_TRY(c, hipEventRecord(c->ev_fork, s));
@@ -467,7 +483,10 @@ static rc_status run_frame_surrogate(rc_ctx *c, const float *depth, const float 
     HIP_TRY(c, launch_mip_rest(c->mip, c->flag_count, s));
     tm.end(RC_STAGE_MIP);
     tm.begin(RC_STAGE_CONE);
-    HIP_TRY(c, launch_cone(cone_args(c), s));
+    {
+        rc_status st = launch_cones(c, s);
+        if (st != RC_OK) return st;
+    }
     tm.end(RC_STAGE_CONE);
     if (fork) HIP_TRY(c, hipStreamWaitEvent(s, c->ev_join, 0));
     SurrLevel1Args a{};
@@ -610,7 +629,8 @@ static rc_status gen_samples(rc_ctx *c, const float *depth, const float *emissiv
     if (st != RC_OK) return st;
     HIP_TRY(c, launch_seed_surr(depth, emissive, c->occ_bits, c->mip, c->W, c->H, s));
     HIP_TRY(c, launch_mip_rest(c->mip, c->flag_count, s));
-    HIP_TRY(c, launch_cone(cone_args(c), s));
+    st = launch_cones(c, s);
+    if (st != RC_OK) return st;
     SurrSampleArgs a{};
     a.cone = c->cone;
     a.upper = c->level[2];

@@ -132,14 +132,22 @@ struct ConeSchedule {
     int pad[7];
     ConeStep s[kConeMaxSteps];
 };
+// Cones are traced in two segments: the near one (t < kConeSplit) from the
+// level-2 probes, the far one (t >= kConeSplit) from the level-3 probes (4x
+// fewer cones; at that distance a cone is wider than the probe spacing); a
+// level-2 cone ends with L += T * F.rgb, T *= F.T, F the far cones
+// interpolated at its probe with the merge's bilinear weights.
+constexpr float kConeSplit = 60.0f;
 struct ConeArgs {
     MipArgs mip;
     uint32_t mip_bytes;     // size of the whole pyramid (range of the buffer loads)
     ConeSchedule sch;
     const float2 *dirs;     // the 16 level-1 directions
-    float4 *cone;           // [QY][QX][16]: far-field radiance rgb, transmittance
-    int QX, QY;             // level-2 probe grid
-    float spacing;          // level-2 probe spacing (8)
+    float4 *cone;           // [QY][QX][16]: radiance rgb, transmittance
+    int QX, QY;             // probe grid of this launch (level 2, or level 3 for the far segment)
+    float spacing;          // its probe spacing (8 or 16)
+    const float4 *far;      // far-segment cones [FY][FX][16] to append, or nullptr
+    int FX, FY;
 };
 struct SurrLevel1Args {
     const float *sdf;
@@ -163,7 +171,8 @@ struct SurrSampleArgs {
 
 void mip_layout(int W, int H, int top_level, MipArgs &m);   // fills w, h, off, levels
 size_t mip_cells(const MipArgs &m);
-void cone_schedule(int W, int H, const MipArgs &m, ConeSchedule &s);
+// near: steps with t < kConeSplit; far: the rest of the same sequence
+void cone_schedule(int W, int H, const MipArgs &m, ConeSchedule &near, ConeSchedule &far);
 int cone_top_level(int W, int H);
 hipError_t launch_seed_surr(const float *depth, const float *emissive, uint64_t *bits, const MipArgs &m, int W, int H,
                             hipStream_t s);

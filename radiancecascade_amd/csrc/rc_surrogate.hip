@@ -16,7 +16,8 @@
 // answer.
 //
 // Cone model (per level-2 probe c and level-1 direction k, 16 cones of
-// half-angle pi/16): samples at the schedule's distances t_n, bilinear in the
+// half-angle pi/16; beyond kConeSplit from the level-3 probes, interpolated
+// as the merge does): samples at the schedule's distances t_n, bilinear in the
 // mip level m_n whose cells match the cone width; occluder opacity
 // a_o = 1 - (1 - min(mo, 0.999))^len_n (a thin silhouette blocks the whole
 // cone cross-section it crosses), total opacity a = min(me + a_o, 1);
@@ -249,13 +250,45 @@ __global__ __launch_bounds__(kBlock) void k_mip_mid(MipArgs m, int tiles_x)
 }
 
 // Levels 7..top from level 6, one workgroup; also clears the fallback counter
-// of the frame (a kernel store, not a memset node, so graph replays stay exact).
+// of the frame (a kernel store, not a memset node, so graph replays stay
+// exact).  When level 6 fits in LDS (up to 4096 cells: images up to
+// 4096x4096) it is read once and the levels are reduced in LDS.
+constexpr int kTopLds = 4096;
+
 __global__ __launch_bounds__(kBlock) void k_mip_top(MipArgs m, int *zero_me)
 {
+    __shared__ float4 sa[kTopLds], sb[kTopLds / 4 + 160];  // ceil(w/2)ceil(h/2) <= (wh + w + h + 1) / 4
     if (zero_me && threadIdx.x == 0) *zero_me = 0;
+    if (m.levels <= kMidLast) return;
+    const float4 zero = make_float4(0.f, 0.f, 0.f, pack_cov(0.f, 0.f));
+    const int w6 = m.w[kMidLast], h6 = m.h[kMidLast];
+    if (w6 * h6 <= kTopLds) {
+        for (int i = threadIdx.x; i < w6 * h6; i += kBlock) sa[i] = m.tex[mip_at(m, kMidLast, i % w6, i / w6)];
+        __syncthreads();
+        float4 *src = sa, *dst = sb;
+        for (int l = kMidLast + 1; l <= m.levels; ++l) {
+            const int w = m.w[l], h = m.h[l], cw = m.w[l - 1], ch = m.h[l - 1];
+            for (int i = threadIdx.x; i < w * h; i += kBlock) {
+                const int x = i % w, y = i / w;
+                float4 v[4];
+#pragma unroll
+                for (int d = 0; d < 4; ++d) {
+                    const int cx = 2 * x + (d & 1), cy = 2 * y + (d >> 1);
+                    v[d] = (cx < cw && cy < ch) ? src[cy * cw + cx] : zero;
+                }
+                const float4 t = mean4(v[0], v[1], v[2], v[3]);
+                dst[i] = t;
+                m.tex[mip_at(m, l, x, y)] = t;
+            }
+            __syncthreads();
+            float4 *tmp = src;
+            src = dst;
+            dst = tmp;
+        }
+        return;
+    }
     for (int l = kMidLast + 1; l <= m.levels; ++l) {
         const int w = m.w[l], h = m.h[l], cw = m.w[l - 1], ch = m.h[l - 1];
-        const float4 zero = make_float4(0.f, 0.f, 0.f, pack_cov(0.f, 0.f));
         for (int i = threadIdx.x; i < w * h; i += kBlock) {
             const int x = i % w, y = i / w;
             float4 v[4];
@@ -309,17 +342,19 @@ hipError_t launch_mip_unpack(const MipArgs &m, int level, float *emc, float *mo,
 // px from it and its far field starts at 10 px), cone diameter
 // dia = max(2 t tan(pi/16), 1), mip level floor(log2 dia) clamped to
 // [1, top], next t = t + 0.75 dia, up to the image diagonal + 2.
-void cone_schedule(int W, int H, const MipArgs &mip, ConeSchedule &s)
+void cone_schedule(int W, int H, const MipArgs &mip, ConeSchedule &near, ConeSchedule &far)
 {
     const int top = mip.levels;
     const double tmax = sqrt((double)W * W + (double)H * H) + 2.0, tn = tan(M_PI / 16.0);
     double t = 6.0;
-    s.n = 0;
-    while (t < tmax && s.n < kConeMaxSteps) {
+    near.n = 0;
+    far.n = 0;
+    while (t < tmax && near.n + far.n < kConeMaxSteps) {
         const double dia = fmax(2.0 * t * tn, 1.0);
         int m = (int)floor(log2(dia));
         m = m < 1 ? 1 : (m > top ? top : m);
         const double step = 0.75 * dia;
+        ConeSchedule &s = t < (double)kConeSplit ? near : far;
         ConeStep &c = s.s[s.n++];
         c.t = (float)t;
         c.len = (float)fmin(step, (double)(1 << m));
@@ -398,6 +433,21 @@ __global__ __launch_bounds__(kBlock) void k_cone(ConeArgs a, int seg_x)
         b = fmaf(T, eb, b);
         T = T * (1.0f - al);
         if (T < 1e-4f) break;
+    }
+    if (a.far) {
+        // the far segment from the level-3 grid, bilinear with the merge's weights
+        const int bx = (qx - 1) >> 1, by = (qy - 1) >> 1;
+        const int fx0 = min(max(bx, 0), a.FX - 1), fx1 = min(max(bx + 1, 0), a.FX - 1);
+        const int fy0 = min(max(by, 0), a.FY - 1), fy1 = min(max(by + 1, 0), a.FY - 1);
+        const float4 f00 = a.far[((size_t)fy0 * a.FX + fx0) * 16 + k], f10 = a.far[((size_t)fy0 * a.FX + fx1) * 16 + k];
+        const float4 f01 = a.far[((size_t)fy1 * a.FX + fx0) * 16 + k], f11 = a.far[((size_t)fy1 * a.FX + fx1) * 16 + k];
+        const float3 f = bilerp4(f00, f10, f01, f11, qx, qy);
+        const float3 ft = bilerp4(make_float4(f00.w, f00.w, f00.w, 0.f), make_float4(f10.w, f10.w, f10.w, 0.f),
+                                  make_float4(f01.w, f01.w, f01.w, 0.f), make_float4(f11.w, f11.w, f11.w, 0.f), qx, qy);
+        r = fmaf(T, f.x, r);
+        g = fmaf(T, f.y, g);
+        b = fmaf(T, f.z, b);
+        T = T * ft.x;
     }
     a.cone[((size_t)qy * a.QX + qx) * 16 + k] = make_float4(r, g, b, T);
 }
@@ -488,10 +538,15 @@ __global__ __launch_bounds__(kBlock) void k_surr_level1(SurrLevel1Args a, MlpDev
 {
     using TT = Texel<S>;
     __shared__ float bias_s[4 * 64], mean_s[kMlpIn], istd_s[kMlpIn];
-    __shared__ __attribute__((aligned(16))) uint16_t xs[64 * kSStrideIn];
-    __shared__ __attribute__((aligned(16))) uint16_t ha[64 * kSStrideHid], hb[64 * kSStrideHid];
-    __shared__ float zs[64][kMlpErr + 2];
+    // xs (layer-1 input) and hb (layer-3 input) are dead when the output layer
+    // writes zs: one buffer holds all three (26.6 KB of LDS per workgroup).
+    __shared__ __attribute__((aligned(16))) uint16_t xs_hb[64 * kSStrideIn + 64 * kSStrideHid];
+    __shared__ __attribute__((aligned(16))) uint16_t ha[64 * kSStrideHid];
     __shared__ float red[kBlock / 64][2];
+    static_assert(64 * (kMlpErr + 2) * 4 <= (64 * kSStrideIn + 64 * kSStrideHid) * 2, "zs fits over xs + hb");
+    uint16_t *const xs = xs_hb;
+    uint16_t *const hb = xs_hb + 64 * kSStrideIn;
+    float(*const zs)[kMlpErr + 2] = reinterpret_cast<float(*)[kMlpErr + 2]>(xs_hb);
 
     for (int i = threadIdx.x; i < 4 * 64; i += kBlock) bias_s[i] = m.bias[i];
     for (int i = threadIdx.x; i < kMlpIn; i += kBlock) {
