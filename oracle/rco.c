@@ -274,23 +274,41 @@ static inline uint32_t seed_d2(uint32_t s, int x, int y)
     return (uint32_t)(dx * dx + dy * dy);
 }
 
-/* SDF in px: fp32 value (float)sqrt((double)d^2); +inf with no seed at all. */
+/* Distance the march reads: a conservative one-byte code of d = sqrt(d^2)
+ * (rc_common.hpp of the product states the same rule): floor(d) below 128 px,
+ * 128 + 16 floor((d - 128) / 16) below 2160 px, 2160 beyond and with no seed at
+ * all.  The value never exceeds d, and is 0 exactly when d is. */
+static uint32_t isqrt_u64(uint64_t n)
+{
+    uint64_t r = (uint64_t)sqrt((double)n);
+    while (r * r > n) --r;
+    while ((r + 1) * (r + 1) <= n) ++r;
+    return (uint32_t)r;
+}
+
+double rco_sdf_value_d2(uint64_t d2)
+{
+    const uint32_t d = isqrt_u64(d2);
+    if (d < 128) return (double)d;
+    if (d < 2160) return 128.0 + 16.0 * (double)((d - 128) / 16);
+    return 2160.0;
+}
+
+static double sdf_value_at(uint32_t s, int x, int y)
+{
+    return s == RCO_SEED_NONE ? 2160.0 : rco_sdf_value_d2(seed_d2(s, x, y));
+}
+
 void rco_sdf_f32(int W, int H, const uint32_t *seed, float *sdf)
 {
     for (int y = 0; y < H; ++y)
-        for (int x = 0; x < W; ++x) {
-            uint32_t s = seed[(size_t)y * W + x];
-            sdf[(size_t)y * W + x] = s == RCO_SEED_NONE ? INFINITY : (float)sqrt((double)seed_d2(s, x, y));
-        }
+        for (int x = 0; x < W; ++x) sdf[(size_t)y * W + x] = (float)sdf_value_at(seed[(size_t)y * W + x], x, y);
 }
 
 void rco_sdf_f64(int W, int H, const uint32_t *seed, double *sdf)
 {
     for (int y = 0; y < H; ++y)
-        for (int x = 0; x < W; ++x) {
-            uint32_t s = seed[(size_t)y * W + x];
-            sdf[(size_t)y * W + x] = s == RCO_SEED_NONE ? INFINITY : sqrt((double)seed_d2(s, x, y));
-        }
+        for (int x = 0; x < W; ++x) sdf[(size_t)y * W + x] = sdf_value_at(seed[(size_t)y * W + x], x, y);
 }
 
 /* Exact nearest seed under the same tie-break (checker for the jump flood). */

@@ -77,6 +77,7 @@ void     rco_seed_init(int W, int H, const float *depth, const float *emissive,
                        uint32_t *seed);
 void     rco_jfa_pass(int W, int H, int step, const uint32_t *in, uint32_t *out);
 void     rco_jfa(int W, int H, uint32_t *seed, uint32_t *tmp);
+double   rco_sdf_value_d2(uint64_t d2);
 void     rco_sdf_f32(int W, int H, const uint32_t *seed, float *sdf);
 void     rco_sdf_f64(int W, int H, const uint32_t *seed, double *sdf);
 void     rco_nearest_seed_bruteforce(int W, int H, const uint32_t *seed_init,

@@ -64,6 +64,7 @@ def lib():
             "rco_seed_init": (None, [i32, i32, P, P, P]),
             "rco_jfa_pass": (None, [i32, i32, i32, P, P]),
             "rco_jfa": (None, [i32, i32, P, P]),
+            "rco_sdf_value_d2": (ctypes.c_double, [u64]),
             "rco_sdf_f32": (None, [i32, i32, P, P]),
             "rco_sdf_f64": (None, [i32, i32, P, P]),
             "rco_nearest_seed_bruteforce": (None, [i32, i32, P, P]),
@@ -192,6 +193,14 @@ def nearest_seed_bruteforce(seed_init_field: np.ndarray) -> np.ndarray:
     out = np.empty_like(s)
     lib().rco_nearest_seed_bruteforce(W, H, _p(s), _p(out))
     return out
+
+
+SDF_FAR = 2160.0
+
+
+def sdf_value_d2(d2: int) -> float:
+    """Distance value the march reads for squared distance d2 (conservative code)."""
+    return lib().rco_sdf_value_d2(d2)
 
 
 def sdf_f32(seeds: np.ndarray) -> np.ndarray:

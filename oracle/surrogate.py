@@ -30,8 +30,9 @@ T_MIN = 1e-4
 
 # ---- near distance field (rc_surrogate.hip k_near_sdf) -------------------------
 def near_sdf(seed_init_field: np.ndarray) -> np.ndarray:
-    """float32 [H][W]: (float)sqrt(d^2) of the nearest occupied pixel where
-    d^2 <= 100, +inf elsewhere (brute force over the 21x21 window)."""
+    """float32 [H][W]: the distance value (floor of the distance, rco's
+    conservative code) of the nearest occupied pixel where d^2 <= 100, the far
+    value 2160 elsewhere (brute force over the 21x21 window)."""
     occ = seed_init_field != rco.SEED_NONE
     H, W = occ.shape
     best = np.full((H, W), NEAR_R * NEAR_R + 1, np.int64)
@@ -44,9 +45,9 @@ def near_sdf(seed_init_field: np.ndarray) -> np.ndarray:
                 continue
             hit = pad[NEAR_R + dy:NEAR_R + dy + H, NEAR_R + dx:NEAR_R + dx + W]
             best = np.where(hit & (d2 < best), d2, best)
-    out = np.full((H, W), np.inf, np.float32)
+    out = np.full((H, W), rco.SDF_FAR, np.float32)
     ok = best <= NEAR_R * NEAR_R
-    out[ok] = np.sqrt(best[ok].astype(np.float64)).astype(np.float32)
+    out[ok] = np.floor(np.sqrt(best[ok].astype(np.float64)) + 1e-9).astype(np.float32)
     return out
 
 

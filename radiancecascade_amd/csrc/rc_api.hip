@@ -24,7 +24,7 @@ struct rc_ctx {
     int W = 0, H = 0, L = 0;
     Storage storage = Storage::F32;
     uint32_t *seed[2] = {nullptr, nullptr};
-    float *sdf = nullptr;
+    uint8_t *sdf = nullptr;  // distance codes (rc_common.hpp)
     void *level[RC_MAX_LEVELS] = {};
     size_t level_texels[RC_MAX_LEVELS] = {};
     float *dbg_level0 = nullptr;
@@ -182,7 +182,7 @@ static rc_status allocate(rc_ctx *c, uint32_t W, uint32_t H, uint32_t L)
     const size_t n = (size_t)W * H;
     HIP_TRY(c, hipMalloc(&c->seed[0], n * sizeof(uint32_t)));
     HIP_TRY(c, hipMalloc(&c->seed[1], n * sizeof(uint32_t)));
-    HIP_TRY(c, hipMalloc(&c->sdf, sdf_elems((int)W, (int)H) * sizeof(float)));
+    HIP_TRY(c, hipMalloc(&c->sdf, sdf_elems((int)W, (int)H)));
     const size_t tb = texel_bytes(c->storage);
     for (int i = 1; i < (int)L; ++i) {
         const size_t texels = (size_t)grid_dim(W, i) * grid_dim(H, i) * (num_dirs(i) / 4);

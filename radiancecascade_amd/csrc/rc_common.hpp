@@ -33,18 +33,41 @@ __host__ __device__ inline int grid_dim(int pixels, int level)
 __host__ __device__ inline int64_t interval_t0(int level) { return (2 * ((1ll << (2 * level)) - 1)) / 3; }
 __host__ __device__ inline int64_t interval_t1(int level) { return (2 * ((1ll << (2 * level + 2)) - 1)) / 3; }
 
-// ---- distance-field layout --------------------------------------------------
-// The SDF is stored in 8x4-pixel tiles of 32 floats (one 128-byte cache
-// line), tiles row-major.  A march step moves about one pixel in any
-// direction, so consecutive samples of a ray -- and the samples of
-// neighbouring rays of a wave -- share lines far more often than in a
-// row-major layout, where every vertical step changes line (measured: the
-// split march is ~25 % faster with the tiles at 1080p).
-__host__ __device__ inline int sdf_tiles_x(int W) { return (W + 7) >> 3; }
-__host__ __device__ inline size_t sdf_elems(int W, int H) { return (size_t)sdf_tiles_x(W) * 8 * (size_t)(((H + 3) >> 2) * 4); }
+// ---- distance field: one byte per pixel, 16x8-pixel tiles ------------------
+// The value stored is a conservative code of the distance d = sqrt(d^2) (in
+// px) to the nearest occupied pixel:
+//   code = floor(d)                        d < 128
+//   code = 128 + floor((d - 128) / 16)     128 <= d < 2160
+//   code = 255                             d >= 2160, or no occupied pixel
+// and the march reads value(code) = code < 128 ? code : 128 + 16 (code - 128)
+// <= d, so a step never passes a surface; value 0 (a hit) is exactly d = 0.
+// At 1 byte per pixel the field of a 1920x1080 frame is 2 MB and stays in
+// each XCD's 4 MB L2 while the long rays of the upper levels sample it.  The
+// bytes are stored in 16x8-pixel tiles (one 128-byte line), tiles row-major:
+// the samples of a ray and of neighbouring rays share lines.
+constexpr uint32_t kSdfFar = 255;
+constexpr float kSdfFarValue = 2160.0f;
+__host__ __device__ inline uint32_t sdf_code_of_distance(uint32_t d)  // d = floor of the distance
+{
+    return d < 128 ? d : (d < 2160 ? 128 + ((d - 128) >> 4) : kSdfFar);
+}
+__host__ __device__ inline float sdf_value(uint32_t code)
+{
+    return code < 128 ? (float)code : (float)(128 + 16 * (code - 128));
+}
+// floor(sqrt(d2)) for d2 < 2^31: fp32 estimate, corrected to the exact integer
+__host__ __device__ inline uint32_t isqrt_u32(uint32_t d2)
+{
+    uint32_t r = (uint32_t)sqrtf((float)d2);
+    while ((uint64_t)r * r > d2) --r;
+    while ((uint64_t)(r + 1) * (r + 1) <= d2) ++r;
+    return r;
+}
+__host__ __device__ inline int sdf_tiles_x(int W) { return (W + 15) >> 4; }
+__host__ __device__ inline size_t sdf_elems(int W, int H) { return (size_t)sdf_tiles_x(W) * 128 * (size_t)((H + 7) >> 3); }
 __host__ __device__ inline int sdf_index(int x, int y, int tiles_x)
 {
-    return (((y >> 2) * tiles_x + (x >> 3)) << 5) + ((y & 3) << 3) + (x & 7);
+    return (((y >> 3) * tiles_x + (x >> 4)) << 7) + ((y & 7) << 4) + (x & 15);
 }
 
 // ---- storage texels ---------------------------------------------------------

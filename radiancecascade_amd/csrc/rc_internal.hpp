@@ -17,11 +17,11 @@ hipError_t launch_raster(const double *boxes_dev, int n, int W, int H, float *de
 int jfa_num_passes(int W, int H);
 // largest step fused into the LDS tail for a requested fuse limit (0 = none)
 int jfa_tail_step(int W, int H, int fuse);
-hipError_t launch_sdf_untile(const float *sdf, float *out, int W, int H, hipStream_t s);
+hipError_t launch_sdf_untile(const uint8_t *sdf, float *out, int W, int H, hipStream_t s);
 hipError_t launch_seed_init(const float *depth, const float *emissive, uint32_t *seed, int W, int H, hipStream_t s);
-hipError_t launch_jfa_pass(const uint32_t *in, uint32_t *out, float *sdf, int W, int H, int step, bool final_pass,
+hipError_t launch_jfa_pass(const uint32_t *in, uint32_t *out, uint8_t *sdf, int W, int H, int step, bool final_pass,
                            hipStream_t s);
-hipError_t launch_jfa_tail(const uint32_t *in, uint32_t *seeds_out, float *sdf, int W, int H, int tail_step,
+hipError_t launch_jfa_tail(const uint32_t *in, uint32_t *seeds_out, uint8_t *sdf, int W, int H, int tail_step,
                            hipStream_t s);
 
 // ---- cascades (rc_cascade.hip) ----------------------------------------------
@@ -32,7 +32,7 @@ int tile_tx(int shape);
 int tile_ty(int shape);
 
 struct LevelArgs {
-    const float *sdf;        // [H][W]
+    const uint8_t *sdf;      // distance codes, 16x8 tiles
     const float *emissive;   // [3][H][W]
     const float2 *dirs;      // [N] unit directions of this level
     const void *upper;       // merged level i+1 (storage texels), nullptr at the top level
@@ -54,7 +54,7 @@ struct MarchLevel {
     int block_begin, blocks; // workgroups of 256*M canonical rays
 };
 struct MarchAllArgs {
-    const float *sdf;
+    const uint8_t *sdf;
     int W, H, nlev;
     MarchLevel lev[RC_MAX_LEVELS];
 };
@@ -150,7 +150,7 @@ struct ConeArgs {
     int FX, FY;
 };
 struct SurrLevel1Args {
-    const float *sdf;
+    const uint8_t *sdf;
     const float *emissive;
     const float2 *dirs;     // level-1 directions
     const float4 *cone;
@@ -176,7 +176,7 @@ void cone_schedule(int W, int H, const MipArgs &m, ConeSchedule &near, ConeSched
 int cone_top_level(int W, int H);
 hipError_t launch_seed_surr(const float *depth, const float *emissive, uint64_t *bits, const MipArgs &m, int W, int H,
                             hipStream_t s);
-hipError_t launch_near_sdf(const uint64_t *bits, float *sdf, int W, int H, hipStream_t s);
+hipError_t launch_near_sdf(const uint64_t *bits, uint8_t *sdf, int W, int H, hipStream_t s);
 hipError_t launch_mip_rest(const MipArgs &m, int *zero_me, hipStream_t s);
 hipError_t launch_mip_unpack(const MipArgs &m, int level, float *emc, float *mo, hipStream_t s);
 hipError_t launch_cone(const ConeArgs &a, hipStream_t s);

@@ -10,7 +10,8 @@
 //   pass(s)  : p takes the candidate with the smallest (d^2, seed_y, seed_x)
 //              among its own seed and the in-bounds seeds at p + s*(dx,dy);
 //   steps    : 2^(P-1), ..., 2, 1 with P = ceil(log2 max(W,H));
-//   SDF(p)   = (float)sqrt((double)d^2), +inf when no pixel is occupied.
+//   SDF(p)   = one-byte conservative code of sqrt(d^2) (rc_common.hpp), 255
+//              when no pixel is occupied.
 //
 // The passes are VALU-bound on CDNA4 (a wave64 VALU op occupies a 16-lane
 // SIMD for 4 cycles; 8 candidates per pixel per pass), so the candidate test
@@ -60,9 +61,9 @@ __device__ inline void consider(uint32_t s, uint32_t pix, uint32_t &bd2, uint32_
     bs = better ? s : bs;
 }
 
-__device__ inline float seed_distance(uint32_t bs, uint32_t bd2)
+__device__ inline uint8_t seed_sdf_code(uint32_t bs, uint32_t bd2)
 {
-    return bs == kFarSeed ? __builtin_huge_valf() : (float)sqrt((double)bd2);
+    return (uint8_t)(bs == kFarSeed ? kSdfFar : sdf_code_of_distance(isqrt_u32(bd2)));
 }
 
 int jfa_num_passes(int W, int H)
@@ -109,7 +110,7 @@ __global__ __launch_bounds__(kBlock) void k_seed_init(const float *__restrict__ 
 // instead of the internal seed field.
 template <bool FINAL>
 __global__ __launch_bounds__(kBlock) void k_jfa_pass(const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
-                                                     float *__restrict__ sdf, int W, int H, int step, int bx_count)
+                                                     uint8_t *__restrict__ sdf, int W, int H, int step, int bx_count)
 {
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int x = (b % bx_count) * 64 + (threadIdx.x & 63);
@@ -149,7 +150,7 @@ __global__ __launch_bounds__(kBlock) void k_jfa_pass(const uint32_t *__restrict_
     const uint32_t bs = (uint32_t)best, bd2 = (uint32_t)(best >> 32);
     const size_t p = (size_t)y * W + x;
     if (FINAL) {
-        sdf[sdf_index(x, y, sdf_tiles_x(W))] = seed_distance(bs, bd2);
+        sdf[sdf_index(x, y, sdf_tiles_x(W))] = seed_sdf_code(bs, bd2);
         if (out) out[p] = bs == kFarSeed ? kSeedNone : bs;
     } else {
         out[p] = bs;
@@ -164,7 +165,7 @@ __global__ __launch_bounds__(kBlock) void k_jfa_pass(const uint32_t *__restrict_
 // skipped".
 template <int S>
 __global__ __launch_bounds__(kBlock) void k_jfa_tail(const uint32_t *__restrict__ in,
-                                                     uint32_t *__restrict__ seeds_out, float *__restrict__ sdf,
+                                                     uint32_t *__restrict__ seeds_out, uint8_t *__restrict__ sdf,
                                                      int W, int H, int bx_count)
 {
     constexpr int T = 32;
@@ -239,7 +240,7 @@ __global__ __launch_bounds__(kBlock) void k_jfa_tail(const uint32_t *__restrict_
             const int gy = y0 + HALO + ty;
             if (ty >= T || gx >= W || gy >= H) continue;
             const uint32_t sd = buf[cur][(HALO + ty) * R + HALO + cx];
-            sdf[sdf_index(gx, gy, sdf_tiles_x(W))] = seed_distance(sd, last_d2[j]);
+            sdf[sdf_index(gx, gy, sdf_tiles_x(W))] = seed_sdf_code(sd, last_d2[j]);
             if (seeds_out) seeds_out[(size_t)gy * W + gx] = sd == kFarSeed ? kSeedNone : sd;
         }
     }
@@ -247,24 +248,24 @@ __global__ __launch_bounds__(kBlock) void k_jfa_tail(const uint32_t *__restrict_
 
 // An image with no pass at all (1x1): only the SDF conversion.
 __global__ void k_jfa_none(const uint32_t *__restrict__ in, uint32_t *__restrict__ seeds_out,
-                           float *__restrict__ sdf, int W, int n)
+                           uint8_t *__restrict__ sdf, int W, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t s = in[i];
-    sdf[sdf_index(i % W, i / W, sdf_tiles_x(W))] = s == kFarSeed ? __builtin_huge_valf() : 0.0f;
+    sdf[sdf_index(i % W, i / W, sdf_tiles_x(W))] = (uint8_t)(s == kFarSeed ? kSdfFar : 0u);
     if (seeds_out) seeds_out[i] = s == kFarSeed ? kSeedNone : s;
 }
 
-// Row-major copy of the tiled distance field (parity readback).
-__global__ void k_sdf_untile(const float *__restrict__ sdf, float *__restrict__ out, int W, int H)
+// Row-major fp32 copy of the distance values the march reads (parity readback).
+__global__ void k_sdf_untile(const uint8_t *__restrict__ sdf, float *__restrict__ out, int W, int H)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= W * H) return;
-    out[i] = sdf[sdf_index(i % W, i / W, sdf_tiles_x(W))];
+    out[i] = sdf_value(sdf[sdf_index(i % W, i / W, sdf_tiles_x(W))]);
 }
 
-hipError_t launch_sdf_untile(const float *sdf, float *out, int W, int H, hipStream_t s)
+hipError_t launch_sdf_untile(const uint8_t *sdf, float *out, int W, int H, hipStream_t s)
 {
     const int n = W * H;
     hipLaunchKernelGGL(k_sdf_untile, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, sdf, out, W, H);
@@ -278,7 +279,7 @@ hipError_t launch_seed_init(const float *depth, const float *emissive, uint32_t 
     return hipGetLastError();
 }
 
-hipError_t launch_jfa_pass(const uint32_t *in, uint32_t *out, float *sdf, int W, int H, int step, bool final_pass,
+hipError_t launch_jfa_pass(const uint32_t *in, uint32_t *out, uint8_t *sdf, int W, int H, int step, bool final_pass,
                            hipStream_t s)
 {
     const int bxc = (W + 63) / 64, byc = (H + 3) / 4;
@@ -289,7 +290,7 @@ hipError_t launch_jfa_pass(const uint32_t *in, uint32_t *out, float *sdf, int W,
     return hipGetLastError();
 }
 
-hipError_t launch_jfa_tail(const uint32_t *in, uint32_t *seeds_out, float *sdf, int W, int H, int tail_step,
+hipError_t launch_jfa_tail(const uint32_t *in, uint32_t *seeds_out, uint8_t *sdf, int W, int H, int tail_step,
                            hipStream_t s)
 {
     const int bxc = (W + 31) / 32, byc = (H + 31) / 32;

@@ -22,9 +22,9 @@ __device__ inline float quad_sum(float v)
 // its value unused.
 constexpr int kBufferDword3 = 0x00020000;  // gfx9-family raw buffer, 32-bit data
 
-__device__ inline __amdgpu_buffer_rsrc_t sdf_rsrc(const float *sdf, int W, int H)
+__device__ inline __amdgpu_buffer_rsrc_t sdf_rsrc(const uint8_t *sdf, int W, int H)
 {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(sdf), (short)0, (int)(sdf_elems(W, H) * 4),
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t *>(sdf), (short)0, (int)sdf_elems(W, H),
                                              kBufferDword3);
 }
 
@@ -32,13 +32,13 @@ __device__ inline __amdgpu_buffer_rsrc_t sdf_rsrc(const float *sdf, int W, int H
 // hit code and the emissive index.
 __device__ inline int pixel_index(int x, int y, int W) { return (int)__umul24((uint32_t)y, (uint32_t)W) + x; }
 
-// Tiled SDF load.  For an off-image (x, y) the offset may be anything; the
-// range-checked load is then unused.
+// Tiled distance load (one byte, decoded).  For an off-image (x, y) the
+// offset may be anything; the range-checked load is then unused.
 __device__ inline float sdf_load(__amdgpu_buffer_rsrc_t r, int x, int y, int tiles_x)
 {
     const uint32_t ux = (uint32_t)x, uy = (uint32_t)y;
-    const uint32_t off = (__umul24(uy >> 2, (uint32_t)tiles_x << 5) + ((ux & ~7u) << 2) + ((uy & 3u) << 3) + (ux & 7u)) << 2;
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+    const uint32_t off = __umul24(uy >> 3, (uint32_t)tiles_x << 7) + ((ux & ~15u) << 3) + ((uy & 7u) << 4) + (ux & 15u);
+    return sdf_value(__builtin_amdgcn_raw_buffer_load_b8(r, (int)off, 0, 0));
 }
 
 // max(d, 1) as one v_med3 (no NaN canonicalisation: d is never NaN).  For

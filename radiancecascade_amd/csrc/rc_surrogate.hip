@@ -121,18 +121,18 @@ hipError_t launch_seed_surr(const float *depth, const float *emissive, uint64_t 
 }
 
 // ---- near-field distance field ---------------------------------------------
-// Exact Euclidean distance to the nearest occupied pixel wherever it is at most
-// kNearR pixels, +inf elsewhere.  Levels 0 and 1 end their intervals at 2 and
+// The distance code of the nearest occupied pixel wherever it is at most
+// kNearR pixels away (exact there), the far code elsewhere.  Levels 0 and 1 end their intervals at 2 and
 // 10 px, so every step they take from such a field (a hit needs distance 0; a
-// distance above 10 ends the interval in one step either way) equals the step
-// they take from the full field.  Per 32x32 tile: the occupancy row masks of
+// value of 10 or more ends the interval in one step either way) equals the
+// step they take from the full field.  Per 32x32 tile: the occupancy row masks of
 // the tile plus a 10-px halo (two words of the bit field per row), the
 // distance to the nearest occupied pixel of the row within +-10 (count of
 // zero bits), then d^2 = min over |dy| <= 10 of g(y+dy)^2 + dy^2.
 constexpr int kNearR = 10, kNearT = 32, kNearS = kNearT + 2 * kNearR;
 
 __global__ __launch_bounds__(kBlock) void k_near_sdf(const uint64_t *__restrict__ bits, int words,
-                                                     float *__restrict__ sdf, int W, int H, int tiles_x)
+                                                     uint8_t *__restrict__ sdf, int W, int H, int tiles_x)
 {
     __shared__ uint64_t mask[kNearS];
     __shared__ uint8_t g[kNearS][kNearT];
@@ -175,11 +175,11 @@ __global__ __launch_bounds__(kBlock) void k_near_sdf(const uint64_t *__restrict_
             const int gv = g[r + kNearR + dy][j];
             best = min(best, gv * gv + dy * dy);
         }
-        sdf[sdf_index(x, y, tx)] = best <= kNearR * kNearR ? (float)sqrt((double)best) : __builtin_huge_valf();
+        sdf[sdf_index(x, y, tx)] = (uint8_t)(best <= kNearR * kNearR ? isqrt_u32((uint32_t)best) : kSdfFar);
     }
 }
 
-hipError_t launch_near_sdf(const uint64_t *bits, float *sdf, int W, int H, hipStream_t s)
+hipError_t launch_near_sdf(const uint64_t *bits, uint8_t *sdf, int W, int H, hipStream_t s)
 {
     const int tx = (W + kNearT - 1) / kNearT, ty = (H + kNearT - 1) / kNearT;
     hipLaunchKernelGGL(k_near_sdf, dim3(tx * ty), dim3(kBlock), 0, s, bits, (W + 63) / 64, sdf, W, H, tx);

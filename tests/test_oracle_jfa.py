@@ -55,13 +55,14 @@ def test_single_seed_exact(oracle):
         assert np.all(s == ((y << 16) | x))
         sdf = oracle.sdf_f32(s)
         yy, xx = np.mgrid[0:H, 0:W]
-        assert np.allclose(sdf, np.sqrt((xx - x) ** 2 + (yy - y) ** 2), rtol=1e-7)
+        d = np.sqrt((xx - x) ** 2 + (yy - y) ** 2)
+        assert np.array_equal(sdf, _coded(d).astype(np.float32))
 
 
 def test_empty_field(oracle):
     s = oracle.jfa(_seed_field(33, 17, []))
     assert np.all(s == oracle.SEED_NONE)
-    assert np.all(np.isinf(oracle.sdf_f32(s)))
+    assert np.all(oracle.sdf_f32(s) == oracle.SDF_FAR)
 
 
 def test_bruteforce_matches_scipy_edt(oracle):
@@ -119,10 +120,30 @@ def test_pass_is_order_independent(oracle):
             assert out[y, x] == (min(cands) & 0xFFFFFFFF if cands else 0xFFFFFFFF)
 
 
+def _coded(d):
+    """Independent restatement of the conservative distance code (float input)."""
+    f = np.floor(np.asarray(d, np.float64) + 1e-9)
+    return np.where(f < 128, f, np.where(f < 2160, 128 + 16 * np.floor((f - 128) / 16), 2160.0))
+
+
 def test_sdf_values(oracle):
     init = _seed_field(9, 9, [(0, 0)])
     s = oracle.jfa(init)
     f32 = oracle.sdf_f32(s)
     f64 = oracle.sdf_f64(s)
-    assert f32[0, 0] == 0 and f32[3, 4] == np.float32(5.0) and f64[8, 8] == pytest.approx(8 * np.sqrt(2), rel=1e-15)
+    assert f32[0, 0] == 0 and f32[3, 4] == np.float32(5.0) and f64[8, 8] == 11.0   # floor(8 sqrt 2)
     assert f32.dtype == np.float32 and f64.dtype == np.float64
+
+
+def test_sdf_code_rule(oracle):
+    # exact squares, one below a square, the band edges and the cap
+    for d2 in [0, 1, 2, 3, 4, 99, 100, 101, 127 ** 2, 128 ** 2 - 1, 128 ** 2, 143 ** 2, 144 ** 2 - 1, 144 ** 2,
+               2159 ** 2, 2160 ** 2, 5000 ** 2, 2 * 16383 ** 2]:
+        v = oracle.sdf_value_d2(d2)
+        d = d2 ** 0.5
+        assert v <= d + 1e-12                                   # conservative: never past a surface
+        assert (v == 0) == (d2 == 0)                            # hits are exact
+        assert v == _coded(np.sqrt(float(d2)))
+    assert oracle.sdf_value_d2(127 ** 2) == 127 and oracle.sdf_value_d2(128 ** 2 - 1) == 127
+    assert oracle.sdf_value_d2(143 ** 2) == 128 and oracle.sdf_value_d2(144 ** 2) == 144
+    assert oracle.sdf_value_d2(10 ** 8) == 2160
