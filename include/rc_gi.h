@@ -189,14 +189,16 @@ rc_status rc_error_norms(rc_ctx *ctx, const float *x, const void *ref, uint32_t 
 
 /* ---- learned far field ------------------------------------------------------
  * No counterpart in the reference (it has no GI).  Mode 1 replaces levels >= 2
- * of the frame: a cone trace through an emitter / occluder mip pyramid on the
- * level-2 probe grid gives, per level-1 probe and direction k, a far-field
- * estimate c_k; an MLP (48 -> 64 -> 64 -> 64 -> 64, ReLU, BF16 weights on MFMA,
- * FP32 accumulation) maps x = standardised log1p(c) to z and the far field is
- * B_k = max(c_k + out_scale * z_k, 0); output 48 predicts log of the squared
- * error.  Tiles of 8x8 level-1 probes whose mean predicted error exceeds
- * `threshold` take the plain cascades' level 1 (fallback: one full plain frame
- * plus a patch, decided on the host after the frame).  Needs levels >= 3. */
+ * of the frame: a cone trace through an emitter / occluder mip pyramid gives,
+ * per level-2 probe and level-1 direction k, a far-field estimate c_k; an MLP
+ * (48 -> 64 -> 64 -> 64 -> 64, ReLU, BF16 weights on MFMA, FP32 accumulation)
+ * maps x = standardised log(1 + c) to z, and the merged level 2 is
+ * A_2 = max(c_k + out_scale * z_k, 0); levels 1 and 0 are then the plain
+ * kernels (marched on a distance field exact up to 10 px).  Output 48
+ * predicts log of the squared error.  Tiles of 8x8 level-2 probes whose mean
+ * predicted error exceeds `threshold` take the plain cascades' level 1
+ * (fallback: one full plain frame plus a patch, decided on the host after the
+ * frame).  Needs levels >= 3. */
 #define RC_MLP_INPUTS 48
 #define RC_MLP_OUTPUTS 64
 #define RC_MLP_PARAMS 15616  /* for l = 0..3: W_l [K_l][64] (K = 48, 64, 64, 64) then b_l [64] */
@@ -229,8 +231,8 @@ rc_status rc_mlp_gradient(rc_ctx *ctx, const float *feats_dev, const float *targ
                           float *loss_dev, void *stream);
 rc_status rc_mlp_adam(rc_ctx *ctx, const float *grad_dev, float lr, void *stream);
 /* Training rows of one G-buffer at the context's size: runs the plain frame,
- * then per level-1 probe writes feats [P][48] = log1p(c) and targets [P][48]
- * = B_plain - c (unnormalised), P = Px_1 * Py_1. */
+ * then per level-2 probe writes feats [P][48] = log(1 + c) and targets
+ * [P][48] = A_2(plain) - c (unnormalised), P = Px_2 * Py_2. */
 rc_status rc_surrogate_samples(rc_ctx *ctx, const rc_gbuffer *dev, float *feats_dev, float *targets_dev,
                                void *stream);
 /* Train the context's network on the GPU on plain-path outputs of seeded
@@ -260,7 +262,7 @@ typedef struct {
 rc_status rc_surrogate_train(rc_ctx *ctx, const rc_train_config *cfg, rc_train_report *report);
 /* Readbacks of the last learned-far-field frame (device destinations, any may
  * be NULL): cone [Py_2][Px_2][16][4] (rgb, transmittance); level1 [Py_1][Px_1][4][4]
- * merged level 1 (fp32); tile_err [ceil(Py_1/8)][ceil(Px_1/8)]. */
+ * merged level 1 (fp32); tile_err [ceil(Py_2/8)][ceil(Px_2/8)]. */
 rc_status rc_debug_surrogate(rc_ctx *ctx, float *cone_dev, float *level1_dev, float *tile_err_dev, void *stream);
 /* Mip level m >= 1 of the last learned-far-field frame: emc [h][w][4]
  * (mean E*[emitter] rgb, mean [emitter]) and mo [h][w] (mean [occluder]). */

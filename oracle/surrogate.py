@@ -1,5 +1,5 @@
 """FP64 reference of the learned far field (near distance field, emitter /
-occluder mip pyramid, cone trace, MLP features and the level-1 composition).
+occluder mip pyramid, cone trace, MLP features and the predicted level 2).
 
 TEST INFRASTRUCTURE ONLY: the checker for the GPU kernels of
 radiancecascade_amd/csrc/rc_surrogate.hip; the product never imports it.
@@ -9,8 +9,8 @@ The reference repository has no global illumination and no learned part
 restated here are the project's own and are pinned in
 tests/test_oracle_surrogate.py by a library EDT (scipy), block means computed
 directly from pixels, closed forms of the cone trace (empty scene, fully
-emissive scene, linearity, mirror symmetry) and the reduction of the
-composition to the plain merge (oracle/rco.c) when the far field is exact.
+emissive scene, linearity, mirror symmetry); the composition is the plain
+merge of oracle/rco.c itself.
 """
 from __future__ import annotations
 
@@ -22,7 +22,7 @@ from . import mlp, rco
 
 NEAR_R = 10                      # exact distances up to the level-1 interval end
 N_DIRS = 16                      # level-1 directions = cones per level-2 probe
-CONE_T0 = 6.0
+CONE_T0 = 10.0                   # the start of level 2's interval
 CONE_GROW = 0.75
 CONE_SPLIT = 60.0                # near segment from level-2 probes, far from level-3
 T_MIN = 1e-4
@@ -192,7 +192,7 @@ def cone_trace(levels, W: int, H: int, schedule=None) -> np.ndarray:
     return out
 
 
-# ---- features and composition (k_surr_level1, k_surr_sample) -------------------
+# ---- features and the predicted level 2 (k_surr_mlp, k_surr_sample) ------------
 def _axis(n: int, un: int):
     p = np.arange(n)
     base = (p - 1) >> 1
@@ -213,29 +213,25 @@ def upper_bilinear(U: np.ndarray, PX: int, PY: int) -> np.ndarray:
     return top * wy0[:, None, None, None] + bot * wy1[:, None, None, None]
 
 
-def features(cone: np.ndarray, W: int, H: int):
-    """(c [P][48] cone radiance at the level-1 probes, x [P][48] = log1p(c))."""
-    PX, PY = rco.probe_grid(W, H, 1)
-    c = upper_bilinear(cone, PX, PY).reshape(PX * PY, 48)
+def features(cone: np.ndarray):
+    """(c [Q][48] cone radiance at the level-2 probes, x [Q][48] = log(1 + c))."""
+    QY, QX = cone.shape[:2]
+    c = cone[..., :3].reshape(QY * QX, 48)
     return c, np.log1p(c)
 
 
-def samples(cone: np.ndarray, A2: np.ndarray, W: int, H: int):
-    """Training rows: (log1p(c), B - c) with B the plain level-2 bilinear."""
-    PX, PY = rco.probe_grid(W, H, 1)
-    c, x = features(cone, W, H)
-    B = upper_bilinear(A2, PX, PY).reshape(PX * PY, 48)
-    return x, B - c
+def samples(cone: np.ndarray, A2: np.ndarray):
+    """Training rows per level-2 probe: (log(1 + c), A2.rgb - c) with A2 the
+    plain path's merged level 2 (16 entries per probe)."""
+    c, x = features(cone)
+    return x, A2[..., :3].reshape(c.shape[0], 48) - c
 
 
-def compose_level1(march1: np.ndarray, c: np.ndarray, z: np.ndarray, out_scale: float) -> np.ndarray:
-    """Merged level 1 [PY][PX][4][4] from the level-1 march [PY][PX][16][4],
-    cone radiance c [P][48] and network outputs z [P][>=48]."""
-    PY, PX = march1.shape[:2]
-    B = np.maximum(c + out_scale * z[:, :48], 0.0).reshape(PY, PX, 16, 3)
-    al = march1[..., 3:4]
-    R = np.concatenate([march1[..., :3] + al * B, al], -1)
-    return 0.25 * ((R[:, :, 0::4] + R[:, :, 1::4]) + (R[:, :, 2::4] + R[:, :, 3::4]))
+def predicted_level2(c: np.ndarray, z: np.ndarray, out_scale: float, QX: int, QY: int) -> np.ndarray:
+    """The network's merged level 2 [QY][QX][16][4]: rgb = max(c + s z, 0),
+    4th channel 1 (level 1 reads only rgb of its upper level)."""
+    A = np.maximum(c + out_scale * z[:, :48], 0.0).reshape(QY, QX, 16, 3)
+    return np.concatenate([A, np.ones((QY, QX, 16, 1))], -1)
 
 
 def params_to_layers(params: np.ndarray):
@@ -257,9 +253,10 @@ def layers_to_params(layers) -> np.ndarray:
 
 
 def run(depth, emissive, L, params, in_mean, in_istd, out_scale, want=False):
-    """Whole learned-far-field frame in FP64 (levels 0 and 1 marched on the
-    near distance field; the network evaluated with the GPU's bf16 rounding
-    points).  Returns irradiance [3,H,W] (and intermediates if want)."""
+    """Whole learned-far-field frame in FP64: the network's level 2 from the
+    cone features, then the plain merge of levels 1 and 0 (marched on the
+    near distance field) and the gather; the network evaluated with the GPU's
+    bf16 rounding points.  Returns irradiance [3,H,W] (and intermediates)."""
     H, W = depth.shape
     seed0 = rco.seed_init(depth, emissive)
     sdf = near_sdf(seed0).astype(np.float64)
@@ -267,12 +264,14 @@ def run(depth, emissive, L, params, in_mean, in_istd, out_scale, want=False):
     m0, _ = rco.march_level(sdf, emissive, 0)
     levels = mip(seed0, emissive)
     cone = cone_trace(levels, W, H)
-    c, x = features(cone, W, H)
+    c, x = features(cone)
     xn = (x - np.asarray(in_mean, np.float64)) * np.asarray(in_istd, np.float64)
     z, _ = mlp.forward(params_to_layers(params), xn, emulate_bf16=True)
-    A1 = compose_level1(m1, c, z, out_scale)
+    QX, QY = rco.probe_grid(W, H, 2)
+    A2 = predicted_level2(c, z, out_scale, QX, QY)
+    A1 = rco.merge_level(W, H, 1, m1, A2)
     A0 = rco.merge_level(W, H, 0, m0, A1)
     irr = rco.gather(W, H, A0)
     if want:
-        return irr, dict(sdf=sdf, m1=m1, m0=m0, levels=levels, cone=cone, c=c, x=x, z=z, A1=A1)
+        return irr, dict(sdf=sdf, m1=m1, m0=m0, levels=levels, cone=cone, c=c, x=x, z=z, A2=A2, A1=A1)
     return irr

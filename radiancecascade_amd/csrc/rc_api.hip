@@ -66,7 +66,8 @@ struct rc_ctx {
     ConeSchedule sch{}, sch_far{};
     float4 *cone = nullptr, *cone_far = nullptr;
     uint64_t *occ_bits = nullptr;  // occupancy, one bit per pixel, rows of ceil(W/64) words
-    void *surr_a1 = nullptr;
+    void *surr_a1 = nullptr;   // learned frame's level 1
+    void *surr_a2 = nullptr;   // the network's merged level 2
     float *tile_err = nullptr;
     int *flag_count = nullptr, *flag_list = nullptr;
     int *flag_host = nullptr;  // pinned
@@ -154,6 +155,8 @@ static void release(rc_ctx *c)
     if (c->cone_far) hipFree(c->cone_far);
     c->cone_far = nullptr;
     if (c->surr_a1) hipFree(c->surr_a1);
+    if (c->surr_a2) hipFree(c->surr_a2);
+    c->surr_a2 = nullptr;
     if (c->tile_err) hipFree(c->tile_err);
     if (c->flag_count) hipFree(c->flag_count);
     if (c->flag_list) hipFree(c->flag_list);
@@ -413,8 +416,9 @@ static rc_status ensure_surrogate(rc_ctx *c)
     HIP_TRY(c, hipMalloc(&c->cone_far, (size_t)grid_dim(c->W, 3) * grid_dim(c->H, 3) * 16 * sizeof(float4)));
     const int PX = grid_dim(c->W, 1), PY = grid_dim(c->H, 1);
     HIP_TRY(c, hipMalloc(&c->surr_a1, (size_t)PX * PY * 4 * texel_bytes(c->storage)));
-    c->surr_tiles_x = (PX + 7) / 8;
-    c->surr_tiles = c->surr_tiles_x * ((PY + 7) / 8);
+    HIP_TRY(c, hipMalloc(&c->surr_a2, (size_t)QX * QY * 16 * texel_bytes(c->storage)));
+    c->surr_tiles_x = (QX + 7) / 8;  // tiles of 8x8 level-2 probes
+    c->surr_tiles = c->surr_tiles_x * ((QY + 7) / 8);
     HIP_TRY(c, hipMalloc(&c->tile_err, (size_t)c->surr_tiles * sizeof(float)));
     HIP_TRY(c, hipMalloc(&c->flag_list, (size_t)c->surr_tiles * sizeof(int)));
     HIP_TRY(c, hipMalloc(&c->flag_count, sizeof(int)));
@@ -489,19 +493,12 @@ static rc_status run_frame_surrogate(rc_ctx *c, const float *depth, const float 
     }
     tm.end(RC_STAGE_CONE);
     if (fork) HIP_TRY(c, hipStreamWaitEvent(s, c->ev_join, 0));
-    SurrLevel1Args a{};
-    a.sdf = c->sdf;
-    a.emissive = emissive;
-    a.dirs = c->dirs + c->dir_off[1];
+    SurrMlpArgs a{};
     a.cone = c->cone;
-    a.out = c->surr_a1;
+    a.out = c->surr_a2;
     a.tile_err = c->tile_err;
     a.flag_count = c->flag_count;
     a.flag_list = c->flag_list;
-    a.W = c->W;
-    a.H = c->H;
-    a.PX = grid_dim(c->W, 1);
-    a.PY = grid_dim(c->H, 1);
     a.QX = grid_dim(c->W, 2);
     a.QY = grid_dim(c->H, 2);
     a.tiles_x = c->surr_tiles_x;
@@ -509,8 +506,17 @@ static rc_status run_frame_surrogate(rc_ctx *c, const float *depth, const float 
     a.out_scale = c->out_scale;
     a.threshold = c->surr_thr;
     tm.begin(RC_STAGE_SURROGATE);
-    HIP_TRY(c, launch_surr_level1(a, c->mlp, c->storage, s));
+    HIP_TRY(c, launch_surr_mlp(a, c->mlp, c->storage, s));
     tm.end(RC_STAGE_SURROGATE);
+    // level 1: the plain level kernel, marching in place on the near field,
+    // merging the network's level 2
+    tm.begin(RC_STAGE_LEVEL0 + 1);
+    LevelArgs a1 = level_args(c, 1, emissive);
+    a1.upper = c->surr_a2;
+    a1.out = c->surr_a1;
+    a1.codes = nullptr;
+    HIP_TRY(c, launch_level(a1, c->storage, c->tile[1], c->rpt[1], s));
+    tm.end(RC_STAGE_LEVEL0 + 1);
     tm.begin(RC_STAGE_LEVEL0);
     LevelArgs a0 = level_args(c, 0, emissive);
     a0.upper = c->surr_a1;
@@ -600,7 +606,7 @@ static rc_status frame(rc_ctx *c, const float *depth, const float *emissive, flo
     st = run_frame(c, depth, emissive, irr, s, true);
     if (st != RC_OK) return st;
     HIP_TRY(c, launch_tile_patch(c->level[1], c->surr_a1, c->tile_err, c->surr_thr, grid_dim(c->W, 1),
-                                 grid_dim(c->H, 1), c->storage, s));
+                                 grid_dim(c->H, 1), c->surr_tiles_x, c->storage, s));
     LevelArgs a0 = level_args(c, 0, emissive);
     a0.upper = c->surr_a1;
     HIP_TRY(c, launch_level0_gather(a0, c->storage, irr, nullptr, s));
@@ -636,8 +642,6 @@ static rc_status gen_samples(rc_ctx *c, const float *depth, const float *emissiv
     a.upper = c->level[2];
     a.feats = feats;
     a.targets = targets;
-    a.PX = grid_dim(c->W, 1);
-    a.PY = grid_dim(c->H, 1);
     a.QX = grid_dim(c->W, 2);
     a.QY = grid_dim(c->H, 2);
     HIP_TRY(c, launch_surr_sample(a, c->storage, s));
@@ -1305,7 +1309,7 @@ rc_status rc_surrogate_train(rc_ctx *c, const rc_train_config *cfg, rc_train_rep
         }
     } guard{tc, {}};
     const size_t px = (size_t)cfg->width * cfg->height;
-    const size_t P = (size_t)grid_dim(cfg->width, 1) * grid_dim(cfg->height, 1);
+    const size_t P = (size_t)grid_dim(cfg->width, 2) * grid_dim(cfg->height, 2);  // rows per scene
     const size_t N = P * cfg->scenes;
     float *depth = nullptr, *emis = nullptr, *feats = nullptr, *targets = nullptr;
     double *boxes = nullptr, *stats = nullptr;

@@ -149,24 +149,21 @@ struct ConeArgs {
     const float4 *far;      // far-segment cones [FY][FX][16] to append, or nullptr
     int FX, FY;
 };
-struct SurrLevel1Args {
-    const uint8_t *sdf;
-    const float *emissive;
-    const float2 *dirs;     // level-1 directions
-    const float4 *cone;
-    void *out;              // merged level 1 (storage texels)
-    float *tile_err;        // [tiles] mean predicted squared error (normalised)
+struct SurrMlpArgs {
+    const float4 *cone;     // [QY][QX][16] cones at the level-2 probes
+    void *out;              // the network's merged level 2 (storage texels, [QY][QX][16])
+    float *tile_err;        // [tiles] mean predicted squared error (normalised), tiles of 8x8 level-2 probes
     int *flag_count;        // tiles above threshold (counter)
     int *flag_list;         // their indices
-    int W, H, PX, PY, QX, QY, tiles_x, tiles;
+    int QX, QY, tiles_x, tiles;
     float out_scale, threshold;
 };
 struct SurrSampleArgs {
     const float4 *cone;
     const void *upper;      // merged level 2 of the plain path (storage texels)
-    float *feats;           // [rows][48] log1p(cone radiance)
-    float *targets;         // [rows][48] upper bilinear minus cone radiance
-    int PX, PY, QX, QY;
+    float *feats;           // [QY*QX][48] log(1 + cone radiance)
+    float *targets;         // [QY*QX][48] merged level 2 minus cone radiance
+    int QX, QY;
 };
 
 void mip_layout(int W, int H, int top_level, MipArgs &m);   // fills w, h, off, levels
@@ -180,10 +177,10 @@ hipError_t launch_near_sdf(const uint64_t *bits, uint8_t *sdf, int W, int H, hip
 hipError_t launch_mip_rest(const MipArgs &m, int *zero_me, hipStream_t s);
 hipError_t launch_mip_unpack(const MipArgs &m, int level, float *emc, float *mo, hipStream_t s);
 hipError_t launch_cone(const ConeArgs &a, hipStream_t s);
-hipError_t launch_surr_level1(const SurrLevel1Args &a, const MlpDev &m, Storage st, hipStream_t s);
+hipError_t launch_surr_mlp(const SurrMlpArgs &a, const MlpDev &m, Storage st, hipStream_t s);
 hipError_t launch_surr_sample(const SurrSampleArgs &a, Storage st, hipStream_t s);
 hipError_t launch_tile_patch(const void *plain, void *surr, const float *tile_err, float threshold, int PX, int PY,
-                             Storage st, hipStream_t s);
+                             int tiles_x, Storage st, hipStream_t s);
 hipError_t launch_col_stats(const float *x, size_t rows, int cols, double *partial, int parts, hipStream_t s);
 hipError_t launch_scale(float *x, size_t n, float k, hipStream_t s);
 

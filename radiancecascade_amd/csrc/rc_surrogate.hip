@@ -9,7 +9,8 @@
 //   k_near_sdf                  exact distance field up to 10 px
 //   k_mip_mid / k_mip_top       mip levels 3..top
 //   k_cone                      16 cones per level-2 probe
-//   k_surr_level1               level-1 march + MLP + merge
+//   k_surr_mlp                  the network's merged level 2
+//   k_level (rc_cascade.hip)    level 1, marched on the near distance field
 //   k_level0 (rc_cascade.hip)   level 0 + gather
 // Cone tracing through a prefiltered pyramid is Crassin et al.'s voxel cone
 // tracing (2011) in screen space; here it is the MLP's input feature, not the
@@ -338,15 +339,15 @@ hipError_t launch_mip_unpack(const MipArgs &m, int level, float *emc, float *mo,
 }
 
 // ---- cone trace --------------------------------------------------------------
-// Schedule: t_0 = 6 px from the level-2 probe centre (a level-1 probe sits 2.8
-// px from it and its far field starts at 10 px), cone diameter
+// Schedule: t_0 = 10 px from the level-2 probe centre (where level 2's
+// interval starts), cone diameter
 // dia = max(2 t tan(pi/16), 1), mip level floor(log2 dia) clamped to
 // [1, top], next t = t + 0.75 dia, up to the image diagonal + 2.
 void cone_schedule(int W, int H, const MipArgs &mip, ConeSchedule &near, ConeSchedule &far)
 {
     const int top = mip.levels;
     const double tmax = sqrt((double)W * W + (double)H * H) + 2.0, tn = tan(M_PI / 16.0);
-    double t = 6.0;
+    double t = 10.0;  // the start of level 2's interval
     near.n = 0;
     far.n = 0;
     while (t < tmax && near.n + far.n < kConeMaxSteps) {
@@ -459,16 +460,16 @@ hipError_t launch_cone(const ConeArgs &a, hipStream_t s)
     return hipGetLastError();
 }
 
-// ---- level 1 with the learned far field ----------------------------------------
-// A workgroup owns 8x8 level-1 probes per iteration (persistent over tiles).  Lane (probe p, quad ko) marches
-// the 4 directions 4ko..4ko+3 of p over [2, 10) on the distance field, and
-// builds their 12 inputs: log1p of the cone radiance bilinearly interpolated
-// from the level-2 grid (the merge's weights).  Wave w then computes the
-// 32x32 output block (rows 32(w&1), columns 32(w>>1)) of each MLP layer on
-// MFMA.  The far field is B = max(c + s * z, 0); a ray that reaches t1 takes
-// B, a hit takes E(hit); the quad is reduced to the merged level-1 texel
-// exactly as the plain merge does.  The error head exp(z_48), averaged over
-// the tile, is the tile's predicted mean squared error (normalised units).
+// ---- the network's level 2 -----------------------------------------------------------
+// A workgroup owns 8x8 level-2 probes per iteration (persistent over tiles).
+// Lane (probe p, quad ko) builds the 12 inputs of directions 4ko..4ko+3 of p,
+// log(1 + c) of the probe's cone radiance; wave w then computes the 32x32
+// output block (rows 32(w&1), columns 32(w>>1)) of each MLP layer on MFMA.
+// The merged level 2 is A_2 = max(c + s * z, 0) per direction (4th channel
+// 1; level 1 reads only rgb), written in the storage format of the merged
+// levels; level 1 and level 0 are then the plain kernels.  The error head
+// exp(z_48), averaged over the tile, is the tile's predicted mean squared
+// error (normalised units).
 constexpr int kSStrideIn = kMlpIn + 8, kSStrideHid = kMlpHid + 8;
 
 // Input feature of a cone radiance value: log(1 + c) through the hardware
@@ -490,56 +491,13 @@ __device__ inline floatx16 mfma_block(const uint16_t *in, int sin, int rb, const
     return c;
 }
 
-template <int RPT>
-__device__ inline void march_rays_dirs(__amdgpu_buffer_rsrc_t sdf, int W, int H, float cx, float cy, const float *dx,
-                                       const float *dy, float t0, float t1, bool active, int *code)
-{
-    float t[RPT];
-    bool alive[RPT];
-#pragma unroll
-    for (int j = 0; j < RPT; ++j) {
-        t[j] = t0;
-        alive[j] = active && t0 < t1;
-        code[j] = kRayT1;
-    }
-    for (;;) {
-        int q[RPT];
-        float d[RPT];
-#pragma unroll
-        for (int j = 0; j < RPT; ++j) {
-            const int ix = __float2int_rd(fmaf(t[j], dx[j], cx));
-            const int iy = __float2int_rd(fmaf(t[j], dy[j], cy));
-            const bool inside = (uint32_t)ix < (uint32_t)W && (uint32_t)iy < (uint32_t)H;
-            if (alive[j] && !inside) code[j] = kRayExit;
-            alive[j] = alive[j] && inside;
-            q[j] = pixel_index(ix, iy, W);
-            d[j] = sdf_load(sdf, ix, iy, sdf_tiles_x(W));
-        }
-        bool any = false;
-#pragma unroll
-        for (int j = 0; j < RPT; ++j) {
-            if (alive[j]) {
-                if (d[j] < 0.5f) {
-                    code[j] = q[j];
-                    alive[j] = false;
-                } else {
-                    t[j] = t[j] + march_step(d[j]);
-                    alive[j] = t[j] < t1;
-                }
-            }
-            any = any || alive[j];
-        }
-        if (!any) break;
-    }
-}
-
 template <Storage S>
-__global__ __launch_bounds__(kBlock) void k_surr_level1(SurrLevel1Args a, MlpDev m)
+__global__ __launch_bounds__(kBlock) void k_surr_mlp(SurrMlpArgs a, MlpDev m)
 {
     using TT = Texel<S>;
     __shared__ float bias_s[4 * 64], mean_s[kMlpIn], istd_s[kMlpIn];
     // xs (layer-1 input) and hb (layer-3 input) are dead when the output layer
-    // writes zs: one buffer holds all three (26.6 KB of LDS per workgroup).
+    // writes zs: one buffer holds all three.
     __shared__ __attribute__((aligned(16))) uint16_t xs_hb[64 * kSStrideIn + 64 * kSStrideHid];
     __shared__ __attribute__((aligned(16))) uint16_t ha[64 * kSStrideHid];
     __shared__ float red[kBlock / 64][2];
@@ -555,53 +513,27 @@ __global__ __launch_bounds__(kBlock) void k_surr_level1(SurrLevel1Args a, MlpDev
     }
     __syncthreads();
     // B operands straight from the bf16 weights in global memory (30 KB, held
-    // in every CU's L1 and the L2): the LDS then fits 4 workgroups per CU.
+    // in every CU's L1 and the L2)
     const uint16_t *wl[4] = {m.wt, m.wt + 64 * kMlpIn, m.wt + 64 * kMlpIn + 64 * kMlpHid,
                              m.wt + 64 * kMlpIn + 2 * 64 * kMlpHid};
-
-    const __amdgpu_buffer_rsrc_t sdf = sdf_rsrc(a.sdf, a.W, a.H);
     const int p = threadIdx.x >> 2, ko = threadIdx.x & 3;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int rb = (wave & 1) * 32, c0 = (wave >> 1) * 32;
-    const size_t plane = (size_t)a.W * a.H;
-    float dx[4], dy[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const float2 d = a.dirs[4 * ko + j];
-        dx[j] = d.x;
-        dy[j] = d.y;
-    }
 
-    // tiles are dealt XCD by XCD in contiguous ranges (the cone and distance
-    // field rows they read stay in that XCD's L2)
+    // tiles are dealt XCD by XCD in contiguous ranges
     const int per_xcd = (int)gridDim.x >> 3, xcd = blockIdx.x & 7, idx = blockIdx.x >> 3;
     const int t_begin = (int)((long long)a.tiles * xcd / 8), t_end = (int)((long long)a.tiles * (xcd + 1) / 8);
 #pragma unroll 1
     for (int tile = t_begin + idx; tile < t_end; tile += per_xcd) {
-        const int px = (tile % a.tiles_x) * 8 + (p & 7), py = (tile / a.tiles_x) * 8 + (p >> 3);
-        const bool valid = px < a.PX && py < a.PY;
-        const float pcx = ((float)px + 0.5f) * 4.0f, pcy = ((float)py + 0.5f) * 4.0f;
-        int code[4];
-        march_rays_dirs<4>(sdf, a.W, a.H, pcx, pcy, dx, dy, 2.0f, 10.0f, valid, code);
-
-        // cone features of the 4 directions (bilinear over the level-2 grid)
-        const int bx = (px - 1) >> 1, by = (py - 1) >> 1;
-        const int qx0 = min(max(bx, 0), a.QX - 1), qx1 = min(max(bx + 1, 0), a.QX - 1);
-        const int qy0 = min(max(by, 0), a.QY - 1), qy1 = min(max(by + 1, 0), a.QY - 1);
-        float3 cf[4];
+        const int qx = (tile % a.tiles_x) * 8 + (p & 7), qy = (tile / a.tiles_x) * 8 + (p >> 3);
+        const bool valid = qx < a.QX && qy < a.QY;
+        const size_t q = (size_t)min(qy, a.QY - 1) * a.QX + min(qx, a.QX - 1);
+        float4 cf[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) cf[j] = a.cone[q * 16 + 4 * ko + j];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const int k = 4 * ko + j;
-            if (valid) {
-                const float4 u00 = a.cone[((size_t)qy0 * a.QX + qx0) * 16 + k];
-                const float4 u10 = a.cone[((size_t)qy0 * a.QX + qx1) * 16 + k];
-                const float4 u01 = a.cone[((size_t)qy1 * a.QX + qx0) * 16 + k];
-                const float4 u11 = a.cone[((size_t)qy1 * a.QX + qx1) * 16 + k];
-                cf[j] = bilerp4(u00, u10, u01, u11, px, py);
-            } else {
-                cf[j] = make_float3(0.f, 0.f, 0.f);
-            }
-            const int col = 3 * k;
+            const int col = 3 * (4 * ko + j);
             xs[p * kSStrideIn + col + 0] =
                 (uint16_t)f32_to_bf16_rne(valid ? (feat_log(cf[j].x) - mean_s[col + 0]) * istd_s[col + 0] : 0.f);
             xs[p * kSStrideIn + col + 1] =
@@ -650,30 +582,17 @@ __global__ __launch_bounds__(kBlock) void k_surr_level1(SurrLevel1Args a, MlpDev
         }
         __syncthreads();
 
-        // compose the merged level-1 texel of directions 4ko..4ko+3
-        float rr[4], gg[4], bbv[4], al[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int k = 4 * ko + j;
-            if (code[j] >= 0) {
-                rr[j] = a.emissive[code[j]];
-                gg[j] = a.emissive[plane + code[j]];
-                bbv[j] = a.emissive[2 * plane + code[j]];
-                al[j] = 0.0f;
-            } else {
-                rr[j] = fmaxf(fmaf(a.out_scale, zs[p][3 * k + 0], cf[j].x), 0.0f);
-                gg[j] = fmaxf(fmaf(a.out_scale, zs[p][3 * k + 1], cf[j].y), 0.0f);
-                bbv[j] = fmaxf(fmaf(a.out_scale, zs[p][3 * k + 2], cf[j].z), 0.0f);
-                al[j] = 1.0f;
-            }
-        }
+        // the predicted merged level 2: 4 texels of directions 4ko..4ko+3
         if (valid) {
-            const float4 o = make_float4(((rr[0] + rr[1]) + (rr[2] + rr[3])) * 0.25f,
-                                         ((gg[0] + gg[1]) + (gg[2] + gg[3])) * 0.25f,
-                                         ((bbv[0] + bbv[1]) + (bbv[2] + bbv[3])) * 0.25f,
-                                         ((al[0] + al[1]) + (al[2] + al[3])) * 0.25f);
             typename TT::T *out = (typename TT::T *)a.out;
-            out[((size_t)py * a.PX + px) * 4 + ko] = TT::pack(o);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int k = 4 * ko + j;
+                const float4 t = make_float4(fmaxf(fmaf(a.out_scale, zs[p][3 * k + 0], cf[j].x), 0.0f),
+                                             fmaxf(fmaf(a.out_scale, zs[p][3 * k + 1], cf[j].y), 0.0f),
+                                             fmaxf(fmaf(a.out_scale, zs[p][3 * k + 2], cf[j].z), 0.0f), 1.0f);
+                out[q * 16 + k] = TT::pack(t);
+            }
         }
 
         // tile error estimate
@@ -703,56 +622,46 @@ __global__ __launch_bounds__(kBlock) void k_surr_level1(SurrLevel1Args a, MlpDev
     }
 }
 
-hipError_t launch_surr_level1(const SurrLevel1Args &a, const MlpDev &m, Storage st, hipStream_t s)
+hipError_t launch_surr_mlp(const SurrMlpArgs &a, const MlpDev &m, Storage st, hipStream_t s)
 {
-    // a multiple of 8 workgroups (one share per XCD), at most 4 per CU
-    int grid = a.tiles < 1024 ? ((a.tiles + 7) / 8) * 8 : 1024;
     if (a.tiles < 1) return hipSuccess;
+    // a multiple of 8 workgroups (one share per XCD), at most 4 per CU
+    const int grid = a.tiles < 1024 ? ((a.tiles + 7) / 8) * 8 : 1024;
     switch (st) {
-    case Storage::F32: hipLaunchKernelGGL(k_surr_level1<Storage::F32>, dim3(grid), dim3(kBlock), 0, s, a, m); break;
-    case Storage::F16: hipLaunchKernelGGL(k_surr_level1<Storage::F16>, dim3(grid), dim3(kBlock), 0, s, a, m); break;
-    case Storage::BF16: hipLaunchKernelGGL(k_surr_level1<Storage::BF16>, dim3(grid), dim3(kBlock), 0, s, a, m); break;
+    case Storage::F32: hipLaunchKernelGGL(k_surr_mlp<Storage::F32>, dim3(grid), dim3(kBlock), 0, s, a, m); break;
+    case Storage::F16: hipLaunchKernelGGL(k_surr_mlp<Storage::F16>, dim3(grid), dim3(kBlock), 0, s, a, m); break;
+    case Storage::BF16: hipLaunchKernelGGL(k_surr_mlp<Storage::BF16>, dim3(grid), dim3(kBlock), 0, s, a, m); break;
     }
     return hipGetLastError();
 }
 
 // ---- training samples ------------------------------------------------------------
-// One row per level-1 probe: inputs log1p(c) and target residual B - c, where c
-// is the cone radiance and B the plain path's far field (the level-2 merged
-// texels interpolated exactly as the level-1 merge reads them).
+// One row per level-2 probe: inputs log(1 + c) and target residual A_2 - c,
+// where c is the probe's cone radiance and A_2 the plain path's merged
+// level 2 (16 entries).
 template <Storage S>
 __global__ __launch_bounds__(kBlock) void k_surr_sample(SurrSampleArgs a)
 {
     using TT = Texel<S>;
-    const int probe = blockIdx.x * kBlock + threadIdx.x;
-    if (probe >= a.PX * a.PY) return;
-    const int px = probe % a.PX, py = probe / a.PX;
-    const int bx = (px - 1) >> 1, by = (py - 1) >> 1;
-    const int qx0 = min(max(bx, 0), a.QX - 1), qx1 = min(max(bx + 1, 0), a.QX - 1);
-    const int qy0 = min(max(by, 0), a.QY - 1), qy1 = min(max(by + 1, 0), a.QY - 1);
+    const int gi = blockIdx.x * kBlock + threadIdx.x;
+    if (gi >= a.QX * a.QY * 16) return;
+    const int k = gi & 15, q = gi >> 4;
     const typename TT::T *up = (const typename TT::T *)a.upper;
-    for (int k = 0; k < 16; ++k) {
-        const float3 c = bilerp4(a.cone[((size_t)qy0 * a.QX + qx0) * 16 + k], a.cone[((size_t)qy0 * a.QX + qx1) * 16 + k],
-                                 a.cone[((size_t)qy1 * a.QX + qx0) * 16 + k], a.cone[((size_t)qy1 * a.QX + qx1) * 16 + k],
-                                 px, py);
-        const float3 u = bilerp4(TT::unpack(up[((size_t)qy0 * a.QX + qx0) * 16 + k]),
-                                 TT::unpack(up[((size_t)qy0 * a.QX + qx1) * 16 + k]),
-                                 TT::unpack(up[((size_t)qy1 * a.QX + qx0) * 16 + k]),
-                                 TT::unpack(up[((size_t)qy1 * a.QX + qx1) * 16 + k]), px, py);
-        float *f = a.feats + (size_t)probe * kMlpIn + 3 * k;
-        float *t = a.targets + (size_t)probe * kMlpRad + 3 * k;
-        f[0] = feat_log(c.x);
-        f[1] = feat_log(c.y);
-        f[2] = feat_log(c.z);
-        t[0] = u.x - c.x;
-        t[1] = u.y - c.y;
-        t[2] = u.z - c.z;
-    }
+    const float4 c = a.cone[gi];
+    const float4 u = TT::unpack(up[gi]);
+    float *f = a.feats + (size_t)q * kMlpIn + 3 * k;
+    float *t = a.targets + (size_t)q * kMlpRad + 3 * k;
+    f[0] = feat_log(c.x);
+    f[1] = feat_log(c.y);
+    f[2] = feat_log(c.z);
+    t[0] = u.x - c.x;
+    t[1] = u.y - c.y;
+    t[2] = u.z - c.z;
 }
 
 hipError_t launch_surr_sample(const SurrSampleArgs &a, Storage st, hipStream_t s)
 {
-    const int n = a.PX * a.PY;
+    const int n = a.QX * a.QY * 16;
     const dim3 g((n + kBlock - 1) / kBlock);
     switch (st) {
     case Storage::F32: hipLaunchKernelGGL(k_surr_sample<Storage::F32>, g, dim3(kBlock), 0, s, a); break;
@@ -763,6 +672,7 @@ hipError_t launch_surr_sample(const SurrSampleArgs &a, Storage st, hipStream_t s
 }
 
 // ---- fallback patch: flagged tiles take the plain path's merged level 1 ----------
+// A tile is 8x8 level-2 probes = 16x16 level-1 probes.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_tile_patch(const T *__restrict__ plain, T *__restrict__ surr,
                                                        const float *__restrict__ tile_err, float threshold, int PX,
@@ -771,20 +681,20 @@ __global__ __launch_bounds__(kBlock) void k_tile_patch(const T *__restrict__ pla
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= PX * PY * 4) return;
     const int probe = i >> 2, px = probe % PX, py = probe / PX;
-    if (tile_err[(py >> 3) * tiles_x + (px >> 3)] > threshold) surr[i] = plain[i];
+    if (tile_err[(py >> 4) * tiles_x + (px >> 4)] > threshold) surr[i] = plain[i];
 }
 
 hipError_t launch_tile_patch(const void *plain, void *surr, const float *tile_err, float threshold, int PX, int PY,
-                             Storage st, hipStream_t s)
+                             int tiles_x, Storage st, hipStream_t s)
 {
-    const int n = PX * PY * 4, tx = (PX + 7) / 8;
+    const int n = PX * PY * 4;
     const dim3 g((n + kBlock - 1) / kBlock);
     if (st == Storage::F32)
         hipLaunchKernelGGL(k_tile_patch<float4>, g, dim3(kBlock), 0, s, (const float4 *)plain, (float4 *)surr, tile_err,
-                           threshold, PX, PY, tx);
+                           threshold, PX, PY, tiles_x);
     else
         hipLaunchKernelGGL(k_tile_patch<uint2>, g, dim3(kBlock), 0, s, (const uint2 *)plain, (uint2 *)surr, tile_err,
-                           threshold, PX, PY, tx);
+                           threshold, PX, PY, tiles_x);
     return hipGetLastError();
 }
 

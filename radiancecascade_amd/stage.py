@@ -304,8 +304,8 @@ class RadianceCascadeStage:
                                     _stream_handle(stream)), self._ctx)
 
     def surrogate_samples(self, depth, emissive, feats, targets, stream=None) -> None:
-        """Training rows of one device G-buffer: feats/targets [Px_1*Py_1][48]."""
-        px, py = probe_grid(self.width, self.height, 1)
+        """Training rows of one device G-buffer: feats/targets [Px_2*Py_2][48]."""
+        px, py = probe_grid(self.width, self.height, 2)
         n = px * py
         g = _abi.GBuffer(_dev_ptr(depth, self.width * self.height * 4, "depth"), None,
                          _dev_ptr(emissive, 3 * self.width * self.height * 4, "emissive"))
@@ -328,7 +328,7 @@ class RadianceCascadeStage:
     def surrogate_shapes(self) -> dict:
         px, py = probe_grid(self.width, self.height, 1)
         qx, qy = probe_grid(self.width, self.height, 2)
-        return {"level1": (py, px), "level2": (qy, qx), "tiles": ((py + 7) // 8, (px + 7) // 8)}
+        return {"level1": (py, px), "level2": (qy, qx), "tiles": ((qy + 7) // 8, (qx + 7) // 8)}
 
     def debug_surrogate(self, cone=None, level1=None, tile_err=None, stream=None) -> None:
         sh = self.surrogate_shapes()

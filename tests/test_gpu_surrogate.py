@@ -183,7 +183,7 @@ def test_training_rows_parity(rc):
     from oracle import surrogate as so
     W, H, L = 160, 128, 5
     depth, emis = _gbuffer(rc, W, H, 9, 8, 2)
-    px, py = rc.probe_grid(W, H, 1)
+    px, py = rc.probe_grid(W, H, 2)
     feats = torch.empty((px * py, 48), dtype=torch.float32, device="cuda:0")
     targ = torch.empty((px * py, 48), dtype=torch.float32, device="cuda:0")
     with rc.RadianceCascadeStage(W, H, levels=L) as st:
@@ -193,7 +193,7 @@ def test_training_rows_parity(rc):
     _, lv = rco.run(d, e, L, want_levels=True)
     s0 = rco.seed_init(d, e)
     cone = so.cone_trace(so.mip(s0, e), W, H)
-    x, t = so.samples(cone, lv[2], W, H)
+    x, t = so.samples(cone, lv[2])
     rx, rt = rel_l2(feats.cpu().numpy(), x), rel_l2(targ.cpu().numpy(), t)
     _say(f"  rows: features rel-L2 {rx:.2e}  targets rel-L2 {rt:.2e}")
     assert rx <= 1e-4 and rt <= 5e-3
@@ -235,10 +235,10 @@ def test_surrogate_graph_replay_matches_direct_launches(rc):
 
 def test_training_reduces_loss(rc):
     with rc.RadianceCascadeStage(64, 64, levels=3) as st:
-        rep = st.train_surrogate(width=128, height=128, levels=5, scene_seed0=10000, scenes=6, epochs=4,
+        rep = st.train_surrogate(width=128, height=128, levels=5, scene_seed0=10000, scenes=24, epochs=4,
                                  batch=512, lr=2e-3, lr_decay=0.8, seed=1)
         _say(f"  losses {rep['epoch_loss']}  rows {rep['samples']}  scale {rep['out_scale']:.4f}")
-        assert rep["samples"] == 6 * 32 * 32 and rep["steps"] == 4 * (6 * 1024 // 512)
+        assert rep["samples"] == 24 * 16 * 16 and rep["steps"] == 4 * (24 * 256 // 512)
         assert all(math.isfinite(v) for v in rep["epoch_loss"])
         assert rep["epoch_loss"][-1] < rep["epoch_loss"][0]
         blob = st.weights_blob()

@@ -1,7 +1,7 @@
 """Pins for the learned-far-field oracle (oracle/surrogate.py): a library
 EDT, block means taken directly from pixels, closed forms and symmetries of
-the cone trace, and the reduction of the level-1 composition to the plain
-merge of oracle/rco.c when the far field is the exact one."""
+the cone trace, and the frame's reduction to the plain path when the
+network's level 2 is the plain one."""
 import numpy as np
 import pytest
 
@@ -77,7 +77,7 @@ def test_mip_equals_direct_block_means():
 def test_cone_schedule_shape():
     s = so.cone_schedule(1920, 1080)
     ts = [t for t, _, _ in s]
-    assert ts[0] == 6.0 and all(b > a for a, b in zip(ts, ts[1:]))
+    assert ts[0] == 10.0 and all(b > a for a, b in zip(ts, ts[1:]))
     assert ts[-1] < np.hypot(1920, 1080) + 2 <= ts[-1] + 0.75 * max(2 * ts[-1] * np.tan(np.pi / 16), 1)
     assert all(1 <= m <= so.top_level(1920, 1080) for _, m, _ in s)
     assert all(ln <= 2 ** m for _, m, ln in s)
@@ -124,26 +124,32 @@ def test_cone_linear_in_emission_and_mirror_symmetric():
     assert c1[..., :3].sum() > 0
 
 
-def test_compose_with_exact_far_field_is_the_plain_merge():
+def test_exact_level2_reproduces_the_plain_frame():
+    # with the network's output equal to the plain level 2 (c + s z = A2), the
+    # frame is the plain frame's levels 1 and 0
     W, H, L = 96, 80, 5
     d, _, e = rco.gbuffer(9, W, H, 8, 2)
     irr, lv = rco.run(d, e, L, want_levels=True)
+    QX, QY = rco.probe_grid(W, H, 2)
+    c = np.zeros((QX * QY, 48))
+    A2 = so.predicted_level2(c, lv[2][..., :3].reshape(-1, 48), 1.0, QX, QY)
+    assert np.array_equal(A2[..., :3], lv[2][..., :3])
     sdf = rco.sdf_f64(rco.jfa(rco.seed_init(d, e)))
     m1, _ = rco.march_level(sdf, e, 1)
-    PX, PY = rco.probe_grid(W, H, 1)
-    B = so.upper_bilinear(lv[2], PX, PY).reshape(-1, 48)
-    A1 = so.compose_level1(m1, B, np.zeros((B.shape[0], 64)), 1.0)
-    assert np.allclose(A1, rco.merge_level(W, H, 1, m1, lv[2]), rtol=1e-12, atol=1e-12)
+    m0, _ = rco.march_level(sdf, e, 0)
+    A1 = rco.merge_level(W, H, 1, m1, A2)
     assert np.allclose(A1, lv[1], rtol=1e-12, atol=1e-12)
+    assert np.allclose(rco.gather(W, H, rco.merge_level(W, H, 0, m0, A1)), irr, rtol=1e-12, atol=1e-12)
 
 
 def test_samples_and_params_roundtrip():
     rng = np.random.default_rng(0)
     cone = rng.random((5, 7, 16, 4))
     A2 = rng.random((5, 7, 16, 4))
-    x, t = so.samples(cone, A2, 7 * 8 - 3, 5 * 8 - 5)
-    c, _ = so.features(cone, 7 * 8 - 3, 5 * 8 - 5)
-    assert np.allclose(np.expm1(x) + t, so.upper_bilinear(A2, *rco.probe_grid(53, 35, 1)).reshape(-1, 48))
+    x, t = so.samples(cone, A2)
+    c, _ = so.features(cone)
+    assert x.shape == (35, 48) and np.allclose(np.expm1(x) + t, A2[..., :3].reshape(-1, 48))
+    assert np.array_equal(c, cone[..., :3].reshape(-1, 48))
     from oracle import mlp
     layers = mlp.init_params(3)
     p = so.layers_to_params(layers)
