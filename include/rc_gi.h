@@ -264,9 +264,11 @@ rc_status rc_surrogate_train(rc_ctx *ctx, const rc_train_config *cfg, rc_train_r
  * be NULL): cone [Py_2][Px_2][16][4] (rgb, transmittance); level1 [Py_1][Px_1][4][4]
  * merged level 1 (fp32); tile_err [ceil(Py_2/8)][ceil(Px_2/8)]. */
 rc_status rc_debug_surrogate(rc_ctx *ctx, float *cone_dev, float *level1_dev, float *tile_err_dev, void *stream);
-/* Mip level m >= 1 of the last learned-far-field frame: emc [h][w][4]
- * (mean E*[emitter] rgb, mean [emitter]) and mo [h][w] (mean [occluder]). */
-rc_status rc_debug_mip(rc_ctx *ctx, uint32_t level, float *emc_dev, float *mo_dev, void *stream);
+/* Mip level m >= 1 of the last learned-far-field frame: emission [h][w][4]
+ * (mean E*[emitter] rgb, emitter area fraction) and coverage [h][w][4] (the
+ * fractions of each cell's pixel columns and rows holding an emitter pixel,
+ * then the same for dark occluder pixels). */
+rc_status rc_debug_mip(rc_ctx *ctx, uint32_t level, float *emission_dev, float *coverage_dev, void *stream);
 
 /* ---- batches of views (one graph launch) ------------------------------------
  * Independent views of uneven sizes (e.g. the sub-views of a renderer's

@@ -7,10 +7,11 @@ radiancecascade_amd/csrc/rc_surrogate.hip; the product never imports it.
 The reference repository has no global illumination and no learned part
 (its only pass is the forward raster, src/renderer.rs:559-613); the rules
 restated here are the project's own and are pinned in
-tests/test_oracle_surrogate.py by a library EDT (scipy), block means computed
-directly from pixels, closed forms of the cone trace (empty scene, fully
-emissive scene, linearity, mirror symmetry); the composition is the plain
-merge of oracle/rco.c itself.
+tests/test_oracle_surrogate.py by a library EDT (scipy), block means and
+row / column fractions computed directly from pixels (exact for lines and
+single pixels, a lower bound elsewhere), closed forms of the cone trace
+(empty scene, fully emissive scene, a line across the cone, linearity,
+mirror symmetry); the composition is the plain merge of oracle/rco.c itself.
 """
 from __future__ import annotations
 
@@ -25,6 +26,7 @@ N_DIRS = 16                      # level-1 directions = cones per level-2 probe
 CONE_T0 = 10.0                   # the start of level 2's interval
 CONE_GROW = 0.75
 CONE_SPLIT = 60.0                # near segment from level-2 probes, far from level-3
+CONE_OCC_GAIN = 3.0              # occluder exponent relative to the emitter one
 T_MIN = 1e-4
 
 
@@ -60,49 +62,60 @@ def top_level(W: int, H: int) -> int:
 
 
 def pixel_fields(seed_init_field: np.ndarray, emissive: np.ndarray):
-    """(E*[emitter] [3,H,W], [emitter] [H,W], [occupied and not emitter] [H,W])."""
-    e = np.asarray(emissive, np.float64)
+    """(E*[emitter] [3,H,W] fp32, [emitter] [H,W] bool, [occupied and not emitter] [H,W] bool)."""
+    e = np.asarray(emissive, np.float32)
     em = (e > 0).any(axis=0)
     occ = seed_init_field != rco.SEED_NONE
-    return e * em[None], em.astype(np.float64), (occ & ~em).astype(np.float64)
-
-
-def _pool(a: np.ndarray) -> np.ndarray:
-    """2x2 means in the array's own precision, zero outside."""
-    h, w = a.shape[-2:]
-    h2, w2 = (h + 1) // 2, (w + 1) // 2
-    pad = [(0, 0)] * (a.ndim - 2) + [(0, 2 * h2 - h), (0, 2 * w2 - w)]
-    a = np.pad(a, pad)
-    q = a.dtype.type(0.25)
-    return q * ((a[..., 0::2, 0::2] + a[..., 0::2, 1::2]) + (a[..., 1::2, 0::2] + a[..., 1::2, 1::2]))
+    return e * em[None], em, occ & ~em
 
 
 def _f16(a: np.ndarray) -> np.ndarray:
     return np.asarray(a, np.float32).astype(np.float16).astype(np.float32)
 
 
+def _quads(a: np.ndarray):
+    """The 2x2 children (x0y0, x1y0, x0y1, x1y1) of every parent cell, zero outside."""
+    h, w = a.shape[-2:]
+    h2, w2 = (h + 1) // 2, (w + 1) // 2
+    a = np.pad(a, [(0, 0)] * (a.ndim - 2) + [(0, 2 * h2 - h), (0, 2 * w2 - w)])
+    return a[..., 0::2, 0::2], a[..., 0::2, 1::2], a[..., 1::2, 0::2], a[..., 1::2, 1::2]
+
+
 def mip(seed_init_field: np.ndarray, emissive: np.ndarray):
-    """levels[m] for m = 1..top: (emc [h][w][4] = mean E*[em] rgb, [em] coverage;
-    mo [h][w] = [occluder] coverage).  Radiance means are exact (FP64);
-    coverages are stored as fp16: each level's is the fp32 mean of its 4
-    stored children, ((x0y0 + x1y0) + (x0y1 + x1y1)) / 4, rounded to fp16
-    (exact up to level 5, where the means are multiples of 4^-5)."""
+    """levels[m] for m = 1..top: (emission [h][w][4], coverage [h][w][4]).
+
+    Per cell of 2^m x 2^m pixels: emission = (mean E*[emitter] rgb, emitter
+    area fraction); coverage = (ex, ey, ox, oy), the fraction of the cell's
+    pixel columns (x) and rows (y) holding an emitter (e) or a dark occluder
+    (o) pixel.  A pixel is a level-0 cell with all of these 0 or 1.  A parent
+    is computed in fp32 from its 4 stored children TL, TR, BL, BR:
+        means  ((TL + TR) + (BL + BR)) / 4
+        x      (max(TL, BL) + max(TR, BR)) / 2   (a column of the left half
+                                                  is held if either child holds it)
+        y      (max(TL, TR) + max(BL, BR)) / 2
+    and every value is stored as fp16 (round to nearest even).  The max is a
+    lower bound of the union, exact when the children's held columns nest
+    (one line, one block)."""
     H, W = seed_init_field.shape
-    EM, ME, MO = pixel_fields(seed_init_field, emissive)
-    rgb = EM
-    cov = np.stack([ME, MO]).astype(np.float32)
+    EM, em, oc = pixel_fields(seed_init_field, emissive)
+    f = np.concatenate([EM, em[None], em[None], em[None], oc[None], oc[None]]).astype(np.float32)
+    q, h = np.float32(0.25), np.float32(0.5)
     levels = [None]
     for _ in range(1, top_level(W, H) + 1):
-        rgb = _pool(rgb)
-        cov = _f16(_pool(cov))
-        levels.append((np.concatenate([np.moveaxis(rgb, 0, -1), cov[0][..., None].astype(np.float64)], -1),
-                       cov[1].astype(np.float64)))
+        TL, TR, BL, BR = _quads(f)
+        mean = q * ((TL[:4] + TR[:4]) + (BL[:4] + BR[:4]))
+        x = h * (np.maximum(TL[[4, 6]], BL[[4, 6]]) + np.maximum(TR[[4, 6]], BR[[4, 6]]))
+        y = h * (np.maximum(TL[[5, 7]], TR[[5, 7]]) + np.maximum(BL[[5, 7]], BR[[5, 7]]))
+        f = _f16(np.concatenate([mean, x[0:1], y[0:1], x[1:2], y[1:2]]))
+        levels.append((np.moveaxis(f[:4], 0, -1).astype(np.float64), np.moveaxis(f[4:], 0, -1).astype(np.float64)))
     return levels
 
 
 # ---- cone trace (k_cone) ------------------------------------------------------
 def cone_schedule(W: int, H: int):
-    """[(t, m, len)] as the host builds it (double, then rounded to fp32)."""
+    """[(t, m, re, ro)] as the host builds it (double, then rounded to fp32):
+    re = len / 2^m and ro = CONE_OCC_GAIN * len / 2^m are the emitter and
+    occluder exponents of the step, len = min(step, 2^m)."""
     top = top_level(W, H)
     tmax = math.sqrt(float(W) * W + float(H) * H) + 2.0
     tn = math.tan(math.pi / 16.0)
@@ -111,22 +124,25 @@ def cone_schedule(W: int, H: int):
         dia = max(2.0 * t * tn, 1.0)
         m = min(max(int(math.floor(math.log2(dia))), 1), top)
         step = CONE_GROW * dia
-        out.append((float(np.float32(t)), m, float(np.float32(min(step, float(1 << m))))))
+        cell = float(1 << m)
+        r = min(step, cell) / cell
+        out.append((float(np.float32(t)), m, float(np.float32(r)), float(np.float32(CONE_OCC_GAIN * r))))
         t += step
     return out
 
 
 def _sample(level, x, y, m):
     """Bilinear sample of level m at pixel positions (x, y); cells outside the
-    level are zero (the stored one-cell border).  Returns (emc [..,4], mo,
-    inside) where `inside` is false once the footprint has left the image
-    (lower cell index outside [-1, w-1] x [-1, h-1]; the cone then stops)."""
-    emc, mo = level
-    h, w = mo.shape
+    level are zero (the stored one-cell border).  Returns (emission [..,4],
+    coverage [..,4], inside) where `inside` is false once the footprint has
+    left the image (lower cell index outside [-1, w-1] x [-1, h-1]; the cone
+    then stops)."""
+    em, cov = level
+    h, w = em.shape[:2]
     pe = np.zeros((h + 2, w + 2, 4))
-    pe[1:-1, 1:-1] = emc
-    po = np.zeros((h + 2, w + 2))
-    po[1:-1, 1:-1] = mo
+    pe[1:-1, 1:-1] = em
+    pc = np.zeros((h + 2, w + 2, 4))
+    pc[1:-1, 1:-1] = cov
     s = float(1 << m)
     fx = x / s - 0.5
     fy = y / s - 0.5
@@ -135,20 +151,28 @@ def _sample(level, x, y, m):
     ax, ay = fx - x0, fy - y0
     inside = (x0 >= -1) & (x0 < w) & (y0 >= -1) & (y0 < h)
     e = np.zeros(x.shape + (4,))
-    o = np.zeros(x.shape)
+    c = np.zeros(x.shape + (4,))
     for dy, wy in ((0, 1 - ay), (1, ay)):
         for dx, wx in ((0, 1 - ax), (1, ax)):
             xc, yc = np.clip(x0 + 1 + dx, 0, w + 1), np.clip(y0 + 1 + dy, 0, h + 1)
-            wt = wx * wy
-            e += pe[yc, xc] * wt[..., None]
-            o += po[yc, xc] * wt
-    return e, o, inside
+            wt = (wx * wy)[..., None]
+            e += pe[yc, xc] * wt
+            c += pc[yc, xc] * wt
+    return e, c, inside
 
 
 def cone_trace_grid(levels, W: int, H: int, level: int, schedule) -> np.ndarray:
     """[QY][QX][16][4] cones from the probes of `level` (spacing 2^(level+1))
-    along the given schedule: radiance rgb and transmittance.  A cone stops
-    once T < 1e-4 or once its footprint has left the image."""
+    along the given schedule: radiance rgb and transmittance.
+
+    Per sample (bilinear in mip level m): the fraction of the cone's parallel
+    rays that a cell stops is taken from its column and row fractions with
+    the direction's weights, c = dy^2 x + dx^2 y (a line across the cone stops
+    it whatever its area); emitters and occluders give opacities
+    ae = 1 - (1 - min(ce, .999))^re and ao = 1 - (1 - min(co, .999))^ro, a =
+    min(ae + ao, 1); L += T ae Ebar with Ebar = E-mean / emitter area (the
+    mean radiance of the footprint's emitter pixels), T *= 1 - a.  A cone
+    stops once T < 1e-4 or once its footprint has left the image."""
     QX, QY = rco.probe_grid(W, H, level)
     sp = float(2 << level)
     dirs = rco.dir_table_f32(1).astype(np.float64)
@@ -161,12 +185,16 @@ def cone_trace_grid(levels, W: int, H: int, level: int, schedule) -> np.ndarray:
         T = np.ones(cx.shape)
         L = np.zeros(cx.shape + (3,))
         live = np.ones(cx.shape, bool)
-        for t, m, ln in schedule:
-            e, o, inside = _sample(levels[m], t * dx + cx, t * dy + cy, m)
+        for t, m, re, ro in schedule:
+            e, c, inside = _sample(levels[m], t * dx + cx, t * dy + cy, m)
             live = live & inside
-            ao = 1.0 - (1.0 - np.minimum(o, 0.999)) ** ln
-            a = np.minimum(e[..., 3] + ao, 1.0)
-            L = np.where(live[..., None], L + T[..., None] * e[..., :3], L)
+            ce = dy * dy * c[..., 0] + dx * dx * c[..., 1]
+            co = dy * dy * c[..., 2] + dx * dx * c[..., 3]
+            ae = 1.0 - (1.0 - np.minimum(ce, 0.999)) ** re
+            ao = 1.0 - (1.0 - np.minimum(co, 0.999)) ** ro
+            a = np.minimum(ae + ao, 1.0)
+            ebar = np.where(e[..., 3:4] > 0, e[..., :3] / np.where(e[..., 3:4] > 0, e[..., 3:4], 1.0), 0.0)
+            L = np.where(live[..., None], L + (T * ae)[..., None] * ebar, L)
             T = np.where(live, T * (1.0 - a), T)
             live = live & (T >= T_MIN)
         out[:, :, k, :3] = L

@@ -408,8 +408,8 @@ static rc_status ensure_surrogate(rc_ctx *c)
     cone_schedule(c->W, c->H, c->mip, c->sch, c->sch_far);
     const size_t cells = mip_cells(c->mip);
     if (cells * sizeof(float4) > 0x7FFFFFFFull) return fail(c, RC_ERR_INVALID, "learned far field: image too large");
-    HIP_TRY(c, hipMalloc(&c->mip.tex, cells * sizeof(float4)));
-    HIP_TRY(c, hipMemset(c->mip.tex, 0, cells * sizeof(float4)));  // the zero borders
+    HIP_TRY(c, hipMalloc(&c->mip.tex, cells * sizeof(uint4)));
+    HIP_TRY(c, hipMemset(c->mip.tex, 0, cells * sizeof(uint4)));  // the zero borders
     HIP_TRY(c, hipMalloc(&c->occ_bits, (size_t)c->H * ((c->W + 63) / 64) * sizeof(uint64_t)));
     const int QX = grid_dim(c->W, 2), QY = grid_dim(c->H, 2);
     HIP_TRY(c, hipMalloc(&c->cone, (size_t)QX * QY * 16 * sizeof(float4)));
@@ -1444,7 +1444,7 @@ rc_status rc_debug_surrogate(rc_ctx *c, float *cone, float *level1, float *tile_
     return RC_OK;
 }
 
-rc_status rc_debug_mip(rc_ctx *c, uint32_t level, float *emc, float *mo, void *stream)
+rc_status rc_debug_mip(rc_ctx *c, uint32_t level, float *emission, float *coverage, void *stream)
 {
     if (!c) return fail(nullptr, RC_ERR_NULL, "rc_debug_mip: ctx is NULL");
     if (!c->surr_alloc || !c->surr_ran) return fail(c, RC_ERR_STATE, "rc_debug_mip: no learned-far-field frame yet");
@@ -1452,7 +1452,7 @@ rc_status rc_debug_mip(rc_ctx *c, uint32_t level, float *emc, float *mo, void *s
     hipStream_t s = (hipStream_t)stream;
     HIP_TRY(c, hipSetDevice(c->cfg.device));
     const size_t n = (size_t)c->mip.w[level] * c->mip.h[level];
-    if (n && (emc || mo)) HIP_TRY(c, launch_mip_unpack(c->mip, (int)level, emc, mo, s));
+    if (n && (emission || coverage)) HIP_TRY(c, launch_mip_unpack(c->mip, (int)level, emission, coverage, s));
     return RC_OK;
 }
 

@@ -198,98 +198,126 @@ __global__ __launch_bounds__(kBlock) void k_level(LevelArgs a)
 }
 
 // Level 0 (4 directions, interval [0,2)) fused with the gather.  A tile owns
-// 16x16 level-0 probes = 32x32 pixels; it marches the tile's probes plus a
-// 1-probe halo (the pixels' bilinear footprint), reduces each probe's 4
-// directions to its fluence in LDS, and writes planar fp32 irradiance.
+// 16x12 level-0 probes = 32x24 pixels (each output row one 128-byte line); a
+// thread marches one probe of the tile plus its 1-probe halo (18x14 = 252
+// probes, the pixels' bilinear footprint), all 4 directions at once, and
+// writes the probe's fluence to LDS; the pixels then read it bilinearly and
+// write planar fp32 irradiance.
+//
+// Every direction of a probe starts at the probe centre (t = 0), so the first
+// sample is shared.  Distances are whole pixels below 128, hence a first
+// value of 0 is a hit for all 4 rays, a value >= 2 ends all 4 intervals, and
+// only a value of exactly 1 takes a second, per-direction sample at t = 1,
+// after which every ray has ended (1 + max(d, 1) >= 2).  These are the steps
+// the generic interval march takes, so the codes are the same; the fluence
+// is the same sum ((v0 + v1) + (v2 + v3)) / 4 of the per-direction values.
 template <Storage S, bool UPPER>
 __global__ __launch_bounds__(kBlock) void k_level0(LevelArgs a, float *__restrict__ irr, float *__restrict__ dbg)
 {
     using TT = Texel<S>;
-    constexpr int TP = 16, NS = TP + 2, UC = TP / 2 + 2;
-    constexpr int RAYS = NS * NS * 4, ROUNDS = (RAYS + kBlock - 1) / kBlock;
-    __shared__ float4 su[UPPER ? UC * UC * 4 : 1];
-    __shared__ float4 fl[NS * NS];
+    constexpr int TX = 16, TY = 12, NX = TX + 2, NY = TY + 2, UCX = TX / 2 + 2, UCY = TY / 2 + 2;
+    static_assert(NX * NY <= kBlock, "one probe per thread");
+    __shared__ float4 su[UPPER ? UCX * UCY * 4 : 1];
+    __shared__ float4 fl[NX * NY];
 
-    const int tiles_x = (a.PX + TP - 1) / TP;
+    const int tiles_x = (a.PX + TX - 1) / TX;
     const int lb = xcd_remap_c(blockIdx.x, gridDim.x);
-    const int px0 = (lb % tiles_x) * TP, py0 = (lb / tiles_x) * TP;
+    const int px0 = (lb % tiles_x) * TX, py0 = (lb / tiles_x) * TY;
     const int ulx = (px0 - 2) >> 1, uly = (py0 - 2) >> 1;
     if (UPPER) {
         const typename TT::T *up = (const typename TT::T *)a.upper;
-        for (int i = threadIdx.x; i < UC * UC * 4; i += kBlock) {
-            const int j = i & 3, sx = (i >> 2) % UC, sy = (i >> 2) / UC;
+        for (int i = threadIdx.x; i < UCX * UCY * 4; i += kBlock) {
+            const int j = i & 3, sx = (i >> 2) % UCX, sy = (i >> 2) / UCX;
             const int ux = min(max(ulx + sx, 0), a.UX - 1), uy = min(max(uly + sy, 0), a.UY - 1);
             su[i] = TT::unpack(up[((size_t)uy * a.UX + ux) * 4 + j]);
         }
         __syncthreads();
     }
 
-    const __amdgpu_buffer_rsrc_t sdf = sdf_rsrc(a.sdf, a.W, a.H);
     const size_t plane = (size_t)a.W * a.H;
-    // the lane's ROUNDS rays (same direction k: 256 rays per round) marched in
-    // lockstep, so their dependent distance-field loads overlap
-    const int k = threadIdx.x & 3;
-    const float2 dir = a.dirs[k];
-    float cxs[ROUNDS], cys[ROUNDS];
-    bool acts[ROUNDS];
-    int codes[ROUNDS];
+    const int slot = threadIdx.x;
+    const int sx = slot % NX, sy = slot / NX;
+    const int px = px0 - 1 + sx, py = py0 - 1 + sy;
+    if (slot < NX * NY && px >= 0 && py >= 0 && px < a.PX && py < a.PY) {
+        const __amdgpu_buffer_rsrc_t sdf = sdf_rsrc(a.sdf, a.W, a.H);
+        const int tiles = sdf_tiles_x(a.W);
+        const float cx = ((float)px + 0.5f) * 2.0f, cy = ((float)py + 0.5f) * 2.0f;
+        int code[4] = {kRayT1, kRayT1, kRayT1, kRayT1};
+        const int ix = __float2int_rd(cx), iy = __float2int_rd(cy);
+        if (!((uint32_t)ix < (uint32_t)a.W && (uint32_t)iy < (uint32_t)a.H)) {
 #pragma unroll
-    for (int round = 0; round < ROUNDS; ++round) {
-        const int slot = (round * kBlock + (int)threadIdx.x) >> 2;
-        const int px = px0 - 1 + slot % NS, py = py0 - 1 + slot / NS;
-        acts[round] = slot < NS * NS && px >= 0 && py >= 0 && px < a.PX && py < a.PY;
-        cxs[round] = ((float)px + 0.5f) * 2.0f;
-        cys[round] = ((float)py + 0.5f) * 2.0f;
-    }
-    march_rays<ROUNDS>(sdf, a.W, a.H, cxs, cys, dir.x, dir.y, 0.0f, 2.0f, acts, codes);
+            for (int k = 0; k < 4; ++k) code[k] = kRayExit;
+        } else {
+            const float d0 = sdf_load(sdf, ix, iy, tiles);
+            if (d0 < 0.5f) {
+                const int q0 = pixel_index(ix, iy, a.W);
 #pragma unroll
-    for (int round = 0; round < ROUNDS; ++round) {
-        const int ray = round * kBlock + threadIdx.x;
-        const int slot = ray >> 2;
-        const int px = px0 - 1 + slot % NS, py = py0 - 1 + slot / NS;
-        const bool active = acts[round];
-        const int code = codes[round];
-        float r = 0.f, g = 0.f, b = 0.f, al = 0.f;
-        if (active) {
-            if (code >= 0) {
-                r = a.emissive[code];
-                g = a.emissive[plane + code];
-                b = a.emissive[2 * plane + code];
-            } else {
-                al = 1.0f;
-            }
-            if (UPPER && code < 0) {
-                const int sx = ((px - 1) >> 1) - ulx, sy = ((py - 1) >> 1) - uly;
-                const float4 u00 = su[(sy * UC + sx) * 4 + k];
-                const float4 u10 = su[(sy * UC + sx + 1) * 4 + k];
-                const float4 u01 = su[((sy + 1) * UC + sx) * 4 + k];
-                const float4 u11 = su[((sy + 1) * UC + sx + 1) * 4 + k];
-                const float3 v = bilerp4(u00, u10, u01, u11, px, py);
-                r = v.x;
-                g = v.y;
-                b = v.z;
+                for (int k = 0; k < 4; ++k) code[k] = q0;
+            } else if (march_step(d0) < 2.0f) {
+                const float t = march_step(d0);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const float2 dir = a.dirs[k];
+                    const int jx = __float2int_rd(fmaf(t, dir.x, cx)), jy = __float2int_rd(fmaf(t, dir.y, cy));
+                    if (!((uint32_t)jx < (uint32_t)a.W && (uint32_t)jy < (uint32_t)a.H)) {
+                        code[k] = kRayExit;
+                    } else if (sdf_load(sdf, jx, jy, tiles) < 0.5f) {
+                        code[k] = pixel_index(jx, jy, a.W);
+                    }
+                }
             }
         }
-        r = quad_sum(r) * 0.25f;
-        g = quad_sum(g) * 0.25f;
-        b = quad_sum(b) * 0.25f;
-        al = quad_sum(al) * 0.25f;
-        if (active && k == 0) {
-            fl[slot] = make_float4(r, g, b, al);
-            const int sx = slot % NS, sy = slot / NS;
-            if (dbg && sx >= 1 && sx <= TP && sy >= 1 && sy <= TP)
-                reinterpret_cast<float4 *>(dbg)[(size_t)py * a.PX + px] = make_float4(r, g, b, al);
+        float r[4], g[4], b[4], al[4];
+        if (code[0] >= 0 && code[0] == code[1] && code[0] == code[2] && code[0] == code[3]) {
+            // one hit pixel for all 4 directions: load its emission once
+            const float e0 = a.emissive[code[0]], e1 = a.emissive[plane + code[0]], e2 = a.emissive[2 * plane + code[0]];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                r[k] = e0;
+                g[k] = e1;
+                b[k] = e2;
+                al[k] = 0.f;
+            }
+        } else {
+            const int usx = ((px - 1) >> 1) - ulx, usy = ((py - 1) >> 1) - uly;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (code[k] >= 0) {
+                    r[k] = a.emissive[code[k]];
+                    g[k] = a.emissive[plane + code[k]];
+                    b[k] = a.emissive[2 * plane + code[k]];
+                    al[k] = 0.f;
+                } else {
+                    al[k] = 1.0f;
+                    r[k] = g[k] = b[k] = 0.f;
+                    if (UPPER) {
+                        const float4 u00 = su[(usy * UCX + usx) * 4 + k];
+                        const float4 u10 = su[(usy * UCX + usx + 1) * 4 + k];
+                        const float4 u01 = su[((usy + 1) * UCX + usx) * 4 + k];
+                        const float4 u11 = su[((usy + 1) * UCX + usx + 1) * 4 + k];
+                        const float3 v = bilerp4(u00, u10, u01, u11, px, py);
+                        r[k] = v.x;
+                        g[k] = v.y;
+                        b[k] = v.z;
+                    }
+                }
+            }
         }
+        const float4 f = make_float4(((r[0] + r[1]) + (r[2] + r[3])) * 0.25f, ((g[0] + g[1]) + (g[2] + g[3])) * 0.25f,
+                                     ((b[0] + b[1]) + (b[2] + b[3])) * 0.25f,
+                                     ((al[0] + al[1]) + (al[2] + al[3])) * 0.25f);
+        fl[slot] = f;
+        if (dbg && sx >= 1 && sx <= TX && sy >= 1 && sy <= TY) reinterpret_cast<float4 *>(dbg)[(size_t)py * a.PX + px] = f;
     }
     __syncthreads();
 
-    for (int i = threadIdx.x; i < 4 * TP * TP; i += kBlock) {
-        const int x = 2 * px0 + (i & (2 * TP - 1)), y = 2 * py0 + i / (2 * TP);
+    for (int i = threadIdx.x; i < 4 * TX * TY; i += kBlock) {
+        const int x = 2 * px0 + (i & (2 * TX - 1)), y = 2 * py0 + i / (2 * TX);
         if (x >= a.W || y >= a.H) continue;
         const int bx0 = (x - 1) >> 1, by0 = (y - 1) >> 1;
         const int sx0 = min(max(bx0, 0), a.PX - 1) - (px0 - 1), sx1 = min(max(bx0 + 1, 0), a.PX - 1) - (px0 - 1);
         const int sy0 = min(max(by0, 0), a.PY - 1) - (py0 - 1), sy1 = min(max(by0 + 1, 0), a.PY - 1) - (py0 - 1);
-        const float3 v = bilerp4(fl[sy0 * NS + sx0], fl[sy0 * NS + sx1], fl[sy1 * NS + sx0], fl[sy1 * NS + sx1], x, y);
+        const float3 v = bilerp4(fl[sy0 * NX + sx0], fl[sy0 * NX + sx1], fl[sy1 * NX + sx0], fl[sy1 * NX + sx1], x, y);
         const size_t p = (size_t)y * a.W + x;
         irr[p] = v.x;
         irr[plane + p] = v.y;
@@ -427,7 +455,7 @@ hipError_t launch_level(const LevelArgs &a, Storage st, int shape, int rpt, hipS
 template <Storage S>
 static hipError_t launch_level0_st(const LevelArgs &a, float *irr, float *dbg, hipStream_t s)
 {
-    const int blocks = ((a.PX + 15) / 16) * ((a.PY + 15) / 16);
+    const int blocks = ((a.PX + 15) / 16) * ((a.PY + 11) / 12);  // 16x12-probe tiles
     if (a.upper) hipLaunchKernelGGL((k_level0<S, true>), dim3(blocks), dim3(kBlock), 0, s, a, irr, dbg);
     else hipLaunchKernelGGL((k_level0<S, false>), dim3(blocks), dim3(kBlock), 0, s, a, irr, dbg);
     return hipGetLastError();

@@ -73,7 +73,7 @@ hipError_t launch_error_norms(const float *x, const void *ref, bool ref_f64, siz
 
 // ---- surrogate MLP (rc_mlp.hip) ---------------------------------------------
 // 48 -> 64 -> 64 -> 64 -> 64, ReLU on the hidden layers; outputs 0..47 the
-// far-field residual (16 directions x rgb, normalised), 48 the log of the
+// far-field residual A_2 - c (16 directions x rgb, normalised), 48 the log of the
 // predicted squared error, 49..63 padding.
 constexpr int kMlpIn = 48, kMlpHid = 64, kMlpOut = 64, kMlpRad = 48, kMlpErr = 48;
 constexpr int kMlpParams = kMlpIn * 64 + 64 + 3 * (64 * 64 + 64);  // 15616 fp32 master parameters
@@ -109,7 +109,7 @@ hipError_t launch_sum_f32(const float *x, int n, float *out, hipStream_t s);
 // per-tap test.
 constexpr int kMipMax = 14;
 struct MipArgs {
-    float4 *tex;            // all levels back to back
+    uint4 *tex;             // all levels back to back, 8 fp16 values per cell
     int levels;             // top level index (>= 1)
     int w[kMipMax + 1], h[kMipMax + 1];
     size_t off[kMipMax + 1];
@@ -119,12 +119,13 @@ __host__ __device__ inline size_t mip_at(const MipArgs &m, int l, int x, int y)
     return m.off[l] + (size_t)(y + 1) * (size_t)(m.w[l] + 2) + (size_t)(x + 1);
 }
 // Cone trace schedule: every cone takes the same distances t_n, so the mip
-// level m_n and the occluder path length len_n are uniform over a launch; a
-// step carries what its samples need (one scalar load of 32 bytes).
+// level m_n and the opacity exponents are uniform over a launch; a step
+// carries what its samples need (one scalar load of 32 bytes).
 constexpr int kConeMaxSteps = 64;
+constexpr double kConeOccGain = 3.0;  // occluder exponent over the emitter one
 struct ConeStep {
-    float t, len, inv_cell;  // distance, occluder path length, 2^-m
-    int m, w, h;             // mip level and its size in cells
+    float t, re, ro, inv_cell;  // distance, emitter / occluder exponents, 2^-m
+    int w, h;                // size of the step's mip level in cells
     uint32_t off, wp;        // first (border) texel of the level, padded row length w+2
 };
 struct ConeSchedule {
@@ -175,7 +176,7 @@ hipError_t launch_seed_surr(const float *depth, const float *emissive, uint64_t 
                             hipStream_t s);
 hipError_t launch_near_sdf(const uint64_t *bits, uint8_t *sdf, int W, int H, hipStream_t s);
 hipError_t launch_mip_rest(const MipArgs &m, int *zero_me, hipStream_t s);
-hipError_t launch_mip_unpack(const MipArgs &m, int level, float *emc, float *mo, hipStream_t s);
+hipError_t launch_mip_unpack(const MipArgs &m, int level, float *emission, float *coverage, hipStream_t s);
 hipError_t launch_cone(const ConeArgs &a, hipStream_t s);
 hipError_t launch_surr_mlp(const SurrMlpArgs &a, const MlpDev &m, Storage st, hipStream_t s);
 hipError_t launch_surr_sample(const SurrSampleArgs &a, Storage st, hipStream_t s);

@@ -1,6 +1,6 @@
 // rc_surrogate.hip -- the learned far field: the levels >= 2 of a frame are
 // replaced by a cone-traced estimate of the radiance arriving from beyond the
-// level-1 interval, corrected per level-1 probe by the MLP (rc_mlp.hip).
+// level-1 interval, corrected per level-2 probe by the MLP (rc_mlp.hip).
 //
 // None of this exists in the reference (it has no global illumination at
 // all; its only pass is the forward raster at src/renderer.rs:559-613).  The
@@ -19,11 +19,16 @@
 // Cone model (per level-2 probe c and level-1 direction k, 16 cones of
 // half-angle pi/16; beyond kConeSplit from the level-3 probes, interpolated
 // as the merge does): samples at the schedule's distances t_n, bilinear in the
-// mip level m_n whose cells match the cone width; occluder opacity
-// a_o = 1 - (1 - min(mo, 0.999))^len_n (a thin silhouette blocks the whole
-// cone cross-section it crosses), total opacity a = min(me + a_o, 1);
-// L += T * emc.rgb, T *= 1 - a; a cone stops once T < 1e-4 or once its
-// bilinear footprint lies wholly outside the image.
+// mip level m_n whose cells match the cone width.  The share of the cone's
+// parallel rays that a cell stops is read from its column and row fractions
+// with the direction's weights, c = dy^2 x + dx^2 y: a thin line across the
+// cone stops all of it, whatever its share of the cell's area.  Emitters and
+// dark occluders give a_e = 1 - (1 - min(c_e, 0.999))^re_n and a_o = 1 -
+// (1 - min(c_o, 0.999))^ro_n (re_n = the step's share of a cell, ro_n = 3
+// re_n), a = min(a_e + a_o, 1); L += T a_e Ebar with Ebar the mean radiance
+// of the footprint's emitter pixels (mean E*[emitter] / emitter area),
+// T *= 1 - a; a cone stops once T < 1e-4 or once its bilinear footprint
+// lies wholly outside the image.
 #include <math.h>
 
 #include "rc_internal.hpp"
@@ -34,31 +39,52 @@ namespace rc {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
-// ---- packed mip texels ----------------------------------------------------------
-// texel = (mean E*[emitter] r, g, b, coverage) with coverage = fp16 [emitter]
-// mean (low half) | fp16 [occluder] mean (high half), both rounded to nearest
-// even.  A parent cell is the mean of its 4 stored children in the order
-// ((x0y0 + x1y0) + (x0y1 + x1y1)) * 1/4, fp32, then packed.
-__device__ inline float pack_cov(float me, float mo)
+// ---- packed mip cells ------------------------------------------------------------
+// A cell of 2^m x 2^m pixels is 8 fp16 values (16 bytes, rounded to nearest
+// even): the mean E*[emitter] rgb, the emitter area fraction, and the
+// fractions of the cell's pixel columns (x) and rows (y) that hold an
+// emitter (ex, ey) or a dark occluder (ox, oy) pixel; a pixel is a cell with
+// these 0 or 1.  A parent is computed in fp32 from its 4 stored children TL,
+// TR, BL, BR: means ((TL + TR) + (BL + BR)) / 4, x fractions (max(TL, BL) +
+// max(TR, BR)) / 2, y fractions (max(TL, TR) + max(BL, BR)) / 2 -- a column
+// of the left half is held if either child holds it; the max is that union
+// whenever the children's held columns nest (a line, a block).
+struct MipCell {
+    float v[8];  // er eg eb ea ex ey ox oy
+};
+__device__ inline uint4 pack_cell(const MipCell &c)
 {
-    return __uint_as_float(f32_to_f16_rne(me) | (f32_to_f16_rne(mo) << 16));
+    return make_uint4(f32_to_f16_rne(c.v[0]) | (f32_to_f16_rne(c.v[1]) << 16),
+                      f32_to_f16_rne(c.v[2]) | (f32_to_f16_rne(c.v[3]) << 16),
+                      f32_to_f16_rne(c.v[4]) | (f32_to_f16_rne(c.v[5]) << 16),
+                      f32_to_f16_rne(c.v[6]) | (f32_to_f16_rne(c.v[7]) << 16));
 }
-__device__ inline void unpack_cov(float w, float &me, float &mo)
+__device__ inline MipCell unpack_cell(uint4 u)
 {
-    const uint32_t u = __float_as_uint(w);
-    me = f16_to_f32(u & 0xFFFFu);
-    mo = f16_to_f32(u >> 16);
+    MipCell c;
+    const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        c.v[2 * i] = f16_to_f32(w[i] & 0xFFFFu);
+        c.v[2 * i + 1] = f16_to_f32(w[i] >> 16);
+    }
+    return c;
 }
-__device__ inline float4 mean4(float4 a, float4 b, float4 c, float4 d)
+__device__ inline MipCell combine4(const MipCell &tl, const MipCell &tr, const MipCell &bl, const MipCell &br)
 {
-    float ma, oa, mb, ob, mc, oc, md, od;
-    unpack_cov(a.w, ma, oa);
-    unpack_cov(b.w, mb, ob);
-    unpack_cov(c.w, mc, oc);
-    unpack_cov(d.w, md, od);
-    return make_float4(0.25f * ((a.x + b.x) + (c.x + d.x)), 0.25f * ((a.y + b.y) + (c.y + d.y)),
-                       0.25f * ((a.z + b.z) + (c.z + d.z)),
-                       pack_cov(0.25f * ((ma + mb) + (mc + md)), 0.25f * ((oa + ob) + (oc + od))));
+    MipCell p;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) p.v[i] = 0.25f * ((tl.v[i] + tr.v[i]) + (bl.v[i] + br.v[i]));
+#pragma unroll
+    for (int i = 4; i < 8; i += 2) {
+        p.v[i] = 0.5f * (fmaxf(tl.v[i], bl.v[i]) + fmaxf(tr.v[i], br.v[i]));
+        p.v[i + 1] = 0.5f * (fmaxf(tl.v[i + 1], tr.v[i + 1]) + fmaxf(bl.v[i + 1], br.v[i + 1]));
+    }
+    return p;
+}
+__device__ inline uint4 combine4(uint4 tl, uint4 tr, uint4 bl, uint4 br)
+{
+    return pack_cell(combine4(unpack_cell(tl), unpack_cell(tr), unpack_cell(bl), unpack_cell(br)));
 }
 
 // ---- occupancy bits + mip levels 1 and 2 -------------------------------------------
@@ -71,8 +97,8 @@ __global__ __launch_bounds__(kBlock) void k_seed_surr(const float *__restrict__ 
                                                       uint64_t *__restrict__ bits, int words, MipArgs m, int W, int H,
                                                       int bx_count)
 {
-    __shared__ float4 pe[4][64];
-    __shared__ float4 c1[2][32];
+    __shared__ MipCell pe[4][64];
+    __shared__ uint4 c1[2][32];
     const int bx = blockIdx.x % bx_count, by = blockIdx.x / bx_count;
     const int lane = threadIdx.x & 63, row = threadIdx.x >> 6;
     const int x = bx * 64 + lane, y = by * 4 + row;
@@ -94,12 +120,15 @@ __global__ __launch_bounds__(kBlock) void k_seed_surr(const float *__restrict__ 
     }
     const uint64_t w = __ballot(occ);
     if (lane == 0 && y < H) bits[(size_t)y * words + bx] = w;
-    pe[row][lane] = em ? make_float4(e0, e1, e2, pack_cov(1.0f, 0.0f))
-                       : make_float4(0.f, 0.f, 0.f, pack_cov(0.0f, occ ? 1.0f : 0.0f));
+    {
+        const float o = (occ && !em) ? 1.0f : 0.0f, f = em ? 1.0f : 0.0f;
+        pe[row][lane] = MipCell{{em ? e0 : 0.f, em ? e1 : 0.f, em ? e2 : 0.f, f, f, f, o, o}};
+    }
     __syncthreads();
     if (threadIdx.x < 64) {
         const int cx = threadIdx.x & 31, cy = threadIdx.x >> 5;
-        const float4 t = mean4(pe[2 * cy][2 * cx], pe[2 * cy][2 * cx + 1], pe[2 * cy + 1][2 * cx], pe[2 * cy + 1][2 * cx + 1]);
+        const uint4 t =
+            pack_cell(combine4(pe[2 * cy][2 * cx], pe[2 * cy][2 * cx + 1], pe[2 * cy + 1][2 * cx], pe[2 * cy + 1][2 * cx + 1]));
         c1[cy][cx] = t;
         const int gx = bx * 32 + cx, gy = by * 2 + cy;
         if (gx < m.w[1] && gy < m.h[1]) m.tex[mip_at(m, 1, gx, gy)] = t;
@@ -107,7 +136,7 @@ __global__ __launch_bounds__(kBlock) void k_seed_surr(const float *__restrict__ 
     __syncthreads();
     if (threadIdx.x < 16 && m.levels >= 2) {
         const int cx = threadIdx.x;
-        const float4 t = mean4(c1[0][2 * cx], c1[0][2 * cx + 1], c1[1][2 * cx], c1[1][2 * cx + 1]);
+        const uint4 t = combine4(c1[0][2 * cx], c1[0][2 * cx + 1], c1[1][2 * cx], c1[1][2 * cx + 1]);
         const int gx = bx * 16 + cx, gy = by;
         if (gx < m.w[2] && gy < m.h[2]) m.tex[mip_at(m, 2, gx, gy)] = t;
     }
@@ -219,13 +248,12 @@ constexpr int kMidTile = 16, kMidLast = 6;
 
 __global__ __launch_bounds__(kBlock) void k_mip_mid(MipArgs m, int tiles_x)
 {
-    __shared__ float4 se[kMidTile * kMidTile];
+    __shared__ uint4 se[kMidTile * kMidTile];
     const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
     {
         const int cx = threadIdx.x % kMidTile, cy = threadIdx.x / kMidTile;
         const int gx = tx * kMidTile + cx, gy = ty * kMidTile + cy;
-        se[threadIdx.x] = (gx < m.w[2] && gy < m.h[2]) ? m.tex[mip_at(m, 2, gx, gy)]
-                                                       : make_float4(0.f, 0.f, 0.f, pack_cov(0.f, 0.f));
+        se[threadIdx.x] = (gx < m.w[2] && gy < m.h[2]) ? m.tex[mip_at(m, 2, gx, gy)] : make_uint4(0u, 0u, 0u, 0u);
     }
     __syncthreads();
     int n = kMidTile;
@@ -233,10 +261,10 @@ __global__ __launch_bounds__(kBlock) void k_mip_mid(MipArgs m, int tiles_x)
     for (int l = 3; l <= lastl; ++l) {
         const int nn = n >> 1;
         const bool have = (int)threadIdx.x < nn * nn;
-        float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
+        uint4 t = make_uint4(0u, 0u, 0u, 0u);
         if (have) {
             const int cx = threadIdx.x % nn, cy = threadIdx.x / nn, i00 = (2 * cy) * n + 2 * cx;
-            t = mean4(se[i00], se[i00 + 1], se[i00 + n], se[i00 + n + 1]);
+            t = combine4(se[i00], se[i00 + 1], se[i00 + n], se[i00 + n + 1]);
         }
         __syncthreads();
         if (have) {
@@ -258,31 +286,31 @@ constexpr int kTopLds = 4096;
 
 __global__ __launch_bounds__(kBlock) void k_mip_top(MipArgs m, int *zero_me)
 {
-    __shared__ float4 sa[kTopLds], sb[kTopLds / 4 + 160];  // ceil(w/2)ceil(h/2) <= (wh + w + h + 1) / 4
+    __shared__ uint4 sa[kTopLds], sb[kTopLds / 4 + 160];  // ceil(w/2)ceil(h/2) <= (wh + w + h + 1) / 4
     if (zero_me && threadIdx.x == 0) *zero_me = 0;
     if (m.levels <= kMidLast) return;
-    const float4 zero = make_float4(0.f, 0.f, 0.f, pack_cov(0.f, 0.f));
+    const uint4 zero = make_uint4(0u, 0u, 0u, 0u);
     const int w6 = m.w[kMidLast], h6 = m.h[kMidLast];
     if (w6 * h6 <= kTopLds) {
         for (int i = threadIdx.x; i < w6 * h6; i += kBlock) sa[i] = m.tex[mip_at(m, kMidLast, i % w6, i / w6)];
         __syncthreads();
-        float4 *src = sa, *dst = sb;
+        uint4 *src = sa, *dst = sb;
         for (int l = kMidLast + 1; l <= m.levels; ++l) {
             const int w = m.w[l], h = m.h[l], cw = m.w[l - 1], ch = m.h[l - 1];
             for (int i = threadIdx.x; i < w * h; i += kBlock) {
                 const int x = i % w, y = i / w;
-                float4 v[4];
+                uint4 v[4];
 #pragma unroll
                 for (int d = 0; d < 4; ++d) {
                     const int cx = 2 * x + (d & 1), cy = 2 * y + (d >> 1);
                     v[d] = (cx < cw && cy < ch) ? src[cy * cw + cx] : zero;
                 }
-                const float4 t = mean4(v[0], v[1], v[2], v[3]);
+                const uint4 t = combine4(v[0], v[1], v[2], v[3]);
                 dst[i] = t;
                 m.tex[mip_at(m, l, x, y)] = t;
             }
             __syncthreads();
-            float4 *tmp = src;
+            uint4 *tmp = src;
             src = dst;
             dst = tmp;
         }
@@ -292,13 +320,13 @@ __global__ __launch_bounds__(kBlock) void k_mip_top(MipArgs m, int *zero_me)
         const int w = m.w[l], h = m.h[l], cw = m.w[l - 1], ch = m.h[l - 1];
         for (int i = threadIdx.x; i < w * h; i += kBlock) {
             const int x = i % w, y = i / w;
-            float4 v[4];
+            uint4 v[4];
 #pragma unroll
             for (int d = 0; d < 4; ++d) {
                 const int cx = 2 * x + (d & 1), cy = 2 * y + (d >> 1);
                 v[d] = (cx < cw && cy < ch) ? m.tex[mip_at(m, l - 1, cx, cy)] : zero;
             }
-            m.tex[mip_at(m, l, x, y)] = mean4(v[0], v[1], v[2], v[3]);
+            m.tex[mip_at(m, l, x, y)] = combine4(v[0], v[1], v[2], v[3]);
         }
         __threadfence_block();
         __syncthreads();
@@ -317,24 +345,24 @@ hipError_t launch_mip_rest(const MipArgs &m, int *zero_me, hipStream_t s)
     return hipGetLastError();
 }
 
-// Unpacked copy of mip level l (parity readback): emc [h][w][4], mo [h][w].
-__global__ __launch_bounds__(kBlock) void k_mip_unpack(MipArgs m, int l, float4 *__restrict__ emc, float *__restrict__ mo)
+// Unpacked copy of mip level l (parity readback): emission [h][w][4] (mean
+// E*[emitter] rgb, emitter area), coverage [h][w][4] (ex, ey, ox, oy).
+__global__ __launch_bounds__(kBlock) void k_mip_unpack(MipArgs m, int l, float4 *__restrict__ emission,
+                                                       float4 *__restrict__ coverage)
 {
     const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
     const size_t n = (size_t)m.w[l] * m.h[l];
     if (i >= n) return;
-    const float4 t = m.tex[mip_at(m, l, (int)(i % m.w[l]), (int)(i / m.w[l]))];
-    float me, o;
-    unpack_cov(t.w, me, o);
-    if (emc) emc[i] = make_float4(t.x, t.y, t.z, me);
-    if (mo) mo[i] = o;
+    const MipCell c = unpack_cell(m.tex[mip_at(m, l, (int)(i % m.w[l]), (int)(i / m.w[l]))]);
+    if (emission) emission[i] = make_float4(c.v[0], c.v[1], c.v[2], c.v[3]);
+    if (coverage) coverage[i] = make_float4(c.v[4], c.v[5], c.v[6], c.v[7]);
 }
 
-hipError_t launch_mip_unpack(const MipArgs &m, int level, float *emc, float *mo, hipStream_t s)
+hipError_t launch_mip_unpack(const MipArgs &m, int level, float *emission, float *coverage, hipStream_t s)
 {
     const size_t n = (size_t)m.w[level] * m.h[level];
     hipLaunchKernelGGL(k_mip_unpack, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, m, level,
-                       (float4 *)emc, mo);
+                       (float4 *)emission, (float4 *)coverage);
     return hipGetLastError();
 }
 
@@ -342,7 +370,8 @@ hipError_t launch_mip_unpack(const MipArgs &m, int level, float *emc, float *mo,
 // Schedule: t_0 = 10 px from the level-2 probe centre (where level 2's
 // interval starts), cone diameter
 // dia = max(2 t tan(pi/16), 1), mip level floor(log2 dia) clamped to
-// [1, top], next t = t + 0.75 dia, up to the image diagonal + 2.
+// [1, top], next t = t + 0.75 dia, up to the image diagonal + 2; the
+// exponents are re = min(0.75 dia, 2^m) / 2^m and ro = kConeOccGain re.
 void cone_schedule(int W, int H, const MipArgs &mip, ConeSchedule &near, ConeSchedule &far)
 {
     const int top = mip.levels;
@@ -357,10 +386,11 @@ void cone_schedule(int W, int H, const MipArgs &mip, ConeSchedule &near, ConeSch
         const double step = 0.75 * dia;
         ConeSchedule &s = t < (double)kConeSplit ? near : far;
         ConeStep &c = s.s[s.n++];
+        const double r = fmin(step, (double)(1 << m)) / (double)(1 << m);
         c.t = (float)t;
-        c.len = (float)fmin(step, (double)(1 << m));
+        c.re = (float)r;
+        c.ro = (float)(kConeOccGain * r);
         c.inv_cell = 1.0f / (float)(1 << m);
-        c.m = m;
         c.w = mip.w[m];
         c.h = mip.h[m];
         c.off = (uint32_t)mip.off[m];
@@ -397,6 +427,7 @@ __global__ __launch_bounds__(kBlock) void k_cone(ConeArgs a, int seg_x)
         __builtin_amdgcn_make_buffer_rsrc(a.mip.tex, (short)0, (int)a.mip_bytes, kBufferDword3);
     const float cx = ((float)qx + 0.5f) * a.spacing, cy = ((float)qy + 0.5f) * a.spacing;
     const float2 d = a.dirs[k];
+    const float dx2 = d.x * d.x, dy2 = d.y * d.y;
     float T = 1.0f, r = 0.f, g = 0.f, b = 0.f;
     for (int n = 0; n < a.sch.n; ++n) {
         const ConeStep st = a.sch.s[n];
@@ -408,30 +439,28 @@ __global__ __launch_bounds__(kBlock) void k_cone(ConeArgs a, int seg_x)
         const float ax = fx - flx, ay = fy - fly;
         const uint32_t o00 = (st.off + __umul24((uint32_t)(iy + 1), st.wp) + (uint32_t)(ix + 1)) << 4;
         const uint32_t orow = st.wp << 4;
-        const float4 v00 = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, (int)o00, 0, 0));
-        const float4 v10 = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, (int)(o00 + 16u), 0, 0));
-        const float4 v01 = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, (int)(o00 + orow), 0, 0));
-        const float4 v11 =
-            __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, (int)(o00 + orow + 16u), 0, 0));
-        const float4 v[4] = {v00, v10, v01, v11};
+        const uint4 v[4] = {
+            __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, (int)o00, 0, 0)),
+            __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, (int)(o00 + 16u), 0, 0)),
+            __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, (int)(o00 + orow), 0, 0)),
+            __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, (int)(o00 + orow + 16u), 0, 0))};
         const float wt[4] = {(1.0f - ax) * (1.0f - ay), ax * (1.0f - ay), (1.0f - ax) * ay, ax * ay};
-        float er = 0.f, eg = 0.f, eb = 0.f, me = 0.f, mo = 0.f;
+        float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int tap = 0; tap < 4; ++tap) {
-            float cme, cmo;
-            unpack_cov(v[tap].w, cme, cmo);
-            er = fmaf(wt[tap], v[tap].x, er);
-            eg = fmaf(wt[tap], v[tap].y, eg);
-            eb = fmaf(wt[tap], v[tap].z, eb);
-            me = fmaf(wt[tap], cme, me);
-            mo = fmaf(wt[tap], cmo, mo);
+            const MipCell c = unpack_cell(v[tap]);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) acc[i] = fmaf(wt[tap], c.v[i], acc[i]);
         }
-        // 1 - (1 - mo)^len with the hardware log2 / exp2 (argument in [1e-3, 1])
-        const float ao = 1.0f - __builtin_amdgcn_exp2f(st.len * __builtin_amdgcn_logf(1.0f - fminf(mo, 0.999f)));
-        const float al = fminf(me + ao, 1.0f);
-        r = fmaf(T, er, r);
-        g = fmaf(T, eg, g);
-        b = fmaf(T, eb, b);
+        const float ce = fmaf(dy2, acc[4], dx2 * acc[5]), co = fmaf(dy2, acc[6], dx2 * acc[7]);
+        // 1 - (1 - c)^r with the hardware log2 / exp2 (argument in [1e-3, 1])
+        const float ae = 1.0f - __builtin_amdgcn_exp2f(st.re * __builtin_amdgcn_logf(1.0f - fminf(ce, 0.999f)));
+        const float ao = 1.0f - __builtin_amdgcn_exp2f(st.ro * __builtin_amdgcn_logf(1.0f - fminf(co, 0.999f)));
+        const float al = fminf(ae + ao, 1.0f);
+        const float we = acc[3] > 0.0f ? T * ae / acc[3] : 0.0f;
+        r = fmaf(we, acc[0], r);
+        g = fmaf(we, acc[1], g);
+        b = fmaf(we, acc[2], b);
         T = T * (1.0f - al);
         if (T < 1e-4f) break;
     }

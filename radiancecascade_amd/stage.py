@@ -339,12 +339,16 @@ class RadianceCascadeStage:
             None if tile_err is None else _dev_ptr(tile_err, tx * ty * 4, "tile_err"), _stream_handle(stream)),
             self._ctx)
 
-    def debug_mip(self, level: int, emc=None, mo=None, stream=None) -> None:
+    def debug_mip(self, level: int, emission=None, coverage=None, stream=None) -> None:
+        """Mip level `level` of the last learned frame: emission [h][w][4] (mean
+        E*[emitter] rgb, emitter area), coverage [h][w][4] (emitter column and
+        row fractions, then the dark occluders')."""
         w = -(-self.width // (1 << level))
         h = -(-self.height // (1 << level))
-        check(self._lib.rc_debug_mip(self._ctx, level, None if emc is None else _dev_ptr(emc, w * h * 16, "emc"),
-                                     None if mo is None else _dev_ptr(mo, w * h * 4, "mo"), _stream_handle(stream)),
-              self._ctx)
+        check(self._lib.rc_debug_mip(self._ctx, level,
+                                     None if emission is None else _dev_ptr(emission, w * h * 16, "emission"),
+                                     None if coverage is None else _dev_ptr(coverage, w * h * 16, "coverage"),
+                                     _stream_handle(stream)), self._ctx)
 
     def close(self) -> None:
         if getattr(self, "_ctx", None) is not None and self._ctx.value:

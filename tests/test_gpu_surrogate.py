@@ -3,7 +3,8 @@ the FP64 oracle (oracle/surrogate.py, oracle/mlp.py).
 
 Bit-exact: the near distance field (integer d^2, the same sqrt rounding) and
 the fallback path with every tile flagged (it is the plain path).  Within
-tolerance: the mip pyramid (fp32 sums of at most 4^m terms), the cone trace
+tolerance: the mip pyramid (fp32 means rounded to fp16 per level, as the
+oracle emulates), the cone trace
 (fp32 bilinear, exp2/log2), the network (bf16 operands; the GPU accumulates
 in fp32 in another order, which can move a hidden activation across a bf16
 rounding boundary), its gradient and one Adam step, and the whole frame.
@@ -92,7 +93,7 @@ def test_surrogate_frame_parity(rc, W, H, L, seed, nb, ne):
         for m in range(1, top + 1):
             w, h = -(-W // (1 << m)), -(-H // (1 << m))
             e = torch.empty((h, w, 4), dtype=torch.float32, device="cuda:0")
-            o = torch.empty((h, w), dtype=torch.float32, device="cuda:0")
+            o = torch.empty((h, w, 4), dtype=torch.float32, device="cuda:0")
             st.debug_mip(m, e, o)
             mips.append((e, o))
         torch.cuda.synchronize()

@@ -1,15 +1,19 @@
 """What the trained network gains: the packaged weights (trained on the GPU
-on 10,000 scenes at 512x512, seeds 10000..19999) evaluated on held-out
-1920x1080 scenes with 6 cascades, against the FP64 oracle.
+on 8,000 scenes at 1920x1080, seeds 10000..17999, 6 cascades) evaluated on
+held-out 1920x1080 scenes with 6 cascades.
 
 Three forms of the frame replace levels >= 2:
   truncated  levels 0 and 1 only (the method without any far field);
   proxy      the cone-traced far field alone (the network's weights zeroed);
   learned    cone trace + trained network (the shipped surrogate).
-The learned frame must beat the truncated one on every scene.  Its error
-against the proxy alone is recorded, not asserted: the network lowers the
-error of the scenes whose light comes from far away and raises it on some
-scenes that the proxy already gets right (numbers printed per scene).
+Against the FP64 oracle on four scenes, the learned frame must beat the
+truncated one on every scene.  Against the plain GPU path (equal to the
+oracle within 1e-4 here) on the lit scenes of seeds 100..159 (not used to
+choose the weights, which were compared on seeds 20..59), the learned frame
+must beat the proxy in median and in mean and on more than half of the
+scenes: the gain of the network itself.  (On some scenes it loses: e.g.
+seed 4, whose light comes from a 4-px emissive strip along the image's top
+edge.)
 """
 import os
 import sys
@@ -73,5 +77,26 @@ def test_learned_far_field_gain(rc):
     med = {k: float(np.median([r[k] for r in rows])) for k in rows[0]}
     _say(f"  median: truncated {med['truncated']:.4f}  proxy {med['proxy']:.4f}  learned {med['learned']:.4f}")
     assert med["learned"] < 0.5 * med["truncated"]
+    # the network against the cones alone, over more scenes
+    ep, el = [], []
+    for seed in range(100, 160):
+        rc.gbuffer_raster(rc.scene_generate(seed, 32, 4), W, H, depth, None, emis)
+        ref = torch.empty((3, H, W), device=dev)
+        plain.run_device(depth, emis, ref)
+        torch.cuda.synchronize()
+        if float(ref.norm()) == 0.0:
+            continue
+        for st, acc in ((proxy, ep), (learned, el)):
+            out = torch.empty((3, H, W), device=dev)
+            st.run_device(depth, emis, out)
+            torch.cuda.synchronize()
+            acc.append(st.error_norms(out, ref)["rel_l2"])
+    ep, el = np.array(ep), np.array(el)
+    wins = int((el < ep).sum())
+    _say(f"  seeds 100..159 ({len(ep)} lit): proxy median {np.median(ep):.4f} mean {ep.mean():.4f}; "
+         f"learned median {np.median(el):.4f} mean {el.mean():.4f}; learned better on {wins}")
+    assert len(ep) >= 20
+    assert wins > 0.5 * len(ep)
+    assert np.median(el) < np.median(ep) and el.mean() < ep.mean()
     for st in (learned, proxy, trunc, plain):
         st.close()
