@@ -261,8 +261,9 @@ typedef struct {
 } rc_train_report;
 rc_status rc_surrogate_train(rc_ctx *ctx, const rc_train_config *cfg, rc_train_report *report);
 /* Readbacks of the last learned-far-field frame (device destinations, any may
- * be NULL): cone [Py_2][Px_2][16][4] (rgb, transmittance); level1 [Py_1][Px_1][4][4]
- * merged level 1 (fp32); tile_err [ceil(Py_2/8)][ceil(Px_2/8)]. */
+ * be NULL): cone [Py_2][Px_2][16][4] (rgb, transmittance; needs a context
+ * created with RC_FLAG_KEEP_DEBUG); level1 [Py_1][Px_1][4][4] merged level 1
+ * (fp32); tile_err [ceil(Py_2/8)][ceil(Px_2/8)]. */
 rc_status rc_debug_surrogate(rc_ctx *ctx, float *cone_dev, float *level1_dev, float *tile_err_dev, void *stream);
 /* Mip level m >= 1 of the last learned-far-field frame: emission [h][w][4]
  * (mean E*[emitter] rgb, emitter area fraction) and coverage [h][w][4] (the

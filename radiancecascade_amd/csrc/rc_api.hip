@@ -74,9 +74,11 @@ struct rc_ctx {
     int surr_tiles = 0, surr_tiles_x = 0;
     bool surr_ran = false;
     float *scratch_irr = nullptr;  // irradiance of frames run only for their levels
-    // the near distance field runs on a side stream beside the mip / cone chain
-    hipStream_t side = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // the near distance field runs on a side stream beside the mip / cone
+    // chain, and the top mip levels + far cones on a second one beside the
+    // near cones
+    hipStream_t side = nullptr, side2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_fork2 = nullptr, ev_join2 = nullptr;
     // profiling
     hipEvent_t ev[2 * RC_NUM_STAGES] = {};
     bool ev_used[RC_NUM_STAGES] = {};
@@ -424,20 +426,30 @@ static rc_status ensure_surrogate(rc_ctx *c)
     HIP_TRY(c, hipMalloc(&c->flag_count, sizeof(int)));
     HIP_TRY(c, hipMemset(c->flag_count, 0, sizeof(int)));
     if (!c->side) HIP_TRY(c, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    if (!c->side2) HIP_TRY(c, hipStreamCreateWithFlags(&c->side2, hipStreamNonBlocking));
     if (!c->ev_fork) HIP_TRY(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
     if (!c->ev_join) HIP_TRY(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    if (!c->ev_fork2) HIP_TRY(c, hipEventCreateWithFlags(&c->ev_fork2, hipEventDisableTiming));
+    if (!c->ev_join2) HIP_TRY(c, hipEventCreateWithFlags(&c->ev_join2, hipEventDisableTiming));
     c->surr_alloc = true;
     return RC_OK;
 }
 
-// The two cone launches: the far segment on the level-3 grid, then the near
-// segment on the level-2 grid, which appends the far one.
-static rc_status launch_cones(rc_ctx *c, hipStream_t s)
+// Cone launches: the far segment on the level-3 grid; the near segment on
+// the level-2 grid, appending the far one when `append` (else the network
+// kernel appends it).
+static ConeArgs cone_args(rc_ctx *c)
 {
     ConeArgs a{};
     a.mip = c->mip;
-    a.mip_bytes = (uint32_t)(mip_cells(c->mip) * sizeof(float4));
+    a.mip_bytes = (uint32_t)(mip_cells(c->mip) * sizeof(uint4));
     a.dirs = c->dirs + c->dir_off[1];
+    return a;
+}
+
+static rc_status launch_cone_far(rc_ctx *c, hipStream_t s)
+{
+    ConeArgs a = cone_args(c);
     a.sch = c->sch_far;
     a.cone = c->cone_far;
     a.QX = grid_dim(c->W, 3);
@@ -445,20 +457,33 @@ static rc_status launch_cones(rc_ctx *c, hipStream_t s)
     a.spacing = (float)probe_spacing(3);
     a.far = nullptr;
     HIP_TRY(c, launch_cone(a, s));
+    return RC_OK;
+}
+
+static rc_status launch_cone_near(rc_ctx *c, bool append, hipStream_t s)
+{
+    ConeArgs a = cone_args(c);
     a.sch = c->sch;
     a.cone = c->cone;
     a.QX = grid_dim(c->W, 2);
     a.QY = grid_dim(c->H, 2);
     a.spacing = (float)probe_spacing(2);
-    a.far = c->cone_far;
+    a.far = append ? c->cone_far : nullptr;
     a.FX = grid_dim(c->W, 3);
     a.FY = grid_dim(c->H, 3);
     HIP_TRY(c, launch_cone(a, s));
     return RC_OK;
 }
 
-// occupancy bits + mip 1..2 -> near distance field -> mip 3..top -> cones ->
-// level 1 (march + MLP + merge) -> level 0 + gather from the learned level 1.
+static rc_status launch_cones(rc_ctx *c, hipStream_t s)
+{
+    rc_status st = launch_cone_far(c, s);
+    return st != RC_OK ? st : launch_cone_near(c, true, s);
+}
+
+// occupancy bits + mip 1..2 -> [near distance field] -> mip 3..6 -> near
+// cones [beside mip 7..top -> far cones] -> network (appending the far
+// segment) -> level 1 (plain kernel on the near field) -> level 0 + gather.
 static rc_status run_frame_surrogate(rc_ctx *c, const float *depth, const float *emissive, float *irr, hipStream_t s,
                                      bool quiet)
 {
@@ -483,18 +508,36 @@ static rc_status run_frame_surrogate(rc_ctx *c, const float *depth, const float 
         HIP_TRY(c, launch_near_sdf(c->occ_bits, c->sdf, c->W, c->H, s));
         tm.end(RC_STAGE_NEAR_SDF);
     }
-    tm.begin(RC_STAGE_MIP);
-    HIP_TRY(c, launch_mip_rest(c->mip, c->flag_count, s));
-    tm.end(RC_STAGE_MIP);
-    tm.begin(RC_STAGE_CONE);
-    {
-        rc_status st = launch_cones(c, s);
+    if (fork) {
+        // near cones (mip levels <= 4, from k_mip_mid) beside the top mip
+        // levels and the far cones; the network kernel appends the far segment
+        HIP_TRY(c, launch_mip_mid(c->mip, s));
+        HIP_TRY(c, hipEventRecord(c->ev_fork2, s));
+        HIP_TRY(c, hipStreamWaitEvent(c->side2, c->ev_fork2, 0));
+        HIP_TRY(c, launch_mip_top(c->mip, c->flag_count, c->side2));
+        rc_status st = launch_cone_far(c, c->side2);
         if (st != RC_OK) return st;
+        HIP_TRY(c, hipEventRecord(c->ev_join2, c->side2));
+        st = launch_cone_near(c, false, s);
+        if (st != RC_OK) return st;
+        HIP_TRY(c, hipStreamWaitEvent(s, c->ev_join2, 0));
+        HIP_TRY(c, hipStreamWaitEvent(s, c->ev_join, 0));
+    } else {
+        tm.begin(RC_STAGE_MIP);
+        HIP_TRY(c, launch_mip_rest(c->mip, c->flag_count, s));
+        tm.end(RC_STAGE_MIP);
+        tm.begin(RC_STAGE_CONE);
+        rc_status st = launch_cone_far(c, s);
+        if (st == RC_OK) st = launch_cone_near(c, false, s);
+        if (st != RC_OK) return st;
+        tm.end(RC_STAGE_CONE);
     }
-    tm.end(RC_STAGE_CONE);
-    if (fork) HIP_TRY(c, hipStreamWaitEvent(s, c->ev_join, 0));
     SurrMlpArgs a{};
     a.cone = c->cone;
+    a.far = c->cone_far;
+    a.FX = grid_dim(c->W, 3);
+    a.FY = grid_dim(c->H, 3);
+    a.cone_out = dbg ? c->cone : nullptr;
     a.out = c->surr_a2;
     a.tile_err = c->tile_err;
     a.flag_count = c->flag_count;
@@ -820,8 +863,11 @@ void rc_destroy(rc_ctx *c)
         if (ev) hipEventDestroy(ev);
     if (c->cap_stream) hipStreamDestroy(c->cap_stream);
     if (c->side) hipStreamDestroy(c->side);
+    if (c->side2) hipStreamDestroy(c->side2);
     if (c->ev_fork) hipEventDestroy(c->ev_fork);
     if (c->ev_join) hipEventDestroy(c->ev_join);
+    if (c->ev_fork2) hipEventDestroy(c->ev_fork2);
+    if (c->ev_join2) hipEventDestroy(c->ev_join2);
     delete c;
 }
 
@@ -1437,6 +1483,9 @@ rc_status rc_debug_surrogate(rc_ctx *c, float *cone, float *level1, float *tile_
     hipStream_t s = (hipStream_t)stream;
     HIP_TRY(c, hipSetDevice(c->cfg.device));
     const size_t Q = (size_t)grid_dim(c->W, 2) * grid_dim(c->H, 2), P = (size_t)grid_dim(c->W, 1) * grid_dim(c->H, 1);
+    // a frame keeps whole cones (near + far segment) only with the debug flag
+    if (cone && !(c->cfg.flags & RC_FLAG_KEEP_DEBUG))
+        return fail(c, RC_ERR_STATE, "rc_debug_surrogate: cones need RC_FLAG_KEEP_DEBUG");
     if (cone) HIP_TRY(c, hipMemcpyAsync(cone, c->cone, Q * 16 * sizeof(float4), hipMemcpyDeviceToDevice, s));
     if (level1) HIP_TRY(c, launch_level_to_f32(c->surr_a1, c->storage, P * 4, level1, s));
     if (tile_err)

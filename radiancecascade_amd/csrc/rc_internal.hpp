@@ -152,6 +152,9 @@ struct ConeArgs {
 };
 struct SurrMlpArgs {
     const float4 *cone;     // [QY][QX][16] cones at the level-2 probes
+    const float4 *far;      // far-segment cones [FY][FX][16] to append, or nullptr (cone already whole)
+    int FX, FY;
+    float4 *cone_out;       // whole cones written back (parity readback), or nullptr
     void *out;              // the network's merged level 2 (storage texels, [QY][QX][16])
     float *tile_err;        // [tiles] mean predicted squared error (normalised), tiles of 8x8 level-2 probes
     int *flag_count;        // tiles above threshold (counter)
@@ -175,7 +178,9 @@ int cone_top_level(int W, int H);
 hipError_t launch_seed_surr(const float *depth, const float *emissive, uint64_t *bits, const MipArgs &m, int W, int H,
                             hipStream_t s);
 hipError_t launch_near_sdf(const uint64_t *bits, uint8_t *sdf, int W, int H, hipStream_t s);
-hipError_t launch_mip_rest(const MipArgs &m, int *zero_me, hipStream_t s);
+hipError_t launch_mip_rest(const MipArgs &m, int *zero_me, hipStream_t s);   // mid + top
+hipError_t launch_mip_mid(const MipArgs &m, hipStream_t s);                  // levels 3..6
+hipError_t launch_mip_top(const MipArgs &m, int *zero_me, hipStream_t s);    // levels 7..top
 hipError_t launch_mip_unpack(const MipArgs &m, int level, float *emission, float *coverage, hipStream_t s);
 hipError_t launch_cone(const ConeArgs &a, hipStream_t s);
 hipError_t launch_surr_mlp(const SurrMlpArgs &a, const MlpDev &m, Storage st, hipStream_t s);

@@ -333,16 +333,24 @@ __global__ __launch_bounds__(kBlock) void k_mip_top(MipArgs m, int *zero_me)
     }
 }
 
-hipError_t launch_mip_rest(const MipArgs &m, int *zero_me, hipStream_t s)
+hipError_t launch_mip_mid(const MipArgs &m, hipStream_t s)
 {
-    if (m.levels >= 3) {
-        const int tx = (m.w[2] + kMidTile - 1) / kMidTile, ty = (m.h[2] + kMidTile - 1) / kMidTile;
-        hipLaunchKernelGGL(k_mip_mid, dim3(tx * ty), dim3(kBlock), 0, s, m, tx);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
+    if (m.levels < 3) return hipSuccess;
+    const int tx = (m.w[2] + kMidTile - 1) / kMidTile, ty = (m.h[2] + kMidTile - 1) / kMidTile;
+    hipLaunchKernelGGL(k_mip_mid, dim3(tx * ty), dim3(kBlock), 0, s, m, tx);
+    return hipGetLastError();
+}
+
+hipError_t launch_mip_top(const MipArgs &m, int *zero_me, hipStream_t s)
+{
     hipLaunchKernelGGL(k_mip_top, dim3(1), dim3(kBlock), 0, s, m, zero_me);
     return hipGetLastError();
+}
+
+hipError_t launch_mip_rest(const MipArgs &m, int *zero_me, hipStream_t s)
+{
+    hipError_t e = launch_mip_mid(m, s);
+    return e != hipSuccess ? e : launch_mip_top(m, zero_me, s);
 }
 
 // Unpacked copy of mip level l (parity readback): emission [h][w][4] (mean
@@ -397,6 +405,22 @@ void cone_schedule(int W, int H, const MipArgs &mip, ConeSchedule &near, ConeSch
         c.wp = (uint32_t)(mip.w[m] + 2);
         t += step;
     }
+}
+
+// The far segment appended to a level-2 cone (L, T): the level-3 grid's cones
+// interpolated at level-2 probe (qx, qy) with the merge's bilinear weights,
+// L += T * F.rgb, T *= F.T.
+__device__ inline float4 far_append(float4 c, const float4 *__restrict__ far, int FX, int FY, int qx, int qy, int k)
+{
+    const int bx = (qx - 1) >> 1, by = (qy - 1) >> 1;
+    const int fx0 = min(max(bx, 0), FX - 1), fx1 = min(max(bx + 1, 0), FX - 1);
+    const int fy0 = min(max(by, 0), FY - 1), fy1 = min(max(by + 1, 0), FY - 1);
+    const float4 f00 = far[((size_t)fy0 * FX + fx0) * 16 + k], f10 = far[((size_t)fy0 * FX + fx1) * 16 + k];
+    const float4 f01 = far[((size_t)fy1 * FX + fx0) * 16 + k], f11 = far[((size_t)fy1 * FX + fx1) * 16 + k];
+    const float3 f = bilerp4(f00, f10, f01, f11, qx, qy);
+    const float3 ft = bilerp4(make_float4(f00.w, f00.w, f00.w, 0.f), make_float4(f10.w, f10.w, f10.w, 0.f),
+                              make_float4(f01.w, f01.w, f01.w, 0.f), make_float4(f11.w, f11.w, f11.w, 0.f), qx, qy);
+    return make_float4(fmaf(c.w, f.x, c.x), fmaf(c.w, f.y, c.y), fmaf(c.w, f.z, c.z), c.w * ft.x);
 }
 
 // Workgroups are dealt to the 8 XCDs round-robin; give each XCD a contiguous
@@ -465,19 +489,11 @@ __global__ __launch_bounds__(kBlock) void k_cone(ConeArgs a, int seg_x)
         if (T < 1e-4f) break;
     }
     if (a.far) {
-        // the far segment from the level-3 grid, bilinear with the merge's weights
-        const int bx = (qx - 1) >> 1, by = (qy - 1) >> 1;
-        const int fx0 = min(max(bx, 0), a.FX - 1), fx1 = min(max(bx + 1, 0), a.FX - 1);
-        const int fy0 = min(max(by, 0), a.FY - 1), fy1 = min(max(by + 1, 0), a.FY - 1);
-        const float4 f00 = a.far[((size_t)fy0 * a.FX + fx0) * 16 + k], f10 = a.far[((size_t)fy0 * a.FX + fx1) * 16 + k];
-        const float4 f01 = a.far[((size_t)fy1 * a.FX + fx0) * 16 + k], f11 = a.far[((size_t)fy1 * a.FX + fx1) * 16 + k];
-        const float3 f = bilerp4(f00, f10, f01, f11, qx, qy);
-        const float3 ft = bilerp4(make_float4(f00.w, f00.w, f00.w, 0.f), make_float4(f10.w, f10.w, f10.w, 0.f),
-                                  make_float4(f01.w, f01.w, f01.w, 0.f), make_float4(f11.w, f11.w, f11.w, 0.f), qx, qy);
-        r = fmaf(T, f.x, r);
-        g = fmaf(T, f.y, g);
-        b = fmaf(T, f.z, b);
-        T = T * ft.x;
+        const float4 v = far_append(make_float4(r, g, b, T), a.far, a.FX, a.FY, qx, qy, k);
+        r = v.x;
+        g = v.y;
+        b = v.z;
+        T = v.w;
     }
     a.cone[((size_t)qy * a.QX + qx) * 16 + k] = make_float4(r, g, b, T);
 }
@@ -560,6 +576,14 @@ __global__ __launch_bounds__(kBlock) void k_surr_mlp(SurrMlpArgs a, MlpDev m)
         float4 cf[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) cf[j] = a.cone[q * 16 + 4 * ko + j];
+        if (a.far) {
+            // the near segment alone was traced: append the far one here
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                cf[j] = far_append(cf[j], a.far, a.FX, a.FY, min(qx, a.QX - 1), min(qy, a.QY - 1), 4 * ko + j);
+                if (a.cone_out && valid) a.cone_out[q * 16 + 4 * ko + j] = cf[j];
+            }
+        }
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int col = 3 * (4 * ko + j);
