@@ -74,11 +74,8 @@ struct rc_ctx {
     int surr_tiles = 0, surr_tiles_x = 0;
     bool surr_ran = false;
     float *scratch_irr = nullptr;  // irradiance of frames run only for their levels
-    // the near distance field runs on a side stream beside the mip / cone
-    // chain, and the top mip levels + far cones on a second one beside the
-    // near cones
-    hipStream_t side = nullptr, side2 = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_fork2 = nullptr, ev_join2 = nullptr;
+    // the cone schedules (near, far) in device memory
+    ConeSchedule *sch_dev = nullptr;
     // profiling
     hipEvent_t ev[2 * RC_NUM_STAGES] = {};
     bool ev_used[RC_NUM_STAGES] = {};
@@ -156,6 +153,8 @@ static void release(rc_ctx *c)
     if (c->cone) hipFree(c->cone);
     if (c->cone_far) hipFree(c->cone_far);
     c->cone_far = nullptr;
+    if (c->sch_dev) hipFree(c->sch_dev);
+    c->sch_dev = nullptr;
     if (c->surr_a1) hipFree(c->surr_a1);
     if (c->surr_a2) hipFree(c->surr_a2);
     c->surr_a2 = nullptr;
@@ -425,12 +424,9 @@ static rc_status ensure_surrogate(rc_ctx *c)
     HIP_TRY(c, hipMalloc(&c->flag_list, (size_t)c->surr_tiles * sizeof(int)));
     HIP_TRY(c, hipMalloc(&c->flag_count, sizeof(int)));
     HIP_TRY(c, hipMemset(c->flag_count, 0, sizeof(int)));
-    if (!c->side) HIP_TRY(c, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
-    if (!c->side2) HIP_TRY(c, hipStreamCreateWithFlags(&c->side2, hipStreamNonBlocking));
-    if (!c->ev_fork) HIP_TRY(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
-    if (!c->ev_join) HIP_TRY(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
-    if (!c->ev_fork2) HIP_TRY(c, hipEventCreateWithFlags(&c->ev_fork2, hipEventDisableTiming));
-    if (!c->ev_join2) HIP_TRY(c, hipEventCreateWithFlags(&c->ev_join2, hipEventDisableTiming));
+    HIP_TRY(c, hipMalloc(&c->sch_dev, 2 * sizeof(ConeSchedule)));
+    HIP_TRY(c, hipMemcpy(c->sch_dev, &c->sch, sizeof(ConeSchedule), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->sch_dev + 1, &c->sch_far, sizeof(ConeSchedule), hipMemcpyHostToDevice));
     c->surr_alloc = true;
     return RC_OK;
 }
@@ -438,32 +434,25 @@ static rc_status ensure_surrogate(rc_ctx *c)
 // Cone launches: the far segment on the level-3 grid; the near segment on
 // the level-2 grid, appending the far one when `append` (else the network
 // kernel appends it).
-static ConeArgs cone_args(rc_ctx *c)
+static ConeArgs cone_args_far(rc_ctx *c)
 {
     ConeArgs a{};
     a.mip = c->mip;
     a.mip_bytes = (uint32_t)(mip_cells(c->mip) * sizeof(uint4));
     a.dirs = c->dirs + c->dir_off[1];
-    return a;
-}
-
-static rc_status launch_cone_far(rc_ctx *c, hipStream_t s)
-{
-    ConeArgs a = cone_args(c);
-    a.sch = c->sch_far;
+    a.sch = c->sch_dev + 1;
     a.cone = c->cone_far;
     a.QX = grid_dim(c->W, 3);
     a.QY = grid_dim(c->H, 3);
     a.spacing = (float)probe_spacing(3);
     a.far = nullptr;
-    HIP_TRY(c, launch_cone(a, s));
-    return RC_OK;
+    return a;
 }
 
-static rc_status launch_cone_near(rc_ctx *c, bool append, hipStream_t s)
+static ConeArgs cone_args_near(rc_ctx *c, bool append)
 {
-    ConeArgs a = cone_args(c);
-    a.sch = c->sch;
+    ConeArgs a = cone_args_far(c);
+    a.sch = c->sch_dev;
     a.cone = c->cone;
     a.QX = grid_dim(c->W, 2);
     a.QY = grid_dim(c->H, 2);
@@ -471,19 +460,19 @@ static rc_status launch_cone_near(rc_ctx *c, bool append, hipStream_t s)
     a.far = append ? c->cone_far : nullptr;
     a.FX = grid_dim(c->W, 3);
     a.FY = grid_dim(c->H, 3);
-    HIP_TRY(c, launch_cone(a, s));
-    return RC_OK;
+    return a;
 }
 
 static rc_status launch_cones(rc_ctx *c, hipStream_t s)
 {
-    rc_status st = launch_cone_far(c, s);
-    return st != RC_OK ? st : launch_cone_near(c, true, s);
+    HIP_TRY(c, launch_cone(cone_args_far(c), s));
+    HIP_TRY(c, launch_cone(cone_args_near(c, true), s));
+    return RC_OK;
 }
 
-// occupancy bits + mip 1..2 -> [near distance field] -> mip 3..6 -> near
-// cones [beside mip 7..top -> far cones] -> network (appending the far
-// segment) -> level 1 (plain kernel on the near field) -> level 0 + gather.
+// occupancy bits + mip 1..2 -> mip 3..6 + near distance field -> mip
+// 7..top -> near + far cones -> network (appending the far segment) ->
+// level 1 (plain kernel on the near field) -> level 0 + gather.
 static rc_status run_frame_surrogate(rc_ctx *c, const float *depth, const float *emissive, float *irr, hipStream_t s,
                                      bool quiet)
 {
@@ -494,42 +483,26 @@ static rc_status run_frame_surrogate(rc_ctx *c, const float *depth, const float 
     tm.begin(RC_STAGE_SEED);
     HIP_TRY(c, launch_seed_surr(depth, emissive, c->occ_bits, c->mip, c->W, c->H, s));
     tm.end(RC_STAGE_SEED);
-    // The near distance field needs only the occupancy bits: it runs on the
-    // side stream beside the mip and cone chain, joined before level 1 (a
-    // profiled frame runs it in line so each stage has its own time).
-    const bool fork = !prof;
-    if (fork) {
-        HIP_TRY(c, hipEventRecord(c->ev_fork, s));
-        HIP_TRY(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
-        HIP_TRY(c, launch_near_sdf(c->occ_bits, c->sdf, c->W, c->H, c->side));
-        HIP_TRY(c, hipEventRecord(c->ev_join, c->side));
+    // One stream: work that could overlap shares a launch instead (kernels
+    // on several captured streams wait microseconds for each other across
+    // hardware queues).  A profiled frame runs the stages one by one.
+    if (!prof) {
+        // mip levels 3..6 beside the near distance field; top levels; near
+        // and far cones in one launch (the network kernel appends the far
+        // segment)
+        HIP_TRY(c, launch_mip_mid_near(c->mip, c->occ_bits, c->sdf, c->W, c->H, s));
+        HIP_TRY(c, launch_mip_top(c->mip, c->flag_count, s));
+        HIP_TRY(c, launch_cone_pair(cone_args_near(c, false), cone_args_far(c), s));
     } else {
         tm.begin(RC_STAGE_NEAR_SDF);
         HIP_TRY(c, launch_near_sdf(c->occ_bits, c->sdf, c->W, c->H, s));
         tm.end(RC_STAGE_NEAR_SDF);
-    }
-    if (fork) {
-        // near cones (mip levels <= 4, from k_mip_mid) beside the top mip
-        // levels and the far cones; the network kernel appends the far segment
-        HIP_TRY(c, launch_mip_mid(c->mip, s));
-        HIP_TRY(c, hipEventRecord(c->ev_fork2, s));
-        HIP_TRY(c, hipStreamWaitEvent(c->side2, c->ev_fork2, 0));
-        HIP_TRY(c, launch_mip_top(c->mip, c->flag_count, c->side2));
-        rc_status st = launch_cone_far(c, c->side2);
-        if (st != RC_OK) return st;
-        HIP_TRY(c, hipEventRecord(c->ev_join2, c->side2));
-        st = launch_cone_near(c, false, s);
-        if (st != RC_OK) return st;
-        HIP_TRY(c, hipStreamWaitEvent(s, c->ev_join2, 0));
-        HIP_TRY(c, hipStreamWaitEvent(s, c->ev_join, 0));
-    } else {
         tm.begin(RC_STAGE_MIP);
         HIP_TRY(c, launch_mip_rest(c->mip, c->flag_count, s));
         tm.end(RC_STAGE_MIP);
         tm.begin(RC_STAGE_CONE);
-        rc_status st = launch_cone_far(c, s);
-        if (st == RC_OK) st = launch_cone_near(c, false, s);
-        if (st != RC_OK) return st;
+        HIP_TRY(c, launch_cone(cone_args_far(c), s));
+        HIP_TRY(c, launch_cone(cone_args_near(c, false), s));
         tm.end(RC_STAGE_CONE);
     }
     SurrMlpArgs a{};
@@ -862,12 +835,6 @@ void rc_destroy(rc_ctx *c)
     for (auto &ev : c->ev)
         if (ev) hipEventDestroy(ev);
     if (c->cap_stream) hipStreamDestroy(c->cap_stream);
-    if (c->side) hipStreamDestroy(c->side);
-    if (c->side2) hipStreamDestroy(c->side2);
-    if (c->ev_fork) hipEventDestroy(c->ev_fork);
-    if (c->ev_join) hipEventDestroy(c->ev_join);
-    if (c->ev_fork2) hipEventDestroy(c->ev_fork2);
-    if (c->ev_join2) hipEventDestroy(c->ev_join2);
     delete c;
 }
 

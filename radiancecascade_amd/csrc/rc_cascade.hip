@@ -224,85 +224,106 @@ __global__ __launch_bounds__(kBlock) void k_level0(LevelArgs a, float *__restric
     const int lb = xcd_remap_c(blockIdx.x, gridDim.x);
     const int px0 = (lb % tiles_x) * TX, py0 = (lb / tiles_x) * TY;
     const int ulx = (px0 - 2) >> 1, uly = (py0 - 2) >> 1;
-    if (UPPER) {
-        const typename TT::T *up = (const typename TT::T *)a.upper;
-        for (int i = threadIdx.x; i < UCX * UCY * 4; i += kBlock) {
-            const int j = i & 3, sx = (i >> 2) % UCX, sy = (i >> 2) / UCX;
-            const int ux = min(max(ulx + sx, 0), a.UX - 1), uy = min(max(uly + sy, 0), a.UY - 1);
-            su[i] = TT::unpack(up[((size_t)uy * a.UX + ux) * 4 + j]);
-        }
-        __syncthreads();
-    }
-
     const size_t plane = (size_t)a.W * a.H;
     const int slot = threadIdx.x;
     const int sx = slot % NX, sy = slot / NX;
     const int px = px0 - 1 + sx, py = py0 - 1 + sy;
-    if (slot < NX * NY && px >= 0 && py >= 0 && px < a.PX && py < a.PY) {
-        const __amdgpu_buffer_rsrc_t sdf = sdf_rsrc(a.sdf, a.W, a.H);
-        const int tiles = sdf_tiles_x(a.W);
-        const float cx = ((float)px + 0.5f) * 2.0f, cy = ((float)py + 0.5f) * 2.0f;
-        int code[4] = {kRayT1, kRayT1, kRayT1, kRayT1};
-        const int ix = __float2int_rd(cx), iy = __float2int_rd(cy);
-        if (!((uint32_t)ix < (uint32_t)a.W && (uint32_t)iy < (uint32_t)a.H)) {
+    const bool probe = slot < NX * NY && px >= 0 && py >= 0 && px < a.PX && py < a.PY;
+    const __amdgpu_buffer_rsrc_t sdf = sdf_rsrc(a.sdf, a.W, a.H);
+    const int tiles = sdf_tiles_x(a.W);
+    const float cx = ((float)px + 0.5f) * 2.0f, cy = ((float)py + 0.5f) * 2.0f;
+    const int ix = __float2int_rd(cx), iy = __float2int_rd(cy);
+    const bool in0 = probe && (uint32_t)ix < (uint32_t)a.W && (uint32_t)iy < (uint32_t)a.H;
+    // the first distance sample is issued before the upper level's loads, so
+    // the march does not wait for them; they land in LDS after the march
+    const float d0 = in0 ? sdf_load(sdf, ix, iy, tiles) : 0.0f;
+    constexpr int UN = UPPER ? (UCX * UCY * 4 + kBlock - 1) / kBlock : 1;
+    typename TT::T ureg[UN];
+    if (UPPER) {
+        const typename TT::T *up = (const typename TT::T *)a.upper;
+#pragma unroll
+        for (int n = 0; n < UN; ++n) {
+            const int i = threadIdx.x + n * kBlock;
+            if (i < UCX * UCY * 4) {
+                const int j = i & 3, ux0 = (i >> 2) % UCX, uy0 = (i >> 2) / UCX;
+                const int ux = min(max(ulx + ux0, 0), a.UX - 1), uy = min(max(uly + uy0, 0), a.UY - 1);
+                ureg[n] = up[((size_t)uy * a.UX + ux) * 4 + j];
+            }
+        }
+    }
+
+    int code[4] = {kRayT1, kRayT1, kRayT1, kRayT1};
+    if (probe) {
+        if (!in0) {
 #pragma unroll
             for (int k = 0; k < 4; ++k) code[k] = kRayExit;
-        } else {
-            const float d0 = sdf_load(sdf, ix, iy, tiles);
-            if (d0 < 0.5f) {
-                const int q0 = pixel_index(ix, iy, a.W);
+        } else if (d0 < 0.5f) {
+            const int q0 = pixel_index(ix, iy, a.W);
 #pragma unroll
-                for (int k = 0; k < 4; ++k) code[k] = q0;
-            } else if (march_step(d0) < 2.0f) {
-                const float t = march_step(d0);
+            for (int k = 0; k < 4; ++k) code[k] = q0;
+        } else if (march_step(d0) < 2.0f) {
+            const float t = march_step(d0);
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const float2 dir = a.dirs[k];
-                    const int jx = __float2int_rd(fmaf(t, dir.x, cx)), jy = __float2int_rd(fmaf(t, dir.y, cy));
-                    if (!((uint32_t)jx < (uint32_t)a.W && (uint32_t)jy < (uint32_t)a.H)) {
-                        code[k] = kRayExit;
-                    } else if (sdf_load(sdf, jx, jy, tiles) < 0.5f) {
-                        code[k] = pixel_index(jx, jy, a.W);
-                    }
+            for (int k = 0; k < 4; ++k) {
+                const float2 dir = a.dirs[k];
+                const int jx = __float2int_rd(fmaf(t, dir.x, cx)), jy = __float2int_rd(fmaf(t, dir.y, cy));
+                if (!((uint32_t)jx < (uint32_t)a.W && (uint32_t)jy < (uint32_t)a.H)) {
+                    code[k] = kRayExit;
+                } else if (sdf_load(sdf, jx, jy, tiles) < 0.5f) {
+                    code[k] = pixel_index(jx, jy, a.W);
                 }
             }
         }
-        float r[4], g[4], b[4], al[4];
-        if (code[0] >= 0 && code[0] == code[1] && code[0] == code[2] && code[0] == code[3]) {
-            // one hit pixel for all 4 directions: load its emission once
-            const float e0 = a.emissive[code[0]], e1 = a.emissive[plane + code[0]], e2 = a.emissive[2 * plane + code[0]];
+    }
+    float r[4], g[4], b[4], al[4];
+    const bool one_hit = code[0] >= 0 && code[0] == code[1] && code[0] == code[2] && code[0] == code[3];
+    if (one_hit) {
+        // one hit pixel for all 4 directions: load its emission once
+        const float e0 = a.emissive[code[0]], e1 = a.emissive[plane + code[0]], e2 = a.emissive[2 * plane + code[0]];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                r[k] = e0;
-                g[k] = e1;
-                b[k] = e2;
-                al[k] = 0.f;
+        for (int k = 0; k < 4; ++k) {
+            r[k] = e0;
+            g[k] = e1;
+            b[k] = e2;
+            al[k] = 0.f;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            r[k] = g[k] = b[k] = 0.f;
+            al[k] = code[k] >= 0 ? 0.f : 1.0f;
+            if (code[k] >= 0) {
+                r[k] = a.emissive[code[k]];
+                g[k] = a.emissive[plane + code[k]];
+                b[k] = a.emissive[2 * plane + code[k]];
             }
-        } else {
+        }
+    }
+    if (UPPER) {
+#pragma unroll
+        for (int n = 0; n < UN; ++n) {
+            const int i = threadIdx.x + n * kBlock;
+            if (i < UCX * UCY * 4) su[i] = TT::unpack(ureg[n]);
+        }
+        __syncthreads();
+        if (probe && !one_hit) {
             const int usx = ((px - 1) >> 1) - ulx, usy = ((py - 1) >> 1) - uly;
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                if (code[k] >= 0) {
-                    r[k] = a.emissive[code[k]];
-                    g[k] = a.emissive[plane + code[k]];
-                    b[k] = a.emissive[2 * plane + code[k]];
-                    al[k] = 0.f;
-                } else {
-                    al[k] = 1.0f;
-                    r[k] = g[k] = b[k] = 0.f;
-                    if (UPPER) {
-                        const float4 u00 = su[(usy * UCX + usx) * 4 + k];
-                        const float4 u10 = su[(usy * UCX + usx + 1) * 4 + k];
-                        const float4 u01 = su[((usy + 1) * UCX + usx) * 4 + k];
-                        const float4 u11 = su[((usy + 1) * UCX + usx + 1) * 4 + k];
-                        const float3 v = bilerp4(u00, u10, u01, u11, px, py);
-                        r[k] = v.x;
-                        g[k] = v.y;
-                        b[k] = v.z;
-                    }
+                if (code[k] < 0) {
+                    const float4 u00 = su[(usy * UCX + usx) * 4 + k];
+                    const float4 u10 = su[(usy * UCX + usx + 1) * 4 + k];
+                    const float4 u01 = su[((usy + 1) * UCX + usx) * 4 + k];
+                    const float4 u11 = su[((usy + 1) * UCX + usx + 1) * 4 + k];
+                    const float3 v = bilerp4(u00, u10, u01, u11, px, py);
+                    r[k] = v.x;
+                    g[k] = v.y;
+                    b[k] = v.z;
                 }
             }
         }
+    }
+    if (probe) {
         const float4 f = make_float4(((r[0] + r[1]) + (r[2] + r[3])) * 0.25f, ((g[0] + g[1]) + (g[2] + g[3])) * 0.25f,
                                      ((b[0] + b[1]) + (b[2] + b[3])) * 0.25f,
                                      ((al[0] + al[1]) + (al[2] + al[3])) * 0.25f);

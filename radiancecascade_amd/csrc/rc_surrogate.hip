@@ -161,12 +161,12 @@ hipError_t launch_seed_surr(const float *depth, const float *emissive, uint64_t 
 // zero bits), then d^2 = min over |dy| <= 10 of g(y+dy)^2 + dy^2.
 constexpr int kNearR = 10, kNearT = 32, kNearS = kNearT + 2 * kNearR;
 
-__global__ __launch_bounds__(kBlock) void k_near_sdf(const uint64_t *__restrict__ bits, int words,
-                                                     uint8_t *__restrict__ sdf, int W, int H, int tiles_x)
+__device__ inline void near_sdf_block(int blk, const uint64_t *__restrict__ bits, int words, uint8_t *__restrict__ sdf,
+                                      int W, int H, int tiles_x)
 {
     __shared__ uint64_t mask[kNearS];
     __shared__ uint8_t g[kNearS][kNearT];
-    const int x0 = (blockIdx.x % tiles_x) * kNearT, y0 = (blockIdx.x / tiles_x) * kNearT;
+    const int x0 = (blk % tiles_x) * kNearT, y0 = (blk / tiles_x) * kNearT;
     if (threadIdx.x < kNearS) {
         const int r = threadIdx.x, y = y0 - kNearR + r;
         uint64_t mk = 0;
@@ -209,6 +209,12 @@ __global__ __launch_bounds__(kBlock) void k_near_sdf(const uint64_t *__restrict_
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_near_sdf(const uint64_t *__restrict__ bits, int words,
+                                                     uint8_t *__restrict__ sdf, int W, int H, int tiles_x)
+{
+    near_sdf_block(blockIdx.x, bits, words, sdf, W, H, tiles_x);
+}
+
 hipError_t launch_near_sdf(const uint64_t *bits, uint8_t *sdf, int W, int H, hipStream_t s)
 {
     const int tx = (W + kNearT - 1) / kNearT, ty = (H + kNearT - 1) / kNearT;
@@ -246,10 +252,10 @@ size_t mip_cells(const MipArgs &m) { return m.off[m.levels] + (size_t)(m.w[m.lev
 // Levels 3..6 of a 16x16 block of level-2 cells (64x64 pixels) in LDS.
 constexpr int kMidTile = 16, kMidLast = 6;
 
-__global__ __launch_bounds__(kBlock) void k_mip_mid(MipArgs m, int tiles_x)
+__device__ inline void mip_mid_block(int blk, const MipArgs &m, int tiles_x)
 {
     __shared__ uint4 se[kMidTile * kMidTile];
-    const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
+    const int tx = blk % tiles_x, ty = blk / tiles_x;
     {
         const int cx = threadIdx.x % kMidTile, cy = threadIdx.x / kMidTile;
         const int gx = tx * kMidTile + cx, gy = ty * kMidTile + cy;
@@ -331,6 +337,29 @@ __global__ __launch_bounds__(kBlock) void k_mip_top(MipArgs m, int *zero_me)
         __threadfence_block();
         __syncthreads();
     }
+}
+
+__global__ __launch_bounds__(kBlock) void k_mip_mid(MipArgs m, int tiles_x) { mip_mid_block(blockIdx.x, m, tiles_x); }
+
+// Mip levels 3..6 and the near distance field in one launch (both read only
+// the seed kernel's output): blocks [0, n_mid) are mip tiles, the rest
+// near-field tiles.
+__global__ __launch_bounds__(kBlock) void k_mip_mid_near(MipArgs m, int mid_tiles_x, int n_mid,
+                                                         const uint64_t *__restrict__ bits, int words,
+                                                         uint8_t *__restrict__ sdf, int W, int H, int near_tiles_x)
+{
+    if ((int)blockIdx.x < n_mid) mip_mid_block(blockIdx.x, m, mid_tiles_x);
+    else near_sdf_block((int)blockIdx.x - n_mid, bits, words, sdf, W, H, near_tiles_x);
+}
+
+hipError_t launch_mip_mid_near(const MipArgs &m, const uint64_t *bits, uint8_t *sdf, int W, int H, hipStream_t s)
+{
+    const int mtx = m.levels >= 3 ? (m.w[2] + kMidTile - 1) / kMidTile : 0;
+    const int n_mid = m.levels >= 3 ? mtx * ((m.h[2] + kMidTile - 1) / kMidTile) : 0;
+    const int ntx = (W + kNearT - 1) / kNearT, n_near = ntx * ((H + kNearT - 1) / kNearT);
+    hipLaunchKernelGGL(k_mip_mid_near, dim3(n_mid + n_near), dim3(kBlock), 0, s, m, mtx, n_mid, bits, (W + 63) / 64,
+                       sdf, W, H, ntx);
+    return hipGetLastError();
 }
 
 hipError_t launch_mip_mid(const MipArgs &m, hipStream_t s)
@@ -438,10 +467,9 @@ __device__ inline int xcd_chunk(int b, int nb)
 // distance, so from mip level 3 up they share cells (and cache lines).  The 4
 // taps of a step are one 32-bit offset and 4 range-checked buffer loads (the
 // zero border stands in for taps beyond the image edge).
-__global__ __launch_bounds__(kBlock) void k_cone(ConeArgs a, int seg_x)
+__device__ inline void cone_block(const ConeArgs &a, int lb, int seg_x)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int lb = xcd_chunk(blockIdx.x, gridDim.x);
     const int kgrp = lb / (seg_x * a.QY), rest = lb % (seg_x * a.QY);
     const int qy = rest / seg_x, seg = rest % seg_x;
     const int k = kgrp * 4 + wave;
@@ -453,8 +481,9 @@ __global__ __launch_bounds__(kBlock) void k_cone(ConeArgs a, int seg_x)
     const float2 d = a.dirs[k];
     const float dx2 = d.x * d.x, dy2 = d.y * d.y;
     float T = 1.0f, r = 0.f, g = 0.f, b = 0.f;
-    for (int n = 0; n < a.sch.n; ++n) {
-        const ConeStep st = a.sch.s[n];
+    const int nsteps = a.sch->n;
+    for (int n = 0; n < nsteps; ++n) {
+        const ConeStep st = a.sch->s[n];
         const float fx = fmaf(st.t, d.x, cx) * st.inv_cell - 0.5f, fy = fmaf(st.t, d.y, cy) * st.inv_cell - 0.5f;
         const float flx = floorf(fx), fly = floorf(fy);
         const int ix = (int)flx, iy = (int)fly;
@@ -498,10 +527,39 @@ __global__ __launch_bounds__(kBlock) void k_cone(ConeArgs a, int seg_x)
     a.cone[((size_t)qy * a.QX + qx) * 16 + k] = make_float4(r, g, b, T);
 }
 
+__global__ __launch_bounds__(kBlock) void k_cone(ConeArgs a, int seg_x)
+{
+    cone_block(a, xcd_chunk(blockIdx.x, gridDim.x), seg_x);
+}
+
+// Far and near cones in one launch: blocks [0, nb_far) (a multiple of 8, so
+// the near blocks keep the XCD pattern) are far cones, dispatched first as
+// they take more steps; each part is dealt to the XCDs in contiguous runs.
+__global__ __launch_bounds__(kBlock) void k_cone_pair(ConeArgs near, int seg_near, int nb_near, ConeArgs far, int seg_far,
+                                                      int nb_far_used, int nb_far)
+{
+    const int b = blockIdx.x;
+    if (b < nb_far) {
+        const int lb = xcd_chunk(b, nb_far);
+        if (lb < nb_far_used) cone_block(far, lb, seg_far);
+    } else {
+        cone_block(near, xcd_chunk(b - nb_far, nb_near), seg_near);
+    }
+}
+
 hipError_t launch_cone(const ConeArgs &a, hipStream_t s)
 {
     const int seg_x = (a.QX + 63) / 64;
     hipLaunchKernelGGL(k_cone, dim3(seg_x * a.QY * 4), dim3(kBlock), 0, s, a, seg_x);
+    return hipGetLastError();
+}
+
+hipError_t launch_cone_pair(const ConeArgs &near, const ConeArgs &far, hipStream_t s)
+{
+    const int seg_n = (near.QX + 63) / 64, seg_f = (far.QX + 63) / 64;
+    const int nb_n = seg_n * near.QY * 4, nb_f_used = seg_f * far.QY * 4, nb_f = (nb_f_used + 7) / 8 * 8;
+    hipLaunchKernelGGL(k_cone_pair, dim3(nb_f + nb_n), dim3(kBlock), 0, s, near, seg_n, nb_n, far, seg_f, nb_f_used,
+                       nb_f);
     return hipGetLastError();
 }
 
@@ -524,15 +582,27 @@ __device__ inline float feat_log(float c) { return __logf(1.0f + c); }
 __device__ inline bf16x8 ld8s(const uint16_t *p) { return *reinterpret_cast<const bf16x8 *>(p); }
 __device__ inline int acc_row_s(int v, int lane) { return ((v >> 2) << 3) + ((lane >> 5) << 2) + (v & 3); }
 
+// The wave's B fragments of one layer (columns c0..c0+31, all K rows), held
+// in registers across the tiles a workgroup processes.
+template <int K> struct BFrag {
+    bf16x8 f[K / 16];
+};
+template <int K> __device__ inline BFrag<K> load_bfrag(const uint16_t *w, int c0)
+{
+    const int lane = threadIdx.x & 63, r = lane & 31, kh = (lane >> 5) * 8;
+    BFrag<K> b;
+#pragma unroll
+    for (int i = 0; i < K / 16; ++i) b.f[i] = ld8s(w + (c0 + r) * K + 16 * i + kh);
+    return b;
+}
 template <int K>
-__device__ inline floatx16 mfma_block(const uint16_t *in, int sin, int rb, const uint16_t *w, int c0)
+__device__ inline floatx16 mfma_block(const uint16_t *in, int sin, int rb, const BFrag<K> &b)
 {
     const int lane = threadIdx.x & 63, r = lane & 31, kh = (lane >> 5) * 8;
     floatx16 c = {};
 #pragma unroll
-    for (int k0 = 0; k0 < K; k0 += 16)
-        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ld8s(in + (rb + r) * sin + k0 + kh), ld8s(w + (c0 + r) * K + k0 + kh),
-                                                    c, 0, 0, 0);
+    for (int i = 0; i < K / 16; ++i)
+        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ld8s(in + (rb + r) * sin + 16 * i + kh), b.f[i], c, 0, 0, 0);
     return c;
 }
 
@@ -557,13 +627,17 @@ __global__ __launch_bounds__(kBlock) void k_surr_mlp(SurrMlpArgs a, MlpDev m)
         istd_s[i] = m.in_istd[i];
     }
     __syncthreads();
-    // B operands straight from the bf16 weights in global memory (30 KB, held
-    // in every CU's L1 and the L2)
+    // B operands from the bf16 weights in global memory (30 KB, held in every
+    // CU's L1 and the L2)
     const uint16_t *wl[4] = {m.wt, m.wt + 64 * kMlpIn, m.wt + 64 * kMlpIn + 64 * kMlpHid,
                              m.wt + 64 * kMlpIn + 2 * 64 * kMlpHid};
     const int p = threadIdx.x >> 2, ko = threadIdx.x & 3;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int rb = (wave & 1) * 32, c0 = (wave >> 1) * 32;
+    // all four layers' B fragments of this wave, loaded once (60 VGPRs)
+    const BFrag<kMlpIn> b0 = load_bfrag<kMlpIn>(wl[0], c0);
+    const BFrag<kMlpHid> b1 = load_bfrag<kMlpHid>(wl[1], c0), b2 = load_bfrag<kMlpHid>(wl[2], c0),
+                         b3 = load_bfrag<kMlpHid>(wl[3], c0);
 
     // tiles are dealt XCD by XCD in contiguous ranges
     const int per_xcd = (int)gridDim.x >> 3, xcd = blockIdx.x & 7, idx = blockIdx.x >> 3;
@@ -598,7 +672,7 @@ __global__ __launch_bounds__(kBlock) void k_surr_mlp(SurrMlpArgs a, MlpDev m)
 
         // MLP, 4 layers; wave (row block, column block)
         {
-            floatx16 acc = mfma_block<kMlpIn>(xs, kSStrideIn, rb, wl[0], c0);
+            floatx16 acc = mfma_block<kMlpIn>(xs, kSStrideIn, rb, b0);
             const float bb = bias_s[0 * 64 + c0 + (lane & 31)];
 #pragma unroll
             for (int v = 0; v < 16; ++v)
@@ -607,7 +681,7 @@ __global__ __launch_bounds__(kBlock) void k_surr_mlp(SurrMlpArgs a, MlpDev m)
         }
         __syncthreads();
         {
-            floatx16 acc = mfma_block<kMlpHid>(ha, kSStrideHid, rb, wl[1], c0);
+            floatx16 acc = mfma_block<kMlpHid>(ha, kSStrideHid, rb, b1);
             const float bb = bias_s[1 * 64 + c0 + (lane & 31)];
 #pragma unroll
             for (int v = 0; v < 16; ++v)
@@ -616,7 +690,7 @@ __global__ __launch_bounds__(kBlock) void k_surr_mlp(SurrMlpArgs a, MlpDev m)
         }
         __syncthreads();
         {
-            floatx16 acc = mfma_block<kMlpHid>(hb, kSStrideHid, rb, wl[2], c0);
+            floatx16 acc = mfma_block<kMlpHid>(hb, kSStrideHid, rb, b2);
             const float bb = bias_s[2 * 64 + c0 + (lane & 31)];
 #pragma unroll
             for (int v = 0; v < 16; ++v)
@@ -625,7 +699,7 @@ __global__ __launch_bounds__(kBlock) void k_surr_mlp(SurrMlpArgs a, MlpDev m)
         }
         __syncthreads();
         {
-            floatx16 acc = mfma_block<kMlpHid>(ha, kSStrideHid, rb, wl[3], c0);
+            floatx16 acc = mfma_block<kMlpHid>(ha, kSStrideHid, rb, b3);
             const int col = c0 + (lane & 31);
             const float bb = bias_s[3 * 64 + col];
             if (col <= kMlpErr) {
