@@ -81,26 +81,50 @@ int jfa_tail_step(int W, int H, int fuse)
     return top < fuse ? top : fuse;
 }
 
-// 64 x 4 pixel blocks, x fastest so each wave covers 64 consecutive pixels.
+// Occupancy of pixel (x, y): any emissive channel > 0, or a 4-neighbour q
+// with D(q) - D(p) > 0.05 D(p).  All loads are unconditional (clamped
+// addresses) so they are in flight together; an out-of-image neighbour reads
+// p itself and is masked out.
+__device__ inline bool occupied(const float *__restrict__ depth, const float *__restrict__ emissive, int x, int y, int W,
+                                int H, float &e0, float &e1, float &e2)
+{
+    const size_t plane = (size_t)W * H, p = (size_t)y * W + x;
+    e0 = emissive[p];
+    e1 = emissive[plane + p];
+    e2 = emissive[2 * plane + p];
+    const float dp = depth[p];
+    const float dl = depth[p - (x > 0)], dr = depth[p + (x + 1 < W)];
+    const float du = depth[p - (y > 0 ? (size_t)W : 0)], dd = depth[p + (y + 1 < H ? (size_t)W : 0)];
+    const float thr = 0.05f * dp;
+    return e0 > 0.0f || e1 > 0.0f || e2 > 0.0f || (x > 0 && dl - dp > thr) || (x + 1 < W && dr - dp > thr) ||
+           (y > 0 && du - dp > thr) || (y + 1 < H && dd - dp > thr);
+}
+
+// 64 x 16 pixel blocks: a wave covers 64 consecutive pixels of rows
+// w, w + 4, w + 8, w + 12; a thread's 4 pixels have their loads in flight at
+// once (one pixel per thread leaves the kernel waiting on load latency).
+constexpr int kSeedRows = 16;
+
 __global__ __launch_bounds__(kBlock) void k_seed_init(const float *__restrict__ depth,
                                                       const float *__restrict__ emissive,
                                                       uint32_t *__restrict__ seed, int W, int H, int bx_count)
 {
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int x = (b % bx_count) * 64 + (threadIdx.x & 63);
-    const int y = (b / bx_count) * 4 + (threadIdx.x >> 6);
-    if (x >= W || y >= H) return;
-    const size_t plane = (size_t)W * H, p = (size_t)y * W + x;
-    // all loads unconditional (clamped addresses) so they are in flight together
-    const float e0 = emissive[p], e1 = emissive[plane + p], e2 = emissive[2 * plane + p];
-    const float dp = depth[p];
-    const float dl = depth[p - (x > 0)], dr = depth[p + (x + 1 < W)];
-    const float du = depth[p - (y > 0 ? (size_t)W : 0)], dd = depth[p + (y + 1 < H ? (size_t)W : 0)];
-    const float thr = 0.05f * dp;
-    // an out-of-image neighbour reads p itself and is masked out
-    const bool occ = e0 > 0.0f || e1 > 0.0f || e2 > 0.0f || (x > 0 && dl - dp > thr) ||
-                     (x + 1 < W && dr - dp > thr) || (y > 0 && du - dp > thr) || (y + 1 < H && dd - dp > thr);
-    seed[p] = occ ? (((uint32_t)y << 16) | (uint32_t)x) : kFarSeed;
+    const int y0 = (b / bx_count) * kSeedRows + (threadIdx.x >> 6);
+    if (x >= W) return;
+    bool occ[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int y = min(y0 + 4 * j, H - 1);
+        float e0, e1, e2;
+        occ[j] = occupied(depth, emissive, x, y, W, H, e0, e1, e2);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int y = y0 + 4 * j;
+        if (y < H) seed[(size_t)y * W + x] = occ[j] ? (((uint32_t)y << 16) | (uint32_t)x) : kFarSeed;
+    }
 }
 
 // One global pass.  A wave owns 64 consecutive pixels of one row (blocks of
@@ -274,7 +298,7 @@ hipError_t launch_sdf_untile(const uint8_t *sdf, float *out, int W, int H, hipSt
 
 hipError_t launch_seed_init(const float *depth, const float *emissive, uint32_t *seed, int W, int H, hipStream_t s)
 {
-    const int bxc = (W + 63) / 64, byc = (H + 3) / 4;
+    const int bxc = (W + 63) / 64, byc = (H + kSeedRows - 1) / kSeedRows;
     hipLaunchKernelGGL(k_seed_init, dim3(bxc * byc), dim3(kBlock), 0, s, depth, emissive, seed, W, H, bxc);
     return hipGetLastError();
 }

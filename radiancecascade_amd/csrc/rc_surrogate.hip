@@ -88,56 +88,74 @@ __device__ inline uint4 combine4(uint4 tl, uint4 tr, uint4 bl, uint4 br)
 }
 
 // ---- occupancy bits + mip levels 1 and 2 -------------------------------------------
-// One 64x4-pixel block (a wave per row): the occupancy rule of the jump
-// flood's seed pass (emissive > 0 in any channel, or the near side of a depth
-// edge), one 64-bit ballot per row into bits[y][x/64], and the block's mip
-// cells: 32x2 of level 1 and 16x1 of level 2.  Pixels outside the image are
-// empty.
+// One 64x16-pixel block: wave w covers 64 consecutive pixels of rows w, w+4,
+// w+8, w+12 (a thread's 4 pixels have their loads in flight at once).  The
+// occupancy rule of the jump flood's seed pass (emissive > 0 in any channel,
+// or the near side of a depth edge), one 64-bit ballot per row into
+// bits[y][x/64], and the block's mip cells: 32x8 of level 1 (one per
+// thread) and 16x4 of level 2.  Pixels outside the image are empty.
+constexpr int kSurrSeedRows = 16;
+
+// A pixel as a mip cell: rgb E*[emitter]; kind 0 empty, 1 emitter, 2 dark occluder.
+__device__ inline MipCell pixel_cell(float4 p)
+{
+    const uint32_t kind = __float_as_uint(p.w);
+    const float f = kind == 1u ? 1.0f : 0.0f, o = kind == 2u ? 1.0f : 0.0f;
+    return MipCell{{p.x, p.y, p.z, f, f, f, o, o}};
+}
+
 __global__ __launch_bounds__(kBlock) void k_seed_surr(const float *__restrict__ depth, const float *__restrict__ emissive,
                                                       uint64_t *__restrict__ bits, int words, MipArgs m, int W, int H,
                                                       int bx_count)
 {
-    __shared__ MipCell pe[4][64];
-    __shared__ uint4 c1[2][32];
+    __shared__ float4 pe[kSurrSeedRows][64];
+    __shared__ uint4 c1[kSurrSeedRows / 2][32];
     const int bx = blockIdx.x % bx_count, by = blockIdx.x / bx_count;
-    const int lane = threadIdx.x & 63, row = threadIdx.x >> 6;
-    const int x = bx * 64 + lane, y = by * 4 + row;
-    const bool in = x < W && y < H;
-    bool occ = false, em = false;
-    float e0 = 0.f, e1 = 0.f, e2 = 0.f;
-    if (in) {
-        const size_t plane = (size_t)W * H, p = (size_t)y * W + x;
-        e0 = emissive[p];
-        e1 = emissive[plane + p];
-        e2 = emissive[2 * plane + p];
-        const float dp = depth[p];
-        const float dl = depth[p - (x > 0)], dr = depth[p + (x + 1 < W)];
-        const float du = depth[p - (y > 0 ? (size_t)W : 0)], dd = depth[p + (y + 1 < H ? (size_t)W : 0)];
-        const float thr = 0.05f * dp;
-        em = e0 > 0.0f || e1 > 0.0f || e2 > 0.0f;
-        occ = em || (x > 0 && dl - dp > thr) || (x + 1 < W && dr - dp > thr) || (y > 0 && du - dp > thr) ||
-              (y + 1 < H && dd - dp > thr);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int x = bx * 64 + lane;
+    const size_t plane = (size_t)W * H;
+    float e[4][3], dp[4], dl[4], dr[4], du[4], dd[4];
+    // all loads first (clamped addresses; out-of-image pixels are masked below)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int y = min(by * kSurrSeedRows + w + 4 * j, H - 1), xc = min(x, W - 1);
+        const size_t p = (size_t)y * W + xc;
+        e[j][0] = emissive[p];
+        e[j][1] = emissive[plane + p];
+        e[j][2] = emissive[2 * plane + p];
+        dp[j] = depth[p];
+        dl[j] = depth[p - (xc > 0)];
+        dr[j] = depth[p + (xc + 1 < W)];
+        du[j] = depth[p - (y > 0 ? (size_t)W : 0)];
+        dd[j] = depth[p + (y + 1 < H ? (size_t)W : 0)];
     }
-    const uint64_t w = __ballot(occ);
-    if (lane == 0 && y < H) bits[(size_t)y * words + bx] = w;
-    {
-        const float o = (occ && !em) ? 1.0f : 0.0f, f = em ? 1.0f : 0.0f;
-        pe[row][lane] = MipCell{{em ? e0 : 0.f, em ? e1 : 0.f, em ? e2 : 0.f, f, f, f, o, o}};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int row = w + 4 * j, y = by * kSurrSeedRows + row;
+        const bool in = x < W && y < H;
+        const float thr = 0.05f * dp[j];
+        const bool em = in && (e[j][0] > 0.0f || e[j][1] > 0.0f || e[j][2] > 0.0f);
+        const bool occ = em || (in && ((x > 0 && dl[j] - dp[j] > thr) || (x + 1 < W && dr[j] - dp[j] > thr) ||
+                                       (y > 0 && du[j] - dp[j] > thr) || (y + 1 < H && dd[j] - dp[j] > thr)));
+        const uint64_t bw = __ballot(occ);
+        if (lane == 0 && y < H) bits[(size_t)y * words + bx] = bw;
+        pe[row][lane] = em ? make_float4(e[j][0], e[j][1], e[j][2], __uint_as_float(1u))
+                           : make_float4(0.f, 0.f, 0.f, __uint_as_float(occ ? 2u : 0u));
     }
     __syncthreads();
-    if (threadIdx.x < 64) {
+    {
         const int cx = threadIdx.x & 31, cy = threadIdx.x >> 5;
-        const uint4 t =
-            pack_cell(combine4(pe[2 * cy][2 * cx], pe[2 * cy][2 * cx + 1], pe[2 * cy + 1][2 * cx], pe[2 * cy + 1][2 * cx + 1]));
+        const uint4 t = pack_cell(combine4(pixel_cell(pe[2 * cy][2 * cx]), pixel_cell(pe[2 * cy][2 * cx + 1]),
+                                           pixel_cell(pe[2 * cy + 1][2 * cx]), pixel_cell(pe[2 * cy + 1][2 * cx + 1])));
         c1[cy][cx] = t;
-        const int gx = bx * 32 + cx, gy = by * 2 + cy;
+        const int gx = bx * 32 + cx, gy = by * (kSurrSeedRows / 2) + cy;
         if (gx < m.w[1] && gy < m.h[1]) m.tex[mip_at(m, 1, gx, gy)] = t;
     }
     __syncthreads();
-    if (threadIdx.x < 16 && m.levels >= 2) {
-        const int cx = threadIdx.x;
-        const uint4 t = combine4(c1[0][2 * cx], c1[0][2 * cx + 1], c1[1][2 * cx], c1[1][2 * cx + 1]);
-        const int gx = bx * 16 + cx, gy = by;
+    if (threadIdx.x < 64 && m.levels >= 2) {
+        const int cx = threadIdx.x & 15, cy = threadIdx.x >> 4;
+        const uint4 t = combine4(c1[2 * cy][2 * cx], c1[2 * cy][2 * cx + 1], c1[2 * cy + 1][2 * cx], c1[2 * cy + 1][2 * cx + 1]);
+        const int gx = bx * 16 + cx, gy = by * (kSurrSeedRows / 4) + cy;
         if (gx < m.w[2] && gy < m.h[2]) m.tex[mip_at(m, 2, gx, gy)] = t;
     }
 }
@@ -145,7 +163,7 @@ __global__ __launch_bounds__(kBlock) void k_seed_surr(const float *__restrict__ 
 hipError_t launch_seed_surr(const float *depth, const float *emissive, uint64_t *bits, const MipArgs &m, int W, int H,
                             hipStream_t s)
 {
-    const int bxc = (W + 63) / 64, byc = (H + 3) / 4;
+    const int bxc = (W + 63) / 64, byc = (H + kSurrSeedRows - 1) / kSurrSeedRows;
     hipLaunchKernelGGL(k_seed_surr, dim3(bxc * byc), dim3(kBlock), 0, s, depth, emissive, bits, bxc, m, W, H, bxc);
     return hipGetLastError();
 }
@@ -506,11 +524,22 @@ __device__ inline void cone_block(const ConeArgs &a, int lb, int seg_x)
             for (int i = 0; i < 8; ++i) acc[i] = fmaf(wt[tap], c.v[i], acc[i]);
         }
         const float ce = fmaf(dy2, acc[4], dx2 * acc[5]), co = fmaf(dy2, acc[6], dx2 * acc[7]);
-        // 1 - (1 - c)^r with the hardware log2 / exp2 (argument in [1e-3, 1])
-        const float ae = 1.0f - __builtin_amdgcn_exp2f(st.re * __builtin_amdgcn_logf(1.0f - fminf(ce, 0.999f)));
-        const float ao = 1.0f - __builtin_amdgcn_exp2f(st.ro * __builtin_amdgcn_logf(1.0f - fminf(co, 0.999f)));
+        // 1 - (1 - c)^r; a step of one whole cell (r = 1, occluder r = 3) needs
+        // no transcendental, the others use the hardware log2 / exp2
+        // (argument in [1e-3, 1]); the branch is uniform (one step for all lanes)
+        float ae, ao;
+        static_assert(kConeOccGain == 3.0, "the whole-cell path cubes");
+        if (st.re == 1.0f) {
+            const float xo = 1.0f - fminf(co, 0.999f);
+            ae = fminf(ce, 0.999f);
+            ao = 1.0f - xo * xo * xo;
+        } else {
+            ae = 1.0f - __builtin_amdgcn_exp2f(st.re * __builtin_amdgcn_logf(1.0f - fminf(ce, 0.999f)));
+            ao = 1.0f - __builtin_amdgcn_exp2f(st.ro * __builtin_amdgcn_logf(1.0f - fminf(co, 0.999f)));
+        }
         const float al = fminf(ae + ao, 1.0f);
-        const float we = acc[3] > 0.0f ? T * ae / acc[3] : 0.0f;
+        // T a_e / emitter area, with the hardware reciprocal (1 ulp)
+        const float we = acc[3] > 0.0f ? T * ae * __builtin_amdgcn_rcpf(acc[3]) : 0.0f;
         r = fmaf(we, acc[0], r);
         g = fmaf(we, acc[1], g);
         b = fmaf(we, acc[2], b);
@@ -621,6 +650,28 @@ __global__ __launch_bounds__(kBlock) void k_surr_mlp(SurrMlpArgs a, MlpDev m)
     uint16_t *const hb = xs_hb + 64 * kSStrideIn;
     float(*const zs)[kMlpErr + 2] = reinterpret_cast<float(*)[kMlpErr + 2]>(xs_hb);
 
+    const int p = threadIdx.x >> 2, ko = threadIdx.x & 3;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int rb = (wave & 1) * 32, c0 = (wave >> 1) * 32;
+    // tiles are dealt XCD by XCD in contiguous ranges
+    const int per_xcd = (int)gridDim.x >> 3, xcd = blockIdx.x & 7, idx = blockIdx.x >> 3;
+    const int t_begin = (int)((long long)a.tiles * xcd / 8), t_end = (int)((long long)a.tiles * (xcd + 1) / 8);
+    // the whole cones of this lane's 4 directions of its probe in a tile (near
+    // segment + appended far segment); the first tile's are in flight while
+    // the constants are staged
+    auto load_cones = [&](int tile, float4 *cf) {
+        const int qx = (tile % a.tiles_x) * 8 + (p & 7), qy = (tile / a.tiles_x) * 8 + (p >> 3);
+        const size_t q = (size_t)min(qy, a.QY - 1) * a.QX + min(qx, a.QX - 1);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) cf[j] = a.cone[q * 16 + 4 * ko + j];
+        if (a.far) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                cf[j] = far_append(cf[j], a.far, a.FX, a.FY, min(qx, a.QX - 1), min(qy, a.QY - 1), 4 * ko + j);
+        }
+    };
+    float4 cf[4];
+    if (t_begin + idx < t_end) load_cones(t_begin + idx, cf);
     for (int i = threadIdx.x; i < 4 * 64; i += kBlock) bias_s[i] = m.bias[i];
     for (int i = threadIdx.x; i < kMlpIn; i += kBlock) {
         mean_s[i] = m.in_mean[i];
@@ -631,32 +682,19 @@ __global__ __launch_bounds__(kBlock) void k_surr_mlp(SurrMlpArgs a, MlpDev m)
     // CU's L1 and the L2)
     const uint16_t *wl[4] = {m.wt, m.wt + 64 * kMlpIn, m.wt + 64 * kMlpIn + 64 * kMlpHid,
                              m.wt + 64 * kMlpIn + 2 * 64 * kMlpHid};
-    const int p = threadIdx.x >> 2, ko = threadIdx.x & 3;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int rb = (wave & 1) * 32, c0 = (wave >> 1) * 32;
     // all four layers' B fragments of this wave, loaded once (60 VGPRs)
     const BFrag<kMlpIn> b0 = load_bfrag<kMlpIn>(wl[0], c0);
     const BFrag<kMlpHid> b1 = load_bfrag<kMlpHid>(wl[1], c0), b2 = load_bfrag<kMlpHid>(wl[2], c0),
                          b3 = load_bfrag<kMlpHid>(wl[3], c0);
 
-    // tiles are dealt XCD by XCD in contiguous ranges
-    const int per_xcd = (int)gridDim.x >> 3, xcd = blockIdx.x & 7, idx = blockIdx.x >> 3;
-    const int t_begin = (int)((long long)a.tiles * xcd / 8), t_end = (int)((long long)a.tiles * (xcd + 1) / 8);
 #pragma unroll 1
     for (int tile = t_begin + idx; tile < t_end; tile += per_xcd) {
         const int qx = (tile % a.tiles_x) * 8 + (p & 7), qy = (tile / a.tiles_x) * 8 + (p >> 3);
         const bool valid = qx < a.QX && qy < a.QY;
         const size_t q = (size_t)min(qy, a.QY - 1) * a.QX + min(qx, a.QX - 1);
-        float4 cf[4];
+        if (a.cone_out && valid) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) cf[j] = a.cone[q * 16 + 4 * ko + j];
-        if (a.far) {
-            // the near segment alone was traced: append the far one here
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                cf[j] = far_append(cf[j], a.far, a.FX, a.FY, min(qx, a.QX - 1), min(qy, a.QY - 1), 4 * ko + j);
-                if (a.cone_out && valid) a.cone_out[q * 16 + 4 * ko + j] = cf[j];
-            }
+            for (int j = 0; j < 4; ++j) a.cone_out[q * 16 + 4 * ko + j] = cf[j];
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -721,6 +759,8 @@ __global__ __launch_bounds__(kBlock) void k_surr_mlp(SurrMlpArgs a, MlpDev m)
                 out[q * 16 + k] = TT::pack(t);
             }
         }
+        // the next tile's cones are in flight during the error reduction
+        if (tile + per_xcd < t_end) load_cones(tile + per_xcd, cf);
 
         // tile error estimate
         float e = (valid && ko == 0) ? expf(zs[p][kMlpErr]) : 0.0f;
