@@ -4,7 +4,9 @@
 Workload (default): 1920x1080 G-buffer rasterised on the GPU from the seeded
 box scene (seed 0, 32 boxes, 4 emissive), 6 cascade levels, fp32 storage; a
 step is one full GI frame (occupancy + jump flood + SDF + 6 levels of
-march/merge + gather) with the G-buffer already resident in HBM.
+march/merge + gather) with the G-buffer already resident in HBM, its kernels
+launched one by one on one stream (the library's default; a replayed
+hipGraph leaves ~4 us between frames on this GPU).
 
 Prints ONE JSON line.  `value` is the plain path (the form that meets the
 2% rel-L2 bound: its error against the FP64 oracle is ~1e-4, pinned by the
@@ -109,9 +111,9 @@ BF16_MFMA_PEAK_TFLOPS = 2500.0  # MI355X dense bf16
 
 
 def surrogate_leg(rc, torch, W, H, L, args, depth, emis, irr_plain, stream, fps_plain):
-    """The learned far field on the same G-buffer: frames/s (graph replay),
-    rel-L2 against the plain path's irradiance (GPU FP64 reduction), per-stage
-    times and the MFMA roofline of the level-1 kernel that runs the MLP."""
+    """The learned far field on the same G-buffer: frames/s (direct launches,
+    as the plain path), rel-L2 against the plain path's irradiance (GPU FP64
+    reduction), per-stage times and the MFMA roofline of the network kernel."""
     sur = rc.RadianceCascadeStage(W, H, levels=L, storage=args.storage)
     sur.load_weights(args.weights)
     sur.set_surrogate(True)
@@ -329,7 +331,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"plain HIP path, {W}x{H} G-buffer, {L} cascades, {args.storage} storage, "
                                f"seeded box scene (seed {args.seed}, {args.boxes} boxes, {args.emissive} emissive)",
-                   "path": "plain", "width": W, "height": H, "levels": L, "storage": args.storage,
+                   "path": "plain", "launch": "direct (one stream)", "width": W, "height": H, "levels": L,
+                   "storage": args.storage,
                    "rays_per_frame": int(sum(rays)),
                    "tiling": {str(k): f"{v[0][0]}x{v[0][1]}x{v[0][2]} r{v[1]}" for k, v in tiling.items()},
                    "march": {"split": split, "speculation": spec, "rays_per_lane": rpl}, "jfa_fuse": stage.jfa_fuse()},

@@ -26,7 +26,7 @@ STATUS_NAMES = {0: "RC_OK", 1: "RC_ERR_INVALID", 2: "RC_ERR_NULL", 3: "RC_ERR_NO
 STORAGE = {"f32": 0, "fp32": 0, "f16": 1, "fp16": 1, "bf16": 2}
 FLAG_PROFILE = 0x1
 FLAG_KEEP_DEBUG = 0x2
-FLAG_NO_GRAPH = 0x4
+FLAG_GRAPH = 0x8
 
 TILE_SHAPES = {0: (8, 8, 4), 1: (4, 4, 16), 2: (2, 2, 64), 3: (1, 1, 256)}
 

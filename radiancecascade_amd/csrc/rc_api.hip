@@ -557,11 +557,12 @@ static rc_status run_frame_any(rc_ctx *c, const float *depth, const float *emiss
     return run_frame(c, depth, emissive, irr, s, quiet);
 }
 
-// The frame's 15 launches (at 1080p) replayed as one graph: profiling and
-// debug runs, which record events or extra outputs, launch directly.
+// With RC_FLAG_GRAPH the frame's launches are replayed as one graph;
+// profiling and debug runs, which record events or extra outputs, always
+// launch directly.
 static bool graph_enabled(const rc_ctx *c)
 {
-    return (c->cfg.flags & (RC_FLAG_PROFILE | RC_FLAG_KEEP_DEBUG | RC_FLAG_NO_GRAPH)) == 0;
+    return (c->cfg.flags & RC_FLAG_GRAPH) && !(c->cfg.flags & (RC_FLAG_PROFILE | RC_FLAG_KEEP_DEBUG));
 }
 
 static rc_status run_frame_graph(rc_ctx *c, const float *depth, const float *emissive, float *irr, hipStream_t s)
@@ -745,7 +746,7 @@ rc_status rc_create(const rc_config *cfg, rc_ctx **out)
     if (!dims_ok(cfg->width, cfg->height)) return fail(nullptr, RC_ERR_INVALID, "rc_create: width/height out of range");
     if (cfg->levels > RC_MAX_LEVELS) return fail(nullptr, RC_ERR_INVALID, "rc_create: levels out of range");
     if (cfg->storage > RC_STORAGE_BF16) return fail(nullptr, RC_ERR_INVALID, "rc_create: unknown storage");
-    if (cfg->flags & ~(RC_FLAG_PROFILE | RC_FLAG_KEEP_DEBUG | RC_FLAG_NO_GRAPH)) return fail(nullptr, RC_ERR_INVALID, "rc_create: unknown flags");
+    if (cfg->flags & ~(RC_FLAG_PROFILE | RC_FLAG_KEEP_DEBUG | RC_FLAG_GRAPH)) return fail(nullptr, RC_ERR_INVALID, "rc_create: unknown flags");
     int ndev = 0;
     if (rc_device_count(&ndev) != RC_OK) return RC_ERR_NO_DEVICE;
     if (cfg->device < 0 || cfg->device >= ndev) return fail(nullptr, RC_ERR_INVALID, "rc_create: device ordinal out of range");
@@ -1306,7 +1307,7 @@ rc_status rc_surrogate_train(rc_ctx *c, const rc_train_config *cfg, rc_train_rep
     const auto t_start = now();
 
     // data: a plain context at the training size generates the rows
-    rc_config tcfg{RC_ABI_VERSION, cfg->width, cfg->height, cfg->levels, RC_STORAGE_F32, RC_FLAG_NO_GRAPH,
+    rc_config tcfg{RC_ABI_VERSION, cfg->width, cfg->height, cfg->levels, RC_STORAGE_F32, 0,
                    c->cfg.device, 0};
     rc_ctx *tc = nullptr;
     rc_status st = rc_create(&tcfg, &tc);
@@ -1527,7 +1528,7 @@ rc_status rc_batch_create(const rc_view *views, uint32_t n, uint32_t storage, ui
     b->device = device;
     b->views.assign(views, views + n);
     for (uint32_t i = 0; i < n; ++i) {
-        rc_config cfg{RC_ABI_VERSION, views[i].width, views[i].height, views[i].levels, storage, RC_FLAG_NO_GRAPH,
+        rc_config cfg{RC_ABI_VERSION, views[i].width, views[i].height, views[i].levels, storage, 0,
                       device, 0};
         rc_ctx *c = nullptr;
         rc_status st = rc_create(&cfg, &c);

@@ -74,14 +74,14 @@ class RadianceCascadeStage:
     """The GI pass: jump flood -> per-level march/merge -> gather."""
 
     def __init__(self, width: int, height: int, levels: int = 0, storage: str = "f32",
-                 device: int = 0, profile: bool = False, debug: bool = False, graph: bool = True):
+                 device: int = 0, profile: bool = False, debug: bool = False, graph: bool = False):
         lib = _abi.load()
         if storage not in _abi.STORAGE:
             raise ValueError(f"storage must be one of {sorted(_abi.STORAGE)}")
         cfg = _abi.Config(version=_abi.ABI_VERSION, width=width, height=height, levels=levels,
                           storage=_abi.STORAGE[storage],
                           flags=(_abi.FLAG_PROFILE if profile else 0) | (_abi.FLAG_KEEP_DEBUG if debug else 0)
-                          | (0 if graph else _abi.FLAG_NO_GRAPH),
+                          | (_abi.FLAG_GRAPH if graph else 0),
                           device=device, reserved=0)
         ctx = ctypes.c_void_p()
         check(lib.rc_create(ctypes.byref(cfg), ctypes.byref(ctx)))
