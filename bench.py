@@ -118,6 +118,8 @@ def surrogate_leg(rc, torch, W, H, L, args, depth, emis, irr_plain, stream, fps_
     sur.load_weights(args.weights)
     sur.set_surrogate(True)
     out = torch.empty_like(irr_plain)
+    if not args.no_autotune:
+        sur.autotune(depth, emis, out, iters=3, stream=stream)   # as the plain path (its level 1 too)
     for _ in range(args.warmup):
         sur.run_device(depth, emis, out, stream)
     torch.cuda.synchronize()

@@ -32,6 +32,9 @@ struct rc_ctx {
     size_t dir_off[RC_MAX_LEVELS] = {};
     int tile[RC_MAX_LEVELS] = {};
     int rpt[RC_MAX_LEVELS] = {};
+    // level 1 of the learned frame (marched in place, the network's level 2
+    // above): its own tiling, -1 until tuned (then the plain level 1's)
+    int surr_tile1 = -1, surr_rpt1 = -1;
     int jfa_fuse = 0;  // largest jump-flood step fused into the LDS tail (0 = none)
     // split march: levels >= 1 marched by one k_march_all launch into `codes`,
     // then merged top-down; otherwise each level kernel marches its own rays
@@ -207,6 +210,7 @@ static rc_status allocate(rc_ctx *c, uint32_t W, uint32_t H, uint32_t L)
         c->tile[i] = default_tile(i, (int)L);
         c->rpt[i] = 2;
     }
+    c->surr_tile1 = c->surr_rpt1 = -1;
     size_t ncodes = 0;
     for (int i = 1; i < (int)L; ++i) {
         c->code_off[i] = ncodes;
@@ -531,7 +535,8 @@ static rc_status run_frame_surrogate(rc_ctx *c, const float *depth, const float 
     a1.upper = c->surr_a2;
     a1.out = c->surr_a1;
     a1.codes = nullptr;
-    HIP_TRY(c, launch_level(a1, c->storage, c->tile[1], c->rpt[1], s));
+    HIP_TRY(c, launch_level(a1, c->storage, c->surr_tile1 >= 0 ? c->surr_tile1 : c->tile[1],
+                            c->surr_rpt1 >= 0 ? c->surr_rpt1 : c->rpt[1], s));
     tm.end(RC_STAGE_LEVEL0 + 1);
     tm.begin(RC_STAGE_LEVEL0);
     LevelArgs a0 = level_args(c, 0, emissive);
@@ -1046,6 +1051,35 @@ rc_status rc_autotune(rc_ctx *c, const rc_gbuffer *g, float *irr, uint32_t iters
         }
         c->tile[i] = best_shape;
         c->rpt[i] = best_rpt;
+    }
+    if (c->surr_mode && c->mlp_ready) {
+        // the learned frame's level 1 is another kernel variant (marched in
+        // place, no ray codes): tune it on a learned frame's level 2
+        st = ensure_surrogate(c);
+        if (st != RC_OK) return st;
+        st = run_frame_surrogate(c, g->depth, g->emissive, irr, s, true);
+        if (st != RC_OK) return st;
+        LevelArgs a1 = level_args(c, 1, g->emissive);
+        a1.upper = c->surr_a2;
+        a1.out = c->surr_a1;
+        a1.codes = nullptr;
+        float best = 1e30f;
+        int best_shape = c->tile[1], best_rpt = c->rpt[1];
+        for (int shape = 0; shape < TILE_COUNT; ++shape) {
+            for (int rpt : {1, 2, 4}) {
+                if (!tiling_legal(1, shape, rpt)) continue;
+                HIP_TRY(c, launch_level(a1, c->storage, shape, rpt, s));  // warm
+                HIP_TRY(c, hipEventRecord(e0, s));
+                for (uint32_t it = 0; it < iters; ++it) HIP_TRY(c, launch_level(a1, c->storage, shape, rpt, s));
+                HIP_TRY(c, hipEventRecord(e1, s));
+                HIP_TRY(c, hipEventSynchronize(e1));
+                float ms = 0.f;
+                HIP_TRY(c, hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) { best = ms; best_shape = shape; best_rpt = rpt; }
+            }
+        }
+        c->surr_tile1 = best_shape;
+        c->surr_rpt1 = best_rpt;
     }
     drop_graph(c);
     // leave the level buffers consistent with one full frame

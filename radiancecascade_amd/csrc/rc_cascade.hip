@@ -237,6 +237,15 @@ __global__ __launch_bounds__(kBlock) void k_level0(LevelArgs a, float *__restric
     // the first distance sample is issued before the upper level's loads, so
     // the march does not wait for them; they land in LDS after the march
     const float d0 = in0 ? sdf_load(sdf, ix, iy, tiles) : 0.0f;
+    // the emission of the first sample's pixel (the usual hit) in the same wave
+    // of loads, so a hit there needs no dependent load
+    const int q0 = in0 ? pixel_index(ix, iy, a.W) : 0;
+    float ep0 = 0.f, ep1 = 0.f, ep2 = 0.f;
+    if (in0) {
+        ep0 = a.emissive[q0];
+        ep1 = a.emissive[plane + q0];
+        ep2 = a.emissive[2 * plane + q0];
+    }
     constexpr int UN = UPPER ? (UCX * UCY * 4 + kBlock - 1) / kBlock : 1;
     typename TT::T ureg[UN];
     if (UPPER) {
@@ -258,7 +267,6 @@ __global__ __launch_bounds__(kBlock) void k_level0(LevelArgs a, float *__restric
 #pragma unroll
             for (int k = 0; k < 4; ++k) code[k] = kRayExit;
         } else if (d0 < 0.5f) {
-            const int q0 = pixel_index(ix, iy, a.W);
 #pragma unroll
             for (int k = 0; k < 4; ++k) code[k] = q0;
         } else if (march_step(d0) < 2.0f) {
@@ -278,8 +286,12 @@ __global__ __launch_bounds__(kBlock) void k_level0(LevelArgs a, float *__restric
     float r[4], g[4], b[4], al[4];
     const bool one_hit = code[0] >= 0 && code[0] == code[1] && code[0] == code[2] && code[0] == code[3];
     if (one_hit) {
-        // one hit pixel for all 4 directions: load its emission once
-        const float e0 = a.emissive[code[0]], e1 = a.emissive[plane + code[0]], e2 = a.emissive[2 * plane + code[0]];
+        // one hit pixel for all 4 directions: its emission once (already
+        // loaded when it is the first sample's pixel)
+        const bool first = code[0] == q0;
+        const float e0 = first ? ep0 : a.emissive[code[0]];
+        const float e1 = first ? ep1 : a.emissive[plane + code[0]];
+        const float e2 = first ? ep2 : a.emissive[2 * plane + code[0]];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             r[k] = e0;
@@ -293,9 +305,10 @@ __global__ __launch_bounds__(kBlock) void k_level0(LevelArgs a, float *__restric
             r[k] = g[k] = b[k] = 0.f;
             al[k] = code[k] >= 0 ? 0.f : 1.0f;
             if (code[k] >= 0) {
-                r[k] = a.emissive[code[k]];
-                g[k] = a.emissive[plane + code[k]];
-                b[k] = a.emissive[2 * plane + code[k]];
+                const bool first = code[k] == q0;
+                r[k] = first ? ep0 : a.emissive[code[k]];
+                g[k] = first ? ep1 : a.emissive[plane + code[k]];
+                b[k] = first ? ep2 : a.emissive[2 * plane + code[k]];
             }
         }
     }
@@ -303,7 +316,9 @@ __global__ __launch_bounds__(kBlock) void k_level0(LevelArgs a, float *__restric
 #pragma unroll
         for (int n = 0; n < UN; ++n) {
             const int i = threadIdx.x + n * kBlock;
-            if (i < UCX * UCY * 4) su[i] = TT::unpack(ureg[n]);
+            // direction-major (su[k][y][x]): neighbouring probes read
+            // neighbouring texels, so the merge's LDS reads do not conflict
+            if (i < UCX * UCY * 4) su[(i & 3) * (UCX * UCY) + (i >> 2)] = TT::unpack(ureg[n]);
         }
         __syncthreads();
         if (probe && !one_hit) {
@@ -311,10 +326,8 @@ __global__ __launch_bounds__(kBlock) void k_level0(LevelArgs a, float *__restric
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 if (code[k] < 0) {
-                    const float4 u00 = su[(usy * UCX + usx) * 4 + k];
-                    const float4 u10 = su[(usy * UCX + usx + 1) * 4 + k];
-                    const float4 u01 = su[((usy + 1) * UCX + usx) * 4 + k];
-                    const float4 u11 = su[((usy + 1) * UCX + usx + 1) * 4 + k];
+                    const float4 *sk = su + k * (UCX * UCY) + usy * UCX + usx;
+                    const float4 u00 = sk[0], u10 = sk[1], u01 = sk[UCX], u11 = sk[UCX + 1];
                     const float3 v = bilerp4(u00, u10, u01, u11, px, py);
                     r[k] = v.x;
                     g[k] = v.y;

@@ -234,6 +234,25 @@ def test_surrogate_graph_replay_matches_direct_launches(rc):
     assert all(torch.equal(outs[0], o) for o in outs[1:])
 
 
+def test_autotuned_learned_frame_is_bit_identical(rc):
+    """Tuning the learned frame's level-1 tiling changes the launch shape only."""
+    W, H, L = 240, 160, 5
+    _say(f"[surrogate autotune] {W}x{H} L={L}")
+    depth, emis = _gbuffer(rc, W, H, 9, 8, 2)
+    params, mean, istd, scale = _net()
+    with rc.RadianceCascadeStage(W, H, levels=L) as st:
+        st.mlp_set(params, mean, istd, scale)
+        st.set_surrogate(True)
+        before = torch.empty((3, H, W), dtype=torch.float32, device="cuda:0")
+        st.run_device(depth, emis, before)
+        scratch = torch.empty_like(before)
+        st.autotune(depth, emis, scratch, iters=1)
+        after = torch.empty_like(before)
+        st.run_device(depth, emis, after)
+        torch.cuda.synchronize()
+    assert torch.equal(before, after)
+
+
 def test_training_reduces_loss(rc):
     with rc.RadianceCascadeStage(64, 64, levels=3) as st:
         rep = st.train_surrogate(width=128, height=128, levels=5, scene_seed0=10000, scenes=24, epochs=4,
