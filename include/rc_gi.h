@@ -190,6 +190,24 @@ rc_status rc_debug_march(rc_ctx *ctx, uint32_t level, int32_t *hit_dev, int64_t 
 rc_status rc_error_norms(rc_ctx *ctx, const float *x, const void *ref, uint32_t ref_is_f64, uint64_t n,
                          double *out_dev, void *stream);
 
+/* ---- composite (the presented frame) -------------------------------------------
+ * Replaces what the reference renderer presents after its one pass
+ * (DefaultRenderer::render, src/renderer.rs:559-613, into the Bgra8UnormSrgb
+ * surface of src/window/app.rs:59), with GI added: per pixel and channel,
+ * linear c = (albedo * irradiance + emissive) + direct in fp32 (direct: the
+ * forward pass's Blinn-Phong term of src/shader.wgsl:76-101, planar fp32
+ * [3][H][W], or NULL for none), encoded to 8-bit sRGB (round to nearest of
+ * the IEC 61966-2-1 code via the thresholds of rc_srgb_thresholds; c <= 0 ->
+ * 0, c >= 1 -> 255) and written as one uint32 per pixel holding the bytes
+ * B, G, R, A = 255.  albedo, emissive, irradiance: planar fp32 [3][H][W] at
+ * the context's size; all device pointers. */
+rc_status rc_composite(rc_ctx *ctx, const float *albedo, const float *emissive, const float *irradiance,
+                       const float *direct, uint32_t *bgra8, void *stream);
+/* The 256 encoding thresholds (host array): t[k], k = 1..255, is the linear
+ * value whose sRGB code is (k - 1/2) / 255 (double, rounded to fp32); t[0] is
+ * -inf.  The 8-bit value of c is the largest k with c >= t[k]. */
+void rc_srgb_thresholds(float *t);
+
 /* ---- learned far field ------------------------------------------------------
  * No counterpart in the reference (it has no GI).  Mode 1 replaces levels >= 2
  * of the frame: a cone trace through an emitter / occluder mip pyramid gives,

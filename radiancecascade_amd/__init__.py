@@ -8,7 +8,7 @@ is no CPU fallback and importing the stage fails if the library is absent.
 """
 from ._abi import RcError, load  # noqa: F401
 from .stage import (RadianceCascadeBatch, RadianceCascadeStage, batch_views_spec, device_count, gbuffer_raster, level_rays,  # noqa: F401
-                    levels_for_coverage, packaged_weights, probe_grid, scene_generate)
+                    levels_for_coverage, packaged_weights, probe_grid, scene_generate, srgb_thresholds)
 
 __all__ = ["RadianceCascadeBatch", "RadianceCascadeStage", "batch_views_spec", "RcError", "device_count", "gbuffer_raster", "level_rays",
-           "levels_for_coverage", "load", "packaged_weights", "probe_grid", "scene_generate"]
+           "levels_for_coverage", "load", "packaged_weights", "probe_grid", "scene_generate", "srgb_thresholds"]

@@ -142,6 +142,8 @@ SIGNATURES = {
     "rc_batch_view_levels": (_i32, [_P, _u32, ctypes.POINTER(_u32)]),
     "rc_batch_last_error": (ctypes.c_char_p, [_P]),
     "rc_batch_destroy": (None, [_P]),
+    "rc_composite": (_i32, [_P, _P, _P, _P, _P, _P, _P]),
+    "rc_srgb_thresholds": (None, [_P]),
     "rc_scene_generate": (_i32, [_u64, _u32, _u32, _P]),
     "rc_gbuffer_raster": (_i32, [_P, _u32, _u32, _u32, _P, _P, _P, _P]),
 }

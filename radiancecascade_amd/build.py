@@ -21,7 +21,8 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "librcgi.so")
 ARCH = os.environ.get("RC_OFFLOAD_ARCH", "gfx950")
-SOURCES = ["rc_scene.hip", "rc_jfa.hip", "rc_cascade.hip", "rc_norms.hip", "rc_mlp.hip", "rc_surrogate.hip", "rc_api.hip"]
+SOURCES = ["rc_scene.hip", "rc_jfa.hip", "rc_cascade.hip", "rc_norms.hip", "rc_mlp.hip", "rc_surrogate.hip",
+           "rc_composite.hip", "rc_api.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 CFLAGS = [

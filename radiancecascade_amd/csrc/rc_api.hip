@@ -44,6 +44,7 @@ struct rc_ctx {
     int *codes = nullptr;
     size_t code_off[RC_MAX_LEVELS] = {};
     double *norm_ws = nullptr;  // error-norm partials
+    float *srgb_table = nullptr;  // composite's 256 sRGB thresholds
     int dbg_seed_buf = -1;
     bool ran = false;
     // host-staging path (rc_update_host / rc_render_host)
@@ -149,6 +150,8 @@ static void release(rc_ctx *c)
     c->codes = nullptr;
     if (c->norm_ws) hipFree(c->norm_ws);
     c->norm_ws = nullptr;
+    if (c->srgb_table) hipFree(c->srgb_table);
+    c->srgb_table = nullptr;
     if (c->mip.tex) hipFree(c->mip.tex);
     c->mip.tex = nullptr;
     if (c->occ_bits) hipFree(c->occ_bits);
@@ -1093,6 +1096,28 @@ rc_status rc_error_norms(rc_ctx *c, const float *x, const void *ref, uint32_t re
     if (n == 0) return fail(c, RC_ERR_INVALID, "rc_error_norms: empty input");
     HIP_TRY(c, hipSetDevice(c->cfg.device));
     HIP_TRY(c, launch_error_norms(x, ref, ref_is_f64 != 0, (size_t)n, c->norm_ws, out_dev, (hipStream_t)stream));
+    return RC_OK;
+}
+
+void rc_srgb_thresholds(float *t)
+{
+    if (t) srgb_thresholds(t);
+}
+
+rc_status rc_composite(rc_ctx *c, const float *albedo, const float *emissive, const float *irradiance,
+                       const float *direct, uint32_t *bgra8, void *stream)
+{
+    if (!c) return fail(nullptr, RC_ERR_NULL, "rc_composite: ctx is NULL");
+    if (!albedo || !emissive || !irradiance || !bgra8) return fail(c, RC_ERR_NULL, "rc_composite: NULL buffer");
+    HIP_TRY(c, hipSetDevice(c->cfg.device));
+    if (!c->srgb_table) {
+        float t[256];
+        srgb_thresholds(t);
+        HIP_TRY(c, hipMalloc(&c->srgb_table, sizeof(t)));
+        HIP_TRY(c, hipMemcpy(c->srgb_table, t, sizeof(t), hipMemcpyHostToDevice));
+    }
+    HIP_TRY(c, launch_composite(albedo, emissive, irradiance, direct, c->srgb_table, bgra8, (size_t)c->W * c->H,
+                                (hipStream_t)stream));
     return RC_OK;
 }
 

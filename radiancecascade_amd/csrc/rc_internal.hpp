@@ -68,6 +68,9 @@ hipError_t launch_march_probe(const LevelArgs &a, int spec, int32_t *hit, int64_
 
 // ---- error norms (rc_norms.hip) ---------------------------------------------
 size_t norm_workspace_bytes();
+void srgb_thresholds(float *t);   // 256 entries (rc_composite.hip)
+hipError_t launch_composite(const float *albedo, const float *emissive, const float *irr, const float *direct,
+                            const float *table, uint32_t *out, size_t n, hipStream_t s);
 hipError_t launch_error_norms(const float *x, const void *ref, bool ref_f64, size_t n, double *workspace,
                               double *out, hipStream_t s);
 

@@ -201,6 +201,17 @@ class RadianceCascadeStage:
         check(self._lib.rc_debug_march(self._ctx, level, _dev_ptr(hit, rays * 4, "hit"),
                                        _dev_ptr(index, rays * 8, "index"), _stream_handle(stream)), self._ctx)
 
+    def composite(self, albedo, emissive, irradiance, out, direct=None, stream=None) -> None:
+        """The presented frame: out [H][W] uint32 (int32 tensor) of bytes B, G, R,
+        A=255 from linear (albedo * irradiance + emissive) + direct, 8-bit sRGB
+        (rc_composite).  All planar [3][H][W] fp32 device tensors; direct optional."""
+        n = self.width * self.height
+        check(self._lib.rc_composite(self._ctx, _dev_ptr(albedo, 3 * n * 4, "albedo"),
+                                     _dev_ptr(emissive, 3 * n * 4, "emissive"),
+                                     _dev_ptr(irradiance, 3 * n * 4, "irradiance"),
+                                     None if direct is None else _dev_ptr(direct, 3 * n * 4, "direct"),
+                                     _dev_ptr(out, n * 4, "out"), _stream_handle(stream)), self._ctx)
+
     def error_norms(self, x, ref, stream=None) -> dict:
         """rel-L2 and max-abs of device fp32 `x` against device `ref` (fp32 or
         fp64 tensor of the same length), reduced in FP64 on the GPU."""
@@ -457,6 +468,13 @@ def packaged_weights() -> str:
 
 
 # ---- geometry and scenes ------------------------------------------------------
+def srgb_thresholds() -> np.ndarray:
+    """The composite's 256 sRGB encoding thresholds (rc_srgb_thresholds)."""
+    t = np.empty(256, np.float32)
+    _abi.load().rc_srgb_thresholds(t.ctypes.data)
+    return t
+
+
 def probe_grid(width: int, height: int, level: int) -> tuple[int, int]:
     lib = _abi.load()
     px, py = ctypes.c_uint32(), ctypes.c_uint32()
