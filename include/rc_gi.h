@@ -208,6 +208,27 @@ rc_status rc_composite(rc_ctx *ctx, const float *albedo, const float *emissive, 
  * -inf.  The 8-bit value of c is the largest k with c >= t[k]. */
 void rc_srgb_thresholds(float *t);
 
+/* ---- dump files (host only) ----------------------------------------------------
+ * No counterpart in the reference (it keeps no GI data on disk).  An
+ * "RCDUMP01" file holds named planar fp32 arrays [channels][height][width]
+ * (G-buffer planes, merged cascades, irradiance) with a SHA-256 of the
+ * payload: fixtures that travel between machines and captured frames for the
+ * trainer.  Readers check magic, version, sizes and the checksum (any
+ * mismatch: RC_ERR_INVALID, message in rc_dump_last_error). */
+typedef struct {
+    char name[16];                   /* NUL-padded; unique within a file */
+    uint32_t channels, height, width, reserved;
+    const float *data;               /* host values (write); NULL from rc_dump_info */
+} rc_plane;
+rc_status rc_dump_write(const char *path, const rc_plane *planes, uint32_t n_planes);
+/* names and shapes of up to `cap` planes; *n_out = the file's plane count */
+rc_status rc_dump_info(const char *path, rc_plane *planes, uint32_t cap, uint32_t *n_out);
+/* copies plane `name` (count = channels*height*width floats) into dst */
+rc_status rc_dump_read(const char *path, const char *name, float *dst, uint64_t count);
+const char *rc_dump_last_error(void);
+/* SHA-256 (FIPS 180-4) of n bytes into out32[32] */
+void rc_sha256(const void *data, uint64_t n, uint8_t *out32);
+
 /* ---- learned far field ------------------------------------------------------
  * No counterpart in the reference (it has no GI).  Mode 1 replaces levels >= 2
  * of the frame: a cone trace through an emitter / occluder mip pyramid gives,

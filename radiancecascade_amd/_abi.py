@@ -46,6 +46,11 @@ class Config(ctypes.Structure):
                 ("device", ctypes.c_int32), ("reserved", ctypes.c_uint32)]
 
 
+class Plane(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 16), ("channels", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("width", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("data", ctypes.c_void_p)]
+
+
 class GBuffer(ctypes.Structure):
     _fields_ = [("depth", ctypes.c_void_p), ("albedo", ctypes.c_void_p), ("emissive", ctypes.c_void_p)]
 
@@ -143,6 +148,11 @@ SIGNATURES = {
     "rc_batch_last_error": (ctypes.c_char_p, [_P]),
     "rc_batch_destroy": (None, [_P]),
     "rc_composite": (_i32, [_P, _P, _P, _P, _P, _P, _P]),
+    "rc_dump_write": (_i32, [ctypes.c_char_p, ctypes.POINTER(Plane), _u32]),
+    "rc_dump_info": (_i32, [ctypes.c_char_p, ctypes.POINTER(Plane), _u32, ctypes.POINTER(_u32)]),
+    "rc_dump_read": (_i32, [ctypes.c_char_p, ctypes.c_char_p, _P, _u64]),
+    "rc_dump_last_error": (ctypes.c_char_p, []),
+    "rc_sha256": (None, [_P, _u64, _P]),
     "rc_srgb_thresholds": (None, [_P]),
     "rc_scene_generate": (_i32, [_u64, _u32, _u32, _P]),
     "rc_gbuffer_raster": (_i32, [_P, _u32, _u32, _u32, _P, _P, _P, _P]),

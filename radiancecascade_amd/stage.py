@@ -468,6 +468,63 @@ def packaged_weights() -> str:
 
 
 # ---- geometry and scenes ------------------------------------------------------
+# ---- dump files (rc_dump_*) ----------------------------------------------------
+def _dump_check(status: int) -> None:
+    if status != 0:
+        raise RcError(status, (_abi.load().rc_dump_last_error() or b"").decode())
+
+
+def dump_write(path: str, planes: dict) -> None:
+    """Write named arrays ([C][H][W] or [H][W], converted to fp32) to an
+    RCDUMP01 file with a SHA-256 of the payload (rc_dump_write)."""
+    lib = _abi.load()
+    keep, recs = [], (_abi.Plane * max(1, len(planes)))()
+    for i, (name, a) in enumerate(planes.items()):
+        nb = name.encode()
+        if len(nb) > 16:
+            raise ValueError(f"plane name {name!r} longer than 16 bytes")
+        a = np.ascontiguousarray(a, np.float32)
+        if a.ndim == 2:
+            a = a[None]
+        if a.ndim != 3:
+            raise ValueError(f"plane {name!r}: expected [C][H][W] or [H][W]")
+        keep.append(a)
+        recs[i].name, (recs[i].channels, recs[i].height, recs[i].width) = nb, a.shape
+        recs[i].data = a.ctypes.data
+    _dump_check(lib.rc_dump_write(os.fsencode(path), recs, len(planes)))
+
+
+def dump_info(path: str) -> dict:
+    """{name: (channels, height, width)} of an RCDUMP01 file (checksum verified)."""
+    lib = _abi.load()
+    n = ctypes.c_uint32()
+    _dump_check(lib.rc_dump_info(os.fsencode(path), None, 0, ctypes.byref(n)))
+    recs = (_abi.Plane * max(1, n.value))()
+    _dump_check(lib.rc_dump_info(os.fsencode(path), recs, n.value, ctypes.byref(n)))
+    return {recs[i].name.decode(): (recs[i].channels, recs[i].height, recs[i].width) for i in range(n.value)}
+
+
+def dump_read(path: str, names=None) -> dict:
+    """{name: float32 [C][H][W]} from an RCDUMP01 file (rc_dump_read)."""
+    lib = _abi.load()
+    out = {}
+    for name, shape in dump_info(path).items():
+        if names is not None and name not in names:
+            continue
+        a = np.empty(shape, np.float32)
+        _dump_check(lib.rc_dump_read(os.fsencode(path), name.encode(), a.ctypes.data, a.size))
+        out[name] = a
+    return out
+
+
+def sha256(data: bytes) -> str:
+    """Hex SHA-256 computed by the library (rc_sha256)."""
+    buf = ctypes.create_string_buffer(bytes(data), len(data))
+    out = ctypes.create_string_buffer(32)
+    _abi.load().rc_sha256(buf, len(data), out)
+    return out.raw.hex()
+
+
 def srgb_thresholds() -> np.ndarray:
     """The composite's 256 sRGB encoding thresholds (rc_srgb_thresholds)."""
     t = np.empty(256, np.float32)
