@@ -211,10 +211,11 @@ void rc_srgb_thresholds(float *t);
 /* ---- dump files (host only) ----------------------------------------------------
  * No counterpart in the reference (it keeps no GI data on disk).  An
  * "RCDUMP01" file holds named planar fp32 arrays [channels][height][width]
- * (G-buffer planes, merged cascades, irradiance) with a SHA-256 of the
- * payload: fixtures that travel between machines and captured frames for the
- * trainer.  Readers check magic, version, sizes and the checksum (any
- * mismatch: RC_ERR_INVALID, message in rc_dump_last_error). */
+ * (G-buffer planes, merged cascades, irradiance) with a SHA-256 of its plane
+ * records and payload: fixtures that travel between machines and captured
+ * frames for the trainer.  Readers check magic, version, that the sizes match
+ * the file, and the checksum (any mismatch: RC_ERR_INVALID, message in
+ * rc_dump_last_error). */
 typedef struct {
     char name[16];                   /* NUL-padded; unique within a file */
     uint32_t channels, height, width, reserved;

@@ -10,14 +10,16 @@
 //   8       4             version (1)
 //   12      4             n_planes
 //   16      8             payload bytes
-//   24      32            SHA-256 of the payload (FIPS 180-4)
+//   24      32            SHA-256 (FIPS 180-4) of everything after this
+//                         field: the plane records, then the payload
 //   56      48 * n        plane records: name[16] (NUL-padded), channels,
 //                         height, width, reserved (u32), offset (u64, from
 //                         the payload start), count (u64, floats)
 //   ..      payload       the planes' fp32 values, little-endian, in order
 //
-// All integers little-endian.  Readers check magic, version, sizes and the
-// checksum before handing out any value.
+// All integers little-endian.  Readers check magic, version, that the sizes
+// account for the file exactly (before allocating), and the checksum, before
+// handing out any value.
 #include <stdint.h>
 #include <stdio.h>
 #include <string.h>
@@ -165,14 +167,21 @@ rc_status load(const char *path, std::vector<rc_plane> &planes, std::vector<std:
     const uint32_t n = get32(hd + 12);
     const uint64_t bytes = get64(hd + 16);
     if (n > 4096 || bytes > (1ull << 40)) return dfail(RC_ERR_INVALID, "rc_dump: implausible sizes");
+    // the header's sizes must account for the file exactly before anything
+    // is allocated from them
+    if (fseek(fh.f, 0, SEEK_END) != 0) return dfail(RC_ERR_INVALID, "rc_dump: cannot seek");
+    const long long fsize = ftell(fh.f);
+    const unsigned long long want = kHeader + kRecord * (unsigned long long)n + bytes;
+    if (fsize < 0 || (unsigned long long)fsize < want) return dfail(RC_ERR_INVALID, "rc_dump: truncated file");
+    if ((unsigned long long)fsize > want) return dfail(RC_ERR_INVALID, "rc_dump: trailing bytes");
+    if (fseek(fh.f, (long)kHeader, SEEK_SET) != 0) return dfail(RC_ERR_INVALID, "rc_dump: cannot seek");
     std::vector<uint8_t> rec(kRecord * (size_t)n);
     if (n && fread(rec.data(), 1, rec.size(), fh.f) != rec.size()) return dfail(RC_ERR_INVALID, "rc_dump: truncated records");
     payload.resize((size_t)bytes);
     if (bytes && fread(payload.data(), 1, (size_t)bytes, fh.f) != (size_t)bytes)
         return dfail(RC_ERR_INVALID, "rc_dump: truncated payload");
-    uint8_t extra;
-    if (fread(&extra, 1, 1, fh.f) != 0) return dfail(RC_ERR_INVALID, "rc_dump: trailing bytes");
     Sha256 sh;
+    sh.update(rec.data(), rec.size());
     sh.update(payload.data(), payload.size());
     uint8_t dg[32];
     sh.finish(dg);
@@ -217,7 +226,6 @@ rc_status rc_dump_write(const char *path, const rc_plane *planes, uint32_t n)
     if (n > 4096) return dfail(RC_ERR_INVALID, "rc_dump_write: too many planes");
     std::vector<uint8_t> rec(kRecord * (size_t)n, 0);
     uint64_t off = 0;
-    Sha256 sh;
     for (uint32_t i = 0; i < n; ++i) {
         const rc_plane &p = planes[i];
         const uint64_t cnt = (uint64_t)p.channels * p.height * p.width;
@@ -231,9 +239,12 @@ rc_status rc_dump_write(const char *path, const rc_plane *planes, uint32_t n)
         put32(r + 24, p.width);
         put64(r + 32, off);
         put64(r + 40, cnt);
-        sh.update(p.data, (size_t)(4 * cnt));
         off += 4 * cnt;
     }
+    Sha256 sh;
+    sh.update(rec.data(), rec.size());
+    for (uint32_t i = 0; i < n; ++i)
+        sh.update(planes[i].data, (size_t)4 * planes[i].channels * planes[i].height * planes[i].width);
     uint8_t hd[kHeader];
     memcpy(hd, kMagic, 8);
     put32(hd + 8, kVersion);

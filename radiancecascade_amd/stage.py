@@ -476,7 +476,7 @@ def _dump_check(status: int) -> None:
 
 def dump_write(path: str, planes: dict) -> None:
     """Write named arrays ([C][H][W] or [H][W], converted to fp32) to an
-    RCDUMP01 file with a SHA-256 of the payload (rc_dump_write)."""
+    RCDUMP01 file with a SHA-256 of its plane records and payload (rc_dump_write)."""
     lib = _abi.load()
     keep, recs = [], (_abi.Plane * max(1, len(planes)))()
     for i, (name, a) in enumerate(planes.items()):

@@ -44,10 +44,11 @@ def test_round_trip(rc, tmp_path):
     back = rc.dump_read(p)
     for k, v in planes.items():
         assert np.array_equal(back[k].reshape(v.shape), v)
-    # the header's checksum is the SHA-256 of the payload (the planes' bytes in order)
+    # the header's checksum is the SHA-256 of the plane records and the
+    # payload (the planes' bytes in order)
     raw = open(p, "rb").read()
     payload = b"".join(np.ascontiguousarray(v, np.float32).tobytes() for v in planes.values())
-    assert raw[:8] == b"RCDUMP01" and raw[24:56] == hashlib.sha256(payload).digest()
+    assert raw[:8] == b"RCDUMP01" and raw[24:56] == hashlib.sha256(raw[56:]).digest()
     assert raw[-len(payload):] == payload and len(raw) == 56 + 48 * 4 + len(payload)
 
 
@@ -66,6 +67,7 @@ def test_rejects_corruption(rc, tmp_path):
     flip[-5] ^= 0x01
     bad(flip, "checksum")
     bad(raw[:-3], "truncated")
+    bad(raw[:40], "truncated header")
     bad(raw + b"x", "trailing")
     m = bytearray(raw)
     m[0:8] = b"RCDUMP02"
@@ -75,7 +77,10 @@ def test_rejects_corruption(rc, tmp_path):
     bad(v, "version")
     r = bytearray(raw)
     r[56 + 16] ^= 0x01          # channels of the first plane
-    bad(r, "bad plane record")
+    bad(r, "checksum")
+    nm = bytearray(raw)
+    nm[56] ^= 0x20              # the first plane's name
+    bad(nm, "checksum")
     with pytest.raises(rc.RcError, match="cannot open"):
         rc.dump_read(str(tmp_path / "missing.rcdump"))
 
