@@ -102,8 +102,8 @@ typedef struct {
 /* learned far field (rc_set_surrogate mode 1) */
 #define RC_STAGE_NEAR_SDF (RC_STAGE_MARCH + 1)  /* exact distance field up to 10 px */
 #define RC_STAGE_MIP (RC_STAGE_MARCH + 2)       /* emitter / occluder mip pyramid */
-#define RC_STAGE_CONE (RC_STAGE_MARCH + 3)      /* cone trace on the level-2 probe grid */
-#define RC_STAGE_SURROGATE (RC_STAGE_MARCH + 4) /* level-1 march + MLP + merge */
+#define RC_STAGE_CONE (RC_STAGE_MARCH + 3)      /* cone trace (level-2 and level-3 probe grids) */
+#define RC_STAGE_SURROGATE (RC_STAGE_MARCH + 4) /* the network's level 2 (features + MLP) */
 #define RC_STAGE_FALLBACK (RC_STAGE_MARCH + 5)  /* plain far field + patch of flagged tiles */
 #define RC_NUM_STAGES (RC_STAGE_MARCH + 6)
 typedef struct {

@@ -6,6 +6,7 @@
 // kernel, all stream ordered on the caller's stream.  The context owns the
 // jump-flood ping-pong buffers, the SDF, the merged levels 1..L-1 and the
 // per-level direction tables; it allocates them at create/resize time only.
+#include <rocprofiler-sdk-roctx/roctx.h>
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
@@ -280,17 +281,47 @@ static MarchAllArgs march_all_args(const rc_ctx *c)
     return m;
 }
 
+// Stage names of the profiled frame's roctx ranges ("rc:jfa", "rc:level3", ...).
+static const char *stage_name(int st)
+{
+    static const std::vector<std::string> names = [] {
+        std::vector<std::string> n(RC_NUM_STAGES, "rc:stage");
+        n[RC_STAGE_SEED] = "rc:seed";
+        n[RC_STAGE_JFA] = "rc:jfa";
+        n[RC_STAGE_JFA_TAIL] = "rc:jfa_tail";
+        for (int i = 0; i < RC_MAX_LEVELS; ++i) n[RC_STAGE_LEVEL0 + i] = "rc:level" + std::to_string(i);
+        n[RC_STAGE_MARCH] = "rc:march";
+        n[RC_STAGE_NEAR_SDF] = "rc:near_sdf";
+        n[RC_STAGE_MIP] = "rc:mip";
+        n[RC_STAGE_CONE] = "rc:cone";
+        n[RC_STAGE_SURROGATE] = "rc:network";
+        n[RC_STAGE_FALLBACK] = "rc:fallback";
+        return n;
+    }();
+    return names[st].c_str();
+}
+
+// Profiled frames: HIP events around every stage on the frame's stream (the
+// stage times of rc_get_stats) and a roctx range with the stage's name (host
+// side, for rocprofv3 --marker-trace).
 struct StageTimer {
     rc_ctx *c;
     hipStream_t s;
     bool on;
     void begin(int stage)
     {
-        if (on) { hipEventRecord(c->ev[2 * stage], s); c->ev_used[stage] = true; }
+        if (on) {
+            roctxRangePushA(stage_name(stage));
+            hipEventRecord(c->ev[2 * stage], s);
+            c->ev_used[stage] = true;
+        }
     }
     void end(int stage)
     {
-        if (on) hipEventRecord(c->ev[2 * stage + 1], s);
+        if (on) {
+            hipEventRecord(c->ev[2 * stage + 1], s);
+            roctxRangePop();
+        }
     }
 };
 
