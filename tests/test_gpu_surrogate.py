@@ -66,7 +66,7 @@ def _gbuffer(rc, W, H, seed, nb, ne):
     return depth, emis
 
 
-CASES = [(256, 256, 5, 9, 8, 2), (200, 136, 4, 9, 8, 2)]
+CASES = [(256, 256, 5, 9, 8, 2), (200, 136, 4, 9, 8, 2), (1920, 1080, 6, 0, 32, 4)]
 
 
 @pytest.mark.parametrize("W,H,L,seed,nb,ne", CASES)
