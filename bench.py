@@ -106,7 +106,7 @@ def cpu_baseline(depth, emissive, L, seconds):
 
 
 # MLP FLOP per level-1 probe (48 -> 64 -> 64 -> 64 -> 64, multiply-adds x 2)
-MLP_FLOP_PER_PROBE = 2 * (48 * 64 + 3 * 64 * 64)
+MLP_FLOP_PER_PROBE = 2 * (64 * 64 + 3 * 64 * 64)
 BF16_MFMA_PEAK_TFLOPS = 2500.0  # MI355X dense bf16
 
 

@@ -234,17 +234,18 @@ void rc_sha256(const void *data, uint64_t n, uint8_t *out32);
  * No counterpart in the reference (it has no GI).  Mode 1 replaces levels >= 2
  * of the frame: a cone trace through an emitter / occluder mip pyramid gives,
  * per level-2 probe and level-1 direction k, a far-field estimate c_k; an MLP
- * (48 -> 64 -> 64 -> 64 -> 64, ReLU, BF16 weights on MFMA, FP32 accumulation)
- * maps x = standardised log(1 + c) to z, and the merged level 2 is
+ * (64 -> 64 -> 64 -> 64 -> 64, ReLU, BF16 weights on MFMA, FP32 accumulation)
+ * maps x = standardised (log(1 + c_k) rgb, cone transmittance T_k) for the 16
+ * directions to z, and the merged level 2 is
  * A_2 = max(c_k + out_scale * z_k, 0); levels 1 and 0 are then the plain
  * kernels (marched on a distance field exact up to 10 px).  Output 48
  * predicts log of the squared error.  Tiles of 8x8 level-2 probes whose mean
  * predicted error exceeds `threshold` take the plain cascades' level 1
  * (fallback: one full plain frame plus a patch, decided on the host after the
  * frame).  Needs levels >= 3. */
-#define RC_MLP_INPUTS 48
+#define RC_MLP_INPUTS 64     /* per direction k: log(1 + c_k) rgb, cone transmittance T_k */
 #define RC_MLP_OUTPUTS 64
-#define RC_MLP_PARAMS 15616  /* for l = 0..3: W_l [K_l][64] (K = 48, 64, 64, 64) then b_l [64] */
+#define RC_MLP_PARAMS 16640  /* for l = 0..3: W_l [K_l][64] (K = 64) then b_l [64] */
 #define RC_SURROGATE_BLOB_MAGIC "RCSURR01"
 typedef struct {
     uint32_t version;   /* RC_ABI_VERSION */
@@ -255,7 +256,7 @@ typedef struct {
 rc_status rc_set_surrogate(rc_ctx *ctx, const rc_surrogate_config *cfg);
 rc_status rc_get_surrogate(const rc_ctx *ctx, rc_surrogate_config *out);
 /* Network parameters (host arrays): params RC_MLP_PARAMS fp32; in_mean,
- * in_istd RC_MLP_INPUTS each (x = (log1p(c) - mean) * istd). */
+ * in_istd RC_MLP_INPUTS each (x = (feature - mean) * istd). */
 rc_status rc_mlp_set(rc_ctx *ctx, const float *params, const float *in_mean, const float *in_istd, float out_scale);
 rc_status rc_mlp_get(const rc_ctx *ctx, float *params, float *in_mean, float *in_istd, float *out_scale);
 /* Serialised weights: "RCSURR01", uint32 version, inputs, outputs, params,
@@ -264,7 +265,7 @@ rc_status rc_mlp_get(const rc_ctx *ctx, float *params, float *in_mean, float *in
 rc_status rc_surrogate_save(const rc_ctx *ctx, void *buf, uint64_t capacity, uint64_t *size);
 rc_status rc_surrogate_load(rc_ctx *ctx, const void *buf, uint64_t size);
 /* The network's kernels on device buffers (parity hooks for the training and
- * inference kernels).  feats: [n][48] log1p features (standardised inside);
+ * inference kernels).  feats: [n][RC_MLP_INPUTS] features (standardised inside);
  * out: [n][64] raw outputs z.  rc_mlp_gradient: targets [n][48] normalised
  * residuals; grad: RC_MLP_PARAMS fp32 of the batch-mean loss
  * L = sum (z - t)^2 / (48 n) + 0.05 sum (z_48 - log(mse_s + 1e-2))^2 / n;
@@ -274,8 +275,9 @@ rc_status rc_mlp_gradient(rc_ctx *ctx, const float *feats_dev, const float *targ
                           float *loss_dev, void *stream);
 rc_status rc_mlp_adam(rc_ctx *ctx, const float *grad_dev, float lr, void *stream);
 /* Training rows of one G-buffer at the context's size: runs the plain frame,
- * then per level-2 probe writes feats [P][48] = log(1 + c) and targets
- * [P][48] = A_2(plain) - c (unnormalised), P = Px_2 * Py_2. */
+ * then per level-2 probe writes feats [P][RC_MLP_INPUTS] = per direction k
+ * (log(1 + c_k) rgb, T_k) and targets [P][48] = A_2(plain) - c
+ * (unnormalised), P = Px_2 * Py_2. */
 rc_status rc_surrogate_samples(rc_ctx *ctx, const rc_gbuffer *dev, float *feats_dev, float *targets_dev,
                                void *stream);
 /* Train the context's network on the GPU on plain-path outputs of seeded
@@ -303,6 +305,16 @@ typedef struct {
     uint32_t reserved;
 } rc_train_report;
 rc_status rc_surrogate_train(rc_ctx *ctx, const rc_train_config *cfg, rc_train_report *report);
+/* Train on rows the caller made, e.g. with rc_surrogate_samples on G-buffers
+ * read from dump files (rc_dump_read): feats [n_rows][RC_MLP_INPUTS] and
+ * targets [n_rows][48] on the device, as rc_surrogate_samples writes them.
+ * targets is scaled in place by 1 / out_scale.  Of cfg, only the optimiser
+ * fields (epochs, batch, lr, lr_decay, seed) are used, and scenes is recorded
+ * in the weights blob as the number of frames the rows came from.  Rows that
+ * rc_surrogate_train would generate give the same network bit for bit.  No
+ * counterpart in the reference (its renderer has no GI). */
+rc_status rc_surrogate_train_rows(rc_ctx *ctx, const float *feats_dev, float *targets_dev, uint64_t n_rows,
+                                  const rc_train_config *cfg, rc_train_report *report);
 /* Readbacks of the last learned-far-field frame (device destinations, any may
  * be NULL): cone [Py_2][Px_2][16][4] (rgb, transmittance; needs a context
  * created with RC_FLAG_KEEP_DEBUG); level1 [Py_1][Px_1][4][4] merged level 1

@@ -4,9 +4,10 @@ TEST INFRASTRUCTURE ONLY: the checker for the GPU training and inference
 kernels (tests/, bench.py's error judge); the product never imports it.
 
 The network and its mixed-precision data path are the project's own (the
-reference has no learned component): 48 -> 64 -> 64 -> 64 -> 64 with ReLU on
-the hidden layers; outputs 0..47 are the level-1 far-field radiance (16
-directions x rgb, normalised), output 48 the log of the predicted squared
+reference has no learned component): 64 -> 64 -> 64 -> 64 -> 64 with ReLU on
+the hidden layers; inputs per direction (log(1 + c) rgb, cone transmittance)
+x 16; outputs 0..47 are the level-2 far-field residual (16 directions x rgb,
+normalised), output 48 the log of the predicted squared
 error, outputs 49..63 padding.  Layer l computes z = bf16(a) . bf16(W) + b
 with FP32 accumulation on the GPU; this module evaluates the same rounding
 points in FP64, so the GPU differs from it only by accumulation order.
@@ -17,7 +18,7 @@ from __future__ import annotations
 
 import numpy as np
 
-IN, HID, OUT_PAD = 48, 64, 64
+IN, HID, OUT_PAD = 64, 64, 64
 N_RAD = 48          # radiance outputs
 ERR = 48            # index of the error head
 SIZES = [(IN, HID), (HID, HID), (HID, HID), (HID, OUT_PAD)]
@@ -48,7 +49,7 @@ def init_params(seed: int = 0):
 
 
 def forward(params, X, emulate_bf16=True):
-    """X [B, 48] normalised features -> (z4 [B, 64], cache)."""
+    """X [B, 64] normalised features -> (z4 [B, 64], cache)."""
     r = bf16 if emulate_bf16 else (lambda v: np.asarray(v, np.float64))
     h = np.asarray(X, np.float64)
     cache = []

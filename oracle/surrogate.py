@@ -242,14 +242,16 @@ def upper_bilinear(U: np.ndarray, PX: int, PY: int) -> np.ndarray:
 
 
 def features(cone: np.ndarray):
-    """(c [Q][48] cone radiance at the level-2 probes, x [Q][48] = log(1 + c))."""
+    """(c [Q][48] cone radiance at the level-2 probes, x [Q][64] = per
+    direction (log(1 + c) rgb, cone transmittance T))."""
     QY, QX = cone.shape[:2]
     c = cone[..., :3].reshape(QY * QX, 48)
-    return c, np.log1p(c)
+    x = np.concatenate([np.log1p(cone[..., :3]), cone[..., 3:4]], -1).reshape(QY * QX, 64)
+    return c, x
 
 
 def samples(cone: np.ndarray, A2: np.ndarray):
-    """Training rows per level-2 probe: (log(1 + c), A2.rgb - c) with A2 the
+    """Training rows per level-2 probe: (features, A2.rgb - c) with A2 the
     plain path's merged level 2 (16 entries per probe)."""
     c, x = features(cone)
     return x, A2[..., :3].reshape(c.shape[0], 48) - c

@@ -35,7 +35,8 @@ STAGE_MARCH = STAGE_LEVEL0 + MAX_LEVELS
 STAGE_NEAR_SDF, STAGE_MIP, STAGE_CONE, STAGE_SURROGATE, STAGE_FALLBACK = range(STAGE_MARCH + 1, STAGE_MARCH + 6)
 NUM_STAGES = STAGE_MARCH + 6
 
-MLP_INPUTS, MLP_OUTPUTS, MLP_PARAMS = 48, 64, 15616
+MLP_INPUTS, MLP_OUTPUTS, MLP_PARAMS = 64, 64, 16640
+MLP_RADIANCE = 48          # targets / radiance outputs per probe (16 directions x rgb)
 SURROGATE_BLOB_MAGIC = b"RCSURR01"
 TRAIN_MAX_EPOCHS = 64
 
@@ -139,6 +140,8 @@ SIGNATURES = {
     "rc_mlp_adam": (_i32, [_P, _P, ctypes.c_float, _P]),
     "rc_surrogate_samples": (_i32, [_P, ctypes.POINTER(GBuffer), _P, _P, _P]),
     "rc_surrogate_train": (_i32, [_P, ctypes.POINTER(TrainConfig), ctypes.POINTER(TrainReport)]),
+    "rc_surrogate_train_rows": (_i32, [_P, _P, _P, ctypes.c_uint64, ctypes.POINTER(TrainConfig),
+                                       ctypes.POINTER(TrainReport)]),
     "rc_debug_surrogate": (_i32, [_P, _P, _P, _P, _P]),
     "rc_debug_mip": (_i32, [_P, _u32, _P, _P, _P]),
     "rc_batch_create": (_i32, [ctypes.POINTER(View), _u32, _u32, _u32, _i32, ctypes.POINTER(_P)]),

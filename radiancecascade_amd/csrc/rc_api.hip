@@ -1247,8 +1247,8 @@ rc_status rc_mlp_get(const rc_ctx *c, float *params, float *mean, float *istd, f
 }
 
 // blob: magic[8], u32 version, inputs, outputs, params, f32 out_scale, u32
-// train_scenes, f32 in_mean[48], in_istd[48], params[15616]
-static const uint32_t kBlobVersion = 1;
+// train_scenes, f32 in_mean[64], in_istd[64], params[16640]
+static const uint32_t kBlobVersion = 2;  // 2: 64 inputs (cone transmittance added)
 static size_t blob_bytes() { return 8 + 6 * 4 + (2 * kMlpIn + kMlpParams) * sizeof(float); }
 
 rc_status rc_surrogate_save(const rc_ctx *c, void *buf, uint64_t cap, uint64_t *size)
@@ -1384,14 +1384,24 @@ static void init_params(uint64_t seed, std::vector<float> &p)
     }
 }
 
+static rc_status train_rows(rc_ctx *c, const float *feats, float *targets, size_t N, const rc_train_config *cfg,
+                            rc_train_report *rep, std::chrono::steady_clock::time_point t_start, uint32_t scenes);
+
+// the optimiser fields shared by both trainers
+static bool train_cfg_ok(const rc_train_config *cfg)
+{
+    return cfg->version == RC_ABI_VERSION && cfg->epochs >= 1 && cfg->epochs <= RC_TRAIN_MAX_EPOCHS &&
+           cfg->batch >= 32 && cfg->batch % 32 == 0 && cfg->batch <= (1u << 22) && cfg->lr > 0.0f;
+}
+
 rc_status rc_surrogate_train(rc_ctx *c, const rc_train_config *cfg, rc_train_report *rep)
 {
     if (!c || !cfg) return fail(c, RC_ERR_NULL, "rc_surrogate_train: NULL argument");
-    if (cfg->version != RC_ABI_VERSION || cfg->scenes < 1 || cfg->epochs < 1 || cfg->epochs > RC_TRAIN_MAX_EPOCHS ||
-        cfg->batch < 32 || cfg->batch % 32 != 0 || cfg->batch > (1u << 22) || cfg->levels < 3 ||
-        cfg->levels > RC_MAX_LEVELS || !dims_ok(cfg->width, cfg->height) || cfg->n_boxes > RC_MAX_BOXES ||
-        cfg->n_emissive > cfg->n_boxes || !(cfg->lr > 0.0f))
+    if (!train_cfg_ok(cfg) || cfg->scenes < 1 || cfg->levels < 3 || cfg->levels > RC_MAX_LEVELS ||
+        !dims_ok(cfg->width, cfg->height) || cfg->n_boxes > RC_MAX_BOXES || cfg->n_emissive > cfg->n_boxes)
         return fail(c, RC_ERR_INVALID, "rc_surrogate_train: bad configuration");
+    const size_t rows = (size_t)grid_dim(cfg->width, 2) * grid_dim(cfg->height, 2) * cfg->scenes;
+    if (rows > (size_t)INT32_MAX) return fail(c, RC_ERR_INVALID, "rc_surrogate_train: more than 2^31 - 1 rows");
     HIP_TRY(c, hipSetDevice(c->cfg.device));
     auto now = []() { return std::chrono::steady_clock::now(); };
     const auto t_start = now();
@@ -1416,9 +1426,7 @@ rc_status rc_surrogate_train(rc_ctx *c, const rc_train_config *cfg, rc_train_rep
     const size_t P = (size_t)grid_dim(cfg->width, 2) * grid_dim(cfg->height, 2);  // rows per scene
     const size_t N = P * cfg->scenes;
     float *depth = nullptr, *emis = nullptr, *feats = nullptr, *targets = nullptr;
-    double *boxes = nullptr, *stats = nullptr;
-    int *idx = nullptr;
-    float *partial = nullptr, *loss_part = nullptr, *grad = nullptr, *step_loss = nullptr;
+    double *boxes = nullptr;
     HIP_TRY(c, hipMalloc(&depth, px * sizeof(float)));
     guard.dev.push_back(depth);
     HIP_TRY(c, hipMalloc(&emis, 3 * px * sizeof(float)));
@@ -1441,10 +1449,34 @@ rc_status rc_surrogate_train(rc_ctx *c, const rc_train_config *cfg, rc_train_rep
         st = gen_samples(tc, depth, emis, feats + sc * P * kMlpIn, targets + sc * P * kMlpRad, s);
         if (st != RC_OK) return fail(c, st, std::string("rc_surrogate_train: ") + tc->err);
     }
+    HIP_TRY(c, hipDeviceSynchronize());
+    return train_rows(c, feats, targets, N, cfg, rep, t_start, cfg->scenes);
+}
+
+// Normalisation, initial weights and the epochs of Adam steps on N rows
+// already on the device (targets are scaled in place by 1 / out_scale).
+static rc_status train_rows(rc_ctx *c, const float *feats, float *targets, size_t N, const rc_train_config *cfg,
+                            rc_train_report *rep, std::chrono::steady_clock::time_point t_start, uint32_t scenes)
+{
+    auto now = []() { return std::chrono::steady_clock::now(); };
+    std::vector<void *> owned;
+    struct Free {
+        std::vector<void *> &v;
+        ~Free()
+        {
+            for (void *p : v)
+                if (p) hipFree(p);
+        }
+    } free_guard{owned};
+    hipStream_t s = nullptr;
+    double *stats = nullptr;
+    int *idx = nullptr;
+    float *partial = nullptr, *loss_part = nullptr, *grad = nullptr, *step_loss = nullptr;
+    rc_status st;
     // normalisation: per-input mean and std, one RMS scale for the targets
     const int parts = 1024;
     HIP_TRY(c, hipMalloc(&stats, (size_t)parts * 2 * kMlpIn * sizeof(double)));
-    guard.dev.push_back(stats);
+    owned.push_back(stats);
     std::vector<double> hs((size_t)parts * 2 * kMlpIn);
     std::vector<float> mean(kMlpIn), istd(kMlpIn);
     HIP_TRY(c, launch_col_stats(feats, N, kMlpIn, stats, parts, s));
@@ -1481,15 +1513,15 @@ rc_status rc_surrogate_train(rc_ctx *c, const rc_train_config *cfg, rc_train_rep
     const size_t steps = N / (size_t)batch;
     if (steps < 1) return fail(c, RC_ERR_INVALID, "rc_surrogate_train: fewer samples than one batch");
     HIP_TRY(c, hipMalloc(&idx, N * sizeof(int)));
-    guard.dev.push_back(idx);
+    owned.push_back(idx);
     HIP_TRY(c, hipMalloc(&partial, (size_t)wgs * kMlpParams * sizeof(float)));
-    guard.dev.push_back(partial);
+    owned.push_back(partial);
     HIP_TRY(c, hipMalloc(&loss_part, (size_t)wgs * sizeof(float)));
-    guard.dev.push_back(loss_part);
+    owned.push_back(loss_part);
     HIP_TRY(c, hipMalloc(&grad, kMlpParams * sizeof(float)));
-    guard.dev.push_back(grad);
+    owned.push_back(grad);
     HIP_TRY(c, hipMalloc(&step_loss, steps * sizeof(float)));
-    guard.dev.push_back(step_loss);
+    owned.push_back(step_loss);
     std::vector<int> perm(N);
     std::vector<float> hl(steps);
     uint64_t rs = cfg->seed ^ 0x5DEECE66Dull;
@@ -1520,7 +1552,7 @@ rc_status rc_surrogate_train(rc_ctx *c, const rc_train_config *cfg, rc_train_rep
         if (rep) rep->epoch_loss[e] = (float)(tot / (double)steps);
     }
     HIP_TRY(c, hipDeviceSynchronize());
-    c->train_scenes = cfg->scenes;
+    c->train_scenes = scenes;
     c->mlp_ready = true;
     drop_graph(c);
     if (rep) {
@@ -1532,6 +1564,18 @@ rc_status rc_surrogate_train(rc_ctx *c, const rc_train_config *cfg, rc_train_rep
         rep->out_scale = out_scale;
     }
     return RC_OK;
+}
+
+rc_status rc_surrogate_train_rows(rc_ctx *c, const float *feats, float *targets, uint64_t n_rows,
+                                  const rc_train_config *cfg, rc_train_report *rep)
+{
+    if (!c || !cfg || !feats || !targets) return fail(c, RC_ERR_NULL, "rc_surrogate_train_rows: NULL argument");
+    if (!train_cfg_ok(cfg)) return fail(c, RC_ERR_INVALID, "rc_surrogate_train_rows: bad configuration");
+    if (n_rows < cfg->batch) return fail(c, RC_ERR_INVALID, "rc_surrogate_train_rows: fewer rows than one batch");
+    if (n_rows > (uint64_t)INT32_MAX) return fail(c, RC_ERR_INVALID, "rc_surrogate_train_rows: more than 2^31 - 1 rows");
+    HIP_TRY(c, hipSetDevice(c->cfg.device));
+    HIP_TRY(c, hipDeviceSynchronize());
+    return train_rows(c, feats, targets, (size_t)n_rows, cfg, rep, std::chrono::steady_clock::now(), cfg->scenes);
 }
 
 rc_status rc_debug_surrogate(rc_ctx *c, float *cone, float *level1, float *tile_err, void *stream)

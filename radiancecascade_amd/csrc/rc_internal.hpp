@@ -75,11 +75,13 @@ hipError_t launch_error_norms(const float *x, const void *ref, bool ref_f64, siz
                               double *out, hipStream_t s);
 
 // ---- surrogate MLP (rc_mlp.hip) ---------------------------------------------
-// 48 -> 64 -> 64 -> 64 -> 64, ReLU on the hidden layers; outputs 0..47 the
+// 64 -> 64 -> 64 -> 64 -> 64, ReLU on the hidden layers; inputs per direction
+// (log(1 + c) rgb, cone transmittance T) x 16; outputs 0..47 the
 // far-field residual A_2 - c (16 directions x rgb, normalised), 48 the log of the
 // predicted squared error, 49..63 padding.
-constexpr int kMlpIn = 48, kMlpHid = 64, kMlpOut = 64, kMlpRad = 48, kMlpErr = 48;
-constexpr int kMlpParams = kMlpIn * 64 + 64 + 3 * (64 * 64 + 64);  // 15616 fp32 master parameters
+constexpr int kMlpIn = 64, kMlpHid = 64, kMlpOut = 64, kMlpRad = 48, kMlpErr = 48;
+constexpr int kMlpFeat = 4;  // inputs per direction: log(1 + c) rgb, T
+constexpr int kMlpParams = kMlpIn * 64 + 64 + 3 * (64 * 64 + 64);  // 16640 fp32 master parameters
 constexpr int kMlpWt = 64 * kMlpIn + 3 * 64 * kMlpHid;            // bf16 weights per copy
 constexpr float kMlpErrEps = 1e-2f, kMlpErrWeight = 0.05f;
 
@@ -89,7 +91,7 @@ struct MlpDev {
     uint16_t *wt;           // bf16 W_l^T [64][K_l] (output-major), layers back to back
     uint16_t *wk;           // bf16 W_l [K_l][64] (input-major)
     float *bias;            // [4][64]
-    float *in_mean, *in_istd;  // [48] input standardisation
+    float *in_mean, *in_istd;  // [kMlpIn] input standardisation
 };
 
 hipError_t launch_mlp_infer(const MlpDev &m, const float *feats, int n, float *out, hipStream_t s);
@@ -168,7 +170,7 @@ struct SurrMlpArgs {
 struct SurrSampleArgs {
     const float4 *cone;
     const void *upper;      // merged level 2 of the plain path (storage texels)
-    float *feats;           // [QY*QX][48] log(1 + cone radiance)
+    float *feats;           // [QY*QX][64] per direction log(1 + cone radiance) rgb, T
     float *targets;         // [QY*QX][48] merged level 2 minus cone radiance
     int QX, QY;
 };

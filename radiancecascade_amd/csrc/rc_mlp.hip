@@ -2,7 +2,7 @@
 // training), backward, gradient reduction and Adam.
 //
 // Network (the project's own; the reference has no learned part):
-// 48 -> 64 -> 64 -> 64 -> 64, ReLU on the hidden layers.  Outputs 0..47 are
+// 64 -> 64 -> 64 -> 64 -> 64, ReLU on the hidden layers.  Outputs 0..47 are
 // the level-1 far-field radiance (16 directions x rgb) in normalised units,
 // output 48 the log of the predicted per-sample squared error (the per-tile
 // fallback signal), outputs 49..63 padding.  Layer l computes
@@ -112,7 +112,7 @@ __device__ inline void mlp_forward32(const uint16_t *x, uint16_t *h1, uint16_t *
 }
 
 // ---- inference over a feature matrix -------------------------------------------
-// features [n][48] (already transformed, standardised here) -> z [n][64]
+// features [n][64] (already transformed, standardised here) -> z [n][64]
 // fp32.  Each wave takes a 32-row block per iteration; the iteration count is
 // the same for every wave of the workgroup (a wave past the end computes on
 // zero rows and stores nothing), so the barriers are uniform.
@@ -196,7 +196,7 @@ __global__ __launch_bounds__(64) void k_mlp_train(MlpDev m, const float *__restr
             xT[c * kStrideT + r] = b;
             tg[r][c] = ok ? targets[(size_t)sid * kMlpRad + c] : 0.0f;
         }
-        for (int i = lane; i < 16 * 32; i += 64) xT[(kMlpIn + i / 32) * kStrideT + (i % 32)] = 0;  // pad rows 48..63
+        for (int i = lane; i < (64 - kMlpIn) * 32; i += 64) xT[(kMlpIn + i / 32) * kStrideT + (i % 32)] = 0;  // pad rows kMlpIn..63
         __syncthreads();
         floatx16 z[2];
         mlp_forward32(x, h1, h2, h3, wt, m.bias, z, hT1, hT2, hT3);

@@ -594,9 +594,10 @@ hipError_t launch_cone_pair(const ConeArgs &near, const ConeArgs &far, hipStream
 
 // ---- the network's level 2 -----------------------------------------------------------
 // A workgroup owns 8x8 level-2 probes per iteration (persistent over tiles).
-// Lane (probe p, quad ko) builds the 12 inputs of directions 4ko..4ko+3 of p,
-// log(1 + c) of the probe's cone radiance; wave w then computes the 32x32
-// output block (rows 32(w&1), columns 32(w>>1)) of each MLP layer on MFMA.
+// Lane (probe p, quad ko) builds the 16 inputs of directions 4ko..4ko+3 of p:
+// log(1 + c) of the probe's cone radiance and the cone's transmittance; wave
+// w then computes the 32x32 output block (rows 32(w&1), columns 32(w>>1)) of
+// each MLP layer on MFMA.
 // The merged level 2 is A_2 = max(c + s * z, 0) per direction (4th channel
 // 1; level 1 reads only rgb), written in the storage format of the merged
 // levels; level 1 and level 0 are then the plain kernels.  The error head
@@ -698,13 +699,12 @@ __global__ __launch_bounds__(kBlock) void k_surr_mlp(SurrMlpArgs a, MlpDev m)
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const int col = 3 * (4 * ko + j);
-            xs[p * kSStrideIn + col + 0] =
-                (uint16_t)f32_to_bf16_rne(valid ? (feat_log(cf[j].x) - mean_s[col + 0]) * istd_s[col + 0] : 0.f);
-            xs[p * kSStrideIn + col + 1] =
-                (uint16_t)f32_to_bf16_rne(valid ? (feat_log(cf[j].y) - mean_s[col + 1]) * istd_s[col + 1] : 0.f);
-            xs[p * kSStrideIn + col + 2] =
-                (uint16_t)f32_to_bf16_rne(valid ? (feat_log(cf[j].z) - mean_s[col + 2]) * istd_s[col + 2] : 0.f);
+            const int col = kMlpFeat * (4 * ko + j);
+            const float f[kMlpFeat] = {feat_log(cf[j].x), feat_log(cf[j].y), feat_log(cf[j].z), cf[j].w};
+#pragma unroll
+            for (int e = 0; e < kMlpFeat; ++e)
+                xs[p * kSStrideIn + col + e] =
+                    (uint16_t)f32_to_bf16_rne(valid ? (f[e] - mean_s[col + e]) * istd_s[col + e] : 0.f);
         }
         __syncthreads();
 
@@ -803,9 +803,9 @@ hipError_t launch_surr_mlp(const SurrMlpArgs &a, const MlpDev &m, Storage st, hi
 }
 
 // ---- training samples ------------------------------------------------------------
-// One row per level-2 probe: inputs log(1 + c) and target residual A_2 - c,
-// where c is the probe's cone radiance and A_2 the plain path's merged
-// level 2 (16 entries).
+// One row per level-2 probe: inputs per direction log(1 + c) rgb and the
+// cone's transmittance T, target residual A_2 - c, where c is the probe's cone
+// radiance and A_2 the plain path's merged level 2 (16 entries).
 template <Storage S>
 __global__ __launch_bounds__(kBlock) void k_surr_sample(SurrSampleArgs a)
 {
@@ -816,11 +816,12 @@ __global__ __launch_bounds__(kBlock) void k_surr_sample(SurrSampleArgs a)
     const typename TT::T *up = (const typename TT::T *)a.upper;
     const float4 c = a.cone[gi];
     const float4 u = TT::unpack(up[gi]);
-    float *f = a.feats + (size_t)q * kMlpIn + 3 * k;
+    float *f = a.feats + (size_t)q * kMlpIn + kMlpFeat * k;
     float *t = a.targets + (size_t)q * kMlpRad + 3 * k;
     f[0] = feat_log(c.x);
     f[1] = feat_log(c.y);
     f[2] = feat_log(c.z);
+    f[3] = c.w;
     t[0] = u.x - c.x;
     t[1] = u.y - c.y;
     t[2] = u.z - c.z;

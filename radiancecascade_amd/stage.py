@@ -298,15 +298,15 @@ class RadianceCascadeStage:
         self.load_weights_blob(blob)
 
     def mlp_forward(self, feats, out, stream=None) -> None:
-        """out [n][64] = raw network outputs for log1p features [n][48] (device fp32)."""
+        """out [n][64] = raw network outputs for features [n][MLP_INPUTS] (device fp32)."""
         n = feats.shape[0]
-        check(self._lib.rc_mlp_forward(self._ctx, _dev_ptr(feats, n * 48 * 4, "feats"), n,
+        check(self._lib.rc_mlp_forward(self._ctx, _dev_ptr(feats, n * _abi.MLP_INPUTS * 4, "feats"), n,
                                        _dev_ptr(out, n * 64 * 4, "out"), _stream_handle(stream)), self._ctx)
 
     def mlp_gradient(self, feats, targets, grad, loss, stream=None) -> None:
         n = feats.shape[0]
-        check(self._lib.rc_mlp_gradient(self._ctx, _dev_ptr(feats, n * 48 * 4, "feats"),
-                                        _dev_ptr(targets, n * 48 * 4, "targets"), n,
+        check(self._lib.rc_mlp_gradient(self._ctx, _dev_ptr(feats, n * _abi.MLP_INPUTS * 4, "feats"),
+                                        _dev_ptr(targets, n * _abi.MLP_RADIANCE * 4, "targets"), n,
                                         _dev_ptr(grad, _abi.MLP_PARAMS * 4, "grad"), _dev_ptr(loss, 4, "loss"),
                                         _stream_handle(stream)), self._ctx)
 
@@ -315,13 +315,15 @@ class RadianceCascadeStage:
                                     _stream_handle(stream)), self._ctx)
 
     def surrogate_samples(self, depth, emissive, feats, targets, stream=None) -> None:
-        """Training rows of one device G-buffer: feats/targets [Px_2*Py_2][48]."""
+        """Training rows of one device G-buffer: feats [Px_2*Py_2][MLP_INPUTS],
+        targets [Px_2*Py_2][48]."""
         px, py = probe_grid(self.width, self.height, 2)
         n = px * py
         g = _abi.GBuffer(_dev_ptr(depth, self.width * self.height * 4, "depth"), None,
                          _dev_ptr(emissive, 3 * self.width * self.height * 4, "emissive"))
-        check(self._lib.rc_surrogate_samples(self._ctx, ctypes.byref(g), _dev_ptr(feats, n * 48 * 4, "feats"),
-                                             _dev_ptr(targets, n * 48 * 4, "targets"), _stream_handle(stream)),
+        check(self._lib.rc_surrogate_samples(self._ctx, ctypes.byref(g), _dev_ptr(feats, n * _abi.MLP_INPUTS * 4, "feats"),
+                                             _dev_ptr(targets, n * _abi.MLP_RADIANCE * 4, "targets"),
+                                             _stream_handle(stream)),
               self._ctx)
 
     def train_surrogate(self, width: int = 512, height: int = 512, levels: int = 6, scene_seed0: int = 10000,
@@ -332,6 +334,23 @@ class RadianceCascadeStage:
                                epochs, batch, lr, lr_decay, 0, seed)
         rep = _abi.TrainReport()
         check(self._lib.rc_surrogate_train(self._ctx, ctypes.byref(cfg), ctypes.byref(rep)), self._ctx)
+        return {"epochs": rep.epochs, "samples": rep.samples, "steps": rep.steps,
+                "epoch_loss": list(rep.epoch_loss)[: rep.epochs], "gen_ms": rep.gen_ms, "train_ms": rep.train_ms,
+                "out_scale": rep.out_scale}
+
+    def train_surrogate_rows(self, feats, targets, epochs: int = 8, batch: int = 16384, lr: float = 2e-3,
+                             lr_decay: float = 0.7, seed: int = 1, frames: int = 0) -> dict:
+        """Train this context's network on rows the caller made
+        (rc_surrogate_train_rows): feats [n][MLP_INPUTS], targets [n][48] as
+        surrogate_samples writes them; targets is scaled in place."""
+        n = feats.shape[0]
+        if targets.shape[0] != n:
+            raise ValueError("feats and targets need the same number of rows")
+        cfg = _abi.TrainConfig(_abi.ABI_VERSION, 0, 0, 0, 0, frames, 0, 0, epochs, batch, lr, lr_decay, 0, seed)
+        rep = _abi.TrainReport()
+        check(self._lib.rc_surrogate_train_rows(self._ctx, _dev_ptr(feats, n * _abi.MLP_INPUTS * 4, "feats"),
+                                                _dev_ptr(targets, n * _abi.MLP_RADIANCE * 4, "targets"), n,
+                                                ctypes.byref(cfg), ctypes.byref(rep)), self._ctx)
         return {"epochs": rep.epochs, "samples": rep.samples, "steps": rep.steps,
                 "epoch_loss": list(rep.epoch_loss)[: rep.epochs], "gen_ms": rep.gen_ms, "train_ms": rep.train_ms,
                 "out_scale": rep.out_scale}

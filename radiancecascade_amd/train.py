@@ -2,11 +2,16 @@
 
     python -m radiancecascade_amd.train --scenes 10000 --out weights/surrogate.bin
 
+    python -m radiancecascade_amd.train --dumps a.rcdump b.rcdump --out weights.bin
+
 Rows come from the plain cascades on seeded box scenes
 (rc_scene_generate(scene_seed0 + s, n_boxes, n_emissive)) at the training
-resolution; the network is trained by the library's own forward / backward /
-Adam kernels (rc_surrogate_train).  The blob gets a `.json` sidecar with its
-SHA-256, the configuration and the per-epoch losses.
+resolution (rc_surrogate_train), or, with --dumps, on the G-buffers
+("depth" [1][H][W] and "emissive" [3][H][W] planes) of RCDUMP01 files, each at
+its own size (rc_surrogate_samples, then rc_surrogate_train_rows).  The network
+is trained by the library's own forward / backward / Adam kernels.  The blob
+gets a `.json` sidecar with its SHA-256, the configuration and the per-epoch
+losses.
 """
 from __future__ import annotations
 
@@ -15,6 +20,45 @@ import json
 import os
 import sys
 import time
+
+
+def rows_from_dumps(paths, levels: int):
+    """Training rows of the G-buffers stored in RCDUMP01 files: (feats
+    [n][MLP_INPUTS], targets [n][48]) device tensors, rows in file order."""
+    import torch
+
+    from . import _abi
+    from .stage import RadianceCascadeStage, dump_info, dump_read, probe_grid
+
+    shapes = []
+    for p in paths:
+        info = dump_info(p)
+        if "depth" not in info or "emissive" not in info:
+            raise ValueError(f"{p}: needs planes 'depth' and 'emissive'")
+        c, h, w = info["depth"]
+        if c != 1 or info["emissive"] != (3, h, w):
+            raise ValueError(f"{p}: depth must be [1][H][W] and emissive [3][H][W] of the same size")
+        qx, qy = probe_grid(w, h, 2)
+        shapes.append((w, h, qx * qy))
+    n = sum(r for _, _, r in shapes)
+    dev = torch.device("cuda:0")
+    feats = torch.empty((n, _abi.MLP_INPUTS), dtype=torch.float32, device=dev)
+    targets = torch.empty((n, _abi.MLP_RADIANCE), dtype=torch.float32, device=dev)
+    stages, row = {}, 0
+    try:
+        for p, (w, h, r) in zip(paths, shapes):
+            if (w, h) not in stages:
+                stages[(w, h)] = RadianceCascadeStage(w, h, levels=levels)
+            g = dump_read(p, names=("depth", "emissive"))
+            depth = torch.from_numpy(g["depth"][0]).to(dev)
+            emis = torch.from_numpy(g["emissive"]).to(dev)
+            stages[(w, h)].surrogate_samples(depth, emis, feats[row:row + r], targets[row:row + r])
+            row += r
+        torch.cuda.synchronize()
+    finally:
+        for st in stages.values():
+            st.close()
+    return feats, targets
 
 
 def main(argv=None) -> int:
@@ -31,22 +75,40 @@ def main(argv=None) -> int:
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--lr-decay", type=float, default=0.7)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--dumps", nargs="+", default=None, metavar="FILE",
+                    help="train on the G-buffers of these RCDUMP01 files instead of seeded scenes")
     ap.add_argument("--out", required=True)
     a = ap.parse_args(argv)
 
     from . import RadianceCascadeStage
     t0 = time.time()
     with RadianceCascadeStage(64, 64, levels=3) as st:
-        print(f"training on {a.scenes} scenes at {a.width}x{a.height} L={a.levels}, {a.epochs} epochs", flush=True)
-        rep = st.train_surrogate(width=a.width, height=a.height, levels=a.levels, scene_seed0=a.scene_seed0,
-                                 scenes=a.scenes, n_boxes=a.boxes, n_emissive=a.emissive, epochs=a.epochs,
-                                 batch=a.batch, lr=a.lr, lr_decay=a.lr_decay, seed=a.seed)
+        if a.dumps:
+            print(f"training on the G-buffers of {len(a.dumps)} dump files, L={a.levels}, {a.epochs} epochs",
+                  flush=True)
+            tg = time.time()
+            feats, targets = rows_from_dumps(a.dumps, a.levels)
+            gen_ms = (time.time() - tg) * 1e3
+            rep = st.train_surrogate_rows(feats, targets, epochs=a.epochs, batch=a.batch, lr=a.lr,
+                                          lr_decay=a.lr_decay, seed=a.seed, frames=len(a.dumps))
+            rep["gen_ms"] += gen_ms
+            del feats, targets
+            from .stage import sha256
+            train = {"dumps": [{"file": os.path.basename(p), "sha256": sha256(open(p, "rb").read())}
+                               for p in a.dumps]}
+        else:
+            print(f"training on {a.scenes} scenes at {a.width}x{a.height} L={a.levels}, {a.epochs} epochs",
+                  flush=True)
+            rep = st.train_surrogate(width=a.width, height=a.height, levels=a.levels, scene_seed0=a.scene_seed0,
+                                     scenes=a.scenes, n_boxes=a.boxes, n_emissive=a.emissive, epochs=a.epochs,
+                                     batch=a.batch, lr=a.lr, lr_decay=a.lr_decay, seed=a.seed)
+            train = {"width": a.width, "height": a.height, "scene_seed0": a.scene_seed0, "scenes": a.scenes,
+                     "n_boxes": a.boxes, "n_emissive": a.emissive}
         for e, v in enumerate(rep["epoch_loss"]):
             print(f"epoch {e}: loss {v:.6f}", flush=True)
-        meta = {"train": {"width": a.width, "height": a.height, "levels": a.levels, "scene_seed0": a.scene_seed0,
-                          "scenes": a.scenes, "n_boxes": a.boxes, "n_emissive": a.emissive, "epochs": a.epochs,
-                          "batch": a.batch, "lr": a.lr, "lr_decay": a.lr_decay, "seed": a.seed},
-                "report": rep}
+        train.update({"levels": a.levels, "epochs": a.epochs, "batch": a.batch, "lr": a.lr,
+                      "lr_decay": a.lr_decay, "seed": a.seed})
+        meta = {"train": train, "report": rep}
         os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
         sha = st.save_weights(a.out, meta)
     print(json.dumps({"out": a.out, "sha256": sha, "samples": rep["samples"], "steps": rep["steps"],

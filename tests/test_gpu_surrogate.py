@@ -51,8 +51,8 @@ def _net(seed=3, scale=0.5):
     from oracle import surrogate as so
     rng = np.random.default_rng(seed)
     params = so.layers_to_params(mlp.init_params(seed)).astype(np.float32)
-    mean = rng.normal(0.2, 0.1, 48).astype(np.float32)
-    istd = rng.uniform(0.5, 2.0, 48).astype(np.float32)
+    mean = rng.normal(0.2, 0.1, mlp.IN).astype(np.float32)
+    istd = rng.uniform(0.5, 2.0, mlp.IN).astype(np.float32)
     return params, mean, istd, np.float32(scale)
 
 
@@ -121,7 +121,7 @@ def test_surrogate_frame_parity(rc, W, H, L, seed, nb, ne):
 
 def _rand_io(n, seed):
     rng = np.random.default_rng(seed)
-    x = rng.gamma(1.0, 0.5, (n, 48)).astype(np.float32)      # log1p features are >= 0
+    x = rng.gamma(1.0, 0.5, (n, 64)).astype(np.float32)      # log1p features and T are >= 0
     t = rng.normal(0.0, 1.0, (n, 48)).astype(np.float32)
     return x, t
 
@@ -158,7 +158,7 @@ def test_mlp_gradient_and_adam_parity(rc):
     with rc.RadianceCascadeStage(64, 64, levels=3) as st:
         st.mlp_set(params, mean, istd, scale)
         xd, td = torch.from_numpy(x).cuda(), torch.from_numpy(t).cuda()
-        grad = torch.empty(15616, dtype=torch.float32, device="cuda:0")
+        grad = torch.empty(rc._abi.MLP_PARAMS, dtype=torch.float32, device="cuda:0")
         lo = torch.empty(1, dtype=torch.float32, device="cuda:0")
         st.mlp_gradient(xd, td, grad, lo)
         g = grad.cpu().numpy()
@@ -185,7 +185,7 @@ def test_training_rows_parity(rc):
     W, H, L = 160, 128, 5
     depth, emis = _gbuffer(rc, W, H, 9, 8, 2)
     px, py = rc.probe_grid(W, H, 2)
-    feats = torch.empty((px * py, 48), dtype=torch.float32, device="cuda:0")
+    feats = torch.empty((px * py, 64), dtype=torch.float32, device="cuda:0")
     targ = torch.empty((px * py, 48), dtype=torch.float32, device="cuda:0")
     with rc.RadianceCascadeStage(W, H, levels=L) as st:
         st.surrogate_samples(depth, emis, feats, targ)
@@ -265,3 +265,48 @@ def test_training_reduces_loss(rc):
     with rc.RadianceCascadeStage(64, 64, levels=3) as st2:
         st2.load_weights_blob(blob)
         assert st2.weights_blob() == blob
+
+
+def test_training_on_dumped_gbuffers_matches_seeded_training(rc, tmp_path):
+    """Rows from G-buffers read back from RCDUMP01 files train the same
+    network, bit for bit, as the seeded trainer on the same scenes."""
+    from radiancecascade_amd import train as tr
+    W, H, L, seeds = 256, 192, 5, [500, 501, 502, 503]
+    print(f"[train rows] {W}x{H} L={L} seeds {seeds}", flush=True)
+    kw = dict(epochs=2, batch=1024, lr=2e-3, lr_decay=0.7, seed=7)
+    with rc.RadianceCascadeStage(64, 64, levels=3) as st:
+        ra = st.train_surrogate(width=W, height=H, levels=L, scene_seed0=seeds[0], scenes=len(seeds), n_boxes=32,
+                                n_emissive=4, **kw)
+        ma = st.mlp_get()
+    dev = torch.device("cuda:0")
+    depth = torch.empty((H, W), device=dev)
+    emis = torch.empty((3, H, W), device=dev)
+    paths = []
+    for s in seeds:
+        rc.gbuffer_raster(rc.scene_generate(s, 32, 4), W, H, depth, None, emis)
+        p = str(tmp_path / f"scene{s}.rcdump")
+        rc.dump_write(p, {"depth": depth.cpu().numpy(), "emissive": emis.cpu().numpy()})
+        paths.append(p)
+    feats, targets = tr.rows_from_dumps(paths, L)
+    qx, qy = rc.probe_grid(W, H, 2)
+    assert feats.shape == (len(seeds) * qx * qy, rc._abi.MLP_INPUTS)
+    with rc.RadianceCascadeStage(64, 64, levels=3) as st:
+        rb = st.train_surrogate_rows(feats, targets, frames=len(seeds), **kw)
+        mb = st.mlp_get()
+    assert rb["samples"] == ra["samples"] and rb["steps"] == ra["steps"]
+    assert rb["epoch_loss"] == ra["epoch_loss"] and rb["out_scale"] == ra["out_scale"]
+    for k in ("params", "in_mean", "in_istd"):
+        assert np.array_equal(ma[k], mb[k]), k
+
+
+def test_train_rows_rejects_bad_input(rc):
+    n = 512
+    feats = torch.zeros((n, rc._abi.MLP_INPUTS), device="cuda:0")
+    targets = torch.zeros((n, rc._abi.MLP_RADIANCE), device="cuda:0")
+    with rc.RadianceCascadeStage(64, 64, levels=3) as st:
+        with pytest.raises(rc.RcError):
+            st.train_surrogate_rows(feats, targets, batch=1024)      # fewer rows than one batch
+        with pytest.raises(rc.RcError):
+            st.train_surrogate_rows(feats, targets, batch=100)       # not a multiple of 32
+        with pytest.raises(ValueError):
+            st.train_surrogate_rows(feats, targets[:256], batch=256)

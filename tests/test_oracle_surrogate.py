@@ -225,11 +225,13 @@ def test_samples_and_params_roundtrip():
     A2 = rng.random((5, 7, 16, 4))
     x, t = so.samples(cone, A2)
     c, _ = so.features(cone)
-    assert x.shape == (35, 48) and np.allclose(np.expm1(x) + t, A2[..., :3].reshape(-1, 48))
+    xd = x.reshape(35, 16, 4)
+    assert x.shape == (35, 64) and np.allclose(np.expm1(xd[..., :3]).reshape(-1, 48) + t, A2[..., :3].reshape(-1, 48))
+    assert np.array_equal(xd[..., 3], cone[..., 3].reshape(35, 16))
     assert np.array_equal(c, cone[..., :3].reshape(-1, 48))
     from oracle import mlp
     layers = mlp.init_params(3)
     p = so.layers_to_params(layers)
-    assert p.size == 15616
+    assert p.size == 16640
     back = so.params_to_layers(p)
     assert all(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) for a, b in zip(layers, back))
