@@ -119,6 +119,34 @@ def test_surrogate_frame_parity(rc, W, H, L, seed, nb, ne):
     assert s.surrogate_frames == 1 and s.fallback_frames == 0 and s.tiles_per_frame == ty * tx
 
 
+def test_mip_pyramid_of_a_wide_frame(rc):
+    """4096x1200 (13 mip levels, level 6 with 1216 cells, a 19-row last tile
+    row): every level against the oracle, on two frames in a row."""
+    from oracle import rco
+    from oracle import surrogate as so
+    W, H, L = 4096, 1200, 7
+    _say(f"[mip pyramid] {W}x{H} L={L}")
+    depth, emis = _gbuffer(rc, W, H, 5, 32, 4)
+    params, mean, istd, scale = _net()
+    irr = torch.empty((3, H, W), dtype=torch.float32, device="cuda:0")
+    d, e = depth.cpu().numpy(), emis.cpu().numpy()
+    ref = so.mip(rco.seed_init(d, e), e)
+    with rc.RadianceCascadeStage(W, H, levels=L) as st:
+        st.mlp_set(params, mean, istd, scale)
+        st.set_surrogate(True)
+        for frame in range(2):
+            st.run_device(depth, emis, irr)
+            for m in range(1, so.top_level(W, H) + 1):
+                w, h = -(-W // (1 << m)), -(-H // (1 << m))
+                eg = torch.empty((h, w, 4), dtype=torch.float32, device="cuda:0")
+                og = torch.empty((h, w, 4), dtype=torch.float32, device="cuda:0")
+                st.debug_mip(m, eg, og)
+                torch.cuda.synchronize()
+                re, ro = ref[m]
+                assert np.allclose(eg.cpu().numpy(), re, rtol=MIP_RTOL, atol=1e-6), (frame, m)
+                assert np.allclose(og.cpu().numpy(), ro, rtol=MIP_RTOL, atol=1e-6), (frame, m)
+
+
 def _rand_io(n, seed):
     rng = np.random.default_rng(seed)
     x = rng.gamma(1.0, 0.5, (n, 64)).astype(np.float32)      # log1p features and T are >= 0
