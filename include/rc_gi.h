@@ -128,6 +128,13 @@ rc_status rc_create(const rc_config *cfg, rc_ctx **out);
 rc_status rc_resize(rc_ctx *ctx, uint32_t width, uint32_t height, uint32_t levels);
 rc_status rc_update_host(rc_ctx *ctx, const rc_gbuffer *host);
 rc_status rc_render_host(rc_ctx *ctx, float *irradiance_host);
+/* Page-locked host memory for the host-staging path: G-buffer planes and
+ * irradiance held in it move by DMA at the link's rate, where pageable memory
+ * goes through the runtime's staging copies.  No counterpart in the
+ * reference; its uploads (src/renderer.rs:620-631) would read their source
+ * planes from such buffers.  rc_host_free(NULL) is a no-op. */
+rc_status rc_host_alloc(uint64_t bytes, void **out);
+void rc_host_free(void *ptr);
 rc_status rc_run_device(rc_ctx *ctx, const rc_gbuffer *dev, float *irradiance_dev, void *stream);
 void rc_destroy(rc_ctx *ctx);
 

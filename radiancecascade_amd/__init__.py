@@ -8,9 +8,9 @@ is no CPU fallback and importing the stage fails if the library is absent.
 """
 from ._abi import RcError, load  # noqa: F401
 from .stage import (RadianceCascadeBatch, RadianceCascadeStage, batch_views_spec, device_count, dump_info, dump_read,  # noqa: F401
-                    dump_write, gbuffer_raster, level_rays, levels_for_coverage, packaged_weights, probe_grid,
+                    dump_write, gbuffer_raster, host_buffer, level_rays, levels_for_coverage, packaged_weights, probe_grid,
                     scene_generate, sha256, srgb_thresholds)
 
 __all__ = ["RadianceCascadeBatch", "RadianceCascadeStage", "batch_views_spec", "RcError", "device_count", "gbuffer_raster", "level_rays",
            "levels_for_coverage", "load", "packaged_weights", "probe_grid", "scene_generate", "srgb_thresholds",
-           "dump_write", "dump_info", "dump_read", "sha256"]
+           "dump_write", "dump_info", "dump_read", "sha256", "host_buffer"]

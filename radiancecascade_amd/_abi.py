@@ -140,6 +140,8 @@ SIGNATURES = {
     "rc_mlp_adam": (_i32, [_P, _P, ctypes.c_float, _P]),
     "rc_surrogate_samples": (_i32, [_P, ctypes.POINTER(GBuffer), _P, _P, _P]),
     "rc_surrogate_train": (_i32, [_P, ctypes.POINTER(TrainConfig), ctypes.POINTER(TrainReport)]),
+    "rc_host_alloc": (_i32, [_u64, ctypes.POINTER(_P)]),
+    "rc_host_free": (None, [_P]),
     "rc_surrogate_train_rows": (_i32, [_P, _P, _P, ctypes.c_uint64, ctypes.POINTER(TrainConfig),
                                        ctypes.POINTER(TrainReport)]),
     "rc_debug_surrogate": (_i32, [_P, _P, _P, _P, _P]),

@@ -865,6 +865,25 @@ rc_status rc_render_host(rc_ctx *c, float *irr_host)
     return RC_OK;
 }
 
+rc_status rc_host_alloc(uint64_t bytes, void **out)
+{
+    if (!out) return fail(nullptr, RC_ERR_NULL, "rc_host_alloc: out is NULL");
+    *out = nullptr;
+    if (bytes == 0) return fail(nullptr, RC_ERR_INVALID, "rc_host_alloc: zero bytes");
+    const hipError_t e = hipHostMalloc(out, (size_t)bytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        *out = nullptr;
+        return fail(nullptr, e == hipErrorOutOfMemory ? RC_ERR_OOM : RC_ERR_HIP,
+                    std::string("rc_host_alloc: ") + hipGetErrorString(e));
+    }
+    return RC_OK;
+}
+
+void rc_host_free(void *ptr)
+{
+    if (ptr) hipHostFree(ptr);
+}
+
 void rc_destroy(rc_ctx *c)
 {
     if (!c) return;

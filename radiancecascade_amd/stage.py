@@ -16,6 +16,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -105,8 +106,13 @@ class RadianceCascadeStage:
         check(self._lib.rc_update_host(self._ctx, ctypes.byref(g)), self._ctx)
 
     # -- RenderStage::render (src/app.rs:4): run the pass --------------------
-    def render(self) -> np.ndarray:
-        out = np.empty((3, self.height, self.width), np.float32)
+    def render(self, out: np.ndarray | None = None) -> np.ndarray:
+        """Irradiance [3][H][W] into `out` (e.g. a host_buffer) or a new array."""
+        shape = (3, self.height, self.width)
+        if out is None:
+            out = np.empty(shape, np.float32)
+        elif out.shape != shape or out.dtype != np.float32 or not out.flags["C_CONTIGUOUS"]:
+            raise ValueError(f"out must be a contiguous float32 array of shape {shape}")
         check(self._lib.rc_render_host(self._ctx, out.ctypes.data), self._ctx)
         return out
 
@@ -491,6 +497,21 @@ def packaged_weights() -> str:
 def _dump_check(status: int) -> None:
     if status != 0:
         raise RcError(status, (_abi.load().rc_dump_last_error() or b"").decode())
+
+
+def host_buffer(shape, dtype=np.float32) -> np.ndarray:
+    """A numpy array in page-locked host memory (rc_host_alloc), freed with the
+    array: G-buffer planes uploaded from it and irradiance rendered into it
+    move by DMA."""
+    lib = _abi.load()
+    dtype = np.dtype(dtype)
+    n = int(np.prod(shape)) * dtype.itemsize
+    ptr = ctypes.c_void_p()
+    check(lib.rc_host_alloc(max(n, 1), ctypes.byref(ptr)))
+    raw = (ctypes.c_ubyte * max(n, 1)).from_address(ptr.value)
+    weakref.finalize(raw, lib.rc_host_free, ptr.value)
+    arr = np.frombuffer(raw, dtype=dtype, count=int(np.prod(shape))).reshape(shape)
+    return arr
 
 
 def dump_write(path: str, planes: dict) -> None:

@@ -59,6 +59,10 @@ def test_null_context_calls():
     assert lib.rc_run_device(None, None, None, None) == _abi.ERR_NULL
     assert lib.rc_autotune(None, None, None, 1, None) == _abi.ERR_NULL
     lib.rc_destroy(None)  # no-op
+    assert lib.rc_host_alloc(8, None) == _abi.ERR_NULL
+    ptr = ctypes.c_void_p(1)
+    assert lib.rc_host_alloc(0, ctypes.byref(ptr)) == _abi.ERR_INVALID and not ptr.value
+    lib.rc_host_free(None)  # no-op
 
 
 @pytest.mark.skipif(rc.device_count() > 0, reason="a HIP device is present")

@@ -170,6 +170,23 @@ def test_host_staging_path(rc, oracle):
     assert np.array_equal(irr.view(np.uint32), irr32.view(np.uint32))
 
 
+def test_host_staging_from_page_locked_buffers(rc, oracle):
+    """The same path with the planes and the output in rc_host_alloc memory."""
+    W, H, L = 160, 120, 5
+    d, a, e = oracle.gbuffer(9, W, H, 8, 2)
+    dp, ep, out = rc.host_buffer((H, W)), rc.host_buffer((3, H, W)), rc.host_buffer((3, H, W))
+    dp[...] = d
+    ep[...] = e
+    with rc.RadianceCascadeStage(W, H, levels=L) as st:
+        st.update(dp, ep)
+        got = st.render(out)
+        assert got is out
+        with pytest.raises(ValueError):
+            st.render(np.empty((3, H, W + 1), np.float32))
+    irr32 = oracle.run(d, e, L, np.float32)
+    assert np.array_equal(out.view(np.uint32), irr32.view(np.uint32))
+
+
 def test_resize(rc, oracle):
     """RenderStage::resize reallocates; the next frame is correct at the new size."""
     with rc.RadianceCascadeStage(64, 64, levels=3) as st:

@@ -338,3 +338,35 @@ def test_train_rows_rejects_bad_input(rc):
             st.train_surrogate_rows(feats, targets, batch=100)       # not a multiple of 32
         with pytest.raises(ValueError):
             st.train_surrogate_rows(feats, targets[:256], batch=256)
+
+
+def test_train_cli_on_dump_files(rc, tmp_path):
+    """`python -m radiancecascade_amd.train --dumps ...` end to end: the blob
+    loads, and its sidecar names the dump files with their SHA-256."""
+    import json
+
+    from radiancecascade_amd import train as tr
+    W, H = 256, 192
+    _say(f"[train cli] {W}x{H} two dumps")
+    dev = torch.device("cuda:0")
+    depth = torch.empty((H, W), device=dev)
+    emis = torch.empty((3, H, W), device=dev)
+    paths = []
+    for s in (600, 601):
+        rc.gbuffer_raster(rc.scene_generate(s, 32, 4), W, H, depth, None, emis)
+        p = str(tmp_path / f"f{s}.rcdump")
+        rc.dump_write(p, {"depth": depth.cpu().numpy(), "emissive": emis.cpu().numpy()})
+        paths.append(p)
+    out = str(tmp_path / "net.bin")
+    assert tr.main(["--dumps", *paths, "--levels", "5", "--epochs", "2", "--batch", "512", "--out", out]) == 0
+    meta = json.load(open(out + ".json"))
+    assert [d["file"] for d in meta["train"]["dumps"]] == ["f600.rcdump", "f601.rcdump"]
+    assert meta["train"]["dumps"][0]["sha256"] == rc.sha256(open(paths[0], "rb").read())
+    assert len(meta["report"]["epoch_loss"]) == 2
+    with rc.RadianceCascadeStage(W, H, levels=5) as st:
+        st.load_weights(out)
+        st.set_surrogate(True)
+        irr = torch.empty((3, H, W), device=dev)
+        st.run_device(depth, emis, irr)
+        torch.cuda.synchronize()
+        assert torch.isfinite(irr).all()
