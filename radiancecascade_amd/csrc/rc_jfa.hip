@@ -162,12 +162,16 @@ __global__ __launch_bounds__(kBlock) void k_jfa_pass(const uint32_t *__restrict_
     }
     const bool up = y - step >= 0, dn = y + step < H, lf = xl >= 0, rt = xr < W;
     const bool valid[9] = {up && lf, up, up && rt, lf, true, rt, dn && lf, dn, dn && rt};
+    // An out-of-bounds neighbour (read at a clamped coordinate) enters with
+    // 2^31 added to its d^2: every in-bounds candidate has d^2 < 2^31 (the
+    // far seed's is at most 2 * 32767^2), so it never wins, and no sum wraps.
     uint64_t best = ((uint64_t)seed_d2(cand[4], pix) << 32) | cand[4];
 #pragma unroll
     for (int j = 0; j < 9; ++j) {
         if (j == 4) continue;
-        const uint32_t sd = valid[j] ? cand[j] : kFarSeed;
-        const uint64_t key = ((uint64_t)seed_d2(sd, pix) << 32) | sd;
+        const short2v d = __builtin_bit_cast(short2v, cand[j]) - __builtin_bit_cast(short2v, pix);
+        const uint32_t d2 = (uint32_t)__builtin_amdgcn_sdot2(d, d, valid[j] ? 0 : INT32_MIN, false);
+        const uint64_t key = ((uint64_t)d2 << 32) | cand[j];
         best = key < best ? key : best;
     }
     if (!xin) return;
