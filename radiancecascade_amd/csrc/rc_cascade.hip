@@ -63,8 +63,8 @@ __device__ inline void ray_round(Ray &r, __amdgpu_buffer_rsrc_t sdf, int W, int 
     for (int j = 1; j < SPEC; ++j) p[j] = p[j - 1] + r.guess;
 #pragma unroll
     for (int j = 0; j < SPEC; ++j) {
-        const int ix = __float2int_rd(fmaf(p[j], r.dx, r.cx));  // floor, one instruction
-        const int iy = __float2int_rd(fmaf(p[j], r.dy, r.cy));
+        const int ix = floor_i32(fmaf(p[j], r.dx, r.cx));
+        const int iy = floor_i32(fmaf(p[j], r.dy, r.cy));
         ins[j] = (uint32_t)ix < (uint32_t)W && (uint32_t)iy < (uint32_t)H;
         q[j] = pixel_index(ix, iy, W);
         d[j] = sdf_load(sdf, ix, iy, sdf_tiles_x(W));
@@ -274,7 +274,7 @@ __global__ __launch_bounds__(kBlock) void k_level0(LevelArgs a, float *__restric
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const float2 dir = a.dirs[k];
-                const int jx = __float2int_rd(fmaf(t, dir.x, cx)), jy = __float2int_rd(fmaf(t, dir.y, cy));
+                const int jx = floor_i32(fmaf(t, dir.x, cx)), jy = floor_i32(fmaf(t, dir.y, cy));
                 if (!((uint32_t)jx < (uint32_t)a.W && (uint32_t)jy < (uint32_t)a.H)) {
                     code[k] = kRayExit;
                 } else if (sdf_load(sdf, jx, jy, tiles) < 0.5f) {

@@ -34,11 +34,31 @@ __device__ inline int pixel_index(int x, int y, int W) { return (int)__umul24((u
 
 // Tiled distance load (one byte, decoded).  For an off-image (x, y) the
 // offset may be anything; the range-checked load is then unused.
+// (m & a) | (~m & b) in one VALU instruction.
+__device__ inline uint32_t bfi(uint32_t m, uint32_t a, uint32_t b)
+{
+    uint32_t r;
+    asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(r) : "s"(m), "v"(a), "v"(b));
+    return r;
+}
+
+// The in-tile offset is two bitfield inserts: bits 0-3 = x & 15, bits 4-6 =
+// y & 7, bits 7+ = x >> 4 (the tile column); plus tile row * tile bytes.
 __device__ inline float sdf_load(__amdgpu_buffer_rsrc_t r, int x, int y, int tiles_x)
 {
     const uint32_t ux = (uint32_t)x, uy = (uint32_t)y;
-    const uint32_t off = __umul24(uy >> 3, (uint32_t)tiles_x << 7) + ((ux & ~15u) << 3) + ((uy & 7u) << 4) + (ux & 15u);
+    const uint32_t in = bfi(127u, bfi(15u, ux, uy << 4), ux << 3);
+    const uint32_t off = __umul24(uy >> 3, (uint32_t)tiles_x << 7) + in;
     return sdf_value(__builtin_amdgcn_raw_buffer_load_b8(r, (int)off, 0, 0));
+}
+
+// floor(v) as an int in one VALU instruction (v_cvt_flr_i32_f32; v is a
+// finite sample position here, far inside the int range).
+__device__ inline int floor_i32(float v)
+{
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(v));
+    return r;
 }
 
 // max(d, 1) as one v_med3 (no NaN canonicalisation: d is never NaN).  For
@@ -71,8 +91,8 @@ __device__ inline void march_rays(__amdgpu_buffer_rsrc_t sdf, int W, int H, cons
 #pragma unroll
         for (int j = 0; j < RPT; ++j) {
             // positions are finite, so the integer test equals 0 <= x < W, 0 <= y < H
-            const int ix = __float2int_rd(fmaf(t[j], dx, cx[j]));  // floor, one instruction
-            const int iy = __float2int_rd(fmaf(t[j], dy, cy[j]));
+            const int ix = floor_i32(fmaf(t[j], dx, cx[j]));
+            const int iy = floor_i32(fmaf(t[j], dy, cy[j]));
             const bool inside = (uint32_t)ix < (uint32_t)W && (uint32_t)iy < (uint32_t)H;
             if (alive[j] && !inside) code[j] = kRayExit;
             alive[j] = alive[j] && inside;
