@@ -128,13 +128,17 @@ __global__ __launch_bounds__(kBlock) void k_level(LevelArgs a)
     const int kk = threadIdx.x % KD, pl = threadIdx.x / KD;
     const int px = px0 + pl % TX, pyb = py0 + pl / TX, k = k0 + kk;
 
+    // 32-bit indices: a level holds fewer than 2^32 rays (W, H <= 16384), N
+    // is a power of two, and probe rows and columns are below 2^24
+    const int log2n = 31 - __builtin_clz(a.N);
     const int ulx = (px0 - 1) >> 1, uly = (py0 - 1) >> 1;
     if (UPPER) {
         const typename TT::T *up = (const typename TT::T *)a.upper;
         for (int i = threadIdx.x; i < UCX * UCY * KD; i += kBlock) {
             const int j = i % KD, sx = (i / KD) % UCX, sy = i / (KD * UCX);
             const int ux = min(max(ulx + sx, 0), a.UX - 1), uy = min(max(uly + sy, 0), a.UY - 1);
-            su[i] = TT::unpack(up[((size_t)uy * a.UX + ux) * a.N + k0 + j]);
+            const uint32_t ui = ((__umul24((uint32_t)uy, (uint32_t)a.UX) + (uint32_t)ux) << log2n) + (uint32_t)(k0 + j);
+            su[i] = TT::unpack(up[ui]);
         }
         __syncthreads();
     }
@@ -155,22 +159,24 @@ __global__ __launch_bounds__(kBlock) void k_level(LevelArgs a)
 #pragma unroll
         for (int j = 0; j < RPT; ++j) {
             const int py = min(pyb + j * TY, a.PY - 1);
-            hitq[j] = a.codes[((size_t)py * a.PX + min(px, a.PX - 1)) * a.N + k];
+            hitq[j] = a.codes[((__umul24((uint32_t)py, (uint32_t)a.PX) + (uint32_t)min(px, a.PX - 1)) << log2n) +
+                              (uint32_t)k];
         }
     } else {
         march_rays<RPT>(sdf_rsrc(a.sdf, a.W, a.H), a.W, a.H, cx, cy, dir.x, dir.y, a.t0, a.t1, active, hitq);
     }
 
-    const size_t plane = (size_t)a.W * a.H;
+    const uint32_t plane = __umul24((uint32_t)a.W, (uint32_t)a.H);
 #pragma unroll
     for (int j = 0; j < RPT; ++j) {
         const int py = pyb + j * TY;
         float r = 0.f, g = 0.f, b = 0.f, al = 0.f;
         if (active[j]) {
             if (hitq[j] >= 0) {
-                r = a.emissive[hitq[j]];
-                g = a.emissive[plane + hitq[j]];
-                b = a.emissive[2 * plane + hitq[j]];
+                const uint32_t q = (uint32_t)hitq[j];
+                r = a.emissive[q];
+                g = a.emissive[plane + q];
+                b = a.emissive[2 * plane + q];
             } else {
                 al = 1.0f;
                 if (UPPER) {
@@ -192,7 +198,8 @@ __global__ __launch_bounds__(kBlock) void k_level(LevelArgs a)
         al = quad_sum(al) * 0.25f;
         if (active[j] && (kk & 3) == 0) {
             typename TT::T *out = (typename TT::T *)a.out;
-            out[((size_t)py * a.PX + px) * (a.N >> 2) + (k >> 2)] = TT::pack(make_float4(r, g, b, al));
+            out[((__umul24((uint32_t)py, (uint32_t)a.PX) + (uint32_t)px) << (log2n - 2)) + (uint32_t)(k >> 2)] =
+                TT::pack(make_float4(r, g, b, al));
         }
     }
 }
@@ -224,7 +231,8 @@ __global__ __launch_bounds__(kBlock) void k_level0(LevelArgs a, float *__restric
     const int lb = xcd_remap_c(blockIdx.x, gridDim.x);
     const int px0 = (lb % tiles_x) * TX, py0 = (lb / tiles_x) * TY;
     const int ulx = (px0 - 2) >> 1, uly = (py0 - 2) >> 1;
-    const size_t plane = (size_t)a.W * a.H;
+    // 32-bit indices (W, H <= 16384: three planes hold fewer than 2^32 floats)
+    const uint32_t plane = __umul24((uint32_t)a.W, (uint32_t)a.H);
     const int slot = threadIdx.x;
     const int sx = slot % NX, sy = slot / NX;
     const int px = px0 - 1 + sx, py = py0 - 1 + sy;
@@ -232,7 +240,7 @@ __global__ __launch_bounds__(kBlock) void k_level0(LevelArgs a, float *__restric
     const __amdgpu_buffer_rsrc_t sdf = sdf_rsrc(a.sdf, a.W, a.H);
     const int tiles = sdf_tiles_x(a.W);
     const float cx = ((float)px + 0.5f) * 2.0f, cy = ((float)py + 0.5f) * 2.0f;
-    const int ix = __float2int_rd(cx), iy = __float2int_rd(cy);
+    const int ix = 2 * px + 1, iy = 2 * py + 1;  // floor of the probe centre (exact)
     const bool in0 = probe && (uint32_t)ix < (uint32_t)a.W && (uint32_t)iy < (uint32_t)a.H;
     // the first distance sample is issued before the upper level's loads, so
     // the march does not wait for them; they land in LDS after the march
@@ -256,7 +264,7 @@ __global__ __launch_bounds__(kBlock) void k_level0(LevelArgs a, float *__restric
             if (i < UCX * UCY * 4) {
                 const int j = i & 3, ux0 = (i >> 2) % UCX, uy0 = (i >> 2) / UCX;
                 const int ux = min(max(ulx + ux0, 0), a.UX - 1), uy = min(max(uly + uy0, 0), a.UY - 1);
-                ureg[n] = up[((size_t)uy * a.UX + ux) * 4 + j];
+                ureg[n] = up[((__umul24((uint32_t)uy, (uint32_t)a.UX) + (uint32_t)ux) << 2) + (uint32_t)j];
             }
         }
     }
@@ -289,9 +297,10 @@ __global__ __launch_bounds__(kBlock) void k_level0(LevelArgs a, float *__restric
         // one hit pixel for all 4 directions: its emission once (already
         // loaded when it is the first sample's pixel)
         const bool first = code[0] == q0;
-        const float e0 = first ? ep0 : a.emissive[code[0]];
-        const float e1 = first ? ep1 : a.emissive[plane + code[0]];
-        const float e2 = first ? ep2 : a.emissive[2 * plane + code[0]];
+        const uint32_t c0 = (uint32_t)code[0];
+        const float e0 = first ? ep0 : a.emissive[c0];
+        const float e1 = first ? ep1 : a.emissive[plane + c0];
+        const float e2 = first ? ep2 : a.emissive[2 * plane + c0];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             r[k] = e0;
@@ -306,9 +315,10 @@ __global__ __launch_bounds__(kBlock) void k_level0(LevelArgs a, float *__restric
             al[k] = code[k] >= 0 ? 0.f : 1.0f;
             if (code[k] >= 0) {
                 const bool first = code[k] == q0;
-                r[k] = first ? ep0 : a.emissive[code[k]];
-                g[k] = first ? ep1 : a.emissive[plane + code[k]];
-                b[k] = first ? ep2 : a.emissive[2 * plane + code[k]];
+                const uint32_t ck = (uint32_t)code[k];
+                r[k] = first ? ep0 : a.emissive[ck];
+                g[k] = first ? ep1 : a.emissive[plane + ck];
+                b[k] = first ? ep2 : a.emissive[2 * plane + ck];
             }
         }
     }
@@ -352,7 +362,7 @@ __global__ __launch_bounds__(kBlock) void k_level0(LevelArgs a, float *__restric
         const int sx0 = min(max(bx0, 0), a.PX - 1) - (px0 - 1), sx1 = min(max(bx0 + 1, 0), a.PX - 1) - (px0 - 1);
         const int sy0 = min(max(by0, 0), a.PY - 1) - (py0 - 1), sy1 = min(max(by0 + 1, 0), a.PY - 1) - (py0 - 1);
         const float3 v = bilerp4(fl[sy0 * NX + sx0], fl[sy0 * NX + sx1], fl[sy1 * NX + sx0], fl[sy1 * NX + sx1], x, y);
-        const size_t p = (size_t)y * a.W + x;
+        const uint32_t p = __umul24((uint32_t)y, (uint32_t)a.W) + (uint32_t)x;
         irr[p] = v.x;
         irr[plane + p] = v.y;
         irr[2 * plane + p] = v.z;
