@@ -671,22 +671,27 @@ __global__ __launch_bounds__(kBlock) void k_surr_mlp(SurrMlpArgs a, MlpDev m)
                 cf[j] = far_append(cf[j], a.far, a.FX, a.FY, min(qx, a.QX - 1), min(qy, a.QY - 1), 4 * ko + j);
         }
     };
-    float4 cf[4];
-    if (t_begin + idx < t_end) load_cones(t_begin + idx, cf);
-    for (int i = threadIdx.x; i < 4 * 64; i += kBlock) bias_s[i] = m.bias[i];
-    for (int i = threadIdx.x; i < kMlpIn; i += kBlock) {
-        mean_s[i] = m.in_mean[i];
-        istd_s[i] = m.in_istd[i];
-    }
-    __syncthreads();
-    // B operands from the bf16 weights in global memory (30 KB, held in every
-    // CU's L1 and the L2)
+    // B operands from the bf16 weights in global memory (32 KB, held in every
+    // CU's L1 and the L2): all four layers' fragments of this wave, loaded
+    // once (64 VGPRs) and issued first, so their latency, the first tile's
+    // cones' and the constants' overlap
     const uint16_t *wl[4] = {m.wt, m.wt + 64 * kMlpIn, m.wt + 64 * kMlpIn + 64 * kMlpHid,
                              m.wt + 64 * kMlpIn + 2 * 64 * kMlpHid};
-    // all four layers' B fragments of this wave, loaded once (60 VGPRs)
     const BFrag<kMlpIn> b0 = load_bfrag<kMlpIn>(wl[0], c0);
     const BFrag<kMlpHid> b1 = load_bfrag<kMlpHid>(wl[1], c0), b2 = load_bfrag<kMlpHid>(wl[2], c0),
                          b3 = load_bfrag<kMlpHid>(wl[3], c0);
+    static_assert(4 * 64 == kBlock && kMlpIn <= kBlock, "one bias (and at most one feature constant) per thread");
+    const float bias_v = m.bias[threadIdx.x];
+    const float mean_v = threadIdx.x < kMlpIn ? m.in_mean[threadIdx.x] : 0.f;
+    const float istd_v = threadIdx.x < kMlpIn ? m.in_istd[threadIdx.x] : 0.f;
+    float4 cf[4];
+    if (t_begin + idx < t_end) load_cones(t_begin + idx, cf);
+    bias_s[threadIdx.x] = bias_v;
+    if (threadIdx.x < kMlpIn) {
+        mean_s[threadIdx.x] = mean_v;
+        istd_s[threadIdx.x] = istd_v;
+    }
+    __syncthreads();
 
 #pragma unroll 1
     for (int tile = t_begin + idx; tile < t_end; tile += per_xcd) {
