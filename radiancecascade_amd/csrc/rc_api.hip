@@ -43,6 +43,7 @@ struct rc_ctx {
     int spec = 1;      // speculation depth of the split march (1, 2, 4)
     int march_m = 1;   // rays per lane, marched back to back (1, 2, 4, 8)
     int *codes = nullptr;
+    float4 *emis4 = nullptr;  // the frame's emission interleaved per pixel (plain path)
     size_t code_off[RC_MAX_LEVELS] = {};
     double *norm_ws = nullptr;  // error-norm partials
     float *srgb_table = nullptr;  // composite's 256 sRGB thresholds
@@ -149,6 +150,8 @@ static void release(rc_ctx *c)
     c->dirs = nullptr;
     if (c->codes) hipFree(c->codes);
     c->codes = nullptr;
+    if (c->emis4) hipFree(c->emis4);
+    c->emis4 = nullptr;
     if (c->norm_ws) hipFree(c->norm_ws);
     c->norm_ws = nullptr;
     if (c->srgb_table) hipFree(c->srgb_table);
@@ -194,6 +197,7 @@ static rc_status allocate(rc_ctx *c, uint32_t W, uint32_t H, uint32_t L)
     HIP_TRY(c, hipMalloc(&c->seed[0], n * sizeof(uint32_t)));
     HIP_TRY(c, hipMalloc(&c->seed[1], n * sizeof(uint32_t)));
     HIP_TRY(c, hipMalloc(&c->sdf, sdf_elems((int)W, (int)H)));
+    HIP_TRY(c, hipMalloc(&c->emis4, n * sizeof(float4)));
     const size_t tb = texel_bytes(c->storage);
     for (int i = 1; i < (int)L; ++i) {
         const size_t texels = (size_t)grid_dim(W, i) * grid_dim(H, i) * (num_dirs(i) / 4);
@@ -349,7 +353,7 @@ static rc_status run_frame(rc_ctx *c, const float *depth, const float *emissive,
     for (auto &u : c->ev_used) u = false;
 
     tm.begin(RC_STAGE_SEED);
-    HIP_TRY(c, launch_seed_init(depth, emissive, c->seed[0], c->W, c->H, s));
+    HIP_TRY(c, launch_seed_init(depth, emissive, c->seed[0], c->emis4, c->W, c->H, s));
     tm.end(RC_STAGE_SEED);
 
     // Jump flood: global passes down to the fused LDS tail (if any); without a
@@ -385,6 +389,7 @@ static rc_status run_frame(rc_ctx *c, const float *depth, const float *emissive,
     for (int i = c->L - 1; i >= 1; --i) {
         tm.begin(RC_STAGE_LEVEL0 + i);
         LevelArgs a = level_args(c, i, emissive);
+        a.emis4 = c->emis4;
         if (split) a.codes = c->codes + c->code_off[i];
         HIP_TRY(c, launch_level(a, c->storage, c->tile[i], c->rpt[i], s));
         tm.end(RC_STAGE_LEVEL0 + i);

@@ -174,9 +174,16 @@ __global__ __launch_bounds__(kBlock) void k_level(LevelArgs a)
         if (active[j]) {
             if (hitq[j] >= 0) {
                 const uint32_t q = (uint32_t)hitq[j];
-                r = a.emissive[q];
-                g = a.emissive[plane + q];
-                b = a.emissive[2 * plane + q];
+                if (a.emis4) {
+                    const float4 e = a.emis4[q];
+                    r = e.x;
+                    g = e.y;
+                    b = e.z;
+                } else {
+                    r = a.emissive[q];
+                    g = a.emissive[plane + q];
+                    b = a.emissive[2 * plane + q];
+                }
             } else {
                 al = 1.0f;
                 if (UPPER) {

@@ -18,7 +18,8 @@ int jfa_num_passes(int W, int H);
 // largest step fused into the LDS tail for a requested fuse limit (0 = none)
 int jfa_tail_step(int W, int H, int fuse);
 hipError_t launch_sdf_untile(const uint8_t *sdf, float *out, int W, int H, hipStream_t s);
-hipError_t launch_seed_init(const float *depth, const float *emissive, uint32_t *seed, int W, int H, hipStream_t s);
+hipError_t launch_seed_init(const float *depth, const float *emissive, uint32_t *seed, float4 *emis4, int W, int H,
+                            hipStream_t s);
 hipError_t launch_jfa_pass(const uint32_t *in, uint32_t *out, uint8_t *sdf, int W, int H, int step, bool final_pass,
                            hipStream_t s);
 hipError_t launch_jfa_tail(const uint32_t *in, uint32_t *seeds_out, uint8_t *sdf, int W, int H, int tail_step,
@@ -34,6 +35,7 @@ int tile_ty(int shape);
 struct LevelArgs {
     const uint8_t *sdf;      // distance codes, 16x8 tiles
     const float *emissive;   // [3][H][W]
+    const float4 *emis4;     // the same, interleaved (r, g, b, 0) per pixel, or nullptr (read the planes)
     const float2 *dirs;      // [N] unit directions of this level
     const void *upper;       // merged level i+1 (storage texels), nullptr at the top level
     void *out;               // merged level i (storage texels)
