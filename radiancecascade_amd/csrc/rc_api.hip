@@ -530,11 +530,11 @@ static rc_status run_frame_surrogate(rc_ctx *c, const float *depth, const float 
     // on several captured streams wait microseconds for each other across
     // hardware queues).  A profiled frame runs the stages one by one.
     if (!prof) {
-        // mip levels 3..6 beside the near distance field; top levels; near
-        // and far cones in one launch (the network kernel appends the far
-        // segment)
-        HIP_TRY(c, launch_mip_mid_near(c->mip, c->occ_bits, c->sdf, c->W, c->H, s));
-        HIP_TRY(c, launch_mip_top(c->mip, c->flag_count, s));
+        // mip levels 3..6; the top levels (one workgroup) beside the near
+        // distance field; near and far cones in one launch (the network
+        // kernel appends the far segment)
+        HIP_TRY(c, launch_mip_mid(c->mip, s));
+        HIP_TRY(c, launch_mip_top_near(c->mip, c->flag_count, c->occ_bits, c->sdf, c->W, c->H, s));
         HIP_TRY(c, launch_cone_pair(cone_args_near(c, false), cone_args_far(c), s));
     } else {
         tm.begin(RC_STAGE_NEAR_SDF);

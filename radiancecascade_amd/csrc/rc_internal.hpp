@@ -191,7 +191,8 @@ hipError_t launch_mip_top(const MipArgs &m, int *zero_me, hipStream_t s);    // 
 hipError_t launch_mip_unpack(const MipArgs &m, int level, float *emission, float *coverage, hipStream_t s);
 hipError_t launch_cone(const ConeArgs &a, hipStream_t s);
 hipError_t launch_cone_pair(const ConeArgs &near, const ConeArgs &far, hipStream_t s);   // one launch, both grids
-hipError_t launch_mip_mid_near(const MipArgs &m, const uint64_t *bits, uint8_t *sdf, int W, int H, hipStream_t s);
+hipError_t launch_mip_top_near(const MipArgs &m, int *zero_me, const uint64_t *bits, uint8_t *sdf, int W, int H,
+                               hipStream_t s);  // levels 7..top beside the near field
 hipError_t launch_surr_mlp(const SurrMlpArgs &a, const MlpDev &m, Storage st, hipStream_t s);
 hipError_t launch_surr_sample(const SurrSampleArgs &a, Storage st, hipStream_t s);
 hipError_t launch_tile_patch(const void *plain, void *surr, const float *tile_err, float threshold, int PX, int PY,

@@ -359,24 +359,60 @@ __global__ __launch_bounds__(kBlock) void k_mip_top(MipArgs m, int *zero_me)
 
 __global__ __launch_bounds__(kBlock) void k_mip_mid(MipArgs m, int tiles_x) { mip_mid_block(blockIdx.x, m, tiles_x); }
 
-// Mip levels 3..6 and the near distance field in one launch (both read only
-// the seed kernel's output): blocks [0, n_mid) are mip tiles, the rest
-// near-field tiles.
-__global__ __launch_bounds__(kBlock) void k_mip_mid_near(MipArgs m, int mid_tiles_x, int n_mid,
-                                                         const uint64_t *__restrict__ bits, int words,
-                                                         uint8_t *__restrict__ sdf, int W, int H, int near_tiles_x)
+// Levels 7..top in one workgroup with little LDS: level 7 from level 6 in
+// global memory (written by the previous launch), each later level from the
+// previous one in LDS when that fits in kTopCap cells, else from global
+// memory after a device-scope fence (images above 8192 px a side).
+constexpr int kTopCap = 512;
+
+__device__ inline void mip_top_small(const MipArgs &m, uint4 *__restrict__ buf_a, uint4 *__restrict__ buf_b)
 {
-    if ((int)blockIdx.x < n_mid) mip_mid_block(blockIdx.x, m, mid_tiles_x);
-    else near_sdf_block((int)blockIdx.x - n_mid, bits, words, sdf, W, H, near_tiles_x);
+    const uint4 zero = make_uint4(0u, 0u, 0u, 0u);
+    const uint4 *src = nullptr;  // level l - 1 in LDS, or nullptr: in global memory
+    for (int l = kMidLast + 1; l <= m.levels; ++l) {
+        const int w = m.w[l], h = m.h[l], cw = m.w[l - 1], ch = m.h[l - 1];
+        uint4 *dst = w * h <= kTopCap ? (src == buf_a ? buf_b : buf_a) : nullptr;
+        for (int i = threadIdx.x; i < w * h; i += kBlock) {
+            const int x = i % w, y = i / w;
+            uint4 v[4];
+#pragma unroll
+            for (int d = 0; d < 4; ++d) {
+                const int cx = 2 * x + (d & 1), cy = 2 * y + (d >> 1);
+                v[d] = (cx < cw && cy < ch) ? (src ? src[cy * cw + cx] : m.tex[mip_at(m, l - 1, cx, cy)]) : zero;
+            }
+            const uint4 t = combine4(v[0], v[1], v[2], v[3]);
+            m.tex[mip_at(m, l, x, y)] = t;
+            if (dst) dst[i] = t;
+        }
+        if (!dst) __threadfence();
+        __syncthreads();
+        src = dst;
+    }
 }
 
-hipError_t launch_mip_mid_near(const MipArgs &m, const uint64_t *bits, uint8_t *sdf, int W, int H, hipStream_t s)
+// Mip levels 7..top (block 0, a latency chain of small reductions) beside
+// the near distance field (blocks 1..): both read only what earlier launches
+// wrote.  Block 0 also clears the fallback counter of the frame (a kernel
+// store, not a memset node, so graph replays stay exact).
+__global__ __launch_bounds__(kBlock) void k_mip_top_near(MipArgs m, int *zero_me, const uint64_t *__restrict__ bits,
+                                                         int words, uint8_t *__restrict__ sdf, int W, int H,
+                                                         int near_tiles_x)
 {
-    const int mtx = m.levels >= 3 ? (m.w[2] + kMidTile - 1) / kMidTile : 0;
-    const int n_mid = m.levels >= 3 ? mtx * ((m.h[2] + kMidTile - 1) / kMidTile) : 0;
+    if (blockIdx.x == 0) {
+        __shared__ uint4 buf_a[kTopCap], buf_b[kTopCap];
+        if (zero_me && threadIdx.x == 0) *zero_me = 0;
+        mip_top_small(m, buf_a, buf_b);
+        return;
+    }
+    near_sdf_block((int)blockIdx.x - 1, bits, words, sdf, W, H, near_tiles_x);
+}
+
+hipError_t launch_mip_top_near(const MipArgs &m, int *zero_me, const uint64_t *bits, uint8_t *sdf, int W, int H,
+                               hipStream_t s)
+{
     const int ntx = (W + kNearT - 1) / kNearT, n_near = ntx * ((H + kNearT - 1) / kNearT);
-    hipLaunchKernelGGL(k_mip_mid_near, dim3(n_mid + n_near), dim3(kBlock), 0, s, m, mtx, n_mid, bits, (W + 63) / 64,
-                       sdf, W, H, ntx);
+    hipLaunchKernelGGL(k_mip_top_near, dim3(1 + n_near), dim3(kBlock), 0, s, m, zero_me, bits, (W + 63) / 64, sdf, W,
+                       H, ntx);
     return hipGetLastError();
 }
 
