@@ -362,17 +362,54 @@ __global__ __launch_bounds__(kBlock) void k_level0(LevelArgs a, float *__restric
     }
     __syncthreads();
 
-    for (int i = threadIdx.x; i < 4 * TX * TY; i += kBlock) {
-        const int x = 2 * px0 + (i & (2 * TX - 1)), y = 2 * py0 + i / (2 * TX);
-        if (x >= a.W || y >= a.H) continue;
-        const int bx0 = (x - 1) >> 1, by0 = (y - 1) >> 1;
-        const int sx0 = min(max(bx0, 0), a.PX - 1) - (px0 - 1), sx1 = min(max(bx0 + 1, 0), a.PX - 1) - (px0 - 1);
-        const int sy0 = min(max(by0, 0), a.PY - 1) - (py0 - 1), sy1 = min(max(by0 + 1, 0), a.PY - 1) - (py0 - 1);
-        const float3 v = bilerp4(fl[sy0 * NX + sx0], fl[sy0 * NX + sx1], fl[sy1 * NX + sx0], fl[sy1 * NX + sx1], x, y);
-        const uint32_t p = __umul24((uint32_t)y, (uint32_t)a.W) + (uint32_t)x;
-        irr[p] = v.x;
-        irr[plane + p] = v.y;
-        irr[2 * plane + p] = v.z;
+    // Pixels in 2x2 quads: pixel x = 2m interpolates probes m-1, m and x =
+    // 2m+1 probes m, m+1 (indices clamped to the grid; the same in y), so a
+    // quad reads 3x3 fluence texels for its 4 pixels, and with W even (and an
+    // 8-byte aligned output) each row of it is one 8-byte store per plane.
+    static_assert(TX * TY <= kBlock, "one quad per lane");
+    if ((int)threadIdx.x < TX * TY) {
+        const int m = px0 + (int)threadIdx.x % TX, n = py0 + (int)threadIdx.x / TX;
+        const int x = 2 * m, y = 2 * n;
+        if (x < a.W && y < a.H) {
+            int cx[3], cy[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                cx[d] = min(max(m - 1 + d, 0), a.PX - 1) - (px0 - 1);
+                cy[d] = min(max(n - 1 + d, 0), a.PY - 1) - (py0 - 1);
+            }
+            float4 f[3][3];
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) f[r][c] = fl[cy[r] * NX + cx[c]];
+            float3 v[2][2];
+#pragma unroll
+            for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+                for (int dx = 0; dx < 2; ++dx)
+                    v[dy][dx] = bilerp4(f[dy][dx], f[dy][dx + 1], f[dy + 1][dx], f[dy + 1][dx + 1], x + dx, y + dy);
+            const bool x1 = x + 1 < a.W;
+            const bool pair = x1 && (a.W & 1) == 0 && (reinterpret_cast<uintptr_t>(irr) & 7) == 0;
+#pragma unroll
+            for (int dy = 0; dy < 2; ++dy) {
+                if (y + dy >= a.H) break;
+                const uint32_t p = __umul24((uint32_t)(y + dy), (uint32_t)a.W) + (uint32_t)x;
+                if (pair) {
+                    *reinterpret_cast<float2 *>(irr + p) = make_float2(v[dy][0].x, v[dy][1].x);
+                    *reinterpret_cast<float2 *>(irr + plane + p) = make_float2(v[dy][0].y, v[dy][1].y);
+                    *reinterpret_cast<float2 *>(irr + 2 * plane + p) = make_float2(v[dy][0].z, v[dy][1].z);
+                } else {
+                    irr[p] = v[dy][0].x;
+                    irr[plane + p] = v[dy][0].y;
+                    irr[2 * plane + p] = v[dy][0].z;
+                    if (x1) {
+                        irr[p + 1] = v[dy][1].x;
+                        irr[plane + p + 1] = v[dy][1].y;
+                        irr[2 * plane + p + 1] = v[dy][1].z;
+                    }
+                }
+            }
+        }
     }
 }
 
