@@ -131,7 +131,7 @@ __host__ __device__ inline size_t mip_at(const MipArgs &m, int l, int x, int y)
 constexpr int kConeMaxSteps = 64;
 constexpr double kConeOccGain = 3.0;  // occluder exponent over the emitter one
 struct ConeStep {
-    float t, re, ro, inv_cell;  // distance, emitter / occluder exponents, 2^-m
+    float t_cell, re, ro, inv_cell;  // distance in cells (t 2^-m), emitter / occluder exponents, 2^-m
     int w, h;                // size of the step's mip level in cells
     uint32_t off, wp;        // first (border) texel of the level, padded row length w+2
 };

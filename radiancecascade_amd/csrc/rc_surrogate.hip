@@ -478,7 +478,7 @@ void cone_schedule(int W, int H, const MipArgs &mip, ConeSchedule &near, ConeSch
         ConeSchedule &s = t < (double)kConeSplit ? near : far;
         ConeStep &c = s.s[s.n++];
         const double r = fmin(step, (double)(1 << m)) / (double)(1 << m);
-        c.t = (float)t;
+        c.t_cell = (float)(t / (double)(1 << m));
         c.re = (float)r;
         c.ro = (float)(kConeOccGain * r);
         c.inv_cell = 1.0f / (float)(1 << m);
@@ -538,12 +538,16 @@ __device__ inline void cone_block(const ConeArgs &a, int lb, int seg_x)
     const int nsteps = a.sch->n;
     for (int n = 0; n < nsteps; ++n) {
         const ConeStep st = a.sch->s[n];
-        const float fx = fmaf(st.t, d.x, cx) * st.inv_cell - 0.5f, fy = fmaf(st.t, d.y, cy) * st.inv_cell - 0.5f;
-        const float flx = floorf(fx), fly = floorf(fy);
-        const int ix = (int)flx, iy = (int)fly;
+        // sample position in cells of level m, less the half cell of the
+        // bilinear taps: (t d + c) 2^-m - 1/2 as two FMAs per axis
+        const float fx = fmaf(st.t_cell, d.x, fmaf(cx, st.inv_cell, -0.5f));
+        const float fy = fmaf(st.t_cell, d.y, fmaf(cy, st.inv_cell, -0.5f));
+        // cell index and fraction in one instruction each (v_cvt_flr_i32_f32,
+        // v_fract_f32; the fraction is below 1 by construction)
+        const int ix = floor_i32(fx), iy = floor_i32(fy);
         // the footprint has left the image: a straight ray does not come back
         if (!(ix >= -1 && ix < st.w && iy >= -1 && iy < st.h)) break;
-        const float ax = fx - flx, ay = fy - fly;
+        const float ax = __builtin_amdgcn_fractf(fx), ay = __builtin_amdgcn_fractf(fy);
         const uint32_t o00 = (st.off + __umul24((uint32_t)(iy + 1), st.wp) + (uint32_t)(ix + 1)) << 4;
         const uint32_t orow = st.wp << 4;
         const uint4 v[4] = {

@@ -509,7 +509,8 @@ def host_buffer(shape, dtype=np.float32) -> np.ndarray:
     ptr = ctypes.c_void_p()
     check(lib.rc_host_alloc(max(n, 1), ctypes.byref(ptr)))
     raw = (ctypes.c_ubyte * max(n, 1)).from_address(ptr.value)
-    weakref.finalize(raw, lib.rc_host_free, ptr.value)
+    fin = weakref.finalize(raw, lib.rc_host_free, ptr.value)
+    fin.atexit = False  # at exit the process's pages go with it; the runtime may be gone
     arr = np.frombuffer(raw, dtype=dtype, count=int(np.prod(shape))).reshape(shape)
     return arr
 
