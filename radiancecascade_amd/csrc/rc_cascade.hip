@@ -26,13 +26,6 @@
 
 namespace rc {
 
-__device__ inline int xcd_remap_c(int b, int nb)
-{
-    const int q = nb >> 3, r = nb & 7;
-    const int xcd = b & 7, idx = b >> 3;
-    return xcd * q + min(xcd, r) + idx;
-}
-
 // One ray with SPEC-deep speculation.  After a sample with distance d the
 // next sample is exactly t + max(d, 1); a ray grazing a surface keeps taking
 // the same step, so each round also loads the SPEC-1 samples that follow if
@@ -121,7 +114,7 @@ __global__ __launch_bounds__(kBlock) void k_level(LevelArgs a)
 
     const int tiles_x = (a.PX + TX - 1) / TX;
     const int slices = a.N / KD;
-    const int lb = xcd_remap_c(blockIdx.x, gridDim.x);
+    const int lb = xcd_band(blockIdx.x, gridDim.x);
     const int slice = lb % slices, tile = lb / slices;
     const int px0 = (tile % tiles_x) * TX, py0 = (tile / tiles_x) * TYR, k0 = slice * KD;
 
@@ -235,7 +228,7 @@ __global__ __launch_bounds__(kBlock) void k_level0(LevelArgs a, float *__restric
     __shared__ float4 fl[NX * NY];
 
     const int tiles_x = (a.PX + TX - 1) / TX;
-    const int lb = xcd_remap_c(blockIdx.x, gridDim.x);
+    const int lb = xcd_band(blockIdx.x, gridDim.x);
     const int px0 = (lb % tiles_x) * TX, py0 = (lb / tiles_x) * TY;
     const int ulx = (px0 - 2) >> 1, uly = (py0 - 2) >> 1;
     // 32-bit indices (W, H <= 16384: three planes hold fewer than 2^32 floats)

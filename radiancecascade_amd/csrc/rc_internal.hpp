@@ -32,6 +32,16 @@ int tile_kd(int shape);
 int tile_tx(int shape);
 int tile_ty(int shape);
 
+// Workgroups are handed to the 8 XCDs round-robin (block b to XCD b % 8);
+// map b to a logical block so that each XCD gets one contiguous run of the
+// nb logical blocks (a band of tiles sharing its L2).  A bijection on [0, nb).
+__device__ inline int xcd_band(int b, int nb)
+{
+    const int q = nb >> 3, r = nb & 7;
+    const int xcd = b & 7, idx = b >> 3;
+    return xcd * q + min(xcd, r) + idx;
+}
+
 struct LevelArgs {
     const uint8_t *sdf;      // distance codes, 16x8 tiles
     const float *emissive;   // [3][H][W]

@@ -26,15 +26,6 @@ namespace rc {
 
 typedef short short2v __attribute__((ext_vector_type(2)));
 
-// Workgroups are handed to the 8 XCDs round-robin; give each XCD one
-// contiguous run of logical block ids so neighbouring tiles share its L2.
-__device__ inline int xcd_remap(int b, int nb)
-{
-    const int q = nb >> 3, r = nb & 7;
-    const int xcd = b & 7, idx = b >> 3;
-    return xcd * q + min(xcd, r) + idx;
-}
-
 // Inside the kernels "no seed" is a fake seed at (32767, 32767).  For every
 // pixel of an image up to 16384 px a side its d^2 >= 2 * 16384^2, above any
 // real candidate's (<= 2 * 16383^2), and every d^2 fits in int32, so the
@@ -115,7 +106,7 @@ __global__ __launch_bounds__(kBlock) void k_seed_init(const float *__restrict__ 
                                                       uint32_t *__restrict__ seed, float4 *__restrict__ emis4, int W,
                                                       int H, int bx_count)
 {
-    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int b = xcd_band(blockIdx.x, gridDim.x);
     const int x = (b % bx_count) * 64 + (threadIdx.x & 63);
     const int y0 = (b / bx_count) * kSeedRows + (threadIdx.x >> 6);
     if (x >= W) return;
@@ -146,7 +137,7 @@ template <bool FINAL>
 __global__ __launch_bounds__(kBlock) void k_jfa_pass(const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
                                                      uint8_t *__restrict__ sdf, int W, int H, int step, int bx_count)
 {
-    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int b = xcd_band(blockIdx.x, gridDim.x);
     const int x = (b % bx_count) * 64 + (threadIdx.x & 63);
     const int y = __builtin_amdgcn_readfirstlane((b / bx_count) * 4 + (threadIdx.x >> 6));
     if (y >= H) return;
@@ -212,7 +203,7 @@ __global__ __launch_bounds__(kBlock) void k_jfa_tail(const uint32_t *__restrict_
     static_assert(S >= 1 && R <= 64, "one column per lane");
     __shared__ uint32_t buf[2][R * R];
 
-    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int b = xcd_band(blockIdx.x, gridDim.x);
     const int x0 = (b % bx_count) * T - HALO;
     const int y0 = (b / bx_count) * T - HALO;
     const int cx = threadIdx.x & 63, cy = threadIdx.x >> 6;

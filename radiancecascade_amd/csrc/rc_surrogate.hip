@@ -110,7 +110,10 @@ __global__ __launch_bounds__(kBlock) void k_seed_surr(const float *__restrict__ 
 {
     __shared__ float4 pe[kSurrSeedRows][64];
     __shared__ uint4 c1[kSurrSeedRows / 2][32];
-    const int bx = blockIdx.x % bx_count, by = blockIdx.x / bx_count;
+    // row bands per XCD: the depth rows above and below a block are mostly
+    // read by the same XCD, from its L2
+    const int b = xcd_band(blockIdx.x, gridDim.x);
+    const int bx = b % bx_count, by = b / bx_count;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int x = bx * 64 + lane;
     const size_t plane = (size_t)W * H;
@@ -506,16 +509,6 @@ __device__ inline float4 far_append(float4 c, const float4 *__restrict__ far, in
     return make_float4(fmaf(c.w, f.x, c.x), fmaf(c.w, f.y, c.y), fmaf(c.w, f.z, c.z), c.w * ft.x);
 }
 
-// Workgroups are dealt to the 8 XCDs round-robin; give each XCD a contiguous
-// run of logical blocks (a band of probe rows), so the fine mip levels it
-// reads stay in its own L2.
-__device__ inline int xcd_chunk(int b, int nb)
-{
-    const int q = nb >> 3, r = nb & 7;
-    const int xcd = b & 7, idx = b >> 3;
-    return xcd * q + min(xcd, r) + idx;
-}
-
 // A workgroup is 64 consecutive level-2 probes of one row x 4 directions, a
 // wave per direction: the lanes of a wave sample points 8 px apart at the same
 // distance, so from mip level 3 up they share cells (and cache lines).  The 4
@@ -598,7 +591,7 @@ __device__ inline void cone_block(const ConeArgs &a, int lb, int seg_x)
 
 __global__ __launch_bounds__(kBlock) void k_cone(ConeArgs a, int seg_x)
 {
-    cone_block(a, xcd_chunk(blockIdx.x, gridDim.x), seg_x);
+    cone_block(a, xcd_band(blockIdx.x, gridDim.x), seg_x);
 }
 
 // Far and near cones in one launch: blocks [0, nb_far) (a multiple of 8, so
@@ -609,10 +602,10 @@ __global__ __launch_bounds__(kBlock) void k_cone_pair(ConeArgs near, int seg_nea
 {
     const int b = blockIdx.x;
     if (b < nb_far) {
-        const int lb = xcd_chunk(b, nb_far);
+        const int lb = xcd_band(b, nb_far);
         if (lb < nb_far_used) cone_block(far, lb, seg_far);
     } else {
-        cone_block(near, xcd_chunk(b - nb_far, nb_near), seg_near);
+        cone_block(near, xcd_band(b - nb_far, nb_near), seg_near);
     }
 }
 
