@@ -360,7 +360,11 @@ __global__ __launch_bounds__(kBlock) void k_mip_top(MipArgs m, int *zero_me)
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_mip_mid(MipArgs m, int tiles_x) { mip_mid_block(blockIdx.x, m, tiles_x); }
+// tiles in row bands per XCD, like the seed blocks that wrote their level 2
+__global__ __launch_bounds__(kBlock) void k_mip_mid(MipArgs m, int tiles_x)
+{
+    mip_mid_block(xcd_band(blockIdx.x, gridDim.x), m, tiles_x);
+}
 
 // Levels 7..top in one workgroup with little LDS: level 7 from level 6 in
 // global memory (written by the previous launch), each later level from the
@@ -401,13 +405,15 @@ __global__ __launch_bounds__(kBlock) void k_mip_top_near(MipArgs m, int *zero_me
                                                          int words, uint8_t *__restrict__ sdf, int W, int H,
                                                          int near_tiles_x)
 {
-    if (blockIdx.x == 0) {
+    // row bands per XCD; hardware block 0 is logical block 0, the top levels
+    const int lb = xcd_band(blockIdx.x, gridDim.x);
+    if (lb == 0) {
         __shared__ uint4 buf_a[kTopCap], buf_b[kTopCap];
         if (zero_me && threadIdx.x == 0) *zero_me = 0;
         mip_top_small(m, buf_a, buf_b);
         return;
     }
-    near_sdf_block((int)blockIdx.x - 1, bits, words, sdf, W, H, near_tiles_x);
+    near_sdf_block(lb - 1, bits, words, sdf, W, H, near_tiles_x);
 }
 
 hipError_t launch_mip_top_near(const MipArgs &m, int *zero_me, const uint64_t *bits, uint8_t *sdf, int W, int H,
