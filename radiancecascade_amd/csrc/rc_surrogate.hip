@@ -523,8 +523,10 @@ __device__ inline float4 far_append(float4 c, const float4 *__restrict__ far, in
 __device__ inline void cone_block(const ConeArgs &a, int lb, int seg_x)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int kgrp = lb / (seg_x * a.QY), rest = lb % (seg_x * a.QY);
-    const int qy = rest / seg_x, seg = rest % seg_x;
+    // probe rows outermost: an XCD's band of blocks is a band of rows (all
+    // four direction groups), so the fine mip levels it reads stay in its L2
+    const int qy = lb / (seg_x * 4), rest = lb % (seg_x * 4);
+    const int kgrp = rest / seg_x, seg = rest % seg_x;
     const int k = kgrp * 4 + wave;
     const int qx = seg * 64 + lane;
     if (qx >= a.QX) return;
