@@ -6,12 +6,15 @@
 // all; its only pass is the forward raster at src/renderer.rs:559-613).  The
 // frame is
 //   k_seed_surr                 occupancy bits + mip levels 1, 2
-//   k_near_sdf                  exact distance field up to 10 px
-//   k_mip_mid / k_mip_top       mip levels 3..top
-//   k_cone                      16 cones per level-2 probe
-//   k_surr_mlp                  the network's merged level 2
+//   k_mip_mid                   mip levels 3..6
+//   k_mip_top_near              mip levels 7..top (one workgroup) beside the
+//                               exact distance field up to 10 px
+//   k_cone_pair                 16 cones per level-3 probe (far segment) and
+//                               per level-2 probe (near segment)
+//   k_surr_mlp                  far segment appended; the network's merged level 2
 //   k_level (rc_cascade.hip)    level 1, marched on the near distance field
 //   k_level0 (rc_cascade.hip)   level 0 + gather
+// (a profiled frame runs k_near_sdf, k_mip_top and k_cone stage by stage).
 // Cone tracing through a prefiltered pyramid is Crassin et al.'s voxel cone
 // tracing (2011) in screen space; here it is the MLP's input feature, not the
 // answer.
