@@ -43,15 +43,6 @@ __device__ inline uint32_t seed_d2(uint32_t s, uint32_t pix)
     return (uint32_t)__builtin_amdgcn_sdot2(d, d, 0, false);
 }
 
-// Keep (bd2, bs) = lexicographic minimum of (d^2, seed).
-__device__ inline void consider(uint32_t s, uint32_t pix, uint32_t &bd2, uint32_t &bs)
-{
-    const uint32_t d2 = seed_d2(s, pix);
-    const bool better = d2 < bd2 || (d2 == bd2 && s < bs);
-    bd2 = better ? d2 : bd2;
-    bs = better ? s : bs;
-}
-
 __device__ inline uint8_t seed_sdf_code(uint32_t bs, uint32_t bd2)
 {
     return (uint8_t)(bs == kFarSeed ? kSdfFar : sdf_code_of_distance(isqrt_u32(bd2)));
@@ -186,48 +177,55 @@ __global__ __launch_bounds__(kBlock) void k_jfa_pass(const uint32_t *__restrict_
     }
 }
 
-// Fused passes S, S/2, ..., 1 over a 32x32 output tile, loaded with a halo of
-// 2S-1 pixels (R <= 62 cells a side); each pass shrinks the valid region by
-// its step.  Threads are laid out 64 x 4: lane = column, so every LDS access
-// of a wave is one row of consecutive words.  Cells outside the image stay the
-// far seed in every pass, which is exactly "out-of-bounds neighbours are
-// skipped".
+// Fused passes S, S/2, ..., 1 over a T x T output tile, loaded with a halo of
+// 2S-1 pixels: T = 64 - 2 (2S - 1), so the loaded region is 64 x 64 cells and
+// every row of it fills a wave (T = 62, 58, 50, 34 for S = 1, 2, 4, 8).  Each
+// pass shrinks the valid region by its step.  A workgroup is 16 waves (there
+// are few tiles, and the LDS passes need the waves to hide their latency),
+// laid out 64 x 16: lane = column, so every LDS access of a wave is one row of
+// consecutive words.  Cells outside the image stay the far seed in every pass, which is
+// exactly "out-of-bounds neighbours are skipped"; the minimum is over the
+// 64-bit keys (d^2, seed) as in the global pass.
+template <int S> constexpr int jfa_tail_tile() { return 64 - 2 * (2 * S - 1); }
+
+constexpr int kTailBlock = 1024, kTailRowsStep = kTailBlock / 64;
+
 template <int S>
-__global__ __launch_bounds__(kBlock) void k_jfa_tail(const uint32_t *__restrict__ in,
+__global__ __launch_bounds__(kTailBlock) void k_jfa_tail(const uint32_t *__restrict__ in,
                                                      uint32_t *__restrict__ seeds_out, uint8_t *__restrict__ sdf,
                                                      int W, int H, int bx_count)
 {
-    constexpr int T = 32;
+    constexpr int T = jfa_tail_tile<S>();
     constexpr int HALO = 2 * S - 1;
     constexpr int R = T + 2 * HALO;
-    static_assert(S >= 1 && R <= 64, "one column per lane");
+    static_assert(S >= 1 && R == 64, "one column per lane");
     __shared__ uint32_t buf[2][R * R];
 
     const int b = xcd_band(blockIdx.x, gridDim.x);
     const int x0 = (b % bx_count) * T - HALO;
     const int y0 = (b / bx_count) * T - HALO;
     const int cx = threadIdx.x & 63, cy = threadIdx.x >> 6;
+    constexpr int RS = kTailRowsStep;
     {
-        constexpr int ROWS = (R + 3) / 4;
+        constexpr int ROWS = R / RS;
         const int gx = x0 + cx, qx = min(max(gx, 0), W - 1);
         uint32_t v[ROWS];
 #pragma unroll
         for (int j = 0; j < ROWS; ++j) {
-            const int gy = y0 + cy + 4 * j, qy = min(max(gy, 0), H - 1);
+            const int gy = y0 + cy + RS * j, qy = min(max(gy, 0), H - 1);
             v[j] = in[(size_t)qy * W + qx];
         }
 #pragma unroll
         for (int j = 0; j < ROWS; ++j) {
-            const int ry = cy + 4 * j, gy = y0 + ry;
-            if (cx < R && ry < R)
-                buf[0][ry * R + cx] = (gx >= 0 && gy >= 0 && gx < W && gy < H) ? v[j] : kFarSeed;
+            const int ry = cy + RS * j, gy = y0 + ry;
+            buf[0][ry * R + cx] = (gx >= 0 && gy >= 0 && gx < W && gy < H) ? v[j] : kFarSeed;
         }
     }
     __syncthreads();
 
     int cur = 0;
     int off = 0;
-    uint32_t last_d2[(T + 3) / 4] = {};
+    uint32_t last_d2[(T + RS - 1) / RS] = {};
 #pragma unroll
     for (int s = S; s >= 1; s >>= 1) {
         off += s;
@@ -235,23 +233,27 @@ __global__ __launch_bounds__(kBlock) void k_jfa_tail(const uint32_t *__restrict_
         const int gx = x0 + rx;
         const bool xin = rx < R - off && gx >= 0 && gx < W;
 #pragma unroll
-        for (int j = 0; j < (R - 2 * off + 3) / 4; ++j) {
-            const int ry = off + cy + 4 * j;
+        for (int j = 0; j < (R - 2 * off + RS - 1) / RS; ++j) {
+            const int ry = off + cy + RS * j;
             const int gy = y0 + ry;
             const uint32_t *src = buf[cur];
             if (rx < R - off && ry < R - off) {
                 uint32_t bs = kFarSeed, bd2 = 0xFFFFFFFFu;
                 if (xin && gy >= 0 && gy < H) {
                     const uint32_t pix = ((uint32_t)gy << 16) | (uint32_t)gx;
-                    bs = src[ry * R + rx];
-                    bd2 = seed_d2(bs, pix);
+                    const uint32_t c0 = src[ry * R + rx];
+                    uint64_t best = ((uint64_t)seed_d2(c0, pix) << 32) | c0;
 #pragma unroll
                     for (int dy = -1; dy <= 1; ++dy)
 #pragma unroll
                         for (int dx = -1; dx <= 1; ++dx) {
                             if (dx == 0 && dy == 0) continue;
-                            consider(src[(ry + dy * s) * R + rx + dx * s], pix, bd2, bs);
+                            const uint32_t cs = src[(ry + dy * s) * R + rx + dx * s];
+                            const uint64_t key = ((uint64_t)seed_d2(cs, pix) << 32) | cs;
+                            best = key < best ? key : best;
                         }
+                    bs = (uint32_t)best;
+                    bd2 = (uint32_t)(best >> 32);
                 }
                 buf[cur ^ 1][ry * R + rx] = bs;
                 if (s == 1) last_d2[j] = bd2;
@@ -264,8 +266,8 @@ __global__ __launch_bounds__(kBlock) void k_jfa_tail(const uint32_t *__restrict_
     if (cx < T) {
         const int gx = x0 + HALO + cx;
 #pragma unroll
-        for (int j = 0; j < (T + 3) / 4; ++j) {
-            const int ty = cy + 4 * j;
+        for (int j = 0; j < (T + RS - 1) / RS; ++j) {
+            const int ty = cy + RS * j;
             const int gy = y0 + HALO + ty;
             if (ty >= T || gx >= W || gy >= H) continue;
             const uint32_t sd = buf[cur][(HALO + ty) * R + HALO + cx];
@@ -320,21 +322,27 @@ hipError_t launch_jfa_pass(const uint32_t *in, uint32_t *out, uint8_t *sdf, int 
     return hipGetLastError();
 }
 
+template <int S> static void launch_tail(const uint32_t *in, uint32_t *seeds_out, uint8_t *sdf, int W, int H,
+                                         hipStream_t s)
+{
+    constexpr int T = jfa_tail_tile<S>();
+    const int bxc = (W + T - 1) / T, byc = (H + T - 1) / T;
+    hipLaunchKernelGGL(k_jfa_tail<S>, dim3(bxc * byc), dim3(kTailBlock), 0, s, in, seeds_out, sdf, W, H, bxc);
+}
+
 hipError_t launch_jfa_tail(const uint32_t *in, uint32_t *seeds_out, uint8_t *sdf, int W, int H, int tail_step,
                            hipStream_t s)
 {
-    const int bxc = (W + 31) / 32, byc = (H + 31) / 32;
-    const dim3 grid(bxc * byc);
     switch (tail_step) {
     case 0: {
         const int n = W * H;
         hipLaunchKernelGGL(k_jfa_none, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, in, seeds_out, sdf, W, n);
         break;
     }
-    case 1: hipLaunchKernelGGL(k_jfa_tail<1>, grid, dim3(kBlock), 0, s, in, seeds_out, sdf, W, H, bxc); break;
-    case 2: hipLaunchKernelGGL(k_jfa_tail<2>, grid, dim3(kBlock), 0, s, in, seeds_out, sdf, W, H, bxc); break;
-    case 4: hipLaunchKernelGGL(k_jfa_tail<4>, grid, dim3(kBlock), 0, s, in, seeds_out, sdf, W, H, bxc); break;
-    case 8: hipLaunchKernelGGL(k_jfa_tail<8>, grid, dim3(kBlock), 0, s, in, seeds_out, sdf, W, H, bxc); break;
+    case 1: launch_tail<1>(in, seeds_out, sdf, W, H, s); break;
+    case 2: launch_tail<2>(in, seeds_out, sdf, W, H, s); break;
+    case 4: launch_tail<4>(in, seeds_out, sdf, W, H, s); break;
+    case 8: launch_tail<8>(in, seeds_out, sdf, W, H, s); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
