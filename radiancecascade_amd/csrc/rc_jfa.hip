@@ -183,9 +183,9 @@ __global__ __launch_bounds__(kBlock) void k_jfa_pass(const uint32_t *__restrict_
 // pass shrinks the valid region by its step.  A workgroup is 16 waves (there
 // are few tiles, and the LDS passes need the waves to hide their latency),
 // laid out 64 x 16: lane = column, so every LDS access of a wave is one row of
-// consecutive words.  Cells outside the image stay the far seed in every pass, which is
-// exactly "out-of-bounds neighbours are skipped"; the minimum is over the
-// 64-bit keys (d^2, seed) as in the global pass.
+// consecutive words.  Cells outside the image stay the far seed in every
+// pass, which is exactly "out-of-bounds neighbours are skipped"; the minimum
+// is over the 64-bit keys (d^2, seed) as in the global pass.
 template <int S> constexpr int jfa_tail_tile() { return 64 - 2 * (2 * S - 1); }
 
 constexpr int kTailBlock = 1024, kTailRowsStep = kTailBlock / 64;
