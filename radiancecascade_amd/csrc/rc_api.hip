@@ -1093,6 +1093,7 @@ rc_status rc_autotune(rc_ctx *c, const rc_gbuffer *g, float *irr, uint32_t iters
         float best = 1e30f;
         int best_shape = c->tile[i], best_rpt = c->rpt[i];
         LevelArgs a = level_args(c, i, g->emissive);
+        a.emis4 = c->emis4;  // as the frame runs them (filled by the frames above)
         if (c->march_split) a.codes = c->codes + c->code_off[i];
         for (int shape = 0; shape < TILE_COUNT; ++shape) {
             for (int rpt : {1, 2, 4}) {
