@@ -157,6 +157,24 @@ def test_deterministic(rc, oracle):
         assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
 
 
+def test_output_at_an_odd_float_offset(rc, oracle):
+    """The gather writes row pairs as 8-byte stores only when the output is
+    8-byte aligned; an irradiance buffer starting 4 bytes in must give the
+    same bits through the per-pixel stores."""
+    W, H, L = 160, 120, 5
+    _say(f"odd output offset {W}x{H}")
+    depth, emis = _device_gbuffer(rc, W, H, 9, 8, 2)
+    flat = torch.zeros(3 * H * W + 1, dtype=torch.float32, device="cuda:0")
+    odd = flat[1:].view(3, H, W)
+    ref = torch.empty((3, H, W), dtype=torch.float32, device="cuda:0")
+    with rc.RadianceCascadeStage(W, H, levels=L) as st:
+        st.run_device(depth, emis, ref)
+        st.run_device(depth, emis, odd)
+        torch.cuda.synchronize()
+    assert odd.data_ptr() % 8 == 4 and flat[0].item() == 0.0
+    assert torch.equal(odd, ref)
+
+
 def test_host_staging_path(rc, oracle):
     """RenderStage-order path: update (upload) then render (run + readback)."""
     W, H, L = 160, 120, 5
