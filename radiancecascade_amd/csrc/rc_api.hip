@@ -530,10 +530,9 @@ static rc_status run_frame_surrogate(rc_ctx *c, const float *depth, const float 
     // on several captured streams wait microseconds for each other across
     // hardware queues).  A profiled frame runs the stages one by one.
     if (!prof) {
-        // mip levels 3..6; the top levels (one workgroup) beside the near
-        // distance field; near and far cones in one launch (the network
-        // kernel appends the far segment)
-        HIP_TRY(c, launch_mip_mid(c->mip, s));
+        // the top mip levels (one workgroup) beside the near distance field;
+        // near and far cones in one launch (the network kernel appends the
+        // far segment)
         HIP_TRY(c, launch_mip_top_near(c->mip, c->flag_count, c->occ_bits, c->sdf, c->W, c->H, s));
         HIP_TRY(c, launch_cone_pair(cone_args_near(c, false), cone_args_far(c), s));
     } else {
@@ -541,7 +540,7 @@ static rc_status run_frame_surrogate(rc_ctx *c, const float *depth, const float 
         HIP_TRY(c, launch_near_sdf(c->occ_bits, c->sdf, c->W, c->H, s));
         tm.end(RC_STAGE_NEAR_SDF);
         tm.begin(RC_STAGE_MIP);
-        HIP_TRY(c, launch_mip_rest(c->mip, c->flag_count, s));
+        HIP_TRY(c, launch_mip_top(c->mip, c->flag_count, s));
         tm.end(RC_STAGE_MIP);
         tm.begin(RC_STAGE_CONE);
         HIP_TRY(c, launch_cone(cone_args_far(c), s));
@@ -695,7 +694,7 @@ static rc_status gen_samples(rc_ctx *c, const float *depth, const float *emissiv
     st = run_frame(c, depth, emissive, c->scratch_irr, s, true);
     if (st != RC_OK) return st;
     HIP_TRY(c, launch_seed_surr(depth, emissive, c->occ_bits, c->mip, c->W, c->H, s));
-    HIP_TRY(c, launch_mip_rest(c->mip, c->flag_count, s));
+    HIP_TRY(c, launch_mip_top(c->mip, c->flag_count, s));
     st = launch_cones(c, s);
     if (st != RC_OK) return st;
     SurrSampleArgs a{};

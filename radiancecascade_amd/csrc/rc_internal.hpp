@@ -193,10 +193,8 @@ size_t mip_cells(const MipArgs &m);
 void cone_schedule(int W, int H, const MipArgs &m, ConeSchedule &near, ConeSchedule &far);
 int cone_top_level(int W, int H);
 hipError_t launch_seed_surr(const float *depth, const float *emissive, uint64_t *bits, const MipArgs &m, int W, int H,
-                            hipStream_t s);
+                            hipStream_t s);  // occupancy bits + mip levels 1..6
 hipError_t launch_near_sdf(const uint64_t *bits, uint8_t *sdf, int W, int H, hipStream_t s);
-hipError_t launch_mip_rest(const MipArgs &m, int *zero_me, hipStream_t s);   // mid + top
-hipError_t launch_mip_mid(const MipArgs &m, hipStream_t s);                  // levels 3..6
 hipError_t launch_mip_top(const MipArgs &m, int *zero_me, hipStream_t s);    // levels 7..top
 hipError_t launch_mip_unpack(const MipArgs &m, int level, float *emission, float *coverage, hipStream_t s);
 hipError_t launch_cone(const ConeArgs &a, hipStream_t s);

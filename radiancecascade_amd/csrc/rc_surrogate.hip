@@ -5,8 +5,7 @@
 // None of this exists in the reference (it has no global illumination at
 // all; its only pass is the forward raster at src/renderer.rs:559-613).  The
 // frame is
-//   k_seed_surr                 occupancy bits + mip levels 1, 2
-//   k_mip_mid                   mip levels 3..6
+//   k_seed_surr                 occupancy bits + mip levels 1..6
 //   k_mip_top_near              mip levels 7..top (one workgroup) beside the
 //                               exact distance field up to 10 px
 //   k_cone_pair                 16 cones per level-3 probe (far segment) and
@@ -90,16 +89,8 @@ __device__ inline uint4 combine4(uint4 tl, uint4 tr, uint4 bl, uint4 br)
     return pack_cell(combine4(unpack_cell(tl), unpack_cell(tr), unpack_cell(bl), unpack_cell(br)));
 }
 
-// ---- occupancy bits + mip levels 1 and 2 -------------------------------------------
-// One 64x16-pixel block: wave w covers 64 consecutive pixels of rows w, w+4,
-// w+8, w+12 (a thread's 4 pixels have their loads in flight at once).  The
-// occupancy rule of the jump flood's seed pass (emissive > 0 in any channel,
-// or the near side of a depth edge), one 64-bit ballot per row into
-// bits[y][x/64], and the block's mip cells: 32x8 of level 1 (one per
-// thread) and 16x4 of level 2.  Pixels outside the image are empty.
-constexpr int kSurrSeedRows = 16;
-
-// A pixel as a mip cell: rgb E*[emitter]; kind 0 empty, 1 emitter, 2 dark occluder.
+// The pixel of a mip cell: emitter (kind 1: rgb, area 1, columns and rows
+// 1) or dark occluder (kind 2) or empty (kind 0), from (rgb, kind bits).
 __device__ inline MipCell pixel_cell(float4 p)
 {
     const uint32_t kind = __float_as_uint(p.w);
@@ -107,80 +98,14 @@ __device__ inline MipCell pixel_cell(float4 p)
     return MipCell{{p.x, p.y, p.z, f, f, f, o, o}};
 }
 
-__global__ __launch_bounds__(kBlock) void k_seed_surr(const float *__restrict__ depth, const float *__restrict__ emissive,
-                                                      uint64_t *__restrict__ bits, int words, MipArgs m, int W, int H,
-                                                      int bx_count)
-{
-    __shared__ float4 pe[kSurrSeedRows][64];
-    __shared__ uint4 c1[kSurrSeedRows / 2][32];
-    // row bands per XCD: the depth rows above and below a block are mostly
-    // read by the same XCD, from its L2
-    const int b = xcd_band(blockIdx.x, gridDim.x);
-    const int bx = b % bx_count, by = b / bx_count;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int x = bx * 64 + lane;
-    const size_t plane = (size_t)W * H;
-    float e[4][3], dp[4], dl[4], dr[4], du[4], dd[4];
-    // all loads first (clamped addresses; out-of-image pixels are masked below)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int y = min(by * kSurrSeedRows + w + 4 * j, H - 1), xc = min(x, W - 1);
-        const size_t p = (size_t)y * W + xc;
-        e[j][0] = emissive[p];
-        e[j][1] = emissive[plane + p];
-        e[j][2] = emissive[2 * plane + p];
-        dp[j] = depth[p];
-        dl[j] = depth[p - (xc > 0)];
-        dr[j] = depth[p + (xc + 1 < W)];
-        du[j] = depth[p - (y > 0 ? (size_t)W : 0)];
-        dd[j] = depth[p + (y + 1 < H ? (size_t)W : 0)];
-    }
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int row = w + 4 * j, y = by * kSurrSeedRows + row;
-        const bool in = x < W && y < H;
-        const float thr = 0.05f * dp[j];
-        const bool em = in && (e[j][0] > 0.0f || e[j][1] > 0.0f || e[j][2] > 0.0f);
-        const bool occ = em || (in && ((x > 0 && dl[j] - dp[j] > thr) || (x + 1 < W && dr[j] - dp[j] > thr) ||
-                                       (y > 0 && du[j] - dp[j] > thr) || (y + 1 < H && dd[j] - dp[j] > thr)));
-        const uint64_t bw = __ballot(occ);
-        if (lane == 0 && y < H) bits[(size_t)y * words + bx] = bw;
-        pe[row][lane] = em ? make_float4(e[j][0], e[j][1], e[j][2], __uint_as_float(1u))
-                           : make_float4(0.f, 0.f, 0.f, __uint_as_float(occ ? 2u : 0u));
-    }
-    __syncthreads();
-    {
-        const int cx = threadIdx.x & 31, cy = threadIdx.x >> 5;
-        const uint4 t = pack_cell(combine4(pixel_cell(pe[2 * cy][2 * cx]), pixel_cell(pe[2 * cy][2 * cx + 1]),
-                                           pixel_cell(pe[2 * cy + 1][2 * cx]), pixel_cell(pe[2 * cy + 1][2 * cx + 1])));
-        c1[cy][cx] = t;
-        const int gx = bx * 32 + cx, gy = by * (kSurrSeedRows / 2) + cy;
-        if (gx < m.w[1] && gy < m.h[1]) m.tex[mip_at(m, 1, gx, gy)] = t;
-    }
-    __syncthreads();
-    if (threadIdx.x < 64 && m.levels >= 2) {
-        const int cx = threadIdx.x & 15, cy = threadIdx.x >> 4;
-        const uint4 t = combine4(c1[2 * cy][2 * cx], c1[2 * cy][2 * cx + 1], c1[2 * cy + 1][2 * cx], c1[2 * cy + 1][2 * cx + 1]);
-        const int gx = bx * 16 + cx, gy = by * (kSurrSeedRows / 4) + cy;
-        if (gx < m.w[2] && gy < m.h[2]) m.tex[mip_at(m, 2, gx, gy)] = t;
-    }
-}
-
-hipError_t launch_seed_surr(const float *depth, const float *emissive, uint64_t *bits, const MipArgs &m, int W, int H,
-                            hipStream_t s)
-{
-    const int bxc = (W + 63) / 64, byc = (H + kSurrSeedRows - 1) / kSurrSeedRows;
-    hipLaunchKernelGGL(k_seed_surr, dim3(bxc * byc), dim3(kBlock), 0, s, depth, emissive, bits, bxc, m, W, H, bxc);
-    return hipGetLastError();
-}
-
 // ---- near-field distance field ---------------------------------------------
 // The distance code of the nearest occupied pixel wherever it is at most
-// kNearR pixels away (exact there), the far code elsewhere.  Levels 0 and 1 end their intervals at 2 and
-// 10 px, so every step they take from such a field (a hit needs distance 0; a
-// value of 10 or more ends the interval in one step either way) equals the
-// step they take from the full field.  Per 32x32 tile: the occupancy row masks of
-// the tile plus a 10-px halo (two words of the bit field per row), the
+// kNearR pixels away (exact there), the far code elsewhere.  Levels 0 and 1
+// end their intervals at 2 and 10 px, so every step they take from such a
+// field (a hit needs distance 0; a value of 10 or more ends the interval in
+// one step either way) equals the step they take from the full field.  Per
+// 32x32 tile: the occupancy row masks of the tile plus a 10-px halo (two
+// words of the bit field per row), the
 // distance to the nearest occupied pixel of the row within +-10 (count of
 // zero bits), then d^2 = min over |dy| <= 10 of g(y+dy)^2 + dy^2.
 constexpr int kNearR = 10, kNearT = 32, kNearS = kNearT + 2 * kNearR;
@@ -276,16 +201,10 @@ size_t mip_cells(const MipArgs &m) { return m.off[m.levels] + (size_t)(m.w[m.lev
 // Levels 3..6 of a 16x16 block of level-2 cells (64x64 pixels) in LDS.
 constexpr int kMidTile = 16, kMidLast = 6;
 
-__device__ inline void mip_mid_block(int blk, const MipArgs &m, int tiles_x)
+// Levels 3..6 of one 16x16 tile of level-2 cells held in LDS (se, row-major;
+// cells outside the image are zero), written to the pyramid.
+__device__ inline void mip_mid_levels(uint4 *se, const MipArgs &m, int tx, int ty)
 {
-    __shared__ uint4 se[kMidTile * kMidTile];
-    const int tx = blk % tiles_x, ty = blk / tiles_x;
-    {
-        const int cx = threadIdx.x % kMidTile, cy = threadIdx.x / kMidTile;
-        const int gx = tx * kMidTile + cx, gy = ty * kMidTile + cy;
-        se[threadIdx.x] = (gx < m.w[2] && gy < m.h[2]) ? m.tex[mip_at(m, 2, gx, gy)] : make_uint4(0u, 0u, 0u, 0u);
-    }
-    __syncthreads();
     int n = kMidTile;
     const int lastl = m.levels < kMidLast ? m.levels : kMidLast;
     for (int l = 3; l <= lastl; ++l) {
@@ -306,6 +225,111 @@ __device__ inline void mip_mid_block(int blk, const MipArgs &m, int tiles_x)
         __syncthreads();
         n = nn;
     }
+}
+
+// ---- occupancy bits + mip levels 1..6 ------------------------------------------
+// One 64x64-pixel block of 16 waves: wave w covers rows 4w..4w+3 of 64
+// consecutive columns (a lane's 4 pixels have their loads in flight at once;
+// the depth rows above and below a lane's middle rows are its own loads).
+// The occupancy rule of the jump flood's seed pass (emissive > 0 in any
+// channel, or the near side of a depth edge), one 64-bit ballot per row into
+// bits[y][x/64]; mip level 1 (2x2-px cells, the lane pair's pixels by
+// shuffles), level 2 (4x4 px, the lane quad's level-1 cells) and levels 3..6
+// of the block's 16x16 level-2 cells in LDS.  Every cell is combined from its
+// stored (fp16-rounded) children in the order TL, TR, BL, BR.  Pixels outside
+// the image are empty.
+constexpr int kSeedMipBlock = 1024, kSeedMipPx = 64;
+static_assert(kSeedMipBlock / 64 * 4 == kSeedMipPx && kSeedMipPx / 4 == kMidTile, "a block is one mid tile");
+
+__device__ inline float4 shfl_xor4(float4 v, int mask)
+{
+    return make_float4(__shfl_xor(v.x, mask, 64), __shfl_xor(v.y, mask, 64), __shfl_xor(v.z, mask, 64),
+                       __shfl_xor(v.w, mask, 64));
+}
+__device__ inline uint4 shfl_xor4(uint4 v, int mask)
+{
+    return make_uint4(__shfl_xor(v.x, mask, 64), __shfl_xor(v.y, mask, 64), __shfl_xor(v.z, mask, 64),
+                      __shfl_xor(v.w, mask, 64));
+}
+
+__global__ __launch_bounds__(kSeedMipBlock) void k_seed_surr(const float *__restrict__ depth,
+                                                             const float *__restrict__ emissive,
+                                                             uint64_t *__restrict__ bits, int words, MipArgs m, int W,
+                                                             int H, int bx_count)
+{
+    __shared__ uint4 se[kMidTile * kMidTile];
+    // row bands per XCD: the depth rows above and below a block are mostly
+    // read by the same XCD, from its L2
+    const int b = xcd_band(blockIdx.x, gridDim.x);
+    const int bx = b % bx_count, by = b / bx_count;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int x = bx * 64 + lane, y0 = by * kSeedMipPx + 4 * w;
+    const int xc = min(x, W - 1);
+    const size_t plane = (size_t)W * H;
+    float e[4][3], dp[4], dl[4], dr[4];
+    // all loads first (clamped addresses; out-of-image pixels are masked below)
+    const float du0 = depth[(size_t)min(max(y0 - 1, 0), H - 1) * W + xc];
+    const float dd3 = depth[(size_t)min(y0 + 4, H - 1) * W + xc];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int y = min(y0 + j, H - 1);
+        const size_t p = (size_t)y * W + xc;
+        e[j][0] = emissive[p];
+        e[j][1] = emissive[plane + p];
+        e[j][2] = emissive[2 * plane + p];
+        dp[j] = depth[p];
+        dl[j] = depth[p - (xc > 0)];
+        dr[j] = depth[p + (xc + 1 < W)];
+    }
+    float4 px[4];  // (rgb, kind bits) of the lane's 4 pixels
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int y = y0 + j;
+        const bool in = x < W && y < H;
+        // the rows above and below: this lane's neighbouring rows, or the two
+        // extra loads at the ends (where y + 1 < H, row y + 1 is not clamped)
+        const float du = j == 0 ? du0 : dp[j - 1], dd = j == 3 ? dd3 : dp[j + 1];
+        const float thr = 0.05f * dp[j];
+        const bool em = in && (e[j][0] > 0.0f || e[j][1] > 0.0f || e[j][2] > 0.0f);
+        const bool occ = em || (in && ((x > 0 && dl[j] - dp[j] > thr) || (x + 1 < W && dr[j] - dp[j] > thr) ||
+                                       (y > 0 && du - dp[j] > thr) || (y + 1 < H && dd - dp[j] > thr)));
+        const uint64_t bw = __ballot(occ);
+        if (lane == 0 && y < H) bits[(size_t)y * words + bx] = bw;
+        px[j] = em ? make_float4(e[j][0], e[j][1], e[j][2], __uint_as_float(1u))
+                   : make_float4(0.f, 0.f, 0.f, __uint_as_float(occ ? 2u : 0u));
+    }
+    // level 1: the cell of rows (2r, 2r + 1) and columns (lane, lane + 1) of
+    // an even lane; TR, BR come from the odd lane
+    uint4 l1[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const float4 tr = shfl_xor4(px[2 * r], 1), br = shfl_xor4(px[2 * r + 1], 1);
+        l1[r] = pack_cell(combine4(pixel_cell(px[2 * r]), pixel_cell(tr), pixel_cell(px[2 * r + 1]), pixel_cell(br)));
+        const int gx = bx * 32 + (lane >> 1), gy = by * (kSeedMipPx / 2) + 2 * w + r;
+        if ((lane & 1) == 0 && gx < m.w[1] && gy < m.h[1]) m.tex[mip_at(m, 1, gx, gy)] = l1[r];
+    }
+    // level 2: the cell of level-1 rows (2w, 2w + 1) and columns of lanes
+    // (4c, 4c + 2); TR, BR come from lane 4c + 2
+    if (m.levels >= 2) {
+        const uint4 tr = shfl_xor4(l1[0], 2), br = shfl_xor4(l1[1], 2);
+        const uint4 l2 = combine4(l1[0], tr, l1[1], br);
+        if ((lane & 3) == 0) {
+            se[w * kMidTile + (lane >> 2)] = l2;
+            const int gx = bx * 16 + (lane >> 2), gy = by * kMidTile + w;
+            if (gx < m.w[2] && gy < m.h[2]) m.tex[mip_at(m, 2, gx, gy)] = l2;
+        }
+    }
+    __syncthreads();
+    if (m.levels >= 3) mip_mid_levels(se, m, bx, by);
+}
+
+hipError_t launch_seed_surr(const float *depth, const float *emissive, uint64_t *bits, const MipArgs &m, int W, int H,
+                            hipStream_t s)
+{
+    const int bxc = (W + 63) / 64, byc = (H + kSeedMipPx - 1) / kSeedMipPx;
+    hipLaunchKernelGGL(k_seed_surr, dim3(bxc * byc), dim3(kSeedMipBlock), 0, s, depth, emissive, bits, (W + 63) / 64, m,
+                       W, H, bxc);
+    return hipGetLastError();
 }
 
 // Levels 7..top from level 6, one workgroup; also clears the fallback counter
@@ -361,12 +385,6 @@ __global__ __launch_bounds__(kBlock) void k_mip_top(MipArgs m, int *zero_me)
         __threadfence_block();
         __syncthreads();
     }
-}
-
-// tiles in row bands per XCD, like the seed blocks that wrote their level 2
-__global__ __launch_bounds__(kBlock) void k_mip_mid(MipArgs m, int tiles_x)
-{
-    mip_mid_block(xcd_band(blockIdx.x, gridDim.x), m, tiles_x);
 }
 
 // Levels 7..top in one workgroup with little LDS: level 7 from level 6 in
@@ -428,25 +446,12 @@ hipError_t launch_mip_top_near(const MipArgs &m, int *zero_me, const uint64_t *b
     return hipGetLastError();
 }
 
-hipError_t launch_mip_mid(const MipArgs &m, hipStream_t s)
-{
-    if (m.levels < 3) return hipSuccess;
-    const int tx = (m.w[2] + kMidTile - 1) / kMidTile, ty = (m.h[2] + kMidTile - 1) / kMidTile;
-    hipLaunchKernelGGL(k_mip_mid, dim3(tx * ty), dim3(kBlock), 0, s, m, tx);
-    return hipGetLastError();
-}
-
 hipError_t launch_mip_top(const MipArgs &m, int *zero_me, hipStream_t s)
 {
     hipLaunchKernelGGL(k_mip_top, dim3(1), dim3(kBlock), 0, s, m, zero_me);
     return hipGetLastError();
 }
 
-hipError_t launch_mip_rest(const MipArgs &m, int *zero_me, hipStream_t s)
-{
-    hipError_t e = launch_mip_mid(m, s);
-    return e != hipSuccess ? e : launch_mip_top(m, zero_me, s);
-}
 
 // Unpacked copy of mip level l (parity readback): emission [h][w][4] (mean
 // E*[emitter] rgb, emitter area), coverage [h][w][4] (ex, ey, ox, oy).
