@@ -65,26 +65,10 @@ int jfa_tail_step(int W, int H, int fuse)
 
 // Occupancy of pixel (x, y): any emissive channel > 0, or a 4-neighbour q
 // with D(q) - D(p) > 0.05 D(p).  All loads are unconditional (clamped
-// addresses) so they are in flight together; an out-of-image neighbour reads
-// p itself and is masked out.
-__device__ inline bool occupied(const float *__restrict__ depth, const float *__restrict__ emissive, int x, int y, int W,
-                                int H, float &e0, float &e1, float &e2)
-{
-    const size_t plane = (size_t)W * H, p = (size_t)y * W + x;
-    e0 = emissive[p];
-    e1 = emissive[plane + p];
-    e2 = emissive[2 * plane + p];
-    const float dp = depth[p];
-    const float dl = depth[p - (x > 0)], dr = depth[p + (x + 1 < W)];
-    const float du = depth[p - (y > 0 ? (size_t)W : 0)], dd = depth[p + (y + 1 < H ? (size_t)W : 0)];
-    const float thr = 0.05f * dp;
-    return e0 > 0.0f || e1 > 0.0f || e2 > 0.0f || (x > 0 && dl - dp > thr) || (x + 1 < W && dr - dp > thr) ||
-           (y > 0 && du - dp > thr) || (y + 1 < H && dd - dp > thr);
-}
-
-// 64 x 16 pixel blocks: a wave covers 64 consecutive pixels of rows
-// w, w + 4, w + 8, w + 12; a thread's 4 pixels have their loads in flight at
-// once (one pixel per thread leaves the kernel waiting on load latency).
+// addresses) so they are in flight together.
+// 64 x 16 pixel blocks: wave w covers rows 4w..4w+3 of 64 consecutive
+// pixels; a thread's 4 pixels have their loads in flight at once, and the
+// depth above and below its middle rows are its own loads (two extra rows).
 constexpr int kSeedRows = 16;
 
 // With emis4 set, also writes the emission interleaved (r, g, b, 0) of every
@@ -99,23 +83,36 @@ __global__ __launch_bounds__(kBlock) void k_seed_init(const float *__restrict__ 
 {
     const int b = xcd_band(blockIdx.x, gridDim.x);
     const int x = (b % bx_count) * 64 + (threadIdx.x & 63);
-    const int y0 = (b / bx_count) * kSeedRows + (threadIdx.x >> 6);
+    const int y0 = (b / bx_count) * kSeedRows + 4 * (threadIdx.x >> 6);
     if (x >= W) return;
-    bool occ[4];
+    const size_t plane = (size_t)W * H;
+    // all loads first (clamped addresses; out-of-image rows are masked below)
+    const float du0 = depth[(size_t)max(y0 - 1, 0) * W + x];
+    const float dd3 = depth[(size_t)min(y0 + 4, H - 1) * W + x];
     float4 e[4];
+    float dp[4], dl[4], dr[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        const int y = min(y0 + 4 * j, H - 1);
-        occ[j] = occupied(depth, emissive, x, y, W, H, e[j].x, e[j].y, e[j].z);
-        e[j].w = 0.0f;
+        const size_t p = (size_t)min(y0 + j, H - 1) * W + x;
+        e[j] = make_float4(emissive[p], emissive[plane + p], emissive[2 * plane + p], 0.0f);
+        dp[j] = depth[p];
+        dl[j] = depth[p - (x > 0)];
+        dr[j] = depth[p + (x + 1 < W)];
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        const int y = y0 + 4 * j;
-        if (y < H) {
-            seed[(size_t)y * W + x] = occ[j] ? (((uint32_t)y << 16) | (uint32_t)x) : kFarSeed;
-            if (emis4 && occ[j]) emis4[(size_t)y * W + x] = e[j];
-        }
+        const int y = y0 + j;
+        if (y >= H) break;
+        // rows y - 1 and y + 1: this lane's neighbouring rows or the two extra
+        // loads (where y + 1 < H, row y + 1 is not clamped)
+        const float du = j == 0 ? du0 : dp[j - 1], dd = j == 3 ? dd3 : dp[j + 1];
+        const float thr = 0.05f * dp[j];
+        const bool occ = e[j].x > 0.0f || e[j].y > 0.0f || e[j].z > 0.0f || (x > 0 && dl[j] - dp[j] > thr) ||
+                         (x + 1 < W && dr[j] - dp[j] > thr) || (y > 0 && du - dp[j] > thr) ||
+                         (y + 1 < H && dd - dp[j] > thr);
+        const size_t p = (size_t)y * W + x;
+        seed[p] = occ ? (((uint32_t)y << 16) | (uint32_t)x) : kFarSeed;
+        if (emis4 && occ) emis4[p] = e[j];
     }
 }
 
