@@ -97,3 +97,21 @@ def test_write_rejects_bad_planes(rc, tmp_path):
             recs[i].name, recs[i].channels, recs[i].height, recs[i].width, recs[i].data = b"same", 1, 1, 1, a.ctypes.data
         from radiancecascade_amd.stage import _dump_check
         _dump_check(_abi.load().rc_dump_write(os.fsencode(str(tmp_path / "y")), recs, 2))
+
+
+def test_training_rows_need_matching_gbuffer_planes(rc, tmp_path):
+    # rows_from_dumps checks every file's planes before it touches a device
+    from radiancecascade_amd.train import rows_from_dumps
+    p = _planes()
+    no_emis = str(tmp_path / "no_emis.rcdump")
+    rc.dump_write(no_emis, {"depth": p["depth"]})
+    with pytest.raises(ValueError, match="needs planes"):
+        rows_from_dumps([no_emis], levels=6)
+    mismatch = str(tmp_path / "mismatch.rcdump")
+    rc.dump_write(mismatch, {"depth": p["depth"], "emissive": p["emissive"][:, :, :50]})
+    with pytest.raises(ValueError, match="same size"):
+        rows_from_dumps([mismatch], levels=6)
+    two_ch = str(tmp_path / "two_ch.rcdump")
+    rc.dump_write(two_ch, {"depth": np.stack([p["depth"]] * 2), "emissive": p["emissive"]})
+    with pytest.raises(ValueError, match="same size"):
+        rows_from_dumps([two_ch], levels=6)
