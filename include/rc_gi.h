@@ -184,9 +184,16 @@ rc_status rc_device_count(int *count);
 rc_status rc_debug_seeds(rc_ctx *ctx, uint32_t *dev_out, void *stream);
 rc_status rc_debug_sdf(rc_ctx *ctx, float *dev_out, void *stream);
 rc_status rc_debug_level(rc_ctx *ctx, uint32_t level, float *dev_out, void *stream);
-/* March-only probe of one level of the last run's distance field: per ray in
- * canonical order ((py*Px+px)*N+k) the hit pixel y*W+x, -1 (left the image)
- * or -2 (reached the interval end); and the ray's canonical index. */
+/* The ray codes the last frame's kernels stored for one level, per ray in
+ * canonical order ((py*Px+px)*N+k): the hit pixel y*W+x, -1 (left the image)
+ * or -2 (reached the interval end).  Levels marched by the split march
+ * (rc_set_march split=1) return its code buffer, read by the merge kernels;
+ * levels marched in place (split=0, level 0, and levels 0-1 of the learned
+ * frame) return the copy those kernels write under RC_FLAG_KEEP_DEBUG
+ * (RC_ERR_STATE otherwise, and for levels the learned frame does not march).
+ * index_dev (optional, levels >= 1): per slot r, the canonical index that the
+ * split march's decode of r into (probe column, row, direction) stands for;
+ * equal to r when the decode is right. */
 rc_status rc_debug_march(rc_ctx *ctx, uint32_t level, int32_t *hit_dev, int64_t *index_dev, void *stream);
 
 /* ---- error norms (FP64 reductions on the GPU) ------------------------------

@@ -331,6 +331,59 @@ void rco_nearest_seed_bruteforce(int W, int H, const uint32_t *seed_init, uint32
     free(list);
 }
 
+/*
+ * Ground truth of the 2D transport the cascades approximate (a checker, not
+ * a flavour of the method): from every listed pixel centre, n_dirs rays at
+ * theta_k = 2 pi (k + 1/2) / n_dirs, each walked through every pixel it
+ * crosses (Amanatides & Woo's grid traversal, FP64) until the first occupied
+ * pixel (seed_init != none), which contributes its emission, or until it
+ * leaves the image or passes distance t_max (no contribution).  A ray that
+ * starts in an occupied pixel hits it.  out[3 * i + c] = mean over the
+ * directions, channel c, for pixel pixels[i] (y * W + x; every pixel in order
+ * when pixels is NULL, n_pix = W * H).
+ */
+void rco_bruteforce_irradiance(int W, int H, const uint32_t *seed_init, const float *emissive, int n_dirs,
+                               double t_max, const int32_t *pixels, int n_pix, double *out)
+{
+    const size_t plane = (size_t)W * H;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int i = 0; i < n_pix; ++i) {
+        const int p = pixels ? pixels[i] : i;
+        const int x0 = p % W, y0 = p / W;
+        double acc[3] = {0.0, 0.0, 0.0};
+        for (int k = 0; k < n_dirs; ++k) {
+            const double th = 2.0 * M_PI * ((double)k + 0.5) / (double)n_dirs;
+            const double dx = cos(th), dy = sin(th);
+            int ix = x0, iy = y0;
+            const int sx = dx > 0 ? 1 : -1, sy = dy > 0 ? 1 : -1;
+            const double adx = fabs(dx), ady = fabs(dy);
+            /* the centre is half a pixel from either boundary */
+            double tx = adx > 0 ? 0.5 / adx : INFINITY, ty = ady > 0 ? 0.5 / ady : INFINITY;
+            const double ddx = adx > 0 ? 1.0 / adx : INFINITY, ddy = ady > 0 ? 1.0 / ady : INFINITY;
+            for (;;) {
+                const size_t q = (size_t)iy * W + ix;
+                if (seed_init[q] != RCO_SEED_NONE) {
+                    acc[0] += emissive[q];
+                    acc[1] += emissive[plane + q];
+                    acc[2] += emissive[2 * plane + q];
+                    break;
+                }
+                if (tx <= ty) {
+                    if (tx > t_max) break;
+                    ix += sx;
+                    tx += ddx;
+                } else {
+                    if (ty > t_max) break;
+                    iy += sy;
+                    ty += ddy;
+                }
+                if (ix < 0 || iy < 0 || ix >= W || iy >= H) break;
+            }
+        }
+        for (int c = 0; c < 3; ++c) out[3 * (size_t)i + c] = acc[c] / (double)n_dirs;
+    }
+}
+
 void rco_set_threads(int n)
 {
 #ifdef _OPENMP

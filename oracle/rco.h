@@ -82,6 +82,9 @@ void     rco_sdf_f32(int W, int H, const uint32_t *seed, float *sdf);
 void     rco_sdf_f64(int W, int H, const uint32_t *seed, double *sdf);
 void     rco_nearest_seed_bruteforce(int W, int H, const uint32_t *seed_init,
                                      uint32_t *nearest);
+/* ground-truth irradiance by an exact per-pixel grid walk (rco.c) */
+void     rco_bruteforce_irradiance(int W, int H, const uint32_t *seed_init, const float *emissive, int n_dirs,
+                                   double t_max, const int32_t *pixels, int n_pix, double *out);
 
 /* ---- march / merge / gather (H4-H6), both flavours ------------------- */
 /* rgba: [Py][Px][N][4] (canonical ray order), hit: [Py][Px][N] or NULL */

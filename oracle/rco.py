@@ -68,6 +68,7 @@ def lib():
             "rco_sdf_f32": (None, [i32, i32, P, P]),
             "rco_sdf_f64": (None, [i32, i32, P, P]),
             "rco_nearest_seed_bruteforce": (None, [i32, i32, P, P]),
+            "rco_bruteforce_irradiance": (None, [i32, i32, P, P, i32, ctypes.c_double, P, i32, P]),
             "rco_march_level_f64": (None, [i32, i32, i32, P, P, P, P]),
             "rco_march_level_f32": (None, [i32, i32, i32, P, P, P, P]),
             "rco_merge_level_f64": (None, [i32, i32, i32, P, P, P]),
@@ -217,6 +218,50 @@ def sdf_f64(seeds: np.ndarray) -> np.ndarray:
     return out
 
 
+def edt_d2_exact(seed_init_field: np.ndarray) -> np.ndarray:
+    """int64 [H][W]: exact squared distance to the nearest occupied pixel
+    (-1 where no pixel is occupied), from scipy's exact Euclidean distance
+    transform (a library; its nearest-feature indices give d^2 in integers)."""
+    from scipy import ndimage
+    occ = np.asarray(seed_init_field) != SEED_NONE
+    H, W = occ.shape
+    if not occ.any():
+        return np.full((H, W), -1, np.int64)
+    _, (iy, ix) = ndimage.distance_transform_edt(~occ, return_indices=True)
+    yy, xx = np.mgrid[0:H, 0:W]
+    return ((iy - yy).astype(np.int64) ** 2 + (ix - xx).astype(np.int64) ** 2)
+
+
+SDF_NONE_EXACT = 1e30
+
+
+def sdf_exact_f64(seed_init_field: np.ndarray) -> np.ndarray:
+    """float64 [H][W]: the exact field sqrt(d^2) of the nearest occupied pixel
+    (1e30 with none) -- the field the one-byte code approximates from below."""
+    d2 = edt_d2_exact(seed_init_field)
+    return np.where(d2 >= 0, np.sqrt(np.maximum(d2, 0).astype(np.float64)), SDF_NONE_EXACT)
+
+
+def bruteforce_irradiance(seed_init_field: np.ndarray, emissive: np.ndarray, L: int, pixels=None) -> np.ndarray:
+    """Ground truth for the whole path (rco_bruteforce_irradiance): per pixel
+    centre, the mean over N_{L-1} directions of the emission of the first
+    occupied pixel each ray crosses within t1_{L-1} px.  Returns [3][H][W], or
+    [3][n] for a list of pixel indices y*W+x."""
+    s = np.ascontiguousarray(seed_init_field, np.uint32)
+    H, W = s.shape
+    n_dirs, t_max = num_dirs(L - 1), float(interval(L - 1)[1])
+    if pixels is None:
+        out = np.empty((H * W, 3), np.float64)
+        lib().rco_bruteforce_irradiance(W, H, _p(s), _p(np.ascontiguousarray(emissive, np.float32)), n_dirs, t_max,
+                                        None, H * W, _p(out))
+        return out.T.reshape(3, H, W).copy()
+    px = np.ascontiguousarray(pixels, np.int32)
+    out = np.empty((px.size, 3), np.float64)
+    lib().rco_bruteforce_irradiance(W, H, _p(s), _p(np.ascontiguousarray(emissive, np.float32)), n_dirs, t_max,
+                                    _p(px), px.size, _p(out))
+    return out.T.copy()
+
+
 # ---- march / merge / gather -------------------------------------------------
 def _flavour(dtype):
     return "f32" if np.dtype(dtype) == np.float32 else "f64"
@@ -276,6 +321,19 @@ def run(depth: np.ndarray, emissive: np.ndarray, L: int, dtype=np.float64, want_
     if want_seeds:
         out.append(seeds)
     return out[0] if len(out) == 1 else tuple(out)
+
+
+def run_on_field(sdf: np.ndarray, emissive: np.ndarray, L: int) -> np.ndarray:
+    """March, merge and gather on a given distance field (in sdf's precision):
+    the composition of the stage functions, top level first.  With the field
+    the jump flood produces this is rco_run; with sdf_exact_f64 it is the
+    exact-field flavour."""
+    H, W = sdf.shape
+    upper = None
+    for i in range(L - 1, -1, -1):
+        m, _ = march_level(sdf, emissive, i)
+        upper = merge_level(W, H, i, m, upper)
+    return gather(W, H, upper)
 
 
 def set_threads(n: int) -> None:

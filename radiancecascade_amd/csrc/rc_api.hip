@@ -29,6 +29,12 @@ struct rc_ctx {
     void *level[RC_MAX_LEVELS] = {};
     size_t level_texels[RC_MAX_LEVELS] = {};
     float *dbg_level0 = nullptr;
+    // ray codes of the last frame's in-place marches (RC_FLAG_KEEP_DEBUG),
+    // canonical order per level; code_src[i]: where level i's codes are
+    // (0 none, 1 the split march's `codes`, 2 `dbg_codes`)
+    int *dbg_codes = nullptr;
+    size_t dbg_code_off[RC_MAX_LEVELS] = {};
+    int code_src[RC_MAX_LEVELS] = {};
     float2 *dirs = nullptr;
     size_t dir_off[RC_MAX_LEVELS] = {};
     int tile[RC_MAX_LEVELS] = {};
@@ -146,6 +152,9 @@ static void release(rc_ctx *c)
     for (auto &p : c->level) { if (p) hipFree(p); p = nullptr; }
     if (c->dbg_level0) hipFree(c->dbg_level0);
     c->dbg_level0 = nullptr;
+    if (c->dbg_codes) hipFree(c->dbg_codes);
+    c->dbg_codes = nullptr;
+    for (auto &v : c->code_src) v = 0;
     if (c->dirs) hipFree(c->dirs);
     c->dirs = nullptr;
     if (c->codes) hipFree(c->codes);
@@ -205,7 +214,15 @@ static rc_status allocate(rc_ctx *c, uint32_t W, uint32_t H, uint32_t L)
         HIP_TRY(c, hipMalloc(&c->level[i], texels * tb));
     }
     c->level_texels[0] = (size_t)grid_dim(W, 0) * grid_dim(H, 0);
-    if (c->cfg.flags & RC_FLAG_KEEP_DEBUG) HIP_TRY(c, hipMalloc(&c->dbg_level0, c->level_texels[0] * 16));
+    if (c->cfg.flags & RC_FLAG_KEEP_DEBUG) {
+        HIP_TRY(c, hipMalloc(&c->dbg_level0, c->level_texels[0] * 16));
+        size_t n_codes = 0;
+        for (int i = 0; i < (int)L; ++i) {
+            c->dbg_code_off[i] = n_codes;
+            n_codes += (size_t)grid_dim(W, i) * grid_dim(H, i) * num_dirs(i);
+        }
+        HIP_TRY(c, hipMalloc(&c->dbg_codes, n_codes * sizeof(int)));
+    }
     // direction tables: theta = 2 pi (k + 1/2) / N in double, rounded to fp32
     std::vector<float2> dirs;
     for (int i = 0; i < (int)L; ++i) {
@@ -391,12 +408,17 @@ static rc_status run_frame(rc_ctx *c, const float *depth, const float *emissive,
         LevelArgs a = level_args(c, i, emissive);
         a.emis4 = c->emis4;
         if (split) a.codes = c->codes + c->code_off[i];
+        else if (dbg) a.dbg_codes = c->dbg_codes + c->dbg_code_off[i];
         HIP_TRY(c, launch_level(a, c->storage, c->tile[i], c->rpt[i], s));
         tm.end(RC_STAGE_LEVEL0 + i);
+        c->code_src[i] = split ? 1 : (dbg ? 2 : 0);
     }
     tm.begin(RC_STAGE_LEVEL0);
-    HIP_TRY(c, launch_level0_gather(level_args(c, 0, emissive), c->storage, irr, dbg ? c->dbg_level0 : nullptr, s));
+    LevelArgs a0 = level_args(c, 0, emissive);
+    if (dbg) a0.dbg_codes = c->dbg_codes + c->dbg_code_off[0];
+    HIP_TRY(c, launch_level0_gather(a0, c->storage, irr, dbg ? c->dbg_level0 : nullptr, s));
     tm.end(RC_STAGE_LEVEL0);
+    c->code_src[0] = dbg ? 2 : 0;
 
     if (prof) {
         rc_status st = collect_profile(c, s, global_passes);
@@ -573,14 +595,17 @@ static rc_status run_frame_surrogate(rc_ctx *c, const float *depth, const float 
     a1.upper = c->surr_a2;
     a1.out = c->surr_a1;
     a1.codes = nullptr;
+    if (dbg) a1.dbg_codes = c->dbg_codes + c->dbg_code_off[1];
     HIP_TRY(c, launch_level(a1, c->storage, c->surr_tile1 >= 0 ? c->surr_tile1 : c->tile[1],
                             c->surr_rpt1 >= 0 ? c->surr_rpt1 : c->rpt[1], s));
     tm.end(RC_STAGE_LEVEL0 + 1);
     tm.begin(RC_STAGE_LEVEL0);
     LevelArgs a0 = level_args(c, 0, emissive);
     a0.upper = c->surr_a1;
+    if (dbg) a0.dbg_codes = c->dbg_codes + c->dbg_code_off[0];
     HIP_TRY(c, launch_level0_gather(a0, c->storage, irr, dbg ? c->dbg_level0 : nullptr, s));
     tm.end(RC_STAGE_LEVEL0);
+    for (int i = 0; i < RC_MAX_LEVELS; ++i) c->code_src[i] = (dbg && i <= 1) ? 2 : 0;
     if (prof) {
         rc_status st = collect_profile(c, s, 0);
         if (st != RC_OK) return st;
@@ -958,12 +983,28 @@ rc_status rc_debug_level(rc_ctx *c, uint32_t level, float *dev_out, void *stream
 
 rc_status rc_debug_march(rc_ctx *c, uint32_t level, int32_t *hit_dev, int64_t *index_dev, void *stream)
 {
-    if (!c || !hit_dev || !index_dev) return fail(c, RC_ERR_NULL, "rc_debug_march: NULL argument");
+    if (!c || !hit_dev) return fail(c, RC_ERR_NULL, "rc_debug_march: NULL argument");
     if ((int)level >= c->L) return fail(c, RC_ERR_INVALID, "rc_debug_march: level out of range");
     if (!c->ran) return fail(c, RC_ERR_STATE, "rc_debug_march: no run yet");
-    // march-only: the probe kernel reads the distance field, never emissive
-    LevelArgs a = level_args(c, (int)level, nullptr);
-    HIP_TRY(c, launch_march_probe(a, c->spec, hit_dev, index_dev, (hipStream_t)stream));
+    const hipStream_t s = (hipStream_t)stream;
+    const size_t rays = (size_t)grid_dim(c->W, (int)level) * grid_dim(c->H, (int)level) * num_dirs((int)level);
+    // the codes the last frame's kernels stored: the split march's buffer,
+    // or the in-place marches' debug copy
+    const int src = c->code_src[level];
+    if (src == 1) {
+        HIP_TRY(c, hipMemcpyAsync(hit_dev, c->codes + c->code_off[level], rays * sizeof(int), hipMemcpyDeviceToDevice, s));
+    } else if (src == 2) {
+        HIP_TRY(c, hipMemcpyAsync(hit_dev, c->dbg_codes + c->dbg_code_off[level], rays * sizeof(int),
+                                  hipMemcpyDeviceToDevice, s));
+    } else {
+        return fail(c, RC_ERR_STATE, "rc_debug_march: this level's rays were not marched in the last frame with "
+                                     "RC_FLAG_KEEP_DEBUG (or, in the split march, at all)");
+    }
+    if (index_dev) {
+        if (level == 0) return fail(c, RC_ERR_INVALID, "rc_debug_march: level 0 has no split-march decode");
+        const MarchAllArgs m = march_all_args(c);
+        HIP_TRY(c, launch_ray_decode(m.lev[c->L - 1 - (int)level], index_dev, s));
+    }
     return RC_OK;
 }
 

@@ -92,15 +92,6 @@ __device__ inline void ray_round(Ray &r, __amdgpu_buffer_rsrc_t sdf, int W, int 
     }
 }
 
-template <int SPEC>
-__device__ inline int march_ray_spec(__amdgpu_buffer_rsrc_t sdf, int W, int H, float cx, float cy, float dx,
-                                     float dy, float t0, float t1, bool active)
-{
-    Ray r = ray_begin(cx, cy, dx, dy, t0, t1, active);
-    while (r.alive) ray_round<SPEC>(r, sdf, W, H, t1);
-    return r.code;
-}
-
 // Level i >= 1.  Workgroup tile: TX x (TY*RPT) probes x KD directions; lane
 // (probe column, direction) owns the RPT probes stacked TY rows apart.
 template <int TX, int TY, int KD, int RPT, Storage S, bool UPPER, bool FROM_CODES>
@@ -157,6 +148,15 @@ __global__ __launch_bounds__(kBlock) void k_level(LevelArgs a)
         }
     } else {
         march_rays<RPT>(sdf_rsrc(a.sdf, a.W, a.H), a.W, a.H, cx, cy, dir.x, dir.y, a.t0, a.t1, active, hitq);
+        if (a.dbg_codes) {
+#pragma unroll
+            for (int j = 0; j < RPT; ++j) {
+                const int py = pyb + j * TY;
+                if (active[j])
+                    a.dbg_codes[((__umul24((uint32_t)py, (uint32_t)a.PX) + (uint32_t)px) << log2n) + (uint32_t)k] =
+                        hitq[j];
+            }
+        }
     }
 
     const uint32_t plane = __umul24((uint32_t)a.W, (uint32_t)a.H);
@@ -291,6 +291,10 @@ __global__ __launch_bounds__(kBlock) void k_level0(LevelArgs a, float *__restric
             }
         }
     }
+    if (a.dbg_codes && probe && sx >= 1 && sx <= TX && sy >= 1 && sy <= TY) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) a.dbg_codes[((size_t)py * a.PX + px) * 4 + k] = code[k];
+    }
     float r[4], g[4], b[4], al[4];
     const bool one_hit = code[0] >= 0 && code[0] == code[1] && code[0] == code[2] && code[0] == code[3];
     if (one_hit) {
@@ -406,6 +410,19 @@ __global__ __launch_bounds__(kBlock) void k_level0(LevelArgs a, float *__restric
     }
 }
 
+// Canonical ray r = (py*PX + px)*N + k -> (px, py, k): the direction is the
+// low log2(N) bits; probe / PX by an fp32 reciprocal, corrected to the exact
+// quotient (probe < 2^24, so the estimate is off by at most one).
+__device__ inline void decode_ray(const MarchLevel &L, int ray, int &px, int &py, int &k)
+{
+    k = ray & (L.N - 1);
+    const int probe = ray >> L.log2n;
+    py = (int)((float)probe * L.inv_px);
+    px = probe - py * L.PX;
+    if (px < 0) { --py; px += L.PX; }
+    else if (px >= L.PX) { ++py; px -= L.PX; }
+}
+
 // All levels >= 1 marched by one launch; each ray's code (hit pixel, -1 left
 // the image, -2 reached t1) is written in canonical order for the merge
 // kernels.  The block list is level-major with the top level first, so the
@@ -434,13 +451,8 @@ __global__ __launch_bounds__(kBlock) void k_march_all(MarchAllArgs a)
     Ray r;
     auto start = [&]() {
         const bool active = ray < total;
-        const int rr = active ? ray : 0;
-        const int k = rr & (L.N - 1), probe = rr >> L.log2n;
-        // probe / PX by an fp32 reciprocal, corrected to the exact quotient
-        int py = (int)((float)probe * L.inv_px);
-        int px = probe - py * L.PX;
-        if (px < 0) { --py; px += L.PX; }
-        else if (px >= L.PX) { ++py; px -= L.PX; }
+        int px, py, k;
+        decode_ray(L, active ? ray : 0, px, py, k);
         const float2 dir = L.dirs[k];
         r = ray_begin(((float)px + 0.5f) * L.spacing, ((float)py + 0.5f) * L.spacing, dir.x, dir.y, L.t0, L.t1,
                       active);
@@ -457,21 +469,16 @@ __global__ __launch_bounds__(kBlock) void k_march_all(MarchAllArgs a)
     }
 }
 
-// March-only probe for parity tests: hit code and canonical index per ray.
-template <int SPEC>
-__global__ __launch_bounds__(kBlock) void k_march_probe(LevelArgs a, int32_t *__restrict__ hit,
-                                                        int64_t *__restrict__ index)
+// The split march's decode of every slot of one level, recomposed into the
+// canonical index it stands for (parity readback: equal to the slot number
+// when the decode is right).
+__global__ __launch_bounds__(kBlock) void k_ray_decode(MarchLevel l, int64_t *__restrict__ index)
 {
-    const int64_t ray = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    const int64_t total = (int64_t)a.PX * a.PY * a.N;
-    if (ray >= total) return;
-    const int k = (int)(ray % a.N);
-    const int64_t p = ray / a.N;
-    const int px = (int)(p % a.PX), py = (int)(p / a.PX);
-    const float2 dir = a.dirs[k];
-    const float cx = ((float)px + 0.5f) * a.spacing, cy = ((float)py + 0.5f) * a.spacing;
-    hit[ray] = march_ray_spec<SPEC>(sdf_rsrc(a.sdf, a.W, a.H), a.W, a.H, cx, cy, dir.x, dir.y, a.t0, a.t1, true);
-    index[ray] = ((int64_t)py * a.PX + px) * a.N + k;
+    const int ray = (int)(blockIdx.x * kBlock + threadIdx.x);
+    if (ray >= l.PX * l.PY * l.N) return;
+    int px, py, k;
+    decode_ray(l, ray, px, py, k);
+    index[ray] = ((int64_t)py * l.PX + px) * l.N + k;
 }
 
 template <Storage S>
@@ -591,17 +598,12 @@ hipError_t launch_level_to_f32(const void *level, Storage st, size_t texels, flo
     return hipGetLastError();
 }
 
-hipError_t launch_march_probe(const LevelArgs &a, int spec, int32_t *hit, int64_t *index, hipStream_t s)
+hipError_t launch_ray_decode(const MarchLevel &l, int64_t *index, hipStream_t s)
 {
-    const int64_t total = (int64_t)a.PX * a.PY * a.N;
+    const int64_t total = (int64_t)l.PX * l.PY * l.N;
     const unsigned blocks = (unsigned)((total + kBlock - 1) / kBlock);
     if (blocks == 0) return hipSuccess;
-    switch (spec) {
-    case 1: hipLaunchKernelGGL(k_march_probe<1>, dim3(blocks), dim3(kBlock), 0, s, a, hit, index); break;
-    case 2: hipLaunchKernelGGL(k_march_probe<2>, dim3(blocks), dim3(kBlock), 0, s, a, hit, index); break;
-    case 4: hipLaunchKernelGGL(k_march_probe<4>, dim3(blocks), dim3(kBlock), 0, s, a, hit, index); break;
-    default: return hipErrorInvalidValue;
-    }
+    hipLaunchKernelGGL(k_ray_decode, dim3(blocks), dim3(kBlock), 0, s, l, index);
     return hipGetLastError();
 }
 

@@ -200,7 +200,12 @@ rc_status load(const char *path, std::vector<rc_plane> &planes, std::vector<std:
         p.width = get32(r + 24);
         p.reserved = 0;
         const uint64_t off = get64(r + 32), cnt = get64(r + 40);
-        if (cnt != (uint64_t)p.channels * p.height * p.width || off % 4 || off + 4 * cnt > bytes)
+        // every product and sum is checked without wrapping: a record whose
+        // sizes overflow 64 bits, or whose values run past the payload, is refused
+        uint64_t prod = 0;
+        const bool wraps = __builtin_mul_overflow((uint64_t)p.channels, (uint64_t)p.height, &prod) ||
+                           __builtin_mul_overflow(prod, (uint64_t)p.width, &prod);
+        if (wraps || cnt != prod || off % 4 || off > bytes || cnt > (bytes - off) / 4)
             return dfail(RC_ERR_INVALID, "rc_dump: bad plane record " + names[i]);
         p.data = (const float *)(uintptr_t)off;  // offset until handed out
     }
@@ -228,7 +233,10 @@ rc_status rc_dump_write(const char *path, const rc_plane *planes, uint32_t n)
     uint64_t off = 0;
     for (uint32_t i = 0; i < n; ++i) {
         const rc_plane &p = planes[i];
-        const uint64_t cnt = (uint64_t)p.channels * p.height * p.width;
+        uint64_t cnt = 0;
+        if (__builtin_mul_overflow((uint64_t)p.channels, (uint64_t)p.height, &cnt) ||
+            __builtin_mul_overflow(cnt, (uint64_t)p.width, &cnt) || cnt > (1ull << 38) || off + 4 * cnt > (1ull << 40))
+            return dfail(RC_ERR_INVALID, "rc_dump_write: plane too large");
         if (cnt && !p.data) return dfail(RC_ERR_NULL, "rc_dump_write: plane without data");
         for (uint32_t j = 0; j < i; ++j)
             if (strncmp(planes[j].name, p.name, 16) == 0) return dfail(RC_ERR_INVALID, "rc_dump_write: duplicate plane name");

@@ -54,6 +54,7 @@ struct LevelArgs {
     int UX, UY;              // probe grid of level i+1
     float t0, t1, spacing;
     const int *codes;        // pre-marched ray codes of this level, or nullptr (march in place)
+    int *dbg_codes;          // in-place march: each ray's code also stored here (canonical order), or nullptr
 };
 
 // k_march_all: every level >= 1 in one launch (level-major, top level first).
@@ -76,7 +77,8 @@ hipError_t launch_level(const LevelArgs &a, Storage st, int shape, int rays_per_
 hipError_t launch_level0_gather(const LevelArgs &a, Storage st, float *irradiance, float *debug_fluence,
                                 hipStream_t s);
 hipError_t launch_level_to_f32(const void *level, Storage st, size_t texels, float *out, hipStream_t s);
-hipError_t launch_march_probe(const LevelArgs &a, int spec, int32_t *hit, int64_t *index, hipStream_t s);
+// canonical ray index of each slot as the split march decodes it (parity readback)
+hipError_t launch_ray_decode(const MarchLevel &l, int64_t *index, hipStream_t s);
 
 // ---- error norms (rc_norms.hip) ---------------------------------------------
 size_t norm_workspace_bytes();

@@ -834,7 +834,7 @@ __global__ __launch_bounds__(kBlock) void k_surr_mlp(SurrMlpArgs a, MlpDev m)
             const float sc = (red[0][1] + red[1][1]) + (red[2][1] + red[3][1]);
             const float mean = sc > 0.f ? se / sc : 0.f;
             a.tile_err[tile] = mean;
-            if (mean > a.threshold) {
+            if (!(mean <= a.threshold)) {  // a NaN estimate counts as flagged
                 const int slot = atomicAdd(a.flag_count, 1);
                 a.flag_list[slot] = tile;
             }
@@ -903,7 +903,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_patch(const T *__restrict__ pla
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= PX * PY * 4) return;
     const int probe = i >> 2, px = probe % PX, py = probe / PX;
-    if (tile_err[(py >> 4) * tiles_x + (px >> 4)] > threshold) surr[i] = plain[i];
+    if (!(tile_err[(py >> 4) * tiles_x + (px >> 4)] <= threshold)) surr[i] = plain[i];  // NaN: flagged
 }
 
 hipError_t launch_tile_patch(const void *plain, void *surr, const float *tile_err, float threshold, int PX, int PY,

@@ -202,10 +202,14 @@ class RadianceCascadeStage:
         check(self._lib.rc_debug_level(self._ctx, level, _dev_ptr(out, n, "out"), _stream_handle(stream)),
               self._ctx)
 
-    def debug_march(self, level: int, hit, index, stream=None) -> None:
+    def debug_march(self, level: int, hit, index=None, stream=None) -> None:
+        """The ray codes the last frame's kernels stored for `level` (int32
+        device tensor, canonical order); `index` (int64, optional, level >= 1)
+        receives the split march's decode of every slot (rc_debug_march)."""
         rays = level_rays(self.width, self.height, level)
         check(self._lib.rc_debug_march(self._ctx, level, _dev_ptr(hit, rays * 4, "hit"),
-                                       _dev_ptr(index, rays * 8, "index"), _stream_handle(stream)), self._ctx)
+                                       None if index is None else _dev_ptr(index, rays * 8, "index"),
+                                       _stream_handle(stream)), self._ctx)
 
     def composite(self, albedo, emissive, irradiance, out, direct=None, stream=None) -> None:
         """The presented frame: out [H][W] uint32 (int32 tensor) of bytes B, G, R,

@@ -74,6 +74,48 @@ int main(int argc, char **argv)
     for (size_t len = 0; len < raw.size(); ++len)
         if (try_file(std::vector<uint8_t>(raw.begin(), raw.begin() + len))) ++accepted;
     bad |= accepted != 0;
+
+    // Crafted files whose checksum is valid but whose plane record has sizes
+    // that wrap in 64 bits or run past the payload: the checksum cannot
+    // refuse them, the record checks must.
+    struct Crafted {
+        uint32_t channels, height, width;
+        uint64_t off, cnt;
+    };
+    const Crafted crafted[] = {
+        {233u, 236248574u, 2429587879u, ~0ull - 3, (1ull << 62) + 2},  // c*h*w and 4*cnt both wrap
+        {1u, 1u, 2u, 0, 2},                                             // past an 4-byte payload
+        {1u, 1u, 1u, 4, 1},                                             // offset at the payload's end
+        {1u, 1u, 1u, 8, 1},                                             // offset beyond the payload
+        {1u, 1u, 1u << 31, 0, 1ull << 31},                              // count past the payload
+        {1u << 31, 1u << 31, 4u, 0, 0},                                 // product wraps to 0
+    };
+    int crafted_accepted = 0;
+    for (const Crafted &cr : crafted) {
+        std::vector<uint8_t> rec(48, 0), payload(4, 0x11);
+        memcpy(rec.data(), "emissive", 8);
+        auto put32 = [&](size_t at, uint32_t v) { for (int k = 0; k < 4; ++k) rec[at + k] = (uint8_t)(v >> (8 * k)); };
+        auto put64 = [&](std::vector<uint8_t> &v8, size_t at, uint64_t v) {
+            for (int k = 0; k < 8; ++k) v8[at + k] = (uint8_t)(v >> (8 * k));
+        };
+        put32(16, cr.channels);
+        put32(20, cr.height);
+        put32(24, cr.width);
+        put64(rec, 32, cr.off);
+        put64(rec, 40, cr.cnt);
+        std::vector<uint8_t> body = rec;
+        body.insert(body.end(), payload.begin(), payload.end());
+        std::vector<uint8_t> file(56, 0);
+        memcpy(file.data(), "RCDUMP01", 8);
+        file[8] = 1;   // version
+        file[12] = 1;  // one plane
+        put64(file, 16, payload.size());
+        rc_sha256(body.data(), body.size(), file.data() + 24);
+        file.insert(file.end(), body.begin(), body.end());
+        if (try_file(file)) ++crafted_accepted;
+    }
+    bad |= crafted_accepted != 0;
+    accepted += crafted_accepted;
     printf(bad ? "dump driver: FAILED (%d accepted)\n" : "dump driver: ok\n", accepted);
     return bad;
 }

@@ -88,6 +88,13 @@ def test_surrogate_frame_parity(rc, W, H, L, seed, nb, ne):
         sdf = torch.empty((H, W), dtype=torch.float32, device="cuda:0")
         st.debug_surrogate(cone, a1, terr)
         st.debug_sdf(sdf)
+        codes = []
+        for lvl in (0, 1):
+            h = torch.empty(rc.level_rays(W, H, lvl), dtype=torch.int32, device="cuda:0")
+            st.debug_march(lvl, h)
+            codes.append(h)
+        with pytest.raises(rc.RcError):   # the learned frame marches no level >= 2
+            st.debug_march(2, torch.empty(rc.level_rays(W, H, 2), dtype=torch.int32, device="cuda:0"))
         mips = []
         top = so.top_level(W, H)
         for m in range(1, top + 1):
@@ -103,6 +110,12 @@ def test_surrogate_frame_parity(rc, W, H, L, seed, nb, ne):
     ref_irr, ref = so.run(d, e, L, params, mean, istd, float(scale), want=True)
     # near distance field: bit-exact
     assert np.array_equal(sdf.cpu().numpy(), ref["sdf"].astype(np.float32))
+    # levels 0 and 1 marched in place on it: every ray code bit-exact
+    from oracle import rco
+    for lvl, h in enumerate(codes):
+        _, hit_ref = rco.march_level(ref["sdf"].astype(np.float32), e, lvl)
+        assert np.array_equal(h.cpu().numpy(), hit_ref.ravel()), f"learned frame, level {lvl} ray codes"
+
     for m, (eg, og) in enumerate(mips, start=1):
         re, ro = ref["levels"][m]
         assert np.allclose(eg.cpu().numpy(), re, rtol=MIP_RTOL, atol=1e-6), m

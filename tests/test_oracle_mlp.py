@@ -105,3 +105,44 @@ def test_adam_decreases_loss():
         _, g4 = mlp.loss_and_grad_out(z4, T)
         opt.step(params, mlp.backward(params, cache, g4, False))
     assert _loss(params, X, T) < 0.5 * l0
+
+
+def test_adam_step1_closed_form():
+    """Step 1 of Adam with bias correction is p - lr * g / (|g| + eps) for every
+    element (m/c1 = g, v/c2 = g^2): a missing or mis-powered bias correction
+    moves p by a different amount."""
+    rng = np.random.default_rng(3)
+    params = [[rng.standard_normal((4, 3)), rng.standard_normal(3)]]
+    p0 = [[a.copy() for a in layer] for layer in params]
+    grads = [[rng.standard_normal((4, 3)), rng.standard_normal(3) * 1e-3]]
+    lr, eps = 2e-3, 1e-8
+    mlp.Adam(params, lr=lr, eps=eps).step(params, grads)
+    for j in range(2):
+        g = grads[0][j]
+        np.testing.assert_allclose(params[0][j], p0[0][j] - lr * g / (np.abs(g) + eps), rtol=0, atol=1e-15)
+
+
+def test_adam_matches_torch_optim():
+    """Six steps with changing gradients and a changing learning rate agree
+    with torch.optim.Adam (a library, FP64) to 1e-12 relative."""
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(11)
+    shapes = [(5, 4), (4,), (4, 2), (2,)]
+    params = [[rng.standard_normal(shapes[0]), rng.standard_normal(shapes[1])],
+              [rng.standard_normal(shapes[2]), rng.standard_normal(shapes[3])]]
+    tp = [torch.tensor(a, dtype=torch.float64, requires_grad=True) for layer in params for a in layer]
+    opt_t = torch.optim.Adam(tp, lr=1e-3, betas=(0.9, 0.999), eps=1e-8)
+    opt_o = mlp.Adam(params, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8)
+    for step in range(6):
+        lr = 1e-3 * (0.7 ** step)
+        grads = [[rng.standard_normal(shapes[0]) * (step + 1), rng.standard_normal(shapes[1])],
+                 [rng.standard_normal(shapes[2]), rng.standard_normal(shapes[3]) * 1e-4]]
+        for t, g in zip(tp, [g for layer in grads for g in layer]):
+            t.grad = torch.tensor(g, dtype=torch.float64)
+        for group in opt_t.param_groups:
+            group["lr"] = lr
+        opt_t.step()
+        opt_o.step(params, grads, lr=lr)
+        for t, a in zip(tp, [a for layer in params for a in layer]):
+            ref = t.detach().numpy()
+            assert np.max(np.abs(a - ref)) <= 1e-12 * max(np.max(np.abs(ref)), 1.0)
