@@ -43,7 +43,7 @@
 extern "C" {
 #endif
 
-#define RC_ABI_VERSION 1u
+#define RC_ABI_VERSION 2u  /* 2: rc_config carries the depth convention */
 #define RC_MAX_LEVELS 8
 #define RC_MAX_DIM 16384
 #define RC_MAX_BOXES 256
@@ -71,6 +71,19 @@ typedef enum { RC_STORAGE_F32 = 0, RC_STORAGE_F16 = 1, RC_STORAGE_BF16 = 2 } rc_
                                     between frames, so direct launches are faster whenever
                                     the host runs ahead of the GPU */
 
+/* rc_config.depth_mode: what the depth plane holds.
+ *   RC_DEPTH_LINEAR: linear view depth D (distance along the optical axis).
+ *   RC_DEPTH_NDC:    the reference renderer's Depth32Float value, NDC depth d
+ *                    in [0,1] of glam's perspective_rh (src/camera.rs:77-79,
+ *                    src/texture.rs:149): d = f (D - n) / ((f - n) D) for the
+ *                    near / far planes n = z_near, f = z_far (the reference
+ *                    uses 0.1 and 100, src/camera.rs:26 via src/app.rs).
+ *                    The pass converts every depth it reads to
+ *                    D = (n f) / (f - d (f - n)) in fp32, n f and f - n
+ *                    rounded to fp32 first, then applies the linear rule. */
+#define RC_DEPTH_LINEAR 0u
+#define RC_DEPTH_NDC    1u
+
 typedef struct {
     uint32_t version;   /* RC_ABI_VERSION */
     uint32_t width;     /* 1 .. RC_MAX_DIM */
@@ -79,10 +92,12 @@ typedef struct {
     uint32_t storage;   /* rc_storage */
     uint32_t flags;     /* RC_FLAG_* */
     int32_t device;     /* HIP device ordinal */
-    uint32_t reserved;
+    uint32_t depth_mode;  /* RC_DEPTH_LINEAR or RC_DEPTH_NDC */
+    float z_near;       /* RC_DEPTH_NDC: near plane, 0 < z_near < z_far (ignored for linear depth) */
+    float z_far;        /* RC_DEPTH_NDC: far plane, finite */
 } rc_config;
 
-/* Planar fp32 G-buffer: depth [H][W] (linear view depth), emissive [3][H][W]
+/* Planar fp32 G-buffer: depth [H][W] (rc_config.depth_mode), emissive [3][H][W]
  * (radiance), albedo [3][H][W].  Albedo is not read by the GI pass (GI is
  * emission transport; compositing multiplies by albedo outside the pass) and
  * may be NULL. */

@@ -160,6 +160,28 @@ def dir_table_f32(level: int) -> np.ndarray:
     return d
 
 
+# ---- depth convention ---------------------------------------------------------
+def ndc_from_linear(depth: np.ndarray, z_near: float, z_far: float) -> np.ndarray:
+    """The reference renderer's Depth32Float value for linear view depth D:
+    NDC depth of glam's perspective_rh (src/camera.rs:77-79; clip z = r (n - D)
+    with r = f / (n - f), clip w = D), d = f (D - n) / ((f - n) D), evaluated in
+    FP64 and rounded to fp32 as the depth target stores it."""
+    D = np.asarray(depth, np.float64)
+    n, f = float(z_near), float(z_far)
+    return (f * (D - n) / ((f - n) * D)).astype(np.float32)
+
+
+def linear_from_ndc(d: np.ndarray, z_near: float, z_far: float) -> np.ndarray:
+    """Linear view depth as the pass reads an NDC depth plane (RC_DEPTH_NDC):
+    D = (n f) / (f - d (f - n)) in fp32, with n f and f - n each rounded to
+    fp32 first and every operation rounded once."""
+    d = np.asarray(d, np.float32)
+    n, f = np.float32(z_near), np.float32(z_far)
+    nf = np.float32(n * f)
+    fmn = np.float32(f - n)
+    return (nf / (f - d * fmn)).astype(np.float32)
+
+
 # ---- occupancy / jump flood -------------------------------------------------
 def jfa_num_passes(W: int, H: int) -> int:
     return lib().rco_jfa_num_passes(W, H)

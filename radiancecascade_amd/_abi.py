@@ -14,7 +14,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "librcgi.so")
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 MAX_LEVELS = 8
 MAX_DIM = 16384
 MAX_BOXES = 256
@@ -44,7 +44,11 @@ TRAIN_MAX_EPOCHS = 64
 class Config(ctypes.Structure):
     _fields_ = [("version", ctypes.c_uint32), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
                 ("levels", ctypes.c_uint32), ("storage", ctypes.c_uint32), ("flags", ctypes.c_uint32),
-                ("device", ctypes.c_int32), ("reserved", ctypes.c_uint32)]
+                ("device", ctypes.c_int32), ("depth_mode", ctypes.c_uint32), ("z_near", ctypes.c_float),
+                ("z_far", ctypes.c_float)]
+
+DEPTH_LINEAR = 0
+DEPTH_NDC = 1
 
 
 class Plane(ctypes.Structure):

@@ -24,6 +24,7 @@ struct rc_ctx {
     rc_config cfg{};
     int W = 0, H = 0, L = 0;
     Storage storage = Storage::F32;
+    DepthConv depth_conv{};
     uint32_t *seed[2] = {nullptr, nullptr};
     uint8_t *sdf = nullptr;  // distance codes (rc_common.hpp)
     void *level[RC_MAX_LEVELS] = {};
@@ -370,7 +371,7 @@ static rc_status run_frame(rc_ctx *c, const float *depth, const float *emissive,
     for (auto &u : c->ev_used) u = false;
 
     tm.begin(RC_STAGE_SEED);
-    HIP_TRY(c, launch_seed_init(depth, emissive, c->seed[0], c->emis4, c->W, c->H, s));
+    HIP_TRY(c, launch_seed_init(depth, c->depth_conv, emissive, c->seed[0], c->emis4, c->W, c->H, s));
     tm.end(RC_STAGE_SEED);
 
     // Jump flood: global passes down to the fused LDS tail (if any); without a
@@ -546,7 +547,7 @@ static rc_status run_frame_surrogate(rc_ctx *c, const float *depth, const float 
     StageTimer tm{c, s, prof};
     for (auto &u : c->ev_used) u = false;
     tm.begin(RC_STAGE_SEED);
-    HIP_TRY(c, launch_seed_surr(depth, emissive, c->occ_bits, c->mip, c->W, c->H, s));
+    HIP_TRY(c, launch_seed_surr(depth, c->depth_conv, emissive, c->occ_bits, c->mip, c->W, c->H, s));
     tm.end(RC_STAGE_SEED);
     // One stream: work that could overlap shares a launch instead (kernels
     // on several captured streams wait microseconds for each other across
@@ -718,7 +719,7 @@ static rc_status gen_samples(rc_ctx *c, const float *depth, const float *emissiv
     if (!c->scratch_irr) HIP_TRY(c, hipMalloc(&c->scratch_irr, 3 * (size_t)c->W * c->H * sizeof(float)));
     st = run_frame(c, depth, emissive, c->scratch_irr, s, true);
     if (st != RC_OK) return st;
-    HIP_TRY(c, launch_seed_surr(depth, emissive, c->occ_bits, c->mip, c->W, c->H, s));
+    HIP_TRY(c, launch_seed_surr(depth, c->depth_conv, emissive, c->occ_bits, c->mip, c->W, c->H, s));
     HIP_TRY(c, launch_mip_top(c->mip, c->flag_count, s));
     st = launch_cones(c, s);
     if (st != RC_OK) return st;
@@ -815,6 +816,10 @@ rc_status rc_create(const rc_config *cfg, rc_ctx **out)
     if (cfg->levels > RC_MAX_LEVELS) return fail(nullptr, RC_ERR_INVALID, "rc_create: levels out of range");
     if (cfg->storage > RC_STORAGE_BF16) return fail(nullptr, RC_ERR_INVALID, "rc_create: unknown storage");
     if (cfg->flags & ~(RC_FLAG_PROFILE | RC_FLAG_KEEP_DEBUG | RC_FLAG_GRAPH)) return fail(nullptr, RC_ERR_INVALID, "rc_create: unknown flags");
+    if (cfg->depth_mode > RC_DEPTH_NDC) return fail(nullptr, RC_ERR_INVALID, "rc_create: unknown depth_mode");
+    if (cfg->depth_mode == RC_DEPTH_NDC &&
+        !(cfg->z_near > 0.0f && cfg->z_far > cfg->z_near && cfg->z_far < __builtin_huge_valf()))
+        return fail(nullptr, RC_ERR_INVALID, "rc_create: NDC depth needs 0 < z_near < z_far < inf");
     int ndev = 0;
     if (rc_device_count(&ndev) != RC_OK) return RC_ERR_NO_DEVICE;
     if (cfg->device < 0 || cfg->device >= ndev) return fail(nullptr, RC_ERR_INVALID, "rc_create: device ordinal out of range");
@@ -825,6 +830,11 @@ rc_status rc_create(const rc_config *cfg, rc_ctx **out)
     if (!c) return fail(nullptr, RC_ERR_OOM, "rc_create: host allocation failed");
     c->cfg = *cfg;
     c->storage = (Storage)cfg->storage;
+    if (cfg->depth_mode == RC_DEPTH_NDC) {
+        // fp32 constants of D = (n f) / (f - d (f - n)), each rounded once
+        const float n = cfg->z_near, f = cfg->z_far;
+        c->depth_conv = DepthConv{1, n * f, f, f - n};
+    }
     const uint32_t L = cfg->levels ? cfg->levels : coverage_levels(cfg->width, cfg->height);
     c->cfg.levels = L;
     for (auto &ev : c->ev) {

@@ -13,13 +13,24 @@ void scene_generate(uint64_t seed, int n_boxes, int n_emissive, rc_box *boxes);
 hipError_t launch_raster(const double *boxes_dev, int n, int W, int H, float *depth, float *albedo, float *emissive,
                          hipStream_t stream);
 
+// ---- depth convention ---------------------------------------------------------
+// Linear view depth as the occupancy rule reads it: the plane itself, or the
+// reference renderer's NDC depth d converted by D = nf / (f - d * fmn) with
+// nf = n*f and fmn = f - n rounded to fp32 on the host (each operation rounds
+// once: the library is built without contraction).
+struct DepthConv {
+    int ndc;
+    float nf, f, fmn;
+};
+__device__ inline float linear_depth(const DepthConv &c, float d) { return c.ndc ? c.nf / (c.f - d * c.fmn) : d; }
+
 // ---- occupancy + jump flood (rc_jfa.hip) --------------------------------------
 int jfa_num_passes(int W, int H);
 // largest step fused into the LDS tail for a requested fuse limit (0 = none)
 int jfa_tail_step(int W, int H, int fuse);
 hipError_t launch_sdf_untile(const uint8_t *sdf, float *out, int W, int H, hipStream_t s);
-hipError_t launch_seed_init(const float *depth, const float *emissive, uint32_t *seed, float4 *emis4, int W, int H,
-                            hipStream_t s);
+hipError_t launch_seed_init(const float *depth, const DepthConv &dc, const float *emissive, uint32_t *seed,
+                            float4 *emis4, int W, int H, hipStream_t s);
 hipError_t launch_jfa_pass(const uint32_t *in, uint32_t *out, uint8_t *sdf, int W, int H, int step, bool final_pass,
                            hipStream_t s);
 hipError_t launch_jfa_tail(const uint32_t *in, uint32_t *seeds_out, uint8_t *sdf, int W, int H, int tail_step,
@@ -194,8 +205,8 @@ size_t mip_cells(const MipArgs &m);
 // near: steps with t < kConeSplit; far: the rest of the same sequence
 void cone_schedule(int W, int H, const MipArgs &m, ConeSchedule &near, ConeSchedule &far);
 int cone_top_level(int W, int H);
-hipError_t launch_seed_surr(const float *depth, const float *emissive, uint64_t *bits, const MipArgs &m, int W, int H,
-                            hipStream_t s);  // occupancy bits + mip levels 1..6
+hipError_t launch_seed_surr(const float *depth, const DepthConv &dc, const float *emissive, uint64_t *bits,
+                            const MipArgs &m, int W, int H, hipStream_t s);  // occupancy bits + mip levels 1..6
 hipError_t launch_near_sdf(const uint64_t *bits, uint8_t *sdf, int W, int H, hipStream_t s);
 hipError_t launch_mip_top(const MipArgs &m, int *zero_me, hipStream_t s);    // levels 7..top
 hipError_t launch_mip_unpack(const MipArgs &m, int level, float *emission, float *coverage, hipStream_t s);

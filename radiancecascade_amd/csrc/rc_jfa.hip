@@ -76,7 +76,7 @@ constexpr int kSeedRows = 16;
 // 16-byte load instead of one 4-byte load from each of three planes 8 MB
 // apart.  Only occupied pixels are written: a ray hits where the distance is
 // 0, i.e. on a seed, and seeds are the occupied pixels.
-__global__ __launch_bounds__(kBlock) void k_seed_init(const float *__restrict__ depth,
+__global__ __launch_bounds__(kBlock) void k_seed_init(const float *__restrict__ depth, DepthConv dc,
                                                       const float *__restrict__ emissive,
                                                       uint32_t *__restrict__ seed, float4 *__restrict__ emis4, int W,
                                                       int H, int bx_count)
@@ -87,8 +87,8 @@ __global__ __launch_bounds__(kBlock) void k_seed_init(const float *__restrict__ 
     if (x >= W) return;
     const size_t plane = (size_t)W * H;
     // all loads first (clamped addresses; out-of-image rows are masked below)
-    const float du0 = depth[(size_t)max(y0 - 1, 0) * W + x];
-    const float dd3 = depth[(size_t)min(y0 + 4, H - 1) * W + x];
+    float du0 = depth[(size_t)max(y0 - 1, 0) * W + x];
+    float dd3 = depth[(size_t)min(y0 + 4, H - 1) * W + x];
     float4 e[4];
     float dp[4], dl[4], dr[4];
 #pragma unroll
@@ -98,6 +98,16 @@ __global__ __launch_bounds__(kBlock) void k_seed_init(const float *__restrict__ 
         dp[j] = depth[p];
         dl[j] = depth[p - (x > 0)];
         dr[j] = depth[p + (x + 1 < W)];
+    }
+    if (dc.ndc) {
+        du0 = linear_depth(dc, du0);
+        dd3 = linear_depth(dc, dd3);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            dp[j] = linear_depth(dc, dp[j]);
+            dl[j] = linear_depth(dc, dl[j]);
+            dr[j] = linear_depth(dc, dr[j]);
+        }
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -300,11 +310,11 @@ hipError_t launch_sdf_untile(const uint8_t *sdf, float *out, int W, int H, hipSt
     return hipGetLastError();
 }
 
-hipError_t launch_seed_init(const float *depth, const float *emissive, uint32_t *seed, float4 *emis4, int W, int H,
-                            hipStream_t s)
+hipError_t launch_seed_init(const float *depth, const DepthConv &dc, const float *emissive, uint32_t *seed,
+                            float4 *emis4, int W, int H, hipStream_t s)
 {
     const int bxc = (W + 63) / 64, byc = (H + kSeedRows - 1) / kSeedRows;
-    hipLaunchKernelGGL(k_seed_init, dim3(bxc * byc), dim3(kBlock), 0, s, depth, emissive, seed, emis4, W, H, bxc);
+    hipLaunchKernelGGL(k_seed_init, dim3(bxc * byc), dim3(kBlock), 0, s, depth, dc, emissive, seed, emis4, W, H, bxc);
     return hipGetLastError();
 }
 

@@ -252,7 +252,7 @@ __device__ inline uint4 shfl_xor4(uint4 v, int mask)
                       __shfl_xor(v.w, mask, 64));
 }
 
-__global__ __launch_bounds__(kSeedMipBlock) void k_seed_surr(const float *__restrict__ depth,
+__global__ __launch_bounds__(kSeedMipBlock) void k_seed_surr(const float *__restrict__ depth, DepthConv dc,
                                                              const float *__restrict__ emissive,
                                                              uint64_t *__restrict__ bits, int words, MipArgs m, int W,
                                                              int H, int bx_count)
@@ -268,8 +268,8 @@ __global__ __launch_bounds__(kSeedMipBlock) void k_seed_surr(const float *__rest
     const size_t plane = (size_t)W * H;
     float e[4][3], dp[4], dl[4], dr[4];
     // all loads first (clamped addresses; out-of-image pixels are masked below)
-    const float du0 = depth[(size_t)min(max(y0 - 1, 0), H - 1) * W + xc];
-    const float dd3 = depth[(size_t)min(y0 + 4, H - 1) * W + xc];
+    float du0 = depth[(size_t)min(max(y0 - 1, 0), H - 1) * W + xc];
+    float dd3 = depth[(size_t)min(y0 + 4, H - 1) * W + xc];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const int y = min(y0 + j, H - 1);
@@ -280,6 +280,16 @@ __global__ __launch_bounds__(kSeedMipBlock) void k_seed_surr(const float *__rest
         dp[j] = depth[p];
         dl[j] = depth[p - (xc > 0)];
         dr[j] = depth[p + (xc + 1 < W)];
+    }
+    if (dc.ndc) {
+        du0 = linear_depth(dc, du0);
+        dd3 = linear_depth(dc, dd3);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            dp[j] = linear_depth(dc, dp[j]);
+            dl[j] = linear_depth(dc, dl[j]);
+            dr[j] = linear_depth(dc, dr[j]);
+        }
     }
     float4 px[4];  // (rgb, kind bits) of the lane's 4 pixels
 #pragma unroll
@@ -323,12 +333,12 @@ __global__ __launch_bounds__(kSeedMipBlock) void k_seed_surr(const float *__rest
     if (m.levels >= 3) mip_mid_levels(se, m, bx, by);
 }
 
-hipError_t launch_seed_surr(const float *depth, const float *emissive, uint64_t *bits, const MipArgs &m, int W, int H,
-                            hipStream_t s)
+hipError_t launch_seed_surr(const float *depth, const DepthConv &dc, const float *emissive, uint64_t *bits,
+                            const MipArgs &m, int W, int H, hipStream_t s)
 {
     const int bxc = (W + 63) / 64, byc = (H + kSeedMipPx - 1) / kSeedMipPx;
-    hipLaunchKernelGGL(k_seed_surr, dim3(bxc * byc), dim3(kSeedMipBlock), 0, s, depth, emissive, bits, (W + 63) / 64, m,
-                       W, H, bxc);
+    hipLaunchKernelGGL(k_seed_surr, dim3(bxc * byc), dim3(kSeedMipBlock), 0, s, depth, dc, emissive, bits,
+                       (W + 63) / 64, m, W, H, bxc);
     return hipGetLastError();
 }
 

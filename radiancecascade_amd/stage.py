@@ -75,15 +75,21 @@ class RadianceCascadeStage:
     """The GI pass: jump flood -> per-level march/merge -> gather."""
 
     def __init__(self, width: int, height: int, levels: int = 0, storage: str = "f32",
-                 device: int = 0, profile: bool = False, debug: bool = False, graph: bool = False):
+                 device: int = 0, profile: bool = False, debug: bool = False, graph: bool = False,
+                 ndc_depth: tuple[float, float] | None = None):
+        """ndc_depth=(z_near, z_far): the depth plane holds the reference
+        renderer's Depth32Float NDC depth of perspective_rh (src/camera.rs:77-79)
+        instead of linear view depth; the pass converts it (RC_DEPTH_NDC)."""
         lib = _abi.load()
         if storage not in _abi.STORAGE:
             raise ValueError(f"storage must be one of {sorted(_abi.STORAGE)}")
+        near, far = ndc_depth if ndc_depth is not None else (0.0, 0.0)
         cfg = _abi.Config(version=_abi.ABI_VERSION, width=width, height=height, levels=levels,
                           storage=_abi.STORAGE[storage],
                           flags=(_abi.FLAG_PROFILE if profile else 0) | (_abi.FLAG_KEEP_DEBUG if debug else 0)
                           | (_abi.FLAG_GRAPH if graph else 0),
-                          device=device, reserved=0)
+                          device=device, depth_mode=_abi.DEPTH_NDC if ndc_depth is not None else _abi.DEPTH_LINEAR,
+                          z_near=near, z_far=far)
         ctx = ctypes.c_void_p()
         check(lib.rc_create(ctypes.byref(cfg), ctypes.byref(ctx)))
         self._lib = lib

@@ -42,10 +42,15 @@ def test_create_argument_validation():
     lib = _abi.load()
     ctx = ctypes.c_void_p()
     assert lib.rc_create(None, ctypes.byref(ctx)) == _abi.ERR_NULL
-    bad = [dict(version=99), dict(width=0), dict(height=_abi.MAX_DIM + 1), dict(levels=9),
-           dict(storage=3), dict(flags=0x80)]
+    bad = [dict(version=99), dict(version=1), dict(width=0), dict(height=_abi.MAX_DIM + 1), dict(levels=9),
+           dict(storage=3), dict(flags=0x80), dict(depth_mode=2),
+           dict(depth_mode=_abi.DEPTH_NDC, z_near=0.0, z_far=100.0),
+           dict(depth_mode=_abi.DEPTH_NDC, z_near=1.0, z_far=1.0),
+           dict(depth_mode=_abi.DEPTH_NDC, z_near=0.1, z_far=float("inf")),
+           dict(depth_mode=_abi.DEPTH_NDC, z_near=float("nan"), z_far=100.0)]
     for b in bad:
-        kw = dict(version=_abi.ABI_VERSION, width=64, height=64, levels=3, storage=0, flags=0, device=0, reserved=0)
+        kw = dict(version=_abi.ABI_VERSION, width=64, height=64, levels=3, storage=0, flags=0, device=0,
+                  depth_mode=0, z_near=0.0, z_far=0.0)
         kw.update(b)
         cfg = _abi.Config(**kw)
         assert lib.rc_create(ctypes.byref(cfg), ctypes.byref(ctx)) == _abi.ERR_INVALID, b

@@ -439,3 +439,39 @@ def test_error_norms_fp64(rc, oracle):
     assert a["max_abs"] == float(np.abs(diff).max())
     r32 = ref.astype(np.float32).astype(np.float64)
     assert c32["max_abs"] == float(np.abs(x.astype(np.float64) - r32).max())
+
+
+@pytest.mark.parametrize("W,H,L,seed,nb,ne", [(256, 256, 4, 7, 8, 2), (1920, 1080, 6, 1, 32, 4)])
+def test_ndc_depth_input(rc, oracle, W, H, L, seed, nb, ne):
+    """The reference renderer's depth target holds perspective_rh NDC depth
+    (Depth32Float, near 0.1, far 100: src/camera.rs:77-79, src/texture.rs:149).
+    Handed that plane with RC_DEPTH_NDC, the pass converts it in the seed
+    kernel: seeds, distance field and irradiance equal the oracle on the
+    converted depth bit for bit, and the occupancy equals the linear-depth
+    run's (the fp32 round trip moves no depth across the 5 % rule here)."""
+    _say(f"NDC depth {W}x{H} L={L} seed={seed}")
+    near, far = 0.1, 100.0
+    depth, emis = _device_gbuffer(rc, W, H, seed, nb, ne)
+    d_lin = depth.cpu().numpy()
+    d_ndc = oracle.ndc_from_linear(d_lin, near, far)
+    ndc_dev = torch.from_numpy(d_ndc).cuda()
+    irr_lin = torch.empty((3, H, W), dtype=torch.float32, device="cuda:0")
+    irr_ndc = torch.empty_like(irr_lin)
+    seeds_lin = torch.empty((H, W), dtype=torch.int32, device="cuda:0")
+    seeds_ndc = torch.empty_like(seeds_lin)
+    with rc.RadianceCascadeStage(W, H, levels=L, debug=True) as st:
+        st.run_device(depth, emis, irr_lin)
+        st.debug_seeds(seeds_lin)
+    with rc.RadianceCascadeStage(W, H, levels=L, debug=True, ndc_depth=(near, far)) as st:
+        st.run_device(ndc_dev, emis, irr_ndc)
+        st.debug_seeds(seeds_ndc)
+    torch.cuda.synchronize()
+    e = emis.cpu().numpy()
+    d_conv = oracle.linear_from_ndc(d_ndc, near, far)
+    irr32, seeds_ref = oracle.run(d_conv, e, L, np.float32, want_seeds=True)
+    assert np.array_equal(seeds_ndc.cpu().numpy().view(np.uint32), seeds_ref)
+    assert np.array_equal(irr_ndc.cpu().numpy().view(np.uint32), irr32.view(np.uint32))
+    n_diff = int((seeds_ndc != seeds_lin).sum().item())
+    _say(f"  seeds differing from the linear-depth run: {n_diff}")
+    assert n_diff == 0
+    assert torch.equal(irr_ndc, irr_lin)

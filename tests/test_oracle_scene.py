@@ -103,3 +103,26 @@ def test_depth_matches_slab_distance(oracle):
                 ok = inside & (t > 0)
                 best = np.where(ok & (t < best), t, best)
     assert np.allclose(d, best.astype(np.float32), rtol=1e-6, atol=0)
+
+
+def test_ndc_depth_conversion(oracle):
+    """perspective_rh NDC depth (src/camera.rs:77-79) and its inverse as the
+    pass evaluates it: the near and far planes map to 0 and 1, the inverse
+    is monotonic, and the fp32 round trip of depths in [0.15, 1] (the room's
+    are within [0.15, 0.87]) stays within 1e-6 relative."""
+    n, f = 0.1, 100.0
+    assert oracle.ndc_from_linear(np.array([n]), n, f)[0] == 0.0
+    assert oracle.ndc_from_linear(np.array([f]), n, f)[0] == 1.0
+    back = oracle.linear_from_ndc(np.array([0.0, 1.0], np.float32), n, f)
+    # at the far plane f - d (f - n) cancels to f - fp32(f - n): 1.5e-5 relative
+    assert abs(back[0] - n) <= 1e-6 * n and abs(back[1] - f) <= 2e-5 * f
+    D = np.linspace(0.15, 1.0, 20001)
+    d = oracle.ndc_from_linear(D, n, f)
+    lin = oracle.linear_from_ndc(d, n, f).astype(np.float64)
+    assert np.all(np.diff(lin) >= 0)
+    assert np.max(np.abs(lin - D) / D) <= 1e-6
+    # the G-buffer of a seeded scene keeps every occupancy bit
+    dep, _, e = oracle.gbuffer(7, 256, 256, 8, 2)
+    s_lin = oracle.seed_init(dep, e)
+    s_ndc = oracle.seed_init(oracle.linear_from_ndc(oracle.ndc_from_linear(dep, n, f), n, f), e)
+    assert np.array_equal(s_lin, s_ndc)
