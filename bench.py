@@ -41,68 +41,115 @@ def texel_bytes_contract(storage: str) -> int:
     return 16 if storage == "f32" else 8
 
 
-def kernel_alg_bytes(W, H, L, storage, rays):
-    """Algorithmic bytes per launch of each kernel, from the compulsory-traffic
-    model B_alg = W H (28 + 8 + 12) + 2 b sum_{i>=1} R_i split by the kernel
-    that moves each term: seed kernel reads the 28 B/px G-buffer; the JFA
-    tail writes the 4 B/px SDF; level i >= 1 writes merged level i (b R_i)
-    and reads merged level i+1 (b R_{i+1}); level 0 reads the SDF (4 B/px),
-    merged level 1 (b R_1) and writes 12 B/px irradiance."""
+# Kernel families of the plain frame and the stages (rc_get_stats) that
+# make them up.  The jump flood is one family whatever the split between
+# global passes and the fused LDS tail, so the dominant family cannot change
+# with the autotuned fusion depth.
+FAMILIES = {"seed": ["seed"], "jfa": ["jfa", "jfa_tail"], "march": ["march"], "levels": None, "level0": ["level0"]}
+
+
+def family_alg_bytes(W, H, L, storage, rays, split, jfa_passes):
+    """Algorithmic bytes per frame of each kernel family: the bytes its
+    kernels must move, charged to the kernel that moves them.
+      seed   : depth + emissive read (16 B/px), seed field written (4 B/px);
+      jfa    : 8 B/px per pass (seed field read and written once); the last
+               pass reads 4 B/px and writes the 1-byte distance field;
+      march  : the 1-byte distance field read once, a 4-byte code per ray of
+               levels >= 1 written;
+      levels : level i >= 1 writes its merged texels (b R_i / 4: the stored
+               form is the 4-child mean) and reads level i+1's (b R_{i+1} / 4)
+               and, after the split march, its 4-byte codes (4 R_i);
+      level0 : the distance field (1 B/px), merged level 1 (b R_1 / 4) and
+               the 12 B/px irradiance written.
+    b = bytes per stored texel (16 fp32, 8 fp16/bf16)."""
     b = texel_bytes_contract(storage)
     n = W * H
-    # a global jump-flood pass is charged the informative 8 B/px (seed field
-    # read once, written once); the contract itself has no JFA term.
-    out = {"seed": 28 * n, "jfa": 8 * n, "jfa_tail": 4 * n}
+    tex = [r // 4 for r in rays]
+    levels = 0
     for i in range(1, L):
-        out[f"level{i}"] = b * rays[i] + (b * rays[i + 1] if i + 1 < L else 0)
-    out["level0"] = 4 * n + 12 * n + (b * rays[1] if L > 1 else 0)
-    # split march of all levels >= 1: reads the SDF once (4 B/px) and produces
-    # one b-byte ray-interval result per ray (the march row of the model);
-    # the merge kernels are then charged the same contract terms as above.
-    out["march"] = 4 * n + b * sum(rays[1:L])
-    return out
+        levels += b * tex[i] + (b * tex[i + 1] if i + 1 < L else 0) + (4 * rays[i] if split else 0)
+    return {"seed": 20 * n,
+            "jfa": (8 * (jfa_passes - 1) + 5) * n if jfa_passes > 0 else 5 * n,
+            "march": (n + 4 * sum(rays[1:L])) if split else 0,
+            "levels": levels,
+            "level0": n + (b * tex[1] if L > 1 else 0) + 12 * n}
 
 
 def b_alg(W, H, L, storage, rays):
+    """The compulsory-traffic contract of the whole frame:
+    W H (28 G-buffer + 8 distance field + 12 irradiance) + 2 b sum_{i>=1} R_i."""
     return W * H * 48 + 2 * texel_bytes_contract(storage) * sum(rays[1:L])
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "frame_1080p_hbm_traffic.md")
-KERNEL_NAMES = {"march": "k_march_all", "jfa": "k_jfa_pass<false>", "seed": "k_seed_init", "level0": "k_level0<"}
 
 
-def profiled_traffic(stage, W, H, L, storage):
-    """HBM bytes per launch of the stage's kernel from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-kernel factor there),
-    for the workload that summary was taken on; None otherwise."""
-    if (W, H, L, storage) != (1920, 1080, 6, "f32") or stage not in KERNEL_NAMES or not os.path.exists(TRAFFIC_PROFILE):
+def profiled_traffic(family, W, H, L, storage):
+    """HBM bytes per frame of a kernel family from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (copy-calibrated, per-family table), for
+    the workload that summary was taken on; None otherwise."""
+    if (W, H, L, storage) != (1920, 1080, 6, "f32") or not os.path.exists(TRAFFIC_PROFILE):
         return None
     for line in open(TRAFFIC_PROFILE):
-        cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 5 and cells[0].strip("`").startswith("rc::" + KERNEL_NAMES[stage]):
+        cells = [c.strip().strip("`") for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 4 and cells[0] == f"family:{family}":
             try:
-                return (float(cells[3]) + float(cells[4])) * 1e6
+                return float(cells[3]) * 1e6
             except ValueError:
                 return None
     return None
 
 
-def cpu_baseline(depth, emissive, L, seconds):
-    """FP32 CPU restatement (OpenMP) on this host's cores: frames/s."""
-    from oracle import rco
-    threads = max(1, min(16, os.cpu_count() or 1))
-    rco.set_threads(threads)
-    rco.run(depth, emissive, L, np.float32)  # warm-up (page-in, thread pool)
+def _host_cpu():
+    """nproc, the cores this process may run on, and the CPU model name."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"nproc": os.cpu_count() or 1, "usable_cores": usable, "model": model}
+
+
+def _time_frames(fn, seconds, min_frames=1):
+    fn()  # warm-up (page-in, thread pool)
     frames, t0 = 0, time.perf_counter()
     while True:
-        rco.run(depth, emissive, L, np.float32)
+        fn()
         frames += 1
         el = time.perf_counter() - t0
-        if el >= seconds:
-            break
-    return {"value": frames / el, "unit": "GI frames/s", "cores": threads, "kind": "port",
-            "sample": f"{frames} full frames of the same workload in {el:.1f} s "
-                      f"(oracle FP32 flavour, OpenMP, {threads} threads)"}
+        if el >= seconds and frames >= min_frames:
+            return frames, el
+
+
+def cpu_baseline(depth, emissive, L, seconds):
+    """The CPU restatement of the same maths (the oracle, C) on this host:
+    FP32 with OpenMP on every core this process may use (the GPU box's share
+    for one GPU is given by OMP_NUM_THREADS when it is set), FP32 on one
+    core, and the FP64 flavour on one core.  Whole frames of the workload."""
+    from oracle import rco
+    host = _host_cpu()
+    threads = host["usable_cores"]
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        threads = max(1, min(threads, int(os.environ["OMP_NUM_THREADS"])))
+    rco.set_threads(threads)
+    fr, el = _time_frames(lambda: rco.run(depth, emissive, L, np.float32), seconds)
+    rco.set_threads(1)
+    f1, e1 = _time_frames(lambda: rco.run(depth, emissive, L, np.float32), seconds / 3)
+    f64, e64 = _time_frames(lambda: rco.run(depth, emissive, L, np.float64), seconds / 3)
+    rco.set_threads(threads)
+    return {"value": fr / el, "unit": "GI frames/s", "cores": threads, "kind": "port",
+            "sample": f"{fr} full frames of the same workload in {el:.1f} s (oracle FP32 flavour, OpenMP, "
+                      f"{threads} threads); single core: FP32 {f1} frames in {e1:.1f} s, FP64 {f64} frames "
+                      f"in {e64:.1f} s",
+            "single_core_fp32": f1 / e1, "single_core_fp64": f64 / e64,
+            "host": host}
 
 
 # MLP FLOP per level-1 probe (48 -> 64 -> 64 -> 64 -> 64, multiply-adds x 2)
@@ -310,12 +357,20 @@ def main():
     torch.cuda.synchronize()
     st = prof.stats()
     rays = [rc.level_rays(W, H, i) for i in range(L)]
-    alg = kernel_alg_bytes(W, H, L, args.storage, rays)
-    per_launch_ms = {k: st.stage_ms[k] / st.stage_launches[k] for k in st.stage_ms}
     per_frame_ms = {k: st.stage_ms[k] / st.runs for k in st.stage_ms}
-    dominant = max(per_frame_ms, key=per_frame_ms.get)
-    dom_ms = per_launch_ms[dominant]
-    achieved = alg[dominant] / (dom_ms * 1e-3) / 1e9
+    fam_alg = family_alg_bytes(W, H, L, args.storage, rays, bool(split), (max(W, H) - 1).bit_length())  # P = ceil(log2 max(W, H))
+    fam_ms, fam_launches = {}, {}
+    for fam, stages in FAMILIES.items():
+        stages = stages or [f"level{i}" for i in range(1, L)]
+        got = [k for k in stages if k in st.stage_ms]
+        if got:
+            fam_ms[fam] = sum(st.stage_ms[k] for k in got) / st.runs
+            fam_launches[fam] = sum(st.stage_launches[k] for k in got) / st.runs
+    dominant = max(fam_ms, key=fam_ms.get)
+    dom_ms = fam_ms[dominant] / fam_launches[dominant]          # mean launch
+    dom_bytes = fam_alg[dominant] / fam_launches[dominant]      # per launch
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    traffic = profiled_traffic(dominant, W, H, L, args.storage)
     frame_alg = b_alg(W, H, L, args.storage, rays)
 
     surrogate = None
@@ -351,10 +406,15 @@ def main():
                    "march": {"split": split, "speculation": spec, "rays_per_lane": rpl}, "jfa_fuse": stage.jfa_fuse()},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": profiled_traffic(dominant, W, H, L, args.storage),
+                     "traffic": None if traffic is None else traffic / fam_launches[dominant],
+                     "traffic_over_alg": None if traffic is None else traffic / fam_alg[dominant],
                      "traffic_source": "profiles/frame_1080p_hbm_traffic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
-                                       "copy-calibrated)",
-                     "alg_bytes_per_launch": alg[dominant], "mean_launch_ms": dom_ms},
+                                       "copy-calibrated, per kernel family and frame)",
+                     "alg_bytes_per_launch": dom_bytes, "mean_launch_ms": dom_ms,
+                     "launches_per_frame": fam_launches[dominant]},
+        "families": {f: {"ms_per_frame": round(fam_ms[f], 5), "launches": fam_launches[f],
+                         "alg_bytes": fam_alg[f],
+                         "frac": fam_alg[f] / (fam_ms[f] * 1e-3) / 1e9 / HBM_PEAK_GBS} for f in fam_ms},
         "frame_roofline": {"alg_bytes": frame_alg, "achieved": frame_alg / (ms_per_step * 1e-3) / 1e9,
                            "frac": frame_alg / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "stage_ms_per_frame": {k: round(v, 5) for k, v in per_frame_ms.items()},

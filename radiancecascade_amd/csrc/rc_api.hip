@@ -46,7 +46,8 @@ struct rc_ctx {
     int jfa_fuse = 0;  // largest jump-flood step fused into the LDS tail (0 = none)
     // split march: levels >= 1 marched by one k_march_all launch into `codes`,
     // then merged top-down; otherwise each level kernel marches its own rays
-    int march_split = 1;
+    int march_split = 1;   // 0 in place, 1 split, 2 split and persistent
+    int num_cu = 0;        // compute units of the device (persistent grid size)
     int spec = 1;      // speculation depth of the split march (1, 2, 4)
     int march_m = 1;   // rays per lane, marched back to back (1, 2, 4, 8)
     int *codes = nullptr;
@@ -299,7 +300,16 @@ static MarchAllArgs march_all_args(const rc_ctx *c)
         const int64_t rays = (int64_t)l.PX * l.PY * l.N, per = (int64_t)kBlock * c->march_m;
         l.blocks = (int)((rays + per - 1) / per);
         blocks += l.blocks;
+        l.code_begin = (uint32_t)c->code_off[i];
+        l.dir_off = (int)c->dir_off[i];
+        m.total_rays += (uint32_t)rays;
     }
+    m.codes_base = c->codes;
+    m.dirs_base = c->dirs;
+    m.n_packets = (m.total_rays + 63) / 64;
+    // resident waves: 8 workgroups of 4 waves per CU (the march's occupancy cap)
+    const uint32_t cap = (uint32_t)(c->num_cu > 0 ? c->num_cu : 256) * 32u;
+    m.n_waves = m.n_packets < cap ? m.n_packets : cap;
     return m;
 }
 
@@ -401,7 +411,8 @@ static rc_status run_frame(rc_ctx *c, const float *depth, const float *emissive,
     const bool split = c->march_split && c->L > 1;
     if (split) {
         tm.begin(RC_STAGE_MARCH);
-        HIP_TRY(c, launch_march_all(march_all_args(c), c->spec, c->march_m, s));
+        if (c->march_split == 2) HIP_TRY(c, launch_march_persist(march_all_args(c), c->spec, s));
+        else HIP_TRY(c, launch_march_all(march_all_args(c), c->spec, c->march_m, s));
         tm.end(RC_STAGE_MARCH);
     }
     for (int i = c->L - 1; i >= 1; --i) {
@@ -828,6 +839,8 @@ rc_status rc_create(const rc_config *cfg, rc_ctx **out)
 
     rc_ctx *c = new (std::nothrow) rc_ctx();
     if (!c) return fail(nullptr, RC_ERR_OOM, "rc_create: host allocation failed");
+    if (hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, cfg->device) != hipSuccess)
+        c->num_cu = 0;
     c->cfg = *cfg;
     c->storage = (Storage)cfg->storage;
     if (cfg->depth_mode == RC_DEPTH_NDC) {
@@ -1050,9 +1063,11 @@ uint32_t rc_get_jfa_fuse(const rc_ctx *c) { return c ? (uint32_t)c->jfa_fuse : 0
 rc_status rc_set_march(rc_ctx *c, uint32_t split, uint32_t speculation, uint32_t rays_per_lane)
 {
     if (!c) return fail(nullptr, RC_ERR_NULL, "rc_set_march: ctx is NULL");
-    if (split > 1 || (speculation != 1 && speculation != 2 && speculation != 4) ||
-        (rays_per_lane != 1 && rays_per_lane != 2 && rays_per_lane != 4 && rays_per_lane != 8))
-        return fail(c, RC_ERR_INVALID, "rc_set_march: split 0/1, speculation 1/2/4, rays_per_lane 1/2/4/8");
+    if (split > 2 || (speculation != 1 && speculation != 2 && speculation != 4) ||
+        (rays_per_lane != 1 && rays_per_lane != 2 && rays_per_lane != 4 && rays_per_lane != 8) ||
+        (split == 2 && rays_per_lane != 1))
+        return fail(c, RC_ERR_INVALID,
+                    "rc_set_march: split 0/1/2, speculation 1/2/4, rays_per_lane 1/2/4/8 (1 with split 2)");
     c->march_split = (int)split;
     c->spec = (int)speculation;
     c->march_m = (int)rays_per_lane;
@@ -1116,10 +1131,11 @@ rc_status rc_autotune(rc_ctx *c, const rc_gbuffer *g, float *irr, uint32_t iters
     {
         float best = 1e30f;
         int best_split = c->march_split, best_spec = c->spec, best_m = c->march_m;
-        for (int split : {0, 1})
+        for (int split : {0, 1, 2})
         for (int spec : {1, 2, 4})
             for (int m : {1, 2, 4, 8}) {
                 if (!split && (spec != 1 || m != 1)) continue;
+                if (split == 2 && m != 1) continue;
                 c->march_split = split;
                 c->spec = spec;
                 c->march_m = m;
@@ -1137,7 +1153,8 @@ rc_status rc_autotune(rc_ctx *c, const rc_gbuffer *g, float *irr, uint32_t iters
         c->march_split = best_split;
         c->spec = best_spec;
         c->march_m = best_m;
-        if (best_split) HIP_TRY(c, launch_march_all(march_all_args(c), c->spec, c->march_m, s));
+        if (best_split == 2) HIP_TRY(c, launch_march_persist(march_all_args(c), c->spec, s));
+        else if (best_split) HIP_TRY(c, launch_march_all(march_all_args(c), c->spec, c->march_m, s));
     }
     for (int i = 1; i < c->L; ++i) {
         float best = 1e30f;

@@ -469,6 +469,92 @@ __global__ __launch_bounds__(kBlock) void k_march_all(MarchAllArgs a)
     }
 }
 
+// Persistent form of the split march.  March lengths differ by two orders of
+// magnitude (most rays take 1-4 samples, rays grazing a surface up to a few
+// hundred), so with one ray per lane a wave idles most lanes while its
+// longest ray runs.  Here a grid of resident waves walks every ray of every
+// level: the rays are cut into packets of 64 consecutive codes, numbered
+// from the top level down (the longest rays first), and wave w owns packets
+// w, w + n_waves, w + 2 n_waves, ... (an even mix of levels and image
+// regions per wave).  Each iteration the wave hands the next entries of its
+// packet sequence to its idle lanes (a ballot and a lane rank; the position
+// in the sequence is a scalar, no memory atomics) and every busy lane takes
+// one march round.  Each ray's samples, and so every code, are those of
+// k_march_all; only the order in which rays are marched differs.
+template <int SPEC>
+__global__ __launch_bounds__(kBlock) void k_march_persist(MarchAllArgs a)
+{
+    // per-level constants, indexed per lane (a lane's ray may be of any level)
+    __shared__ int tab_i[RC_MAX_LEVELS][4];     // code_begin, PX, log2n, dir_off
+    __shared__ float tab_f[RC_MAX_LEVELS][4];   // inv_px, spacing, t0, t1
+    if ((int)threadIdx.x < a.nlev) {
+        const MarchLevel &L = a.lev[threadIdx.x];
+        tab_i[threadIdx.x][0] = (int)L.code_begin;
+        tab_i[threadIdx.x][1] = L.PX;
+        tab_i[threadIdx.x][2] = L.log2n;
+        tab_i[threadIdx.x][3] = L.dir_off;
+        tab_f[threadIdx.x][0] = L.inv_px;
+        tab_f[threadIdx.x][1] = L.spacing;
+        tab_f[threadIdx.x][2] = L.t0;
+        tab_f[threadIdx.x][3] = L.t1;
+    }
+    __syncthreads();
+    const __amdgpu_buffer_rsrc_t sdf = sdf_rsrc(a.sdf, a.W, a.H);
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6));
+    const uint32_t np = a.n_packets, nw = a.n_waves;
+    // this wave's sequence length (waves past n_waves in the last workgroup have none)
+    const uint32_t vend = (wave < nw && wave < np) ? ((np - 1 - wave) / nw + 1) * 64 : 0;
+    uint32_t vnext = 0;                                                    // wave-uniform
+    Ray r = ray_begin(0.f, 0.f, 0.f, 0.f, 0.f, 0.f, false);
+    float t1 = 0.f;
+    uint32_t out = 0;
+    bool have = false;
+    for (;;) {
+        const uint64_t idle = __ballot(!have);
+        if (idle != 0 && vnext < vend) {
+            const uint32_t rank =
+                __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+            const uint32_t v = vnext + rank;
+            if (!have && v < vend) {
+                const uint32_t pc = np - 1 - (wave + (v >> 6) * nw);  // packet in code order
+                const uint32_t ray = pc * 64 + (v & 63);
+                if (ray < a.total_rays) {
+                    int lv = 0;  // levels are top first; code_begin falls with the level
+#pragma unroll
+                    for (int j = 0; j < RC_MAX_LEVELS; ++j)
+                        if (j < a.nlev && ray < a.lev[j].code_begin) lv = j + 1;
+                    const int rr = (int)(ray - (uint32_t)tab_i[lv][0]);
+                    const int PX = tab_i[lv][1], log2n = tab_i[lv][2];
+                    const int k = rr & ((1 << log2n) - 1), probe = rr >> log2n;
+                    int py = (int)((float)probe * tab_f[lv][0]);
+                    int px = probe - py * PX;
+                    if (px < 0) { --py; px += PX; }
+                    else if (px >= PX) { ++py; px -= PX; }
+                    const float2 dir = a.dirs_base[tab_i[lv][3] + k];
+                    const float sp = tab_f[lv][1];
+                    t1 = tab_f[lv][3];
+                    r = ray_begin(((float)px + 0.5f) * sp, ((float)py + 0.5f) * sp, dir.x, dir.y, tab_f[lv][2], t1,
+                                  true);
+                    out = ray;
+                    have = true;
+                }
+            }
+            vnext += (uint32_t)__builtin_popcountll(idle);
+        }
+        if (!__any(have)) {
+            if (vnext >= vend) break;
+            continue;
+        }
+        if (have) {
+            ray_round<SPEC>(r, sdf, a.W, a.H, t1);
+            if (!r.alive) {
+                a.codes_base[out] = r.code;
+                have = false;
+            }
+        }
+    }
+}
+
 // The split march's decode of every slot of one level, recomposed into the
 // canonical index it stands for (parity readback: equal to the slot number
 // when the decode is right).
@@ -567,6 +653,19 @@ static hipError_t launch_march_all_spec(const MarchAllArgs &a, int m, int blocks
     case 2: hipLaunchKernelGGL((k_march_all<SPEC, 2>), dim3(blocks), dim3(kBlock), 0, s, a); break;
     case 4: hipLaunchKernelGGL((k_march_all<SPEC, 4>), dim3(blocks), dim3(kBlock), 0, s, a); break;
     case 8: hipLaunchKernelGGL((k_march_all<SPEC, 8>), dim3(blocks), dim3(kBlock), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_march_persist(const MarchAllArgs &a, int spec, hipStream_t s)
+{
+    if (a.total_rays == 0 || a.n_waves == 0) return hipSuccess;
+    const dim3 grid((a.n_waves + 3) / 4);
+    switch (spec) {
+    case 1: hipLaunchKernelGGL(k_march_persist<1>, grid, dim3(kBlock), 0, s, a); break;
+    case 2: hipLaunchKernelGGL(k_march_persist<2>, grid, dim3(kBlock), 0, s, a); break;
+    case 4: hipLaunchKernelGGL(k_march_persist<4>, grid, dim3(kBlock), 0, s, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -76,13 +76,22 @@ struct MarchLevel {
     float inv_px;            // 1/PX in fp32 (probe -> row, corrected)
     float t0, t1, spacing;
     int block_begin, blocks; // workgroups of 256*M canonical rays
+    uint32_t code_begin;     // first ray of this level in the code buffer (levels 1, 2, ... back to back)
+    int dir_off;             // first direction of this level in dirs_base
 };
 struct MarchAllArgs {
     const uint8_t *sdf;
     int W, H, nlev;
-    MarchLevel lev[RC_MAX_LEVELS];
+    MarchLevel lev[RC_MAX_LEVELS];  // top level first
+    // persistent form: every ray of every level, in 64-ray packets
+    int *codes_base;
+    const float2 *dirs_base;
+    uint32_t total_rays, n_packets, n_waves;
 };
 hipError_t launch_march_all(const MarchAllArgs &a, int speculation, int rays_per_lane, hipStream_t s);
+// The persistent split march: a grid of `waves` waves (a multiple of 4), each
+// walking its share of the packets and refilling finished lanes.
+hipError_t launch_march_persist(const MarchAllArgs &a, int speculation, hipStream_t s);
 
 hipError_t launch_level(const LevelArgs &a, Storage st, int shape, int rays_per_lane, hipStream_t s);
 hipError_t launch_level0_gather(const LevelArgs &a, Storage st, float *irradiance, float *debug_fluence,

@@ -134,13 +134,15 @@ class RadianceCascadeStage:
     def set_tiling(self, level: int, shape: int, rays_per_lane: int) -> None:
         check(self._lib.rc_set_level_tiling(self._ctx, level, shape, rays_per_lane), self._ctx)
 
-    def set_march(self, split: bool, speculation: int = 1, rays_per_lane: int = 1) -> None:
+    def set_march(self, split: int, speculation: int = 1, rays_per_lane: int = 1) -> None:
+        """split: 0 in place, 1 one launch for every level >= 1, 2 the same by
+        a persistent grid (rc_set_march)."""
         check(self._lib.rc_set_march(self._ctx, int(split), speculation, rays_per_lane), self._ctx)
 
-    def march(self) -> tuple[bool, int, int]:
+    def march(self) -> tuple[int, int, int]:
         sp, k, m = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
         check(self._lib.rc_get_march(self._ctx, ctypes.byref(sp), ctypes.byref(k), ctypes.byref(m)), self._ctx)
-        return bool(sp.value), k.value, m.value
+        return sp.value, k.value, m.value
 
     def set_jfa_fuse(self, fused_step: int) -> None:
         check(self._lib.rc_set_jfa_fuse(self._ctx, fused_step), self._ctx)

@@ -331,9 +331,10 @@ def test_every_march_mode_is_bit_exact(rc, oracle, W, H, L, seed):
     hits_ref = [oracle.march_level(sdf, e, i)[1].ravel() for i in range(L)]
     irr = torch.empty((3, H, W), dtype=torch.float32, device="cuda:0")
     with rc.RadianceCascadeStage(W, H, levels=L, debug=True) as st:
-        for split, spec, rpl in [(0, 1, 1), (1, 1, 1), (1, 2, 1), (1, 4, 1), (1, 1, 2), (1, 2, 4), (1, 4, 8)]:
+        for split, spec, rpl in [(0, 1, 1), (1, 1, 1), (1, 2, 1), (1, 4, 1), (1, 1, 2), (1, 2, 4), (1, 4, 8),
+                                 (2, 1, 1), (2, 2, 1), (2, 4, 1)]:
             st.set_march(split, spec, rpl)
-            assert st.march() == (bool(split), spec, rpl)
+            assert st.march() == (split, spec, rpl)
             st.run_device(depth, emis, irr)
             torch.cuda.synchronize()
             assert np.array_equal(irr.cpu().numpy().view(np.uint32), irr32.view(np.uint32)), (split, spec)
@@ -352,6 +353,10 @@ def test_every_march_mode_is_bit_exact(rc, oracle, W, H, L, seed):
             st.set_march(1, 3, 1)
         with pytest.raises(rc.RcError):
             st.set_march(1, 1, 3)
+        with pytest.raises(rc.RcError):
+            st.set_march(2, 1, 2)
+        with pytest.raises(rc.RcError):
+            st.set_march(3, 1, 1)
 
 
 def _round_storage(a, storage):
