@@ -193,7 +193,10 @@ uint32_t rc_abi_version(void);
 rc_status rc_device_count(int *count);
 
 /* Parity readbacks of the last run (device destination, stream ordered).
- * rc_debug_seeds / rc_debug_level(.., 0, ..) need RC_FLAG_KEEP_DEBUG.
+ * rc_debug_seeds / rc_debug_level(.., 0, ..) need RC_FLAG_KEEP_DEBUG.  After a
+ * learned frame (rc_set_surrogate mode 1) rc_debug_level returns that frame's
+ * levels: 2 the network's, 1 merged from it (with flagged tiles patched), 0
+ * the fluence; it has no level above 2 (RC_ERR_STATE).
  *   seeds : uint32 [H][W]  final jump-flood field, (y<<16)|x or 0xFFFFFFFF
  *   sdf   : float  [H][W]  distance field in px (+inf with no seed)
  *   level : float  [Py_i][Px_i][max(N_i/4,1)][4], merged level i averaged over

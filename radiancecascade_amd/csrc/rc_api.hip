@@ -87,6 +87,7 @@ struct rc_ctx {
     int *flag_host = nullptr;  // pinned
     int surr_tiles = 0, surr_tiles_x = 0;
     bool surr_ran = false;
+    bool last_learned = false;  // the last frame was the learned form (debug readbacks follow it)
     float *scratch_irr = nullptr;  // irradiance of frames run only for their levels
     // the cone schedules (near, far) in device memory
     ConeSchedule *sch_dev = nullptr;
@@ -439,6 +440,7 @@ static rc_status run_frame(rc_ctx *c, const float *depth, const float *emissive,
     if (!quiet) {
         c->stats.runs += 1;
         c->ran = true;
+        c->last_learned = false;
     }
     return RC_OK;
 }
@@ -627,6 +629,7 @@ static rc_status run_frame_surrogate(rc_ctx *c, const float *depth, const float 
         c->ran = true;
     }
     c->surr_ran = true;
+    c->last_learned = true;
     return RC_OK;
 }
 
@@ -1000,7 +1003,14 @@ rc_status rc_debug_level(rc_ctx *c, uint32_t level, float *dev_out, void *stream
                                   (hipStream_t)stream));
         return RC_OK;
     }
-    HIP_TRY(c, launch_level_to_f32(c->level[level], c->storage, c->level_texels[level], dev_out, (hipStream_t)stream));
+    const void *src = c->level[level];
+    if (c->last_learned) {
+        // the learned frame's own levels: the network's level 2 and level 1
+        // merged from it; it computes no plain level above 2
+        if (level > 2) return fail(c, RC_ERR_STATE, "rc_debug_level: the learned frame has no level above 2");
+        src = level == 2 ? c->surr_a2 : c->surr_a1;
+    }
+    HIP_TRY(c, launch_level_to_f32(src, c->storage, c->level_texels[level], dev_out, (hipStream_t)stream));
     return RC_OK;
 }
 

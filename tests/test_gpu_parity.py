@@ -480,3 +480,40 @@ def test_ndc_depth_input(rc, oracle, W, H, L, seed, nb, ne):
     _say(f"  seeds differing from the linear-depth run: {n_diff}")
     assert n_diff == 0
     assert torch.equal(irr_ndc, irr_lin)
+
+
+def test_4k_seven_levels_plain_and_learned(rc, oracle):
+    """Config 4's test resolution: 3840x2160 with 7 cascades.  The plain frame
+    equals the FP32 oracle bit for bit (and the FP64 oracle within the
+    irradiance tolerance); the learned frame with the packaged network equals
+    the FP64 oracle of the same network (oracle/surrogate.py, bf16 rounding
+    points emulated) within 1e-6 rel-L2 on irradiance."""
+    import os
+    from oracle import surrogate as so
+    W, H, L, seed = 3840, 2160, 7, 0
+    _say(f"4K W={W} H={H} L={L} seed={seed}")
+    if not os.path.exists(rc.packaged_weights()):
+        pytest.skip("no packaged weights")
+    depth, emis = _device_gbuffer(rc, W, H, seed, 32, 4)
+    irr = torch.empty((3, H, W), dtype=torch.float32, device="cuda:0")
+    irr_l = torch.empty_like(irr)
+    with rc.RadianceCascadeStage(W, H, levels=L) as st:
+        st.run_device(depth, emis, irr)
+    with rc.RadianceCascadeStage(W, H, levels=L) as st:
+        st.load_weights()
+        st.set_surrogate(True)
+        st.run_device(depth, emis, irr_l)
+        net = st.mlp_get()
+    torch.cuda.synchronize()
+    d, e = depth.cpu().numpy(), emis.cpu().numpy()
+    irr32 = oracle.run(d, e, L, np.float32)
+    got = irr.cpu().numpy()
+    assert np.array_equal(got.view(np.uint32), irr32.view(np.uint32)), "4K plain frame differs from the FP32 oracle"
+    irr64 = oracle.run(d, e, L, np.float64)
+    e64 = rel_l2(got, irr64)
+    ref_l = so.run(d, e, L, net["params"], net["in_mean"], net["in_istd"], net["out_scale"])
+    r_l = rel_l2(irr_l.cpu().numpy(), ref_l)
+    _say(f"  plain vs FP64 {e64:.2e}; learned vs its FP64 oracle {r_l:.2e}; "
+         f"learned vs plain FP64 {rel_l2(irr_l.cpu().numpy(), irr64):.4f}")
+    assert e64 <= REL_L2_TOL_F32
+    assert r_l <= 1e-6
