@@ -87,6 +87,13 @@ struct rc_ctx {
     int *flag_host = nullptr;  // pinned
     int surr_tiles = 0, surr_tiles_x = 0;
     bool surr_ran = false;
+    uint8_t *sdf_near = nullptr;   // the learned frame's near distance field (levels 0 and 1)
+    // tile fallback: per-level masks of the probes the flagged tiles need, and
+    // the device tally of flagged tiles / frames with a fallback
+    uint8_t *need = nullptr;
+    size_t need_off[RC_MAX_LEVELS] = {};
+    size_t need_bytes = 0;
+    unsigned long long *tally = nullptr;
     bool last_learned = false;  // the last frame was the learned form (debug readbacks follow it)
     float *scratch_irr = nullptr;  // irradiance of frames run only for their levels
     // the cone schedules (near, far) in device memory
@@ -113,6 +120,12 @@ static rc_status fail(rc_ctx *c, rc_status s, const std::string &msg)
         if (e_ != hipSuccess)                                                                          \
             return fail((ctx), e_ == hipErrorOutOfMemory ? RC_ERR_OOM : RC_ERR_HIP,                    \
                         std::string(#expr) + ": " + hipGetErrorString(e_));                            \
+    } while (0)
+
+#define HIP_TRY_ST(expr)                  \
+    do {                                  \
+        const rc_status st_ = (expr);     \
+        if (st_ != RC_OK) return st_;     \
     } while (0)
 
 static bool dims_ok(uint32_t W, uint32_t H) { return W >= 1 && H >= 1 && W <= RC_MAX_DIM && H <= RC_MAX_DIM; }
@@ -183,6 +196,13 @@ static void release(rc_ctx *c)
     if (c->tile_err) hipFree(c->tile_err);
     if (c->flag_count) hipFree(c->flag_count);
     if (c->flag_list) hipFree(c->flag_list);
+    if (c->sdf_near) hipFree(c->sdf_near);
+    if (c->need) hipFree(c->need);
+    if (c->tally) hipFree(c->tally);
+    c->sdf_near = nullptr;
+    c->need = nullptr;
+    c->tally = nullptr;
+    c->need_bytes = 0;
     c->cone = nullptr;
     c->surr_a1 = nullptr;
     c->tile_err = nullptr;
@@ -373,6 +393,40 @@ static rc_status collect_profile(rc_ctx *c, hipStream_t s, int jfa_launches)
     return RC_OK;
 }
 
+// Occupancy seeds, jump flood and the distance field c->sdf: global passes
+// down to the fused LDS tail (if any); without a tail the last global pass
+// writes the field.  A 1x1 image has no pass.  With `gate` every kernel does
+// nothing when *gate == 0 (the learned frame's tile fallback).
+static rc_status seeds_and_jfa(rc_ctx *c, const float *depth, const float *emissive, hipStream_t s, bool dbg,
+                               const int *gate, StageTimer *tm, int *global_passes)
+{
+    if (tm) tm->begin(RC_STAGE_SEED);
+    HIP_TRY(c, launch_seed_init(depth, c->depth_conv, emissive, c->seed[0], c->emis4, c->W, c->H, s, gate));
+    if (tm) tm->end(RC_STAGE_SEED);
+    const int P = jfa_num_passes(c->W, c->H);
+    const int tail = jfa_tail_step(c->W, c->H, c->jfa_fuse);
+    int cur = 0, passes = 0;
+    c->dbg_seed_buf = -1;
+    if (tm) tm->begin(RC_STAGE_JFA);
+    for (int step = P > 0 ? 1 << (P - 1) : 0; step >= 1 && step > tail; step >>= 1) {
+        const bool final_pass = tail == 0 && step == 1;
+        uint32_t *dst = final_pass ? (dbg ? c->seed[cur ^ 1] : nullptr) : c->seed[cur ^ 1];
+        HIP_TRY(c, launch_jfa_pass(c->seed[cur], dst, c->sdf, c->W, c->H, step, final_pass, s, gate));
+        if (final_pass) c->dbg_seed_buf = dbg ? (cur ^ 1) : -1;
+        else cur ^= 1;
+        ++passes;
+    }
+    if (tm) tm->end(RC_STAGE_JFA);
+    if (tail > 0 || P == 0) {
+        if (tm) tm->begin(RC_STAGE_JFA_TAIL);
+        HIP_TRY(c, launch_jfa_tail(c->seed[cur], dbg ? c->seed[cur ^ 1] : nullptr, c->sdf, c->W, c->H, tail, s, gate));
+        if (tm) tm->end(RC_STAGE_JFA_TAIL);
+        c->dbg_seed_buf = dbg ? (cur ^ 1) : -1;
+    }
+    if (global_passes) *global_passes = passes;
+    return RC_OK;
+}
+
 static rc_status run_frame(rc_ctx *c, const float *depth, const float *emissive, float *irr, hipStream_t s,
                            bool quiet = false)
 {
@@ -381,33 +435,9 @@ static rc_status run_frame(rc_ctx *c, const float *depth, const float *emissive,
     StageTimer tm{c, s, prof};
     for (auto &u : c->ev_used) u = false;
 
-    tm.begin(RC_STAGE_SEED);
-    HIP_TRY(c, launch_seed_init(depth, c->depth_conv, emissive, c->seed[0], c->emis4, c->W, c->H, s));
-    tm.end(RC_STAGE_SEED);
-
-    // Jump flood: global passes down to the fused LDS tail (if any); without a
-    // tail the last global pass writes the SDF.  A 1x1 image has no pass.
-    const int P = jfa_num_passes(c->W, c->H);
-    const int tail = jfa_tail_step(c->W, c->H, c->jfa_fuse);
-    int cur = 0, global_passes = 0;
-    c->dbg_seed_buf = -1;
-    tm.begin(RC_STAGE_JFA);
-    for (int step = P > 0 ? 1 << (P - 1) : 0; step >= 1 && step > tail; step >>= 1) {
-        const bool final_pass = tail == 0 && step == 1;
-        uint32_t *dst = final_pass ? (dbg ? c->seed[cur ^ 1] : nullptr) : c->seed[cur ^ 1];
-        HIP_TRY(c, launch_jfa_pass(c->seed[cur], dst, c->sdf, c->W, c->H, step, final_pass, s));
-        if (final_pass) c->dbg_seed_buf = dbg ? (cur ^ 1) : -1;
-        else cur ^= 1;
-        ++global_passes;
-    }
-    tm.end(RC_STAGE_JFA);
-    const bool tail_kernel = tail > 0 || P == 0;
-    if (tail_kernel) {
-        tm.begin(RC_STAGE_JFA_TAIL);
-        HIP_TRY(c, launch_jfa_tail(c->seed[cur], dbg ? c->seed[cur ^ 1] : nullptr, c->sdf, c->W, c->H, tail, s));
-        tm.end(RC_STAGE_JFA_TAIL);
-        c->dbg_seed_buf = dbg ? (cur ^ 1) : -1;
-    }
+    int global_passes = 0;
+    rc_status js = seeds_and_jfa(c, depth, emissive, s, dbg, nullptr, &tm, &global_passes);
+    if (js != RC_OK) return js;
 
     const bool split = c->march_split && c->L > 1;
     if (split) {
@@ -503,6 +533,15 @@ static rc_status ensure_surrogate(rc_ctx *c)
     HIP_TRY(c, hipMalloc(&c->flag_list, (size_t)c->surr_tiles * sizeof(int)));
     HIP_TRY(c, hipMalloc(&c->flag_count, sizeof(int)));
     HIP_TRY(c, hipMemset(c->flag_count, 0, sizeof(int)));
+    HIP_TRY(c, hipMalloc(&c->sdf_near, sdf_elems(c->W, c->H)));
+    c->need_bytes = 0;
+    for (int i = 1; i < c->L; ++i) {
+        c->need_off[i] = c->need_bytes;
+        c->need_bytes += (size_t)grid_dim(c->W, i) * grid_dim(c->H, i);
+    }
+    HIP_TRY(c, hipMalloc(&c->need, c->need_bytes));
+    HIP_TRY(c, hipMalloc(&c->tally, 2 * sizeof(unsigned long long)));
+    HIP_TRY(c, hipMemset(c->tally, 0, 2 * sizeof(unsigned long long)));
     HIP_TRY(c, hipMalloc(&c->sch_dev, 2 * sizeof(ConeSchedule)));
     HIP_TRY(c, hipMemcpy(c->sch_dev, &c->sch, sizeof(ConeSchedule), hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemcpy(c->sch_dev + 1, &c->sch_far, sizeof(ConeSchedule), hipMemcpyHostToDevice));
@@ -569,11 +608,11 @@ static rc_status run_frame_surrogate(rc_ctx *c, const float *depth, const float 
         // the top mip levels (one workgroup) beside the near distance field;
         // near and far cones in one launch (the network kernel appends the
         // far segment)
-        HIP_TRY(c, launch_mip_top_near(c->mip, c->flag_count, c->occ_bits, c->sdf, c->W, c->H, s));
+        HIP_TRY(c, launch_mip_top_near(c->mip, c->flag_count, c->occ_bits, c->sdf_near, c->W, c->H, s));
         HIP_TRY(c, launch_cone_pair(cone_args_near(c, false), cone_args_far(c), s));
     } else {
         tm.begin(RC_STAGE_NEAR_SDF);
-        HIP_TRY(c, launch_near_sdf(c->occ_bits, c->sdf, c->W, c->H, s));
+        HIP_TRY(c, launch_near_sdf(c->occ_bits, c->sdf_near, c->W, c->H, s));
         tm.end(RC_STAGE_NEAR_SDF);
         tm.begin(RC_STAGE_MIP);
         HIP_TRY(c, launch_mip_top(c->mip, c->flag_count, s));
@@ -602,23 +641,69 @@ static rc_status run_frame_surrogate(rc_ctx *c, const float *depth, const float 
     tm.begin(RC_STAGE_SURROGATE);
     HIP_TRY(c, launch_surr_mlp(a, c->mlp, c->storage, s));
     tm.end(RC_STAGE_SURROGATE);
+    // Tile fallback (finite threshold): the plain far field for the flagged
+    // tiles only, decided on the device.  Every kernel reads the flagged-tile
+    // count the network kernel left and does nothing when it is zero; the
+    // masked level kernels compute only the probes the flagged tiles need
+    // (their bilinear neighbourhood at each level up), marched in place on
+    // the whole frame's jump-flood field.
+    const bool fallback = c->surr_thr < __builtin_huge_valf();
+    if (fallback) {
+        tm.begin(RC_STAGE_FALLBACK);
+        HIP_TRY_ST(seeds_and_jfa(c, depth, emissive, s, false, c->flag_count, nullptr, nullptr));
+        FallbackMaskArgs fm{};
+        fm.count = c->flag_count;
+        fm.list = c->flag_list;
+        fm.need = c->need;
+        for (int i = 1; i < c->L; ++i) {
+            fm.need_off[i] = c->need_off[i];
+            fm.PX[i] = grid_dim(c->W, i);
+            fm.PY[i] = grid_dim(c->H, i);
+        }
+        fm.L = c->L;
+        fm.tiles_x = c->surr_tiles_x;
+        fm.max_tiles = c->surr_tiles;
+        fm.need_bytes = c->need_bytes;
+        HIP_TRY(c, launch_fallback_masks(fm, s));
+        for (int i = c->L - 1; i >= 2; --i) {
+            LevelArgs ai = level_args(c, i, emissive);
+            ai.emis4 = c->emis4;
+            ai.gate = c->flag_count;
+            ai.need = c->need + c->need_off[i];
+            HIP_TRY(c, launch_level(ai, c->storage, c->tile[i], c->rpt[i], s));
+        }
+        tm.end(RC_STAGE_FALLBACK);
+    }
     // level 1: the plain level kernel, marching in place on the near field,
     // merging the network's level 2
     tm.begin(RC_STAGE_LEVEL0 + 1);
     LevelArgs a1 = level_args(c, 1, emissive);
+    a1.sdf = c->sdf_near;
     a1.upper = c->surr_a2;
     a1.out = c->surr_a1;
     a1.codes = nullptr;
     if (dbg) a1.dbg_codes = c->dbg_codes + c->dbg_code_off[1];
     HIP_TRY(c, launch_level(a1, c->storage, c->surr_tile1 >= 0 ? c->surr_tile1 : c->tile[1],
                             c->surr_rpt1 >= 0 ? c->surr_rpt1 : c->rpt[1], s));
+    if (fallback) {
+        // the flagged tiles' level 1 again, plain: marched on the jump-flood
+        // field and merged from the plain level 2, over the learned one
+        LevelArgs af = level_args(c, 1, emissive);
+        af.emis4 = c->emis4;
+        af.out = c->surr_a1;
+        af.gate = c->flag_count;
+        af.need = c->need + c->need_off[1];
+        HIP_TRY(c, launch_level(af, c->storage, c->tile[1], c->rpt[1], s));
+    }
     tm.end(RC_STAGE_LEVEL0 + 1);
     tm.begin(RC_STAGE_LEVEL0);
     LevelArgs a0 = level_args(c, 0, emissive);
+    a0.sdf = c->sdf_near;
     a0.upper = c->surr_a1;
     if (dbg) a0.dbg_codes = c->dbg_codes + c->dbg_code_off[0];
     HIP_TRY(c, launch_level0_gather(a0, c->storage, irr, dbg ? c->dbg_level0 : nullptr, s));
     tm.end(RC_STAGE_LEVEL0);
+    if (fallback) HIP_TRY(c, launch_fallback_tally(c->flag_count, c->tally, s));
     for (int i = 0; i < RC_MAX_LEVELS; ++i) c->code_src[i] = (dbg && i <= 1) ? 2 : 0;
     if (prof) {
         rc_status st = collect_profile(c, s, 0);
@@ -678,11 +763,9 @@ static rc_status run_frame_graph(rc_ctx *c, const float *depth, const float *emi
     return RC_OK;
 }
 
-// One frame on stream s.  In learned mode with a finite threshold the host
-// reads the flagged-tile count after the frame (one 4-byte copy and a stream
-// sync); if any tile is flagged the plain frame runs, the flagged tiles of the
-// learned level 1 are replaced by the plain ones and level 0 is gathered
-// again.
+// One frame on stream s.  In learned mode with a finite threshold the tile
+// fallback is part of the frame's launches and decided on the device (no
+// host synchronisation); rc_get_stats reads its tally.
 static rc_status frame(rc_ctx *c, const float *depth, const float *emissive, float *irr, hipStream_t s)
 {
     if (c->surr_mode) {
@@ -696,30 +779,6 @@ static rc_status frame(rc_ctx *c, const float *depth, const float *emissive, flo
     if (st != RC_OK || !c->surr_mode) return st;
     c->stats.surrogate_frames += 1;
     c->stats.tiles_per_frame = (uint64_t)c->surr_tiles;
-    if (!(c->surr_thr < __builtin_huge_valf())) return RC_OK;
-    HIP_TRY(c, hipMemcpyAsync(c->flag_host, c->flag_count, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIP_TRY(c, hipStreamSynchronize(s));
-    const int n = *c->flag_host;
-    if (n <= 0) return RC_OK;
-    const bool prof = (c->cfg.flags & RC_FLAG_PROFILE) != 0;
-    if (prof) HIP_TRY(c, hipEventRecord(c->ev[2 * RC_STAGE_FALLBACK], s));
-    st = run_frame(c, depth, emissive, irr, s, true);
-    if (st != RC_OK) return st;
-    HIP_TRY(c, launch_tile_patch(c->level[1], c->surr_a1, c->tile_err, c->surr_thr, grid_dim(c->W, 1),
-                                 grid_dim(c->H, 1), c->surr_tiles_x, c->storage, s));
-    LevelArgs a0 = level_args(c, 0, emissive);
-    a0.upper = c->surr_a1;
-    HIP_TRY(c, launch_level0_gather(a0, c->storage, irr, nullptr, s));
-    if (prof) {
-        HIP_TRY(c, hipEventRecord(c->ev[2 * RC_STAGE_FALLBACK + 1], s));
-        HIP_TRY(c, hipStreamSynchronize(s));
-        float ms = 0.f;
-        HIP_TRY(c, hipEventElapsedTime(&ms, c->ev[2 * RC_STAGE_FALLBACK], c->ev[2 * RC_STAGE_FALLBACK + 1]));
-        c->stats.stage_ms[RC_STAGE_FALLBACK] += ms;
-        c->stats.stage_launches[RC_STAGE_FALLBACK] += 1;
-    }
-    c->stats.fallback_frames += 1;
-    c->stats.flagged_tiles += (uint64_t)n;
     return RC_OK;
 }
 
@@ -963,6 +1022,15 @@ rc_status rc_get_stats(const rc_ctx *c, rc_stats *out)
 {
     if (!c || !out) return fail(nullptr, RC_ERR_NULL, "rc_get_stats: NULL argument");
     *out = c->stats;
+    if (c->tally) {
+        // the tile fallback's device tally (waits for the frames enqueued so far)
+        unsigned long long t[2] = {0, 0};
+        hipError_t e = hipSetDevice(c->cfg.device);
+        if (e == hipSuccess) e = hipMemcpy(t, c->tally, sizeof(t), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return fail(nullptr, RC_ERR_HIP, std::string("rc_get_stats: ") + hipGetErrorString(e));
+        out->flagged_tiles += t[0];
+        out->fallback_frames += t[1];
+    }
     return RC_OK;
 }
 
@@ -972,6 +1040,12 @@ rc_status rc_reset_stats(rc_ctx *c)
     for (auto &v : c->stats.stage_ms) v = 0.0;
     for (auto &v : c->stats.stage_launches) v = 0;
     c->stats.runs = 0;
+    c->stats.surrogate_frames = c->stats.fallback_frames = c->stats.flagged_tiles = 0;
+    if (c->tally) {
+        HIP_TRY(c, hipSetDevice(c->cfg.device));
+        HIP_TRY(c, hipDeviceSynchronize());
+        HIP_TRY(c, hipMemset(c->tally, 0, 2 * sizeof(unsigned long long)));
+    }
     return RC_OK;
 }
 
@@ -988,7 +1062,8 @@ rc_status rc_debug_sdf(rc_ctx *c, float *dev_out, void *stream)
 {
     if (!c || !dev_out) return fail(c, RC_ERR_NULL, "rc_debug_sdf: NULL argument");
     if (!c->ran) return fail(c, RC_ERR_STATE, "rc_debug_sdf: no run yet");
-    HIP_TRY(c, launch_sdf_untile(c->sdf, dev_out, c->W, c->H, (hipStream_t)stream));
+    // after a learned frame: the near field its levels 0 and 1 marched on
+    HIP_TRY(c, launch_sdf_untile(c->last_learned ? c->sdf_near : c->sdf, dev_out, c->W, c->H, (hipStream_t)stream));
     return RC_OK;
 }
 
@@ -1196,6 +1271,7 @@ rc_status rc_autotune(rc_ctx *c, const rc_gbuffer *g, float *irr, uint32_t iters
         st = run_frame_surrogate(c, g->depth, g->emissive, irr, s, true);
         if (st != RC_OK) return st;
         LevelArgs a1 = level_args(c, 1, g->emissive);
+        a1.sdf = c->sdf_near;
         a1.upper = c->surr_a2;
         a1.out = c->surr_a1;
         a1.codes = nullptr;

@@ -112,6 +112,22 @@ __global__ __launch_bounds__(kBlock) void k_level(LevelArgs a)
     const int kk = threadIdx.x % KD, pl = threadIdx.x / KD;
     const int px = px0 + pl % TX, pyb = py0 + pl / TX, k = k0 + kk;
 
+    // the tile fallback: nothing when no tile is flagged; only the probes
+    // the flagged tiles need (a workgroup without one leaves at once)
+    if (a.gate && *a.gate == 0) return;
+    bool needed[RPT];
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+        const int py = pyb + j * TY;
+        needed[j] = px < a.PX && py < a.PY && (!a.need || a.need[py * a.PX + px] != 0);
+    }
+    if (a.need) {
+        bool any = false;
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) any = any || needed[j];
+        if (!__syncthreads_or(any)) return;
+    }
+
     // 32-bit indices: a level holds fewer than 2^32 rays (W, H <= 16384), N
     // is a power of two, and probe rows and columns are below 2^24
     const int log2n = 31 - __builtin_clz(a.N);
@@ -136,7 +152,7 @@ __global__ __launch_bounds__(kBlock) void k_level(LevelArgs a)
         const int py = pyb + j * TY;
         cx[j] = ((float)px + 0.5f) * a.spacing;
         cy[j] = ((float)py + 0.5f) * a.spacing;
-        active[j] = px < a.PX && py < a.PY;
+        active[j] = needed[j];
     }
     if (FROM_CODES) {
         // the rays were marched by k_march_all; read their codes (4 per quad)

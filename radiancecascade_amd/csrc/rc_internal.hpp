@@ -29,12 +29,14 @@ int jfa_num_passes(int W, int H);
 // largest step fused into the LDS tail for a requested fuse limit (0 = none)
 int jfa_tail_step(int W, int H, int fuse);
 hipError_t launch_sdf_untile(const uint8_t *sdf, float *out, int W, int H, hipStream_t s);
+// `gate` (nullptr: always run): the kernel does nothing when *gate == 0 (the
+// learned frame's tile fallback runs them only when some tile is flagged).
 hipError_t launch_seed_init(const float *depth, const DepthConv &dc, const float *emissive, uint32_t *seed,
-                            float4 *emis4, int W, int H, hipStream_t s);
+                            float4 *emis4, int W, int H, hipStream_t s, const int *gate = nullptr);
 hipError_t launch_jfa_pass(const uint32_t *in, uint32_t *out, uint8_t *sdf, int W, int H, int step, bool final_pass,
-                           hipStream_t s);
+                           hipStream_t s, const int *gate = nullptr);
 hipError_t launch_jfa_tail(const uint32_t *in, uint32_t *seeds_out, uint8_t *sdf, int W, int H, int tail_step,
-                           hipStream_t s);
+                           hipStream_t s, const int *gate = nullptr);
 
 // ---- cascades (rc_cascade.hip) ----------------------------------------------
 // Workgroup tile of a level kernel: TX x TY probes x KD directions, TX*TY*KD = 256.
@@ -66,6 +68,8 @@ struct LevelArgs {
     float t0, t1, spacing;
     const int *codes;        // pre-marched ray codes of this level, or nullptr (march in place)
     int *dbg_codes;          // in-place march: each ray's code also stored here (canonical order), or nullptr
+    const int *gate;         // the kernel does nothing when *gate == 0 (nullptr: always runs)
+    const uint8_t *need;     // [PY][PX] probes to compute (nullptr: all); the others are left untouched
 };
 
 // k_march_all: every level >= 1 in one launch (level-major, top level first).
@@ -225,8 +229,22 @@ hipError_t launch_mip_top_near(const MipArgs &m, int *zero_me, const uint64_t *b
                                hipStream_t s);  // levels 7..top beside the near field
 hipError_t launch_surr_mlp(const SurrMlpArgs &a, const MlpDev &m, Storage st, hipStream_t s);
 hipError_t launch_surr_sample(const SurrSampleArgs &a, Storage st, hipStream_t s);
-hipError_t launch_tile_patch(const void *plain, void *surr, const float *tile_err, float threshold, int PX, int PY,
-                             int tiles_x, Storage st, hipStream_t s);
+// Tile fallback of the learned frame: the probes of each level that the
+// flagged tiles need (level 1: the tiles' own 16x16 probes; level i+1: the
+// bilinear neighbourhood of level i's), one byte per probe, levels 1..L-1
+// back to back at need_off[i].
+struct FallbackMaskArgs {
+    const int *count;        // flagged tiles (device counter)
+    const int *list;         // their indices (tiles of 8x8 level-2 probes)
+    uint8_t *need;
+    size_t need_off[RC_MAX_LEVELS];
+    int PX[RC_MAX_LEVELS], PY[RC_MAX_LEVELS];
+    int L, tiles_x, max_tiles;
+    size_t need_bytes;
+};
+hipError_t launch_fallback_masks(const FallbackMaskArgs &a, hipStream_t s);
+// tally[0] += flagged tiles of this frame, tally[1] += 1 if any
+hipError_t launch_fallback_tally(const int *count, unsigned long long *tally, hipStream_t s);
 hipError_t launch_col_stats(const float *x, size_t rows, int cols, double *partial, int parts, hipStream_t s);
 hipError_t launch_scale(float *x, size_t n, float k, hipStream_t s);
 

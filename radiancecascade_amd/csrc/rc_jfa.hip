@@ -79,8 +79,9 @@ constexpr int kSeedRows = 16;
 __global__ __launch_bounds__(kBlock) void k_seed_init(const float *__restrict__ depth, DepthConv dc,
                                                       const float *__restrict__ emissive,
                                                       uint32_t *__restrict__ seed, float4 *__restrict__ emis4, int W,
-                                                      int H, int bx_count)
+                                                      int H, int bx_count, const int *gate)
 {
+    if (gate && *gate == 0) return;
     const int b = xcd_band(blockIdx.x, gridDim.x);
     const int x = (b % bx_count) * 64 + (threadIdx.x & 63);
     const int y0 = (b / bx_count) * kSeedRows + 4 * (threadIdx.x >> 6);
@@ -133,8 +134,10 @@ __global__ __launch_bounds__(kBlock) void k_seed_init(const float *__restrict__ 
 // instead of the internal seed field.
 template <bool FINAL>
 __global__ __launch_bounds__(kBlock) void k_jfa_pass(const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
-                                                     uint8_t *__restrict__ sdf, int W, int H, int step, int bx_count)
+                                                     uint8_t *__restrict__ sdf, int W, int H, int step, int bx_count,
+                                                     const int *gate)
 {
+    if (gate && *gate == 0) return;
     const int b = xcd_band(blockIdx.x, gridDim.x);
     const int x = (b % bx_count) * 64 + (threadIdx.x & 63);
     const int y = __builtin_amdgcn_readfirstlane((b / bx_count) * 4 + (threadIdx.x >> 6));
@@ -200,8 +203,9 @@ constexpr int kTailBlock = 1024, kTailRowsStep = kTailBlock / 64;
 template <int S>
 __global__ __launch_bounds__(kTailBlock) void k_jfa_tail(const uint32_t *__restrict__ in,
                                                      uint32_t *__restrict__ seeds_out, uint8_t *__restrict__ sdf,
-                                                     int W, int H, int bx_count)
+                                                     int W, int H, int bx_count, const int *gate)
 {
+    if (gate && *gate == 0) return;
     constexpr int T = jfa_tail_tile<S>();
     constexpr int HALO = 2 * S - 1;
     constexpr int R = T + 2 * HALO;
@@ -286,8 +290,9 @@ __global__ __launch_bounds__(kTailBlock) void k_jfa_tail(const uint32_t *__restr
 
 // An image with no pass at all (1x1): only the SDF conversion.
 __global__ void k_jfa_none(const uint32_t *__restrict__ in, uint32_t *__restrict__ seeds_out,
-                           uint8_t *__restrict__ sdf, int W, int n)
+                           uint8_t *__restrict__ sdf, int W, int n, const int *gate)
 {
+    if (gate && *gate == 0) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t s = in[i];
@@ -311,45 +316,47 @@ hipError_t launch_sdf_untile(const uint8_t *sdf, float *out, int W, int H, hipSt
 }
 
 hipError_t launch_seed_init(const float *depth, const DepthConv &dc, const float *emissive, uint32_t *seed,
-                            float4 *emis4, int W, int H, hipStream_t s)
+                            float4 *emis4, int W, int H, hipStream_t s, const int *gate)
 {
     const int bxc = (W + 63) / 64, byc = (H + kSeedRows - 1) / kSeedRows;
-    hipLaunchKernelGGL(k_seed_init, dim3(bxc * byc), dim3(kBlock), 0, s, depth, dc, emissive, seed, emis4, W, H, bxc);
+    hipLaunchKernelGGL(k_seed_init, dim3(bxc * byc), dim3(kBlock), 0, s, depth, dc, emissive, seed, emis4, W, H, bxc,
+                       gate);
     return hipGetLastError();
 }
 
 hipError_t launch_jfa_pass(const uint32_t *in, uint32_t *out, uint8_t *sdf, int W, int H, int step, bool final_pass,
-                           hipStream_t s)
+                           hipStream_t s, const int *gate)
 {
     const int bxc = (W + 63) / 64, byc = (H + 3) / 4;
     if (final_pass)
-        hipLaunchKernelGGL(k_jfa_pass<true>, dim3(bxc * byc), dim3(kBlock), 0, s, in, out, sdf, W, H, step, bxc);
+        hipLaunchKernelGGL(k_jfa_pass<true>, dim3(bxc * byc), dim3(kBlock), 0, s, in, out, sdf, W, H, step, bxc, gate);
     else
-        hipLaunchKernelGGL(k_jfa_pass<false>, dim3(bxc * byc), dim3(kBlock), 0, s, in, out, sdf, W, H, step, bxc);
+        hipLaunchKernelGGL(k_jfa_pass<false>, dim3(bxc * byc), dim3(kBlock), 0, s, in, out, sdf, W, H, step, bxc, gate);
     return hipGetLastError();
 }
 
 template <int S> static void launch_tail(const uint32_t *in, uint32_t *seeds_out, uint8_t *sdf, int W, int H,
-                                         hipStream_t s)
+                                         hipStream_t s, const int *gate)
 {
     constexpr int T = jfa_tail_tile<S>();
     const int bxc = (W + T - 1) / T, byc = (H + T - 1) / T;
-    hipLaunchKernelGGL(k_jfa_tail<S>, dim3(bxc * byc), dim3(kTailBlock), 0, s, in, seeds_out, sdf, W, H, bxc);
+    hipLaunchKernelGGL(k_jfa_tail<S>, dim3(bxc * byc), dim3(kTailBlock), 0, s, in, seeds_out, sdf, W, H, bxc, gate);
 }
 
 hipError_t launch_jfa_tail(const uint32_t *in, uint32_t *seeds_out, uint8_t *sdf, int W, int H, int tail_step,
-                           hipStream_t s)
+                           hipStream_t s, const int *gate)
 {
     switch (tail_step) {
     case 0: {
         const int n = W * H;
-        hipLaunchKernelGGL(k_jfa_none, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, in, seeds_out, sdf, W, n);
+        hipLaunchKernelGGL(k_jfa_none, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, in, seeds_out, sdf, W, n,
+                           gate);
         break;
     }
-    case 1: launch_tail<1>(in, seeds_out, sdf, W, H, s); break;
-    case 2: launch_tail<2>(in, seeds_out, sdf, W, H, s); break;
-    case 4: launch_tail<4>(in, seeds_out, sdf, W, H, s); break;
-    case 8: launch_tail<8>(in, seeds_out, sdf, W, H, s); break;
+    case 1: launch_tail<1>(in, seeds_out, sdf, W, H, s, gate); break;
+    case 2: launch_tail<2>(in, seeds_out, sdf, W, H, s, gate); break;
+    case 4: launch_tail<4>(in, seeds_out, sdf, W, H, s, gate); break;
+    case 8: launch_tail<8>(in, seeds_out, sdf, W, H, s, gate); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -903,30 +903,61 @@ hipError_t launch_surr_sample(const SurrSampleArgs &a, Storage st, hipStream_t s
     return hipGetLastError();
 }
 
-// ---- fallback patch: flagged tiles take the plain path's merged level 1 ----------
-// A tile is 8x8 level-2 probes = 16x16 level-1 probes.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_tile_patch(const T *__restrict__ plain, T *__restrict__ surr,
-                                                       const float *__restrict__ tile_err, float threshold, int PX,
-                                                       int PY, int tiles_x)
+// ---- tile fallback: the probes the flagged tiles need -------------------------
+// A tile is 8x8 level-2 probes = 16x16 level-1 probes (64x64 px).  Level 1
+// needs the tiles' own probes; level i + 1 the probes level i's bilinear
+// merge reads, [(lo - 1) >> 1, ((hi - 1) >> 1) + 1] per axis, clamped to the
+// grid.  The first kernel clears the masks, the second marks one flagged
+// tile's rectangle at every level per workgroup (overlapping rectangles store
+// the same byte).  Both do nothing when no tile is flagged.
+__global__ __launch_bounds__(kBlock) void k_fallback_clear(FallbackMaskArgs a)
 {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= PX * PY * 4) return;
-    const int probe = i >> 2, px = probe % PX, py = probe / PX;
-    if (!(tile_err[(py >> 4) * tiles_x + (px >> 4)] <= threshold)) surr[i] = plain[i];  // NaN: flagged
+    if (*a.count == 0) return;
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < a.need_bytes; i += (size_t)gridDim.x * kBlock)
+        a.need[i] = 0;
 }
 
-hipError_t launch_tile_patch(const void *plain, void *surr, const float *tile_err, float threshold, int PX, int PY,
-                             int tiles_x, Storage st, hipStream_t s)
+__global__ __launch_bounds__(kBlock) void k_fallback_mark(FallbackMaskArgs a)
 {
-    const int n = PX * PY * 4;
-    const dim3 g((n + kBlock - 1) / kBlock);
-    if (st == Storage::F32)
-        hipLaunchKernelGGL(k_tile_patch<float4>, g, dim3(kBlock), 0, s, (const float4 *)plain, (float4 *)surr, tile_err,
-                           threshold, PX, PY, tiles_x);
-    else
-        hipLaunchKernelGGL(k_tile_patch<uint2>, g, dim3(kBlock), 0, s, (const uint2 *)plain, (uint2 *)surr, tile_err,
-                           threshold, PX, PY, tiles_x);
+    const int n = *a.count;
+    for (int t = blockIdx.x; t < n && t < a.max_tiles; t += gridDim.x) {
+        const int tile = a.list[t];
+        int x0 = (tile % a.tiles_x) * 16, y0 = (tile / a.tiles_x) * 16;
+        int x1 = min(x0 + 15, a.PX[1] - 1), y1 = min(y0 + 15, a.PY[1] - 1);
+        for (int i = 1; i < a.L; ++i) {
+            if (i > 1) {
+                x0 = max((x0 - 1) >> 1, 0);
+                y0 = max((y0 - 1) >> 1, 0);
+                x1 = min(((x1 - 1) >> 1) + 1, a.PX[i] - 1);
+                y1 = min(((y1 - 1) >> 1) + 1, a.PY[i] - 1);
+            }
+            const int w = x1 - x0 + 1, h = y1 - y0 + 1;
+            uint8_t *m = a.need + a.need_off[i];
+            for (int j = threadIdx.x; j < w * h; j += kBlock) m[(size_t)(y0 + j / w) * a.PX[i] + x0 + j % w] = 1;
+        }
+    }
+}
+
+hipError_t launch_fallback_masks(const FallbackMaskArgs &a, hipStream_t s)
+{
+    if (a.need_bytes == 0 || a.max_tiles < 1) return hipSuccess;
+    const size_t per = (size_t)kBlock * 16;
+    const int clear_blocks = (int)((a.need_bytes + per - 1) / per < 256 ? (a.need_bytes + per - 1) / per : 256);
+    hipLaunchKernelGGL(k_fallback_clear, dim3(clear_blocks), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_fallback_mark, dim3(min(a.max_tiles, 1024)), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+__global__ void k_fallback_tally(const int *__restrict__ count, unsigned long long *__restrict__ tally)
+{
+    const int n = *count;
+    tally[0] += (unsigned long long)n;
+    tally[1] += n > 0 ? 1ull : 0ull;
+}
+
+hipError_t launch_fallback_tally(const int *count, unsigned long long *tally, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fallback_tally, dim3(1), dim3(1), 0, s, count, tally);
     return hipGetLastError();
 }
 

@@ -258,6 +258,63 @@ def test_fallback_everywhere_is_the_plain_path(rc):
     assert torch.equal(plain, surr)
 
 
+@pytest.mark.parametrize("W,H,L,seed,nb,ne", [(512, 384, 5, 1, 32, 4), (1920, 1080, 6, 0, 32, 4)])
+def test_fallback_of_some_tiles(rc, W, H, L, seed, nb, ne):
+    """A finite threshold at the median tile estimate flags about half the
+    tiles, decided on the device.  The flagged tiles' level 1 equals the plain
+    frame's bit for bit (plain far field on the jump-flood field, masked to the
+    probes they need), the others equal the learned frame's bit for bit, and
+    the irradiance equals the oracle's composition (plain FP64 level 1 on the
+    flagged tiles, the network's FP64 level 1 elsewhere, level 0 on the near
+    field) within the frame tolerance."""
+    from oracle import rco
+    from oracle import surrogate as so
+    _say(f"[partial fallback] {W}x{H} L={L} seed={seed}")
+    depth, emis = _gbuffer(rc, W, H, seed, nb, ne)
+    params, mean, istd, scale = _net()
+    dev = "cuda:0"
+    irr_p, irr_l, irr_f = (torch.empty((3, H, W), dtype=torch.float32, device=dev) for _ in range(3))
+    with rc.RadianceCascadeStage(W, H, levels=L) as plain, rc.RadianceCascadeStage(W, H, levels=L) as st:
+        a1_p = torch.empty(plain.level_shape(1), device=dev)
+        a1_l, a1_f = torch.empty_like(a1_p), torch.empty_like(a1_p)
+        plain.run_device(depth, emis, irr_p)
+        plain.debug_level(1, a1_p)
+        st.mlp_set(params, mean, istd, scale)
+        st.set_surrogate(True)
+        st.run_device(depth, emis, irr_l)
+        st.debug_level(1, a1_l)
+        (ty, tx) = st.surrogate_shapes()["tiles"]
+        terr = torch.empty((ty, tx), device=dev)
+        st.debug_surrogate(tile_err=terr)
+        torch.cuda.synchronize()
+        est = terr.cpu().numpy()
+        thr = float(np.median(est))
+        st.reset_stats()
+        st.set_surrogate(True, threshold=thr)
+        st.run_device(depth, emis, irr_f)
+        st.debug_level(1, a1_f)
+        torch.cuda.synchronize()
+        s = st.stats()
+    flagged = ~(est <= thr)
+    n_flag = int(flagged.sum())
+    _say(f"  {n_flag} of {flagged.size} tiles flagged")
+    assert 0 < n_flag < flagged.size
+    assert s.fallback_frames == 1 and s.flagged_tiles == n_flag
+    py, px = a1_p.shape[:2]
+    probe_flag = np.repeat(np.repeat(flagged, 16, axis=0), 16, axis=1)[:py, :px]
+    f, p, l = a1_f.cpu().numpy(), a1_p.cpu().numpy(), a1_l.cpu().numpy()
+    assert np.array_equal(f[probe_flag].view(np.uint32), p[probe_flag].view(np.uint32))
+    assert np.array_equal(f[~probe_flag].view(np.uint32), l[~probe_flag].view(np.uint32))
+    d, e = depth.cpu().numpy(), emis.cpu().numpy()
+    _, ref = so.run(d, e, L, params, mean, istd, float(scale), want=True)
+    _, lv64 = rco.run(d, e, L, np.float64, want_levels=True)
+    a1_mix = np.where(probe_flag[:, :, None, None], lv64[1], ref["A1"])
+    irr_ref = rco.gather(W, H, rco.merge_level(W, H, 0, ref["m0"], a1_mix))
+    r_a1, r_irr = rel_l2(f, a1_mix), rel_l2(irr_f.cpu().numpy(), irr_ref)
+    _say(f"  level 1 rel-L2 {r_a1:.2e}, irradiance rel-L2 {r_irr:.2e} vs the oracle's composition")
+    assert r_a1 <= FRAME_REL_L2 and r_irr <= FRAME_REL_L2
+
+
 def test_surrogate_graph_replay_matches_direct_launches(rc):
     W, H, L = 240, 160, 5
     depth, emis = _gbuffer(rc, W, H, 9, 8, 2)
