@@ -151,6 +151,7 @@ rc_status rc_render_host(rc_ctx *ctx, float *irradiance_host);
 rc_status rc_host_alloc(uint64_t bytes, void **out);
 void rc_host_free(void *ptr);
 rc_status rc_run_device(rc_ctx *ctx, const rc_gbuffer *dev, float *irradiance_dev, void *stream);
+
 void rc_destroy(rc_ctx *ctx);
 
 /* ---- level-kernel tiling (autotuner) ----------------------------------------
@@ -172,14 +173,12 @@ rc_status rc_set_jfa_fuse(rc_ctx *ctx, uint32_t fused_step);
 uint32_t rc_get_jfa_fuse(const rc_ctx *ctx);
 /* split = 1: every level >= 1 is marched by one launch (top level first, so
  * the longest rays start earliest and the levels' dependent chains overlap),
- * then the levels are merged top-down from the stored ray codes; split = 2:
- * the same march by a persistent grid of resident waves, each walking an
- * interleaved share of 64-ray packets and handing a new ray to a lane as soon
- * as its ray ends (rays_per_lane must be 1); split = 0: each level kernel
- * marches and merges its own rays.  speculation (1, 2, 4) is the split
- * march's look-ahead depth; rays_per_lane (1, 2, 4, 8) how many rays a lane
- * of the split = 1 march takes one after another.  The result is identical;
- * rc_autotune also picks these. */
+ * then the levels are merged top-down from the stored ray codes; split = 0:
+ * each level kernel marches and merges its own rays.  speculation (1, 2, 4)
+ * is the split march's look-ahead depth; rays_per_lane (1, 2, 4, 8) how many
+ * rays a lane of the split march takes one after another.  The result is
+ * identical; rc_autotune also picks these.  (split = 2, a persistent form of
+ * the split march, is switched off and refused with RC_ERR_INVALID.) */
 rc_status rc_set_march(rc_ctx *ctx, uint32_t split, uint32_t speculation, uint32_t rays_per_lane);
 rc_status rc_get_march(const rc_ctx *ctx, uint32_t *split, uint32_t *speculation, uint32_t *rays_per_lane);
 

@@ -331,8 +331,7 @@ def test_every_march_mode_is_bit_exact(rc, oracle, W, H, L, seed):
     hits_ref = [oracle.march_level(sdf, e, i)[1].ravel() for i in range(L)]
     irr = torch.empty((3, H, W), dtype=torch.float32, device="cuda:0")
     with rc.RadianceCascadeStage(W, H, levels=L, debug=True) as st:
-        for split, spec, rpl in [(0, 1, 1), (1, 1, 1), (1, 2, 1), (1, 4, 1), (1, 1, 2), (1, 2, 4), (1, 4, 8),
-                                 (2, 1, 1), (2, 2, 1), (2, 4, 1)]:
+        for split, spec, rpl in [(0, 1, 1), (1, 1, 1), (1, 2, 1), (1, 4, 1), (1, 1, 2), (1, 2, 4), (1, 4, 8)]:
             st.set_march(split, spec, rpl)
             assert st.march() == (split, spec, rpl)
             st.run_device(depth, emis, irr)
@@ -354,9 +353,30 @@ def test_every_march_mode_is_bit_exact(rc, oracle, W, H, L, seed):
         with pytest.raises(rc.RcError):
             st.set_march(1, 1, 3)
         with pytest.raises(rc.RcError):
-            st.set_march(2, 1, 2)
+            st.set_march(2, 1, 1)      # the persistent form is switched off
         with pytest.raises(rc.RcError):
             st.set_march(3, 1, 1)
+
+
+PERSISTENT_OFF = ("the persistent split march (rc_set_march split 2) is switched off: a GPU run of the "
+                  "batch test that autotuned it faulted the GPU, and the cause has not been shown")
+
+
+@pytest.mark.skip(reason=PERSISTENT_OFF)
+@pytest.mark.parametrize("spec", [1, 2, 4])
+def test_persistent_march_is_bit_exact(rc, oracle, spec):
+    """The persistent split march (split 2) stores the same ray codes and
+    frame as the per-level march."""
+    W, H, L, seed = 512, 384, 5, 1
+    depth, emis = _device_gbuffer(rc, W, H, seed, 32, 4)
+    d, _, e = oracle.gbuffer(seed, W, H, 32, 4)
+    irr32 = oracle.run(d, e, L, np.float32)
+    irr = torch.empty((3, H, W), dtype=torch.float32, device="cuda:0")
+    with rc.RadianceCascadeStage(W, H, levels=L) as st:
+        st.set_march(2, spec, 1)
+        st.run_device(depth, emis, irr)
+        torch.cuda.synchronize()
+    assert np.array_equal(irr.cpu().numpy().view(np.uint32), irr32.view(np.uint32))
 
 
 def _round_storage(a, storage):

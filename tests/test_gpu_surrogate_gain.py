@@ -156,3 +156,72 @@ def test_error_head_ranks_tiles_by_their_true_error(rc):
     assert np.median(overlaps) >= 0.4
     plain.close()
     learned.close()
+
+
+SWEEP_SEEDS = [0, 101, 102, 106, 108, 111, 112]       # the benchmark scene, then lit held-out scenes
+SWEEP_THRESHOLDS = [float("inf"), 2.0, 1.0, 0.5, 0.25, 0.1, -1.0]
+
+
+def _fps(st, depth, emis, out, frames=20):
+    for _ in range(5):
+        st.run_device(depth, emis, out)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(frames):
+        st.run_device(depth, emis, out)
+    e1.record()
+    torch.cuda.synchronize()
+    return 1000.0 * frames / e0.elapsed_time(e1)
+
+
+def test_fallback_threshold_sweep(rc):
+    """The learned frame with the device-side tile fallback at a range of
+    thresholds (mean predicted squared error per tile, network units): frames/s,
+    the ratio to the plain frame, the fraction of tiles flagged and the
+    irradiance rel-L2 against the FP64 oracle, on the benchmark scene and on
+    lit held-out scenes.  The table is printed (the design notes quote it).
+    Checked: every tile flagged gives the plain frame bit for bit; fewer tiles
+    are flagged as the threshold rises; the fallback lowers the error."""
+    from oracle import rco
+    rco.set_threads(16)
+    dev = torch.device("cuda:0")
+    depth = torch.empty((H, W), device=dev)
+    emis = torch.empty((3, H, W), device=dev)
+    plain = rc.RadianceCascadeStage(W, H, levels=L)
+    learned = rc.RadianceCascadeStage(W, H, levels=L)
+    learned.load_weights()
+    rows = {}
+    for seed in SWEEP_SEEDS:
+        rc.gbuffer_raster(rc.scene_generate(seed, 32, 4), W, H, depth, None, emis)
+        ref = torch.from_numpy(rco.run(depth.cpu().numpy(), emis.cpu().numpy(), L)).to(dev)
+        out_p = torch.empty((3, H, W), device=dev)
+        fps_p = _fps(plain, depth, emis, out_p)
+        e_p = plain.error_norms(out_p, ref)["rel_l2"]
+        _say(f"  seed {seed}: plain {fps_p:.0f} frames/s, rel-L2 vs FP64 {e_p:.2e}")
+        prev_flag = None
+        for thr in SWEEP_THRESHOLDS:
+            learned.set_surrogate(True, threshold=thr)
+            out = torch.empty((3, H, W), device=dev)
+            fps = _fps(learned, depth, emis, out)
+            learned.reset_stats()
+            learned.run_device(depth, emis, out)
+            torch.cuda.synchronize()
+            s = learned.stats()
+            frac = s.flagged_tiles / max(s.tiles_per_frame, 1)
+            err = learned.error_norms(out, ref)["rel_l2"]
+            rows.setdefault(thr, []).append((fps / fps_p, err, frac))
+            _say(f"    threshold {thr:>5}: {fps:7.0f} frames/s = {fps / fps_p:4.2f}x plain, "
+                 f"{100 * frac:5.1f}% tiles flagged, rel-L2 vs FP64 {err:.4f}")
+            if thr < 0:
+                assert torch.equal(out, out_p)
+            if prev_flag is not None:
+                assert frac >= prev_flag - 1e-12
+            prev_flag = frac
+    _say("  median over the scenes (threshold: ratio, rel-L2, flagged):")
+    for thr in SWEEP_THRESHOLDS:
+        r = np.array(rows[thr])
+        _say(f"    {thr:>5}: {np.median(r[:, 0]):4.2f}x  {np.median(r[:, 1]):.4f}  {100 * np.median(r[:, 2]):5.1f}%")
+    assert np.median(np.array(rows[0.25])[:, 1]) < np.median(np.array(rows[float("inf")])[:, 1])
+    plain.close()
+    learned.close()
