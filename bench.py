@@ -286,10 +286,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
-    ap.add_argument("--no-batch", action="store_true", help="(the default; kept for older command lines)")
-    ap.add_argument("--batch", action="store_true",
-                    help="also run the 64-view batch leg (off by default: a GPU run of the batch test faulted the "
-                         "GPU and its cause has not been shown)")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--weights", default=None, help="surrogate weights (default: the packaged network)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 256 MiB device buffer 10 times (the reference kernel for scaling the "
@@ -380,7 +377,7 @@ def main():
     if not args.no_surrogate and L >= 3:
         surrogate = surrogate_leg(rc, torch, W, H, L, args, depth, emis, irr, stream, fps)
 
-    batch = batch_leg(rc, torch, args, stream) if args.batch and not args.no_batch else None
+    batch = None if args.no_batch else batch_leg(rc, torch, args, stream)
     staging = host_staging_leg(rc, W, H, L, args.storage, depth, emis)
 
     cpu = None

@@ -77,7 +77,7 @@ typedef enum { RC_STORAGE_F32 = 0, RC_STORAGE_F16 = 1, RC_STORAGE_BF16 = 2 } rc_
  *                    in [0,1] of glam's perspective_rh (src/camera.rs:77-79,
  *                    src/texture.rs:149): d = f (D - n) / ((f - n) D) for the
  *                    near / far planes n = z_near, f = z_far (the reference
- *                    uses 0.1 and 100, src/camera.rs:26 via src/app.rs).
+ *                    uses 0.1 and 100: Projection::new at src/app.rs:26).
  *                    The pass converts every depth it reads to
  *                    D = (n f) / (f - d (f - n)) in fp32, n f and f - n
  *                    rounded to fp32 first, then applies the linear rule. */
@@ -173,12 +173,14 @@ rc_status rc_set_jfa_fuse(rc_ctx *ctx, uint32_t fused_step);
 uint32_t rc_get_jfa_fuse(const rc_ctx *ctx);
 /* split = 1: every level >= 1 is marched by one launch (top level first, so
  * the longest rays start earliest and the levels' dependent chains overlap),
- * then the levels are merged top-down from the stored ray codes; split = 0:
- * each level kernel marches and merges its own rays.  speculation (1, 2, 4)
- * is the split march's look-ahead depth; rays_per_lane (1, 2, 4, 8) how many
- * rays a lane of the split march takes one after another.  The result is
- * identical; rc_autotune also picks these.  (split = 2, a persistent form of
- * the split march, is switched off and refused with RC_ERR_INVALID.) */
+ * then the levels are merged top-down from the stored ray codes; split = 2:
+ * the same march by a persistent grid of resident waves, each walking an
+ * interleaved share of 64-ray packets and handing a new ray to a lane as soon
+ * as its ray ends (rays_per_lane must be 1); split = 0: each level kernel
+ * marches and merges its own rays.  speculation (1, 2, 4) is the split
+ * march's look-ahead depth; rays_per_lane (1, 2, 4, 8) how many rays a lane
+ * of the split = 1 march takes one after another.  The result is identical;
+ * rc_autotune also picks these (a one-level frame has nothing to march). */
 rc_status rc_set_march(rc_ctx *ctx, uint32_t split, uint32_t speculation, uint32_t rays_per_lane);
 rc_status rc_get_march(const rc_ctx *ctx, uint32_t *split, uint32_t *speculation, uint32_t *rays_per_lane);
 

@@ -1148,13 +1148,11 @@ uint32_t rc_get_jfa_fuse(const rc_ctx *c) { return c ? (uint32_t)c->jfa_fuse : 0
 rc_status rc_set_march(rc_ctx *c, uint32_t split, uint32_t speculation, uint32_t rays_per_lane)
 {
     if (!c) return fail(nullptr, RC_ERR_NULL, "rc_set_march: ctx is NULL");
-    if (split == 2)
+    if (split > 2 || (speculation != 1 && speculation != 2 && speculation != 4) ||
+        (rays_per_lane != 1 && rays_per_lane != 2 && rays_per_lane != 4 && rays_per_lane != 8) ||
+        (split == 2 && rays_per_lane != 1))
         return fail(c, RC_ERR_INVALID,
-                    "rc_set_march: the persistent split march (split 2) is switched off: a frame batch that "
-                    "autotuned it faulted the GPU and the cause is not yet shown");
-    if (split > 1 || (speculation != 1 && speculation != 2 && speculation != 4) ||
-        (rays_per_lane != 1 && rays_per_lane != 2 && rays_per_lane != 4 && rays_per_lane != 8))
-        return fail(c, RC_ERR_INVALID, "rc_set_march: split 0/1, speculation 1/2/4, rays_per_lane 1/2/4/8");
+                    "rc_set_march: split 0/1/2, speculation 1/2/4, rays_per_lane 1/2/4/8 (1 with split 2)");
     c->march_split = (int)split;
     c->spec = (int)speculation;
     c->march_m = (int)rays_per_lane;
@@ -1215,14 +1213,16 @@ rc_status rc_autotune(rc_ctx *c, const rc_gbuffer *g, float *irr, uint32_t iters
         }
         c->jfa_fuse = best_fuse;
     }
-    {
+    // A frame with one level marches nothing beyond level 0: there is no
+    // march mode to tune (and no split march to run).
+    if (c->L > 1) {
         float best = 1e30f;
         int best_split = c->march_split, best_spec = c->spec, best_m = c->march_m;
-        // (the persistent form, split 2, is switched off; see rc_set_march)
-        for (int split : {0, 1})
+        for (int split : {0, 1, 2})
         for (int spec : {1, 2, 4})
             for (int m : {1, 2, 4, 8}) {
                 if (!split && (spec != 1 || m != 1)) continue;
+                if (split == 2 && m != 1) continue;
                 c->march_split = split;
                 c->spec = spec;
                 c->march_m = m;

@@ -676,7 +676,7 @@ static hipError_t launch_march_all_spec(const MarchAllArgs &a, int m, int blocks
 
 hipError_t launch_march_persist(const MarchAllArgs &a, int spec, hipStream_t s)
 {
-    if (a.total_rays == 0 || a.n_waves == 0) return hipSuccess;
+    if (a.nlev < 1 || a.total_rays == 0 || a.n_waves == 0) return hipSuccess;
     const dim3 grid((a.n_waves + 3) / 4);
     switch (spec) {
     case 1: hipLaunchKernelGGL(k_march_persist<1>, grid, dim3(kBlock), 0, s, a); break;
@@ -689,6 +689,7 @@ hipError_t launch_march_persist(const MarchAllArgs &a, int spec, hipStream_t s)
 
 hipError_t launch_march_all(const MarchAllArgs &a, int spec, int m, hipStream_t s)
 {
+    if (a.nlev < 1) return hipSuccess;  // one-level frame: no level >= 1 to march
     const int blocks = a.lev[a.nlev - 1].block_begin + a.lev[a.nlev - 1].blocks;
     if (blocks <= 0) return hipSuccess;
     switch (spec) {

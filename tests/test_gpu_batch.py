@@ -25,11 +25,6 @@ def rc():
     return rc
 
 
-BATCH_FAULTED = ("a GPU run of this test faulted the GPU while the autotuner tried the persistent split march; "
-                 "that march is switched off, but the cause has not been shown, so the batch case stays off the GPU")
-
-
-@pytest.mark.skip(reason=BATCH_FAULTED)
 def test_batch_matches_single_views(rc):
     dev = torch.device("cuda:0")
     views, refs = [], []
@@ -63,7 +58,6 @@ def test_batch_rejects_bad_arguments(rc):
         rc.RadianceCascadeBatch([(None, None, None)])
 
 
-@pytest.mark.skip(reason=BATCH_FAULTED)
 def test_batch_workload_matches_single_views_and_oracle(rc):
     """The 64-view batch workload (rc.batch_views_spec(1000, 64): seeds
     1000..1063, 16..64 boxes, W and H from {256, ..., 2048}, 5..7 levels by the
