@@ -270,6 +270,88 @@ def batch_leg(rc, torch, args, stream):
             "streams": 8, "launches": "one graph launch per batch"}
 
 
+# kernel name prefix -> family (the plain frame's kernels)
+KERNEL_FAMILY = [("k_seed_init", "seed"), ("k_jfa_pass", "jfa"), ("k_jfa_tail", "jfa"), ("k_jfa_none", "jfa"),
+                 ("k_march_all", "march"), ("k_march_persist", "march"), ("k_level0", "level0"),
+                 ("k_level<", "levels")]
+
+
+def _family_of(name):
+    short = name.split("rc::", 1)[-1]
+    for prefix, fam in KERNEL_FAMILY:
+        if short.startswith(prefix):
+            return fam
+    return None
+
+
+def _rows(path):
+    import csv
+    with open(path, newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def _col(row, *names):
+    low = {k.lower(): v for k, v in row.items()}
+    for n in names:
+        if n.lower() in low:
+            return low[n.lower()]
+    raise KeyError(names)
+
+
+def _counter_per_kernel(path, counter):
+    """{kernel name: (dispatches, mean counter value)} from a rocprofv3
+    counter-collection CSV (one row per dispatch and counter)."""
+    acc = {}
+    for r in _rows(path):
+        if _col(r, "Counter_Name") != counter:
+            continue
+        name = _col(r, "Kernel_Name")
+        n, v = acc.get(name, (0, 0.0))
+        acc[name] = (n + 1, v + float(_col(r, "Counter_Value")))
+    return {k: (n, v / n) for k, (n, v) in acc.items()}
+
+
+def summarize_profile(trace_stats, fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, copy_bytes, workload):
+    """Markdown summary of one profiled bench configuration: per kernel the
+    calls and mean duration (kernel trace) and HBM bytes per dispatch
+    (FETCH_SIZE and WRITE_SIZE in KiB, each scaled by what the same counter
+    reads for a plain device copy of known size), then per kernel family the
+    time and bytes per frame (frames = level-0 launches)."""
+    stats = {}
+    for r in _rows(trace_stats):
+        stats[_col(r, "Name", "Kernel_Name")] = (int(_col(r, "Calls")), float(_col(r, "AverageNs")) / 1e3)
+    fetch = _counter_per_kernel(fetch_csv, "FETCH_SIZE")
+    write = _counter_per_kernel(write_csv, "WRITE_SIZE")
+    cf = _counter_per_kernel(copy_fetch_csv, "FETCH_SIZE")
+    cw = _counter_per_kernel(copy_write_csv, "WRITE_SIZE")
+    copy_name = max(cf, key=lambda k: cf[k][1])
+    kf = copy_bytes / (cf[copy_name][1] * 1024.0)   # bytes per FETCH_SIZE KiB unit
+    kw = copy_bytes / (cw[copy_name][1] * 1024.0)
+    frames = sum(c for n, (c, _) in stats.items() if _family_of(n) == "level0")
+    lines = [f"Workload: {workload}", "",
+             f"Copy calibration ({copy_name}, {copy_bytes} B read and written per launch): FETCH_SIZE reads "
+             f"{cf[copy_name][1]:.0f} KiB, WRITE_SIZE {cw[copy_name][1]:.0f} KiB, so bytes = FETCH_SIZE x {kf * 1024:.3f} "
+             f"and WRITE_SIZE x {kw * 1024:.3f} (per KiB).", "",
+             f"Frames in the run: {frames}", "",
+             "| kernel | family | calls | avg us | read MB / dispatch | written MB / dispatch |",
+             "|---|---|---|---|---|---|"]
+    fam = {}
+    for name, (calls, us) in sorted(stats.items(), key=lambda kv: -kv[1][0] * kv[1][1]):
+        f = _family_of(name)
+        rd = fetch.get(name, (0, float("nan")))[1] * 1024 * kf / 1e6
+        wr = write.get(name, (0, float("nan")))[1] * 1024 * kw / 1e6
+        short = name if len(name) < 90 else name[:87] + "..."
+        lines.append(f"| `{short}` | {f or '-'} | {calls} | {us:.2f} | {rd:.2f} | {wr:.2f} |")
+        if f and frames:
+            t, b, n = fam.get(f, (0.0, 0.0, 0.0))
+            fam[f] = (t + calls * us / frames, b + calls * (rd + wr) / frames, n + calls / frames)
+    lines += ["", "Per kernel family and frame (the bench's `traffic` reads the `family:` rows):", "",
+              "| family | launches / frame | us / frame | HBM MB / frame |", "|---|---|---|---|"]
+    for f, (t, b, n) in fam.items():
+        lines.append(f"| `family:{f}` | {n:.2f} | {t:.2f} | {b:.2f} |")
+    return "\n".join(lines)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -288,10 +370,20 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--weights", default=None, help="surrogate weights (default: the packaged network)")
+    ap.add_argument("--save-tuning", default=None, help="write the autotuned tiling / march / jump-flood choice (JSON)")
+    ap.add_argument("--load-tuning", default=None, help="use a saved tuning instead of autotuning (profiled runs)")
+    ap.add_argument("--summarize-profile", nargs=6, default=None,
+                    metavar=("TRACE_STATS", "FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE", "WORKLOAD"),
+                    help="print the markdown summary of rocprofv3 CSVs of this bench (kernel stats, FETCH_SIZE and "
+                         "WRITE_SIZE runs of the bench and of --copy-calibration)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 256 MiB device buffer 10 times (the reference kernel for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters of rocprofv3 to bytes)")
     args = ap.parse_args()
+    if args.summarize_profile:
+        print(summarize_profile(*args.summarize_profile[:5], copy_bytes=4 * (64 << 20),
+                                workload=args.summarize_profile[5]))
+        return
     if args.copy_calibration:
         import torch
         n = 64 << 20
@@ -323,7 +415,21 @@ def main():
     stream = torch.cuda.current_stream()
 
     stage = rc.RadianceCascadeStage(W, H, levels=L, storage=args.storage)
-    tiling = stage.autotune(depth, emis, irr, iters=3, stream=stream) if not args.no_autotune else stage.tiling()
+    if args.load_tuning:
+        tun = json.load(open(args.load_tuning))
+        for lvl, (shape, rpt) in tun["tiling"].items():
+            stage.set_tiling(int(lvl), shape, rpt)
+        stage.set_march(*tun["march"])
+        stage.set_jfa_fuse(tun["jfa_fuse"])
+        tiling = stage.tiling()
+    elif not args.no_autotune:
+        tiling = stage.autotune(depth, emis, irr, iters=3, stream=stream)
+    else:
+        tiling = stage.tiling()
+    if args.save_tuning:
+        inv = {v: k for k, v in rc._abi.TILE_SHAPES.items()}
+        json.dump({"tiling": {str(l): [inv[tuple(v[0])], v[1]] for l, v in tiling.items()},
+                   "march": list(stage.march()), "jfa_fuse": stage.jfa_fuse()}, open(args.save_tuning, "w"))
     for _ in range(args.warmup):
         stage.run_device(depth, emis, irr, stream)
     torch.cuda.synchronize()

@@ -492,11 +492,14 @@ __global__ __launch_bounds__(kBlock) void k_march_all(MarchAllArgs a)
 // level: the rays are cut into packets of 64 consecutive codes, numbered
 // from the top level down (the longest rays first), and wave w owns packets
 // w, w + n_waves, w + 2 n_waves, ... (an even mix of levels and image
-// regions per wave).  Each iteration the wave hands the next entries of its
-// packet sequence to its idle lanes (a ballot and a lane rank; the position
-// in the sequence is a scalar, no memory atomics) and every busy lane takes
-// one march round.  Each ray's samples, and so every code, are those of
-// k_march_all; only the order in which rays are marched differs.
+// regions per wave).  Each lane holds its current ray and the next one,
+// prepared ahead (its direction load issued an iteration or more before the
+// ray starts, so starting it costs no dependent load): every iteration a lane
+// whose ray ended takes its prepared one, the lanes without a prepared ray
+// take the next entries of the wave's packet sequence (a ballot and a lane
+// rank; the position in the sequence is a scalar, no memory atomics), and
+// every busy lane takes one march round.  Each ray's samples, and so every
+// code, are those of k_march_all; only the order of the rays differs.
 template <int SPEC>
 __global__ __launch_bounds__(kBlock) void k_march_persist(MarchAllArgs a)
 {
@@ -520,18 +523,30 @@ __global__ __launch_bounds__(kBlock) void k_march_persist(MarchAllArgs a)
     const uint32_t np = a.n_packets, nw = a.n_waves;
     // this wave's sequence length (waves past n_waves in the last workgroup have none)
     const uint32_t vend = (wave < nw && wave < np) ? ((np - 1 - wave) / nw + 1) * 64 : 0;
-    uint32_t vnext = 0;                                                    // wave-uniform
+    uint32_t vnext = 0;  // wave-uniform
     Ray r = ray_begin(0.f, 0.f, 0.f, 0.f, 0.f, 0.f, false);
     float t1 = 0.f;
     uint32_t out = 0;
     bool have = false;
+    // the prepared next ray
+    float ncx = 0.f, ncy = 0.f, nt0 = 0.f, nt1 = 0.f;
+    float2 ndir = make_float2(0.f, 0.f);
+    uint32_t nout = 0;
+    bool have_n = false;
     for (;;) {
-        const uint64_t idle = __ballot(!have);
-        if (idle != 0 && vnext < vend) {
+        if (!have && have_n) {
+            r = ray_begin(ncx, ncy, ndir.x, ndir.y, nt0, nt1, true);
+            t1 = nt1;
+            out = nout;
+            have = true;
+            have_n = false;
+        }
+        const uint64_t want = __ballot(!have_n);
+        if (want != 0 && vnext < vend) {
             const uint32_t rank =
-                __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+                __builtin_amdgcn_mbcnt_hi((uint32_t)(want >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)want, 0u));
             const uint32_t v = vnext + rank;
-            if (!have && v < vend) {
+            if (!have_n && v < vend) {
                 const uint32_t pc = np - 1 - (wave + (v >> 6) * nw);  // packet in code order
                 const uint32_t ray = pc * 64 + (v & 63);
                 if (ray < a.total_rays) {
@@ -546,18 +561,19 @@ __global__ __launch_bounds__(kBlock) void k_march_persist(MarchAllArgs a)
                     int px = probe - py * PX;
                     if (px < 0) { --py; px += PX; }
                     else if (px >= PX) { ++py; px -= PX; }
-                    const float2 dir = a.dirs_base[tab_i[lv][3] + k];
+                    ndir = a.dirs_base[tab_i[lv][3] + k];  // used when the ray starts
                     const float sp = tab_f[lv][1];
-                    t1 = tab_f[lv][3];
-                    r = ray_begin(((float)px + 0.5f) * sp, ((float)py + 0.5f) * sp, dir.x, dir.y, tab_f[lv][2], t1,
-                                  true);
-                    out = ray;
-                    have = true;
+                    ncx = ((float)px + 0.5f) * sp;
+                    ncy = ((float)py + 0.5f) * sp;
+                    nt0 = tab_f[lv][2];
+                    nt1 = tab_f[lv][3];
+                    nout = ray;
+                    have_n = true;
                 }
             }
-            vnext += (uint32_t)__builtin_popcountll(idle);
+            vnext += (uint32_t)__builtin_popcountll(want);
         }
-        if (!__any(have)) {
+        if (!__any(have || have_n)) {
             if (vnext >= vend) break;
             continue;
         }

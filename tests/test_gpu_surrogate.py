@@ -19,11 +19,17 @@ torch = pytest.importorskip("torch")
 
 pytestmark = pytest.mark.gpu
 
+# Bounds about 10x the errors measured on MI355X (in parentheses).
 MIP_RTOL = 1e-5
-CONE_REL_L2 = 1e-4
-MLP_REL_L2 = 2e-3          # forward outputs vs the bf16-emulating FP64 network
-GRAD_REL_L2 = 1e-2         # per layer weight gradient
-FRAME_REL_L2 = 2e-3        # merged level 1 and irradiance vs the FP64 frame
+CONE_REL_L2 = 5e-6         # cone radiance (4.1e-7 at 1080p)
+CONE_T_MAX = 1e-4          # cone transmittance, max-abs (1.4e-5)
+MLP_REL_L2 = 1.5e-3        # forward outputs vs the bf16-emulating FP64 network (1.6e-4)
+GRAD_REL_L2 = 1e-6         # per layer weight and bias gradient (9e-8)
+FRAME_REL_L2 = 3e-5        # learned frame: merged level 1 and irradiance vs its FP64 oracle (2.9e-6)
+# the fallback's composition also carries the plain path's fp32-vs-FP64
+# error on the flagged tiles (8.7e-5 for level 1 at 1080p), bounded like the
+# plain path's own (1e-3)
+FALLBACK_REL_L2 = 1e-3
 
 
 def _say(msg):
@@ -126,7 +132,7 @@ def test_surrogate_frame_parity(rc, W, H, L, seed, nb, ne):
     r_a1 = rel_l2(a1.cpu().numpy(), ref["A1"])
     r_irr = rel_l2(irr.cpu().numpy(), ref_irr)
     _say(f"  cone rel-L2 {r_cone:.2e}  T max {t_cone:.2e}  level1 rel-L2 {r_a1:.2e}  irradiance rel-L2 {r_irr:.2e}")
-    assert r_cone <= CONE_REL_L2 and t_cone <= 1e-4
+    assert r_cone <= CONE_REL_L2 and t_cone <= CONE_T_MAX
     assert r_a1 <= FRAME_REL_L2 and r_irr <= FRAME_REL_L2
     assert np.isfinite(terr.cpu().numpy()).all()
     assert s.surrogate_frames == 1 and s.fallback_frames == 0 and s.tiles_per_frame == ty * tx
@@ -208,7 +214,7 @@ def test_mlp_gradient_and_adam_parity(rc):
     gl = so.params_to_layers(g)
     r_loss = abs(float(lo.item()) - loss) / abs(loss)
     _say(f"  loss {lo.item():.6f} vs {loss:.6f} (rel {r_loss:.2e})")
-    assert r_loss <= 1e-4
+    assert r_loss <= 1e-7          # (9.9e-9)
     for l in range(4):
         rw = rel_l2(gl[l][0], grads[l][0])
         rb = rel_l2(gl[l][1], grads[l][1])
@@ -238,7 +244,7 @@ def test_training_rows_parity(rc):
     x, t = so.samples(cone, lv[2])
     rx, rt = rel_l2(feats.cpu().numpy(), x), rel_l2(targ.cpu().numpy(), t)
     _say(f"  rows: features rel-L2 {rx:.2e}  targets rel-L2 {rt:.2e}")
-    assert rx <= 1e-4 and rt <= 5e-3
+    assert rx <= 1e-6 and rt <= 1e-5    # (8.6e-8, 7.6e-7)
 
 
 def test_fallback_everywhere_is_the_plain_path(rc):
@@ -312,7 +318,7 @@ def test_fallback_of_some_tiles(rc, W, H, L, seed, nb, ne):
     irr_ref = rco.gather(W, H, rco.merge_level(W, H, 0, ref["m0"], a1_mix))
     r_a1, r_irr = rel_l2(f, a1_mix), rel_l2(irr_f.cpu().numpy(), irr_ref)
     _say(f"  level 1 rel-L2 {r_a1:.2e}, irradiance rel-L2 {r_irr:.2e} vs the oracle's composition")
-    assert r_a1 <= FRAME_REL_L2 and r_irr <= FRAME_REL_L2
+    assert r_a1 <= FALLBACK_REL_L2 and r_irr <= FALLBACK_REL_L2
 
 
 def test_surrogate_graph_replay_matches_direct_launches(rc):
