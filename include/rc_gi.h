@@ -152,6 +152,24 @@ rc_status rc_host_alloc(uint64_t bytes, void **out);
 void rc_host_free(void *ptr);
 rc_status rc_run_device(rc_ctx *ctx, const rc_gbuffer *dev, float *irradiance_dev, void *stream);
 
+/* ---- allocation-free frame -----------------------------------------------------
+ * No counterpart in the reference: an entry point for callers that cannot
+ * allocate device memory (code under hipGraph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The plain GI frame of rc_run_device --
+ * linear depth, fp32 storage; the irradiance is equal bit for bit -- with no
+ * context: its working buffers (jump-flood fields, distance field, merged
+ * levels) live in a static 64 MiB device array that each GPU holds until the
+ * process ends (enough for 1920x1080 with 6 levels; rc_noalloc_scratch_bytes
+ * gives a size's need, larger sizes are refused with RC_ERR_INVALID), and
+ * its direction tables in a static array set on the first call on a device.
+ * It calls no hipMalloc, hipFree or other allocating routine.  Runs on the
+ * current device, stream ordered.  Every call on one GPU shares that GPU's
+ * scratch: calls that overlap on one GPU (several streams or threads) race.
+ * levels: 1..7.  Errors: rc_noalloc_last_error. */
+rc_status rc_run_device_noalloc(uint32_t width, uint32_t height, uint32_t levels, const rc_gbuffer *dev,
+                                float *irradiance_dev, void *stream);
+uint64_t rc_noalloc_scratch_bytes(uint32_t width, uint32_t height, uint32_t levels);  /* 0 if out of range */
+const char *rc_noalloc_last_error(void);
 void rc_destroy(rc_ctx *ctx);
 
 /* ---- level-kernel tiling (autotuner) ----------------------------------------

@@ -110,6 +110,9 @@ SIGNATURES = {
     "rc_update_host": (_i32, [_P, ctypes.POINTER(GBuffer)]),
     "rc_render_host": (_i32, [_P, _P]),
     "rc_run_device": (_i32, [_P, ctypes.POINTER(GBuffer), _P, _P]),
+    "rc_run_device_noalloc": (_i32, [_u32, _u32, _u32, ctypes.POINTER(GBuffer), _P, _P]),
+    "rc_noalloc_scratch_bytes": (ctypes.c_uint64, [_u32, _u32, _u32]),
+    "rc_noalloc_last_error": (ctypes.c_char_p, []),
     "rc_destroy": (None, [_P]),
     "rc_set_level_tiling": (_i32, [_P, _u32, _u32, _u32]),
     "rc_get_level_tiling": (_i32, [_P, _u32, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
