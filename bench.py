@@ -81,7 +81,7 @@ def b_alg(W, H, L, storage, rays):
     return W * H * 48 + 2 * texel_bytes_contract(storage) * sum(rays[1:L])
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "frame_1080p_hbm_traffic.md")
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "frame_1080p_plain.md")
 
 
 def profiled_traffic(family, W, H, L, storage):
@@ -165,7 +165,9 @@ def surrogate_leg(rc, torch, W, H, L, args, depth, emis, irr_plain, stream, fps_
     sur.load_weights(args.weights)
     sur.set_surrogate(True)
     out = torch.empty_like(irr_plain)
-    if not args.no_autotune:
+    if args.load_tuning:
+        apply_tuning(sur, load_tuning(args.load_tuning))
+    elif not args.no_autotune:
         sur.autotune(depth, emis, out, iters=3, stream=stream)   # as the plain path (its level 1 too)
     for _ in range(args.warmup):
         sur.run_device(depth, emis, out, stream)
@@ -206,6 +208,20 @@ def surrogate_leg(rc, torch, W, H, L, args, depth, emis, irr_plain, stream, fps_
                              "unit": "TFLOP/s", "frac": tflops / BF16_MFMA_PEAK_TFLOPS}}
 
 
+def load_tuning(arg):
+    """A tuning as written by --save-tuning: a JSON file, or the JSON text itself."""
+    text = arg.strip()
+    return json.loads(text) if text.startswith("{") else json.load(open(arg))
+
+
+def apply_tuning(stage, tun):
+    for lvl, (shape, rpt) in tun["tiling"].items():
+        if int(lvl) < stage.levels:
+            stage.set_tiling(int(lvl), shape, rpt)
+    stage.set_march(*tun["march"])
+    stage.set_jfa_fuse(tun["jfa_fuse"])
+
+
 def host_staging_leg(rc, W, H, L, storage, depth, emis, frames=20):
     """rc_update_host + rc_render_host per frame (host G-buffer in, host
     irradiance out: the PCIe-inclusive rate of the staging boundary)."""
@@ -235,11 +251,14 @@ def host_staging_leg(rc, W, H, L, storage, depth, emis, frames=20):
             "note": "host G-buffer upload + frame + irradiance download; pageable, then page-locked host buffers"}
 
 
-def batch_leg(rc, torch, args, stream):
+def batch_leg(rc, torch, args, stream, fps_plain):
     """The multi-view workload: 64 seeded views (seeds 1000..1063) of uneven
     sizes (rc.batch_views_spec), plain cascades with the coverage rule's level
-    count, run as one graph launch (8 captured streams), tiling autotuned per
-    distinct view size.  Aggregate megapixels of irradiance per second."""
+    count, one graph launch per batch, in both batch modes: the work list (one
+    launch per stage over every view, one shared workspace, tuned as a whole)
+    and one context per view on 8 captured streams (tuned per distinct view
+    size).  Aggregate megapixels of irradiance per second, and the ratio to the
+    single-view 1080p rate of the plain path."""
     dev = torch.device("cuda:0")
     spec = rc.batch_views_spec(1000, 64)
     views = []
@@ -248,26 +267,34 @@ def batch_leg(rc, torch, args, stream):
         e = torch.empty((3, H, W), device=dev)
         rc.gbuffer_raster(rc.scene_generate(seed, nb, ne), W, H, d, None, e)
         views.append((d, e, torch.empty((3, H, W), device=dev)))
-    b = rc.RadianceCascadeBatch(views, storage=args.storage, streams=8)
-    b.autotune(iters=1, stream=stream)
-    for _ in range(3):
-        b.run(stream)
-    torch.cuda.synchronize()
-    n = 10
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    ev0.record(stream)
-    for _ in range(n):
-        b.run(stream)
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    ms = ev0.elapsed_time(ev1) / n
     mp = sum(W * H for _, W, H, _, _ in spec) / 1e6
-    levels = [b.levels(i) for i in range(len(spec))]
-    b.close()
-    return {"views": len(spec), "megapixels": mp, "ms_per_batch": ms, "mpix_per_s": mp / (ms * 1e-3),
-            "views_per_s": len(spec) / (ms * 1e-3), "levels": {str(L): levels.count(L) for L in sorted(set(levels))},
-            "streams": 8, "launches": "one graph launch per batch"}
+    single_mps = fps_plain * args.width * args.height / 1e6
+    out = {"views": len(spec), "megapixels": mp, "single_view_mpix_per_s": single_mps}
+    for mode in ("worklist", "streams"):
+        b = rc.RadianceCascadeBatch(views, storage=args.storage, mode=mode, streams=8)
+        b.autotune(iters=1, stream=stream)
+        for _ in range(3):
+            b.run(stream)
+        torch.cuda.synchronize()
+        n = 10
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        for _ in range(n):
+            b.run(stream)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        ms = ev0.elapsed_time(ev1) / n
+        info = b.info()
+        levels = [b.levels(i) for i in range(len(spec))]
+        b.close()
+        out[mode] = {"ms_per_batch": ms, "mpix_per_s": mp / (ms * 1e-3), "views_per_s": len(spec) / (ms * 1e-3),
+                     "vs_single_view": mp / (ms * 1e-3) / single_mps, "kernel_nodes": info["kernel_nodes"],
+                     "workspace_bytes": info["workspace_bytes"]}
+        out["levels"] = {str(L): levels.count(L) for L in sorted(set(levels))}
+    out["mpix_per_s"] = out["worklist"]["mpix_per_s"]
+    out["launches"] = "one graph launch per batch"
+    return out
 
 
 # kernel name prefix -> family (the plain frame's kernels)
@@ -369,9 +396,13 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-host-staging", action="store_true",
+                    help="skip the host-staging leg (profiled runs: its contexts use the default configuration)")
     ap.add_argument("--weights", default=None, help="surrogate weights (default: the packaged network)")
     ap.add_argument("--save-tuning", default=None, help="write the autotuned tiling / march / jump-flood choice (JSON)")
-    ap.add_argument("--load-tuning", default=None, help="use a saved tuning instead of autotuning (profiled runs)")
+    ap.add_argument("--load-tuning", default=None,
+                    help="use a saved tuning instead of autotuning (profiled runs): a JSON file as written by "
+                         "--save-tuning, or the JSON text itself")
     ap.add_argument("--summarize-profile", nargs=6, default=None,
                     metavar=("TRACE_STATS", "FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE", "WORKLOAD"),
                     help="print the markdown summary of rocprofv3 CSVs of this bench (kernel stats, FETCH_SIZE and "
@@ -416,11 +447,7 @@ def main():
 
     stage = rc.RadianceCascadeStage(W, H, levels=L, storage=args.storage)
     if args.load_tuning:
-        tun = json.load(open(args.load_tuning))
-        for lvl, (shape, rpt) in tun["tiling"].items():
-            stage.set_tiling(int(lvl), shape, rpt)
-        stage.set_march(*tun["march"])
-        stage.set_jfa_fuse(tun["jfa_fuse"])
+        apply_tuning(stage, load_tuning(args.load_tuning))
         tiling = stage.tiling()
     elif not args.no_autotune:
         tiling = stage.autotune(depth, emis, irr, iters=3, stream=stream)
@@ -483,8 +510,8 @@ def main():
     if not args.no_surrogate and L >= 3:
         surrogate = surrogate_leg(rc, torch, W, H, L, args, depth, emis, irr, stream, fps)
 
-    batch = None if args.no_batch else batch_leg(rc, torch, args, stream)
-    staging = host_staging_leg(rc, W, H, L, args.storage, depth, emis)
+    batch = None if args.no_batch else batch_leg(rc, torch, args, stream, fps)
+    staging = None if args.no_host_staging else host_staging_leg(rc, W, H, L, args.storage, depth, emis)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -514,7 +541,7 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None if traffic is None else traffic / fam_launches[dominant],
                      "traffic_over_alg": None if traffic is None else traffic / fam_alg[dominant],
-                     "traffic_source": "profiles/frame_1080p_hbm_traffic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                     "traffic_source": "profiles/frame_1080p_plain.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
                                        "copy-calibrated, per kernel family and frame)",
                      "alg_bytes_per_launch": dom_bytes, "mean_launch_ms": dom_ms,
                      "launches_per_frame": fam_launches[dominant]},

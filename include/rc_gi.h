@@ -382,12 +382,25 @@ rc_status rc_debug_mip(rc_ctx *ctx, uint32_t level, float *emission_dev, float *
 
 /* ---- batches of views (one graph launch) ------------------------------------
  * Independent views of uneven sizes (e.g. the sub-views of a renderer's
- * frame, or many cameras) run as ONE hipGraph launch.  Each view's frame is
- * captured once; the views are dealt round-robin to `streams` captured streams
- * so that small views run concurrently.  rc_batch_autotune tunes each distinct
- * (width, height, levels) once, on its first view, and gives the choice to the
- * views of that size.  The view buffers are the caller's and are bound at
- * create time (the graph keeps their addresses).  Plain cascades only. */
+ * frame, or many cameras) run as ONE hipGraph launch.  No counterpart in the
+ * reference, which renders one view per frame (src/window/app.rs:221-267).
+ *
+ * RC_BATCH_WORKLIST (default): a fixed handful of kernels per batch, each
+ *   walking a work list of (view, stage block) jobs over ONE shared workspace:
+ *   the seeds, one launch per jump-flood pass (right-aligned: a view takes
+ *   the passes its size needs), the fused tail, the split march of every
+ *   (view, level >= 1), one launch per level index, and level 0 + gather:
+ *   about 20 kernel nodes whatever the number of views.  rc_batch_autotune
+ *   tunes each stage for the batch as a whole.
+ * RC_BATCH_STREAMS: one context per view; each view's frame is captured once
+ *   and the views are dealt round-robin to `streams` captured streams so that
+ *   small views run concurrently (about 17 kernel nodes per view).
+ *   rc_batch_autotune tunes each distinct (width, height, levels) once.
+ *
+ * Either way every view's irradiance equals its single-view rc_run_device
+ * frame bit for bit.  The view buffers are the caller's and are bound at
+ * create time (the graph keeps their addresses); depth is linear view depth.
+ * Plain cascades only. */
 typedef struct {
     uint32_t width, height;
     uint32_t levels;                 /* 0 = coverage rule */
@@ -396,11 +409,22 @@ typedef struct {
     const float *emissive;           /* device [3][H][W] */
     float *irradiance;               /* device [3][H][W] */
 } rc_view;
+#define RC_BATCH_WORKLIST 0u
+#define RC_BATCH_STREAMS 1u
+typedef struct {
+    uint32_t storage;                /* RC_STORAGE_* */
+    uint32_t mode;                   /* RC_BATCH_WORKLIST or RC_BATCH_STREAMS */
+    uint32_t streams;                /* RC_BATCH_STREAMS: 1 .. 16 captured streams (ignored otherwise) */
+    int device;
+} rc_batch_config;
 typedef struct rc_batch rc_batch;
-rc_status rc_batch_create(const rc_view *views, uint32_t n_views, uint32_t storage, uint32_t streams, int device,
-                          rc_batch **out);
+rc_status rc_batch_create(const rc_view *views, uint32_t n_views, const rc_batch_config *cfg, rc_batch **out);
 rc_status rc_batch_autotune(rc_batch *batch, uint32_t iters, void *stream);
 rc_status rc_batch_run(rc_batch *batch, void *stream);
+/* Kernel nodes of one batch launch (streams mode: known after the first
+ * rc_batch_run) and the bytes of the shared workspace (work-list mode;
+ * streams mode reports 0: each view's context owns its buffers). */
+rc_status rc_batch_info(const rc_batch *batch, uint32_t *kernel_nodes, uint64_t *workspace_bytes);
 /* levels of view i after the coverage rule */
 rc_status rc_batch_view_levels(const rc_batch *batch, uint32_t i, uint32_t *levels);
 const char *rc_batch_last_error(const rc_batch *batch);

@@ -96,6 +96,16 @@ class View(ctypes.Structure):
                 ("irradiance", ctypes.c_void_p)]
 
 
+BATCH_WORKLIST = 0
+BATCH_STREAMS = 1
+BATCH_MODES = {"worklist": BATCH_WORKLIST, "streams": BATCH_STREAMS}
+
+
+class BatchConfig(ctypes.Structure):
+    _fields_ = [("storage", ctypes.c_uint32), ("mode", ctypes.c_uint32), ("streams", ctypes.c_uint32),
+                ("device", ctypes.c_int32)]
+
+
 class Box(ctypes.Structure):
     _fields_ = [("lo", ctypes.c_double * 3), ("hi", ctypes.c_double * 3),
                 ("albedo", ctypes.c_double * 3), ("emit", ctypes.c_double * 3)]
@@ -153,7 +163,8 @@ SIGNATURES = {
                                        ctypes.POINTER(TrainReport)]),
     "rc_debug_surrogate": (_i32, [_P, _P, _P, _P, _P]),
     "rc_debug_mip": (_i32, [_P, _u32, _P, _P, _P]),
-    "rc_batch_create": (_i32, [ctypes.POINTER(View), _u32, _u32, _u32, _i32, ctypes.POINTER(_P)]),
+    "rc_batch_create": (_i32, [ctypes.POINTER(View), _u32, ctypes.POINTER(BatchConfig), ctypes.POINTER(_P)]),
+    "rc_batch_info": (_i32, [_P, ctypes.POINTER(_u32), ctypes.POINTER(ctypes.c_uint64)]),
     "rc_batch_autotune": (_i32, [_P, _u32, _P]),
     "rc_batch_run": (_i32, [_P, _P]),
     "rc_batch_view_levels": (_i32, [_P, _u32, ctypes.POINTER(_u32)]),

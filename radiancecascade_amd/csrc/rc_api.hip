@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "rc_internal.hpp"
+#include "rc_batch.hpp"
 
 using namespace rc;
 
@@ -140,17 +141,6 @@ static uint32_t coverage_levels(uint32_t W, uint32_t H)
     return RC_MAX_LEVELS;
 }
 
-// Default tile per level: the top level has no upper neighbourhood to stage,
-// so it takes whole 256-direction slices of one probe (no idle lanes at ragged
-// grid edges); lower levels take 4x4 probes x 16 directions, whose staged
-// upper block is exactly one texel per ray.
-static int default_tile(int level, int L)
-{
-    const int N = num_dirs(level);
-    if (level == L - 1 && N >= 256) return TILE_1x1x256;
-    if (N >= 16) return TILE_4x4x16;
-    return TILE_8x8x4;
-}
 
 static void drop_graph(rc_ctx *c)
 {
@@ -255,7 +245,7 @@ static rc_status allocate(rc_ctx *c, uint32_t W, uint32_t H, uint32_t L)
             const double th = 2.0 * M_PI * ((double)k + 0.5) / (double)N;
             dirs.push_back(make_float2((float)cos(th), (float)sin(th)));
         }
-        c->tile[i] = default_tile(i, (int)L);
+        c->tile[i] = default_level_tile(i, (int)L);
         c->rpt[i] = 2;
     }
     c->surr_tile1 = c->surr_rpt1 = -1;
@@ -1788,15 +1778,22 @@ rc_status rc_debug_mip(rc_ctx *c, uint32_t level, float *emission, float *covera
 }
 
 // ---- batches of views --------------------------------------------------------
+// Work-list mode runs the batch through rc_batch.hip's engine; streams mode
+// keeps one context per view and captures their frames on several streams.
 struct rc_batch {
-    std::vector<rc_ctx *> ctx;
-    std::vector<rc_view> views;
+    uint32_t mode = RC_BATCH_WORKLIST;
     int device = 0;
+    std::vector<rc_view> views;
+    std::vector<uint32_t> levels;
+    WorkList *wl = nullptr;
+    // streams mode
+    std::vector<rc_ctx *> ctx;
     std::vector<hipStream_t> streams;
     std::vector<hipEvent_t> joins;
     hipEvent_t fork = nullptr;
     hipStream_t cap = nullptr;
     hipGraphExec_t gexec = nullptr;
+    uint32_t kernel_nodes = 0;
     std::string err;
 };
 
@@ -1818,6 +1815,7 @@ void rc_batch_destroy(rc_batch *b)
     if (!b) return;
     hipSetDevice(b->device);
     hipDeviceSynchronize();
+    worklist_destroy(b->wl);
     if (b->gexec) hipGraphExecDestroy(b->gexec);
     for (auto *c : b->ctx) rc_destroy(c);
     for (auto st : b->streams) hipStreamDestroy(st);
@@ -1827,35 +1825,62 @@ void rc_batch_destroy(rc_batch *b)
     delete b;
 }
 
-rc_status rc_batch_create(const rc_view *views, uint32_t n, uint32_t storage, uint32_t nstreams, int device,
-                          rc_batch **out)
+rc_status rc_batch_create(const rc_view *views, uint32_t n, const rc_batch_config *cfg, rc_batch **out)
 {
-    if (!views || !out) return bfail(nullptr, RC_ERR_NULL, "rc_batch_create: views or out is NULL");
+    if (!views || !cfg || !out) return bfail(nullptr, RC_ERR_NULL, "rc_batch_create: views, cfg or out is NULL");
     *out = nullptr;
     if (n == 0 || n > 4096) return bfail(nullptr, RC_ERR_INVALID, "rc_batch_create: 1 .. 4096 views");
-    if (nstreams == 0 || nstreams > 16) return bfail(nullptr, RC_ERR_INVALID, "rc_batch_create: 1 .. 16 streams");
-    for (uint32_t i = 0; i < n; ++i)
+    if (cfg->storage > RC_STORAGE_BF16) return bfail(nullptr, RC_ERR_INVALID, "rc_batch_create: unknown storage");
+    if (cfg->mode > RC_BATCH_STREAMS) return bfail(nullptr, RC_ERR_INVALID, "rc_batch_create: unknown mode");
+    if (cfg->mode == RC_BATCH_STREAMS && (cfg->streams == 0 || cfg->streams > 16))
+        return bfail(nullptr, RC_ERR_INVALID, "rc_batch_create: 1 .. 16 streams");
+    for (uint32_t i = 0; i < n; ++i) {
         if (!views[i].depth || !views[i].emissive || !views[i].irradiance)
             return bfail(nullptr, RC_ERR_NULL, "rc_batch_create: a view has a NULL buffer");
+        if (!dims_ok(views[i].width, views[i].height) || views[i].levels > RC_MAX_LEVELS)
+            return bfail(nullptr, RC_ERR_INVALID, "rc_batch_create: view " + std::to_string(i) + " out of range");
+    }
+    int ndev = 0;
+    if (rc_device_count(&ndev) != RC_OK) return RC_ERR_NO_DEVICE;
+    if (cfg->device < 0 || cfg->device >= ndev)
+        return bfail(nullptr, RC_ERR_INVALID, "rc_batch_create: device ordinal out of range");
     rc_batch *b = new (std::nothrow) rc_batch();
     if (!b) return bfail(nullptr, RC_ERR_OOM, "rc_batch_create: host allocation failed");
-    b->device = device;
+    b->mode = cfg->mode;
+    b->device = cfg->device;
     b->views.assign(views, views + n);
+    for (uint32_t i = 0; i < n; ++i)
+        b->levels.push_back(views[i].levels ? views[i].levels : coverage_levels(views[i].width, views[i].height));
+    hipError_t e = hipSetDevice(cfg->device);
+    if (e != hipSuccess) {
+        rc_batch_destroy(b);
+        return bfail(nullptr, RC_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+    }
+    if (b->mode == RC_BATCH_WORKLIST) {
+        std::string msg;
+        e = worklist_create(views, n, b->levels.data(), (Storage)cfg->storage, cfg->device, &b->wl, msg);
+        if (e != hipSuccess) {
+            rc_batch_destroy(b);
+            return bfail(nullptr, e == hipErrorOutOfMemory ? RC_ERR_OOM : (e == hipErrorInvalidValue ? RC_ERR_INVALID : RC_ERR_HIP), msg);
+        }
+        b->kernel_nodes = (uint32_t)worklist_launches(b->wl);
+        *out = b;
+        return RC_OK;
+    }
     for (uint32_t i = 0; i < n; ++i) {
-        rc_config cfg{RC_ABI_VERSION, views[i].width, views[i].height, views[i].levels, storage, 0,
-                      device, 0};
-        rc_ctx *c = nullptr;
-        rc_status st = rc_create(&cfg, &c);
+        rc_config c{RC_ABI_VERSION, views[i].width, views[i].height, b->levels[i], cfg->storage, 0, cfg->device, 0};
+        rc_ctx *ctx = nullptr;
+        rc_status st = rc_create(&c, &ctx);
         if (st != RC_OK) {
             std::string msg = std::string("rc_batch_create: view ") + std::to_string(i) + ": " + rc_last_error(nullptr);
             rc_batch_destroy(b);
             return bfail(nullptr, st, msg);
         }
-        b->ctx.push_back(c);
+        b->ctx.push_back(ctx);
     }
-    hipError_t e = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking);
+    e = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
-    for (uint32_t k = 0; k < nstreams && e == hipSuccess; ++k) {
+    for (uint32_t k = 0; k < cfg->streams && e == hipSuccess; ++k) {
         hipStream_t st = nullptr;
         hipEvent_t ev = nullptr;
         e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
@@ -1874,8 +1899,16 @@ rc_status rc_batch_create(const rc_view *views, uint32_t n, uint32_t storage, ui
 rc_status rc_batch_view_levels(const rc_batch *b, uint32_t i, uint32_t *levels)
 {
     if (!b || !levels) return bfail(nullptr, RC_ERR_NULL, "rc_batch_view_levels: NULL argument");
-    if (i >= b->ctx.size()) return bfail(const_cast<rc_batch *>(b), RC_ERR_INVALID, "rc_batch_view_levels: no such view");
-    *levels = (uint32_t)b->ctx[i]->L;
+    if (i >= b->views.size()) return bfail(const_cast<rc_batch *>(b), RC_ERR_INVALID, "rc_batch_view_levels: no such view");
+    *levels = b->levels[i];
+    return RC_OK;
+}
+
+rc_status rc_batch_info(const rc_batch *b, uint32_t *kernel_nodes, uint64_t *workspace_bytes)
+{
+    if (!b) return bfail(nullptr, RC_ERR_NULL, "rc_batch_info: batch is NULL");
+    if (kernel_nodes) *kernel_nodes = b->kernel_nodes;
+    if (workspace_bytes) *workspace_bytes = b->wl ? (uint64_t)worklist_workspace_bytes(b->wl) : 0;
     return RC_OK;
 }
 
@@ -1883,6 +1916,13 @@ rc_status rc_batch_autotune(rc_batch *b, uint32_t iters, void *stream)
 {
     if (!b) return bfail(nullptr, RC_ERR_NULL, "rc_batch_autotune: batch is NULL");
     BHIP_TRY(b, hipSetDevice(b->device));
+    if (b->wl) {
+        std::string msg;
+        const hipError_t e = worklist_autotune(b->wl, (int)iters, (hipStream_t)stream, msg);
+        if (e != hipSuccess) return bfail(b, RC_ERR_HIP, "rc_batch_autotune: " + msg);
+        b->kernel_nodes = (uint32_t)worklist_launches(b->wl);
+        return RC_OK;
+    }
     std::vector<int> leader(b->ctx.size(), -1);
     for (size_t i = 0; i < b->ctx.size(); ++i) {
         for (size_t j = 0; j < i; ++j)
@@ -1914,8 +1954,22 @@ rc_status rc_batch_autotune(rc_batch *b, uint32_t iters, void *stream)
     return RC_OK;
 }
 
-// Capture: fork the capture stream into the worker streams, each view's frame
-// on stream (view mod streams), join them back.
+static uint32_t count_kernel_nodes(hipGraph_t g)
+{
+    size_t n = 0;
+    if (hipGraphGetNodes(g, nullptr, &n) != hipSuccess || n == 0) return 0;
+    std::vector<hipGraphNode_t> nodes(n);
+    if (hipGraphGetNodes(g, nodes.data(), &n) != hipSuccess) return 0;
+    uint32_t k = 0;
+    for (size_t i = 0; i < n; ++i) {
+        hipGraphNodeType t;
+        if (hipGraphNodeGetType(nodes[i], &t) == hipSuccess && t == hipGraphNodeTypeKernel) ++k;
+    }
+    return k;
+}
+
+// Streams mode: fork the capture stream into the worker streams, each view's
+// frame on stream (view mod streams), join them back.
 static rc_status batch_capture(rc_batch *b)
 {
     BHIP_TRY(b, hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
@@ -1937,6 +1991,7 @@ static rc_status batch_capture(rc_batch *b)
         if (st != RC_OK) return bfail(b, st, "rc_batch_run: capture of a view failed");
         return bfail(b, RC_ERR_HIP, std::string("rc_batch_run: capture: ") + hipGetErrorString(e != hipSuccess ? e : e2));
     }
+    b->kernel_nodes = count_kernel_nodes(g);
     e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
     hipGraphDestroy(g);
     if (e != hipSuccess) {
@@ -1950,6 +2005,12 @@ rc_status rc_batch_run(rc_batch *b, void *stream)
 {
     if (!b) return bfail(nullptr, RC_ERR_NULL, "rc_batch_run: batch is NULL");
     BHIP_TRY(b, hipSetDevice(b->device));
+    if (b->wl) {
+        std::string msg;
+        const hipError_t e = worklist_run(b->wl, (hipStream_t)stream, msg);
+        if (e != hipSuccess) return bfail(b, RC_ERR_HIP, msg);
+        return RC_OK;
+    }
     if (!b->gexec) {
         rc_status st = batch_capture(b);
         if (st != RC_OK) return st;

@@ -95,7 +95,7 @@ __device__ inline void ray_round(Ray &r, __amdgpu_buffer_rsrc_t sdf, int W, int 
 // Level i >= 1.  Workgroup tile: TX x (TY*RPT) probes x KD directions; lane
 // (probe column, direction) owns the RPT probes stacked TY rows apart.
 template <int TX, int TY, int KD, int RPT, Storage S, bool UPPER, bool FROM_CODES>
-__global__ __launch_bounds__(kBlock) void k_level(LevelArgs a)
+__device__ __forceinline__ void level_body(const LevelArgs &a, int lb)
 {
     static_assert(TX * TY * KD == kBlock, "a tile row block holds 256 rays");
     using TT = Texel<S>;
@@ -105,7 +105,6 @@ __global__ __launch_bounds__(kBlock) void k_level(LevelArgs a)
 
     const int tiles_x = (a.PX + TX - 1) / TX;
     const int slices = a.N / KD;
-    const int lb = xcd_band(blockIdx.x, gridDim.x);
     const int slice = lb % slices, tile = lb / slices;
     const int px0 = (tile % tiles_x) * TX, py0 = (tile / tiles_x) * TYR, k0 = slice * KD;
 
@@ -220,6 +219,26 @@ __global__ __launch_bounds__(kBlock) void k_level(LevelArgs a)
     }
 }
 
+template <int TX, int TY, int KD, int RPT, Storage S, bool UPPER, bool FROM_CODES>
+__global__ __launch_bounds__(kBlock) void k_level(LevelArgs a)
+{
+    level_body<TX, TY, KD, RPT, S, UPPER, FROM_CODES>(a, xcd_band(blockIdx.x, gridDim.x));
+}
+
+// One level over a work list of views (each job a view's LevelArgs; the top
+// level of a view has no upper level, so UPPER is taken per job).
+template <int TX, int TY, int KD, int RPT, Storage S, bool FROM_CODES>
+__global__ __launch_bounds__(kBlock) void k_level_batch(const LevelArgs *__restrict__ jobs,
+                                                        const int *__restrict__ begins, int njobs)
+{
+    const int b = xcd_band(blockIdx.x, gridDim.x);
+    const int j = batch_job(begins, njobs, b);
+    const LevelArgs a = jobs[j];
+    const int lb = b - begins[j];
+    if (a.upper) level_body<TX, TY, KD, RPT, S, true, FROM_CODES>(a, lb);
+    else level_body<TX, TY, KD, RPT, S, false, FROM_CODES>(a, lb);
+}
+
 // Level 0 (4 directions, interval [0,2)) fused with the gather.  A tile owns
 // 16x12 level-0 probes = 32x24 pixels (each output row one 128-byte line); a
 // thread marches one probe of the tile plus its 1-probe halo (18x14 = 252
@@ -235,7 +254,8 @@ __global__ __launch_bounds__(kBlock) void k_level(LevelArgs a)
 // the generic interval march takes, so the codes are the same; the fluence
 // is the same sum ((v0 + v1) + (v2 + v3)) / 4 of the per-direction values.
 template <Storage S, bool UPPER>
-__global__ __launch_bounds__(kBlock) void k_level0(LevelArgs a, float *__restrict__ irr, float *__restrict__ dbg)
+__device__ __forceinline__ void level0_body(const LevelArgs &a, float *__restrict__ irr, float *__restrict__ dbg,
+                                            int lb)
 {
     using TT = Texel<S>;
     constexpr int TX = 16, TY = 12, NX = TX + 2, NY = TY + 2, UCX = TX / 2 + 2, UCY = TY / 2 + 2;
@@ -244,7 +264,6 @@ __global__ __launch_bounds__(kBlock) void k_level0(LevelArgs a, float *__restric
     __shared__ float4 fl[NX * NY];
 
     const int tiles_x = (a.PX + TX - 1) / TX;
-    const int lb = xcd_band(blockIdx.x, gridDim.x);
     const int px0 = (lb % tiles_x) * TX, py0 = (lb / tiles_x) * TY;
     const int ulx = (px0 - 2) >> 1, uly = (py0 - 2) >> 1;
     // 32-bit indices (W, H <= 16384: three planes hold fewer than 2^32 floats)
@@ -426,6 +445,25 @@ __global__ __launch_bounds__(kBlock) void k_level0(LevelArgs a, float *__restric
     }
 }
 
+template <Storage S, bool UPPER>
+__global__ __launch_bounds__(kBlock) void k_level0(LevelArgs a, float *__restrict__ irr, float *__restrict__ dbg)
+{
+    level0_body<S, UPPER>(a, irr, dbg, xcd_band(blockIdx.x, gridDim.x));
+}
+
+template <Storage S>
+__global__ __launch_bounds__(kBlock) void k_level0_batch(const Level0Job *__restrict__ jobs,
+                                                         const int *__restrict__ begins, int njobs)
+{
+    const int b = xcd_band(blockIdx.x, gridDim.x);
+    const int j = batch_job(begins, njobs, b);
+    const LevelArgs a = jobs[j].a;
+    float *irr = jobs[j].irr;
+    const int lb = b - begins[j];
+    if (a.upper) level0_body<S, true>(a, irr, nullptr, lb);
+    else level0_body<S, false>(a, irr, nullptr, lb);
+}
+
 // Canonical ray r = (py*PX + px)*N + k -> (px, py, k): the direction is the
 // low log2(N) bits; probe / PX by an fp32 reciprocal, corrected to the exact
 // quotient (probe < 2^24, so the estimate is off by at most one).
@@ -453,14 +491,10 @@ __device__ inline void decode_ray(const MarchLevel &L, int ray, int &px, int &py
 // one-ray-per-lane waves idle most lanes while their longest ray runs).
 // Blocks go round-robin to the XCDs, which interleaves heavy and light tiles.
 template <int SPEC, int M>
-__global__ __launch_bounds__(kBlock) void k_march_all(MarchAllArgs a)
+__device__ __forceinline__ void march_all_body(const uint8_t *sdf_codes, int W, int H, const MarchLevel &L, int lb)
 {
-    int lv = 0;
-#pragma unroll 1
-    while (lv + 1 < a.nlev && (int)blockIdx.x >= a.lev[lv + 1].block_begin) ++lv;
-    const MarchLevel &L = a.lev[lv];
-    const __amdgpu_buffer_rsrc_t sdf = sdf_rsrc(a.sdf, a.W, a.H);
-    const int base = ((int)blockIdx.x - L.block_begin) * kBlock * M + (int)threadIdx.x;
+    const __amdgpu_buffer_rsrc_t sdf = sdf_rsrc(sdf_codes, W, H);
+    const int base = lb * kBlock * M + (int)threadIdx.x;
     const int total = L.PX * L.PY * L.N;
 
     int j = 0, ray = base;
@@ -475,7 +509,7 @@ __global__ __launch_bounds__(kBlock) void k_march_all(MarchAllArgs a)
     };
     start();
     for (;;) {
-        if (r.alive) ray_round<SPEC>(r, sdf, a.W, a.H, L.t1);
+        if (r.alive) ray_round<SPEC>(r, sdf, W, H, L.t1);
         if (j < M && !r.alive) {
             if (ray < total) L.codes[ray] = r.code;
             ray += kBlock;
@@ -483,6 +517,27 @@ __global__ __launch_bounds__(kBlock) void k_march_all(MarchAllArgs a)
         }
         if (!__any(j < M)) break;
     }
+}
+
+template <int SPEC, int M>
+__global__ __launch_bounds__(kBlock) void k_march_all(MarchAllArgs a)
+{
+    int lv = 0;
+#pragma unroll 1
+    while (lv + 1 < a.nlev && (int)blockIdx.x >= a.lev[lv + 1].block_begin) ++lv;
+    march_all_body<SPEC, M>(a.sdf, a.W, a.H, a.lev[lv], (int)blockIdx.x - a.lev[lv].block_begin);
+}
+
+// The split march over a work list of (view, level) jobs, blocks in launch
+// order as k_march_all's (the job list puts every view's top level first).
+template <int SPEC, int M>
+__global__ __launch_bounds__(kBlock) void k_march_batch(const MarchJob *__restrict__ jobs,
+                                                        const int *__restrict__ begins, int njobs)
+{
+    const int b = (int)blockIdx.x;
+    const int j = batch_job(begins, njobs, b);
+    const MarchJob J = jobs[j];
+    march_all_body<SPEC, M>(J.sdf, J.W, J.H, J.L, b - begins[j]);
 }
 
 // Persistent form of the split march.  March lengths differ by two orders of
@@ -717,6 +772,96 @@ hipError_t launch_march_all(const MarchAllArgs &a, int spec, int m, hipStream_t 
 }
 
 
+
+int level_blocks(int PX, int PY, int N, int shape, int rpt)
+{
+    const int tx = tile_tx(shape), ty = tile_ty(shape) * rpt, kd = tile_kd(shape);
+    return ((PX + tx - 1) / tx) * ((PY + ty - 1) / ty) * (N / kd);
+}
+
+int level0_blocks(int PX, int PY) { return ((PX + 15) / 16) * ((PY + 11) / 12); }
+
+template <int SPEC>
+static hipError_t launch_march_batch_spec(const BatchLaunch &b, int m, hipStream_t s)
+{
+    const MarchJob *j = (const MarchJob *)b.jobs;
+    switch (m) {
+    case 1: hipLaunchKernelGGL((k_march_batch<SPEC, 1>), dim3(b.blocks), dim3(kBlock), 0, s, j, b.begins, b.njobs); break;
+    case 2: hipLaunchKernelGGL((k_march_batch<SPEC, 2>), dim3(b.blocks), dim3(kBlock), 0, s, j, b.begins, b.njobs); break;
+    case 4: hipLaunchKernelGGL((k_march_batch<SPEC, 4>), dim3(b.blocks), dim3(kBlock), 0, s, j, b.begins, b.njobs); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_march_batch(const BatchLaunch &b, int spec, int m, hipStream_t s)
+{
+    if (b.blocks <= 0) return hipSuccess;
+    switch (spec) {
+    case 1: return launch_march_batch_spec<1>(b, m, s);
+    case 2: return launch_march_batch_spec<2>(b, m, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+template <int TX, int TY, int KD, int RPT, Storage S>
+static void launch_level_batch_shape(const BatchLaunch &b, bool from_codes, hipStream_t s)
+{
+    const LevelArgs *j = (const LevelArgs *)b.jobs;
+    if (from_codes)
+        hipLaunchKernelGGL((k_level_batch<TX, TY, KD, RPT, S, true>), dim3(b.blocks), dim3(kBlock), 0, s, j, b.begins,
+                           b.njobs);
+    else
+        hipLaunchKernelGGL((k_level_batch<TX, TY, KD, RPT, S, false>), dim3(b.blocks), dim3(kBlock), 0, s, j, b.begins,
+                           b.njobs);
+}
+
+template <int RPT, Storage S>
+static hipError_t launch_level_batch_rpt(const BatchLaunch &b, int shape, bool from_codes, hipStream_t s)
+{
+    switch (shape) {
+    case TILE_8x8x4: launch_level_batch_shape<8, 8, 4, RPT, S>(b, from_codes, s); break;
+    case TILE_4x4x16: launch_level_batch_shape<4, 4, 16, RPT, S>(b, from_codes, s); break;
+    case TILE_2x2x64: launch_level_batch_shape<2, 2, 64, RPT, S>(b, from_codes, s); break;
+    case TILE_1x1x256: launch_level_batch_shape<1, 1, 256, RPT, S>(b, from_codes, s); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <Storage S>
+static hipError_t launch_level_batch_st(const BatchLaunch &b, int shape, int rpt, bool from_codes, hipStream_t s)
+{
+    switch (rpt) {
+    case 1: return launch_level_batch_rpt<1, S>(b, shape, from_codes, s);
+    case 2: return launch_level_batch_rpt<2, S>(b, shape, from_codes, s);
+    case 4: return launch_level_batch_rpt<4, S>(b, shape, from_codes, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_level_batch(const BatchLaunch &b, Storage st, int shape, int rpt, bool from_codes, hipStream_t s)
+{
+    if (b.blocks <= 0) return hipSuccess;
+    switch (st) {
+    case Storage::F32: return launch_level_batch_st<Storage::F32>(b, shape, rpt, from_codes, s);
+    case Storage::F16: return launch_level_batch_st<Storage::F16>(b, shape, rpt, from_codes, s);
+    case Storage::BF16: return launch_level_batch_st<Storage::BF16>(b, shape, rpt, from_codes, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_level0_batch(const BatchLaunch &b, Storage st, hipStream_t s)
+{
+    if (b.blocks <= 0) return hipSuccess;
+    const Level0Job *j = (const Level0Job *)b.jobs;
+    switch (st) {
+    case Storage::F32: hipLaunchKernelGGL(k_level0_batch<Storage::F32>, dim3(b.blocks), dim3(kBlock), 0, s, j, b.begins, b.njobs); break;
+    case Storage::F16: hipLaunchKernelGGL(k_level0_batch<Storage::F16>, dim3(b.blocks), dim3(kBlock), 0, s, j, b.begins, b.njobs); break;
+    case Storage::BF16: hipLaunchKernelGGL(k_level0_batch<Storage::BF16>, dim3(b.blocks), dim3(kBlock), 0, s, j, b.begins, b.njobs); break;
+    }
+    return hipGetLastError();
+}
 
 hipError_t launch_level_to_f32(const void *level, Storage st, size_t texels, float *out, hipStream_t s)
 {

@@ -38,9 +38,81 @@ hipError_t launch_jfa_pass(const uint32_t *in, uint32_t *out, uint8_t *sdf, int 
 hipError_t launch_jfa_tail(const uint32_t *in, uint32_t *seeds_out, uint8_t *sdf, int W, int H, int tail_step,
                            hipStream_t s, const int *gate = nullptr);
 
+// ---- work lists of views (one launch per stage over many views) --------------
+// A batched launch runs one kernel over the jobs of many views.  Job j owns
+// the logical blocks [begins[j], begins[j+1]) (begins ascending from 0, no
+// empty job); a workgroup takes its logical block b from the XCD banding of
+// the whole grid, finds its job, and runs the single-view kernel body on
+// block b - begins[j] of that job, so every view's result is the single-view
+// one bit for bit.
+struct BatchLaunch {
+    const void *jobs;        // device array of the launch's job type
+    const int *begins;       // device [njobs]
+    int njobs, blocks;
+};
+
+// Job index of logical block b: a 64-ary search over begins, one wave-wide
+// load and ballot per level (one level up to 64 jobs, two up to 4096).  Every
+// lane of the wave must be active; the result is wave-uniform.
+__device__ inline int batch_job(const int *__restrict__ begins, int njobs, int b)
+{
+    const int lane = (int)(threadIdx.x & 63);
+    int lo = 0, stride = 1;
+    while (stride < njobs / 64 + (njobs % 64 != 0)) stride *= 64;
+    for (; stride >= 1; stride >>= 6) {
+        const int idx = lo + lane * stride;
+        const bool le = idx < njobs && begins[idx] <= b;
+        lo += (__popcll(__ballot(le)) - 1) * stride;
+    }
+    return __builtin_amdgcn_readfirstlane(lo);
+}
+
+struct SeedJob {
+    const float *depth, *emissive;
+    uint32_t *seed;
+    float4 *emis4;
+    DepthConv dc;
+    int W, H, bx_count;
+};
+struct JfaJob {
+    const uint32_t *in;
+    uint32_t *out;           // next seed field (nullptr on a view's last pass)
+    uint8_t *sdf;
+    int W, H, step, bx_count, final_pass;
+};
+struct TailJob {
+    const uint32_t *in;
+    uint32_t *seeds_out;
+    uint8_t *sdf;
+    int W, H, bx_count;
+};
+int seed_blocks(int W, int H);
+int seed_bx(int W);
+int jfa_pass_blocks(int W, int H);
+int jfa_pass_bx(int W);
+int jfa_tail_blocks(int W, int H, int S);
+int jfa_tail_bx(int W, int S);
+hipError_t launch_seed_batch(const BatchLaunch &b, hipStream_t s);
+hipError_t launch_jfa_batch(const BatchLaunch &b, hipStream_t s);
+hipError_t launch_tail_batch(const BatchLaunch &b, int S, hipStream_t s);
+
 // ---- cascades (rc_cascade.hip) ----------------------------------------------
 // Workgroup tile of a level kernel: TX x TY probes x KD directions, TX*TY*KD = 256.
 enum TileShape : int { TILE_8x8x4 = 0, TILE_4x4x16 = 1, TILE_2x2x64 = 2, TILE_1x1x256 = 3, TILE_COUNT = 4 };
+// Default tile of level i of L: the top level has no upper neighbourhood to
+// stage, so it takes whole 256-direction slices of one probe (no idle lanes
+// at ragged grid edges); lower levels take 4x4 probes x 16 directions, whose
+// staged upper block is exactly one texel per ray.
+inline int default_level_tile(int level, int L)
+{
+    const int N = num_dirs(level);
+    if (level == L - 1 && N >= 256) return TILE_1x1x256;
+    if (N >= 16) return TILE_4x4x16;
+    return TILE_8x8x4;
+}
+// First direction of level i in a table holding levels 0, 1, ... back to
+// back: sum of N_j for j < i.
+inline int dir_table_offset(int level) { return 4 * ((1 << (2 * level)) - 1) / 3; }
 int tile_kd(int shape);
 int tile_tx(int shape);
 int tile_ty(int shape);
@@ -93,6 +165,23 @@ struct MarchAllArgs {
     uint32_t total_rays, n_packets, n_waves;
 };
 hipError_t launch_march_all(const MarchAllArgs &a, int speculation, int rays_per_lane, hipStream_t s);
+// one level of one view in a batched split march (block_begin unused)
+struct MarchJob {
+    const uint8_t *sdf;
+    int W, H;
+    MarchLevel L;
+};
+struct Level0Job {
+    LevelArgs a;
+    float *irr;
+};
+int level_blocks(int PX, int PY, int N, int shape, int rays_per_lane);
+int level0_blocks(int PX, int PY);
+// jobs: MarchJob (march), LevelArgs (levels, `upper` may be null per job), Level0Job
+hipError_t launch_march_batch(const BatchLaunch &b, int speculation, int rays_per_lane, hipStream_t s);
+hipError_t launch_level_batch(const BatchLaunch &b, Storage st, int shape, int rays_per_lane, bool from_codes,
+                              hipStream_t s);
+hipError_t launch_level0_batch(const BatchLaunch &b, Storage st, hipStream_t s);
 // The persistent split march: a grid of `waves` waves (a multiple of 4), each
 // walking its share of the packets and refilling finished lanes.
 hipError_t launch_march_persist(const MarchAllArgs &a, int speculation, hipStream_t s);

@@ -76,13 +76,10 @@ constexpr int kSeedRows = 16;
 // 16-byte load instead of one 4-byte load from each of three planes 8 MB
 // apart.  Only occupied pixels are written: a ray hits where the distance is
 // 0, i.e. on a seed, and seeds are the occupied pixels.
-__global__ __launch_bounds__(kBlock) void k_seed_init(const float *__restrict__ depth, DepthConv dc,
-                                                      const float *__restrict__ emissive,
-                                                      uint32_t *__restrict__ seed, float4 *__restrict__ emis4, int W,
-                                                      int H, int bx_count, const int *gate)
+__device__ __forceinline__ void seed_init_body(const float *__restrict__ depth, const DepthConv &dc,
+                                               const float *__restrict__ emissive, uint32_t *__restrict__ seed,
+                                               float4 *__restrict__ emis4, int W, int H, int bx_count, int b)
 {
-    if (gate && *gate == 0) return;
-    const int b = xcd_band(blockIdx.x, gridDim.x);
     const int x = (b % bx_count) * 64 + (threadIdx.x & 63);
     const int y0 = (b / bx_count) * kSeedRows + 4 * (threadIdx.x >> 6);
     if (x >= W) return;
@@ -127,18 +124,35 @@ __global__ __launch_bounds__(kBlock) void k_seed_init(const float *__restrict__ 
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_seed_init(const float *__restrict__ depth, DepthConv dc,
+                                                      const float *__restrict__ emissive,
+                                                      uint32_t *__restrict__ seed, float4 *__restrict__ emis4, int W,
+                                                      int H, int bx_count, const int *gate)
+{
+    if (gate && *gate == 0) return;
+    seed_init_body(depth, dc, emissive, seed, emis4, W, H, bx_count, xcd_band(blockIdx.x, gridDim.x));
+}
+
+// The same over a work list of views: logical block b (XCD-banded over the
+// whole grid) belongs to the job whose block range holds it.
+__global__ __launch_bounds__(kBlock) void k_seed_batch(const SeedJob *__restrict__ jobs,
+                                                       const int *__restrict__ begins, int njobs)
+{
+    const int b = xcd_band(blockIdx.x, gridDim.x);
+    const int j = batch_job(begins, njobs, b);
+    const SeedJob &J = jobs[j];
+    seed_init_body(J.depth, J.dc, J.emissive, J.seed, J.emis4, J.W, J.H, J.bx_count, b - begins[j]);
+}
+
 // One global pass.  A wave owns 64 consecutive pixels of one row (blocks of
 // 64 x 4); the row index is made wave-uniform so the three source rows are
 // scalar base pointers and each of the 9 loads is a 32-bit lane offset.
 // FINAL: write the SDF (and, when `out` is set, the external seed field)
 // instead of the internal seed field.
 template <bool FINAL>
-__global__ __launch_bounds__(kBlock) void k_jfa_pass(const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
-                                                     uint8_t *__restrict__ sdf, int W, int H, int step, int bx_count,
-                                                     const int *gate)
+__device__ __forceinline__ void jfa_pass_body(const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
+                                              uint8_t *__restrict__ sdf, int W, int H, int step, int bx_count, int b)
 {
-    if (gate && *gate == 0) return;
-    const int b = xcd_band(blockIdx.x, gridDim.x);
     const int x = (b % bx_count) * 64 + (threadIdx.x & 63);
     const int y = __builtin_amdgcn_readfirstlane((b / bx_count) * 4 + (threadIdx.x >> 6));
     if (y >= H) return;
@@ -187,6 +201,28 @@ __global__ __launch_bounds__(kBlock) void k_jfa_pass(const uint32_t *__restrict_
     }
 }
 
+template <bool FINAL>
+__global__ __launch_bounds__(kBlock) void k_jfa_pass(const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
+                                                     uint8_t *__restrict__ sdf, int W, int H, int step, int bx_count,
+                                                     const int *gate)
+{
+    if (gate && *gate == 0) return;
+    jfa_pass_body<FINAL>(in, out, sdf, W, H, step, bx_count, xcd_band(blockIdx.x, gridDim.x));
+}
+
+// One pass over a work list of views; each job has its own step (a view
+// takes the passes its size needs) and says whether this is its last pass
+// (step 0 leaves every pixel its own seed: the field of a 1x1 view).
+__global__ __launch_bounds__(kBlock) void k_jfa_batch(const JfaJob *__restrict__ jobs, const int *__restrict__ begins,
+                                                      int njobs)
+{
+    const int b = xcd_band(blockIdx.x, gridDim.x);
+    const int j = batch_job(begins, njobs, b);
+    const JfaJob &J = jobs[j];
+    if (J.final_pass) jfa_pass_body<true>(J.in, J.out, J.sdf, J.W, J.H, J.step, J.bx_count, b - begins[j]);
+    else jfa_pass_body<false>(J.in, J.out, J.sdf, J.W, J.H, J.step, J.bx_count, b - begins[j]);
+}
+
 // Fused passes S, S/2, ..., 1 over a T x T output tile, loaded with a halo of
 // 2S-1 pixels: T = 64 - 2 (2S - 1), so the loaded region is 64 x 64 cells and
 // every row of it fills a wave (T = 62, 58, 50, 34 for S = 1, 2, 4, 8).  Each
@@ -201,18 +237,15 @@ template <int S> constexpr int jfa_tail_tile() { return 64 - 2 * (2 * S - 1); }
 constexpr int kTailBlock = 1024, kTailRowsStep = kTailBlock / 64;
 
 template <int S>
-__global__ __launch_bounds__(kTailBlock) void k_jfa_tail(const uint32_t *__restrict__ in,
-                                                     uint32_t *__restrict__ seeds_out, uint8_t *__restrict__ sdf,
-                                                     int W, int H, int bx_count, const int *gate)
+__device__ __forceinline__ void jfa_tail_body(const uint32_t *__restrict__ in, uint32_t *__restrict__ seeds_out,
+                                              uint8_t *__restrict__ sdf, int W, int H, int bx_count, int b)
 {
-    if (gate && *gate == 0) return;
     constexpr int T = jfa_tail_tile<S>();
     constexpr int HALO = 2 * S - 1;
     constexpr int R = T + 2 * HALO;
     static_assert(S >= 1 && R == 64, "one column per lane");
     __shared__ uint32_t buf[2][R * R];
 
-    const int b = xcd_band(blockIdx.x, gridDim.x);
     const int x0 = (b % bx_count) * T - HALO;
     const int y0 = (b / bx_count) * T - HALO;
     const int cx = threadIdx.x & 63, cy = threadIdx.x >> 6;
@@ -288,6 +321,25 @@ __global__ __launch_bounds__(kTailBlock) void k_jfa_tail(const uint32_t *__restr
     }
 }
 
+template <int S>
+__global__ __launch_bounds__(kTailBlock) void k_jfa_tail(const uint32_t *__restrict__ in,
+                                                     uint32_t *__restrict__ seeds_out, uint8_t *__restrict__ sdf,
+                                                     int W, int H, int bx_count, const int *gate)
+{
+    if (gate && *gate == 0) return;
+    jfa_tail_body<S>(in, seeds_out, sdf, W, H, bx_count, xcd_band(blockIdx.x, gridDim.x));
+}
+
+template <int S>
+__global__ __launch_bounds__(kTailBlock) void k_tail_batch(const TailJob *__restrict__ jobs,
+                                                         const int *__restrict__ begins, int njobs)
+{
+    const int b = xcd_band(blockIdx.x, gridDim.x);
+    const int j = batch_job(begins, njobs, b);
+    const TailJob &J = jobs[j];
+    jfa_tail_body<S>(J.in, J.seeds_out, J.sdf, J.W, J.H, J.bx_count, b - begins[j]);
+}
+
 // An image with no pass at all (1x1): only the SDF conversion.
 __global__ void k_jfa_none(const uint32_t *__restrict__ in, uint32_t *__restrict__ seeds_out,
                            uint8_t *__restrict__ sdf, int W, int n, const int *gate)
@@ -357,6 +409,51 @@ hipError_t launch_jfa_tail(const uint32_t *in, uint32_t *seeds_out, uint8_t *sdf
     case 2: launch_tail<2>(in, seeds_out, sdf, W, H, s, gate); break;
     case 4: launch_tail<4>(in, seeds_out, sdf, W, H, s, gate); break;
     case 8: launch_tail<8>(in, seeds_out, sdf, W, H, s, gate); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+int seed_blocks(int W, int H) { return ((W + 63) / 64) * ((H + kSeedRows - 1) / kSeedRows); }
+int seed_bx(int W) { return (W + 63) / 64; }
+int jfa_pass_blocks(int W, int H) { return ((W + 63) / 64) * ((H + 3) / 4); }
+int jfa_pass_bx(int W) { return (W + 63) / 64; }
+
+int jfa_tail_blocks(int W, int H, int S)
+{
+    const int T = 64 - 2 * (2 * S - 1);
+    return ((W + T - 1) / T) * ((H + T - 1) / T);
+}
+
+int jfa_tail_bx(int W, int S)
+{
+    const int T = 64 - 2 * (2 * S - 1);
+    return (W + T - 1) / T;
+}
+
+hipError_t launch_seed_batch(const BatchLaunch &b, hipStream_t s)
+{
+    if (b.blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_seed_batch, dim3(b.blocks), dim3(kBlock), 0, s, (const SeedJob *)b.jobs, b.begins, b.njobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_jfa_batch(const BatchLaunch &b, hipStream_t s)
+{
+    if (b.blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_jfa_batch, dim3(b.blocks), dim3(kBlock), 0, s, (const JfaJob *)b.jobs, b.begins, b.njobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_tail_batch(const BatchLaunch &b, int S, hipStream_t s)
+{
+    if (b.blocks <= 0) return hipSuccess;
+    const TailJob *j = (const TailJob *)b.jobs;
+    switch (S) {
+    case 1: hipLaunchKernelGGL(k_tail_batch<1>, dim3(b.blocks), dim3(kTailBlock), 0, s, j, b.begins, b.njobs); break;
+    case 2: hipLaunchKernelGGL(k_tail_batch<2>, dim3(b.blocks), dim3(kTailBlock), 0, s, j, b.begins, b.njobs); break;
+    case 4: hipLaunchKernelGGL(k_tail_batch<4>, dim3(b.blocks), dim3(kTailBlock), 0, s, j, b.begins, b.njobs); break;
+    case 8: hipLaunchKernelGGL(k_tail_batch<8>, dim3(b.blocks), dim3(kTailBlock), 0, s, j, b.begins, b.njobs); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -71,10 +71,6 @@ static NoAllocLayout noalloc_layout(int W, int H, int L)
     return m;
 }
 
-static int dir_offset(int level)  // first direction of `level` in the table: sum of N_j, j < level
-{
-    return 4 * ((1 << (2 * level)) - 1) / 3;
-}
 
 // Once per device, with that device current: the device addresses of the
 // two arrays, and the direction tables (theta = 2 pi (k + 1/2) / N in double,
@@ -180,7 +176,7 @@ rc_status rc_run_device_noalloc(uint32_t width, uint32_t height, uint32_t levels
         a.sdf = sdf;
         a.emissive = g->emissive;
         a.emis4 = nullptr;
-        a.dirs = d->dirs + dir_offset(i);
+        a.dirs = d->dirs + dir_table_offset(i);
         a.upper = i + 1 < L ? (const void *)(d->scratch + lay.level[i + 1]) : nullptr;
         a.out = i >= 1 ? (void *)(d->scratch + lay.level[i]) : nullptr;
         a.W = W;
@@ -196,8 +192,7 @@ rc_status rc_run_device_noalloc(uint32_t width, uint32_t height, uint32_t levels
         a.spacing = (float)probe_spacing(i);
         if (i >= 1) {
             // the context path's default tiling (the result does not depend on it)
-            const int shape = (i == L - 1 && a.N >= 256) ? TILE_1x1x256 : (a.N >= 16 ? TILE_4x4x16 : TILE_8x8x4);
-            NTRY(launch_level(a, Storage::F32, shape, 2, s));
+            NTRY(launch_level(a, Storage::F32, default_level_tile(i, L), 2, s));
         } else {
             NTRY(launch_level0_gather(a, Storage::F32, irr, nullptr, s));
         }

@@ -420,9 +420,12 @@ class RadianceCascadeBatch:
     """Many views of uneven sizes as one graph launch (rc_batch_*).
 
     views: sequence of (depth [H,W], emissive [3,H,W], irradiance [3,H,W])
-    device tensors; levels: per-view cascade counts (0 = coverage rule)."""
+    device tensors; levels: per-view cascade counts (0 = coverage rule).
+    mode "worklist": one launch per stage over all views (shared workspace);
+    "streams": one context per view, frames captured on `streams` streams."""
 
-    def __init__(self, views, levels=None, storage: str = "f32", streams: int = 8, device: int = 0):
+    def __init__(self, views, levels=None, storage: str = "f32", mode: str = "worklist", streams: int = 8,
+                 device: int = 0):
         lib = _abi.load()
         n = len(views)
         arr = (_abi.View * n)()
@@ -435,7 +438,8 @@ class RadianceCascadeBatch:
             arr[i] = _abi.View(W, H, 0 if levels is None else int(levels[i]), 0, _dev_ptr(d, W * H * 4, "depth"),
                                _dev_ptr(e, 3 * W * H * 4, "emissive"), _dev_ptr(o, 3 * W * H * 4, "irradiance"))
         h = ctypes.c_void_p()
-        st = lib.rc_batch_create(arr, n, _abi.STORAGE[storage], streams, device, ctypes.byref(h))
+        cfg = _abi.BatchConfig(_abi.STORAGE[storage], _abi.BATCH_MODES[mode], streams, device)
+        st = lib.rc_batch_create(arr, n, ctypes.byref(cfg), ctypes.byref(h))
         if st != _abi.OK:
             raise RcError(st, (lib.rc_batch_last_error(None) or b"").decode())
         self._lib, self._h, self._views = lib, h, list(views)
@@ -455,6 +459,13 @@ class RadianceCascadeBatch:
         v = ctypes.c_uint32()
         self._check(self._lib.rc_batch_view_levels(self._h, i, ctypes.byref(v)))
         return int(v.value)
+
+    def info(self) -> dict:
+        """Kernel nodes of one batch launch (streams mode: after the first
+        run) and the bytes of the shared workspace (work-list mode)."""
+        k, b = ctypes.c_uint32(), ctypes.c_uint64()
+        self._check(self._lib.rc_batch_info(self._h, ctypes.byref(k), ctypes.byref(b)))
+        return {"kernel_nodes": int(k.value), "workspace_bytes": int(b.value)}
 
     def close(self) -> None:
         if getattr(self, "_h", None) is not None and self._h.value:

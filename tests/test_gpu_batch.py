@@ -1,6 +1,8 @@
 """Batches of views (rc_batch_*): one graph launch over views of uneven
 sizes must give, for every view, exactly the bits the single-view pass gives
-on the same G-buffer -- before and after autotuning, and on graph replay."""
+on the same G-buffer -- before and after autotuning, and on graph replay --
+in both batch modes: the work list (one launch per stage over every view,
+one shared workspace) and one context per view on captured streams."""
 import sys
 
 import pytest
@@ -25,32 +27,73 @@ def rc():
     return rc
 
 
-def test_batch_matches_single_views(rc):
+def _views(rc, sizes, levels=None, seed0=9):
     dev = torch.device("cuda:0")
-    views, refs = [], []
-    for j, (W, H) in enumerate(SIZES):
-        _say(f"[batch] view {j}: {W}x{H}")
+    views = []
+    for j, (W, H) in enumerate(sizes):
         d = torch.empty((H, W), device=dev)
         e = torch.empty((3, H, W), device=dev)
-        rc.gbuffer_raster(rc.scene_generate(9 + j, 12, 3), W, H, d, None, e)
-        o = torch.full((3, H, W), -1.0, device=dev)
-        views.append((d, e, o))
-        ref = torch.empty((3, H, W), device=dev)
-        with rc.RadianceCascadeStage(W, H, levels=0) as st:
+        rc.gbuffer_raster(rc.scene_generate(seed0 + j, 12, 3), W, H, d, None, e)
+        views.append((d, e, torch.full((3, H, W), -1.0, device=dev)))
+    return views
+
+
+def _single(rc, views, levels, storage="f32"):
+    refs = []
+    for (d, e, _), L in zip(views, levels):
+        H, W = d.shape
+        ref = torch.empty((3, H, W), device=d.device)
+        with rc.RadianceCascadeStage(W, H, levels=L, storage=storage) as st:
             st.run_device(d, e, ref)
         refs.append(ref)
     torch.cuda.synchronize()
-    with rc.RadianceCascadeBatch(views, streams=3) as b:
+    return refs
+
+
+@pytest.mark.parametrize("mode", ["worklist", "streams"])
+def test_batch_matches_single_views(rc, mode):
+    _say(f"[batch] {mode}: {len(SIZES)} views {SIZES}")
+    views = _views(rc, SIZES)
+    refs = _single(rc, views, [0] * len(SIZES))
+    with rc.RadianceCascadeBatch(views, mode=mode, streams=3) as b:
         for rnd in range(3):
             if rnd == 2:
+                _say(f"[batch] {mode}: autotune")
                 b.autotune(iters=1)
                 for _, _, o in views:
                     o.fill_(-1.0)
             b.run()
             torch.cuda.synchronize()
             for j, ((_, _, o), ref) in enumerate(zip(views, refs)):
-                assert torch.equal(o, ref), (rnd, j, SIZES[j])
+                assert torch.equal(o, ref), (mode, rnd, j, SIZES[j])
         assert [b.levels(j) for j in range(len(SIZES))] == [rc.levels_for_coverage(W, H) for W, H in SIZES]
+        info = b.info()
+        if mode == "worklist":
+            # seeds + global jump-flood passes + tail + march + levels + level 0
+            assert 10 <= info["kernel_nodes"] <= 24, info
+            assert info["workspace_bytes"] > 0
+        else:
+            assert info["kernel_nodes"] >= 10 * len(SIZES), info
+
+
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_worklist_levels_and_storage(rc, storage):
+    """Explicit level counts from 1 (no level >= 1: no march, level 0 only)
+    to 8 beside coverage-rule views, in half-precision storage: each view
+    equals its single-view frame, replayed twice."""
+    sizes = [(160, 90), (64, 64), (333, 77), (1, 1), (48, 200), (257, 129)]
+    levels = [1, 2, 8, 3, 0, 5]
+    _say(f"[batch] worklist {storage}: {list(zip(sizes, levels))}")
+    views = _views(rc, sizes, seed0=40)
+    refs = _single(rc, views, levels, storage)
+    with rc.RadianceCascadeBatch(views, levels=levels, storage=storage) as b:
+        for rnd in range(2):
+            for _, _, o in views:
+                o.fill_(-1.0)
+            b.run()
+            torch.cuda.synchronize()
+            for j, ((_, _, o), ref) in enumerate(zip(views, refs)):
+                assert torch.equal(o, ref), (storage, rnd, j, sizes[j], levels[j])
 
 
 def test_batch_rejects_bad_arguments(rc):
@@ -58,7 +101,8 @@ def test_batch_rejects_bad_arguments(rc):
         rc.RadianceCascadeBatch([(None, None, None)])
 
 
-def test_batch_workload_matches_single_views_and_oracle(rc):
+@pytest.mark.parametrize("mode", ["worklist", "streams"])
+def test_batch_workload_matches_single_views_and_oracle(rc, mode):
     """The 64-view batch workload (rc.batch_views_spec(1000, 64): seeds
     1000..1063, 16..64 boxes, W and H from {256, ..., 2048}, 5..7 levels by the
     coverage rule, 86 MP) in one graph launch, autotuned: every view equals its
@@ -76,8 +120,8 @@ def test_batch_workload_matches_single_views_and_oracle(rc):
         views.append((d, e, torch.full((3, H, W), -1.0, device=dev)))
         gb.append((seed, W, H, nb, ne))
     torch.cuda.synchronize()
-    _say(f"[batch workload] {len(spec)} views, {sum(W * H for _, W, H, _, _ in spec) / 1e6:.1f} MP")
-    with rc.RadianceCascadeBatch(views, streams=8) as b:
+    _say(f"[batch workload] {mode}: {len(spec)} views, {sum(W * H for _, W, H, _, _ in spec) / 1e6:.1f} MP")
+    with rc.RadianceCascadeBatch(views, mode=mode, streams=8) as b:
         b.autotune(iters=1)
         b.run()
         torch.cuda.synchronize()
