@@ -208,6 +208,64 @@ def surrogate_leg(rc, torch, W, H, L, args, depth, emis, irr_plain, stream, fps_
                              "unit": "TFLOP/s", "frac": tflops / BF16_MFMA_PEAK_TFLOPS}}
 
 
+def _time_stage(torch, stage, depth, emis, out, stream, warmup, steps):
+    """Mean ms per frame of `steps` frames launched back to back on `stream`."""
+    for _ in range(warmup):
+        stage.run_device(depth, emis, out, stream)
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(steps):
+        stage.run_device(depth, emis, out, stream)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    return ev0.elapsed_time(ev1) / steps
+
+
+def storage_leg(rc, torch, W, H, L, args, depth, emis, irr_f32, stream):
+    """The plain frame with 16-bit storage of the merged levels (fp16, bf16),
+    autotuned like the fp32 one: frames/s and irradiance rel-L2 / max-abs
+    against the fp32 frame (GPU FP64 reduction)."""
+    out = {}
+    for st in ("f16", "bf16"):
+        stage = rc.RadianceCascadeStage(W, H, levels=L, storage=st)
+        o = torch.empty_like(irr_f32)
+        stage.autotune(depth, emis, o, iters=2, stream=stream)
+        ms = _time_stage(torch, stage, depth, emis, o, stream, args.warmup, args.steps)
+        err = stage.error_norms(o, irr_f32, stream)
+        out[st] = {"fps": 1000.0 / ms, "ms_per_step": ms, "rel_l2_vs_fp32": err["rel_l2"],
+                   "max_abs_vs_fp32": err["max_abs"]}
+        stage.close()
+    return out
+
+
+def uhd_leg(rc, torch, args, stream):
+    """3840x2160 with 7 levels (the seeded scene of the main workload):
+    plain and learned frames/s, both autotuned, and the learned frame's
+    rel-L2 against the plain one."""
+    W, H, L = 3840, 2160, 7
+    dev = torch.device("cuda:0")
+    depth = torch.empty((H, W), device=dev)
+    emis = torch.empty((3, H, W), device=dev)
+    rc.gbuffer_raster(rc.scene_generate(args.seed, args.boxes, args.emissive), W, H, depth, None, emis)
+    irr = torch.empty((3, H, W), device=dev)
+    out = torch.empty_like(irr)
+    plain = rc.RadianceCascadeStage(W, H, levels=L, storage=args.storage)
+    plain.autotune(depth, emis, irr, iters=2, stream=stream)
+    ms_p = _time_stage(torch, plain, depth, emis, irr, stream, args.warmup, args.steps)
+    sur = rc.RadianceCascadeStage(W, H, levels=L, storage=args.storage)
+    sur.load_weights(args.weights)
+    sur.set_surrogate(True)
+    sur.autotune(depth, emis, out, iters=2, stream=stream)
+    ms_s = _time_stage(torch, sur, depth, emis, out, stream, args.warmup, args.steps)
+    err = sur.error_norms(out, irr, stream)
+    plain.close()
+    sur.close()
+    return {"workload": f"{W}x{H}, {L} cascades, seed {args.seed}", "plain_fps": 1000.0 / ms_p,
+            "learned_fps": 1000.0 / ms_s, "ratio": ms_p / ms_s, "learned_rel_l2_vs_plain": err["rel_l2"]}
+
+
 def load_tuning(arg):
     """A tuning as written by --save-tuning: a JSON file, or the JSON text itself."""
     text = arg.strip()
@@ -396,6 +454,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-storage", action="store_true", help="skip the fp16 / bf16 storage leg")
+    ap.add_argument("--no-uhd", action="store_true", help="skip the 3840x2160, 7-level leg")
     ap.add_argument("--no-host-staging", action="store_true",
                     help="skip the host-staging leg (profiled runs: its contexts use the default configuration)")
     ap.add_argument("--weights", default=None, help="surrogate weights (default: the packaged network)")
@@ -510,6 +570,8 @@ def main():
     if not args.no_surrogate and L >= 3:
         surrogate = surrogate_leg(rc, torch, W, H, L, args, depth, emis, irr, stream, fps)
 
+    storage = None if args.no_storage else storage_leg(rc, torch, W, H, L, args, depth, emis, irr, stream)
+    uhd = None if args.no_uhd else uhd_leg(rc, torch, args, stream)
     batch = None if args.no_batch else batch_leg(rc, torch, args, stream, fps)
     staging = None if args.no_host_staging else host_staging_leg(rc, W, H, L, args.storage, depth, emis)
 
@@ -552,6 +614,8 @@ def main():
                            "frac": frame_alg / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "stage_ms_per_frame": {k: round(v, 5) for k, v in per_frame_ms.items()},
         "surrogate": surrogate,
+        "storage": storage,
+        "uhd": uhd,
         "batch": batch,
         "host_staging": staging,
         "cpu_baseline": cpu,

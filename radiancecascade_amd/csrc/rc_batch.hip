@@ -496,18 +496,6 @@ hipError_t worklist_autotune(WorkList *w, int iters, hipStream_t s, std::string 
     return hipSuccess;
 }
 
-void worklist_get_tuning(const WorkList *w, int *fuse, int *split, int *spec, int *m, int *tile, int *rpt)
-{
-    *fuse = w->fuse;
-    *split = w->split;
-    *spec = w->spec;
-    *m = w->m;
-    for (int i = 0; i < RC_MAX_LEVELS; ++i) {
-        tile[i] = w->tile[i];
-        rpt[i] = w->rpt[i];
-    }
-}
-
 int worklist_launches(const WorkList *w) { return (int)w->launches.size(); }
 
 }  // namespace rc

@@ -16,7 +16,5 @@ hipError_t worklist_run(WorkList *w, hipStream_t s, std::string &err);
 hipError_t worklist_autotune(WorkList *w, int iters, hipStream_t s, std::string &err);
 size_t worklist_workspace_bytes(const WorkList *w);
 int worklist_launches(const WorkList *w);
-// tile / rpt: RC_MAX_LEVELS entries
-void worklist_get_tuning(const WorkList *w, int *fuse, int *split, int *spec, int *m, int *tile, int *rpt);
 
 }  // namespace rc
