@@ -201,6 +201,15 @@ uint32_t rc_get_jfa_fuse(const rc_ctx *ctx);
  * rc_autotune also picks these (a one-level frame has nothing to march). */
 rc_status rc_set_march(rc_ctx *ctx, uint32_t split, uint32_t speculation, uint32_t rays_per_lane);
 rc_status rc_get_march(const rc_ctx *ctx, uint32_t *split, uint32_t *speculation, uint32_t *rays_per_lane);
+/* Order of a level's rays over the lanes of the split march (split = 1; no
+ * counterpart in the reference; every order stores the same codes): 0 =
+ * canonical order (a wave marches 64 consecutive rays: 64 directions of one
+ * probe, or 64/N whole probes); wave tiles of probes x directions, 1 = 2x2 x 16,
+ * 2 = 4x4 x 4, 3 = 4x2 x 8, 4 = 2x1 x 32, 5 = 8x8 x 1 (the slice at most
+ * N_level).  Rays from nearby probes in nearby directions march for similar
+ * lengths, so fewer lanes idle; rc_autotune picks the order per level. */
+rc_status rc_set_march_order(rc_ctx *ctx, uint32_t level, uint32_t order);
+rc_status rc_get_march_order(const rc_ctx *ctx, uint32_t level, uint32_t *order);
 
 /* ---- introspection -------------------------------------------------------- */
 rc_status rc_get_config(const rc_ctx *ctx, rc_config *out);
@@ -231,9 +240,11 @@ rc_status rc_debug_level(rc_ctx *ctx, uint32_t level, float *dev_out, void *stre
  * levels marched in place (split=0, level 0, and levels 0-1 of the learned
  * frame) return the copy those kernels write under RC_FLAG_KEEP_DEBUG
  * (RC_ERR_STATE otherwise, and for levels the learned frame does not march).
- * index_dev (optional, levels >= 1): per slot r, the canonical index that the
- * split march's decode of r into (probe column, row, direction) stands for;
- * equal to r when the decode is right. */
+ * index_dev (optional, levels >= 1): per canonical ray, the slot of the
+ * split march (with the level's current lane order) whose decode into (probe
+ * column, row, direction) lands on that ray, -1 if none: a permutation of
+ * the slots when every ray is marched exactly once (the identity in the
+ * canonical order). */
 rc_status rc_debug_march(rc_ctx *ctx, uint32_t level, int32_t *hit_dev, int64_t *index_dev, void *stream);
 
 /* ---- error norms (FP64 reductions on the GPU) ------------------------------

@@ -130,6 +130,8 @@ SIGNATURES = {
     "rc_set_jfa_fuse": (_i32, [_P, _u32]),
     "rc_get_jfa_fuse": (_u32, [_P]),
     "rc_set_march": (_i32, [_P, _u32, _u32, _u32]),
+    "rc_set_march_order": (_i32, [_P, _u32, _u32]),
+    "rc_get_march_order": (_i32, [_P, _u32, ctypes.POINTER(_u32)]),
     "rc_get_march": (_i32, [_P, ctypes.POINTER(_u32), ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
     "rc_get_config": (_i32, [_P, ctypes.POINTER(Config)]),
     "rc_get_stats": (_i32, [_P, ctypes.POINTER(Stats)]),

@@ -45,6 +45,7 @@ struct rc_ctx {
     // above): its own tiling, -1 until tuned (then the plain level 1's)
     int surr_tile1 = -1, surr_rpt1 = -1;
     int jfa_fuse = 0;  // largest jump-flood step fused into the LDS tail (0 = none)
+    int mmap[RC_MAX_LEVELS] = {};  // split march: wave-tile order of level i's rays (0 = canonical)
     // split march: levels >= 1 marched by one k_march_all launch into `codes`,
     // then merged top-down; otherwise each level kernel marches its own rays
     int march_split = 1;   // 0 in place, 1 split, 2 split and persistent
@@ -307,9 +308,10 @@ static MarchAllArgs march_all_args(const rc_ctx *c)
         l.spacing = (float)probe_spacing(i);
         l.log2n = 2 + 2 * i;
         l.inv_px = 1.0f / (float)l.PX;
+        march_level_map(l, c->march_split == 1 ? c->mmap[i] : 0);  // the persistent form is canonical
         l.block_begin = blocks;
         const int64_t rays = (int64_t)l.PX * l.PY * l.N, per = (int64_t)kBlock * c->march_m;
-        l.blocks = (int)((rays + per - 1) / per);
+        l.blocks = (int)(((int64_t)l.slots + per - 1) / per);
         blocks += l.blocks;
         l.code_begin = (uint32_t)c->code_off[i];
         l.dir_off = (int)c->dir_off[i];
@@ -1101,6 +1103,7 @@ rc_status rc_debug_march(rc_ctx *c, uint32_t level, int32_t *hit_dev, int64_t *i
     if (index_dev) {
         if (level == 0) return fail(c, RC_ERR_INVALID, "rc_debug_march: level 0 has no split-march decode");
         const MarchAllArgs m = march_all_args(c);
+        HIP_TRY(c, hipMemsetAsync(index_dev, 0xFF, rays * sizeof(int64_t), s));  // -1: no slot
         HIP_TRY(c, launch_ray_decode(m.lev[c->L - 1 - (int)level], index_dev, s));
     }
     return RC_OK;
@@ -1147,6 +1150,24 @@ rc_status rc_set_march(rc_ctx *c, uint32_t split, uint32_t speculation, uint32_t
     c->spec = (int)speculation;
     c->march_m = (int)rays_per_lane;
     drop_graph(c);
+    return RC_OK;
+}
+
+rc_status rc_set_march_order(rc_ctx *c, uint32_t level, uint32_t order)
+{
+    if (!c) return fail(nullptr, RC_ERR_NULL, "rc_set_march_order: ctx is NULL");
+    if ((int)level >= c->L || !march_map_legal((int)level, (int)order))
+        return fail(c, RC_ERR_INVALID, "rc_set_march_order: level 1 .. L-1, order 0 .. 5 with its slice at most N_level");
+    c->mmap[level] = (int)order;
+    drop_graph(c);
+    return RC_OK;
+}
+
+rc_status rc_get_march_order(const rc_ctx *c, uint32_t level, uint32_t *order)
+{
+    if (!c || !order) return fail(nullptr, RC_ERR_NULL, "rc_get_march_order: NULL argument");
+    if (level == 0 || (int)level >= c->L) return fail(nullptr, RC_ERR_INVALID, "rc_get_march_order: level out of range");
+    *order = (uint32_t)c->mmap[level];
     return RC_OK;
 }
 
@@ -1230,6 +1251,28 @@ rc_status rc_autotune(rc_ctx *c, const rc_gbuffer *g, float *irr, uint32_t iters
         c->march_split = best_split;
         c->spec = best_spec;
         c->march_m = best_m;
+        if (best_split == 1) {
+            // the order of each level's rays over the lanes, level by level
+            // (the march launch timed whole, the other levels as chosen)
+            for (int i = c->L - 1; i >= 1; --i) {
+                float lbest = 1e30f;
+                int bmap = c->mmap[i];
+                for (int map = 0; map < kMarchMaps; ++map) {
+                    if (!march_map_legal(i, map)) continue;
+                    c->mmap[i] = map;
+                    const MarchAllArgs m = march_all_args(c);
+                    HIP_TRY(c, launch_march_all(m, c->spec, c->march_m, s));  // warm
+                    HIP_TRY(c, hipEventRecord(e0, s));
+                    for (uint32_t it = 0; it < iters; ++it) HIP_TRY(c, launch_march_all(m, c->spec, c->march_m, s));
+                    HIP_TRY(c, hipEventRecord(e1, s));
+                    HIP_TRY(c, hipEventSynchronize(e1));
+                    float ms = 0.f;
+                    HIP_TRY(c, hipEventElapsedTime(&ms, e0, e1));
+                    if (ms < lbest) { lbest = ms; bmap = map; }
+                }
+                c->mmap[i] = bmap;
+            }
+        }
         if (best_split == 2) HIP_TRY(c, launch_march_persist(march_all_args(c), c->spec, s));
         else if (best_split) HIP_TRY(c, launch_march_all(march_all_args(c), c->spec, c->march_m, s));
     }

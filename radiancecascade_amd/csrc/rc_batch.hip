@@ -58,7 +58,7 @@ struct WorkList {
     float2 *dirs = nullptr;
     // tuning
     int fuse = 4, split = 1, spec = 1, m = 1;
-    int tile[RC_MAX_LEVELS] = {}, rpt[RC_MAX_LEVELS] = {};
+    int tile[RC_MAX_LEVELS] = {}, rpt[RC_MAX_LEVELS] = {}, mmap[RC_MAX_LEVELS] = {};
     // launches of the current tuning
     std::vector<WlLaunch> launches;
     uint8_t *tables = nullptr;
@@ -210,8 +210,9 @@ static hipError_t build_tables(WorkList &w, std::string &err)
                 l.t1 = (float)interval_t1(i);
                 l.spacing = (float)probe_spacing(i);
                 l.dir_off = dir_table_offset(i);
-                const int64_t rays = (int64_t)l.PX * l.PY * l.N, per = (int64_t)kBlock * w.m;
-                l.blocks = (int)((rays + per - 1) / per);
+                march_level_map(l, w.mmap[i]);
+                const int64_t per = (int64_t)kBlock * w.m;
+                l.blocks = (int)(((int64_t)l.slots + per - 1) / per);
                 j.push_back(MarchJob{w.ws + x.sdf, x.W, x.H, l});
                 b.push_back(l.blocks);
             }
@@ -414,8 +415,9 @@ static hipError_t time_launches(WorkList &w, hipStream_t s, int iters, Pick pick
     return hipSuccess;
 }
 
-// Stage by stage: the jump-flood tail depth, the march's speculation depth
-// and rays per lane, then per level the tile shape and probes per thread,
+// Stage by stage: the jump-flood tail depth, the march's speculation depth,
+// rays per lane and per-level lane order, then per level the tile shape and
+// probes per thread,
 // for the split march and for marching in the level kernels; the split form
 // is kept if its march plus levels is faster.
 hipError_t worklist_autotune(WorkList *w, int iters, hipStream_t s, std::string &err)
@@ -477,6 +479,18 @@ hipError_t worklist_autotune(WorkList *w, int iters, hipStream_t s, std::string 
         }
         x.spec = bspec;
         x.m = bm;
+        for (int i = x.maxL - 1; i >= 1; --i) {  // each level's lane order, the march timed whole
+            best = 1e30f;
+            int bmap = x.mmap[i];
+            for (int map = 0; map < kMarchMaps; ++map) {
+                if (!march_map_legal(i, map)) continue;
+                x.mmap[i] = map;
+                WL_TRY(build_tables(x, err));
+                WL_TRY(time_launches(x, s, iters, [](const WlLaunch &L) { return L.kind == WL_MARCH; }, ms, err));
+                if (ms < best) best = ms, bmap = map;
+            }
+            x.mmap[i] = bmap;
+        }
         WL_TRY(frame());
         float lv = 0.f;
         WL_TRY(tune_levels(lv));

@@ -477,6 +477,33 @@ __device__ inline void decode_ray(const MarchLevel &L, int ray, int &px, int &py
     else if (px >= L.PX) { ++py; px -= L.PX; }
 }
 
+// Split-march slot -> (px, py, k).  Canonical order: the slot is the
+// canonical index.  Wave tiles: slot = T*64 + lane, T = (tile row * ntx +
+// tile column) * slices + slice, lane = (probe in the tile, direction in the
+// slice) with the direction fastest; false for the lanes of a ragged edge
+// tile that fall outside the probe grid.  Rays with nearby start points and
+// directions share a wave, so their march lengths are alike.
+__device__ inline bool decode_slot(const MarchLevel &L, int slot, int &px, int &py, int &k)
+{
+    if (!L.map) {
+        decode_ray(L, slot, px, py, k);
+        return true;
+    }
+    const int lane = slot & 63, T = slot >> 6;
+    const int ns_log = L.log2n - L.kd_log;
+    const int slice = T & ((1 << ns_log) - 1);
+    const int tt = T >> ns_log;
+    int ty = (int)((float)tt * L.inv_ntx);
+    int tx = tt - ty * L.ntx;
+    if (tx < 0) { --ty; tx += L.ntx; }
+    else if (tx >= L.ntx) { ++ty; tx -= L.ntx; }
+    const int pl = lane >> L.kd_log;
+    k = (slice << L.kd_log) | (lane & ((1 << L.kd_log) - 1));
+    px = (tx << L.tx_log) | (pl & ((1 << L.tx_log) - 1));
+    py = (ty << L.ty_log) | (pl >> L.tx_log);
+    return px < L.PX && py < L.PY;
+}
+
 // All levels >= 1 marched by one launch; each ray's code (hit pixel, -1 left
 // the image, -2 reached t1) is written in canonical order for the merge
 // kernels.  The block list is level-major with the top level first, so the
@@ -495,23 +522,25 @@ __device__ __forceinline__ void march_all_body(const uint8_t *sdf_codes, int W, 
 {
     const __amdgpu_buffer_rsrc_t sdf = sdf_rsrc(sdf_codes, W, H);
     const int base = lb * kBlock * M + (int)threadIdx.x;
-    const int total = L.PX * L.PY * L.N;
+    const int total = L.map ? (int)L.slots : L.PX * L.PY * L.N;
 
     int j = 0, ray = base;
+    bool valid = false;      // the slot holds a ray of the grid
+    uint32_t at = 0;         // its canonical index (where its code goes)
     Ray r;
     auto start = [&]() {
-        const bool active = ray < total;
         int px, py, k;
-        decode_ray(L, active ? ray : 0, px, py, k);
+        valid = decode_slot(L, ray < total ? ray : 0, px, py, k) && ray < total;
+        at = ((__umul24((uint32_t)py, (uint32_t)L.PX) + (uint32_t)px) << L.log2n) + (uint32_t)k;
         const float2 dir = L.dirs[k];
         r = ray_begin(((float)px + 0.5f) * L.spacing, ((float)py + 0.5f) * L.spacing, dir.x, dir.y, L.t0, L.t1,
-                      active);
+                      valid);
     };
     start();
     for (;;) {
         if (r.alive) ray_round<SPEC>(r, sdf, W, H, L.t1);
         if (j < M && !r.alive) {
-            if (ray < total) L.codes[ray] = r.code;
+            if (valid) L.codes[at] = r.code;
             ray += kBlock;
             if (++j < M) start();
         }
@@ -645,13 +674,14 @@ __global__ __launch_bounds__(kBlock) void k_march_persist(MarchAllArgs a)
 // The split march's decode of every slot of one level, recomposed into the
 // canonical index it stands for (parity readback: equal to the slot number
 // when the decode is right).
+// For every slot of a level's split march, its ray's canonical index gets
+// the slot number (the slot that marches it).
 __global__ __launch_bounds__(kBlock) void k_ray_decode(MarchLevel l, int64_t *__restrict__ index)
 {
-    const int ray = (int)(blockIdx.x * kBlock + threadIdx.x);
-    if (ray >= l.PX * l.PY * l.N) return;
+    const int slot = (int)(blockIdx.x * kBlock + threadIdx.x);
+    if (slot >= (l.map ? (int)l.slots : l.PX * l.PY * l.N)) return;
     int px, py, k;
-    decode_ray(l, ray, px, py, k);
-    index[ray] = ((int64_t)py * l.PX + px) * l.N + k;
+    if (decode_slot(l, slot, px, py, k)) index[((int64_t)py * l.PX + px) * l.N + k] = slot;
 }
 
 template <Storage S>
@@ -875,9 +905,35 @@ hipError_t launch_level_to_f32(const void *level, Storage st, size_t texels, flo
     return hipGetLastError();
 }
 
+bool march_map_legal(int level, int map)
+{
+    static const int kd_log[kMarchMaps] = {0, 4, 2, 3, 5, 0};
+    return level >= 1 && map >= 0 && map < kMarchMaps && (map == 0 || (1 << kd_log[map]) <= num_dirs(level));
+}
+
+void march_level_map(MarchLevel &l, int map)
+{
+    // (tx_log, ty_log, kd_log) of each wave tile
+    static const int tile[kMarchMaps][3] = {{0, 0, 0}, {1, 1, 4}, {2, 2, 2}, {2, 1, 3}, {1, 0, 5}, {3, 3, 0}};
+    l.map = map;
+    l.tx_log = tile[map][0];
+    l.ty_log = tile[map][1];
+    l.kd_log = tile[map][2];
+    if (map == 0) {
+        l.ntx = 0;
+        l.inv_ntx = 0.0f;
+        l.slots = (uint32_t)((int64_t)l.PX * l.PY * l.N);
+        return;
+    }
+    l.ntx = (l.PX + (1 << l.tx_log) - 1) >> l.tx_log;
+    const int nty = (l.PY + (1 << l.ty_log) - 1) >> l.ty_log;
+    l.inv_ntx = 1.0f / (float)l.ntx;
+    l.slots = (uint32_t)((int64_t)l.ntx * nty * (l.N >> l.kd_log) * 64);
+}
+
 hipError_t launch_ray_decode(const MarchLevel &l, int64_t *index, hipStream_t s)
 {
-    const int64_t total = (int64_t)l.PX * l.PY * l.N;
+    const int64_t total = l.map ? (int64_t)l.slots : (int64_t)l.PX * l.PY * l.N;
     const unsigned blocks = (unsigned)((total + kBlock - 1) / kBlock);
     if (blocks == 0) return hipSuccess;
     hipLaunchKernelGGL(k_ray_decode, dim3(blocks), dim3(kBlock), 0, s, l, index);

@@ -154,7 +154,22 @@ struct MarchLevel {
     int block_begin, blocks; // workgroups of 256*M canonical rays
     uint32_t code_begin;     // first ray of this level in the code buffer (levels 1, 2, ... back to back)
     int dir_off;             // first direction of this level in dirs_base
+    // Order of the rays over the lanes (k_march_all only): map 0 = canonical
+    // order (slot = canonical index); map > 0 = wave tiles of
+    // 2^tx_log x 2^ty_log probes x 2^kd_log directions (tx_log + ty_log +
+    // kd_log = 6), tiles numbered (tile row, tile column, direction slice)
+    // with the slice fastest; `slots` counts the padded tiles' lanes.
+    int map, tx_log, ty_log, kd_log, ntx;
+    float inv_ntx;
+    uint32_t slots;
 };
+// Wave-tile orders of the split march: 0 canonical, 1 = 2x2 probes x 16
+// directions, 2 = 4x4 x 4, 3 = 4x2 x 8, 4 = 2x1 x 32, 5 = 8x8 x 1.
+constexpr int kMarchMaps = 6;
+bool march_map_legal(int level, int map);
+// fills map, the wave tile, ntx, inv_ntx and slots of a level (PX, PY, N,
+// log2n set); the caller sizes blocks from slots
+void march_level_map(MarchLevel &l, int map);
 struct MarchAllArgs {
     const uint8_t *sdf;
     int W, H, nlev;

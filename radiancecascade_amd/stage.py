@@ -144,6 +144,20 @@ class RadianceCascadeStage:
         check(self._lib.rc_get_march(self._ctx, ctypes.byref(sp), ctypes.byref(k), ctypes.byref(m)), self._ctx)
         return sp.value, k.value, m.value
 
+    def set_march_order(self, level: int, order: int) -> None:
+        """Order of level `level`'s rays over the split march's lanes: 0
+        canonical, 1..5 wave tiles of probes x directions (rc_set_march_order)."""
+        check(self._lib.rc_set_march_order(self._ctx, level, order), self._ctx)
+
+    def march_order(self) -> dict:
+        """{level: order} for levels >= 1."""
+        out = {}
+        for i in range(1, self.levels):
+            v = ctypes.c_uint32()
+            check(self._lib.rc_get_march_order(self._ctx, i, ctypes.byref(v)), self._ctx)
+            out[i] = int(v.value)
+        return out
+
     def set_jfa_fuse(self, fused_step: int) -> None:
         check(self._lib.rc_set_jfa_fuse(self._ctx, fused_step), self._ctx)
 

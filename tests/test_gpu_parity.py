@@ -359,6 +359,52 @@ def test_every_march_mode_is_bit_exact(rc, oracle, W, H, L, seed):
             st.set_march(3, 1, 1)
 
 
+@pytest.mark.parametrize("W,H,L,seed", [(1920, 1080, 6, 0), (97, 61, 6, 3), (300, 1, 3, 1)])
+def test_every_march_order_is_bit_exact(rc, oracle, W, H, L, seed):
+    """Every order of a level's rays over the split march's lanes (canonical,
+    wave tiles of probes x directions, ragged edge tiles included; with one and
+    two rays per lane) stores the oracle's code for every ray, and the decode
+    places each ray in exactly one slot: the slot index per ray is a
+    permutation of the slots that hold a ray (the identity in the canonical
+    order)."""
+    _say(f"march orders W={W} H={H} L={L} seed={seed}")
+    depth, emis = _device_gbuffer(rc, W, H, seed, 32, 4)
+    d, _, e = oracle.gbuffer(seed, W, H, 32, 4)
+    irr32, lv32, seeds = oracle.run(d, e, L, np.float32, want_levels=True, want_seeds=True)
+    sdf = oracle.sdf_f32(seeds)
+    hits_ref = [oracle.march_level(sdf, e, i)[1].ravel() for i in range(L)]
+    irr = torch.empty((3, H, W), dtype=torch.float32, device="cuda:0")
+    with rc.RadianceCascadeStage(W, H, levels=L) as st:
+        n_checked = 0
+        for order in range(6):
+            for rpl in (1, 2):
+                st.set_march(1, 1, rpl)
+                for i in range(1, L):
+                    if order == 1 and 4 * 4 ** i < 16 or order == 4 and 4 * 4 ** i < 32:
+                        with pytest.raises(rc.RcError):
+                            st.set_march_order(i, order)
+                        st.set_march_order(i, 0)
+                    else:
+                        st.set_march_order(i, order)
+                st.run_device(depth, emis, irr)
+                torch.cuda.synchronize()
+                assert np.array_equal(irr.cpu().numpy().view(np.uint32), irr32.view(np.uint32)), (order, rpl)
+                for i in range(1, L):
+                    n = rc.level_rays(W, H, i)
+                    h = torch.empty(n, dtype=torch.int32, device="cuda:0")
+                    ix = torch.empty(n, dtype=torch.int64, device="cuda:0")
+                    st.debug_march(i, h, ix)
+                    torch.cuda.synchronize()
+                    assert np.array_equal(h.cpu().numpy(), hits_ref[i]), (order, rpl, i)
+                    slots = ix.cpu().numpy()
+                    assert slots.min() >= 0, (order, i, "a ray has no slot")
+                    assert np.unique(slots).size == n, (order, i, "two rays share a slot")
+                    if st.march_order()[i] == 0:
+                        assert np.array_equal(slots, np.arange(n)), (order, i)
+                    n_checked += 1
+        assert n_checked >= 12 * (L - 1) - 12
+
+
 @pytest.mark.parametrize("spec", [1, 2, 4])
 def test_persistent_march_is_bit_exact(rc, oracle, spec):
     """The persistent split march (split 2) stores the same ray codes and
