@@ -310,6 +310,7 @@ static MarchAllArgs march_all_args(const rc_ctx *c)
         l.inv_px = 1.0f / (float)l.PX;
         march_level_map(l, c->march_split == 1 ? c->mmap[i] : 0);  // the persistent form is canonical
         l.block_begin = blocks;
+        m.level_begin[m.nlev - 1] = blocks;
         const int64_t rays = (int64_t)l.PX * l.PY * l.N, per = (int64_t)kBlock * c->march_m;
         l.blocks = (int)(((int64_t)l.slots + per - 1) / per);
         blocks += l.blocks;

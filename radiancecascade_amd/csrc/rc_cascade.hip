@@ -536,6 +536,39 @@ __device__ __forceinline__ void march_all_body(const uint8_t *sdf_codes, int W, 
         r = ray_begin(((float)px + 0.5f) * L.spacing, ((float)py + 0.5f) * L.spacing, dir.x, dir.y, L.t0, L.t1,
                       valid);
     };
+    if (M == 1 && SPEC == 1) {
+        // One ray per lane, one sample per step: a branch-free body in a
+        // divergent loop (lanes leave as their rays end) on the raw byte
+        // code.  The samples are those of ray_round<1> -- the same positions,
+        // the same step -- and the ray's end is read from where t stopped: t
+        // advances only on an in-image sample, by 0 on a hit (code 0), so a
+        // ray that ends with t < t1 ended at the sample at t, which is
+        // off-image (exit) or a hit; otherwise it reached t1.
+        int px, py, k;
+        valid = decode_slot(L, ray < total ? ray : 0, px, py, k) && ray < total;
+        at = ((__umul24((uint32_t)py, (uint32_t)L.PX) + (uint32_t)px) << L.log2n) + (uint32_t)k;
+        const float2 dir = L.dirs[k];
+        const float cx = ((float)px + 0.5f) * L.spacing, cy = ((float)py + 0.5f) * L.spacing;
+        const int tiles = sdf_tiles_x(W);
+        const float t1 = L.t1;
+        float t = L.t0;
+        bool alive = valid && t < t1;
+        while (alive) {
+            const int ix = floor_i32(fmaf(t, dir.x, cx)), iy = floor_i32(fmaf(t, dir.y, cy));
+            const uint32_t c = sdf_code_load(sdf, ix, iy, tiles);
+            const bool ins = (uint32_t)ix < (uint32_t)W && (uint32_t)iy < (uint32_t)H;
+            const float tn = t + step_of_code(c);
+            t = ins ? tn : t;
+            alive = ins && c != 0 && tn < t1;
+        }
+        int code = kRayT1;
+        if (t < t1) {
+            const int ix = floor_i32(fmaf(t, dir.x, cx)), iy = floor_i32(fmaf(t, dir.y, cy));
+            code = ((uint32_t)ix < (uint32_t)W && (uint32_t)iy < (uint32_t)H) ? pixel_index(ix, iy, W) : kRayExit;
+        }
+        if (valid) L.codes[at] = code;
+        return;
+    }
     start();
     for (;;) {
         if (r.alive) ray_round<SPEC>(r, sdf, W, H, L.t1);
@@ -551,9 +584,11 @@ __device__ __forceinline__ void march_all_body(const uint8_t *sdf_codes, int W, 
 template <int SPEC, int M>
 __global__ __launch_bounds__(kBlock) void k_march_all(MarchAllArgs a)
 {
+    // the level: the number of later levels whose blocks start at or before
+    // this one (independent scalar loads, one wait)
     int lv = 0;
-#pragma unroll 1
-    while (lv + 1 < a.nlev && (int)blockIdx.x >= a.lev[lv + 1].block_begin) ++lv;
+#pragma unroll
+    for (int j = 1; j < RC_MAX_LEVELS; ++j) lv += (j < a.nlev && (int)blockIdx.x >= a.level_begin[j]) ? 1 : 0;
     march_all_body<SPEC, M>(a.sdf, a.W, a.H, a.lev[lv], (int)blockIdx.x - a.lev[lv].block_begin);
 }
 

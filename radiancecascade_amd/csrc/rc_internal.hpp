@@ -173,6 +173,7 @@ void march_level_map(MarchLevel &l, int map);
 struct MarchAllArgs {
     const uint8_t *sdf;
     int W, H, nlev;
+    int level_begin[RC_MAX_LEVELS];  // lev[j].block_begin, contiguous (one scalar load)
     MarchLevel lev[RC_MAX_LEVELS];  // top level first
     // persistent form: every ray of every level, in 64-ray packets
     int *codes_base;

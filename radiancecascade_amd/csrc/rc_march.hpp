@@ -52,6 +52,25 @@ __device__ inline float sdf_load(__amdgpu_buffer_rsrc_t r, int x, int y, int til
     return sdf_value(__builtin_amdgcn_raw_buffer_load_b8(r, (int)off, 0, 0));
 }
 
+// The raw byte code at (x, y) (same offset and range check as sdf_load).
+__device__ inline uint32_t sdf_code_load(__amdgpu_buffer_rsrc_t r, int x, int y, int tiles_x)
+{
+    const uint32_t ux = (uint32_t)x, uy = (uint32_t)y;
+    const uint32_t in = bfi(127u, bfi(15u, ux, uy << 4), ux << 3);
+    const uint32_t off = __umul24(uy >> 3, (uint32_t)tiles_x << 7) + in;
+    return __builtin_amdgcn_raw_buffer_load_b8(r, (int)off, 0, 0);
+}
+
+// march_step(sdf_value(c)) for a code c >= 1 (c = 0 is a hit and takes no
+// step): sdf_value is c below 128 and 16 c - 1920 from 128 on, and
+// 16 c - 1920 < c exactly when c < 128, so the value is max(c, 16 c - 1920)
+// (exact in fp32), already >= 1.
+__device__ inline float step_of_code(uint32_t c)
+{
+    const float f = (float)c;
+    return fmaxf(f, fmaf(f, 16.0f, -1920.0f));
+}
+
 // floor(v) as an int in one VALU instruction (v_cvt_flr_i32_f32; v is a
 // finite sample position here, far inside the int range).
 __device__ inline int floor_i32(float v)
