@@ -91,48 +91,53 @@ constexpr int kRayExit = -1, kRayT1 = -2;
 // distance-field loads back to back, so a lane keeps RPT independent load
 // chains in flight (the march is a chain of dependent gathers and is bound by
 // their latency, not by bandwidth).  A finished ray's slot keeps issuing a
-// (cached, ignored) load until every ray of the lane is done.
+// (cached, ignored) load until every ray of the lane is done.  The body is
+// branch-free on the raw byte code: t advances only on an in-image sample,
+// by step_of_code (0 on a hit), so where t stops tells how the ray ended --
+// at t1 if t >= t1, else at the sample at t, off-image (exit) or a hit.  The
+// samples, and so the codes, are those of the one-sample-per-step march.
 template <int RPT>
 __device__ inline void march_rays(__amdgpu_buffer_rsrc_t sdf, int W, int H, const float *cx, const float *cy,
                                   float dx, float dy, float t0, float t1, const bool *active, int *code)
 {
+    const int tiles = sdf_tiles_x(W);
     float t[RPT];
     bool alive[RPT];
+    bool any = false;
 #pragma unroll
     for (int j = 0; j < RPT; ++j) {
         t[j] = t0;
         alive[j] = active[j] && t0 < t1;
-        code[j] = kRayT1;
+        any = any || alive[j];
     }
-    for (;;) {
-        int q[RPT];
-        float d[RPT];
+    while (any) {
+        uint32_t c[RPT];
+        bool ins[RPT];
 #pragma unroll
         for (int j = 0; j < RPT; ++j) {
             // positions are finite, so the integer test equals 0 <= x < W, 0 <= y < H
             const int ix = floor_i32(fmaf(t[j], dx, cx[j]));
             const int iy = floor_i32(fmaf(t[j], dy, cy[j]));
-            const bool inside = (uint32_t)ix < (uint32_t)W && (uint32_t)iy < (uint32_t)H;
-            if (alive[j] && !inside) code[j] = kRayExit;
-            alive[j] = alive[j] && inside;
-            q[j] = pixel_index(ix, iy, W);
-            d[j] = sdf_load(sdf, ix, iy, sdf_tiles_x(W));
+            ins[j] = (uint32_t)ix < (uint32_t)W && (uint32_t)iy < (uint32_t)H;
+            c[j] = sdf_code_load(sdf, ix, iy, tiles);
         }
-        bool any = false;
+        any = false;
 #pragma unroll
         for (int j = 0; j < RPT; ++j) {
-            if (alive[j]) {
-                if (d[j] < 0.5f) {
-                    code[j] = q[j];
-                    alive[j] = false;
-                } else {
-                    t[j] = t[j] + march_step(d[j]);
-                    alive[j] = t[j] < t1;
-                }
-            }
+            const float tn = t[j] + step_of_code(c[j]);
+            if (alive[j] && ins[j]) t[j] = tn;
+            alive[j] = alive[j] && ins[j] && c[j] != 0 && tn < t1;
             any = any || alive[j];
         }
-        if (!any) break;
+    }
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+        code[j] = kRayT1;
+        if (active[j] && t[j] < t1) {
+            const int ix = floor_i32(fmaf(t[j], dx, cx[j]));
+            const int iy = floor_i32(fmaf(t[j], dy, cy[j]));
+            code[j] = ((uint32_t)ix < (uint32_t)W && (uint32_t)iy < (uint32_t)H) ? pixel_index(ix, iy, W) : kRayExit;
+        }
     }
 }
 
