@@ -131,37 +131,80 @@ __device__ __forceinline__ void level_body(const LevelArgs &a, int lb)
     // is a power of two, and probe rows and columns are below 2^24
     const int log2n = 31 - __builtin_clz(a.N);
     const int ulx = (px0 - 1) >> 1, uly = (py0 - 1) >> 1;
-    if (UPPER) {
-        const typename TT::T *up = (const typename TT::T *)a.upper;
-        for (int i = threadIdx.x; i < UCX * UCY * KD; i += kBlock) {
-            const int j = i % KD, sx = (i / KD) % UCX, sy = i / (KD * UCX);
-            const int ux = min(max(ulx + sx, 0), a.UX - 1), uy = min(max(uly + sy, 0), a.UY - 1);
-            const uint32_t ui = ((__umul24((uint32_t)uy, (uint32_t)a.UX) + (uint32_t)ux) << log2n) + (uint32_t)(k0 + j);
-            su[i] = TT::unpack(up[ui]);
-        }
-        __syncthreads();
-    }
-
-    const float2 dir = a.dirs[k];
-    float cx[RPT], cy[RPT];
+    const uint32_t plane = __umul24((uint32_t)a.W, (uint32_t)a.H);
     bool active[RPT];
     int hitq[RPT];
 #pragma unroll
-    for (int j = 0; j < RPT; ++j) {
-        const int py = pyb + j * TY;
-        cx[j] = ((float)px + 0.5f) * a.spacing;
-        cy[j] = ((float)py + 0.5f) * a.spacing;
-        active[j] = needed[j];
-    }
+    for (int j = 0; j < RPT; ++j) active[j] = needed[j];
+
+    // The upper level's neighbourhood for this direction slice, staged in
+    // LDS: UN texels per thread, loaded to registers first.
+    constexpr int UTOT = UCX * UCY * KD, UN = UPPER ? (UTOT + kBlock - 1) / kBlock : 0;
+    typename TT::T ureg[UN > 0 ? UN : 1];
+    auto stage_loads = [&]() {
+        const typename TT::T *up = (const typename TT::T *)a.upper;
+#pragma unroll
+        for (int n = 0; n < UN; ++n) {
+            const int i = (int)threadIdx.x + n * kBlock;
+            if (i < UTOT) {
+                const int j = i % KD, sx = (i / KD) % UCX, sy = i / (KD * UCX);
+                const int ux = min(max(ulx + sx, 0), a.UX - 1), uy = min(max(uly + sy, 0), a.UY - 1);
+                ureg[n] = up[((__umul24((uint32_t)uy, (uint32_t)a.UX) + (uint32_t)ux) << log2n) + (uint32_t)(k0 + j)];
+            }
+        }
+    };
+    auto stage_stores = [&]() {
+#pragma unroll
+        for (int n = 0; n < UN; ++n) {
+            const int i = (int)threadIdx.x + n * kBlock;
+            if (i < UTOT) su[i] = TT::unpack(ureg[n]);
+        }
+        __syncthreads();
+    };
+    // the emission of each ray's hit pixel (rgb; zero for a miss)
+    float3 ev[RPT];
+    auto gather_hits = [&]() {
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            ev[j] = make_float3(0.f, 0.f, 0.f);
+            if (active[j] && hitq[j] >= 0) {
+                const uint32_t q = (uint32_t)hitq[j];
+                if (a.emis4) {
+                    const float4 e = a.emis4[q];
+                    ev[j] = make_float3(e.x, e.y, e.z);
+                } else {
+                    ev[j] = make_float3(a.emissive[q], a.emissive[plane + q], a.emissive[2 * plane + q]);
+                }
+            }
+        }
+    };
+
     if (FROM_CODES) {
-        // the rays were marched by k_march_all; read their codes (4 per quad)
+        // The rays were marched by k_march_all.  Their codes, the staging
+        // loads and the hits' emission gathers are all issued before the
+        // barrier, so the three round trips overlap.
 #pragma unroll
         for (int j = 0; j < RPT; ++j) {
             const int py = min(pyb + j * TY, a.PY - 1);
             hitq[j] = a.codes[((__umul24((uint32_t)py, (uint32_t)a.PX) + (uint32_t)min(px, a.PX - 1)) << log2n) +
                               (uint32_t)k];
         }
+        if (UPPER) stage_loads();
+        gather_hits();
+        if (UPPER) stage_stores();
     } else {
+        if (UPPER) {
+            stage_loads();
+            stage_stores();
+        }
+        const float2 dir = a.dirs[k];
+        float cx[RPT], cy[RPT];
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            const int py = pyb + j * TY;
+            cx[j] = ((float)px + 0.5f) * a.spacing;
+            cy[j] = ((float)py + 0.5f) * a.spacing;
+        }
         march_rays<RPT>(sdf_rsrc(a.sdf, a.W, a.H), a.W, a.H, cx, cy, dir.x, dir.y, a.t0, a.t1, active, hitq);
         if (a.dbg_codes) {
 #pragma unroll
@@ -172,26 +215,18 @@ __device__ __forceinline__ void level_body(const LevelArgs &a, int lb)
                         hitq[j];
             }
         }
+        gather_hits();
     }
 
-    const uint32_t plane = __umul24((uint32_t)a.W, (uint32_t)a.H);
 #pragma unroll
     for (int j = 0; j < RPT; ++j) {
         const int py = pyb + j * TY;
         float r = 0.f, g = 0.f, b = 0.f, al = 0.f;
         if (active[j]) {
             if (hitq[j] >= 0) {
-                const uint32_t q = (uint32_t)hitq[j];
-                if (a.emis4) {
-                    const float4 e = a.emis4[q];
-                    r = e.x;
-                    g = e.y;
-                    b = e.z;
-                } else {
-                    r = a.emissive[q];
-                    g = a.emissive[plane + q];
-                    b = a.emissive[2 * plane + q];
-                }
+                r = ev[j].x;
+                g = ev[j].y;
+                b = ev[j].z;
             } else {
                 al = 1.0f;
                 if (UPPER) {
