@@ -278,6 +278,7 @@ def apply_tuning(stage, tun):
             stage.set_tiling(int(lvl), shape, rpt)
     stage.set_march(*tun["march"])
     stage.set_jfa_fuse(tun["jfa_fuse"])
+    stage.set_jfa_rows(tun.get("jfa_rows", 1))
     for lvl, order in tun.get("march_order", {}).items():
         if int(lvl) < stage.levels:
             stage.set_march_order(int(lvl), order)
@@ -519,7 +520,7 @@ def main():
     if args.save_tuning:
         inv = {v: k for k, v in rc._abi.TILE_SHAPES.items()}
         json.dump({"tiling": {str(l): [inv[tuple(v[0])], v[1]] for l, v in tiling.items()},
-                   "march": list(stage.march()), "jfa_fuse": stage.jfa_fuse(),
+                   "march": list(stage.march()), "jfa_fuse": stage.jfa_fuse(), "jfa_rows": stage.jfa_rows(),
                    "march_order": {str(k): v for k, v in stage.march_order().items()}}, open(args.save_tuning, "w"))
     for _ in range(args.warmup):
         stage.run_device(depth, emis, irr, stream)
@@ -545,6 +546,7 @@ def main():
     split, spec, rpl = stage.march()
     prof.set_march(split, spec, rpl)
     prof.set_jfa_fuse(stage.jfa_fuse())
+    prof.set_jfa_rows(stage.jfa_rows())
     for lvl, order in stage.march_order().items():
         prof.set_march_order(lvl, order)
     for _ in range(5):
@@ -606,7 +608,7 @@ def main():
                    "tiling": {str(k): f"{v[0][0]}x{v[0][1]}x{v[0][2]} r{v[1]}" for k, v in tiling.items()},
                    "march": {"split": split, "speculation": spec, "rays_per_lane": rpl,
                              "order": {str(k): v for k, v in stage.march_order().items()}},
-                   "jfa_fuse": stage.jfa_fuse()},
+                   "jfa_fuse": stage.jfa_fuse(), "jfa_rows": stage.jfa_rows()},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None if traffic is None else traffic / fam_launches[dominant],

@@ -189,6 +189,11 @@ rc_status rc_autotune(rc_ctx *ctx, const rc_gbuffer *dev, float *irradiance_dev,
  * identical; rc_autotune also picks this. */
 rc_status rc_set_jfa_fuse(rc_ctx *ctx, uint32_t fused_step);
 uint32_t rc_get_jfa_fuse(const rc_ctx *ctx);
+/* Image rows per wave of a global jump-flood pass: 1, 2 or 4 (no counterpart
+ * in the reference).  More rows share a wave's row setup and keep more loads
+ * in flight per lane; the result is identical and rc_autotune picks it. */
+rc_status rc_set_jfa_rows(rc_ctx *ctx, uint32_t rows);
+uint32_t rc_get_jfa_rows(const rc_ctx *ctx);
 /* split = 1: every level >= 1 is marched by one launch (top level first, so
  * the longest rays start earliest and the levels' dependent chains overlap),
  * then the levels are merged top-down from the stored ray codes; split = 2:

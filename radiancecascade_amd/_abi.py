@@ -129,6 +129,8 @@ SIGNATURES = {
     "rc_autotune": (_i32, [_P, ctypes.POINTER(GBuffer), _P, _u32, _P]),
     "rc_set_jfa_fuse": (_i32, [_P, _u32]),
     "rc_get_jfa_fuse": (_u32, [_P]),
+    "rc_set_jfa_rows": (_i32, [_P, _u32]),
+    "rc_get_jfa_rows": (_u32, [_P]),
     "rc_set_march": (_i32, [_P, _u32, _u32, _u32]),
     "rc_set_march_order": (_i32, [_P, _u32, _u32]),
     "rc_get_march_order": (_i32, [_P, _u32, ctypes.POINTER(_u32)]),

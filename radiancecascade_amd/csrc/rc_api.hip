@@ -45,6 +45,7 @@ struct rc_ctx {
     // above): its own tiling, -1 until tuned (then the plain level 1's)
     int surr_tile1 = -1, surr_rpt1 = -1;
     int jfa_fuse = 0;  // largest jump-flood step fused into the LDS tail (0 = none)
+    int jfa_rows = 1;  // image rows per wave of a global jump-flood pass
     int mmap[RC_MAX_LEVELS] = {};  // split march: wave-tile order of level i's rays (0 = canonical)
     // split march: levels >= 1 marched by one k_march_all launch into `codes`,
     // then merged top-down; otherwise each level kernel marches its own rays
@@ -404,7 +405,7 @@ static rc_status seeds_and_jfa(rc_ctx *c, const float *depth, const float *emiss
     for (int step = P > 0 ? 1 << (P - 1) : 0; step >= 1 && step > tail; step >>= 1) {
         const bool final_pass = tail == 0 && step == 1;
         uint32_t *dst = final_pass ? (dbg ? c->seed[cur ^ 1] : nullptr) : c->seed[cur ^ 1];
-        HIP_TRY(c, launch_jfa_pass(c->seed[cur], dst, c->sdf, c->W, c->H, step, final_pass, s, gate));
+        HIP_TRY(c, launch_jfa_pass(c->seed[cur], dst, c->sdf, c->W, c->H, step, final_pass, s, gate, c->jfa_rows));
         if (final_pass) c->dbg_seed_buf = dbg ? (cur ^ 1) : -1;
         else cur ^= 1;
         ++passes;
@@ -1139,6 +1140,17 @@ rc_status rc_set_jfa_fuse(rc_ctx *c, uint32_t fused_step)
 
 uint32_t rc_get_jfa_fuse(const rc_ctx *c) { return c ? (uint32_t)c->jfa_fuse : 0u; }
 
+rc_status rc_set_jfa_rows(rc_ctx *c, uint32_t rows)
+{
+    if (!c) return fail(nullptr, RC_ERR_NULL, "rc_set_jfa_rows: ctx is NULL");
+    if (rows != 1 && rows != 2 && rows != 4) return fail(c, RC_ERR_INVALID, "rc_set_jfa_rows: rows must be 1, 2 or 4");
+    c->jfa_rows = (int)rows;
+    drop_graph(c);
+    return RC_OK;
+}
+
+uint32_t rc_get_jfa_rows(const rc_ctx *c) { return c ? (uint32_t)c->jfa_rows : 0u; }
+
 rc_status rc_set_march(rc_ctx *c, uint32_t split, uint32_t speculation, uint32_t rays_per_lane)
 {
     if (!c) return fail(nullptr, RC_ERR_NULL, "rc_set_march: ctx is NULL");
@@ -1208,22 +1220,25 @@ rc_status rc_autotune(rc_ctx *c, const rc_gbuffer *g, float *irr, uint32_t iters
     hipEvent_t e0 = c->ev[0], e1 = c->ev[1];
     {
         float best = 1e30f;
-        int best_fuse = c->jfa_fuse;
-        const int saved = c->jfa_fuse;
+        int best_fuse = c->jfa_fuse, best_rows = c->jfa_rows;
+        const int saved = c->jfa_fuse, saved_rows = c->jfa_rows;
+        for (int rows : {1, 2, 4})
         for (int fuse : {0, 2, 4, 8}) {
             c->jfa_fuse = fuse;
+            c->jfa_rows = rows;
             HIP_TRY(c, hipEventRecord(e0, s));
             for (uint32_t it = 0; it < iters; ++it) {
                 rc_status r = run_frame(c, g->depth, g->emissive, irr, s, true);
-                if (r != RC_OK) { c->jfa_fuse = saved; return r; }
+                if (r != RC_OK) { c->jfa_fuse = saved; c->jfa_rows = saved_rows; return r; }
             }
             HIP_TRY(c, hipEventRecord(e1, s));
             HIP_TRY(c, hipEventSynchronize(e1));
             float ms = 0.f;
             HIP_TRY(c, hipEventElapsedTime(&ms, e0, e1));
-            if (ms < best) { best = ms; best_fuse = fuse; }
+            if (ms < best) { best = ms; best_fuse = fuse; best_rows = rows; }
         }
         c->jfa_fuse = best_fuse;
+        c->jfa_rows = best_rows;
     }
     // A frame with one level marches nothing beyond level 0: there is no
     // march mode to tune (and no split march to run).

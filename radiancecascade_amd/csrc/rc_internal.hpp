@@ -33,8 +33,9 @@ hipError_t launch_sdf_untile(const uint8_t *sdf, float *out, int W, int H, hipSt
 // learned frame's tile fallback runs them only when some tile is flagged).
 hipError_t launch_seed_init(const float *depth, const DepthConv &dc, const float *emissive, uint32_t *seed,
                             float4 *emis4, int W, int H, hipStream_t s, const int *gate = nullptr);
+// rows: image rows per wave (1, 2 or 4; the result does not depend on it)
 hipError_t launch_jfa_pass(const uint32_t *in, uint32_t *out, uint8_t *sdf, int W, int H, int step, bool final_pass,
-                           hipStream_t s, const int *gate = nullptr);
+                           hipStream_t s, const int *gate = nullptr, int rows = 1);
 hipError_t launch_jfa_tail(const uint32_t *in, uint32_t *seeds_out, uint8_t *sdf, int W, int H, int tail_step,
                            hipStream_t s, const int *gate = nullptr);
 

@@ -144,70 +144,83 @@ __global__ __launch_bounds__(kBlock) void k_seed_batch(const SeedJob *__restrict
     seed_init_body(J.depth, J.dc, J.emissive, J.seed, J.emis4, J.W, J.H, J.bx_count, b - begins[j]);
 }
 
-// One global pass.  A wave owns 64 consecutive pixels of one row (blocks of
-// 64 x 4); the row index is made wave-uniform so the three source rows are
-// scalar base pointers and each of the 9 loads is a 32-bit lane offset.
-// FINAL: write the SDF (and, when `out` is set, the external seed field)
-// instead of the internal seed field.
-template <bool FINAL>
+// One global pass.  A wave owns 64 consecutive pixels of R consecutive rows
+// (blocks of 64 x 4R): the row setup is shared and every lane keeps 9R loads
+// in flight.  The row index is wave-uniform, so the source rows are scalar
+// base offsets and each load is a 32-bit lane offset.  FINAL: write the SDF
+// (and, when `out` is set, the external seed field) instead of the internal
+// seed field.  Every R computes the same field.
+template <bool FINAL, int R>
 __device__ __forceinline__ void jfa_pass_body(const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
                                               uint8_t *__restrict__ sdf, int W, int H, int step, int bx_count, int b)
 {
     const int x = (b % bx_count) * 64 + (threadIdx.x & 63);
-    const int y = __builtin_amdgcn_readfirstlane((b / bx_count) * 4 + (threadIdx.x >> 6));
-    if (y >= H) return;
+    const int y0 = __builtin_amdgcn_readfirstlane((b / bx_count) * (4 * R) + (int)(threadIdx.x >> 6) * R);
+    if (y0 >= H) return;
     const bool xin = x < W;
     const int xc = min(x, W - 1);
-    const uint32_t pix = ((uint32_t)y << 16) | (uint32_t)xc;
     const int xl = x - step, xr = x + step;
     // unsigned 32-bit lane offsets let the loads use the scalar-base form
     const uint32_t ol = (uint32_t)max(xl, 0), oc = (uint32_t)xc, orr = (uint32_t)min(xr, W - 1);
+    const bool lf = xl >= 0, rt = xr < W;
 
-    // 9 unconditional buffer loads at clamped coordinates, all in flight
+    // 9R unconditional buffer loads at clamped coordinates, all in flight
     // together: the row offset is a scalar, the column offset a lane VGPR
     const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<uint32_t *>(in), (short)0, W * H * 4, kBufferDword3);
-    uint32_t cand[9];
+    uint32_t cand[R][9];
 #pragma unroll
-    for (int r = 0; r < 3; ++r) {
-        const int qy = min(max(y + (r - 1) * step, 0), H - 1);
-        const int rowb = qy * W * 4;
-        cand[3 * r + 0] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, ol * 4, rowb, 0);
-        cand[3 * r + 1] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, oc * 4, rowb, 0);
-        cand[3 * r + 2] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, orr * 4, rowb, 0);
-    }
-    const bool up = y - step >= 0, dn = y + step < H, lf = xl >= 0, rt = xr < W;
-    const bool valid[9] = {up && lf, up, up && rt, lf, true, rt, dn && lf, dn, dn && rt};
-    // An out-of-bounds neighbour (read at a clamped coordinate) enters with
-    // 2^31 added to its d^2: every in-bounds candidate has d^2 < 2^31 (the
-    // far seed's is at most 2 * 32767^2), so it never wins, and no sum wraps.
-    uint64_t best = ((uint64_t)seed_d2(cand[4], pix) << 32) | cand[4];
+    for (int r = 0; r < R; ++r) {
+        const int y = min(y0 + r, H - 1);
 #pragma unroll
-    for (int j = 0; j < 9; ++j) {
-        if (j == 4) continue;
-        const short2v d = __builtin_bit_cast(short2v, cand[j]) - __builtin_bit_cast(short2v, pix);
-        const uint32_t d2 = (uint32_t)__builtin_amdgcn_sdot2(d, d, valid[j] ? 0 : INT32_MIN, false);
-        const uint64_t key = ((uint64_t)d2 << 32) | cand[j];
-        best = key < best ? key : best;
+        for (int k = 0; k < 3; ++k) {
+            const int qy = min(max(y + (k - 1) * step, 0), H - 1);
+            const int rowb = qy * W * 4;
+            cand[r][3 * k + 0] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, ol * 4, rowb, 0);
+            cand[r][3 * k + 1] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, oc * 4, rowb, 0);
+            cand[r][3 * k + 2] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, orr * 4, rowb, 0);
+        }
     }
-    if (!xin) return;
-    const uint32_t bs = (uint32_t)best, bd2 = (uint32_t)(best >> 32);
-    const size_t p = (size_t)y * W + x;
-    if (FINAL) {
-        sdf[sdf_index(x, y, sdf_tiles_x(W))] = seed_sdf_code(bs, bd2);
-        if (out) out[p] = bs == kFarSeed ? kSeedNone : bs;
-    } else {
-        out[p] = bs;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int y = y0 + r;
+        if (y >= H) break;
+        const uint32_t pix = ((uint32_t)y << 16) | (uint32_t)xc;
+        const bool up = y - step >= 0, dn = y + step < H;
+        const bool valid[9] = {up && lf, up, up && rt, lf, true, rt, dn && lf, dn, dn && rt};
+        // An out-of-bounds neighbour (read at a clamped coordinate) enters
+        // with 2^31 added to its d^2: every in-bounds candidate has d^2 < 2^31
+        // (the far seed's is at most 2 * 32767^2), so it never wins, and no
+        // sum wraps.
+        uint64_t best = ((uint64_t)seed_d2(cand[r][4], pix) << 32) | cand[r][4];
+#pragma unroll
+        for (int j = 0; j < 9; ++j) {
+            if (j == 4) continue;
+            const short2v d = __builtin_bit_cast(short2v, cand[r][j]) - __builtin_bit_cast(short2v, pix);
+            const uint32_t d2 = (uint32_t)__builtin_amdgcn_sdot2(d, d, valid[j] ? 0 : INT32_MIN, false);
+            const uint64_t key = ((uint64_t)d2 << 32) | cand[r][j];
+            best = key < best ? key : best;
+        }
+        if (xin) {
+            const uint32_t bs = (uint32_t)best, bd2 = (uint32_t)(best >> 32);
+            const size_t p = (size_t)y * W + x;
+            if (FINAL) {
+                sdf[sdf_index(x, y, sdf_tiles_x(W))] = seed_sdf_code(bs, bd2);
+                if (out) out[p] = bs == kFarSeed ? kSeedNone : bs;
+            } else {
+                out[p] = bs;
+            }
+        }
     }
 }
 
-template <bool FINAL>
+template <bool FINAL, int R>
 __global__ __launch_bounds__(kBlock) void k_jfa_pass(const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
                                                      uint8_t *__restrict__ sdf, int W, int H, int step, int bx_count,
                                                      const int *gate)
 {
     if (gate && *gate == 0) return;
-    jfa_pass_body<FINAL>(in, out, sdf, W, H, step, bx_count, xcd_band(blockIdx.x, gridDim.x));
+    jfa_pass_body<FINAL, R>(in, out, sdf, W, H, step, bx_count, xcd_band(blockIdx.x, gridDim.x));
 }
 
 // One pass over a work list of views; each job has its own step (a view
@@ -219,8 +232,8 @@ __global__ __launch_bounds__(kBlock) void k_jfa_batch(const JfaJob *__restrict__
     const int b = xcd_band(blockIdx.x, gridDim.x);
     const int j = batch_job(begins, njobs, b);
     const JfaJob &J = jobs[j];
-    if (J.final_pass) jfa_pass_body<true>(J.in, J.out, J.sdf, J.W, J.H, J.step, J.bx_count, b - begins[j]);
-    else jfa_pass_body<false>(J.in, J.out, J.sdf, J.W, J.H, J.step, J.bx_count, b - begins[j]);
+    if (J.final_pass) jfa_pass_body<true, 1>(J.in, J.out, J.sdf, J.W, J.H, J.step, J.bx_count, b - begins[j]);
+    else jfa_pass_body<false, 1>(J.in, J.out, J.sdf, J.W, J.H, J.step, J.bx_count, b - begins[j]);
 }
 
 // Fused passes S, S/2, ..., 1 over a T x T output tile, loaded with a halo of
@@ -376,14 +389,25 @@ hipError_t launch_seed_init(const float *depth, const DepthConv &dc, const float
     return hipGetLastError();
 }
 
-hipError_t launch_jfa_pass(const uint32_t *in, uint32_t *out, uint8_t *sdf, int W, int H, int step, bool final_pass,
-                           hipStream_t s, const int *gate)
+template <bool FINAL>
+static void launch_pass_rows(const uint32_t *in, uint32_t *out, uint8_t *sdf, int W, int H, int step, int rows,
+                             hipStream_t s, const int *gate)
 {
-    const int bxc = (W + 63) / 64, byc = (H + 3) / 4;
-    if (final_pass)
-        hipLaunchKernelGGL(k_jfa_pass<true>, dim3(bxc * byc), dim3(kBlock), 0, s, in, out, sdf, W, H, step, bxc, gate);
-    else
-        hipLaunchKernelGGL(k_jfa_pass<false>, dim3(bxc * byc), dim3(kBlock), 0, s, in, out, sdf, W, H, step, bxc, gate);
+    const int bxc = (W + 63) / 64, byc = (H + 4 * rows - 1) / (4 * rows);
+    const dim3 grid(bxc * byc), block(kBlock);
+    switch (rows) {
+    case 2: hipLaunchKernelGGL((k_jfa_pass<FINAL, 2>), grid, block, 0, s, in, out, sdf, W, H, step, bxc, gate); break;
+    case 4: hipLaunchKernelGGL((k_jfa_pass<FINAL, 4>), grid, block, 0, s, in, out, sdf, W, H, step, bxc, gate); break;
+    default: hipLaunchKernelGGL((k_jfa_pass<FINAL, 1>), grid, block, 0, s, in, out, sdf, W, H, step, bxc, gate); break;
+    }
+}
+
+hipError_t launch_jfa_pass(const uint32_t *in, uint32_t *out, uint8_t *sdf, int W, int H, int step, bool final_pass,
+                           hipStream_t s, const int *gate, int rows)
+{
+    if (rows != 1 && rows != 2 && rows != 4) return hipErrorInvalidValue;
+    if (final_pass) launch_pass_rows<true>(in, out, sdf, W, H, step, rows, s, gate);
+    else launch_pass_rows<false>(in, out, sdf, W, H, step, rows, s, gate);
     return hipGetLastError();
 }
 

@@ -164,6 +164,13 @@ class RadianceCascadeStage:
     def jfa_fuse(self) -> int:
         return self._lib.rc_get_jfa_fuse(self._ctx)
 
+    def set_jfa_rows(self, rows: int) -> None:
+        """Image rows per wave of a global jump-flood pass (1, 2 or 4)."""
+        check(self._lib.rc_set_jfa_rows(self._ctx, rows), self._ctx)
+
+    def jfa_rows(self) -> int:
+        return self._lib.rc_get_jfa_rows(self._ctx)
+
     def tiling(self) -> dict:
         """{level: ((TX, TY, KD), rays_per_lane)} for levels >= 1."""
         out = {}

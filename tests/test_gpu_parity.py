@@ -291,10 +291,11 @@ def test_graph_replay_matches_direct(rc, oracle):
             assert torch.equal(a, b), f"replay {rep}"
 
 
-@pytest.mark.parametrize("W,H", [(300, 200), (1920, 1080), (65, 7)])
+@pytest.mark.parametrize("W,H", [(300, 200), (1920, 1080), (65, 7), (33, 5)])
 def test_every_jfa_fusion_is_bit_exact(rc, oracle, W, H):
-    """Global passes only, or the last passes fused in LDS: same jump-flood
-    field and distance field."""
+    """Global passes only, or the last passes fused in LDS, with 1, 2 or 4
+    image rows per wave of a global pass (ragged last row blocks included):
+    same jump-flood field and distance field."""
     _say(f"jfa fusion W={W} H={H}")
     seed, nb, ne = 3, 32, 4
     depth, emis = _device_gbuffer(rc, W, H, seed, nb, ne)
@@ -306,16 +307,21 @@ def test_every_jfa_fusion_is_bit_exact(rc, oracle, W, H):
     sdf = torch.empty((H, W), dtype=torch.float32, device="cuda:0")
     L = oracle.levels_for_coverage(W, H)
     with rc.RadianceCascadeStage(W, H, levels=L, debug=True) as st:
-        for fuse in (0, 1, 2, 4, 8):
-            st.set_jfa_fuse(fuse)
-            st.run_device(depth, emis, irr)
-            st.debug_seeds(seeds)
-            st.debug_sdf(sdf)
-            torch.cuda.synchronize()
-            assert np.array_equal(seeds.cpu().numpy().view(np.uint32), seeds_ref), fuse
-            assert np.array_equal(sdf.cpu().numpy().view(np.uint32), sdf_ref.view(np.uint32)), fuse
+        for rows in (1, 2, 4):
+            st.set_jfa_rows(rows)
+            assert st.jfa_rows() == rows
+            for fuse in (0, 1, 2, 4, 8):
+                st.set_jfa_fuse(fuse)
+                st.run_device(depth, emis, irr)
+                st.debug_seeds(seeds)
+                st.debug_sdf(sdf)
+                torch.cuda.synchronize()
+                assert np.array_equal(seeds.cpu().numpy().view(np.uint32), seeds_ref), (rows, fuse)
+                assert np.array_equal(sdf.cpu().numpy().view(np.uint32), sdf_ref.view(np.uint32)), (rows, fuse)
         with pytest.raises(rc.RcError):
             st.set_jfa_fuse(3)
+        with pytest.raises(rc.RcError):
+            st.set_jfa_rows(3)
 
 
 @pytest.mark.parametrize("W,H,L,seed", [(1920, 1080, 6, 0), (512, 384, 5, 1), (97, 61, 6, 3)])
