@@ -48,14 +48,15 @@ def texel_bytes_contract(storage: str) -> int:
 FAMILIES = {"seed": ["seed"], "jfa": ["jfa", "jfa_tail"], "march": ["march"], "levels": None, "level0": ["level0"]}
 
 
-def family_alg_bytes(W, H, L, storage, rays, split, jfa_passes):
+def family_alg_bytes(W, H, L, storage, rays, split, jfa_passes, top_in_march=False):
     """Algorithmic bytes per frame of each kernel family: the bytes its
     kernels must move, charged to the kernel that moves them.
       seed   : depth + emissive read (16 B/px), seed field written (4 B/px);
       jfa    : 8 B/px per pass (seed field read and written once); the last
                pass reads 4 B/px and writes the 1-byte distance field;
       march  : the 1-byte distance field read once, a 4-byte code per ray of
-               levels >= 1 written;
+               levels >= 1 written, and (top_in_march: the march merges the
+               top level itself) the top level's merged texels (b R_top / 4);
       levels : level i >= 1 writes its merged texels (b R_i / 4: the stored
                form is the 4-child mean) and reads level i+1's (b R_{i+1} / 4)
                and, after the split march, its 4-byte codes (4 R_i);
@@ -66,11 +67,13 @@ def family_alg_bytes(W, H, L, storage, rays, split, jfa_passes):
     n = W * H
     tex = [r // 4 for r in rays]
     levels = 0
-    for i in range(1, L):
+    last = L - 2 if top_in_march else L - 1
+    for i in range(1, last + 1):
         levels += b * tex[i] + (b * tex[i + 1] if i + 1 < L else 0) + (4 * rays[i] if split else 0)
+    march = (n + 4 * sum(rays[1:L]) + (b * tex[L - 1] if top_in_march else 0)) if split else 0
     return {"seed": 20 * n,
             "jfa": (8 * (jfa_passes - 1) + 5) * n if jfa_passes > 0 else 5 * n,
-            "march": (n + 4 * sum(rays[1:L])) if split else 0,
+            "march": march,
             "levels": levels,
             "level0": n + (b * tex[1] if L > 1 else 0) + 12 * n}
 
@@ -559,7 +562,9 @@ def main():
     st = prof.stats()
     rays = [rc.level_rays(W, H, i) for i in range(L)]
     per_frame_ms = {k: st.stage_ms[k] / st.runs for k in st.stage_ms}
-    fam_alg = family_alg_bytes(W, H, L, args.storage, rays, bool(split), (max(W, H) - 1).bit_length())  # P = ceil(log2 max(W, H))
+    top_in_march = bool(split) and L > 1 and st.stage_launches.get(f"level{L - 1}", 0) == 0
+    fam_alg = family_alg_bytes(W, H, L, args.storage, rays, bool(split), (max(W, H) - 1).bit_length(),  # P = ceil(log2 max(W, H))
+                               top_in_march)
     fam_ms, fam_launches = {}, {}
     for fam, stages in FAMILIES.items():
         stages = stages or [f"level{i}" for i in range(1, L)]
@@ -606,7 +611,7 @@ def main():
                    "storage": args.storage,
                    "rays_per_frame": int(sum(rays)),
                    "tiling": {str(k): f"{v[0][0]}x{v[0][1]}x{v[0][2]} r{v[1]}" for k, v in tiling.items()},
-                   "march": {"split": split, "speculation": spec, "rays_per_lane": rpl,
+                   "march": {"split": split, "speculation": spec, "rays_per_lane": rpl, "top_level_merged": top_in_march,
                              "order": {str(k): v for k, v in stage.march_order().items()}},
                    "jfa_fuse": stage.jfa_fuse(), "jfa_rows": stage.jfa_rows()},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,

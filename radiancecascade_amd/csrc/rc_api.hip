@@ -290,6 +290,14 @@ static LevelArgs level_args(const rc_ctx *c, int i, const float *emissive)
     return a;
 }
 
+// The split march merges the top level itself (no level kernel for it) when
+// it takes one ray per lane and one sample per step, and the top level's lane
+// order keeps each texel's 4 directions in one lane quad.
+static bool top_merged_in_march(const rc_ctx *c)
+{
+    return c->L > 1 && c->march_split == 1 && c->spec == 1 && c->march_m == 1 && c->mmap[c->L - 1] != 5;
+}
+
 static MarchAllArgs march_all_args(const rc_ctx *c)
 {
     MarchAllArgs m{};
@@ -310,6 +318,8 @@ static MarchAllArgs march_all_args(const rc_ctx *c)
         l.log2n = 2 + 2 * i;
         l.inv_px = 1.0f / (float)l.PX;
         march_level_map(l, c->march_split == 1 ? c->mmap[i] : 0);  // the persistent form is canonical
+        l.merged = (i == c->L - 1 && top_merged_in_march(c)) ? c->level[i] : nullptr;
+        l.storage = (int)c->storage;
         l.block_begin = blocks;
         m.level_begin[m.nlev - 1] = blocks;
         const int64_t rays = (int64_t)l.PX * l.PY * l.N, per = (int64_t)kBlock * c->march_m;
@@ -321,6 +331,7 @@ static MarchAllArgs march_all_args(const rc_ctx *c)
     }
     m.codes_base = c->codes;
     m.dirs_base = c->dirs;
+    m.emis4 = c->emis4;
     m.n_packets = (m.total_rays + 63) / 64;
     // resident waves: 8 workgroups of 4 waves per CU (the march's occupancy cap)
     const uint32_t cap = (uint32_t)(c->num_cu > 0 ? c->num_cu : 256) * 32u;
@@ -440,7 +451,8 @@ static rc_status run_frame(rc_ctx *c, const float *depth, const float *emissive,
         else HIP_TRY(c, launch_march_all(march_all_args(c), c->spec, c->march_m, s));
         tm.end(RC_STAGE_MARCH);
     }
-    for (int i = c->L - 1; i >= 1; --i) {
+    const int top = split && top_merged_in_march(c) ? c->L - 2 : c->L - 1;  // the march merged level L-1
+    for (int i = top; i >= 1; --i) {
         tm.begin(RC_STAGE_LEVEL0 + i);
         LevelArgs a = level_args(c, i, emissive);
         a.emis4 = c->emis4;
@@ -450,6 +462,7 @@ static rc_status run_frame(rc_ctx *c, const float *depth, const float *emissive,
         tm.end(RC_STAGE_LEVEL0 + i);
         c->code_src[i] = split ? 1 : (dbg ? 2 : 0);
     }
+    if (top < c->L - 1) c->code_src[c->L - 1] = 1;  // the march stored the top level's codes too
     tm.begin(RC_STAGE_LEVEL0);
     LevelArgs a0 = level_args(c, 0, emissive);
     if (dbg) a0.dbg_codes = c->dbg_codes + c->dbg_code_off[0];

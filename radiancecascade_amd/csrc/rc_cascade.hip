@@ -552,8 +552,39 @@ __device__ inline bool decode_slot(const MarchLevel &L, int slot, int &px, int &
 // same loop iteration (march lengths differ by two orders of magnitude, so
 // one-ray-per-lane waves idle most lanes while their longest ray runs).
 // Blocks go round-robin to the XCDs, which interleaves heavy and light tiles.
+// A_top texel of a lane quad, merged in the march: the level kernel's
+// arithmetic for a level without an upper level (quad sums of the 4 rays,
+// times 1/4), stored by the quad's first lane.
+__device__ inline void merge_top_quad(const MarchLevel &L, const float4 *__restrict__ emis4, bool valid, int code,
+                                      uint32_t at)
+{
+    float r = 0.f, g = 0.f, b = 0.f, al = 0.f;
+    if (valid) {
+        if (code >= 0) {
+            const float4 e = emis4[code];
+            r = e.x;
+            g = e.y;
+            b = e.z;
+        } else {
+            al = 1.0f;
+        }
+    }
+    r = quad_sum(r) * 0.25f;
+    g = quad_sum(g) * 0.25f;
+    b = quad_sum(b) * 0.25f;
+    al = quad_sum(al) * 0.25f;
+    if (valid && (at & 3) == 0) {
+        const float4 v = make_float4(r, g, b, al);
+        const uint32_t i = at >> 2;
+        if (L.storage == (int)Storage::F32) ((float4 *)L.merged)[i] = v;
+        else if (L.storage == (int)Storage::F16) ((uint2 *)L.merged)[i] = Texel<Storage::F16>::pack(v);
+        else ((uint2 *)L.merged)[i] = Texel<Storage::BF16>::pack(v);
+    }
+}
+
 template <int SPEC, int M>
-__device__ __forceinline__ void march_all_body(const uint8_t *sdf_codes, int W, int H, const MarchLevel &L, int lb)
+__device__ __forceinline__ void march_all_body(const uint8_t *sdf_codes, int W, int H, const MarchLevel &L, int lb,
+                                               const float4 *__restrict__ emis4)
 {
     const __amdgpu_buffer_rsrc_t sdf = sdf_rsrc(sdf_codes, W, H);
     const int base = lb * kBlock * M + (int)threadIdx.x;
@@ -602,6 +633,7 @@ __device__ __forceinline__ void march_all_body(const uint8_t *sdf_codes, int W, 
             code = ((uint32_t)ix < (uint32_t)W && (uint32_t)iy < (uint32_t)H) ? pixel_index(ix, iy, W) : kRayExit;
         }
         if (valid) L.codes[at] = code;
+        if (L.merged) merge_top_quad(L, emis4, valid, code, at);
         return;
     }
     start();
@@ -624,7 +656,7 @@ __global__ __launch_bounds__(kBlock) void k_march_all(MarchAllArgs a)
     int lv = 0;
 #pragma unroll
     for (int j = 1; j < RC_MAX_LEVELS; ++j) lv += (j < a.nlev && (int)blockIdx.x >= a.level_begin[j]) ? 1 : 0;
-    march_all_body<SPEC, M>(a.sdf, a.W, a.H, a.lev[lv], (int)blockIdx.x - a.lev[lv].block_begin);
+    march_all_body<SPEC, M>(a.sdf, a.W, a.H, a.lev[lv], (int)blockIdx.x - a.lev[lv].block_begin, a.emis4);
 }
 
 // The split march over a work list of (view, level) jobs, blocks in launch
@@ -636,7 +668,7 @@ __global__ __launch_bounds__(kBlock) void k_march_batch(const MarchJob *__restri
     const int b = (int)blockIdx.x;
     const int j = batch_job(begins, njobs, b);
     const MarchJob J = jobs[j];
-    march_all_body<SPEC, M>(J.sdf, J.W, J.H, J.L, b - begins[j]);
+    march_all_body<SPEC, M>(J.sdf, J.W, J.H, J.L, b - begins[j], nullptr);
 }
 
 // Persistent form of the split march.  March lengths differ by two orders of

@@ -163,6 +163,14 @@ struct MarchLevel {
     int map, tx_log, ty_log, kd_log, ntx;
     float inv_ntx;
     uint32_t slots;
+    // The top level (no upper level) merged by the march itself: each lane
+    // quad holds the 4 child directions of one texel, so the march writes
+    // A_top = quad mean of (hit ? (E, 0) : (0, 1)) here, in `storage`
+    // (a Storage value), and no level kernel runs for it.  nullptr: not merged.
+    // Needs one ray per lane, one sample per step and a lane order whose
+    // direction slice is at least 4 (every order but 5).
+    void *merged;
+    int storage;
 };
 // Wave-tile orders of the split march: 0 canonical, 1 = 2x2 probes x 16
 // directions, 2 = 4x4 x 4, 3 = 4x2 x 8, 4 = 2x1 x 32, 5 = 8x8 x 1.
@@ -176,6 +184,7 @@ struct MarchAllArgs {
     int W, H, nlev;
     int level_begin[RC_MAX_LEVELS];  // lev[j].block_begin, contiguous (one scalar load)
     MarchLevel lev[RC_MAX_LEVELS];  // top level first
+    const float4 *emis4;     // interleaved emission (a merged level's hits)
     // persistent form: every ray of every level, in 64-ray packets
     int *codes_base;
     const float2 *dirs_base;
