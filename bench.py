@@ -444,6 +444,51 @@ def summarize_profile(trace_stats, fetch_csv, write_csv, copy_fetch_csv, copy_wr
     return "\n".join(lines)
 
 
+def summarize_counters(csv_paths, title):
+    """Markdown table of rocprofv3 counter-collection CSVs (one or more
+    --pmc passes of the same command): per kernel, the dispatches and the
+    mean per dispatch of every counter (summed over its dimension
+    instances), plus, where the counters are present, VALU and SALU
+    instructions per wave."""
+    per = {}
+    for path in csv_paths:
+        disp = {}
+        for r in _rows(path):
+            name = _col(r, "Kernel_Name")
+            key = (name, _col(r, "Counter_Name"))
+            d = _col(r, "Dispatch_Id")
+            disp.setdefault(key, {}).setdefault(d, 0.0)
+            disp[key][d] += float(_col(r, "Counter_Value"))
+        for (name, counter), by in disp.items():
+            per.setdefault(name, {})[counter] = (len(by), sum(by.values()) / len(by))
+    counters = sorted({c for v in per.values() for c in v})
+    lines = [f"Counters: {title}", "", "| kernel | dispatches | " + " | ".join(counters) +
+             " | VALU / wave | SALU / wave |", "|---" * (len(counters) + 4) + "|"]
+    for name, v in sorted(per.items(), key=lambda kv: kv[0]):
+        short = name.split("(")[0].replace("void ", "")
+        n = max(c[0] for c in v.values())
+        cells = [f"{v[c][1]:.4g}" if c in v else "-" for c in counters]
+        waves = v.get("SQ_WAVES", (0, 0.0))[1]
+        valu = f"{v['SQ_INSTS_VALU'][1] / waves:.1f}" if waves and "SQ_INSTS_VALU" in v else "-"
+        salu = f"{v['SQ_INSTS_SALU'][1] / waves:.1f}" if waves and "SQ_INSTS_SALU" in v else "-"
+        lines.append(f"| `{short}` | {n} | " + " | ".join(cells) + f" | {valu} | {salu} |")
+    return "\n".join(lines)
+
+
+def summarize_trace(stats_csv, title):
+    """Markdown table of a rocprofv3 kernel-trace stats CSV: calls, mean, min
+    and max duration and share of GPU time per kernel."""
+    lines = [f"Kernel trace: {title}", "", "| kernel | calls | avg us | min us | max us | % of GPU time |",
+             "|---|---|---|---|---|---|"]
+    for r in _rows(stats_csv):
+        name = _col(r, "Name", "Kernel_Name")
+        short = name if len(name) < 90 else name[:87] + "..."
+        lines.append(f"| `{short}` | {_col(r, 'Calls')} | {float(_col(r, 'AverageNs')) / 1e3:.2f} | "
+                     f"{float(_col(r, 'MinNs')) / 1e3:.2f} | {float(_col(r, 'MaxNs')) / 1e3:.2f} | "
+                     f"{float(_col(r, 'Percentage')):.2f} |")
+    return "\n".join(lines)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -474,6 +519,11 @@ def main():
                     metavar=("TRACE_STATS", "FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE", "WORKLOAD"),
                     help="print the markdown summary of rocprofv3 CSVs of this bench (kernel stats, FETCH_SIZE and "
                          "WRITE_SIZE runs of the bench and of --copy-calibration)")
+    ap.add_argument("--summarize-counters", nargs="+", default=None, metavar="CSV_OR_TITLE",
+                    help="print the per-kernel markdown table of rocprofv3 counter CSVs (the last argument is the "
+                         "title)")
+    ap.add_argument("--summarize-trace", nargs=2, default=None, metavar=("TRACE_STATS", "TITLE"),
+                    help="print the markdown table of a rocprofv3 kernel-trace stats CSV")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 256 MiB device buffer 10 times (the reference kernel for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters of rocprofv3 to bytes)")
@@ -481,6 +531,12 @@ def main():
     if args.summarize_profile:
         print(summarize_profile(*args.summarize_profile[:5], copy_bytes=4 * (64 << 20),
                                 workload=args.summarize_profile[5]))
+        return
+    if args.summarize_counters:
+        print(summarize_counters(args.summarize_counters[:-1], args.summarize_counters[-1]))
+        return
+    if args.summarize_trace:
+        print(summarize_trace(*args.summarize_trace))
         return
     if args.copy_calibration:
         import torch

@@ -304,7 +304,6 @@ static MarchAllArgs march_all_args(const rc_ctx *c)
     m.sdf = c->sdf;
     m.W = c->W;
     m.H = c->H;
-    int blocks = 0;
     for (int i = c->L - 1; i >= 1; --i) {
         MarchLevel &l = m.lev[m.nlev++];
         l.dirs = c->dirs + c->dir_off[i];
@@ -320,11 +319,8 @@ static MarchAllArgs march_all_args(const rc_ctx *c)
         march_level_map(l, c->march_split == 1 ? c->mmap[i] : 0);  // the persistent form is canonical
         l.merged = (i == c->L - 1 && top_merged_in_march(c)) ? c->level[i] : nullptr;
         l.storage = (int)c->storage;
-        l.block_begin = blocks;
-        m.level_begin[m.nlev - 1] = blocks;
         const int64_t rays = (int64_t)l.PX * l.PY * l.N, per = (int64_t)kBlock * c->march_m;
         l.blocks = (int)(((int64_t)l.slots + per - 1) / per);
-        blocks += l.blocks;
         l.code_begin = (uint32_t)c->code_off[i];
         l.dir_off = (int)c->dir_off[i];
         m.total_rays += (uint32_t)rays;

@@ -651,12 +651,12 @@ __device__ __forceinline__ void march_all_body(const uint8_t *sdf_codes, int W, 
 template <int SPEC, int M>
 __global__ __launch_bounds__(kBlock) void k_march_all(MarchAllArgs a)
 {
-    // the level: the number of later levels whose blocks start at or before
-    // this one (independent scalar loads, one wait)
-    int lv = 0;
-#pragma unroll
-    for (int j = 1; j < RC_MAX_LEVELS; ++j) lv += (j < a.nlev && (int)blockIdx.x >= a.level_begin[j]) ? 1 : 0;
-    march_all_body<SPEC, M>(a.sdf, a.W, a.H, a.lev[lv], (int)blockIdx.x - a.lev[lv].block_begin, a.emis4);
+    // the grid is (blocks of the largest level) x (levels): the level is
+    // blockIdx.y, and workgroups are dispatched x-fastest, so the top level's
+    // (longest) rays start first
+    const MarchLevel &L = a.lev[blockIdx.y];
+    if ((int)blockIdx.x >= L.blocks) return;
+    march_all_body<SPEC, M>(a.sdf, a.W, a.H, L, (int)blockIdx.x, a.emis4);
 }
 
 // The split march over a work list of (view, level) jobs, blocks in launch
@@ -867,11 +867,12 @@ hipError_t launch_level0_gather(const LevelArgs &a, Storage st, float *irr, floa
 template <int SPEC>
 static hipError_t launch_march_all_spec(const MarchAllArgs &a, int m, int blocks, hipStream_t s)
 {
+    const dim3 grid(blocks, a.nlev);
     switch (m) {
-    case 1: hipLaunchKernelGGL((k_march_all<SPEC, 1>), dim3(blocks), dim3(kBlock), 0, s, a); break;
-    case 2: hipLaunchKernelGGL((k_march_all<SPEC, 2>), dim3(blocks), dim3(kBlock), 0, s, a); break;
-    case 4: hipLaunchKernelGGL((k_march_all<SPEC, 4>), dim3(blocks), dim3(kBlock), 0, s, a); break;
-    case 8: hipLaunchKernelGGL((k_march_all<SPEC, 8>), dim3(blocks), dim3(kBlock), 0, s, a); break;
+    case 1: hipLaunchKernelGGL((k_march_all<SPEC, 1>), grid, dim3(kBlock), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_march_all<SPEC, 2>), grid, dim3(kBlock), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((k_march_all<SPEC, 4>), grid, dim3(kBlock), 0, s, a); break;
+    case 8: hipLaunchKernelGGL((k_march_all<SPEC, 8>), grid, dim3(kBlock), 0, s, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -893,7 +894,8 @@ hipError_t launch_march_persist(const MarchAllArgs &a, int spec, hipStream_t s)
 hipError_t launch_march_all(const MarchAllArgs &a, int spec, int m, hipStream_t s)
 {
     if (a.nlev < 1) return hipSuccess;  // one-level frame: no level >= 1 to march
-    const int blocks = a.lev[a.nlev - 1].block_begin + a.lev[a.nlev - 1].blocks;
+    int blocks = 0;
+    for (int i = 0; i < a.nlev; ++i) blocks = a.lev[i].blocks > blocks ? a.lev[i].blocks : blocks;
     if (blocks <= 0) return hipSuccess;
     switch (spec) {
     case 1: return launch_march_all_spec<1>(a, m, blocks, s);

@@ -152,7 +152,7 @@ struct MarchLevel {
     int PX, PY, N, log2n;
     float inv_px;            // 1/PX in fp32 (probe -> row, corrected)
     float t0, t1, spacing;
-    int block_begin, blocks; // workgroups of 256*M canonical rays
+    int blocks;              // workgroups of 256*M ray slots (the march grid is blocks x levels)
     uint32_t code_begin;     // first ray of this level in the code buffer (levels 1, 2, ... back to back)
     int dir_off;             // first direction of this level in dirs_base
     // Order of the rays over the lanes (k_march_all only): map 0 = canonical
@@ -182,7 +182,6 @@ void march_level_map(MarchLevel &l, int map);
 struct MarchAllArgs {
     const uint8_t *sdf;
     int W, H, nlev;
-    int level_begin[RC_MAX_LEVELS];  // lev[j].block_begin, contiguous (one scalar load)
     MarchLevel lev[RC_MAX_LEVELS];  // top level first
     const float4 *emis4;     // interleaved emission (a merged level's hits)
     // persistent form: every ray of every level, in 64-ray packets
@@ -191,7 +190,7 @@ struct MarchAllArgs {
     uint32_t total_rays, n_packets, n_waves;
 };
 hipError_t launch_march_all(const MarchAllArgs &a, int speculation, int rays_per_lane, hipStream_t s);
-// one level of one view in a batched split march (block_begin unused)
+// one level of one view in a batched split march
 struct MarchJob {
     const uint8_t *sdf;
     int W, H;
