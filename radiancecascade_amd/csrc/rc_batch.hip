@@ -132,6 +132,18 @@ static LevelArgs view_level_args(const WorkList &w, const WlView &x, int i)
     return a;
 }
 
+// As in the single-view frame: the split march merges each view's top level
+// when it takes one ray per lane and one sample per step and the top level's
+// lane order keeps a texel's 4 directions in one lane quad.  The views' top
+// levels differ, so the batch asks it of every level's order.
+static bool top_merged(const WorkList &w)
+{
+    if (!w.split || w.spec != 1 || w.m != 1) return false;
+    for (int i = 1; i < w.maxL; ++i)
+        if (w.mmap[i] == 5) return false;
+    return true;
+}
+
 // Job tables of the current tuning, uploaded to the device.
 static hipError_t build_tables(WorkList &w, std::string &err)
 {
@@ -211,9 +223,12 @@ static hipError_t build_tables(WorkList &w, std::string &err)
                 l.spacing = (float)probe_spacing(i);
                 l.dir_off = dir_table_offset(i);
                 march_level_map(l, w.mmap[i]);
+                // a view's top level is merged by the march, as in the single-view frame
+                l.merged = (i == x.L - 1 && top_merged(w)) ? (void *)(w.ws + x.level[i]) : nullptr;
+                l.storage = (int)w.st;
                 const int64_t per = (int64_t)kBlock * w.m;
                 l.blocks = (int)(((int64_t)l.slots + per - 1) / per);
-                j.push_back(MarchJob{w.ws + x.sdf, x.W, x.H, l});
+                j.push_back(MarchJob{w.ws + x.sdf, (const float4 *)(w.ws + x.emis4), x.W, x.H, l});
                 b.push_back(l.blocks);
             }
         }
@@ -225,6 +240,7 @@ static hipError_t build_tables(WorkList &w, std::string &err)
         std::vector<int64_t> b;
         for (const WlView &x : w.v) {
             if (i >= x.L) continue;
+            if (i == x.L - 1 && w.split && top_merged(w)) continue;  // merged by the march
             LevelArgs a = view_level_args(w, x, i);
             a.emis4 = (const float4 *)(w.ws + x.emis4);
             if (w.split) a.codes = (const int *)(w.ws + x.codes[i]);

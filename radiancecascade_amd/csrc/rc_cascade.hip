@@ -668,7 +668,7 @@ __global__ __launch_bounds__(kBlock) void k_march_batch(const MarchJob *__restri
     const int b = (int)blockIdx.x;
     const int j = batch_job(begins, njobs, b);
     const MarchJob J = jobs[j];
-    march_all_body<SPEC, M>(J.sdf, J.W, J.H, J.L, b - begins[j], nullptr);
+    march_all_body<SPEC, M>(J.sdf, J.W, J.H, J.L, b - begins[j], J.emis4);
 }
 
 // Persistent form of the split march.  March lengths differ by two orders of

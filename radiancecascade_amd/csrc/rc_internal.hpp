@@ -193,6 +193,7 @@ hipError_t launch_march_all(const MarchAllArgs &a, int speculation, int rays_per
 // one level of one view in a batched split march
 struct MarchJob {
     const uint8_t *sdf;
+    const float4 *emis4;     // the view's interleaved emission (a merged top level's hits)
     int W, H;
     MarchLevel L;
 };
