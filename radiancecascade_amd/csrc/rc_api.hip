@@ -291,11 +291,11 @@ static LevelArgs level_args(const rc_ctx *c, int i, const float *emissive)
 }
 
 // The split march merges the top level itself (no level kernel for it) when
-// it takes one ray per lane and one sample per step, and the top level's lane
-// order keeps each texel's 4 directions in one lane quad.
+// it takes one ray per lane and the top level's lane order keeps each
+// texel's 4 directions in one lane quad.
 static bool top_merged_in_march(const rc_ctx *c)
 {
-    return c->L > 1 && c->march_split == 1 && c->spec == 1 && c->march_m == 1 && c->mmap[c->L - 1] != 5;
+    return c->L > 1 && c->march_split == 1 && c->march_m == 1 && c->mmap[c->L - 1] != 5;
 }
 
 static MarchAllArgs march_all_args(const rc_ctx *c)
@@ -319,7 +319,7 @@ static MarchAllArgs march_all_args(const rc_ctx *c)
         march_level_map(l, c->march_split == 1 ? c->mmap[i] : 0);  // the persistent form is canonical
         l.merged = (i == c->L - 1 && top_merged_in_march(c)) ? c->level[i] : nullptr;
         l.storage = (int)c->storage;
-        const int64_t rays = (int64_t)l.PX * l.PY * l.N, per = (int64_t)kBlock * c->march_m;
+        const int64_t rays = (int64_t)l.PX * l.PY * l.N, per = (int64_t)kMarchBlock * c->march_m;
         l.blocks = (int)(((int64_t)l.slots + per - 1) / per);
         l.code_begin = (uint32_t)c->code_off[i];
         l.dir_off = (int)c->dir_off[i];

@@ -133,12 +133,12 @@ static LevelArgs view_level_args(const WorkList &w, const WlView &x, int i)
 }
 
 // As in the single-view frame: the split march merges each view's top level
-// when it takes one ray per lane and one sample per step and the top level's
-// lane order keeps a texel's 4 directions in one lane quad.  The views' top
+// when it takes one ray per lane and the top level's lane order keeps a
+// texel's 4 directions in one lane quad.  The views' top
 // levels differ, so the batch asks it of every level's order.
 static bool top_merged(const WorkList &w)
 {
-    if (!w.split || w.spec != 1 || w.m != 1) return false;
+    if (!w.split || w.m != 1) return false;
     for (int i = 1; i < w.maxL; ++i)
         if (w.mmap[i] == 5) return false;
     return true;
@@ -226,13 +226,13 @@ static hipError_t build_tables(WorkList &w, std::string &err)
                 // a view's top level is merged by the march, as in the single-view frame
                 l.merged = (i == x.L - 1 && top_merged(w)) ? (void *)(w.ws + x.level[i]) : nullptr;
                 l.storage = (int)w.st;
-                const int64_t per = (int64_t)kBlock * w.m;
+                const int64_t per = (int64_t)kMarchBlock * w.m;
                 l.blocks = (int)(((int64_t)l.slots + per - 1) / per);
                 j.push_back(MarchJob{w.ws + x.sdf, (const float4 *)(w.ws + x.emis4), x.W, x.H, l});
                 b.push_back(l.blocks);
             }
         }
-        ok = ok && add_launch(w, blob, WL_MARCH, 0, j, b, kBlock, err);
+        ok = ok && add_launch(w, blob, WL_MARCH, 0, j, b, kMarchBlock, err);
     }
 
     for (int i = w.maxL - 1; i >= 1 && ok; --i) {

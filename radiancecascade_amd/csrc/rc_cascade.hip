@@ -587,7 +587,7 @@ __device__ __forceinline__ void march_all_body(const uint8_t *sdf_codes, int W, 
                                                const float4 *__restrict__ emis4)
 {
     const __amdgpu_buffer_rsrc_t sdf = sdf_rsrc(sdf_codes, W, H);
-    const int base = lb * kBlock * M + (int)threadIdx.x;
+    const int base = lb * kMarchBlock * M + (int)threadIdx.x;
     const int total = L.map ? (int)L.slots : L.PX * L.PY * L.N;
 
     int j = 0, ray = base;
@@ -602,6 +602,64 @@ __device__ __forceinline__ void march_all_body(const uint8_t *sdf_codes, int W, 
         r = ray_begin(((float)px + 0.5f) * L.spacing, ((float)py + 0.5f) * L.spacing, dir.x, dir.y, L.t0, L.t1,
                       valid);
     };
+    if (M == 1 && SPEC > 1) {
+        // One ray per lane, SPEC samples per round: the samples of
+        // ray_round<SPEC> (p[0] = t, p[j] = p[j-1] + guess; sample j accepted
+        // while its step equals the guess), with the lanes' paths as masked
+        // updates and the ray's end read from where t stopped, as below: t is
+        // left at the sample that ended the ray (off-image or a hit), or at or
+        // past t1 when the interval ended.
+        int px, py, k;
+        valid = decode_slot(L, ray < total ? ray : 0, px, py, k) && ray < total;
+        at = ((__umul24((uint32_t)py, (uint32_t)L.PX) + (uint32_t)px) << L.log2n) + (uint32_t)k;
+        const float2 dir = L.dirs[k];
+        const float cx = ((float)px + 0.5f) * L.spacing, cy = ((float)py + 0.5f) * L.spacing;
+        const int tiles = sdf_tiles_x(W);
+        const float t1 = L.t1;
+        float t = L.t0, guess = 1.0f;
+        bool alive = valid && t < t1;
+        while (alive) {
+            float p[SPEC];
+            uint32_t c[SPEC];
+            bool ins[SPEC];
+            p[0] = t;
+#pragma unroll
+            for (int j = 1; j < SPEC; ++j) p[j] = p[j - 1] + guess;
+#pragma unroll
+            for (int j = 0; j < SPEC; ++j) {
+                const int ix = floor_i32(fmaf(p[j], dir.x, cx)), iy = floor_i32(fmaf(p[j], dir.y, cy));
+                ins[j] = (uint32_t)ix < (uint32_t)W && (uint32_t)iy < (uint32_t)H;
+                c[j] = sdf_code_load(sdf, ix, iy, tiles);
+            }
+            bool go = true;
+#pragma unroll
+            for (int j = 0; j < SPEC; ++j) {
+                if (!go) continue;
+                if ((j > 0 && !(p[j] < t1)) || !ins[j] || c[j] == 0) {
+                    // the interval ended at the previous step, or this sample
+                    // left the image or hit: the ray ends at p[j]
+                    t = p[j];
+                    alive = false;
+                    go = false;
+                    continue;
+                }
+                const float step = step_of_code(c[j]);
+                if (j + 1 < SPEC && step == guess) continue;  // p[j+1] == p[j] + step
+                t = p[j] + step;
+                guess = step;
+                alive = t < t1;
+                go = false;
+            }
+        }
+        int code = kRayT1;
+        if (t < t1) {
+            const int ix = floor_i32(fmaf(t, dir.x, cx)), iy = floor_i32(fmaf(t, dir.y, cy));
+            code = ((uint32_t)ix < (uint32_t)W && (uint32_t)iy < (uint32_t)H) ? pixel_index(ix, iy, W) : kRayExit;
+        }
+        if (valid) L.codes[at] = code;
+        if (L.merged) merge_top_quad(L, emis4, valid, code, at);
+        return;
+    }
     if (M == 1 && SPEC == 1) {
         // One ray per lane, one sample per step: a branch-free body in a
         // divergent loop (lanes leave as their rays end) on the raw byte
@@ -641,7 +699,7 @@ __device__ __forceinline__ void march_all_body(const uint8_t *sdf_codes, int W, 
         if (r.alive) ray_round<SPEC>(r, sdf, W, H, L.t1);
         if (j < M && !r.alive) {
             if (valid) L.codes[at] = r.code;
-            ray += kBlock;
+            ray += kMarchBlock;
             if (++j < M) start();
         }
         if (!__any(j < M)) break;
@@ -649,7 +707,7 @@ __device__ __forceinline__ void march_all_body(const uint8_t *sdf_codes, int W, 
 }
 
 template <int SPEC, int M>
-__global__ __launch_bounds__(kBlock) void k_march_all(MarchAllArgs a)
+__global__ __launch_bounds__(kMarchBlock) void k_march_all(MarchAllArgs a)
 {
     // the grid is (blocks of the largest level) x (levels): the level is
     // blockIdx.y, and workgroups are dispatched x-fastest, so the top level's
@@ -662,7 +720,7 @@ __global__ __launch_bounds__(kBlock) void k_march_all(MarchAllArgs a)
 // The split march over a work list of (view, level) jobs, blocks in launch
 // order as k_march_all's (the job list puts every view's top level first).
 template <int SPEC, int M>
-__global__ __launch_bounds__(kBlock) void k_march_batch(const MarchJob *__restrict__ jobs,
+__global__ __launch_bounds__(kMarchBlock) void k_march_batch(const MarchJob *__restrict__ jobs,
                                                         const int *__restrict__ begins, int njobs)
 {
     const int b = (int)blockIdx.x;
@@ -869,10 +927,10 @@ static hipError_t launch_march_all_spec(const MarchAllArgs &a, int m, int blocks
 {
     const dim3 grid(blocks, a.nlev);
     switch (m) {
-    case 1: hipLaunchKernelGGL((k_march_all<SPEC, 1>), grid, dim3(kBlock), 0, s, a); break;
-    case 2: hipLaunchKernelGGL((k_march_all<SPEC, 2>), grid, dim3(kBlock), 0, s, a); break;
-    case 4: hipLaunchKernelGGL((k_march_all<SPEC, 4>), grid, dim3(kBlock), 0, s, a); break;
-    case 8: hipLaunchKernelGGL((k_march_all<SPEC, 8>), grid, dim3(kBlock), 0, s, a); break;
+    case 1: hipLaunchKernelGGL((k_march_all<SPEC, 1>), grid, dim3(kMarchBlock), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_march_all<SPEC, 2>), grid, dim3(kMarchBlock), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((k_march_all<SPEC, 4>), grid, dim3(kMarchBlock), 0, s, a); break;
+    case 8: hipLaunchKernelGGL((k_march_all<SPEC, 8>), grid, dim3(kMarchBlock), 0, s, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -920,9 +978,9 @@ static hipError_t launch_march_batch_spec(const BatchLaunch &b, int m, hipStream
 {
     const MarchJob *j = (const MarchJob *)b.jobs;
     switch (m) {
-    case 1: hipLaunchKernelGGL((k_march_batch<SPEC, 1>), dim3(b.blocks), dim3(kBlock), 0, s, j, b.begins, b.njobs); break;
-    case 2: hipLaunchKernelGGL((k_march_batch<SPEC, 2>), dim3(b.blocks), dim3(kBlock), 0, s, j, b.begins, b.njobs); break;
-    case 4: hipLaunchKernelGGL((k_march_batch<SPEC, 4>), dim3(b.blocks), dim3(kBlock), 0, s, j, b.begins, b.njobs); break;
+    case 1: hipLaunchKernelGGL((k_march_batch<SPEC, 1>), dim3(b.blocks), dim3(kMarchBlock), 0, s, j, b.begins, b.njobs); break;
+    case 2: hipLaunchKernelGGL((k_march_batch<SPEC, 2>), dim3(b.blocks), dim3(kMarchBlock), 0, s, j, b.begins, b.njobs); break;
+    case 4: hipLaunchKernelGGL((k_march_batch<SPEC, 4>), dim3(b.blocks), dim3(kMarchBlock), 0, s, j, b.begins, b.njobs); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

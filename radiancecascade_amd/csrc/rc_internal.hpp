@@ -167,8 +167,8 @@ struct MarchLevel {
     // quad holds the 4 child directions of one texel, so the march writes
     // A_top = quad mean of (hit ? (E, 0) : (0, 1)) here, in `storage`
     // (a Storage value), and no level kernel runs for it.  nullptr: not merged.
-    // Needs one ray per lane, one sample per step and a lane order whose
-    // direction slice is at least 4 (every order but 5).
+    // Needs one ray per lane and a lane order whose direction slice is at
+    // least 4 (every order but 5).
     void *merged;
     int storage;
 };
@@ -189,6 +189,10 @@ struct MarchAllArgs {
     const float2 *dirs_base;
     uint32_t total_rays, n_packets, n_waves;
 };
+// The split march runs one-wave workgroups: march lengths differ by two
+// orders of magnitude between lanes and waves, and a workgroup's wave slots
+// are freed only when its last wave ends, so a wave is its own workgroup.
+constexpr int kMarchBlock = 64;
 hipError_t launch_march_all(const MarchAllArgs &a, int speculation, int rays_per_lane, hipStream_t s);
 // one level of one view in a batched split march
 struct MarchJob {
