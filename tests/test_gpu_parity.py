@@ -572,6 +572,35 @@ def test_ndc_depth_input(rc, oracle, W, H, L, seed, nb, ne):
     assert torch.equal(irr_ndc, irr_lin)
 
 
+@pytest.mark.parametrize("W,H", [(16384, 32), (32, 16384)])
+def test_largest_dimension(rc, oracle, W, H):
+    """The advertised limit RC_MAX_DIM = 16384 on either axis (8 cascades by
+    the coverage rule, 14 jump-flood passes, level 7 with 65,536
+    directions): the frame equals the FP32 oracle bit for bit, with the
+    autotuned configuration too; one pixel more is refused."""
+    _say(f"largest dimension W={W} H={H}")
+    seed, nb, ne = 1, 64, 16
+    L = oracle.levels_for_coverage(W, H)
+    assert L == 8
+    depth, emis = _device_gbuffer(rc, W, H, seed, nb, ne)
+    d, _, e = oracle.gbuffer(seed, W, H, nb, ne)
+    ref = oracle.run(d, e, L, np.float32)
+    assert float(ref.mean()) > 0.1   # emitters in view
+    irr = torch.empty((3, H, W), dtype=torch.float32, device="cuda:0")
+    with rc.RadianceCascadeStage(W, H, levels=0) as st:
+        assert st.levels == L
+        st.run_device(depth, emis, irr)
+        torch.cuda.synchronize()
+        assert np.array_equal(irr.cpu().numpy().view(np.uint32), ref.view(np.uint32))
+        st.autotune(depth, emis, irr, iters=1)
+        irr.fill_(-1.0)
+        st.run_device(depth, emis, irr)
+        torch.cuda.synchronize()
+        assert np.array_equal(irr.cpu().numpy().view(np.uint32), ref.view(np.uint32))
+    with pytest.raises(rc.RcError):
+        rc.RadianceCascadeStage(W + (W == 16384), H + (H == 16384))
+
+
 def test_4k_seven_levels_plain_and_learned(rc, oracle):
     """Config 4's test resolution: 3840x2160 with 7 cascades.  The plain frame
     equals the FP32 oracle bit for bit (and the FP64 oracle within the
