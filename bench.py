@@ -42,18 +42,25 @@ def texel_bytes_contract(storage: str) -> int:
 
 
 # Kernel families of the plain frame and the stages (rc_get_stats) that
-# make them up.  The jump flood is one family whatever the split between
-# global passes and the fused LDS tail, so the dominant family cannot change
-# with the autotuned fusion depth.
+# make them up.  "seed" is the occupancy kernel and "jfa" the distance
+# transform: the exact transform's column + row kernels (the default), or
+# the jump flood as one family whatever the split between global passes and
+# the fused LDS tail, so the dominant family cannot change with the
+# autotuned fusion depth.
 FAMILIES = {"seed": ["seed"], "jfa": ["jfa", "jfa_tail"], "march": ["march"], "levels": None, "level0": ["level0"]}
 
 
-def family_alg_bytes(W, H, L, storage, rays, split, jfa_passes, top_in_march=False):
+def family_alg_bytes(W, H, L, storage, rays, split, jfa_passes, top_in_march=False, sdf="exact"):
     """Algorithmic bytes per frame of each kernel family: the bytes its
     kernels must move, charged to the kernel that moves them.
-      seed   : depth + emissive read (16 B/px), seed field written (4 B/px);
-      jfa    : 8 B/px per pass (seed field read and written once); the last
-               pass reads 4 B/px and writes the 1-byte distance field;
+      seed   : depth + emissive read (16 B/px); written: the exact mode's
+               occupancy bits (1/8 B/px), or the jump flood's seed field
+               (4 B/px);
+      jfa    : exact mode: the column kernel reads the bits (1/8 B/px) and
+               writes 16-bit vertical distances (2 B/px), the row kernel
+               reads them (2 B/px) and writes the 1-byte distance field;
+               jump flood: 8 B/px per pass (seed field read and written
+               once), the last pass reads 4 B/px and writes the field;
       march  : the 1-byte distance field read once, a 4-byte code per ray of
                levels >= 1 written, and (top_in_march: the march merges the
                top level itself) the top level's merged texels (b R_top / 4);
@@ -71,8 +78,12 @@ def family_alg_bytes(W, H, L, storage, rays, split, jfa_passes, top_in_march=Fal
     for i in range(1, last + 1):
         levels += b * tex[i] + (b * tex[i + 1] if i + 1 < L else 0) + (4 * rays[i] if split else 0)
     march = (n + 4 * sum(rays[1:L]) + (b * tex[L - 1] if top_in_march else 0)) if split else 0
-    return {"seed": 20 * n,
-            "jfa": (8 * (jfa_passes - 1) + 5) * n if jfa_passes > 0 else 5 * n,
+    if sdf == "exact":
+        seed, jfa = 16 * n + n / 8, (n / 8 + 2 * n) + (2 * n + n)
+    else:
+        seed, jfa = 20 * n, (8 * (jfa_passes - 1) + 5) * n if jfa_passes > 0 else 5 * n
+    return {"seed": seed,
+            "jfa": jfa,
             "march": march,
             "levels": levels,
             "level0": n + (b * tex[1] if L > 1 else 0) + 12 * n}
@@ -363,7 +374,8 @@ def batch_leg(rc, torch, args, stream, fps_plain):
 
 
 # kernel name prefix -> family (the plain frame's kernels)
-KERNEL_FAMILY = [("k_seed_init", "seed"), ("k_jfa_pass", "jfa"), ("k_jfa_tail", "jfa"), ("k_jfa_none", "jfa"),
+KERNEL_FAMILY = [("k_seed_init", "seed"), ("k_occ_bits", "seed"), ("k_edt_cols", "jfa"), ("k_edt_rows", "jfa"),
+                 ("k_jfa_pass", "jfa"), ("k_jfa_tail", "jfa"), ("k_jfa_none", "jfa"),
                  ("k_march_all", "march"), ("k_march_persist", "march"), ("k_level0", "level0"),
                  ("k_level<", "levels")]
 
@@ -501,6 +513,8 @@ def main():
     ap.add_argument("--boxes", type=int, default=32)
     ap.add_argument("--emissive", type=int, default=4)
     ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"])
+    ap.add_argument("--sdf", default="exact", choices=["exact", "jfa"],
+                    help="distance transform of the timed frame (exact: the library default)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
@@ -569,6 +583,7 @@ def main():
     stream = torch.cuda.current_stream()
 
     stage = rc.RadianceCascadeStage(W, H, levels=L, storage=args.storage)
+    stage.set_sdf_mode(args.sdf)
     if args.load_tuning:
         apply_tuning(stage, load_tuning(args.load_tuning))
         tiling = stage.tiling()
@@ -604,6 +619,7 @@ def main():
         prof.set_tiling(lvl, shape, rpt)
     split, spec, rpl = stage.march()
     prof.set_march(split, spec, rpl)
+    prof.set_sdf_mode(stage.sdf_mode())
     prof.set_jfa_fuse(stage.jfa_fuse())
     prof.set_jfa_rows(stage.jfa_rows())
     for lvl, order in stage.march_order().items():
@@ -620,7 +636,7 @@ def main():
     per_frame_ms = {k: st.stage_ms[k] / st.runs for k in st.stage_ms}
     top_in_march = bool(split) and L > 1 and st.stage_launches.get(f"level{L - 1}", 0) == 0
     fam_alg = family_alg_bytes(W, H, L, args.storage, rays, bool(split), (max(W, H) - 1).bit_length(),  # P = ceil(log2 max(W, H))
-                               top_in_march)
+                               top_in_march, stage.sdf_mode())
     fam_ms, fam_launches = {}, {}
     for fam, stages in FAMILIES.items():
         stages = stages or [f"level{i}" for i in range(1, L)]
@@ -669,7 +685,7 @@ def main():
                    "tiling": {str(k): f"{v[0][0]}x{v[0][1]}x{v[0][2]} r{v[1]}" for k, v in tiling.items()},
                    "march": {"split": split, "speculation": spec, "rays_per_lane": rpl, "top_level_merged": top_in_march,
                              "order": {str(k): v for k, v in stage.march_order().items()}},
-                   "jfa_fuse": stage.jfa_fuse(), "jfa_rows": stage.jfa_rows()},
+                   "sdf": stage.sdf_mode(), "jfa_fuse": stage.jfa_fuse(), "jfa_rows": stage.jfa_rows()},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None if traffic is None else traffic / fam_launches[dominant],

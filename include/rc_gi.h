@@ -64,7 +64,7 @@ typedef enum { RC_STORAGE_F32 = 0, RC_STORAGE_F16 = 1, RC_STORAGE_BF16 = 2 } rc_
 
 /* rc_config.flags */
 #define RC_FLAG_PROFILE    0x1u  /* time every stage with HIP events (rc_get_stats) */
-#define RC_FLAG_KEEP_DEBUG 0x2u  /* also write the jump-flood field and level 0 for rc_debug_* */
+#define RC_FLAG_KEEP_DEBUG 0x2u  /* also write the jump-flood seed field (that mode) and level 0 for rc_debug_* */
 #define RC_FLAG_GRAPH      0x8u  /* rc_run_device replays the frame as one captured hipGraph
                                     (one host call per frame) instead of launching its
                                     kernels one by one; on MI355X each replay leaves ~4 us
@@ -109,8 +109,8 @@ typedef struct {
 
 /* Per-stage device times in milliseconds, summed over the runs since the last
  * rc_reset_stats (only with RC_FLAG_PROFILE), and the run count. */
-#define RC_STAGE_SEED 0
-#define RC_STAGE_JFA 1          /* all global jump-flood passes */
+#define RC_STAGE_SEED 0         /* occupancy: exact mode's column words, or the jump flood's seeds */
+#define RC_STAGE_JFA 1          /* the distance transform: exact mode's column + row kernels, or all global jump-flood passes */
 #define RC_STAGE_JFA_TAIL 2     /* fused short-step passes + distance field */
 #define RC_STAGE_LEVEL0 3       /* RC_STAGE_LEVEL0 + i = level i march/merge; level 0 includes the gather */
 #define RC_STAGE_MARCH (RC_STAGE_LEVEL0 + RC_MAX_LEVELS) /* split march of all levels >= 1 */
@@ -184,9 +184,21 @@ void rc_destroy(rc_ctx *ctx);
 rc_status rc_set_level_tiling(rc_ctx *ctx, uint32_t level, uint32_t shape, uint32_t rays_per_lane);
 rc_status rc_get_level_tiling(const rc_ctx *ctx, uint32_t level, uint32_t *shape, uint32_t *rays_per_lane);
 rc_status rc_autotune(rc_ctx *ctx, const rc_gbuffer *dev, float *irradiance_dev, uint32_t iters, void *stream);
-/* Jump-flood passes with step <= fused_step (0, 1, 2, 4 or 8) run fused in
- * one LDS-tiled kernel; 0 runs every pass as a global kernel.  The result is
- * identical; rc_autotune also picks this. */
+/* Distance transform of the SDF (no counterpart in the reference, which has
+ * no distance field).  RC_SDF_EXACT (the default): the exact squared
+ * Euclidean distance to the nearest occupied pixel, by an occupancy kernel,
+ * a column kernel and a row kernel (3 launches).  RC_SDF_JUMP_FLOOD: the jump
+ * flood of ceil(log2 max(W, H)) passes, whose distance can exceed the exact
+ * one at a few pixels (< 1 %; never below it); its seed field is readable
+ * with rc_debug_seeds.  Both fields are stored as the same one-byte code.
+ * Changing the mode allocates the mode's buffers if needed. */
+#define RC_SDF_EXACT 0u
+#define RC_SDF_JUMP_FLOOD 1u
+rc_status rc_set_sdf_mode(rc_ctx *ctx, uint32_t mode);
+uint32_t rc_get_sdf_mode(const rc_ctx *ctx);
+/* Jump-flood mode: passes with step <= fused_step (0, 1, 2, 4 or 8) run fused
+ * in one LDS-tiled kernel; 0 runs every pass as a global kernel.  The result
+ * is identical; rc_autotune also picks this. */
 rc_status rc_set_jfa_fuse(rc_ctx *ctx, uint32_t fused_step);
 uint32_t rc_get_jfa_fuse(const rc_ctx *ctx);
 /* Image rows per wave of a global jump-flood pass: 1, 2 or 4 (no counterpart
@@ -226,12 +238,14 @@ uint32_t rc_abi_version(void);
 rc_status rc_device_count(int *count);
 
 /* Parity readbacks of the last run (device destination, stream ordered).
- * rc_debug_seeds / rc_debug_level(.., 0, ..) need RC_FLAG_KEEP_DEBUG.  After a
+ * rc_debug_seeds / rc_debug_level(.., 0, ..) need RC_FLAG_KEEP_DEBUG (and
+ * rc_debug_seeds the jump-flood mode).  After a
  * learned frame (rc_set_surrogate mode 1) rc_debug_level returns that frame's
  * levels: 2 the network's, 1 merged from it (with flagged tiles patched), 0
  * the fluence; it has no level above 2 (RC_ERR_STATE).
  *   seeds : uint32 [H][W]  final jump-flood field, (y<<16)|x or 0xFFFFFFFF
- *   sdf   : float  [H][W]  distance field in px (+inf with no seed)
+ *   sdf   : float  [H][W]  distance values the march reads, in px (2160 with
+ *           no occupied pixel)
  *   level : float  [Py_i][Px_i][max(N_i/4,1)][4], merged level i averaged over
  *           the 4 children of each level-(i-1) direction (fp32 whatever the
  *           storage; level 0 is the per-probe fluence) */
@@ -403,18 +417,17 @@ rc_status rc_debug_mip(rc_ctx *ctx, uint32_t level, float *emission_dev, float *
  *
  * RC_BATCH_WORKLIST (default): a fixed handful of kernels per batch, each
  *   walking a work list of (view, stage block) jobs over ONE shared workspace:
- *   the seeds, one launch per jump-flood pass (right-aligned: a view takes
- *   the passes its size needs), the fused tail, the split march of every
- *   (view, level >= 1), one launch per level index, and level 0 + gather:
- *   about 20 kernel nodes whatever the number of views.  rc_batch_autotune
- *   tunes each stage for the batch as a whole.
+ *   the exact distance transform's occupancy, column and row kernels, the
+ *   split march of every (view, level >= 1), one launch per level index, and
+ *   level 0 + gather: about 10 kernel nodes whatever the number of views.
+ *   rc_batch_autotune tunes each stage for the batch as a whole.
  * RC_BATCH_STREAMS: one context per view; each view's frame is captured once
  *   and the views are dealt round-robin to `streams` captured streams so that
  *   small views run concurrently (about 17 kernel nodes per view).
  *   rc_batch_autotune tunes each distinct (width, height, levels) once.
  *
  * Either way every view's irradiance equals its single-view rc_run_device
- * frame bit for bit.  The view buffers are the caller's and are bound at
+ * frame (default distance transform, RC_SDF_EXACT) bit for bit.  The view buffers are the caller's and are bound at
  * create time (the graph keeps their addresses); depth is linear view depth.
  * Plain cascades only. */
 typedef struct {

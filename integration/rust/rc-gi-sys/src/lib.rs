@@ -15,6 +15,8 @@ pub const RC_FLAG_KEEP_DEBUG: u32 = 0x2;
 pub const RC_FLAG_GRAPH: u32 = 0x8;
 pub const RC_DEPTH_LINEAR: u32 = 0;
 pub const RC_DEPTH_NDC: u32 = 1;
+pub const RC_SDF_EXACT: u32 = 0;
+pub const RC_SDF_JUMP_FLOOD: u32 = 1;
 pub const RC_STAGE_SEED: usize = 0;
 pub const RC_STAGE_JFA: usize = 1;
 pub const RC_STAGE_JFA_TAIL: usize = 2;
@@ -190,6 +192,8 @@ extern "C" {
         -> rc_status;
     pub fn rc_autotune(ctx: *mut rc_ctx, dev: *const rc_gbuffer, irradiance_dev: *mut f32, iters: u32,
                        stream: *mut c_void) -> rc_status;
+    pub fn rc_set_sdf_mode(ctx: *mut rc_ctx, mode: u32) -> rc_status;
+    pub fn rc_get_sdf_mode(ctx: *const rc_ctx) -> u32;
     pub fn rc_set_jfa_fuse(ctx: *mut rc_ctx, fused_step: u32) -> rc_status;
     pub fn rc_get_jfa_fuse(ctx: *const rc_ctx) -> u32;
     pub fn rc_set_jfa_rows(ctx: *mut rc_ctx, rows: u32) -> rc_status;

@@ -311,6 +311,90 @@ void rco_sdf_f64(int W, int H, const uint32_t *seed, double *sdf)
         for (int x = 0; x < W; ++x) sdf[(size_t)y * W + x] = sdf_value_at(seed[(size_t)y * W + x], x, y);
 }
 
+/*
+ * Exact squared Euclidean distance transform of the occupancy (seed_init !=
+ * none): d2[p] = min over occupied q of |p - q|^2, -1 when no pixel is
+ * occupied.  Felzenszwalb & Huttenlocher, "Distance Transforms of Sampled
+ * Functions" (Theory of Computing 8, 2012): per column the vertical distance
+ * g to the nearest occupied pixel of that column, then per row the lower
+ * envelope of the parabolas (x - q)^2 + g(q)^2 over the columns q with a
+ * finite g, built left to right with a stack.  The envelope's breakpoints are
+ * compared as exact integer cross products (no floating point), so the
+ * result is exact for every image up to 16384 px a side.
+ */
+void rco_edt_d2(int W, int H, const uint32_t *seed_init, int64_t *d2)
+{
+    const int64_t NONE = -1;
+    int64_t *g = (int64_t *)malloc((size_t)W * H * sizeof(int64_t));
+#pragma omp parallel for schedule(static)
+    for (int x = 0; x < W; ++x) {
+        int64_t last = NONE;  /* row of the nearest occupied pixel above (or at) y */
+        for (int y = 0; y < H; ++y) {
+            if (seed_init[(size_t)y * W + x] != RCO_SEED_NONE) last = y;
+            g[(size_t)y * W + x] = last == NONE ? NONE : y - last;
+        }
+        last = NONE;
+        for (int y = H - 1; y >= 0; --y) {
+            if (seed_init[(size_t)y * W + x] != RCO_SEED_NONE) last = y;
+            if (last != NONE) {
+                int64_t *v = &g[(size_t)y * W + x];
+                if (*v == NONE || last - y < *v) *v = last - y;
+            }
+        }
+    }
+#pragma omp parallel
+    {
+        int *site = (int *)malloc((size_t)W * sizeof(int));
+        int64_t *F = (int64_t *)malloc((size_t)W * sizeof(int64_t));  /* g(q)^2 + q^2 of stacked site */
+#pragma omp for schedule(static)
+        for (int y = 0; y < H; ++y) {
+            const int64_t *gr = g + (size_t)y * W;
+            int k = -1;
+            for (int q = 0; q < W; ++q) {
+                if (gr[q] == NONE) continue;
+                const int64_t Fq = gr[q] * gr[q] + (int64_t)q * q;
+                /* pop the top while the new parabola's crossing with it lies at
+                 * or before the top's own crossing with the one below it:
+                 * (Fq - Fv)/(q - v) <= (Fv - Fu)/(v - u), denominators > 0 */
+                while (k >= 1) {
+                    const int64_t v = site[k], u = site[k - 1];
+                    if ((Fq - F[k]) * (v - u) <= (F[k] - F[k - 1]) * (q - v)) --k;
+                    else break;
+                }
+                ++k;
+                site[k] = q;
+                F[k] = Fq;
+            }
+            int64_t *out = d2 + (size_t)y * W;
+            if (k < 0) {
+                for (int x = 0; x < W; ++x) out[x] = NONE;
+                continue;
+            }
+            int j = 0;
+            for (int x = 0; x < W; ++x) {
+                /* advance while the next site's parabola is strictly lower at
+                 * x: crossing (F[j+1] - F[j]) / (2 (s[j+1] - s[j])) < x */
+                while (j < k && F[j + 1] - F[j] < 2 * (int64_t)x * (site[j + 1] - site[j])) ++j;
+                const int64_t dx = x - site[j];
+                out[x] = dx * dx + F[j] - (int64_t)site[j] * site[j];
+            }
+        }
+        free(F);
+        free(site);
+    }
+    free(g);
+}
+
+void rco_sdf_from_d2_f32(int W, int H, const int64_t *d2, float *sdf)
+{
+    for (size_t p = 0; p < (size_t)W * H; ++p) sdf[p] = d2[p] < 0 ? 2160.0f : (float)rco_sdf_value_d2((uint64_t)d2[p]);
+}
+
+void rco_sdf_from_d2_f64(int W, int H, const int64_t *d2, double *sdf)
+{
+    for (size_t p = 0; p < (size_t)W * H; ++p) sdf[p] = d2[p] < 0 ? 2160.0 : rco_sdf_value_d2((uint64_t)d2[p]);
+}
+
 /* Exact nearest seed under the same tie-break (checker for the jump flood). */
 void rco_nearest_seed_bruteforce(int W, int H, const uint32_t *seed_init, uint32_t *nearest)
 {

@@ -82,6 +82,11 @@ void     rco_sdf_f32(int W, int H, const uint32_t *seed, float *sdf);
 void     rco_sdf_f64(int W, int H, const uint32_t *seed, double *sdf);
 void     rco_nearest_seed_bruteforce(int W, int H, const uint32_t *seed_init,
                                      uint32_t *nearest);
+/* exact squared distance to the nearest occupied pixel (-1: none occupied),
+ * and the field the march reads from it                                  */
+void     rco_edt_d2(int W, int H, const uint32_t *seed_init, int64_t *d2);
+void     rco_sdf_from_d2_f32(int W, int H, const int64_t *d2, float *sdf);
+void     rco_sdf_from_d2_f64(int W, int H, const int64_t *d2, double *sdf);
 /* ground-truth irradiance by an exact per-pixel grid walk (rco.c) */
 void     rco_bruteforce_irradiance(int W, int H, const uint32_t *seed_init, const float *emissive, int n_dirs,
                                    double t_max, const int32_t *pixels, int n_pix, double *out);
@@ -103,12 +108,16 @@ void rco_gather_f64(int W, int H, const double *A0, double *irr);
 void rco_gather_f32(int W, int H, const float *A0, float *irr);
 
 /* Whole path from the G-buffer.  levels_out (optional, may be NULL): array of
- * L pointers, entry i receives A_i (size Py_i*Px_i*max(N_i/4,1)*4).  seeds_out
- * (optional) receives the final JFA field.                                  */
+ * L pointers, entry i receives A_i (size Py_i*Px_i*max(N_i/4,1)*4).
+ * sdf_mode: RCO_SDF_EXACT (the distance field from the exact transform) or
+ * RCO_SDF_JFA (from the jump flood).  seeds_out (optional, RCO_SDF_JFA only)
+ * receives the final JFA field.                                             */
+#define RCO_SDF_EXACT 0
+#define RCO_SDF_JFA   1
 int rco_run_f64(int W, int H, int L, const float *depth, const float *emissive,
-                double *irr, double **levels_out, uint32_t *seeds_out);
+                double *irr, double **levels_out, uint32_t *seeds_out, int sdf_mode);
 int rco_run_f32(int W, int H, int L, const float *depth, const float *emissive,
-                float *irr, float **levels_out, uint32_t *seeds_out);
+                float *irr, float **levels_out, uint32_t *seeds_out, int sdf_mode);
 
 /* OpenMP thread count for the f32/f64 level loops (cpu_baseline). */
 void rco_set_threads(int n);

@@ -68,6 +68,9 @@ def lib():
             "rco_sdf_f32": (None, [i32, i32, P, P]),
             "rco_sdf_f64": (None, [i32, i32, P, P]),
             "rco_nearest_seed_bruteforce": (None, [i32, i32, P, P]),
+            "rco_edt_d2": (None, [i32, i32, P, P]),
+            "rco_sdf_from_d2_f32": (None, [i32, i32, P, P]),
+            "rco_sdf_from_d2_f64": (None, [i32, i32, P, P]),
             "rco_bruteforce_irradiance": (None, [i32, i32, P, P, i32, ctypes.c_double, P, i32, P]),
             "rco_march_level_f64": (None, [i32, i32, i32, P, P, P, P]),
             "rco_march_level_f32": (None, [i32, i32, i32, P, P, P, P]),
@@ -75,8 +78,8 @@ def lib():
             "rco_merge_level_f32": (None, [i32, i32, i32, P, P, P]),
             "rco_gather_f64": (None, [i32, i32, P, P]),
             "rco_gather_f32": (None, [i32, i32, P, P]),
-            "rco_run_f64": (i32, [i32, i32, i32, P, P, P, P, P]),
-            "rco_run_f32": (i32, [i32, i32, i32, P, P, P, P, P]),
+            "rco_run_f64": (i32, [i32, i32, i32, P, P, P, P, P, i32]),
+            "rco_run_f32": (i32, [i32, i32, i32, P, P, P, P, P, i32]),
             "rco_set_threads": (None, [i32]),
             "rco_get_threads": (i32, []),
         }
@@ -257,6 +260,27 @@ def edt_d2_exact(seed_init_field: np.ndarray) -> np.ndarray:
 SDF_NONE_EXACT = 1e30
 
 
+def edt_d2(seed_init_field: np.ndarray) -> np.ndarray:
+    """int64 [H][W]: exact squared distance to the nearest occupied pixel, -1
+    where no pixel is occupied (rco_edt_d2: Felzenszwalb & Huttenlocher's
+    separable transform in exact integer arithmetic)."""
+    s = np.ascontiguousarray(seed_init_field, np.uint32)
+    H, W = s.shape
+    out = np.empty((H, W), np.int64)
+    lib().rco_edt_d2(W, H, _p(s), _p(out))
+    return out
+
+
+def sdf_from_d2(d2: np.ndarray, dtype=np.float32) -> np.ndarray:
+    """The distance values the march reads, from exact squared distances."""
+    d2 = np.ascontiguousarray(d2, np.int64)
+    H, W = d2.shape
+    out = np.empty((H, W), dtype)
+    fn = lib().rco_sdf_from_d2_f32 if np.dtype(dtype) == np.float32 else lib().rco_sdf_from_d2_f64
+    fn(W, H, _p(d2), _p(out))
+    return out
+
+
 def sdf_exact_f64(seed_init_field: np.ndarray) -> np.ndarray:
     """float64 [H][W]: the exact field sqrt(d^2) of the nearest occupied pixel
     (1e30 with none) -- the field the one-byte code approximates from below."""
@@ -318,8 +342,14 @@ def gather(W: int, H: int, A0: np.ndarray) -> np.ndarray:
     return irr
 
 
-def run(depth: np.ndarray, emissive: np.ndarray, L: int, dtype=np.float64, want_levels=False, want_seeds=False):
-    """Whole path.  Returns irr [3,H,W] (and levels list A_0..A_{L-1}, seeds)."""
+SDF_MODES = {"exact": 0, "jfa": 1}
+
+
+def run(depth: np.ndarray, emissive: np.ndarray, L: int, dtype=np.float64, want_levels=False, want_seeds=False,
+        sdf: str = "exact"):
+    """Whole path.  Returns irr [3,H,W] (and levels list A_0..A_{L-1}, seeds).
+    sdf: "exact" (the distance field of the exact transform) or "jfa" (of the
+    jump flood; the only mode with a seed field, want_seeds)."""
     H, W = depth.shape
     dtype = np.dtype(dtype)
     irr = np.empty((3, H, W), dtype)
@@ -333,8 +363,10 @@ def run(depth: np.ndarray, emissive: np.ndarray, L: int, dtype=np.float64, want_
         lv_ptrs = (ctypes.c_void_p * L)(*[a.ctypes.data for a in levels])
     seeds = np.empty((H, W), np.uint32) if want_seeds else None
     fn = getattr(lib(), "rco_run_" + _flavour(dtype))
+    if want_seeds and sdf != "jfa":
+        raise ValueError("want_seeds needs sdf='jfa'")
     r = fn(W, H, L, _p(np.ascontiguousarray(depth, np.float32)), _p(np.ascontiguousarray(emissive, np.float32)),
-           _p(irr), lv_ptrs, None if seeds is None else _p(seeds))
+           _p(irr), lv_ptrs, None if seeds is None else _p(seeds), SDF_MODES[sdf])
     if r != 0:
         raise ValueError("invalid run parameters")
     out = [irr]
@@ -348,8 +380,9 @@ def run(depth: np.ndarray, emissive: np.ndarray, L: int, dtype=np.float64, want_
 def run_on_field(sdf: np.ndarray, emissive: np.ndarray, L: int) -> np.ndarray:
     """March, merge and gather on a given distance field (in sdf's precision):
     the composition of the stage functions, top level first.  With the field
-    the jump flood produces this is rco_run; with sdf_exact_f64 it is the
-    exact-field flavour."""
+    the jump flood produces this is rco_run(sdf="jfa"); with sdf_from_d2 of the
+    exact transform it is rco_run(sdf="exact"); with sdf_exact_f64 it is the
+    real-valued exact-field flavour."""
     H, W = sdf.shape
     upper = None
     for i in range(L - 1, -1, -1):

@@ -50,6 +50,10 @@ class Config(ctypes.Structure):
 DEPTH_LINEAR = 0
 DEPTH_NDC = 1
 
+SDF_EXACT = 0
+SDF_JUMP_FLOOD = 1
+SDF_MODES = {"exact": SDF_EXACT, "jfa": SDF_JUMP_FLOOD}
+
 
 class Plane(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 16), ("channels", ctypes.c_uint32), ("height", ctypes.c_uint32),
@@ -131,6 +135,8 @@ SIGNATURES = {
     "rc_get_jfa_fuse": (_u32, [_P]),
     "rc_set_jfa_rows": (_i32, [_P, _u32]),
     "rc_get_jfa_rows": (_u32, [_P]),
+    "rc_set_sdf_mode": (_i32, [_P, _u32]),
+    "rc_get_sdf_mode": (_u32, [_P]),
     "rc_set_march": (_i32, [_P, _u32, _u32, _u32]),
     "rc_set_march_order": (_i32, [_P, _u32, _u32]),
     "rc_get_march_order": (_i32, [_P, _u32, ctypes.POINTER(_u32)]),

@@ -21,7 +21,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "librcgi.so")
 ARCH = os.environ.get("RC_OFFLOAD_ARCH", "gfx950")
-SOURCES = ["rc_scene.hip", "rc_jfa.hip", "rc_cascade.hip", "rc_norms.hip", "rc_mlp.hip", "rc_surrogate.hip",
+SOURCES = ["rc_scene.hip", "rc_jfa.hip", "rc_edt.hip", "rc_cascade.hip", "rc_norms.hip", "rc_mlp.hip", "rc_surrogate.hip",
            "rc_composite.hip", "rc_dump.cpp", "rc_api.hip", "rc_noalloc.hip", "rc_batch.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -1,11 +1,13 @@
 // rc_api.hip -- the C ABI (include/rc_gi.h): context, frame orchestration,
 // profiling, parity readbacks, geometry and scene helpers.
 //
-// A frame is 1 seed kernel + (P - 4) global jump-flood passes + 1 fused
-// tail (short passes + SDF) + (L - 1) level kernels + 1 level-0/gather
-// kernel, all stream ordered on the caller's stream.  The context owns the
-// jump-flood ping-pong buffers, the SDF, the merged levels 1..L-1 and the
-// per-level direction tables; it allocates them at create/resize time only.
+// A frame is the distance field -- by default the exact transform (an
+// occupancy kernel, a column kernel and a row kernel), or 1 seed kernel +
+// the global jump-flood passes (+ an optional fused LDS tail) -- then the
+// split march, (L - 2) level kernels and 1 level-0/gather kernel, all stream
+// ordered on the caller's stream.  The context owns the distance-field
+// buffers of its mode, the SDF, the merged levels 1..L-1 and the per-level
+// direction tables; it allocates them at create/resize/mode-change time only.
 #include <rocprofiler-sdk-roctx/roctx.h>
 #include <math.h>
 #include <stdio.h>
@@ -26,7 +28,10 @@ struct rc_ctx {
     int W = 0, H = 0, L = 0;
     Storage storage = Storage::F32;
     DepthConv depth_conv{};
-    uint32_t *seed[2] = {nullptr, nullptr};
+    int sdf_mode = RC_SDF_EXACT;
+    uint32_t *seed[2] = {nullptr, nullptr};  // jump flood: seed ping-pong
+    uint64_t *edt_bits = nullptr;            // exact transform: occupancy column words
+    uint16_t *edt_g = nullptr;               // exact transform: vertical distances
     uint8_t *sdf = nullptr;  // distance codes (rc_common.hpp)
     void *level[RC_MAX_LEVELS] = {};
     size_t level_texels[RC_MAX_LEVELS] = {};
@@ -155,6 +160,10 @@ static void release(rc_ctx *c)
 {
     drop_graph(c);
     for (auto &p : c->seed) { if (p) hipFree(p); p = nullptr; }
+    if (c->edt_bits) hipFree(c->edt_bits);
+    if (c->edt_g) hipFree(c->edt_g);
+    c->edt_bits = nullptr;
+    c->edt_g = nullptr;
     if (c->sdf) hipFree(c->sdf);
     c->sdf = nullptr;
     for (auto &p : c->level) { if (p) hipFree(p); p = nullptr; }
@@ -211,6 +220,21 @@ static void release(rc_ctx *c)
     c->ran = false;
 }
 
+// The distance-field buffers of the context's mode (the other mode's, if
+// allocated earlier, are kept until the next resize).
+static rc_status ensure_sdf_buffers(rc_ctx *c)
+{
+    const size_t n = (size_t)c->W * c->H;
+    if (c->sdf_mode == RC_SDF_JUMP_FLOOD) {
+        for (auto &p : c->seed)
+            if (!p) HIP_TRY(c, hipMalloc(&p, n * sizeof(uint32_t)));
+    } else {
+        if (!c->edt_bits) HIP_TRY(c, hipMalloc(&c->edt_bits, edt_bits_bytes(c->W, c->H)));
+        if (!c->edt_g) HIP_TRY(c, hipMalloc(&c->edt_g, edt_g_bytes(c->W, c->H)));
+    }
+    return RC_OK;
+}
+
 static rc_status allocate(rc_ctx *c, uint32_t W, uint32_t H, uint32_t L)
 {
     release(c);
@@ -218,8 +242,7 @@ static rc_status allocate(rc_ctx *c, uint32_t W, uint32_t H, uint32_t L)
     c->H = (int)H;
     c->L = (int)L;
     const size_t n = (size_t)W * H;
-    HIP_TRY(c, hipMalloc(&c->seed[0], n * sizeof(uint32_t)));
-    HIP_TRY(c, hipMalloc(&c->seed[1], n * sizeof(uint32_t)));
+    HIP_TRY_ST(ensure_sdf_buffers(c));
     HIP_TRY(c, hipMalloc(&c->sdf, sdf_elems((int)W, (int)H)));
     HIP_TRY(c, hipMalloc(&c->emis4, n * sizeof(float4)));
     const size_t tb = texel_bytes(c->storage);
@@ -394,13 +417,28 @@ static rc_status collect_profile(rc_ctx *c, hipStream_t s, int jfa_launches)
     return RC_OK;
 }
 
-// Occupancy seeds, jump flood and the distance field c->sdf: global passes
-// down to the fused LDS tail (if any); without a tail the last global pass
-// writes the field.  A 1x1 image has no pass.  With `gate` every kernel does
-// nothing when *gate == 0 (the learned frame's tile fallback).
+// The distance field c->sdf (and the interleaved emission c->emis4).  Exact
+// mode: the occupancy kernel (stage "seed"), then the column and row kernels
+// (stage "jfa", 2 launches).  Jump-flood mode: occupancy seeds, then global
+// passes down to the fused LDS tail (if any); without a tail the last global
+// pass writes the field, and a 1x1 image has no pass.  With `gate` every
+// kernel does nothing when *gate == 0 (the learned frame's tile fallback).
 static rc_status seeds_and_jfa(rc_ctx *c, const float *depth, const float *emissive, hipStream_t s, bool dbg,
                                const int *gate, StageTimer *tm, int *global_passes)
 {
+    if (c->sdf_mode == RC_SDF_EXACT) {
+        c->dbg_seed_buf = -1;
+        if (tm) tm->begin(RC_STAGE_SEED);
+        HIP_TRY(c, launch_edt(depth, c->depth_conv, emissive, c->edt_bits, c->edt_g, c->sdf, c->emis4, c->W, c->H, s,
+                              gate, 1));
+        if (tm) tm->end(RC_STAGE_SEED);
+        if (tm) tm->begin(RC_STAGE_JFA);
+        HIP_TRY(c, launch_edt(depth, c->depth_conv, emissive, c->edt_bits, c->edt_g, c->sdf, c->emis4, c->W, c->H, s,
+                              gate, 2));
+        if (tm) tm->end(RC_STAGE_JFA);
+        if (global_passes) *global_passes = 2;
+        return RC_OK;
+    }
     if (tm) tm->begin(RC_STAGE_SEED);
     HIP_TRY(c, launch_seed_init(depth, c->depth_conv, emissive, c->seed[0], c->emis4, c->W, c->H, s, gate));
     if (tm) tm->end(RC_STAGE_SEED);
@@ -1055,6 +1093,8 @@ rc_status rc_reset_stats(rc_ctx *c)
 rc_status rc_debug_seeds(rc_ctx *c, uint32_t *dev_out, void *stream)
 {
     if (!c || !dev_out) return fail(c, RC_ERR_NULL, "rc_debug_seeds: NULL argument");
+    if (c->sdf_mode != RC_SDF_JUMP_FLOOD)
+        return fail(c, RC_ERR_STATE, "rc_debug_seeds: the seed field exists in the jump-flood mode only");
     if (!c->ran || c->dbg_seed_buf < 0) return fail(c, RC_ERR_STATE, "rc_debug_seeds: needs a run with RC_FLAG_KEEP_DEBUG");
     HIP_TRY(c, hipMemcpyAsync(dev_out, c->seed[c->dbg_seed_buf], (size_t)c->W * c->H * 4, hipMemcpyDeviceToDevice,
                               (hipStream_t)stream));
@@ -1160,6 +1200,27 @@ rc_status rc_set_jfa_rows(rc_ctx *c, uint32_t rows)
 
 uint32_t rc_get_jfa_rows(const rc_ctx *c) { return c ? (uint32_t)c->jfa_rows : 0u; }
 
+rc_status rc_set_sdf_mode(rc_ctx *c, uint32_t mode)
+{
+    if (!c) return fail(nullptr, RC_ERR_NULL, "rc_set_sdf_mode: ctx is NULL");
+    if (mode != RC_SDF_EXACT && mode != RC_SDF_JUMP_FLOOD)
+        return fail(c, RC_ERR_INVALID, "rc_set_sdf_mode: mode must be RC_SDF_EXACT or RC_SDF_JUMP_FLOOD");
+    if ((int)mode == c->sdf_mode) return RC_OK;
+    HIP_TRY(c, hipSetDevice(c->cfg.device));
+    const int old = c->sdf_mode;
+    c->sdf_mode = (int)mode;
+    const rc_status st = ensure_sdf_buffers(c);
+    if (st != RC_OK) {
+        c->sdf_mode = old;
+        return st;
+    }
+    c->ran = false;  // the debug readbacks follow the next frame
+    drop_graph(c);
+    return RC_OK;
+}
+
+uint32_t rc_get_sdf_mode(const rc_ctx *c) { return c ? (uint32_t)c->sdf_mode : 0u; }
+
 rc_status rc_set_march(rc_ctx *c, uint32_t split, uint32_t speculation, uint32_t rays_per_lane)
 {
     if (!c) return fail(nullptr, RC_ERR_NULL, "rc_set_march: ctx is NULL");
@@ -1227,7 +1288,7 @@ rc_status rc_autotune(rc_ctx *c, const rc_gbuffer *g, float *irr, uint32_t iters
     c->cfg.flags = saved_flags;
     if (st != RC_OK) return st;
     hipEvent_t e0 = c->ev[0], e1 = c->ev[1];
-    {
+    if (c->sdf_mode == RC_SDF_JUMP_FLOOD) {
         float best = 1e30f;
         int best_fuse = c->jfa_fuse, best_rows = c->jfa_rows;
         const int saved = c->jfa_fuse, saved_rows = c->jfa_rows;

@@ -5,11 +5,9 @@
 // view.  Here the frames of all views run as a fixed handful of launches,
 // each covering every view that has work in that stage:
 //
-//   seeds      1 launch, every view
-//   jump flood one launch per global pass, right-aligned (a view with fewer
-//              passes joins the later launches, so every view's last global
-//              pass is in the last one), then one fused-tail launch per tail
-//              step in use (normally one)
+//   distance   3 launches, every view: occupancy words, vertical distances,
+//              row lower envelopes (the exact transform, a context's default;
+//              the row kernel's LDS is sized for the widest view)
 //   march      1 launch of (view, level >= 1) jobs, top level first
 //   levels     one launch per level index i, top down, over the views with
 //              more than i levels (each view's top level has no upper level)
@@ -19,7 +17,7 @@
 // so each view's irradiance equals its single-view frame bit for bit.  The
 // views share one workspace allocation; the job tables are built on the host
 // for the current tuning and uploaded once, and the launches are captured in
-// one hipGraph, so a batch is one graph launch of ~20 kernel nodes whatever
+// one hipGraph, so a batch is one graph launch of ~10 kernel nodes whatever
 // the number of views.
 #include <algorithm>
 #include <cmath>
@@ -33,18 +31,18 @@ namespace rc {
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-enum WlKind { WL_SEED, WL_JFA, WL_TAIL, WL_MARCH, WL_LEVEL, WL_LEVEL0 };
+enum WlKind { WL_OCC, WL_COLS, WL_ROWS, WL_MARCH, WL_LEVEL, WL_LEVEL0 };
 
 struct WlView {
     int W, H, L;
     const float *depth, *emissive;
     float *irr;
-    size_t seed[2], sdf, emis4, codes[RC_MAX_LEVELS], level[RC_MAX_LEVELS];
+    size_t bits, g, sdf, emis4, codes[RC_MAX_LEVELS], level[RC_MAX_LEVELS];
 };
 
 struct WlLaunch {
     WlKind kind;
-    int arg;               // tail step (WL_TAIL) or level (WL_LEVEL)
+    int arg;               // LDS row capacity (WL_ROWS) or level (WL_LEVEL)
     size_t jobs, begins;   // byte offsets into the table buffer
     int njobs, blocks;
 };
@@ -57,7 +55,7 @@ struct WorkList {
     uint8_t *ws = nullptr;
     float2 *dirs = nullptr;
     // tuning
-    int fuse = 4, split = 1, spec = 1, m = 1;
+    int split = 1, spec = 1, m = 1;
     int tile[RC_MAX_LEVELS] = {}, rpt[RC_MAX_LEVELS] = {}, mmap[RC_MAX_LEVELS] = {};
     // launches of the current tuning
     std::vector<WlLaunch> launches;
@@ -152,56 +150,27 @@ static hipError_t build_tables(WorkList &w, std::string &err)
     const int n = (int)w.v.size();
     bool ok = true;
 
-    {  // occupancy seeds
-        std::vector<SeedJob> j;
-        std::vector<int64_t> b;
+    {  // exact distance transform: occupancy words, vertical distances, rows
+        std::vector<OccJob> jo;
+        std::vector<ColJob> jc;
+        std::vector<RowJob> jr;
+        std::vector<int64_t> bo, bc, br;
+        int wmax = 0;
         for (const WlView &x : w.v) {
-            j.push_back(SeedJob{x.depth, x.emissive, (uint32_t *)(w.ws + x.seed[0]), (float4 *)(w.ws + x.emis4),
-                                DepthConv{}, x.W, x.H, seed_bx(x.W)});
-            b.push_back(seed_blocks(x.W, x.H));
+            uint64_t *bits = (uint64_t *)(w.ws + x.bits);
+            uint16_t *g = (uint16_t *)(w.ws + x.g);
+            jo.push_back(OccJob{x.depth, x.emissive, bits, (float4 *)(w.ws + x.emis4), DepthConv{}, x.W, x.H,
+                                occ_bx(x.W)});
+            bo.push_back(occ_blocks(x.W, x.H));
+            jc.push_back(ColJob{bits, g, x.W, x.H, occ_bx(x.W)});
+            bc.push_back(cols_blocks(x.W, x.H));
+            jr.push_back(RowJob{g, w.ws + x.sdf, x.W});
+            br.push_back(x.H);
+            wmax = std::max(wmax, edt_row_wmax(x.W));
         }
-        ok = ok && add_launch(w, blob, WL_SEED, 0, j, b, kBlock, err);
-    }
-
-    // jump flood: each view's own global steps, right-aligned across views
-    std::vector<std::vector<int>> steps(n);
-    std::vector<int> tail(n, 0), cur(n, 0);
-    size_t G = 0;
-    for (int v = 0; v < n; ++v) {
-        const int P = jfa_num_passes(w.v[v].W, w.v[v].H);
-        tail[v] = jfa_tail_step(w.v[v].W, w.v[v].H, w.fuse);
-        if (P == 0) steps[v].push_back(0);  // a 1x1 view: one step-0 pass writes its field
-        for (int s = P > 0 ? 1 << (P - 1) : 0; s >= 1 && s > tail[v]; s >>= 1) steps[v].push_back(s);
-        G = std::max(G, steps[v].size());
-    }
-    for (size_t g = 0; g < G && ok; ++g) {
-        std::vector<JfaJob> j;
-        std::vector<int64_t> b;
-        for (int v = 0; v < n; ++v) {
-            const size_t cnt = steps[v].size();
-            if (g + cnt < G) continue;
-            const size_t k = g + cnt - G;
-            const WlView &x = w.v[v];
-            const bool fin = tail[v] == 0 && k + 1 == cnt;
-            uint32_t *in = (uint32_t *)(w.ws + x.seed[cur[v]]);
-            uint32_t *out = fin ? nullptr : (uint32_t *)(w.ws + x.seed[cur[v] ^ 1]);
-            j.push_back(JfaJob{in, out, w.ws + x.sdf, x.W, x.H, steps[v][k], jfa_pass_bx(x.W), fin ? 1 : 0});
-            b.push_back(jfa_pass_blocks(x.W, x.H));
-            if (!fin) cur[v] ^= 1;
-        }
-        ok = ok && add_launch(w, blob, WL_JFA, (int)g, j, b, kBlock, err);
-    }
-    for (int S = 1; S <= 8 && ok; S <<= 1) {
-        std::vector<TailJob> j;
-        std::vector<int64_t> b;
-        for (int v = 0; v < n; ++v) {
-            if (tail[v] != S) continue;
-            const WlView &x = w.v[v];
-            j.push_back(TailJob{(const uint32_t *)(w.ws + x.seed[cur[v]]), nullptr, w.ws + x.sdf, x.W, x.H,
-                                jfa_tail_bx(x.W, S)});
-            b.push_back(jfa_tail_blocks(x.W, x.H, S));
-        }
-        ok = ok && add_launch(w, blob, WL_TAIL, S, j, b, 1024, err);
+        ok = ok && add_launch(w, blob, WL_OCC, 0, jo, bo, 1024, err);
+        ok = ok && add_launch(w, blob, WL_COLS, 0, jc, bc, kBlock, err);
+        ok = ok && add_launch(w, blob, WL_ROWS, wmax, jr, br, 64, err);  // one wave per row
     }
 
     if (w.split && ok) {  // split march: every (view, level >= 1), top level first
@@ -281,9 +250,9 @@ static hipError_t enqueue_launch(const WorkList &w, const WlLaunch &L, hipStream
 {
     const BatchLaunch b{w.tables + L.jobs, (const int *)(w.tables + L.begins), L.njobs, L.blocks};
     switch (L.kind) {
-    case WL_SEED: return launch_seed_batch(b, s);
-    case WL_JFA: return launch_jfa_batch(b, s);
-    case WL_TAIL: return launch_tail_batch(b, L.arg, s);
+    case WL_OCC: return launch_occ_batch(b, s);
+    case WL_COLS: return launch_cols_batch(b, s);
+    case WL_ROWS: return launch_rows_batch(b, L.arg, s);
     case WL_MARCH: return launch_march_batch(b, w.spec, w.m, s);
     case WL_LEVEL: return launch_level_batch(b, w.st, w.tile[L.arg], w.rpt[L.arg], w.split != 0, s);
     case WL_LEVEL0: return launch_level0_batch(b, w.st, s);
@@ -336,10 +305,10 @@ hipError_t worklist_create(const rc_view *views, uint32_t n, const uint32_t *lev
         x.emissive = views[i].emissive;
         x.irr = views[i].irradiance;
         const size_t px = (size_t)x.W * x.H;
-        x.seed[0] = off;
-        off = align256(off + 4 * px);
-        x.seed[1] = off;
-        off = align256(off + 4 * px);
+        x.bits = off;
+        off = align256(off + edt_bits_bytes(x.W, x.H));
+        x.g = off;
+        off = align256(off + edt_g_bytes(x.W, x.H));
         x.sdf = off;
         off = align256(off + sdf_elems(x.W, x.H));
         x.emis4 = off;
@@ -431,9 +400,8 @@ static hipError_t time_launches(WorkList &w, hipStream_t s, int iters, Pick pick
     return hipSuccess;
 }
 
-// Stage by stage: the jump-flood tail depth, the march's speculation depth,
-// rays per lane and per-level lane order, then per level the tile shape and
-// probes per thread,
+// Stage by stage: the march's speculation depth, rays per lane and per-level
+// lane order, then per level the tile shape and probes per thread,
 // for the split march and for marching in the level kernels; the split form
 // is kept if its march plus levels is faster.
 hipError_t worklist_autotune(WorkList *w, int iters, hipStream_t s, std::string &err)
@@ -447,15 +415,6 @@ hipError_t worklist_autotune(WorkList *w, int iters, hipStream_t s, std::string 
         return hipSuccess;
     };
     float best = 1e30f, ms = 0.f;
-    int best_fuse = x.fuse;
-    for (int f : {0, 1, 2, 4, 8}) {
-        x.fuse = f;
-        WL_TRY(frame());
-        WL_TRY(time_launches(x, s, iters, [](const WlLaunch &L) { return L.kind == WL_JFA || L.kind == WL_TAIL; }, ms,
-                             err));
-        if (ms < best) best = ms, best_fuse = f;
-    }
-    x.fuse = best_fuse;
 
     auto tune_levels = [&](float &total) -> hipError_t {
         total = 0.f;

@@ -24,6 +24,51 @@ struct DepthConv {
 };
 __device__ inline float linear_depth(const DepthConv &c, float d) { return c.ndc ? c.nf / (c.f - d * c.fmn) : d; }
 
+// Occupancy of pixels (x, y0 + j), j < 4, x < W: any emissive channel > 0, or
+// a 4-neighbour q with D(q) - D(p) > 0.05 D(p) (the near side of a depth
+// silhouette), in fp32.  All loads are unconditional at clamped
+// addresses, so they are in flight together; the depth above and below the
+// middle rows are the lane's own loads (two extra rows).  occ[j] is false for
+// rows at or beyond H; e[j] is the pixel's emission (r, g, b, 0).
+__device__ __forceinline__ void occupancy4(const float *__restrict__ depth, const DepthConv &dc,
+                                           const float *__restrict__ emissive, int W, int H, int x, int y0,
+                                           bool occ[4], float4 e[4])
+{
+    const size_t plane = (size_t)W * H;
+    float du0 = depth[(size_t)max(y0 - 1, 0) * W + x];
+    float dd3 = depth[(size_t)min(y0 + 4, H - 1) * W + x];
+    float dp[4], dl[4], dr[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const size_t p = (size_t)min(y0 + j, H - 1) * W + x;
+        e[j] = make_float4(emissive[p], emissive[plane + p], emissive[2 * plane + p], 0.0f);
+        dp[j] = depth[p];
+        dl[j] = depth[p - (x > 0)];
+        dr[j] = depth[p + (x + 1 < W)];
+    }
+    if (dc.ndc) {
+        du0 = linear_depth(dc, du0);
+        dd3 = linear_depth(dc, dd3);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            dp[j] = linear_depth(dc, dp[j]);
+            dl[j] = linear_depth(dc, dl[j]);
+            dr[j] = linear_depth(dc, dr[j]);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int y = y0 + j;
+        // rows y - 1 and y + 1: this lane's neighbouring rows or the two extra
+        // loads (where y + 1 < H, row y + 1 is not clamped)
+        const float du = j == 0 ? du0 : dp[j - 1], dd = j == 3 ? dd3 : dp[j + 1];
+        const float thr = 0.05f * dp[j];
+        occ[j] = y < H && (e[j].x > 0.0f || e[j].y > 0.0f || e[j].z > 0.0f || (x > 0 && dl[j] - dp[j] > thr) ||
+                           (x + 1 < W && dr[j] - dp[j] > thr) || (y > 0 && du - dp[j] > thr) ||
+                           (y + 1 < H && dd - dp[j] > thr));
+    }
+}
+
 // ---- occupancy + jump flood (rc_jfa.hip) --------------------------------------
 int jfa_num_passes(int W, int H);
 // largest step fused into the LDS tail for a requested fuse limit (0 = none)
@@ -38,6 +83,16 @@ hipError_t launch_jfa_pass(const uint32_t *in, uint32_t *out, uint8_t *sdf, int 
                            hipStream_t s, const int *gate = nullptr, int rows = 1);
 hipError_t launch_jfa_tail(const uint32_t *in, uint32_t *seeds_out, uint8_t *sdf, int W, int H, int tail_step,
                            hipStream_t s, const int *gate = nullptr);
+
+// ---- exact distance transform (rc_edt.hip) ------------------------------------
+// Buffers: bits = column words [ceil(H/64)][W] (u64), g = vertical distances
+// [H][W] (u16).  stage: 1 = the occupancy kernel (bits + emis4), 2 = the
+// column and row kernels (g, then the SDF), 3 = all three.
+size_t edt_bits_bytes(int W, int H);
+size_t edt_g_bytes(int W, int H);
+hipError_t launch_edt(const float *depth, const DepthConv &dc, const float *emissive, uint64_t *bits, uint16_t *g,
+                      uint8_t *sdf, float4 *emis4, int W, int H, hipStream_t s, const int *gate = nullptr,
+                      int stage = 3);
 
 // ---- work lists of views (one launch per stage over many views) --------------
 // A batched launch runs one kernel over the jobs of many views.  Job j owns
@@ -68,34 +123,30 @@ __device__ inline int batch_job(const int *__restrict__ begins, int njobs, int b
     return __builtin_amdgcn_readfirstlane(lo);
 }
 
-struct SeedJob {
+struct OccJob {
     const float *depth, *emissive;
-    uint32_t *seed;
+    uint64_t *bits;
     float4 *emis4;
     DepthConv dc;
     int W, H, bx_count;
 };
-struct JfaJob {
-    const uint32_t *in;
-    uint32_t *out;           // next seed field (nullptr on a view's last pass)
-    uint8_t *sdf;
-    int W, H, step, bx_count, final_pass;
-};
-struct TailJob {
-    const uint32_t *in;
-    uint32_t *seeds_out;
-    uint8_t *sdf;
+struct ColJob {
+    const uint64_t *bits;
+    uint16_t *g;
     int W, H, bx_count;
 };
-int seed_blocks(int W, int H);
-int seed_bx(int W);
-int jfa_pass_blocks(int W, int H);
-int jfa_pass_bx(int W);
-int jfa_tail_blocks(int W, int H, int S);
-int jfa_tail_bx(int W, int S);
-hipError_t launch_seed_batch(const BatchLaunch &b, hipStream_t s);
-hipError_t launch_jfa_batch(const BatchLaunch &b, hipStream_t s);
-hipError_t launch_tail_batch(const BatchLaunch &b, int S, hipStream_t s);
+struct RowJob {
+    const uint16_t *g;
+    uint8_t *sdf;
+    int W;
+};
+int occ_blocks(int W, int H);   // 1024-lane workgroups of the occupancy kernel
+int occ_bx(int W);
+int cols_blocks(int W, int H);  // 256-lane workgroups of the column kernel
+int edt_row_wmax(int W);        // LDS row capacity of the row kernel (1024 .. 16384)
+hipError_t launch_occ_batch(const BatchLaunch &b, hipStream_t s);
+hipError_t launch_cols_batch(const BatchLaunch &b, hipStream_t s);
+hipError_t launch_rows_batch(const BatchLaunch &b, int wmax, hipStream_t s);  // one workgroup per row
 
 // ---- cascades (rc_cascade.hip) ----------------------------------------------
 // Workgroup tile of a level kernel: TX x TY probes x KD directions, TX*TY*KD = 256.

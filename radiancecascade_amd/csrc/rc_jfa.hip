@@ -63,12 +63,8 @@ int jfa_tail_step(int W, int H, int fuse)
     return top < fuse ? top : fuse;
 }
 
-// Occupancy of pixel (x, y): any emissive channel > 0, or a 4-neighbour q
-// with D(q) - D(p) > 0.05 D(p).  All loads are unconditional (clamped
-// addresses) so they are in flight together.
 // 64 x 16 pixel blocks: wave w covers rows 4w..4w+3 of 64 consecutive
-// pixels; a thread's 4 pixels have their loads in flight at once, and the
-// depth above and below its middle rows are its own loads (two extra rows).
+// pixels (occupancy4: a thread's 4 pixels have their loads in flight at once).
 constexpr int kSeedRows = 16;
 
 // With emis4 set, also writes the emission interleaved (r, g, b, 0) of every
@@ -83,44 +79,16 @@ __device__ __forceinline__ void seed_init_body(const float *__restrict__ depth, 
     const int x = (b % bx_count) * 64 + (threadIdx.x & 63);
     const int y0 = (b / bx_count) * kSeedRows + 4 * (threadIdx.x >> 6);
     if (x >= W) return;
-    const size_t plane = (size_t)W * H;
-    // all loads first (clamped addresses; out-of-image rows are masked below)
-    float du0 = depth[(size_t)max(y0 - 1, 0) * W + x];
-    float dd3 = depth[(size_t)min(y0 + 4, H - 1) * W + x];
+    bool occ[4];
     float4 e[4];
-    float dp[4], dl[4], dr[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const size_t p = (size_t)min(y0 + j, H - 1) * W + x;
-        e[j] = make_float4(emissive[p], emissive[plane + p], emissive[2 * plane + p], 0.0f);
-        dp[j] = depth[p];
-        dl[j] = depth[p - (x > 0)];
-        dr[j] = depth[p + (x + 1 < W)];
-    }
-    if (dc.ndc) {
-        du0 = linear_depth(dc, du0);
-        dd3 = linear_depth(dc, dd3);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            dp[j] = linear_depth(dc, dp[j]);
-            dl[j] = linear_depth(dc, dl[j]);
-            dr[j] = linear_depth(dc, dr[j]);
-        }
-    }
+    occupancy4(depth, dc, emissive, W, H, x, y0, occ, e);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const int y = y0 + j;
         if (y >= H) break;
-        // rows y - 1 and y + 1: this lane's neighbouring rows or the two extra
-        // loads (where y + 1 < H, row y + 1 is not clamped)
-        const float du = j == 0 ? du0 : dp[j - 1], dd = j == 3 ? dd3 : dp[j + 1];
-        const float thr = 0.05f * dp[j];
-        const bool occ = e[j].x > 0.0f || e[j].y > 0.0f || e[j].z > 0.0f || (x > 0 && dl[j] - dp[j] > thr) ||
-                         (x + 1 < W && dr[j] - dp[j] > thr) || (y > 0 && du - dp[j] > thr) ||
-                         (y + 1 < H && dd - dp[j] > thr);
         const size_t p = (size_t)y * W + x;
-        seed[p] = occ ? (((uint32_t)y << 16) | (uint32_t)x) : kFarSeed;
-        if (emis4 && occ) emis4[p] = e[j];
+        seed[p] = occ[j] ? (((uint32_t)y << 16) | (uint32_t)x) : kFarSeed;
+        if (emis4 && occ[j]) emis4[p] = e[j];
     }
 }
 
@@ -131,17 +99,6 @@ __global__ __launch_bounds__(kBlock) void k_seed_init(const float *__restrict__ 
 {
     if (gate && *gate == 0) return;
     seed_init_body(depth, dc, emissive, seed, emis4, W, H, bx_count, xcd_band(blockIdx.x, gridDim.x));
-}
-
-// The same over a work list of views: logical block b (XCD-banded over the
-// whole grid) belongs to the job whose block range holds it.
-__global__ __launch_bounds__(kBlock) void k_seed_batch(const SeedJob *__restrict__ jobs,
-                                                       const int *__restrict__ begins, int njobs)
-{
-    const int b = xcd_band(blockIdx.x, gridDim.x);
-    const int j = batch_job(begins, njobs, b);
-    const SeedJob &J = jobs[j];
-    seed_init_body(J.depth, J.dc, J.emissive, J.seed, J.emis4, J.W, J.H, J.bx_count, b - begins[j]);
 }
 
 // One global pass.  A wave owns 64 consecutive pixels of R consecutive rows
@@ -221,19 +178,6 @@ __global__ __launch_bounds__(kBlock) void k_jfa_pass(const uint32_t *__restrict_
 {
     if (gate && *gate == 0) return;
     jfa_pass_body<FINAL, R>(in, out, sdf, W, H, step, bx_count, xcd_band(blockIdx.x, gridDim.x));
-}
-
-// One pass over a work list of views; each job has its own step (a view
-// takes the passes its size needs) and says whether this is its last pass
-// (step 0 leaves every pixel its own seed: the field of a 1x1 view).
-__global__ __launch_bounds__(kBlock) void k_jfa_batch(const JfaJob *__restrict__ jobs, const int *__restrict__ begins,
-                                                      int njobs)
-{
-    const int b = xcd_band(blockIdx.x, gridDim.x);
-    const int j = batch_job(begins, njobs, b);
-    const JfaJob &J = jobs[j];
-    if (J.final_pass) jfa_pass_body<true, 1>(J.in, J.out, J.sdf, J.W, J.H, J.step, J.bx_count, b - begins[j]);
-    else jfa_pass_body<false, 1>(J.in, J.out, J.sdf, J.W, J.H, J.step, J.bx_count, b - begins[j]);
 }
 
 // Fused passes S, S/2, ..., 1 over a T x T output tile, loaded with a halo of
@@ -343,16 +287,6 @@ __global__ __launch_bounds__(kTailBlock) void k_jfa_tail(const uint32_t *__restr
     jfa_tail_body<S>(in, seeds_out, sdf, W, H, bx_count, xcd_band(blockIdx.x, gridDim.x));
 }
 
-template <int S>
-__global__ __launch_bounds__(kTailBlock) void k_tail_batch(const TailJob *__restrict__ jobs,
-                                                         const int *__restrict__ begins, int njobs)
-{
-    const int b = xcd_band(blockIdx.x, gridDim.x);
-    const int j = batch_job(begins, njobs, b);
-    const TailJob &J = jobs[j];
-    jfa_tail_body<S>(J.in, J.seeds_out, J.sdf, J.W, J.H, J.bx_count, b - begins[j]);
-}
-
 // An image with no pass at all (1x1): only the SDF conversion.
 __global__ void k_jfa_none(const uint32_t *__restrict__ in, uint32_t *__restrict__ seeds_out,
                            uint8_t *__restrict__ sdf, int W, int n, const int *gate)
@@ -433,51 +367,6 @@ hipError_t launch_jfa_tail(const uint32_t *in, uint32_t *seeds_out, uint8_t *sdf
     case 2: launch_tail<2>(in, seeds_out, sdf, W, H, s, gate); break;
     case 4: launch_tail<4>(in, seeds_out, sdf, W, H, s, gate); break;
     case 8: launch_tail<8>(in, seeds_out, sdf, W, H, s, gate); break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
-
-int seed_blocks(int W, int H) { return ((W + 63) / 64) * ((H + kSeedRows - 1) / kSeedRows); }
-int seed_bx(int W) { return (W + 63) / 64; }
-int jfa_pass_blocks(int W, int H) { return ((W + 63) / 64) * ((H + 3) / 4); }
-int jfa_pass_bx(int W) { return (W + 63) / 64; }
-
-int jfa_tail_blocks(int W, int H, int S)
-{
-    const int T = 64 - 2 * (2 * S - 1);
-    return ((W + T - 1) / T) * ((H + T - 1) / T);
-}
-
-int jfa_tail_bx(int W, int S)
-{
-    const int T = 64 - 2 * (2 * S - 1);
-    return (W + T - 1) / T;
-}
-
-hipError_t launch_seed_batch(const BatchLaunch &b, hipStream_t s)
-{
-    if (b.blocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_seed_batch, dim3(b.blocks), dim3(kBlock), 0, s, (const SeedJob *)b.jobs, b.begins, b.njobs);
-    return hipGetLastError();
-}
-
-hipError_t launch_jfa_batch(const BatchLaunch &b, hipStream_t s)
-{
-    if (b.blocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_jfa_batch, dim3(b.blocks), dim3(kBlock), 0, s, (const JfaJob *)b.jobs, b.begins, b.njobs);
-    return hipGetLastError();
-}
-
-hipError_t launch_tail_batch(const BatchLaunch &b, int S, hipStream_t s)
-{
-    if (b.blocks <= 0) return hipSuccess;
-    const TailJob *j = (const TailJob *)b.jobs;
-    switch (S) {
-    case 1: hipLaunchKernelGGL(k_tail_batch<1>, dim3(b.blocks), dim3(kTailBlock), 0, s, j, b.begins, b.njobs); break;
-    case 2: hipLaunchKernelGGL(k_tail_batch<2>, dim3(b.blocks), dim3(kTailBlock), 0, s, j, b.begins, b.njobs); break;
-    case 4: hipLaunchKernelGGL(k_tail_batch<4>, dim3(b.blocks), dim3(kTailBlock), 0, s, j, b.begins, b.njobs); break;
-    case 8: hipLaunchKernelGGL(k_tail_batch<8>, dim3(b.blocks), dim3(kTailBlock), 0, s, j, b.begins, b.njobs); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

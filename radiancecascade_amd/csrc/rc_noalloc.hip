@@ -3,17 +3,19 @@
 // callers that cannot allocate (code under hipGraph capture, tight serving
 // loops, harnesses that forbid hipMalloc).
 //
-// The frame's working buffers -- the jump-flood ping-pong, the distance
-// field and the merged levels 1..L-1 -- live in one file-scope device array,
+// The frame's working buffers -- the exact transform's occupancy words and
+// vertical distances, the distance field and the merged levels 1..L-1 --
+// live in one file-scope device array,
 // `scratch`, of kNoAllocScratchBytes; the per-level direction tables in a
 // second one, `scratch_dirs`.  Each GPU the library runs on holds its own copy
 // of both until the process ends.  In host code the arrays' names are
 // host-side stand-ins, not device addresses: the first call on a device (with
 // that device current) resolves their device addresses with
 // hipGetSymbolAddress, once per device, and every launch uses those.  The
-// kernels are the library's own (seed init, global jump-flood passes, the
-// level kernels marching in place, level 0 + gather), so the irradiance
-// equals rc_run_device's bit for bit.
+// kernels are the library's own (the exact transform's occupancy, column and
+// row kernels -- the default distance field of a context -- the level kernels
+// marching in place, level 0 + gather), so the irradiance equals
+// rc_run_device's bit for bit.
 //
 // Every call on a GPU shares that GPU's scratch: calls that overlap on one GPU
 // (two streams, two threads) race and give wrong results.  Nothing carried
@@ -29,7 +31,7 @@
 
 namespace rc {
 
-// 64 MiB: enough for a 1920x1080 frame with 6 levels (60.3 MB).
+// 64 MiB: enough for a 1920x1080 frame with 6 levels (48.1 MB).
 constexpr size_t kNoAllocScratchBytes = 64ull << 20;
 constexpr int kNoAllocMaxLevels = 7;
 constexpr int kNoAllocDirs = 4 * ((1 << (2 * kNoAllocMaxLevels)) - 1) / 3;  // sum of N_i, i < 7
@@ -47,7 +49,7 @@ static NoAllocDevice g_noalloc[kNoAllocMaxDevices];
 static std::mutex g_noalloc_mu;
 
 struct NoAllocLayout {
-    size_t seed[2], sdf, level[RC_MAX_LEVELS], total;
+    size_t bits, g, sdf, level[RC_MAX_LEVELS], total;
 };
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -55,12 +57,11 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 static NoAllocLayout noalloc_layout(int W, int H, int L)
 {
     NoAllocLayout m{};
-    const size_t n = (size_t)W * H;
     size_t off = 0;
-    m.seed[0] = off;
-    off = align256(off + 4 * n);
-    m.seed[1] = off;
-    off = align256(off + 4 * n);
+    m.bits = off;
+    off = align256(off + edt_bits_bytes(W, H));
+    m.g = off;
+    off = align256(off + edt_g_bytes(W, H));
     m.sdf = off;
     off = align256(off + sdf_elems(W, H));
     for (int i = 1; i < L; ++i) {
@@ -153,7 +154,6 @@ rc_status rc_run_device_noalloc(uint32_t width, uint32_t height, uint32_t levels
         return nfail(RC_ERR_HIP, std::string("rc_run_device_noalloc: set-up: ") + hipGetErrorString(e));
 
     const hipStream_t s = (hipStream_t)stream;
-    uint32_t *seed[2] = {(uint32_t *)(d->scratch + lay.seed[0]), (uint32_t *)(d->scratch + lay.seed[1])};
     uint8_t *sdf = d->scratch + lay.sdf;
     const DepthConv linear{};
 #define NTRY(expr)                                                                                              \
@@ -162,15 +162,8 @@ rc_status rc_run_device_noalloc(uint32_t width, uint32_t height, uint32_t levels
         if (e_ != hipSuccess) return nfail(RC_ERR_HIP, std::string("rc_run_device_noalloc: ") + #expr + ": " + \
                                                            hipGetErrorString(e_));                             \
     } while (0)
-    NTRY(launch_seed_init(g->depth, linear, g->emissive, seed[0], nullptr, W, H, s));
-    const int P = jfa_num_passes(W, H);
-    int cur = 0;
-    for (int step = P > 0 ? 1 << (P - 1) : 0; step >= 1; step >>= 1) {
-        const bool final_pass = step == 1;
-        NTRY(launch_jfa_pass(seed[cur], final_pass ? nullptr : seed[cur ^ 1], sdf, W, H, step, final_pass, s));
-        cur ^= 1;
-    }
-    if (P == 0) NTRY(launch_jfa_tail(seed[0], nullptr, sdf, W, H, 0, s));
+    NTRY(launch_edt(g->depth, linear, g->emissive, (uint64_t *)(d->scratch + lay.bits),
+                    (uint16_t *)(d->scratch + lay.g), sdf, nullptr, W, H, s));
     for (int i = L - 1; i >= 0; --i) {
         LevelArgs a{};
         a.sdf = sdf;

@@ -158,6 +158,15 @@ class RadianceCascadeStage:
             out[i] = int(v.value)
         return out
 
+    def set_sdf_mode(self, mode: str) -> None:
+        """Distance transform: "exact" (the default; occupancy, column and row
+        kernels) or "jfa" (the jump flood, whose seed field debug_seeds reads)."""
+        check(self._lib.rc_set_sdf_mode(self._ctx, _abi.SDF_MODES[mode]), self._ctx)
+
+    def sdf_mode(self) -> str:
+        v = self._lib.rc_get_sdf_mode(self._ctx)
+        return {k: n for n, k in _abi.SDF_MODES.items()}[v]
+
     def set_jfa_fuse(self, fused_step: int) -> None:
         check(self._lib.rc_set_jfa_fuse(self._ctx, fused_step), self._ctx)
 

@@ -1,5 +1,5 @@
 /* The oracle's whole path under AddressSanitizer / UndefinedBehaviorSanitizer
- * (host only): scenes, raster, jump flood, both flavours of the cascades with
+ * (host only): scenes, raster, jump flood, exact transform, both flavours of the cascades with
  * every level and the seed field read back, on ragged and degenerate shapes.
  * Exits non-zero on a failed invariant; the sanitizers abort on any memory
  * or UB error. */
@@ -30,8 +30,8 @@ static int check_shape(int W, int H, int L, uint64_t seed)
         lv64[i] = malloc((size_t)px * py * k * 4 * sizeof(double));
         lv32[i] = malloc((size_t)px * py * k * 4 * sizeof(float));
     }
-    int bad = rco_run_f64(W, H, L, depth, em, irr64, lv64, seeds) != 0;
-    bad |= rco_run_f32(W, H, L, depth, em, irr32, lv32, NULL) != 0;
+    int bad = rco_run_f64(W, H, L, depth, em, irr64, lv64, seeds, RCO_SDF_JFA) != 0;
+    bad |= rco_run_f32(W, H, L, depth, em, irr32, lv32, NULL, RCO_SDF_JFA) != 0;
     for (size_t i = 0; i < 3 * n && !bad; ++i)
         if (!isfinite(irr64[i]) || irr64[i] < 0 || fabs(irr64[i] - irr32[i]) > 1e-2 * (1 + fabs(irr64[i]))) bad = 1;
     /* the jump flood never finds a nearer seed than the exact search */
@@ -47,6 +47,20 @@ static int check_shape(int W, int H, int L, uint64_t seed)
         const int64_t dn = nx * nx + ny * ny, dj = jx * jx + jy * jy;
         if (dj < dn) bad = 1;
     }
+    /* the exact transform equals the brute-force nearest distance, and the
+     * exact-field frame is finite */
+    int64_t *d2 = malloc(n * sizeof(int64_t));
+    rco_edt_d2(W, H, init, d2);
+    for (size_t p = 0; p < n && !bad; ++p) {
+        if (near[p] == 0xFFFFFFFFu) { if (d2[p] != -1) bad = 1; continue; }
+        const int x = (int)(p % W), y = (int)(p / W);
+        const int64_t nx = (int64_t)(near[p] & 0xFFFF) - x, ny = (int64_t)(near[p] >> 16) - y;
+        if (d2[p] != nx * nx + ny * ny) bad = 1;
+    }
+    bad |= rco_run_f32(W, H, L, depth, em, irr32, lv32, NULL, RCO_SDF_EXACT) != 0;
+    for (size_t i = 0; i < 3 * n && !bad; ++i)
+        if (!isfinite(irr32[i]) || irr32[i] < 0) bad = 1;
+    free(d2);
     for (int i = 0; i < L; ++i) {
         free(lv64[i]);
         free(lv32[i]);

@@ -69,11 +69,12 @@ def test_batch_matches_single_views(rc, mode):
         assert [b.levels(j) for j in range(len(SIZES))] == [rc.levels_for_coverage(W, H) for W, H in SIZES]
         info = b.info()
         if mode == "worklist":
-            # seeds + global jump-flood passes + tail + march + levels + level 0
-            assert 10 <= info["kernel_nodes"] <= 24, info
+            # distance transform (3) + march + levels below the merged top + level 0
+            assert 6 <= info["kernel_nodes"] <= 12, info
             assert info["workspace_bytes"] > 0
         else:
-            assert info["kernel_nodes"] >= 10 * len(SIZES), info
+            # per view at least the distance transform (3) and level 0
+            assert info["kernel_nodes"] >= 4 * len(SIZES), info
 
 
 @pytest.mark.parametrize("storage", ["f16", "bf16"])

@@ -46,8 +46,10 @@ def rc():
     return rc
 
 
-def run_case(rc, oracle, W, H, L, seed, n_boxes, n_emissive, storage="f32"):
-    """Raster on the GPU, run the pass with debug readbacks, return numpy."""
+def run_case(rc, oracle, W, H, L, seed, n_boxes, n_emissive, storage="f32", sdf="exact"):
+    """Raster on the GPU, run the pass with debug readbacks, return numpy.
+    sdf: the distance transform ("exact", the default, or "jfa", whose seed
+    field is read back too)."""
     dev = torch.device("cuda:0")
     boxes = rc.scene_generate(seed, n_boxes, n_emissive)
     depth = torch.empty((H, W), dtype=torch.float32, device=dev)
@@ -57,11 +59,17 @@ def run_case(rc, oracle, W, H, L, seed, n_boxes, n_emissive, storage="f32"):
     irr = torch.empty((3, H, W), dtype=torch.float32, device=dev)
     out = {"depth": depth, "albedo": albedo, "emissive": emis}
     with rc.RadianceCascadeStage(W, H, levels=L, storage=storage, debug=True) as st:
+        assert st.sdf_mode() == "exact"
+        st.set_sdf_mode(sdf)
         st.run_device(depth, emis, irr)
         seeds = torch.empty((H, W), dtype=torch.int32, device=dev)
-        sdf = torch.empty((H, W), dtype=torch.float32, device=dev)
-        st.debug_seeds(seeds)
-        st.debug_sdf(sdf)
+        sdf_t = torch.empty((H, W), dtype=torch.float32, device=dev)
+        if sdf == "jfa":
+            st.debug_seeds(seeds)
+        else:
+            with pytest.raises(rc.RcError):
+                st.debug_seeds(seeds)  # no seed field in the exact mode
+        st.debug_sdf(sdf_t)
         levels = []
         for i in range(L):
             t = torch.empty(st.level_shape(i), dtype=torch.float32, device=dev)
@@ -77,8 +85,8 @@ def run_case(rc, oracle, W, H, L, seed, n_boxes, n_emissive, storage="f32"):
         torch.cuda.synchronize()
     res = {k: v.cpu().numpy() for k, v in out.items()}
     res["irr"] = irr.cpu().numpy()
-    res["seeds"] = seeds.cpu().numpy().view(np.uint32)
-    res["sdf"] = sdf.cpu().numpy()
+    res["seeds"] = seeds.cpu().numpy().view(np.uint32) if sdf == "jfa" else None
+    res["sdf"] = sdf_t.cpu().numpy()
     res["levels"] = [t.cpu().numpy() for t in levels]
     res["hits"] = [(h.cpu().numpy(), None if ix is None else ix.cpu().numpy()) for h, ix in hits]
     res["boxes"] = boxes
@@ -97,12 +105,11 @@ CASES = [
 ]
 
 
-@pytest.mark.parametrize("W,H,L,seed,nb,ne", CASES)
-def test_pass_parity(rc, oracle, W, H, L, seed, nb, ne):
-    _say(f"case W={W} H={H} L={L} seed={seed} boxes={nb} emissive={ne}")
+def _check_pass(rc, oracle, W, H, L, seed, nb, ne, sdf):
+    _say(f"case W={W} H={H} L={L} seed={seed} boxes={nb} emissive={ne} sdf={sdf}")
     if L == 0:
         L = oracle.levels_for_coverage(W, H)
-    g = run_case(rc, oracle, W, H, L, seed, nb, ne)
+    g = run_case(rc, oracle, W, H, L, seed, nb, ne, sdf=sdf)
 
     # H0: analytic FP64 raster, bit-exact
     d_ref, a_ref, e_ref = oracle.raster(oracle.scene(seed, nb, ne), W, H)
@@ -111,18 +118,23 @@ def test_pass_parity(rc, oracle, W, H, L, seed, nb, ne):
     assert np.array_equal(g["albedo"].view(np.uint32), a_ref.view(np.uint32)), "albedo raster differs"
     assert np.array_equal(g["emissive"].view(np.uint32), e_ref.view(np.uint32)), "emissive raster differs"
 
-    # H1+H2: occupancy and jump flood, bit-exact
-    seeds_ref = oracle.jfa(oracle.seed_init(d_ref, e_ref))
-    mism = int((g["seeds"] != seeds_ref).sum())
-    assert mism == 0, f"{mism} jump-flood pixels differ"
-    sdf_ref = oracle.sdf_f32(seeds_ref)
-    assert np.array_equal(g["sdf"].view(np.uint32), sdf_ref.view(np.uint32)), "distance field differs"
+    # H1+H2: occupancy and the distance transform, bit-exact
+    init = oracle.seed_init(d_ref, e_ref)
+    if sdf == "jfa":
+        seeds_ref = oracle.jfa(init)
+        mism = int((g["seeds"] != seeds_ref).sum())
+        assert mism == 0, f"{mism} jump-flood pixels differ"
+        sdf_ref, sdf64 = oracle.sdf_f32(seeds_ref), oracle.sdf_f64(seeds_ref)
+    else:
+        d2 = oracle.edt_d2(init)
+        sdf_ref, sdf64 = oracle.sdf_from_d2(d2), oracle.sdf_from_d2(d2, np.float64)
+    n_bad = int((g["sdf"].view(np.uint32) != sdf_ref.view(np.uint32)).sum())
+    assert n_bad == 0, f"distance field differs at {n_bad} pixels"
 
     # H3/H4: the codes the production kernels stored (the split march's code
     # buffer for levels >= 1, the level-0 kernel's in-place march for level
     # 0), against the oracle's per canonical ray; and the split march's decode
     # of every slot into (column, row, direction)
-    sdf64 = oracle.sdf_f64(seeds_ref)
     kinds = set()
     for i in range(L):
         hit, idx = g["hits"][i]
@@ -142,8 +154,8 @@ def test_pass_parity(rc, oracle, W, H, L, seed, nb, ne):
         assert kinds == {-2, -1, 0}, kinds
 
     # H5/H6: merged levels and irradiance
-    irr32, lv32 = oracle.run(d_ref, e_ref, L, np.float32, want_levels=True)
-    irr64, lv64 = oracle.run(d_ref, e_ref, L, np.float64, want_levels=True)
+    irr32, lv32 = oracle.run(d_ref, e_ref, L, np.float32, want_levels=True, sdf=sdf)
+    irr64, lv64 = oracle.run(d_ref, e_ref, L, np.float64, want_levels=True, sdf=sdf)
     for i in range(L):
         assert g["levels"][i].shape == lv32[i].shape
         exact = np.array_equal(g["levels"][i].view(np.uint32), lv32[i].view(np.uint32))
@@ -156,6 +168,18 @@ def test_pass_parity(rc, oracle, W, H, L, seed, nb, ne):
     e64 = rel_l2(g["irr"], irr64)
     _say(f"  irradiance rel-L2 vs fp64 {e64:.3e}, max-abs {np.abs(g['irr'] - irr64).max():.3e}")
     assert e64 <= REL_L2_TOL_F32
+
+
+@pytest.mark.parametrize("W,H,L,seed,nb,ne", CASES)
+def test_pass_parity(rc, oracle, W, H, L, seed, nb, ne):
+    """The default frame (exact distance transform) against the oracle."""
+    _check_pass(rc, oracle, W, H, L, seed, nb, ne, "exact")
+
+
+@pytest.mark.parametrize("W,H,L,seed,nb,ne", [c for c in CASES if c[0] * c[1] != 256 * 256 or c[3] == 7])
+def test_jump_flood_mode_parity(rc, oracle, W, H, L, seed, nb, ne):
+    """The jump-flood mode (seed field read back) against the oracle's flood."""
+    _check_pass(rc, oracle, W, H, L, seed, nb, ne, "jfa")
 
 
 def test_deterministic(rc, oracle):
@@ -236,6 +260,44 @@ def _device_gbuffer(rc, W, H, seed, nb, ne):
     return depth, emis
 
 
+# widths either side of the row kernel's LDS capacities (1024 .. 16384) and
+# heights either side of the 64-row column words
+EXACT_SHAPES = [(1, 1), (1, 130), (130, 1), (64, 64), (65, 129), (1023, 7), (1024, 9), (1025, 65), (2049, 3),
+                (4097, 2), (8193, 2), (16384, 2), (3, 4097), (2, 16384)]
+
+
+@pytest.mark.parametrize("W,H", EXACT_SHAPES)
+def test_exact_transform_shapes_and_densities(rc, oracle, W, H):
+    """The exact transform's field (occupancy words, column distances, row
+    envelopes) equals the oracle's exact transform bit for bit on occupancy
+    patterns of every density: none (every code 255), a single pixel, sparse
+    random pixels, dense and full (every code 0).  Flat depth, so the
+    occupancy is exactly the emissive pixels."""
+    _say(f"exact transform {W}x{H}")
+    rng = np.random.default_rng(W * 7919 + H)
+    d = np.ones((H, W), np.float32)
+    sdf = torch.empty((H, W), dtype=torch.float32, device="cuda:0")
+    irr = torch.empty((3, H, W), dtype=torch.float32, device="cuda:0")
+    with rc.RadianceCascadeStage(W, H, levels=1, debug=True) as st:
+        for p in (0.0, -1.0, 1e-4, 0.01, 0.3, 1.0):
+            e = np.zeros((3, H, W), np.float32)
+            if p < 0:  # one pixel, far from the centre
+                e[1, int(rng.integers(H)), int(rng.integers(W))] = 1.0
+            else:
+                e[0] = (rng.random((H, W)) < p).astype(np.float32)
+            st.run_device(torch.from_numpy(d).cuda(), torch.from_numpy(e).cuda(), irr)
+            st.debug_sdf(sdf)
+            torch.cuda.synchronize()
+            ref = oracle.sdf_from_d2(oracle.edt_d2(oracle.seed_init(d, e)))
+            got = sdf.cpu().numpy()
+            n_bad = int((got.view(np.uint32) != ref.view(np.uint32)).sum())
+            assert n_bad == 0, f"density {p}: {n_bad} pixels differ"
+            if p == 0.0:
+                assert np.all(got == 2160.0)
+            if p == 1.0:
+                assert np.all(got == 0.0)
+
+
 @pytest.mark.parametrize("split", [0, 1])
 def test_every_tiling_is_bit_exact(rc, oracle, split):
     """Every (tile shape, rays per lane) of every level computes the same
@@ -307,6 +369,8 @@ def test_every_jfa_fusion_is_bit_exact(rc, oracle, W, H):
     sdf = torch.empty((H, W), dtype=torch.float32, device="cuda:0")
     L = oracle.levels_for_coverage(W, H)
     with rc.RadianceCascadeStage(W, H, levels=L, debug=True) as st:
+        st.set_sdf_mode("jfa")
+        assert st.sdf_mode() == "jfa"
         for rows in (1, 2, 4):
             st.set_jfa_rows(rows)
             assert st.jfa_rows() == rows
@@ -332,8 +396,8 @@ def test_every_march_mode_is_bit_exact(rc, oracle, W, H, L, seed):
     _say(f"march modes W={W} H={H} L={L} seed={seed}")
     depth, emis = _device_gbuffer(rc, W, H, seed, 32, 4)
     d, _, e = oracle.gbuffer(seed, W, H, 32, 4)
-    irr32, lv32, seeds = oracle.run(d, e, L, np.float32, want_levels=True, want_seeds=True)
-    sdf = oracle.sdf_f32(seeds)
+    irr32, lv32 = oracle.run(d, e, L, np.float32, want_levels=True)
+    sdf = oracle.sdf_from_d2(oracle.edt_d2(oracle.seed_init(d, e)))
     hits_ref = [oracle.march_level(sdf, e, i)[1].ravel() for i in range(L)]
     irr = torch.empty((3, H, W), dtype=torch.float32, device="cuda:0")
     with rc.RadianceCascadeStage(W, H, levels=L, debug=True) as st:
@@ -376,8 +440,8 @@ def test_every_march_order_is_bit_exact(rc, oracle, W, H, L, seed):
     _say(f"march orders W={W} H={H} L={L} seed={seed}")
     depth, emis = _device_gbuffer(rc, W, H, seed, 32, 4)
     d, _, e = oracle.gbuffer(seed, W, H, 32, 4)
-    irr32, lv32, seeds = oracle.run(d, e, L, np.float32, want_levels=True, want_seeds=True)
-    sdf = oracle.sdf_f32(seeds)
+    irr32, lv32 = oracle.run(d, e, L, np.float32, want_levels=True)
+    sdf = oracle.sdf_from_d2(oracle.edt_d2(oracle.seed_init(d, e)))
     hits_ref = [oracle.march_level(sdf, e, i)[1].ravel() for i in range(L)]
     irr = torch.empty((3, H, W), dtype=torch.float32, device="cuda:0")
     with rc.RadianceCascadeStage(W, H, levels=L) as st:
@@ -463,7 +527,7 @@ def _oracle_run_storage(oracle, d, e, L, storage):
     """FP32 oracle flavour with each materialised level (1..L-1) stored in
     `storage`: the composition of the oracle's stage functions."""
     H, W = d.shape
-    sdf = oracle.sdf_f32(oracle.jfa(oracle.seed_init(d, e)))
+    sdf = oracle.sdf_from_d2(oracle.edt_d2(oracle.seed_init(d, e)))
     up, levels = None, [None] * L
     for i in range(L - 1, -1, -1):
         m, _ = oracle.march_level(sdf, e, i)
@@ -541,35 +605,52 @@ def test_ndc_depth_input(rc, oracle, W, H, L, seed, nb, ne):
     """The reference renderer's depth target holds perspective_rh NDC depth
     (Depth32Float, near 0.1, far 100: src/camera.rs:77-79, src/texture.rs:149).
     Handed that plane with RC_DEPTH_NDC, the pass converts it in the seed
-    kernel: seeds, distance field and irradiance equal the oracle on the
-    converted depth bit for bit, and the occupancy equals the linear-depth
-    run's (the fp32 round trip moves no depth across the 5 % rule here)."""
+    kernel (the jump flood's seed kernel and the exact transform's occupancy
+    kernel): seeds (jump-flood mode), distance field and irradiance (both
+    modes) equal the oracle on the converted depth bit for bit, and equal the
+    linear-depth run's (the fp32 round trip moves no depth across the 5 % rule
+    here)."""
     _say(f"NDC depth {W}x{H} L={L} seed={seed}")
     near, far = 0.1, 100.0
     depth, emis = _device_gbuffer(rc, W, H, seed, nb, ne)
     d_lin = depth.cpu().numpy()
     d_ndc = oracle.ndc_from_linear(d_lin, near, far)
     ndc_dev = torch.from_numpy(d_ndc).cuda()
-    irr_lin = torch.empty((3, H, W), dtype=torch.float32, device="cuda:0")
-    irr_ndc = torch.empty_like(irr_lin)
-    seeds_lin = torch.empty((H, W), dtype=torch.int32, device="cuda:0")
-    seeds_ndc = torch.empty_like(seeds_lin)
-    with rc.RadianceCascadeStage(W, H, levels=L, debug=True) as st:
-        st.run_device(depth, emis, irr_lin)
-        st.debug_seeds(seeds_lin)
-    with rc.RadianceCascadeStage(W, H, levels=L, debug=True, ndc_depth=(near, far)) as st:
-        st.run_device(ndc_dev, emis, irr_ndc)
-        st.debug_seeds(seeds_ndc)
-    torch.cuda.synchronize()
     e = emis.cpu().numpy()
     d_conv = oracle.linear_from_ndc(d_ndc, near, far)
-    irr32, seeds_ref = oracle.run(d_conv, e, L, np.float32, want_seeds=True)
-    assert np.array_equal(seeds_ndc.cpu().numpy().view(np.uint32), seeds_ref)
-    assert np.array_equal(irr_ndc.cpu().numpy().view(np.uint32), irr32.view(np.uint32))
-    n_diff = int((seeds_ndc != seeds_lin).sum().item())
-    _say(f"  seeds differing from the linear-depth run: {n_diff}")
-    assert n_diff == 0
-    assert torch.equal(irr_ndc, irr_lin)
+    for mode in ("jfa", "exact"):
+        irr_lin = torch.empty((3, H, W), dtype=torch.float32, device="cuda:0")
+        irr_ndc = torch.empty_like(irr_lin)
+        seeds_lin = torch.empty((H, W), dtype=torch.int32, device="cuda:0")
+        seeds_ndc = torch.empty_like(seeds_lin)
+        sdf_lin = torch.empty((H, W), dtype=torch.float32, device="cuda:0")
+        sdf_ndc = torch.empty_like(sdf_lin)
+        with rc.RadianceCascadeStage(W, H, levels=L, debug=True) as st:
+            st.set_sdf_mode(mode)
+            st.run_device(depth, emis, irr_lin)
+            st.debug_sdf(sdf_lin)
+            if mode == "jfa":
+                st.debug_seeds(seeds_lin)
+        with rc.RadianceCascadeStage(W, H, levels=L, debug=True, ndc_depth=(near, far)) as st:
+            st.set_sdf_mode(mode)
+            st.run_device(ndc_dev, emis, irr_ndc)
+            st.debug_sdf(sdf_ndc)
+            if mode == "jfa":
+                st.debug_seeds(seeds_ndc)
+        torch.cuda.synchronize()
+        if mode == "jfa":
+            irr32, seeds_ref = oracle.run(d_conv, e, L, np.float32, want_seeds=True, sdf="jfa")
+            assert np.array_equal(seeds_ndc.cpu().numpy().view(np.uint32), seeds_ref)
+            n_diff = int((seeds_ndc != seeds_lin).sum().item())
+            _say(f"  seeds differing from the linear-depth run: {n_diff}")
+            assert n_diff == 0
+        else:
+            irr32 = oracle.run(d_conv, e, L, np.float32)
+            sdf_ref = oracle.sdf_from_d2(oracle.edt_d2(oracle.seed_init(d_conv, e)))
+            assert np.array_equal(sdf_ndc.cpu().numpy().view(np.uint32), sdf_ref.view(np.uint32))
+        assert torch.equal(sdf_ndc, sdf_lin)
+        assert np.array_equal(irr_ndc.cpu().numpy().view(np.uint32), irr32.view(np.uint32))
+        assert torch.equal(irr_ndc, irr_lin)
 
 
 @pytest.mark.parametrize("W,H", [(16384, 32), (32, 16384)])

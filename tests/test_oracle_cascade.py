@@ -237,15 +237,18 @@ def test_run_levels_consistent(oracle):
     """The whole-path driver equals chaining march/merge/gather by hand."""
     W, H, L = 64, 48, 4
     d, a, e = oracle.gbuffer(9, W, H, 16, 4)
-    irr, lv, seeds = oracle.run(d, e, L, np.float64, want_levels=True, want_seeds=True)
-    assert np.array_equal(seeds, oracle.jfa(oracle.seed_init(d, e)))
-    sdf = oracle.sdf_f64(seeds)
-    up = None
-    for i in range(L - 1, -1, -1):
-        m, _ = oracle.march_level(sdf, e, i)
-        up = oracle.merge_level(W, H, i, m, up)
-        assert np.array_equal(up, lv[i])
-    assert np.array_equal(oracle.gather(W, H, up), irr)
+    init = oracle.seed_init(d, e)
+    irr, lv, seeds = oracle.run(d, e, L, np.float64, want_levels=True, want_seeds=True, sdf="jfa")
+    assert np.array_equal(seeds, oracle.jfa(init))
+    irr_x, lv_x = oracle.run(d, e, L, np.float64, want_levels=True)
+    for sdf, irr, lv in ((oracle.sdf_f64(seeds), irr, lv),
+                         (oracle.sdf_from_d2(oracle.edt_d2(init), np.float64), irr_x, lv_x)):
+        up = None
+        for i in range(L - 1, -1, -1):
+            m, _ = oracle.march_level(sdf, e, i)
+            up = oracle.merge_level(W, H, i, m, up)
+            assert np.array_equal(up, lv[i])
+        assert np.array_equal(oracle.gather(W, H, up), irr)
 
 
 def test_uniform_sky_irradiance(oracle):

@@ -122,10 +122,14 @@ def test_full_size_frame_against_ground_truth(oracle, seed, b_truth, b_fields):
 
 
 def test_run_on_field_is_the_driver(oracle):
-    """The stage composition on the jump-flood field equals rco_run."""
+    """The stage composition on the jump-flood field equals rco_run's
+    jump-flood mode, and on the exact transform's field its default mode."""
     W, H, L = 97, 61, 5
     d, _, e = oracle.gbuffer(3, W, H, 16, 6)
-    sdf = oracle.sdf_f64(oracle.jfa(oracle.seed_init(d, e)))
+    init = oracle.seed_init(d, e)
+    sdf = oracle.sdf_f64(oracle.jfa(init))
+    assert np.array_equal(oracle.run_on_field(sdf, e, L), oracle.run(d, e, L, np.float64, sdf="jfa"))
+    sdf = oracle.sdf_from_d2(oracle.edt_d2(init), np.float64)
     assert np.array_equal(oracle.run_on_field(sdf, e, L), oracle.run(d, e, L, np.float64))
 
 

@@ -147,3 +147,68 @@ def test_sdf_code_rule(oracle):
     assert oracle.sdf_value_d2(127 ** 2) == 127 and oracle.sdf_value_d2(128 ** 2 - 1) == 127
     assert oracle.sdf_value_d2(143 ** 2) == 128 and oracle.sdf_value_d2(144 ** 2) == 144
     assert oracle.sdf_value_d2(10 ** 8) == 2160
+
+
+# ---- exact transform (rco_edt_d2): the default distance field -------------------
+def test_edt_matches_bruteforce_and_scipy(oracle):
+    """rco_edt_d2 (Felzenszwalb & Huttenlocher, exact integer envelope) equals
+    the brute-force nearest seed's d^2 and scipy's exact transform (a library)
+    on random fields of every density, ragged and degenerate shapes."""
+    rng = np.random.default_rng(11)
+    shapes = [(1, 1), (1, 9), (9, 1), (2, 2), (64, 1), (1, 64), (17, 23), (64, 64), (65, 63), (130, 70)]
+    for (W, H) in shapes:
+        for p in (0.0, 0.001, 0.02, 0.2, 0.7, 1.0):
+            occ = rng.random((H, W)) < p
+            init = _seed_field(W, H, [(int(x), int(y)) for y, x in zip(*np.nonzero(occ))])
+            d2 = oracle.edt_d2(init)
+            assert np.array_equal(d2, _d2(oracle.nearest_seed_bruteforce(init))), (W, H, p)
+            if occ.any():
+                edt = ndimage.distance_transform_edt(~occ)
+                assert np.array_equal(d2, np.rint(edt ** 2).astype(np.int64)), (W, H, p)
+            else:
+                assert np.all(d2 == -1)
+
+
+def test_edt_closed_forms(oracle):
+    # one seed: d^2 is the squared distance to it everywhere
+    W, H = 37, 29
+    d2 = oracle.edt_d2(_seed_field(W, H, [(30, 4)]))
+    yy, xx = np.mgrid[0:H, 0:W]
+    assert np.array_equal(d2, (xx - 30) ** 2 + (yy - 4) ** 2)
+    # a full column of seeds at x = 0: d^2 = x^2 (the row envelope's first site only)
+    d2 = oracle.edt_d2(_seed_field(W, H, [(0, y) for y in range(H)]))
+    assert np.array_equal(d2, np.broadcast_to(np.arange(W) ** 2, (H, W)))
+    # the four corners of a large image: the farthest pixel is the centre
+    W, H = 16384, 3
+    d2 = oracle.edt_d2(_seed_field(W, H, [(0, 0), (W - 1, 0), (0, H - 1), (W - 1, H - 1)]))
+    assert d2.max() == (W // 2 - 1) ** 2 + 1 and d2[1, W // 2 - 1] == (W // 2 - 1) ** 2 + 1
+
+
+def test_edt_is_never_farther_than_the_jump_flood(oracle):
+    """The exact field is a lower bound of the jump flood's, equal except at
+    the flood's rare misses; the SDF codes follow."""
+    rng = np.random.default_rng(5)
+    for (W, H, n) in [(64, 64, 3), (64, 64, 40), (90, 50, 200), (200, 120, 25)]:
+        pts = {(int(rng.integers(W)), int(rng.integers(H))) for _ in range(n)}
+        init = _seed_field(W, H, pts)
+        de, dj = oracle.edt_d2(init), _d2(oracle.jfa(init))
+        assert np.all(de <= dj) and (de != dj).mean() < 0.01
+        se, sj = oracle.sdf_from_d2(de), oracle.sdf_f32(oracle.jfa(init))
+        assert np.all(se <= sj)
+        assert np.array_equal(oracle.sdf_from_d2(de, np.float64), np.vectorize(oracle.sdf_value_d2)(de))
+
+
+def test_whole_path_modes(oracle):
+    """rco_run's exact mode equals the stage chain on the exact field; its jump-flood
+    mode equals the chain on the flood's field; with no flood miss they agree."""
+    d, _, e = oracle.gbuffer(3, 96, 64, 8, 2)
+    L = oracle.levels_for_coverage(96, 64)
+    init = oracle.seed_init(d, e)
+    for dt in (np.float32, np.float64):
+        ex = oracle.run(d, e, L, dt)
+        assert np.array_equal(ex, oracle.run_on_field(oracle.sdf_from_d2(oracle.edt_d2(init), dt), e, L))
+        jf = oracle.run(d, e, L, dt, sdf="jfa")
+        sj = oracle.sdf_f32(oracle.jfa(init)) if dt == np.float32 else oracle.sdf_f64(oracle.jfa(init))
+        assert np.array_equal(jf, oracle.run_on_field(sj, e, L))
+    with pytest.raises(ValueError):
+        oracle.run(d, e, L, np.float32, want_seeds=True)

@@ -211,7 +211,7 @@ def test_exact_level2_reproduces_the_plain_frame():
     c = np.zeros((QX * QY, 48))
     A2 = so.predicted_level2(c, lv[2][..., :3].reshape(-1, 48), 1.0, QX, QY)
     assert np.array_equal(A2[..., :3], lv[2][..., :3])
-    sdf = rco.sdf_f64(rco.jfa(rco.seed_init(d, e)))
+    sdf = rco.sdf_from_d2(rco.edt_d2(rco.seed_init(d, e)), np.float64)
     m1, _ = rco.march_level(sdf, e, 1)
     m0, _ = rco.march_level(sdf, e, 0)
     A1 = rco.merge_level(W, H, 1, m1, A2)
