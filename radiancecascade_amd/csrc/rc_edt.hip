@@ -149,31 +149,6 @@ __global__ __launch_bounds__(kBlock) void k_cols_batch(const ColJob *__restrict_
 }
 
 // ---- per-row lower envelope -------------------------------------------------------
-// Exclusive prefix sum of v over the RL lanes of the workgroup; `total` is the
-// sum over all lanes.  wsum: RL / 64 words of LDS.
-template <int RL>
-__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *wsum, uint32_t &total)
-{
-    const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
-    uint32_t inc = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t n = __shfl_up(inc, o, 64);
-        if (lane >= o) inc += n;
-    }
-    if (lane == 63) wsum[wave] = inc;
-    __syncthreads();
-    uint32_t off = 0;
-    total = 0;
-#pragma unroll
-    for (int w = 0; w < RL / 64; ++w) {
-        const uint32_t sw = wsum[w];
-        off += w < wave ? sw : 0u;
-        total += sw;
-    }
-    return off + inc - v;
-}
-
 // Candidate range [lo, hi] of query q (at x = (2q + 1) s - 1) at a level of
 // step s: the argmins the earlier levels fixed at x -+ s (0 and W - 1 beyond
 // the ends).
@@ -183,14 +158,16 @@ __device__ __forceinline__ int edt_hi(int q, int s, int W, const uint16_t *opt)
     return (2 * q + 2) * s > W ? W - 1 : opt[(2 * q + 2) * s - 1];
 }
 
-// Row y of a W-pixel image (W <= WMAX) by one workgroup of RL lanes.  A
-// level's work is the list of its (query, candidate) pairs, about W + 2^k
-// long but very uneven per query (a query whose neighbours' argmins lie on
-// either side of a wide gap spans the gap), so it is balanced over the
-// lanes: the queries' range sizes are prefix-summed, each lane takes an
-// equal run of consecutive pairs (finding its first query by binary search
-// over the prefix sums, then walking forward, keeping the first smallest
-// value so ties go to the smallest candidate), and the per-query minima meet
+// Row y of a W-pixel image (W <= WMAX) by one workgroup of RL lanes.
+// Consecutive queries of a level share an end (hi(q) = lo(q + 1)), so the
+// level's (query, candidate) pairs form one staircase of T = hi(nq - 1) + nq
+// steps: query q owns the steps [lo(q) + q, hi(q) + q], step t of it being
+// candidate j = t - q.  The work per query is very uneven (a query whose
+// neighbours' argmins lie on either side of a wide gap spans the gap), so
+// the lanes take equal runs of steps instead: a lane finds the query of its
+// first step by binary search on lo(q) + q (the argmins in LDS), walks its
+// run in one loop of uniform trip count, keeping the first smallest value
+// (ties go to the smallest candidate), and meets the other lanes of a query
 // in LDS by a 64-bit atomic minimum of (value << 14 | candidate).
 template <int WMAX, int RL>
 __device__ __forceinline__ void edt_row_body(const uint16_t *__restrict__ g, uint8_t *__restrict__ sdf, int W, int y)
@@ -198,60 +175,40 @@ __device__ __forceinline__ void edt_row_body(const uint16_t *__restrict__ g, uin
     __shared__ uint16_t gs[WMAX];
     __shared__ uint16_t opt[WMAX];                 // opt[x]: smallest argmin at x
     __shared__ unsigned long long best[WMAX / 2];  // per query of the current level (at most ceil(W / 2))
-    __shared__ uint16_t pre[WMAX / 2 + 1];         // exclusive prefix of the range sizes (total <= W + W / 2)
-    __shared__ uint32_t wsum[RL / 64];
     const int tid = (int)threadIdx.x;
     const uint16_t *gr = g + (size_t)y * W;
     for (int x = tid; x < W; x += RL) gs[x] = gr[x];
+    if (tid == 0) best[0] = ~0ull;  // level 0 has one query
     __syncthreads();
     const int K = 32 - __builtin_clz((uint32_t)W);  // smallest K with 2^K > W
     for (int k = 0; k < K; ++k) {
         const int ls = K - 1 - k, s = 1 << ls;
         const int nq = ((W >> ls) + 1) >> 1;  // odd multiples of s up to W
-        const int per = (nq + RL - 1) / RL;
-        const int qa = min(tid * per, nq), qb = min(qa + per, nq);
-        uint32_t local = 0;
-        for (int q = qa; q < qb; ++q) local += (uint32_t)(edt_hi(q, s, W, opt) - edt_lo(q, s, opt) + 1);
-        uint32_t total;
-        uint32_t off = block_excl_scan<RL>(local, wsum, total);
-        for (int q = qa; q < qb; ++q) {
-            pre[q] = (uint16_t)off;
-            off += (uint32_t)(edt_hi(q, s, W, opt) - edt_lo(q, s, opt) + 1);
-            best[q] = ~0ull;
-        }
-        if (tid == 0) pre[nq] = (uint16_t)total;
-        __syncthreads();
-        const uint32_t c = (total + RL - 1) / RL;
-        const uint32_t t0 = (uint32_t)tid * c, t1 = min(t0 + c, total);
+        const int T = edt_hi(nq - 1, s, W, opt) + nq;
+        const int c = (T + RL - 1) / RL;
+        const int t0 = tid * c, t1 = min(t0 + c, T);
         if (t0 < t1) {
-            int a = 0, b = nq;  // largest q with pre[q] <= t0
+            int a = 0, b = nq;  // the last query starting at or before t0
             while (b - a > 1) {
                 const int m = (a + b) >> 1;
-                if (pre[m] <= t0) a = m;
+                if (opt[2 * m * s - 1] + m <= t0) a = m;
                 else b = m;
             }
-            // one uniform-count loop over the lane's run (the step into the
-            // next query is a short divergent branch); branch-free values:
-            // |x - j| < 2^14 and g <= 0xFFFF fit the 24-bit multiplier, and an
-            // empty column's g^2 (> 2^29) is clamped to kEdtBig, above every
-            // real value
-            int q = a;
-            int x = (2 * q + 1) * s - 1, lo = edt_lo(q, s, opt);
-            uint32_t start = pre[q], end = pre[q + 1];
+            // branch-free values: |x - j| < 2^14 and g <= 0xFFFF fit the
+            // 24-bit multiplier, and an empty column's g^2 (> 2^29) is clamped
+            // to kEdtBig, above every real value
+            int q = a, x = (2 * q + 1) * s - 1;
+            int end = edt_hi(q, s, W, opt) + q + 1;  // the first step of query q + 1
             uint32_t bv = 0xFFFFFFFFu, bj = 0;
-            for (uint32_t t = t0; t < t1; ++t) {
-                if (t >= end) {
-                    do {  // flush, then enter the query holding step t
-                        atomicMin(&best[q], ((unsigned long long)bv << 14) | bj);
-                        bv = 0xFFFFFFFFu;
-                        ++q;
-                        start = end;
-                        end = pre[q + 1];
-                    } while (t >= end);
-                    x = (2 * q + 1) * s - 1;
-                    lo = opt[2 * q * s - 1];
+            for (int t = t0; t < t1; ++t) {
+                if (t >= end) {  // the run enters the next query (sizes >= 1: one step)
+                    atomicMin(&best[q], ((unsigned long long)bv << 14) | bj);
+                    bv = 0xFFFFFFFFu;
+                    ++q;
+                    x += 2 * s;
+                    end = edt_hi(q, s, W, opt) + q + 1;
                 }
-                const int j = lo + (int)(t - start), dx = x - j;
+                const int j = t - q, dx = x - j;
                 const uint32_t gj = gs[j], g2 = (uint32_t)__umul24(gj, gj);
                 const uint32_t v = (uint32_t)__mul24(dx, dx) + (g2 < kEdtBig ? g2 : kEdtBig);
                 if (v < bv) {
@@ -262,7 +219,12 @@ __device__ __forceinline__ void edt_row_body(const uint16_t *__restrict__ g, uin
             atomicMin(&best[q], ((unsigned long long)bv << 14) | bj);
         }
         __syncthreads();
-        for (int q = qa; q < qb; ++q) opt[(2 * q + 1) * s - 1] = (uint16_t)(best[q] & 0x3FFFu);
+        // this level's argmins; the next level's minima start empty
+        const int nq_next = k + 1 < K ? ((W >> (ls - 1)) + 1) >> 1 : 0;
+        for (int q = tid; q < max(nq, nq_next); q += RL) {
+            if (q < nq) opt[(2 * q + 1) * s - 1] = (uint16_t)(best[q] & 0x3FFFu);
+            best[q] = ~0ull;
+        }
         __syncthreads();
     }
     const int tx = sdf_tiles_x(W);
