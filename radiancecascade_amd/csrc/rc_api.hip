@@ -31,7 +31,7 @@ struct rc_ctx {
     int sdf_mode = RC_SDF_EXACT;
     uint32_t *seed[2] = {nullptr, nullptr};  // jump flood: seed ping-pong
     uint64_t *edt_bits = nullptr;            // exact transform: occupancy column words
-    uint16_t *edt_g = nullptr;               // exact transform: vertical distances
+    uint16_t *edt_ends = nullptr;            // exact transform: occupied rows nearest each band
     uint8_t *sdf = nullptr;  // distance codes (rc_common.hpp)
     void *level[RC_MAX_LEVELS] = {};
     size_t level_texels[RC_MAX_LEVELS] = {};
@@ -161,9 +161,9 @@ static void release(rc_ctx *c)
     drop_graph(c);
     for (auto &p : c->seed) { if (p) hipFree(p); p = nullptr; }
     if (c->edt_bits) hipFree(c->edt_bits);
-    if (c->edt_g) hipFree(c->edt_g);
+    if (c->edt_ends) hipFree(c->edt_ends);
     c->edt_bits = nullptr;
-    c->edt_g = nullptr;
+    c->edt_ends = nullptr;
     if (c->sdf) hipFree(c->sdf);
     c->sdf = nullptr;
     for (auto &p : c->level) { if (p) hipFree(p); p = nullptr; }
@@ -230,7 +230,7 @@ static rc_status ensure_sdf_buffers(rc_ctx *c)
             if (!p) HIP_TRY(c, hipMalloc(&p, n * sizeof(uint32_t)));
     } else {
         if (!c->edt_bits) HIP_TRY(c, hipMalloc(&c->edt_bits, edt_bits_bytes(c->W, c->H)));
-        if (!c->edt_g) HIP_TRY(c, hipMalloc(&c->edt_g, edt_g_bytes(c->W, c->H)));
+        if (!c->edt_ends) HIP_TRY(c, hipMalloc(&c->edt_ends, edt_ends_bytes(c->W, c->H)));
     }
     return RC_OK;
 }
@@ -418,7 +418,7 @@ static rc_status collect_profile(rc_ctx *c, hipStream_t s, int jfa_launches)
 }
 
 // The distance field c->sdf (and the interleaved emission c->emis4).  Exact
-// mode: the occupancy kernel (stage "seed"), then the column and row kernels
+// mode: the occupancy kernel (stage "seed"), then the band-end and row kernels
 // (stage "jfa", 2 launches).  Jump-flood mode: occupancy seeds, then global
 // passes down to the fused LDS tail (if any); without a tail the last global
 // pass writes the field, and a 1x1 image has no pass.  With `gate` every
@@ -429,11 +429,11 @@ static rc_status seeds_and_jfa(rc_ctx *c, const float *depth, const float *emiss
     if (c->sdf_mode == RC_SDF_EXACT) {
         c->dbg_seed_buf = -1;
         if (tm) tm->begin(RC_STAGE_SEED);
-        HIP_TRY(c, launch_edt(depth, c->depth_conv, emissive, c->edt_bits, c->edt_g, c->sdf, c->emis4, c->W, c->H, s,
+        HIP_TRY(c, launch_edt(depth, c->depth_conv, emissive, c->edt_bits, c->edt_ends, c->sdf, c->emis4, c->W, c->H, s,
                               gate, 1));
         if (tm) tm->end(RC_STAGE_SEED);
         if (tm) tm->begin(RC_STAGE_JFA);
-        HIP_TRY(c, launch_edt(depth, c->depth_conv, emissive, c->edt_bits, c->edt_g, c->sdf, c->emis4, c->W, c->H, s,
+        HIP_TRY(c, launch_edt(depth, c->depth_conv, emissive, c->edt_bits, c->edt_ends, c->sdf, c->emis4, c->W, c->H, s,
                               gate, 2));
         if (tm) tm->end(RC_STAGE_JFA);
         if (global_passes) *global_passes = 2;

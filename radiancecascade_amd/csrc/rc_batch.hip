@@ -5,8 +5,8 @@
 // view.  Here the frames of all views run as a fixed handful of launches,
 // each covering every view that has work in that stage:
 //
-//   distance   3 launches, every view: occupancy words, vertical distances,
-//              row lower envelopes (the exact transform, a context's default;
+//   distance   3 launches, every view: occupancy words, band ends, row
+//              lower envelopes (the exact transform, a context's default;
 //              the row kernel's LDS is sized for the widest view)
 //   march      1 launch of (view, level >= 1) jobs, top level first
 //   levels     one launch per level index i, top down, over the views with
@@ -31,13 +31,13 @@ namespace rc {
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-enum WlKind { WL_OCC, WL_COLS, WL_ROWS, WL_MARCH, WL_LEVEL, WL_LEVEL0 };
+enum WlKind { WL_OCC, WL_ENDS, WL_ROWS, WL_MARCH, WL_LEVEL, WL_LEVEL0 };
 
 struct WlView {
     int W, H, L;
     const float *depth, *emissive;
     float *irr;
-    size_t bits, g, sdf, emis4, codes[RC_MAX_LEVELS], level[RC_MAX_LEVELS];
+    size_t bits, ends, sdf, emis4, codes[RC_MAX_LEVELS], level[RC_MAX_LEVELS];
 };
 
 struct WlLaunch {
@@ -150,27 +150,27 @@ static hipError_t build_tables(WorkList &w, std::string &err)
     const int n = (int)w.v.size();
     bool ok = true;
 
-    {  // exact distance transform: occupancy words, vertical distances, rows
+    {  // exact distance transform: occupancy words, band ends, rows
         std::vector<OccJob> jo;
-        std::vector<ColJob> jc;
+        std::vector<EndsJob> je;
         std::vector<RowJob> jr;
-        std::vector<int64_t> bo, bc, br;
+        std::vector<int64_t> bo, be, br;
         int wmax = 0;
         for (const WlView &x : w.v) {
             uint64_t *bits = (uint64_t *)(w.ws + x.bits);
-            uint16_t *g = (uint16_t *)(w.ws + x.g);
+            uint16_t *ends = (uint16_t *)(w.ws + x.ends);
             jo.push_back(OccJob{x.depth, x.emissive, bits, (float4 *)(w.ws + x.emis4), DepthConv{}, x.W, x.H,
                                 occ_bx(x.W)});
             bo.push_back(occ_blocks(x.W, x.H));
-            jc.push_back(ColJob{bits, g, x.W, x.H, occ_bx(x.W)});
-            bc.push_back(cols_blocks(x.W, x.H));
-            jr.push_back(RowJob{g, w.ws + x.sdf, x.W});
+            je.push_back(EndsJob{bits, ends, x.W, x.H});
+            be.push_back(ends_blocks(x.W));
+            jr.push_back(RowJob{bits, ends, w.ws + x.sdf, x.W, x.H});
             br.push_back(x.H);
             wmax = std::max(wmax, edt_row_wmax(x.W));
         }
         ok = ok && add_launch(w, blob, WL_OCC, 0, jo, bo, 1024, err);
-        ok = ok && add_launch(w, blob, WL_COLS, 0, jc, bc, kBlock, err);
-        ok = ok && add_launch(w, blob, WL_ROWS, wmax, jr, br, 64, err);  // one wave per row
+        ok = ok && add_launch(w, blob, WL_ENDS, 0, je, be, kBlock, err);
+        ok = ok && add_launch(w, blob, WL_ROWS, wmax, jr, br, kBlock, err);
     }
 
     if (w.split && ok) {  // split march: every (view, level >= 1), top level first
@@ -251,7 +251,7 @@ static hipError_t enqueue_launch(const WorkList &w, const WlLaunch &L, hipStream
     const BatchLaunch b{w.tables + L.jobs, (const int *)(w.tables + L.begins), L.njobs, L.blocks};
     switch (L.kind) {
     case WL_OCC: return launch_occ_batch(b, s);
-    case WL_COLS: return launch_cols_batch(b, s);
+    case WL_ENDS: return launch_ends_batch(b, s);
     case WL_ROWS: return launch_rows_batch(b, L.arg, s);
     case WL_MARCH: return launch_march_batch(b, w.spec, w.m, s);
     case WL_LEVEL: return launch_level_batch(b, w.st, w.tile[L.arg], w.rpt[L.arg], w.split != 0, s);
@@ -307,8 +307,8 @@ hipError_t worklist_create(const rc_view *views, uint32_t n, const uint32_t *lev
         const size_t px = (size_t)x.W * x.H;
         x.bits = off;
         off = align256(off + edt_bits_bytes(x.W, x.H));
-        x.g = off;
-        off = align256(off + edt_g_bytes(x.W, x.H));
+        x.ends = off;
+        off = align256(off + edt_ends_bytes(x.W, x.H));
         x.sdf = off;
         off = align256(off + sdf_elems(x.W, x.H));
         x.emis4 = off;

@@ -17,10 +17,12 @@
 //                bits[y / 64][x] bit y % 64 (one ballot-free LDS transpose of
 //                16 waves' 4-row nibbles), and the interleaved emission of
 //                occupied pixels for the merge kernels;
-//   k_edt_cols : g of one 64-row band of 64 columns per wave: the band's word
-//                and the nearest non-empty words above and below (8 words per
-//                round in flight), 16-bit g rows (0xFFFF: the column is empty);
-//   k_edt_rows : one workgroup per row, the row's g in LDS.  The smallest argmin
+//   k_edt_ends : per 64-row band and column, the nearest occupied rows above
+//                and below the band (one sweep per column and direction, 8
+//                words in flight): with the band's word they give g(x, y) in
+//                a few bit operations, so g itself is never stored;
+//   k_edt_rows : one workgroup per row, the row's g (from the words and the
+//                band ends) in LDS.  The smallest argmin
 //                opt(x) of (x - x')^2 + g(x')^2 never decreases with x (for
 //                x1 < x2 and j < opt(x1), f_j(x2) - f_opt(x2) = f_j(x1) -
 //                f_opt(x1) + 2 (x2 - x1)(opt - j) > 0), so it is found by divide
@@ -40,7 +42,7 @@ constexpr uint32_t kEdtBig = 1u << 29;        // value of an empty column: above
 
 __host__ __device__ inline int edt_bands(int H) { return (H + 63) >> 6; }
 size_t edt_bits_bytes(int W, int H) { return (size_t)edt_bands(H) * W * sizeof(uint64_t); }
-size_t edt_g_bytes(int W, int H) { return (size_t)W * H * sizeof(uint16_t); }
+size_t edt_ends_bytes(int W, int H) { return 2 * (size_t)edt_bands(H) * W * sizeof(uint16_t); }
 
 // ---- occupancy as column words --------------------------------------------
 __device__ __forceinline__ void occ_bits_body(const float *__restrict__ depth, const DepthConv &dc,
@@ -92,60 +94,67 @@ __global__ __launch_bounds__(kOccBlock) void k_occ_batch(const OccJob *__restric
     occ_bits_body(J.depth, J.dc, J.emissive, J.bits, J.emis4, J.W, J.H, J.bx_count, b - begins[j]);
 }
 
-// ---- vertical distances -------------------------------------------------------
-// Wave `wid` of a view: columns 64 (wid % bx_count) + lane of band wid / bx_count.
-__device__ __forceinline__ void edt_cols_body(const uint64_t *__restrict__ bits, uint16_t *__restrict__ g, int W,
-                                              int H, int bx_count, int wid)
+// ---- band ends -------------------------------------------------------------
+// For every 64-row band w and column x, the nearest occupied row above the
+// band and below it: ends[w][x] (rows < 64 w) and ends[nb + w][x] (rows >=
+// 64 (w + 1)), 0xFFFF for none.  One lane sweeps one column, down for the
+// rows above (lanes [0, W)) or up for the rows below (lanes [W, 2W)), its
+// band words loaded 8 at a time.  With the band's own word, these give any
+// pixel's vertical distance g in a few bit operations (edt_g).
+__device__ __forceinline__ void edt_ends_body(const uint64_t *__restrict__ bits, uint16_t *__restrict__ ends, int W,
+                                              int H, int i)
 {
     const int nb = edt_bands(H);
-    const int band = wid / bx_count;
-    const int x = (wid % bx_count) * 64 + (int)(threadIdx.x & 63);
-    if (band >= nb || x >= W) return;  // no cross-lane operation follows
-    const uint64_t word = bits[(size_t)band * W + x];
-    // nearest occupied row above the band and below it (-1: none)
-    int above = -1, below = -1;
-    for (int w0 = band - 1; w0 >= 0 && above < 0; w0 -= 8) {
+    if (i >= 2 * W) return;
+    const bool down = i < W;
+    const int x = down ? i : i - W;
+    uint32_t near = kGNone;  // the occupied row nearest the band boundary reached so far
+    for (int k0 = 0; k0 < nb; k0 += 8) {
         uint64_t v[8];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) v[k] = w0 - k >= 0 ? bits[(size_t)(w0 - k) * W + x] : 0ull;
+        for (int k = 0; k < 8; ++k) {
+            const int w = down ? k0 + k : nb - 1 - (k0 + k);
+            v[k] = k0 + k < nb ? bits[(size_t)w * W + x] : 0ull;
+        }
 #pragma unroll
-        for (int k = 0; k < 8; ++k)
-            if (above < 0 && v[k]) above = (w0 - k) * 64 + 63 - __builtin_clzll(v[k]);
-    }
-    for (int w0 = band + 1; w0 < nb && below < 0; w0 += 8) {
-        uint64_t v[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) v[k] = w0 + k < nb ? bits[(size_t)(w0 + k) * W + x] : 0ull;
-#pragma unroll
-        for (int k = 0; k < 8; ++k)
-            if (below < 0 && v[k]) below = (w0 + k) * 64 + __builtin_ctzll(v[k]);
-    }
-    const int y0 = band * 64, rows = min(64, H - y0);
-    uint16_t *out = g + (size_t)y0 * W + x;
-    for (int r = 0; r < rows; ++r) {
-        const int y = y0 + r;
-        const uint64_t le = word & ((2ull << r) - 1);  // rows <= r of the band (all 64 for r = 63)
-        const uint64_t ge = word >> r;                 // rows >= r
-        const uint32_t up = le ? (uint32_t)(r - 63 + __builtin_clzll(le)) : (above >= 0 ? (uint32_t)(y - above) : kGNone);
-        const uint32_t dn = ge ? (uint32_t)__builtin_ctzll(ge) : (below >= 0 ? (uint32_t)(below - y) : kGNone);
-        out[(size_t)r * W] = (uint16_t)min(up, dn);
+        for (int k = 0; k < 8; ++k) {
+            if (k0 + k >= nb) break;
+            const int w = down ? k0 + k : nb - 1 - (k0 + k);
+            ends[(size_t)(down ? w : nb + w) * W + x] = (uint16_t)near;
+            if (v[k]) near = (uint32_t)(down ? w * 64 + 63 - __builtin_clzll(v[k]) : w * 64 + __builtin_ctzll(v[k]));
+        }
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_edt_cols(const uint64_t *__restrict__ bits, uint16_t *__restrict__ g,
-                                                     int W, int H, int bx_count, const int *gate)
+__global__ __launch_bounds__(kBlock) void k_edt_ends(const uint64_t *__restrict__ bits, uint16_t *__restrict__ ends,
+                                                     int W, int H, const int *gate)
 {
     if (gate && *gate == 0) return;
-    edt_cols_body(bits, g, W, H, bx_count, xcd_band(blockIdx.x, gridDim.x) * 4 + (int)(threadIdx.x >> 6));
+    edt_ends_body(bits, ends, W, H, (int)(blockIdx.x * kBlock + threadIdx.x));
 }
 
-__global__ __launch_bounds__(kBlock) void k_cols_batch(const ColJob *__restrict__ jobs,
+__global__ __launch_bounds__(kBlock) void k_ends_batch(const EndsJob *__restrict__ jobs,
                                                        const int *__restrict__ begins, int njobs)
 {
-    const int b = xcd_band(blockIdx.x, gridDim.x);
+    const int b = (int)blockIdx.x;
     const int j = batch_job(begins, njobs, b);
-    const ColJob &J = jobs[j];
-    edt_cols_body(J.bits, J.g, J.W, J.H, J.bx_count, (b - begins[j]) * 4 + (int)(threadIdx.x >> 6));
+    const EndsJob &J = jobs[j];
+    edt_ends_body(J.bits, J.ends, J.W, J.H, (b - begins[j]) * kBlock + (int)threadIdx.x);
+}
+
+// Vertical distance of pixel (x, y) to the nearest occupied pixel of its
+// column (kGNone: the column has none): the band's word, else the band ends.
+__device__ __forceinline__ uint32_t edt_g(const uint64_t *__restrict__ bits, const uint16_t *__restrict__ ends, int W,
+                                          int nb, int x, int y)
+{
+    const int w = y >> 6, r = y & 63;
+    const uint64_t word = bits[(size_t)w * W + x];
+    const uint32_t ua = ends[(size_t)w * W + x], db = ends[(size_t)(nb + w) * W + x];
+    const uint64_t le = word & ((2ull << r) - 1);  // rows <= r of the band (all 64 for r = 63)
+    const uint64_t ge = word >> r;                 // rows >= r
+    const uint32_t up = le ? (uint32_t)(r - 63 + __builtin_clzll(le)) : (ua != kGNone ? (uint32_t)y - ua : kGNone);
+    const uint32_t dn = ge ? (uint32_t)__builtin_ctzll(ge) : (db != kGNone ? db - (uint32_t)y : kGNone);
+    return up < dn ? up : dn;
 }
 
 // ---- per-row lower envelope -------------------------------------------------------
@@ -170,14 +179,15 @@ __device__ __forceinline__ int edt_hi(int q, int s, int W, const uint16_t *opt)
 // (ties go to the smallest candidate), and meets the other lanes of a query
 // in LDS by a 64-bit atomic minimum of (value << 14 | candidate).
 template <int WMAX, int RL>
-__device__ __forceinline__ void edt_row_body(const uint16_t *__restrict__ g, uint8_t *__restrict__ sdf, int W, int y)
+__device__ __forceinline__ void edt_row_body(const uint64_t *__restrict__ bits, const uint16_t *__restrict__ ends,
+                                             uint8_t *__restrict__ sdf, int W, int H, int y)
 {
     __shared__ uint16_t gs[WMAX];
     __shared__ uint16_t opt[WMAX];                 // opt[x]: smallest argmin at x
     __shared__ unsigned long long best[WMAX / 2];  // per query of the current level (at most ceil(W / 2))
     const int tid = (int)threadIdx.x;
-    const uint16_t *gr = g + (size_t)y * W;
-    for (int x = tid; x < W; x += RL) gs[x] = gr[x];
+    const int nb = edt_bands(H);
+    for (int x = tid; x < W; x += RL) gs[x] = (uint16_t)edt_g(bits, ends, W, nb, x, y);
     if (tid == 0) best[0] = ~0ull;  // level 0 has one query
     __syncthreads();
     const int K = 32 - __builtin_clz((uint32_t)W);  // smallest K with 2^K > W
@@ -241,11 +251,12 @@ __device__ __forceinline__ void edt_row_body(const uint16_t *__restrict__ g, uin
 }
 
 template <int WMAX>
-__global__ __launch_bounds__(kRowLanes) void k_edt_rows(const uint16_t *__restrict__ g, uint8_t *__restrict__ sdf,
-                                                        int W, const int *gate)
+__global__ __launch_bounds__(kRowLanes) void k_edt_rows(const uint64_t *__restrict__ bits,
+                                                        const uint16_t *__restrict__ ends, uint8_t *__restrict__ sdf,
+                                                        int W, int H, const int *gate)
 {
     if (gate && *gate == 0) return;
-    edt_row_body<WMAX, kRowLanes>(g, sdf, W, xcd_band(blockIdx.x, gridDim.x));
+    edt_row_body<WMAX, kRowLanes>(bits, ends, sdf, W, H, xcd_band(blockIdx.x, gridDim.x));
 }
 
 template <int WMAX>
@@ -255,13 +266,13 @@ __global__ __launch_bounds__(kRowLanes) void k_rows_batch(const RowJob *__restri
     const int b = xcd_band(blockIdx.x, gridDim.x);
     const int j = batch_job(begins, njobs, b);
     const RowJob &J = jobs[j];
-    edt_row_body<WMAX, kRowLanes>(J.g, J.sdf, J.W, b - begins[j]);
+    edt_row_body<WMAX, kRowLanes>(J.bits, J.ends, J.sdf, J.W, J.H, b - begins[j]);
 }
 
 // ---- launchers ---------------------------------------------------------------------
 int occ_blocks(int W, int H) { return ((W + 63) / 64) * edt_bands(H); }
 int occ_bx(int W) { return (W + 63) / 64; }
-int cols_blocks(int W, int H) { return (occ_blocks(W, H) + 3) / 4; }
+int ends_blocks(int W) { return (2 * W + kBlock - 1) / kBlock; }
 int edt_row_wmax(int W)
 {
     int m = 1024;
@@ -270,12 +281,13 @@ int edt_row_wmax(int W)
 }
 
 template <int WMAX>
-static void launch_rows_t(const uint16_t *g, uint8_t *sdf, int W, int H, hipStream_t s, const int *gate)
+static void launch_rows_t(const uint64_t *bits, const uint16_t *ends, uint8_t *sdf, int W, int H, hipStream_t s,
+                          const int *gate)
 {
-    hipLaunchKernelGGL(k_edt_rows<WMAX>, dim3(H), dim3(kRowLanes), 0, s, g, sdf, W, gate);
+    hipLaunchKernelGGL(k_edt_rows<WMAX>, dim3(H), dim3(kRowLanes), 0, s, bits, ends, sdf, W, H, gate);
 }
 
-hipError_t launch_edt(const float *depth, const DepthConv &dc, const float *emissive, uint64_t *bits, uint16_t *g,
+hipError_t launch_edt(const float *depth, const DepthConv &dc, const float *emissive, uint64_t *bits, uint16_t *ends,
                       uint8_t *sdf, float4 *emis4, int W, int H, hipStream_t s, const int *gate, int stage)
 {
     if (W < 1 || H < 1 || W > RC_MAX_DIM || H > RC_MAX_DIM) return hipErrorInvalidValue;
@@ -284,14 +296,14 @@ hipError_t launch_edt(const float *depth, const DepthConv &dc, const float *emis
         hipLaunchKernelGGL(k_occ_bits, dim3(occ_blocks(W, H)), dim3(kOccBlock), 0, s, depth, dc, emissive, bits, emis4,
                            W, H, bx, gate);
     if (stage & 2) {
-        hipLaunchKernelGGL(k_edt_cols, dim3(cols_blocks(W, H)), dim3(kBlock), 0, s, (const uint64_t *)bits, g, W, H,
-                           bx, gate);
+        const uint64_t *cb = bits;
+        hipLaunchKernelGGL(k_edt_ends, dim3(ends_blocks(W)), dim3(kBlock), 0, s, cb, ends, W, H, gate);
         switch (edt_row_wmax(W)) {
-        case 1024: launch_rows_t<1024>(g, sdf, W, H, s, gate); break;
-        case 2048: launch_rows_t<2048>(g, sdf, W, H, s, gate); break;
-        case 4096: launch_rows_t<4096>(g, sdf, W, H, s, gate); break;
-        case 8192: launch_rows_t<8192>(g, sdf, W, H, s, gate); break;
-        default: launch_rows_t<16384>(g, sdf, W, H, s, gate); break;
+        case 1024: launch_rows_t<1024>(cb, ends, sdf, W, H, s, gate); break;
+        case 2048: launch_rows_t<2048>(cb, ends, sdf, W, H, s, gate); break;
+        case 4096: launch_rows_t<4096>(cb, ends, sdf, W, H, s, gate); break;
+        case 8192: launch_rows_t<8192>(cb, ends, sdf, W, H, s, gate); break;
+        default: launch_rows_t<16384>(cb, ends, sdf, W, H, s, gate); break;
         }
     }
     return hipGetLastError();
@@ -304,10 +316,10 @@ hipError_t launch_occ_batch(const BatchLaunch &b, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_cols_batch(const BatchLaunch &b, hipStream_t s)
+hipError_t launch_ends_batch(const BatchLaunch &b, hipStream_t s)
 {
     if (b.blocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_cols_batch, dim3(b.blocks), dim3(kBlock), 0, s, (const ColJob *)b.jobs, b.begins, b.njobs);
+    hipLaunchKernelGGL(k_ends_batch, dim3(b.blocks), dim3(kBlock), 0, s, (const EndsJob *)b.jobs, b.begins, b.njobs);
     return hipGetLastError();
 }
 

@@ -85,12 +85,13 @@ hipError_t launch_jfa_tail(const uint32_t *in, uint32_t *seeds_out, uint8_t *sdf
                            hipStream_t s, const int *gate = nullptr);
 
 // ---- exact distance transform (rc_edt.hip) ------------------------------------
-// Buffers: bits = column words [ceil(H/64)][W] (u64), g = vertical distances
-// [H][W] (u16).  stage: 1 = the occupancy kernel (bits + emis4), 2 = the
-// column and row kernels (g, then the SDF), 3 = all three.
+// Buffers: bits = column words [ceil(H/64)][W] (u64), ends = the nearest
+// occupied rows above / below each 64-row band [2][ceil(H/64)][W] (u16).
+// stage: 1 = the occupancy kernel (bits + emis4), 2 = the band-end and row
+// kernels (ends, then the SDF), 3 = all three.
 size_t edt_bits_bytes(int W, int H);
-size_t edt_g_bytes(int W, int H);
-hipError_t launch_edt(const float *depth, const DepthConv &dc, const float *emissive, uint64_t *bits, uint16_t *g,
+size_t edt_ends_bytes(int W, int H);
+hipError_t launch_edt(const float *depth, const DepthConv &dc, const float *emissive, uint64_t *bits, uint16_t *ends,
                       uint8_t *sdf, float4 *emis4, int W, int H, hipStream_t s, const int *gate = nullptr,
                       int stage = 3);
 
@@ -130,22 +131,23 @@ struct OccJob {
     DepthConv dc;
     int W, H, bx_count;
 };
-struct ColJob {
+struct EndsJob {
     const uint64_t *bits;
-    uint16_t *g;
-    int W, H, bx_count;
+    uint16_t *ends;
+    int W, H;
 };
 struct RowJob {
-    const uint16_t *g;
+    const uint64_t *bits;
+    const uint16_t *ends;
     uint8_t *sdf;
-    int W;
+    int W, H;
 };
 int occ_blocks(int W, int H);   // 1024-lane workgroups of the occupancy kernel
 int occ_bx(int W);
-int cols_blocks(int W, int H);  // 256-lane workgroups of the column kernel
+int ends_blocks(int W);         // 256-lane workgroups of the band-end kernel
 int edt_row_wmax(int W);        // LDS row capacity of the row kernel (1024 .. 16384)
 hipError_t launch_occ_batch(const BatchLaunch &b, hipStream_t s);
-hipError_t launch_cols_batch(const BatchLaunch &b, hipStream_t s);
+hipError_t launch_ends_batch(const BatchLaunch &b, hipStream_t s);
 hipError_t launch_rows_batch(const BatchLaunch &b, int wmax, hipStream_t s);  // one workgroup per row
 
 // ---- cascades (rc_cascade.hip) ----------------------------------------------

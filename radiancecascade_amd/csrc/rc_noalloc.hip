@@ -4,7 +4,7 @@
 // loops, harnesses that forbid hipMalloc).
 //
 // The frame's working buffers -- the exact transform's occupancy words and
-// vertical distances, the distance field and the merged levels 1..L-1 --
+// band ends, the distance field and the merged levels 1..L-1 --
 // live in one file-scope device array,
 // `scratch`, of kNoAllocScratchBytes; the per-level direction tables in a
 // second one, `scratch_dirs`.  Each GPU the library runs on holds its own copy
@@ -49,7 +49,7 @@ static NoAllocDevice g_noalloc[kNoAllocMaxDevices];
 static std::mutex g_noalloc_mu;
 
 struct NoAllocLayout {
-    size_t bits, g, sdf, level[RC_MAX_LEVELS], total;
+    size_t bits, ends, sdf, level[RC_MAX_LEVELS], total;
 };
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -60,8 +60,8 @@ static NoAllocLayout noalloc_layout(int W, int H, int L)
     size_t off = 0;
     m.bits = off;
     off = align256(off + edt_bits_bytes(W, H));
-    m.g = off;
-    off = align256(off + edt_g_bytes(W, H));
+    m.ends = off;
+    off = align256(off + edt_ends_bytes(W, H));
     m.sdf = off;
     off = align256(off + sdf_elems(W, H));
     for (int i = 1; i < L; ++i) {
@@ -163,7 +163,7 @@ rc_status rc_run_device_noalloc(uint32_t width, uint32_t height, uint32_t levels
                                                            hipGetErrorString(e_));                             \
     } while (0)
     NTRY(launch_edt(g->depth, linear, g->emissive, (uint64_t *)(d->scratch + lay.bits),
-                    (uint16_t *)(d->scratch + lay.g), sdf, nullptr, W, H, s));
+                    (uint16_t *)(d->scratch + lay.ends), sdf, nullptr, W, H, s));
     for (int i = L - 1; i >= 0; --i) {
         LevelArgs a{};
         a.sdf = sdf;
