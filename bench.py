@@ -56,9 +56,10 @@ def family_alg_bytes(W, H, L, storage, rays, split, jfa_passes, top_in_march=Fal
       seed   : depth + emissive read (16 B/px); written: the exact mode's
                occupancy bits (1/8 B/px), or the jump flood's seed field
                (4 B/px);
-      jfa    : exact mode: the column kernel reads the bits (1/8 B/px) and
-               writes 16-bit vertical distances (2 B/px), the row kernel
-               reads them (2 B/px) and writes the 1-byte distance field;
+      jfa    : exact mode: the band-end kernel reads the bits (1/8 B/px)
+               and writes 2 x 2 B per band and column (1/16 B/px), the row
+               kernel reads the bits and band ends of its row (1/8 + 1/16
+               B/px) and writes the 1-byte distance field;
                jump flood: 8 B/px per pass (seed field read and written
                once), the last pass reads 4 B/px and writes the field;
       march  : the 1-byte distance field read once, a 4-byte code per ray of
@@ -79,7 +80,7 @@ def family_alg_bytes(W, H, L, storage, rays, split, jfa_passes, top_in_march=Fal
         levels += b * tex[i] + (b * tex[i + 1] if i + 1 < L else 0) + (4 * rays[i] if split else 0)
     march = (n + 4 * sum(rays[1:L]) + (b * tex[L - 1] if top_in_march else 0)) if split else 0
     if sdf == "exact":
-        seed, jfa = 16 * n + n / 8, (n / 8 + 2 * n) + (2 * n + n)
+        seed, jfa = 16 * n + n / 8, (n / 8 + n / 16) + (n / 8 + n / 16 + n)
     else:
         seed, jfa = 20 * n, (8 * (jfa_passes - 1) + 5) * n if jfa_passes > 0 else 5 * n
     return {"seed": seed,
@@ -374,7 +375,7 @@ def batch_leg(rc, torch, args, stream, fps_plain):
 
 
 # kernel name prefix -> family (the plain frame's kernels)
-KERNEL_FAMILY = [("k_seed_init", "seed"), ("k_occ_bits", "seed"), ("k_edt_cols", "jfa"), ("k_edt_rows", "jfa"),
+KERNEL_FAMILY = [("k_seed_init", "seed"), ("k_occ_bits", "seed"), ("k_edt_ends", "jfa"), ("k_edt_rows", "jfa"),
                  ("k_jfa_pass", "jfa"), ("k_jfa_tail", "jfa"), ("k_jfa_none", "jfa"),
                  ("k_march_all", "march"), ("k_march_persist", "march"), ("k_level0", "level0"),
                  ("k_level<", "levels")]
