@@ -159,8 +159,9 @@ class RadianceCascadeStage:
         return out
 
     def set_sdf_mode(self, mode: str) -> None:
-        """Distance transform: "exact" (the default; occupancy, column and row
-        kernels) or "jfa" (the jump flood, whose seed field debug_seeds reads)."""
+        """Distance transform: "exact" (the default; occupancy, band-end and
+        row kernels) or "jfa" (the jump flood, whose seed field debug_seeds
+        reads)."""
         check(self._lib.rc_set_sdf_mode(self._ctx, _abi.SDF_MODES[mode]), self._ctx)
 
     def sdf_mode(self) -> str:

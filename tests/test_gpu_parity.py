@@ -600,6 +600,48 @@ def test_error_norms_fp64(rc, oracle):
     assert c32["max_abs"] == float(np.abs(x.astype(np.float64) - r32).max())
 
 
+def test_sdf_mode_switches(rc, oracle):
+    """One context switched between the distance transforms (each mode's
+    buffers allocated on first use, kept across switches and a resize): every
+    frame equals the oracle of its mode bit for bit, the seed field exists in
+    the jump-flood mode only, and an unknown mode is refused."""
+    W, H, L, seed = 331, 197, 5, 3
+    _say(f"sdf mode switches {W}x{H}")
+    depth, emis = _device_gbuffer(rc, W, H, seed, 32, 4)
+    d, _, e = oracle.gbuffer(seed, W, H, 32, 4)
+    irr = torch.empty((3, H, W), dtype=torch.float32, device="cuda:0")
+    seeds = torch.empty((H, W), dtype=torch.int32, device="cuda:0")
+    with rc.RadianceCascadeStage(W, H, levels=L, debug=True) as st:
+        for mode in ("exact", "jfa", "exact", "jfa", "jfa", "exact"):
+            st.set_sdf_mode(mode)
+            assert st.sdf_mode() == mode
+            st.run_device(depth, emis, irr)
+            torch.cuda.synchronize()
+            ref = oracle.run(d, e, L, np.float32, sdf=mode)
+            assert np.array_equal(irr.cpu().numpy().view(np.uint32), ref.view(np.uint32)), mode
+            if mode == "jfa":
+                st.debug_seeds(seeds)
+                torch.cuda.synchronize()
+                assert np.array_equal(seeds.cpu().numpy().view(np.uint32), oracle.jfa(oracle.seed_init(d, e)))
+            else:
+                with pytest.raises(rc.RcError):
+                    st.debug_seeds(seeds)
+        assert st._lib.rc_set_sdf_mode(st._ctx, 7) == rc._abi.ERR_INVALID
+        assert st.sdf_mode() == "exact"
+        # a resize keeps the mode and allocates the new size's buffers
+        st.set_sdf_mode("jfa")
+        W2, H2 = 97, 61
+        st.resize(W2, H2, L)
+        assert st.sdf_mode() == "jfa"
+        d2, e2 = _device_gbuffer(rc, W2, H2, seed, 16, 6)
+        irr2 = torch.empty((3, H2, W2), dtype=torch.float32, device="cuda:0")
+        st.run_device(d2, e2, irr2)
+        torch.cuda.synchronize()
+        dd, _, ee = oracle.gbuffer(seed, W2, H2, 16, 6)
+        ref = oracle.run(dd, ee, L, np.float32, sdf="jfa")
+        assert np.array_equal(irr2.cpu().numpy().view(np.uint32), ref.view(np.uint32))
+
+
 @pytest.mark.parametrize("W,H,L,seed,nb,ne", [(256, 256, 4, 7, 8, 2), (1920, 1080, 6, 1, 32, 4)])
 def test_ndc_depth_input(rc, oracle, W, H, L, seed, nb, ne):
     """The reference renderer's depth target holds perspective_rh NDC depth
