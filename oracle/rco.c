@@ -1,6 +1,7 @@
 /*
  * rco.c -- CPU oracle: scene generator, G-buffer raster, cascade geometry,
- * occupancy and jump flood.  TEST INFRASTRUCTURE ONLY (see rco.h): only the
+ * occupancy, exact distance transform and jump flood.  TEST INFRASTRUCTURE
+ * ONLY (see rco.h): only the
  * tests, the smoke check and bench.py's cpu_baseline leg load it.
  *
  * Built with -ffp-contract=off so every float/double operation is the single

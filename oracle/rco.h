@@ -10,8 +10,11 @@
  * radiance-cascade code: its only pass is the Blinn-Phong forward raster
  * (src/renderer.rs:559-613, src/shader.wgsl:76-101).  The maths restated here
  * therefore follows the published method -- Sannikov, "Radiance Cascades"
- * (2023) for the cascade hierarchy and bilinear merge, Rong & Tan, "Jump
- * Flooding in GPU" (I3D 2006) for the distance field -- with every free choice
+ * (2023) for the cascade hierarchy and bilinear merge; for the distance field
+ * the exact Euclidean distance transform (Felzenszwalb & Huttenlocher,
+ * "Distance Transforms of Sampled Functions", Theory of Computing 8, 2012),
+ * or on request Rong & Tan, "Jump Flooding in GPU" (I3D 2006) -- with every
+ * free choice
  * frozen in the specification section of the design notes.  Where the path
  * touches the reference (depth convention, pass boundary, albedo) the function
  * cites the reference file:line it follows.
@@ -25,8 +28,10 @@
  * pixel indices) have one flavour and are compared bit-exactly.
  *
  * Parity pins (see tests/test_oracle_*.py): SplitMix64 published vector;
- * raster closed forms (room-only depth); JFA == brute-force nearest seed for a
- * single seed and within the JFA error bound otherwise; empty-scene and
+ * raster closed forms (room-only depth); exact transform == scipy's exact EDT
+ * and the brute-force nearest seed on fields of every density and shape;
+ * JFA == brute-force nearest seed for a single seed and within the JFA error
+ * bound otherwise; empty-scene and
  * constant-upper-level closed forms of march/merge/gather; analytic
  * emissive-disc hit test of the march; FP32 vs FP64 agreement.
  */
