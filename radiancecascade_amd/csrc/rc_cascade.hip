@@ -60,7 +60,7 @@ __device__ inline void ray_round(Ray &r, __amdgpu_buffer_rsrc_t sdf, int W, int 
         const int iy = floor_i32(fmaf(p[j], r.dy, r.cy));
         ins[j] = (uint32_t)ix < (uint32_t)W && (uint32_t)iy < (uint32_t)H;
         q[j] = pixel_index(ix, iy, W);
-        d[j] = sdf_load(sdf, ix, iy, sdf_tiles_x(W));
+        d[j] = sdf_load(sdf, ix, iy, sdf_skip(H));
     }
     bool go = true;
 #pragma unroll
@@ -308,13 +308,13 @@ __device__ __forceinline__ void level0_body(const LevelArgs &a, float *__restric
     const int px = px0 - 1 + sx, py = py0 - 1 + sy;
     const bool probe = slot < NX * NY && px >= 0 && py >= 0 && px < a.PX && py < a.PY;
     const __amdgpu_buffer_rsrc_t sdf = sdf_rsrc(a.sdf, a.W, a.H);
-    const int tiles = sdf_tiles_x(a.W);
+    const int skip = sdf_skip(a.H);
     const float cx = ((float)px + 0.5f) * 2.0f, cy = ((float)py + 0.5f) * 2.0f;
     const int ix = 2 * px + 1, iy = 2 * py + 1;  // floor of the probe centre (exact)
     const bool in0 = probe && (uint32_t)ix < (uint32_t)a.W && (uint32_t)iy < (uint32_t)a.H;
     // the first distance sample is issued before the upper level's loads, so
     // the march does not wait for them; they land in LDS after the march
-    const float d0 = in0 ? sdf_load(sdf, ix, iy, tiles) : 0.0f;
+    const float d0 = in0 ? sdf_load(sdf, ix, iy, skip) : 0.0f;
     // the emission of the first sample's pixel (the usual hit) in the same wave
     // of loads, so a hit there needs no dependent load
     const int q0 = in0 ? pixel_index(ix, iy, a.W) : 0;
@@ -355,7 +355,7 @@ __device__ __forceinline__ void level0_body(const LevelArgs &a, float *__restric
                 const int jx = floor_i32(fmaf(t, dir.x, cx)), jy = floor_i32(fmaf(t, dir.y, cy));
                 if (!((uint32_t)jx < (uint32_t)a.W && (uint32_t)jy < (uint32_t)a.H)) {
                     code[k] = kRayExit;
-                } else if (sdf_load(sdf, jx, jy, tiles) < 0.5f) {
+                } else if (sdf_load(sdf, jx, jy, skip) < 0.5f) {
                     code[k] = pixel_index(jx, jy, a.W);
                 }
             }
@@ -614,7 +614,7 @@ __device__ __forceinline__ void march_all_body(const uint8_t *sdf_codes, int W, 
         at = ((__umul24((uint32_t)py, (uint32_t)L.PX) + (uint32_t)px) << L.log2n) + (uint32_t)k;
         const float2 dir = L.dirs[k];
         const float cx = ((float)px + 0.5f) * L.spacing, cy = ((float)py + 0.5f) * L.spacing;
-        const int tiles = sdf_tiles_x(W);
+        const int skip = sdf_skip(H);
         const float t1 = L.t1;
         float t = L.t0, guess = 1.0f;
         bool alive = valid && t < t1;
@@ -629,7 +629,7 @@ __device__ __forceinline__ void march_all_body(const uint8_t *sdf_codes, int W, 
             for (int j = 0; j < SPEC; ++j) {
                 const int ix = floor_i32(fmaf(p[j], dir.x, cx)), iy = floor_i32(fmaf(p[j], dir.y, cy));
                 ins[j] = (uint32_t)ix < (uint32_t)W && (uint32_t)iy < (uint32_t)H;
-                c[j] = sdf_code_load(sdf, ix, iy, tiles);
+                c[j] = sdf_code_load(sdf, ix, iy, skip);
             }
             bool go = true;
 #pragma unroll
@@ -673,13 +673,13 @@ __device__ __forceinline__ void march_all_body(const uint8_t *sdf_codes, int W, 
         at = ((__umul24((uint32_t)py, (uint32_t)L.PX) + (uint32_t)px) << L.log2n) + (uint32_t)k;
         const float2 dir = L.dirs[k];
         const float cx = ((float)px + 0.5f) * L.spacing, cy = ((float)py + 0.5f) * L.spacing;
-        const int tiles = sdf_tiles_x(W);
+        const int skip = sdf_skip(H);
         const float t1 = L.t1;
         float t = L.t0;
         bool alive = valid && t < t1;
         while (alive) {
             const int ix = floor_i32(fmaf(t, dir.x, cx)), iy = floor_i32(fmaf(t, dir.y, cy));
-            const uint32_t c = sdf_code_load(sdf, ix, iy, tiles);
+            const uint32_t c = sdf_code_load(sdf, ix, iy, skip);
             const bool ins = (uint32_t)ix < (uint32_t)W && (uint32_t)iy < (uint32_t)H;
             const float tn = t + step_of_code(c);
             t = ins ? tn : t;

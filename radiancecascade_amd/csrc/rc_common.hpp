@@ -33,7 +33,7 @@ __host__ __device__ inline int grid_dim(int pixels, int level)
 __host__ __device__ inline int64_t interval_t0(int level) { return (2 * ((1ll << (2 * level)) - 1)) / 3; }
 __host__ __device__ inline int64_t interval_t1(int level) { return (2 * ((1ll << (2 * level + 2)) - 1)) / 3; }
 
-// ---- distance field: one byte per pixel, 16x8-pixel tiles ------------------
+// ---- distance field: one byte per pixel, 16-pixel-wide strips --------------
 // The value stored is a conservative code of the distance d = sqrt(d^2) (in
 // px) to the nearest occupied pixel:
 //   code = floor(d)                        d < 128
@@ -43,8 +43,11 @@ __host__ __device__ inline int64_t interval_t1(int level) { return (2 * ((1ll <<
 // <= d, so a step never passes a surface; value 0 (a hit) is exactly d = 0.
 // At 1 byte per pixel the field of a 1920x1080 frame is 2 MB and stays in
 // each XCD's 4 MB L2 while the long rays of the upper levels sample it.  The
-// bytes are stored in 16x8-pixel tiles (one 128-byte line), tiles row-major:
-// the samples of a ray and of neighbouring rays share lines.
+// bytes are stored in vertical strips 16 pixels wide, each padded to a
+// multiple of 8 rows, one row of a strip every 16 bytes: an aligned 128-byte
+// line holds a 16x8-pixel block, so the samples of a ray and of neighbouring
+// rays share lines, and the byte of (x, y) is at x + 16 y + (x >> 4) * skip
+// with skip = 16 * rows - 16 (three instructions per sample in the march).
 constexpr uint32_t kSdfFar = 255;
 constexpr float kSdfFarValue = 2160.0f;
 __host__ __device__ inline uint32_t sdf_code_of_distance(uint32_t d)  // d = floor of the distance
@@ -63,12 +66,12 @@ __host__ __device__ inline uint32_t isqrt_u32(uint32_t d2)
     while ((uint64_t)(r + 1) * (r + 1) <= d2) ++r;
     return r;
 }
-__host__ __device__ inline int sdf_tiles_x(int W) { return (W + 15) >> 4; }
-__host__ __device__ inline size_t sdf_elems(int W, int H) { return (size_t)sdf_tiles_x(W) * 128 * (size_t)((H + 7) >> 3); }
-__host__ __device__ inline int sdf_index(int x, int y, int tiles_x)
-{
-    return (((y >> 3) * tiles_x + (x >> 4)) << 7) + ((y & 7) << 4) + (x & 15);
-}
+__host__ __device__ inline int sdf_strips(int W) { return (W + 15) >> 4; }
+__host__ __device__ inline int sdf_rows(int H) { return (H + 7) & ~7; }
+__host__ __device__ inline size_t sdf_elems(int W, int H) { return (size_t)sdf_strips(W) * 16 * (size_t)sdf_rows(H); }
+// the strip-to-strip step less the 16 bytes x itself advances per strip
+__host__ __device__ inline int sdf_skip(int H) { return 16 * sdf_rows(H) - 16; }
+__host__ __device__ inline int sdf_index(int x, int y, int skip) { return x + 16 * y + (x >> 4) * skip; }
 
 // ---- storage texels ---------------------------------------------------------
 // fp32: float4.  fp16 / bf16: four 16-bit lanes packed in a uint2 (8 bytes).

@@ -237,7 +237,7 @@ __device__ __forceinline__ void edt_row_body(const uint64_t *__restrict__ bits, 
         }
         __syncthreads();
     }
-    const int tx = sdf_tiles_x(W);
+    const int skip = sdf_skip(H);
     for (int x = tid; x < W; x += RL) {
         const int j = opt[x];
         const uint32_t gj = gs[j];
@@ -246,7 +246,7 @@ __device__ __forceinline__ void edt_row_body(const uint64_t *__restrict__ bits, 
             const int dx = x - j;
             code = sdf_code_of_distance(isqrt_u32((uint32_t)(dx * dx) + gj * gj));
         }
-        sdf[sdf_index(x, y, tx)] = (uint8_t)code;
+        sdf[sdf_index(x, y, skip)] = (uint8_t)code;
     }
 }
 

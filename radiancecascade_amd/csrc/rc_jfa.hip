@@ -162,7 +162,7 @@ __device__ __forceinline__ void jfa_pass_body(const uint32_t *__restrict__ in, u
             const uint32_t bs = (uint32_t)best, bd2 = (uint32_t)(best >> 32);
             const size_t p = (size_t)y * W + x;
             if (FINAL) {
-                sdf[sdf_index(x, y, sdf_tiles_x(W))] = seed_sdf_code(bs, bd2);
+                sdf[sdf_index(x, y, sdf_skip(H))] = seed_sdf_code(bs, bd2);
                 if (out) out[p] = bs == kFarSeed ? kSeedNone : bs;
             } else {
                 out[p] = bs;
@@ -272,7 +272,7 @@ __device__ __forceinline__ void jfa_tail_body(const uint32_t *__restrict__ in, u
             const int gy = y0 + HALO + ty;
             if (ty >= T || gx >= W || gy >= H) continue;
             const uint32_t sd = buf[cur][(HALO + ty) * R + HALO + cx];
-            sdf[sdf_index(gx, gy, sdf_tiles_x(W))] = seed_sdf_code(sd, last_d2[j]);
+            sdf[sdf_index(gx, gy, sdf_skip(H))] = seed_sdf_code(sd, last_d2[j]);
             if (seeds_out) seeds_out[(size_t)gy * W + gx] = sd == kFarSeed ? kSeedNone : sd;
         }
     }
@@ -295,7 +295,7 @@ __global__ void k_jfa_none(const uint32_t *__restrict__ in, uint32_t *__restrict
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t s = in[i];
-    sdf[sdf_index(i % W, i / W, sdf_tiles_x(W))] = (uint8_t)(s == kFarSeed ? kSdfFar : 0u);
+    sdf[sdf_index(i % W, i / W, sdf_skip(n / W))] = (uint8_t)(s == kFarSeed ? kSdfFar : 0u);
     if (seeds_out) seeds_out[i] = s == kFarSeed ? kSeedNone : s;
 }
 
@@ -304,7 +304,7 @@ __global__ void k_sdf_untile(const uint8_t *__restrict__ sdf, float *__restrict_
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= W * H) return;
-    out[i] = sdf_value(sdf[sdf_index(i % W, i / W, sdf_tiles_x(W))]);
+    out[i] = sdf_value(sdf[sdf_index(i % W, i / W, sdf_skip(H))]);
 }
 
 hipError_t launch_sdf_untile(const uint8_t *sdf, float *out, int W, int H, hipStream_t s)

@@ -1,5 +1,5 @@
 // rc_march.hpp -- device helpers shared by the cascade kernels and the
-// surrogate path: lane-quad sums, the tiled distance-field load, the interval
+// surrogate path: lane-quad sums, the distance-field load, the interval
 // march and the bilinear upper-probe interpolation.
 #pragma once
 
@@ -16,7 +16,7 @@ __device__ inline float quad_sum(float v)
     return v;
 }
 
-// The distance field is read through a buffer resource: the tiled index is a
+// The distance field is read through a buffer resource: the strip index is a
 // 32-bit lane offset (no 64-bit address arithmetic per step), and a lane whose
 // sample is off-image may compute any offset -- the load is range-checked and
 // its value unused.
@@ -32,33 +32,29 @@ __device__ inline __amdgpu_buffer_rsrc_t sdf_rsrc(const uint8_t *sdf, int W, int
 // hit code and the emissive index.
 __device__ inline int pixel_index(int x, int y, int W) { return (int)__umul24((uint32_t)y, (uint32_t)W) + x; }
 
-// Tiled distance load (one byte, decoded).  For an off-image (x, y) the
-// offset may be anything; the range-checked load is then unused.
-// (m & a) | (~m & b) in one VALU instruction.
-__device__ inline uint32_t bfi(uint32_t m, uint32_t a, uint32_t b)
+// Byte offset of (x, y) in the strip layout, sdf_index's value for an
+// in-image pixel (x >> 4 < 2^10 and skip < 2^18 fit the 24-bit multiplier):
+// a shift, a 24-bit multiply-add and a shift-add.  For an off-image (x, y)
+// the offset may be anything; the range-checked load is then unused.
+__device__ inline uint32_t sdf_offset(int x, int y, int skip)
 {
-    uint32_t r;
-    asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(r) : "s"(m), "v"(a), "v"(b));
+    const uint32_t ux = (uint32_t)x;
+    const uint32_t m = __umul24(ux >> 4, (uint32_t)skip) + ux;  // v_mad_u32_u24
+    uint32_t r;                                                  // (y << 4) + m
+    asm("v_lshl_add_u32 %0, %1, 4, %2" : "=v"(r) : "v"(y), "v"(m));
     return r;
 }
 
-// The in-tile offset is two bitfield inserts: bits 0-3 = x & 15, bits 4-6 =
-// y & 7, bits 7+ = x >> 4 (the tile column); plus tile row * tile bytes.
-__device__ inline float sdf_load(__amdgpu_buffer_rsrc_t r, int x, int y, int tiles_x)
+// Distance load (one byte, decoded).
+__device__ inline float sdf_load(__amdgpu_buffer_rsrc_t r, int x, int y, int skip)
 {
-    const uint32_t ux = (uint32_t)x, uy = (uint32_t)y;
-    const uint32_t in = bfi(127u, bfi(15u, ux, uy << 4), ux << 3);
-    const uint32_t off = __umul24(uy >> 3, (uint32_t)tiles_x << 7) + in;
-    return sdf_value(__builtin_amdgcn_raw_buffer_load_b8(r, (int)off, 0, 0));
+    return sdf_value(__builtin_amdgcn_raw_buffer_load_b8(r, (int)sdf_offset(x, y, skip), 0, 0));
 }
 
 // The raw byte code at (x, y) (same offset and range check as sdf_load).
-__device__ inline uint32_t sdf_code_load(__amdgpu_buffer_rsrc_t r, int x, int y, int tiles_x)
+__device__ inline uint32_t sdf_code_load(__amdgpu_buffer_rsrc_t r, int x, int y, int skip)
 {
-    const uint32_t ux = (uint32_t)x, uy = (uint32_t)y;
-    const uint32_t in = bfi(127u, bfi(15u, ux, uy << 4), ux << 3);
-    const uint32_t off = __umul24(uy >> 3, (uint32_t)tiles_x << 7) + in;
-    return __builtin_amdgcn_raw_buffer_load_b8(r, (int)off, 0, 0);
+    return __builtin_amdgcn_raw_buffer_load_b8(r, (int)sdf_offset(x, y, skip), 0, 0);
 }
 
 // march_step(sdf_value(c)) for a code c >= 1 (c = 0 is a hit and takes no
@@ -100,7 +96,7 @@ template <int RPT>
 __device__ inline void march_rays(__amdgpu_buffer_rsrc_t sdf, int W, int H, const float *cx, const float *cy,
                                   float dx, float dy, float t0, float t1, const bool *active, int *code)
 {
-    const int tiles = sdf_tiles_x(W);
+    const int skip = sdf_skip(H);
     float t[RPT];
     bool alive[RPT];
     bool any = false;
@@ -119,7 +115,7 @@ __device__ inline void march_rays(__amdgpu_buffer_rsrc_t sdf, int W, int H, cons
             const int ix = floor_i32(fmaf(t[j], dx, cx[j]));
             const int iy = floor_i32(fmaf(t[j], dy, cy[j]));
             ins[j] = (uint32_t)ix < (uint32_t)W && (uint32_t)iy < (uint32_t)H;
-            c[j] = sdf_code_load(sdf, ix, iy, tiles);
+            c[j] = sdf_code_load(sdf, ix, iy, skip);
         }
         any = false;
 #pragma unroll

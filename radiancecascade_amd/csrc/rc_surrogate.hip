@@ -144,7 +144,7 @@ __device__ inline void near_sdf_block(int blk, const uint64_t *__restrict__ bits
         g[r][j] = (uint8_t)d;
     }
     __syncthreads();
-    const int tx = sdf_tiles_x(W);
+    const int skip = sdf_skip(H);
     for (int i = threadIdx.x; i < kNearT * kNearT; i += kBlock) {
         const int j = i % kNearT, r = i / kNearT, x = x0 + j, y = y0 + r;
         if (x >= W || y >= H) continue;
@@ -154,7 +154,7 @@ __device__ inline void near_sdf_block(int blk, const uint64_t *__restrict__ bits
             const int gv = g[r + kNearR + dy][j];
             best = min(best, gv * gv + dy * dy);
         }
-        sdf[sdf_index(x, y, tx)] = (uint8_t)(best <= kNearR * kNearR ? isqrt_u32((uint32_t)best) : kSdfFar);
+        sdf[sdf_index(x, y, skip)] = (uint8_t)(best <= kNearR * kNearR ? isqrt_u32((uint32_t)best) : kSdfFar);
     }
 }
 
