@@ -198,11 +198,13 @@ __device__ __forceinline__ void edt_row_body(const uint64_t *__restrict__ bits, 
         const int c = (T + RL - 1) / RL;
         const int t0 = tid * c, t1 = min(t0 + c, T);
         if (t0 < t1) {
-            int a = 0, b = nq;  // the last query starting at or before t0
-            while (b - a > 1) {
-                const int m = (a + b) >> 1;
-                if (opt[2 * m * s - 1] + m <= t0) a = m;
-                else b = m;
+            // the last query starting at or before t0: a lower bound of
+            // uniform width (a + n <= nq throughout, so m <= nq - 1)
+            int a = 0;
+            for (int n = nq; n > 1;) {
+                const int h = n >> 1, m = a + h;
+                if (opt[(m << (ls + 1)) - 1] + m <= t0) a = m;  // lo(m) + m
+                n -= h;
             }
             // branch-free values: |x - j| < 2^14 and g <= 0xFFFF fit the
             // 24-bit multiplier, and an empty column's g^2 (> 2^29) is clamped
