@@ -158,13 +158,19 @@ __device__ __forceinline__ uint32_t edt_g(const uint64_t *__restrict__ bits, con
 }
 
 // ---- per-row lower envelope -------------------------------------------------------
+// The argmin of a solved pixel x: the low 16 bits of its 64-bit key
+// (value << 16 | argmin), read through a 16-bit view of the key array.
+__device__ __forceinline__ int edt_opt(const uint16_t *key16, int x) { return key16[4 * x]; }
 // Candidate range [lo, hi] of query q (at x = (2q + 1) s - 1) at a level of
 // step s: the argmins the earlier levels fixed at x -+ s (0 and W - 1 beyond
 // the ends).
-__device__ __forceinline__ int edt_lo(int q, int s, const uint16_t *opt) { return q == 0 ? 0 : opt[2 * q * s - 1]; }
-__device__ __forceinline__ int edt_hi(int q, int s, int W, const uint16_t *opt)
+__device__ __forceinline__ int edt_lo(int q, int s, const uint16_t *key16)
 {
-    return (2 * q + 2) * s > W ? W - 1 : opt[(2 * q + 2) * s - 1];
+    return q == 0 ? 0 : edt_opt(key16, 2 * q * s - 1);
+}
+__device__ __forceinline__ int edt_hi(int q, int s, int W, const uint16_t *key16)
+{
+    return (2 * q + 2) * s > W ? W - 1 : edt_opt(key16, (2 * q + 2) * s - 1);
 }
 
 // Row y of a W-pixel image (W <= WMAX) by one workgroup of RL lanes.
@@ -177,18 +183,23 @@ __device__ __forceinline__ int edt_hi(int q, int s, int W, const uint16_t *opt)
 // first step by binary search on lo(q) + q (the argmins in LDS), walks its
 // run in one loop of uniform trip count, keeping the first smallest value
 // (ties go to the smallest candidate), and meets the other lanes of a query
-// in LDS by a 64-bit atomic minimum of (value << 14 | candidate).
+// by a 64-bit LDS atomic minimum of (value << 16 | candidate) at the query's
+// own pixel.  Every pixel is solved at exactly one level, so the keys start
+// at all ones and need no reset between levels, and the next level reads the
+// argmins straight from them: one barrier per level.
 template <int WMAX, int RL>
 __device__ __forceinline__ void edt_row_body(const uint64_t *__restrict__ bits, const uint16_t *__restrict__ ends,
                                              uint8_t *__restrict__ sdf, int W, int H, int y)
 {
     __shared__ uint16_t gs[WMAX];
-    __shared__ uint16_t opt[WMAX];                 // opt[x]: smallest argmin at x
-    __shared__ unsigned long long best[WMAX / 2];  // per query of the current level (at most ceil(W / 2))
+    __shared__ unsigned long long key[WMAX];  // key[x] = value << 16 | smallest argmin, all ones until solved
+    const uint16_t *opt = reinterpret_cast<const uint16_t *>(key);
     const int tid = (int)threadIdx.x;
     const int nb = edt_bands(H);
-    for (int x = tid; x < W; x += RL) gs[x] = (uint16_t)edt_g(bits, ends, W, nb, x, y);
-    if (tid == 0) best[0] = ~0ull;  // level 0 has one query
+    for (int x = tid; x < W; x += RL) {
+        gs[x] = (uint16_t)edt_g(bits, ends, W, nb, x, y);
+        key[x] = ~0ull;
+    }
     __syncthreads();
     const int K = 32 - __builtin_clz((uint32_t)W);  // smallest K with 2^K > W
     for (int k = 0; k < K; ++k) {
@@ -203,7 +214,7 @@ __device__ __forceinline__ void edt_row_body(const uint64_t *__restrict__ bits, 
             int a = 0;
             for (int n = nq; n > 1;) {
                 const int h = n >> 1, m = a + h;
-                if (opt[(m << (ls + 1)) - 1] + m <= t0) a = m;  // lo(m) + m
+                if (edt_opt(opt, (m << (ls + 1)) - 1) + m <= t0) a = m;  // lo(m) + m
                 n -= h;
             }
             // branch-free values: |x - j| < 2^14 and g <= 0xFFFF fit the
@@ -214,7 +225,7 @@ __device__ __forceinline__ void edt_row_body(const uint64_t *__restrict__ bits, 
             uint32_t bv = 0xFFFFFFFFu, bj = 0;
             for (int t = t0; t < t1; ++t) {
                 if (t >= end) {  // the run enters the next query (sizes >= 1: one step)
-                    atomicMin(&best[q], ((unsigned long long)bv << 14) | bj);
+                    atomicMin(&key[x], ((unsigned long long)bv << 16) | bj);
                     bv = 0xFFFFFFFFu;
                     ++q;
                     x += 2 * s;
@@ -228,20 +239,13 @@ __device__ __forceinline__ void edt_row_body(const uint64_t *__restrict__ bits, 
                     bj = (uint32_t)j;
                 }
             }
-            atomicMin(&best[q], ((unsigned long long)bv << 14) | bj);
-        }
-        __syncthreads();
-        // this level's argmins; the next level's minima start empty
-        const int nq_next = k + 1 < K ? ((W >> (ls - 1)) + 1) >> 1 : 0;
-        for (int q = tid; q < max(nq, nq_next); q += RL) {
-            if (q < nq) opt[(2 * q + 1) * s - 1] = (uint16_t)(best[q] & 0x3FFFu);
-            best[q] = ~0ull;
+            atomicMin(&key[x], ((unsigned long long)bv << 16) | bj);
         }
         __syncthreads();
     }
     const int skip = sdf_skip(H);
     for (int x = tid; x < W; x += RL) {
-        const int j = opt[x];
+        const int j = edt_opt(opt, x);
         const uint32_t gj = gs[j];
         uint32_t code = kSdfFar;
         if (gj != kGNone) {
