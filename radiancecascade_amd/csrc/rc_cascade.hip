@@ -293,7 +293,7 @@ __device__ __forceinline__ void level0_body(const LevelArgs &a, float *__restric
                                             int lb)
 {
     using TT = Texel<S>;
-    constexpr int TX = 16, TY = 12, NX = TX + 2, NY = TY + 2, UCX = TX / 2 + 2, UCY = TY / 2 + 2;
+    constexpr int TX = kL0TX, TY = kL0TY, NX = TX + 2, NY = TY + 2, UCX = TX / 2 + 2, UCY = TY / 2 + 2;
     static_assert(NX * NY <= kBlock, "one probe per thread");
     __shared__ float4 su[UPPER ? UCX * UCY * 4 : 1];
     __shared__ float4 fl[NX * NY];
@@ -906,7 +906,7 @@ hipError_t launch_level(const LevelArgs &a, Storage st, int shape, int rpt, hipS
 template <Storage S>
 static hipError_t launch_level0_st(const LevelArgs &a, float *irr, float *dbg, hipStream_t s)
 {
-    const int blocks = ((a.PX + 15) / 16) * ((a.PY + 11) / 12);  // 16x12-probe tiles
+    const int blocks = level0_blocks(a.PX, a.PY);
     if (a.upper) hipLaunchKernelGGL((k_level0<S, true>), dim3(blocks), dim3(kBlock), 0, s, a, irr, dbg);
     else hipLaunchKernelGGL((k_level0<S, false>), dim3(blocks), dim3(kBlock), 0, s, a, irr, dbg);
     return hipGetLastError();
@@ -971,7 +971,7 @@ int level_blocks(int PX, int PY, int N, int shape, int rpt)
     return ((PX + tx - 1) / tx) * ((PY + ty - 1) / ty) * (N / kd);
 }
 
-int level0_blocks(int PX, int PY) { return ((PX + 15) / 16) * ((PY + 11) / 12); }
+int level0_blocks(int PX, int PY) { return ((PX + kL0TX - 1) / kL0TX) * ((PY + kL0TY - 1) / kL0TY); }
 
 template <int SPEC>
 static hipError_t launch_march_batch_spec(const BatchLaunch &b, int m, hipStream_t s)

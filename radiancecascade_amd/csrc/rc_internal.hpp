@@ -259,6 +259,8 @@ struct Level0Job {
     float *irr;
 };
 int level_blocks(int PX, int PY, int N, int shape, int rays_per_lane);
+// level 0 + gather: a workgroup's tile of kL0TX x kL0TY level-0 probes
+constexpr int kL0TX = 16, kL0TY = 12;
 int level0_blocks(int PX, int PY);
 // jobs: MarchJob (march), LevelArgs (levels, `upper` may be null per job), Level0Job
 hipError_t launch_march_batch(const BatchLaunch &b, int speculation, int rays_per_lane, hipStream_t s);
