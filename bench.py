@@ -3,7 +3,7 @@
 
 Workload (default): 1920x1080 G-buffer rasterised on the GPU from the seeded
 box scene (seed 0, 32 boxes, 4 emissive), 6 cascade levels, fp32 storage; a
-step is one full GI frame (occupancy + jump flood + SDF + 6 levels of
+step is one full GI frame (occupancy + exact distance transform + 6 levels of
 march/merge + gather) with the G-buffer already resident in HBM, its kernels
 launched one by one on one stream (the library's default; a replayed
 hipGraph leaves ~4 us between frames on this GPU).
@@ -13,7 +13,8 @@ Prints ONE JSON line.  `value` is the plain path (the form that meets the
 parity tests).  `surrogate` reports the learned far field (levels >= 2
 replaced by the cone-traced features + MLP, packaged weights) on the same
 G-buffer: its frames/s, the ratio to the plain path, and its rel-L2 judged by
-the plain path on the GPU (FP64 reduction).  `batch` reports the
+the plain path on the GPU (FP64 reduction) and, in the CPU leg, by the FP64
+oracle frame of the same G-buffer (`accuracy`, with the plain frame's).  `batch` reports the
 multi-view workload (64 seeded views of uneven sizes, one graph launch) in
 aggregate megapixels/s.  `roofline` is the dominant
 kernel's algorithmic bytes per launch over its mean launch time, measured
@@ -143,11 +144,13 @@ def _time_frames(fn, seconds, min_frames=1):
             return frames, el
 
 
-def cpu_baseline(depth, emissive, L, seconds):
+def cpu_baseline(depth, emissive, L, seconds, frames=None):
     """The CPU restatement of the same maths (the oracle, C) on this host:
     FP32 with OpenMP on every core this process may use (the GPU box's share
     for one GPU is given by OMP_NUM_THREADS when it is set), FP32 on one
-    core, and the FP64 flavour on one core.  Whole frames of the workload."""
+    core, and the FP64 flavour on one core.  Whole frames of the workload.
+    `frames` ({name: GPU irradiance [3, H, W]}) are then checked against one
+    FP64 oracle frame of the same G-buffer: rel-L2 per frame under "checks"."""
     from oracle import rco
     host = _host_cpu()
     threads = host["usable_cores"]
@@ -159,7 +162,13 @@ def cpu_baseline(depth, emissive, L, seconds):
     f1, e1 = _time_frames(lambda: rco.run(depth, emissive, L, np.float32), seconds / 3)
     f64, e64 = _time_frames(lambda: rco.run(depth, emissive, L, np.float64), seconds / 3)
     rco.set_threads(threads)
-    return {"value": fr / el, "unit": "GI frames/s", "cores": threads, "kind": "port",
+    checks = None
+    if frames:
+        ref = rco.run(depth, emissive, L, np.float64)
+        den = float(np.linalg.norm(ref.ravel()))
+        checks = {name: (float(np.linalg.norm((np.asarray(f, np.float64) - ref).ravel()) / den) if den > 0 else None)
+                  for name, f in frames.items()}   # None: an unlit frame has no relative error
+    return {"value": fr / el, "unit": "GI frames/s", "cores": threads, "kind": "port", "checks": checks,
             "sample": f"{fr} full frames of the same workload in {el:.1f} s (oracle FP32 flavour, OpenMP, "
                       f"{threads} threads); single core: FP32 {f1} frames in {e1:.1f} s, FP64 {f64} frames "
                       f"in {e64:.1f} s",
@@ -167,7 +176,8 @@ def cpu_baseline(depth, emissive, L, seconds):
             "host": host}
 
 
-# MLP FLOP per level-1 probe (48 -> 64 -> 64 -> 64 -> 64, multiply-adds x 2)
+# MLP FLOP per level-2 probe (64 cone features -> 64 -> 64 -> 64 -> 64: four 64x64
+# layers, multiply-adds x 2)
 MLP_FLOP_PER_PROBE = 2 * (64 * 64 + 3 * 64 * 64)
 BF16_MFMA_PEAK_TFLOPS = 2500.0  # MI355X dense bf16
 
@@ -211,9 +221,10 @@ def surrogate_leg(rc, torch, W, H, L, args, depth, emis, irr_plain, stream, fps_
     flop = MLP_FLOP_PER_PROBE * px * py
     mlp_ms = per_frame.get("surrogate", float("nan"))
     tflops = flop / (mlp_ms * 1e-3) / 1e12
+    learned = out.cpu().numpy()
     sur.close()
     prof.close()
-    return {"fps": 1000.0 / ms, "ms_per_step": ms, "ratio_vs_plain": (1000.0 / ms) / fps_plain,
+    return learned, {"fps": 1000.0 / ms, "ms_per_step": ms, "ratio_vs_plain": (1000.0 / ms) / fps_plain,
             "rel_l2_vs_plain": err["rel_l2"], "max_abs_vs_plain": err["max_abs"],
             "meets_2pct": bool(err["rel_l2"] <= 0.02),
             "fallback": "off (threshold +inf)",
@@ -652,18 +663,28 @@ def main():
     traffic = profiled_traffic(dominant, W, H, L, args.storage)
     frame_alg = b_alg(W, H, L, args.storage, rays)
 
-    surrogate = None
+    surrogate, learned = None, None
     if not args.no_surrogate and L >= 3:
-        surrogate = surrogate_leg(rc, torch, W, H, L, args, depth, emis, irr, stream, fps)
+        learned, surrogate = surrogate_leg(rc, torch, W, H, L, args, depth, emis, irr, stream, fps)
 
     storage = None if args.no_storage else storage_leg(rc, torch, W, H, L, args, depth, emis, irr, stream)
     uhd = None if args.no_uhd else uhd_leg(rc, torch, args, stream)
     batch = None if args.no_batch else batch_leg(rc, torch, args, stream, fps)
     staging = None if args.no_host_staging else host_staging_leg(rc, W, H, L, args.storage, depth, emis)
 
-    cpu = None
+    cpu, accuracy = None, None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(depth.cpu().numpy(), emis.cpu().numpy(), L, args.cpu_seconds)
+        frames = {"plain": irr.cpu().numpy()}
+        if learned is not None:
+            frames["learned"] = learned
+        cpu = cpu_baseline(depth.cpu().numpy(), emis.cpu().numpy(), L, args.cpu_seconds, frames)
+        accuracy = {"plain_rel_l2_vs_fp64": cpu["checks"]["plain"],
+                    "learned_rel_l2_vs_fp64": cpu["checks"].get("learned"),
+                    "note": "irradiance rel-L2 of the GPU frames against the FP64 oracle frame of the same "
+                            "G-buffer (exact distance transform), computed in the CPU leg"}
+        if surrogate is not None and accuracy["learned_rel_l2_vs_fp64"] is not None:
+            surrogate["rel_l2_vs_fp64"] = accuracy["learned_rel_l2_vs_fp64"]
+            surrogate["meets_2pct_vs_fp64"] = bool(accuracy["learned_rel_l2_vs_fp64"] <= 0.02)
 
     line = {
         "metric": "GI frames/s @1080p 6 cascades: surrogate vs plain HIP at <=2% rel-L2 error",
@@ -702,6 +723,7 @@ def main():
                            "frac": frame_alg / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "stage_ms_per_frame": {k: round(v, 5) for k, v in per_frame_ms.items()},
         "surrogate": surrogate,
+        "accuracy": accuracy,
         "storage": storage,
         "uhd": uhd,
         "batch": batch,
