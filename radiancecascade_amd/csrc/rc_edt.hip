@@ -158,6 +158,17 @@ __device__ __forceinline__ uint32_t edt_g(const uint64_t *__restrict__ bits, con
 }
 
 // ---- per-row lower envelope -------------------------------------------------------
+// floor(sqrt(d2)) for d2 < 2^29: the hardware square root (within an ulp, so
+// within 0.01 of the root here) truncated, then one integer correction step
+// each way, which is all the truncation can be off by.
+__device__ __forceinline__ uint32_t edt_isqrt(uint32_t d2)
+{
+    uint32_t r = (uint32_t)__builtin_amdgcn_sqrtf((float)d2);  // r < 2^15: 24-bit squares
+    r -= __umul24(r, r) > d2 ? 1u : 0u;
+    r += __umul24(r + 1, r + 1) <= d2 ? 1u : 0u;
+    return r;
+}
+
 // The argmin of a solved pixel x: the low 16 bits of its 64-bit key
 // (value << 16 | argmin), read through a 16-bit view of the key array.
 __device__ __forceinline__ int edt_opt(const uint16_t *key16, int x) { return key16[4 * x]; }
@@ -243,16 +254,18 @@ __device__ __forceinline__ void edt_row_body(const uint64_t *__restrict__ bits, 
         }
         __syncthreads();
     }
-    const int skip = sdf_skip(H);
+    // the byte of (x, y) at x + 16 y + (x >> 4) skip (sdf_index) in 32-bit
+    // arithmetic: x >> 4 < 2^10 and skip < 2^18 fit the 24-bit multiplier
+    const uint32_t skip = (uint32_t)sdf_skip(H), row = (uint32_t)y << 4;
     for (int x = tid; x < W; x += RL) {
         const int j = edt_opt(opt, x);
         const uint32_t gj = gs[j];
         uint32_t code = kSdfFar;
-        if (gj != kGNone) {
+        if (gj != kGNone) {  // |x - j| < 2^14 and g < 2^14: d^2 < 2^29
             const int dx = x - j;
-            code = sdf_code_of_distance(isqrt_u32((uint32_t)(dx * dx) + gj * gj));
+            code = sdf_code_of_distance(edt_isqrt((uint32_t)__mul24(dx, dx) + (uint32_t)__umul24(gj, gj)));
         }
-        sdf[sdf_index(x, y, skip)] = (uint8_t)code;
+        sdf[__umul24((uint32_t)x >> 4, skip) + (uint32_t)x + row] = (uint8_t)code;
     }
 }
 
