@@ -143,17 +143,19 @@ __global__ __launch_bounds__(kBlock) void k_ends_batch(const EndsJob *__restrict
 }
 
 // Vertical distance of pixel (x, y) to the nearest occupied pixel of its
-// column (kGNone: the column has none): the band's word, else the band ends.
-__device__ __forceinline__ uint32_t edt_g(const uint64_t *__restrict__ bits, const uint16_t *__restrict__ ends, int W,
-                                          int nb, int x, int y)
+// column (kGNone: the column has none) from the band's word, else the band
+// ends (r = y % 64).
+__device__ __forceinline__ uint32_t edt_g_of(uint64_t word, uint32_t ua, uint32_t db, int r, int y)
 {
-    const int w = y >> 6, r = y & 63;
-    const uint64_t word = bits[(size_t)w * W + x];
-    const uint32_t ua = ends[(size_t)w * W + x], db = ends[(size_t)(nb + w) * W + x];
     const uint64_t le = word & ((2ull << r) - 1);  // rows <= r of the band (all 64 for r = 63)
     const uint64_t ge = word >> r;                 // rows >= r
-    const uint32_t up = le ? (uint32_t)(r - 63 + __builtin_clzll(le)) : (ua != kGNone ? (uint32_t)y - ua : kGNone);
-    const uint32_t dn = ge ? (uint32_t)__builtin_ctzll(ge) : (db != kGNone ? db - (uint32_t)y : kGNone);
+    // both candidates, then a select (no branch for the loads to sink into;
+    // the sentinel bits keep clz / ctz defined, their value unused then)
+    const uint32_t up_in = (uint32_t)(r - 63 + __builtin_clzll(le | 1ull));
+    const uint32_t dn_in = (uint32_t)__builtin_ctzll(ge | (1ull << 63));
+    const uint32_t up_out = ua != kGNone ? (uint32_t)y - ua : kGNone;
+    const uint32_t dn_out = db != kGNone ? db - (uint32_t)y : kGNone;
+    const uint32_t up = le ? up_in : up_out, dn = ge ? dn_in : dn_out;
     return up < dn ? up : dn;
 }
 
@@ -207,9 +209,36 @@ __device__ __forceinline__ void edt_row_body(const uint64_t *__restrict__ bits, 
     const uint16_t *opt = reinterpret_cast<const uint16_t *>(key);
     const int tid = (int)threadIdx.x;
     const int nb = edt_bands(H);
-    for (int x = tid; x < W; x += RL) {
-        gs[x] = (uint16_t)edt_g(bits, ends, W, nb, x, y);
-        key[x] = ~0ull;
+    {
+        // the row's g: every load of a batch of columns issued before any is
+        // used (one round trip per batch, not one or two per column)
+        constexpr int GB = 8;
+        const int w = y >> 6, r = y & 63;
+        const uint64_t *brow = bits + (size_t)w * W;
+        const uint16_t *urow = ends + (size_t)w * W, *drow = ends + (size_t)(nb + w) * W;
+        for (int x0 = tid; x0 < W; x0 += GB * RL) {
+            uint64_t wd[GB];
+            uint32_t ua[GB], db[GB];
+#pragma unroll
+            for (int i = 0; i < GB; ++i) {
+                const int x = min(x0 + i * RL, W - 1);
+                wd[i] = brow[x];
+                ua[i] = urow[x];
+                db[i] = drow[x];
+            }
+            // the values are pinned here, so the compiler neither sinks a
+            // load into the branch below nor issues it again there
+#pragma unroll
+            for (int i = 0; i < GB; ++i) asm volatile("" : "+v"(wd[i]), "+v"(ua[i]), "+v"(db[i]));
+#pragma unroll
+            for (int i = 0; i < GB; ++i) {
+                const int x = x0 + i * RL;
+                if (x < W) {
+                    gs[x] = (uint16_t)edt_g_of(wd[i], ua[i], db[i], r, y);
+                    key[x] = ~0ull;
+                }
+            }
+        }
     }
     __syncthreads();
     const int K = 32 - __builtin_clz((uint32_t)W);  // smallest K with 2^K > W
