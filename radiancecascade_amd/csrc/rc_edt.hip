@@ -263,15 +263,24 @@ __device__ __forceinline__ void edt_row_body(const uint64_t *__restrict__ bits, 
             int q = a, x = (2 * q + 1) * s - 1;
             int end = edt_hi(q, s, W, opt) + q + 1;  // the first step of query q + 1
             uint32_t bv = 0xFFFFFFFFu, bj = 0;
-            for (int t = t0; t < t1; ++t) {
+            // key = value << 16 | candidate from 32-bit halves (value < 2^30)
+            auto flush = [&]() {
+                atomicMin(&key[x], ((unsigned long long)(bv >> 16) << 32) | ((bv << 16) | bj));
+            };
+            // candidate j = t - q and dx = x - j carried step to step: the
+            // next step of a query is j + 1, dx - 1; entering the next query
+            // keeps j (t and q both advance) and moves x on by 2 s
+            int j = t0 - q, dx = x - j;
+            for (int t = t0; t < t1; ++t, ++j, --dx) {
                 if (t >= end) {  // the run enters the next query (sizes >= 1: one step)
-                    atomicMin(&key[x], ((unsigned long long)bv << 16) | bj);
+                    flush();
                     bv = 0xFFFFFFFFu;
                     ++q;
                     x += 2 * s;
+                    --j;
+                    dx += 2 * s + 1;
                     end = edt_hi(q, s, W, opt) + q + 1;
                 }
-                const int j = t - q, dx = x - j;
                 const uint32_t gj = gs[j], g2 = (uint32_t)__umul24(gj, gj);
                 const uint32_t v = (uint32_t)__mul24(dx, dx) + (g2 < kEdtBig ? g2 : kEdtBig);
                 if (v < bv) {
@@ -279,7 +288,7 @@ __device__ __forceinline__ void edt_row_body(const uint64_t *__restrict__ bits, 
                     bj = (uint32_t)j;
                 }
             }
-            atomicMin(&key[x], ((unsigned long long)bv << 16) | bj);
+            flush();
         }
         __syncthreads();
     }
