@@ -38,7 +38,10 @@ namespace rc {
 constexpr uint32_t kGNone = 0xFFFFu;          // g of a column with no occupied pixel
 constexpr int kOccBlock = 1024;               // k_occ_bits: 16 waves x 4 rows = 64 rows
 constexpr int kRowLanes = 256;                // k_edt_rows: lanes per row (4 waves)
-constexpr uint32_t kEdtBig = 1u << 29;        // value of an empty column: above any real 2 * 16383^2
+// g of an empty column in the row kernel's LDS: its square (~2^30) is above
+// every real value dx^2 + g^2 <= 2 * 16383^2 < 2^29 and dx^2 + 32767^2 < 2^32,
+// so the pair value needs no clamp
+constexpr uint32_t kGEmpty = 32767u;
 
 __host__ __device__ inline int edt_bands(int H) { return (H + 63) >> 6; }
 size_t edt_bits_bytes(int W, int H) { return (size_t)edt_bands(H) * W * sizeof(uint64_t); }
@@ -234,7 +237,7 @@ __device__ __forceinline__ void edt_row_body(const uint64_t *__restrict__ bits, 
             for (int i = 0; i < GB; ++i) {
                 const int x = x0 + i * RL;
                 if (x < W) {
-                    gs[x] = (uint16_t)edt_g_of(wd[i], ua[i], db[i], r, y);
+                    gs[x] = (uint16_t)min(edt_g_of(wd[i], ua[i], db[i], r, y), kGEmpty);
                     key[x] = ~0ull;
                 }
             }
@@ -257,9 +260,8 @@ __device__ __forceinline__ void edt_row_body(const uint64_t *__restrict__ bits, 
                 if (edt_opt(opt, (m << (ls + 1)) - 1) + m <= t0) a = m;  // lo(m) + m
                 n -= h;
             }
-            // branch-free values: |x - j| < 2^14 and g <= 0xFFFF fit the
-            // 24-bit multiplier, and an empty column's g^2 (> 2^29) is clamped
-            // to kEdtBig, above every real value
+            // branch-free values: |x - j| < 2^14 and g <= kGEmpty fit the
+            // 24-bit multiplier, and dx^2 + g^2 the 32 bits
             int q = a, x = (2 * q + 1) * s - 1;
             int end = edt_hi(q, s, W, opt) + q + 1;  // the first step of query q + 1
             uint32_t bv = 0xFFFFFFFFu, bj = 0;
@@ -282,7 +284,7 @@ __device__ __forceinline__ void edt_row_body(const uint64_t *__restrict__ bits, 
                     end = edt_hi(q, s, W, opt) + q + 1;
                 }
                 const uint32_t gj = gs[j], g2 = (uint32_t)__umul24(gj, gj);
-                const uint32_t v = (uint32_t)__mul24(dx, dx) + (g2 < kEdtBig ? g2 : kEdtBig);
+                const uint32_t v = (uint32_t)__mul24(dx, dx) + g2;
                 if (v < bv) {
                     bv = v;
                     bj = (uint32_t)j;
@@ -299,7 +301,7 @@ __device__ __forceinline__ void edt_row_body(const uint64_t *__restrict__ bits, 
         const int j = edt_opt(opt, x);
         const uint32_t gj = gs[j];
         uint32_t code = kSdfFar;
-        if (gj != kGNone) {  // |x - j| < 2^14 and g < 2^14: d^2 < 2^29
+        if (gj != kGEmpty) {  // |x - j| < 2^14 and g < 2^14: d^2 < 2^29
             const int dx = x - j;
             code = sdf_code_of_distance(edt_isqrt((uint32_t)__mul24(dx, dx) + (uint32_t)__umul24(gj, gj)));
         }
