@@ -175,12 +175,13 @@ void rc_destroy(rc_ctx *ctx);
 /* ---- level-kernel tiling (autotuner) ----------------------------------------
  * A level-i kernel (i >= 1) runs workgroups of TX x TY probes x KD directions
  * (TX*TY*KD = 256 lanes) with R rays per lane (probe rows TY apart).
- * shape: 0 = 8x8x4, 1 = 4x4x16, 2 = 2x2x64, 3 = 1x1x256 (KD <= N_i required);
+ * shape: 0 = 8x8x4, 1 = 4x4x16, 2 = 2x2x64, 3 = 1x1x256, 4 = 8x4x8 (TX x TY x KD;
+ * KD <= N_i required);
  * rays_per_lane: 1, 2 or 4.  Every tiling computes identical results.
  * rc_autotune times every legal (shape, rays_per_lane) of every level on the
  * given device G-buffer (iters launches each, HIP events) and keeps the
  * fastest; rc_get_level_tiling reports the current choice. */
-#define RC_NUM_TILE_SHAPES 4
+#define RC_NUM_TILE_SHAPES 5
 rc_status rc_set_level_tiling(rc_ctx *ctx, uint32_t level, uint32_t shape, uint32_t rays_per_lane);
 rc_status rc_get_level_tiling(const rc_ctx *ctx, uint32_t level, uint32_t *shape, uint32_t *rays_per_lane);
 rc_status rc_autotune(rc_ctx *ctx, const rc_gbuffer *dev, float *irradiance_dev, uint32_t iters, void *stream);

@@ -28,7 +28,7 @@ pub const RC_STAGE_CONE: usize = RC_STAGE_MARCH + 3;
 pub const RC_STAGE_SURROGATE: usize = RC_STAGE_MARCH + 4;
 pub const RC_STAGE_FALLBACK: usize = RC_STAGE_MARCH + 5;
 pub const RC_NUM_STAGES: usize = RC_STAGE_MARCH + 6;
-pub const RC_NUM_TILE_SHAPES: u32 = 4;
+pub const RC_NUM_TILE_SHAPES: u32 = 5;
 pub const RC_MLP_INPUTS: usize = 64;
 pub const RC_MLP_OUTPUTS: usize = 64;
 pub const RC_MLP_PARAMS: usize = 16640;

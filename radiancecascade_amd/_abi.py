@@ -28,7 +28,7 @@ FLAG_PROFILE = 0x1
 FLAG_KEEP_DEBUG = 0x2
 FLAG_GRAPH = 0x8
 
-TILE_SHAPES = {0: (8, 8, 4), 1: (4, 4, 16), 2: (2, 2, 64), 3: (1, 1, 256)}
+TILE_SHAPES = {0: (8, 8, 4), 1: (4, 4, 16), 2: (2, 2, 64), 3: (1, 1, 256), 4: (8, 4, 8)}
 
 STAGE_SEED, STAGE_JFA, STAGE_JFA_TAIL, STAGE_LEVEL0 = 0, 1, 2, 3
 STAGE_MARCH = STAGE_LEVEL0 + MAX_LEVELS

@@ -851,9 +851,15 @@ __global__ __launch_bounds__(kBlock) void k_to_f32(const void *__restrict__ in, 
     if (i < texels) out[i] = Texel<S>::unpack(((const typename Texel<S>::T *)in)[i]);
 }
 
-int tile_kd(int shape) { return shape == TILE_8x8x4 ? 4 : shape == TILE_4x4x16 ? 16 : shape == TILE_2x2x64 ? 64 : 256; }
-int tile_tx(int shape) { return shape == TILE_8x8x4 ? 8 : shape == TILE_4x4x16 ? 4 : shape == TILE_2x2x64 ? 2 : 1; }
-int tile_ty(int shape) { return tile_tx(shape); }
+int tile_kd(int shape)
+{
+    return shape == TILE_8x8x4 ? 4 : shape == TILE_4x4x16 ? 16 : shape == TILE_2x2x64 ? 64 : shape == TILE_8x4x8 ? 8 : 256;
+}
+int tile_tx(int shape)
+{
+    return shape == TILE_8x8x4 || shape == TILE_8x4x8 ? 8 : shape == TILE_4x4x16 ? 4 : shape == TILE_2x2x64 ? 2 : 1;
+}
+int tile_ty(int shape) { return shape == TILE_8x4x8 ? 4 : tile_tx(shape); }
 
 template <int TX, int TY, int KD, int RPT, Storage S>
 static void launch_level_shape(const LevelArgs &a, hipStream_t s)
@@ -876,6 +882,7 @@ static hipError_t launch_level_rpt(const LevelArgs &a, int shape, hipStream_t s)
     case TILE_4x4x16: launch_level_shape<4, 4, 16, RPT, S>(a, s); break;
     case TILE_2x2x64: launch_level_shape<2, 2, 64, RPT, S>(a, s); break;
     case TILE_1x1x256: launch_level_shape<1, 1, 256, RPT, S>(a, s); break;
+    case TILE_8x4x8: launch_level_shape<8, 4, 8, RPT, S>(a, s); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1016,6 +1023,7 @@ static hipError_t launch_level_batch_rpt(const BatchLaunch &b, int shape, bool f
     case TILE_4x4x16: launch_level_batch_shape<4, 4, 16, RPT, S>(b, from_codes, s); break;
     case TILE_2x2x64: launch_level_batch_shape<2, 2, 64, RPT, S>(b, from_codes, s); break;
     case TILE_1x1x256: launch_level_batch_shape<1, 1, 256, RPT, S>(b, from_codes, s); break;
+    case TILE_8x4x8: launch_level_batch_shape<8, 4, 8, RPT, S>(b, from_codes, s); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

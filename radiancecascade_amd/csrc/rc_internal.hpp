@@ -152,7 +152,9 @@ hipError_t launch_rows_batch(const BatchLaunch &b, int wmax, hipStream_t s);  //
 
 // ---- cascades (rc_cascade.hip) ----------------------------------------------
 // Workgroup tile of a level kernel: TX x TY probes x KD directions, TX*TY*KD = 256.
-enum TileShape : int { TILE_8x8x4 = 0, TILE_4x4x16 = 1, TILE_2x2x64 = 2, TILE_1x1x256 = 3, TILE_COUNT = 4 };
+// 8x4x8 is the one non-square tile: per ray, fewer upper texels staged than
+// 4x4x16 (a wider probe block shares its halo) with longer output runs than 8x8x4.
+enum TileShape : int { TILE_8x8x4 = 0, TILE_4x4x16 = 1, TILE_2x2x64 = 2, TILE_1x1x256 = 3, TILE_8x4x8 = 4, TILE_COUNT = 5 };
 // Default tile of level i of L: the top level has no upper neighbourhood to
 // stage, so it takes whole 256-direction slices of one probe (no idle lanes
 // at ragged grid edges); lower levels take 4x4 probes x 16 directions, whose
