@@ -12,8 +12,8 @@ Prints ONE JSON line.  `value` is the plain path (the form that meets the
 2% rel-L2 bound: its error against the FP64 oracle is ~1e-4, pinned by the
 parity tests).  `surrogate` reports the learned far field (levels >= 2
 replaced by the cone-traced features + MLP, packaged weights) on the same
-G-buffer: its frames/s, the ratio to the plain path, and its rel-L2 judged by
-the plain path on the GPU (FP64 reduction) and, in the CPU leg, by the FP64
+G-buffer: its frames/s, the ratio to the plain path, and its rel-L2 against
+the plain path on the GPU (FP64 reduction) and, in the CPU leg, against the FP64
 oracle frame of the same G-buffer (`accuracy`, with the plain frame's).  `batch` reports the
 multi-view workload (64 seeded views of uneven sizes, one graph launch) in
 aggregate megapixels/s.  `roofline` is the dominant
